@@ -1,0 +1,102 @@
+"""ctypes bindings of libelph_vertex.so (the C ABI declared in include/elph_vertex.h).
+
+The library is loaded from the package directory; importing the product path
+without it raises immediately (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libelph_vertex.so")
+
+c_i64p = ct.POINTER(ct.c_int64)
+c_dblp = ct.POINTER(ct.c_double)
+
+ELPH_OK, ELPH_EINVAL, ELPH_EDEVICE, ELPH_EIO, ELPH_ENEGPHONON, ELPH_EWINDOW, ELPH_ENOMEM, ELPH_EWEDGE = range(8)
+ELPH_FP64, ELPH_MIXED = 0, 1
+
+
+class ElphDims(ct.Structure):
+    _fields_ = [("numwannbands", ct.c_int64), ("numbranches", ct.c_int64), ("nwsk", ct.c_int64),
+                ("nwsq", ct.c_int64), ("nwsg", ct.c_int64), ("natoms", ct.c_int64),
+                ("coarse_qmesh", ct.c_int64 * 3), ("lattvecs", ct.c_double * 9), ("polar", ct.c_int32),
+                ("epsilon", ct.c_double * 9)]
+
+
+class ElphElectrons(ct.Structure):
+    _fields_ = [("wvmesh", ct.c_int64 * 3), ("enref", ct.c_double), ("fsthick", ct.c_double),
+                ("nwv_irred", ct.c_int64), ("indexlist_irred", c_i64p), ("ens_irred", c_dblp),
+                ("evecs_irred", c_dblp), ("nwv", ct.c_int64), ("indexlist", c_i64p), ("ens", c_dblp),
+                ("evecs", c_dblp), ("nstates_inwindow", ct.c_int64)]
+
+
+class ElphPhonons(ct.Structure):
+    _fields_ = [("wvmesh", ct.c_int64 * 3), ("nwv", ct.c_int64), ("ens", c_dblp), ("evecs", c_dblp),
+                ("nwv_irred", ct.c_int64), ("indexlist_irred", c_i64p)]
+
+
+# exported symbol -> (restype, argtypes)
+PROTOTYPES = {
+    "elph_ctx_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.POINTER(ElphDims), c_i64p, c_i64p, c_i64p,
+                                   c_i64p, c_i64p, c_i64p, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp]),
+    "elph_ctx_create_epw": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_char_p, ct.c_int64, c_dblp, c_dblp,
+                                       c_dblp, c_i64p, ct.c_int32, c_dblp, c_dblp]),
+    "elph_ctx_destroy": (ct.c_int, [ct.c_void_p]),
+    "elph_last_error": (ct.c_char_p, [ct.c_void_p]),
+    "elph_ctx_dims": (ct.c_int, [ct.c_void_p, c_i64p]),
+    "elph_el_wann": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp]),
+    "elph_ph_wann": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_dblp, c_dblp, c_dblp]),
+    "elph_gkRp": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_dblp]),
+    "elph_gReq": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_dblp]),
+    "elph_g2": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp,
+                           ct.c_int64, c_i64p, ct.c_char, c_dblp]),
+    "elph_ibzk_plan_create": (ct.c_int, [ct.c_void_p, ct.POINTER(ElphElectrons), ct.POINTER(ElphPhonons), ct.c_int64,
+                                         ct.c_int32, ct.POINTER(ct.c_void_p)]),
+    "elph_ibzk_plan_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p, ct.c_int32]),
+    "elph_ibzk_plan_nprocs": (ct.c_int64, [ct.c_void_p]),
+    "elph_ibzk_plan_destroy": (ct.c_int, [ct.c_void_p]),
+    "elph_ibzk_g2": (ct.c_int, [ct.c_void_p, ct.POINTER(ElphElectrons), ct.POINTER(ElphPhonons), ct.c_int64, ct.c_int32,
+                                ct.c_int64, ct.c_int64, c_i64p, c_dblp]),
+    "elph_ibzq_plan_create": (ct.c_int, [ct.c_void_p, ct.POINTER(ElphElectrons), ct.POINTER(ElphPhonons), ct.c_int32,
+                                         ct.POINTER(ct.c_void_p)]),
+    "elph_ibzq_plan_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p, ct.c_int32]),
+    "elph_ibzq_plan_stride": (ct.c_int64, [ct.c_void_p]),
+    "elph_ibzq_plan_destroy": (ct.c_int, [ct.c_void_p]),
+    "elph_write_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
+    "elph_symmetry": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p, c_dblp]),
+    "elph_irred_wedge": (ct.c_int, [c_i64p, ct.c_int64, c_dblp, ct.c_int64, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "elph_calculate_electrons": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_i64p, ct.c_double, ct.c_double,
+                                            ct.POINTER(ct.c_void_p), ct.POINTER(ElphElectrons)]),
+    "elph_calculate_phonons": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_i64p, ct.POINTER(ct.c_void_p),
+                                          ct.POINTER(ElphPhonons)]),
+    "elph_state_free": (ct.c_int, [ct.c_void_p]),
+    "elph_selftest_mfma": (ct.c_int, [ct.c_int, c_dblp]),
+    "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with elphbolt_amd/csrc/build.sh "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ct.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class ElphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[elph {code}] {msg}")
+        self.code = code

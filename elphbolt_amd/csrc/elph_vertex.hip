@@ -1,0 +1,2105 @@
+// elph_vertex: electron-phonon vertex path for MI355X (gfx950).
+//
+// Reference path (elphbolt): Wannier%el_wann/ph_wann/gkRp/gReq/g2
+// (src/wannier.f90:430-1188) driven by calculate_gkRp and
+// calculate_eph_interaction_ibzk/ibzq (src/interactions.f90:1603-2400).
+//
+// Device pipeline of one IBZ-k batch (FP64):
+//   1. k_phase_list    phi(k,Re) = e^{i 2pi k.Re} / elwsdeg(Re)
+//   2. k_gkRp          gkRp[k] = sum_Re phi(k,Re) gwann(:,:,Re,:,:)    (FP64 MFMA GEMM)
+//   3. k_rotate_h      h[k,m](np,sp,Rp) = sum_mp U_k(m,mp) gkRp[k](np,mp,sp,Rp) / phwsdeg(Rp)
+//   4. k_pair_fp64     Y[k'](np,sp) = sum_Rp e^{i 2pi q.Rp} h[k,m](np,sp,Rp), q = k'-k   (MFMA)
+//                      fused epilogue: g = sum_sp u_q(s,sp) sum_np conj(U_k'(n,np)) Y(np,sp),
+//                      |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu, written in the
+//                      reference's (k', n, s) order.
+// The order of the linear contractions differs from the reference's per-vertex
+// g2 (overlap, phonon dot, then Rp sum); the results agree to rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace elph;
+
+#define HCHECK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            throw Error{ELPH_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+        }                                                                               \
+    } while (0)
+
+namespace {
+
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{ELPH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+        }
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        HCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+}  // namespace
+
+struct elph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
+    int64_t nb = 0, nbr = 0, nwsk = 0, nwsq = 0, nwsg = 0, natoms = 0;
+    int64_t coarse_qmesh[3] = {0, 0, 0};
+    double lattvecs[9], reclattvecs[9], volume = 0;
+    int polar = 0;
+    double epsilon[9];
+    std::vector<double> basis, masses, born;
+    std::vector<int64_t> rcells_k, rcells_q, rcells_g, elwsdeg, phwsdeg, gwsdeg;
+    // HBM-resident tensors
+    DBuf<double2> gwann;   // (nb*nb, nwsk, nbr*nwsg)
+    DBuf<double2> HwT;     // [Re][i + nb j]
+    DBuf<double2> DphT;    // [iuc][i + nbr j]
+    DBuf<int> rk, rq, rg;  // [cell][3]
+    DBuf<int> degk, degq, degg;
+    DBuf<double> rcart_k, rcart_q;  // [cell][3] Cartesian (nm)
+    // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
+    DBuf<int> term_off, term_iuc, term_T, term_div;
+    std::string err;
+    double t_dominant_ms = 0, t_all_ms = 0;
+    int64_t n_dominant = 0;
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Phases
+// ---------------------------------------------------------------------------
+
+// out[v][c] = expi(2 pi vecs(v,:) . cells(c,:)) / deg(c), the factor of
+// src/wannier.f90:466-467 / 1096-1099 / 1165.  vecs is (n,3) column-major.
+// Element (v, c) goes to out[v * ld_v + c * ld_c]; vecs(v, d) is vecs[v + vstride * d].
+__global__ void k_phase_list(int n, int ncell, const double *__restrict__ vecs, int64_t vstride,
+                             const int *__restrict__ cells, const int *__restrict__ deg, double2 *__restrict__ out,
+                             int64_t ld_v, int64_t ld_c) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (c >= ncell || v >= n) return;
+    double dot = vecs[v] * (double)cells[3 * c] + vecs[v + vstride] * (double)cells[3 * c + 1] +
+                 vecs[v + 2 * vstride] * (double)cells[3 * c + 2];
+    double s, co;
+    sincos(kTwoPi * dot, &s, &co);
+    const double d = (double)deg[c];
+    out[(size_t)v * ld_v + (size_t)c * ld_c] = make_double2(co / d, s / d);
+}
+
+// Zero fill (used instead of hipMemsetAsync so the launches stay graph-safe).
+__global__ void k_zero2(double2 *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double2(0.0, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Re contraction (gkRp) as an FP64 MFMA GEMM per (sp,Rp) slice:
+//   out[k][slice][i] = sum_Re gwann[i][Re][slice] * phiT[Re][k]
+// One workgroup per slice; wave w takes row blocks w, w+4, ...
+// ---------------------------------------------------------------------------
+template <int NCB>
+__global__ __launch_bounds__(256) void k_gkRp(const double2 *__restrict__ gw, int nb2, int nwsk, int nslices,
+                                              const double2 *__restrict__ phiT, int ldphi, int nk,
+                                              double2 *__restrict__ out, int64_t out_kstride) {
+    const int slice = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const double2 *A = gw + (size_t)slice * nb2 * nwsk;
+    const int nrb = (nb2 + 15) >> 4;
+    for (int rb = wave; rb < nrb; rb += 4) {
+        d4 cr[NCB], ci[NCB];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
+        const int row = rb * 16 + li;
+        for (int k0 = 0; k0 < nwsk; k0 += 4) {
+            const int kk = k0 + lk;
+            double2 a = make_double2(0.0, 0.0);
+            if (row < nb2 && kk < nwsk) a = A[(size_t)kk * nb2 + row];
+            const double nai = -a.y;
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) {
+                const int col = c * 16 + li;
+                double2 b = make_double2(0.0, 0.0);
+                if (kk < nwsk) b = phiT[(size_t)kk * ldphi + col];
+                cmfma(a.x, a.y, nai, b.x, b.y, cr[c], ci[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            const int col = c * 16 + li;
+            if (col >= nk) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = rb * 16 + lk + 4 * r;
+                if (i < nb2) out[(size_t)col * out_kstride + (size_t)slice * nb2 + i] = make_double2(cr[c][r], ci[c][r]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Band rotation of gkRp for IBZ state (k, m):
+//   h[st][Rp][c] = sum_mp U_k(m,mp) gkRp[k](np,mp,sp,Rp) / phwsdeg(Rp),  c = np + nb sp
+// (the initial-band half of the overlap of src/wannier.f90:960-965).
+// ---------------------------------------------------------------------------
+__global__ void k_rotate_h(int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *__restrict__ gk,
+                           int64_t gk_kstride, const int *__restrict__ st_kb, const int *__restrict__ st_m,
+                           const double2 *__restrict__ Uk /*[kb][m][mp]*/, const int *__restrict__ degq,
+                           double2 *__restrict__ h) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)Kp * CP;
+    if (idx >= per * nst) return;
+    const int st = (int)(idx / per);
+    const int rem = (int)(idx % per);
+    const int rp = rem / CP, c = rem % CP;
+    double2 acc = make_double2(0.0, 0.0);
+    if (rp < nwsg && c < nb * nbr) {
+        const int np = c % nb, sp = c / nb;
+        const int kb = st_kb[st], m = st_m[st];
+        const double2 *g = gk + (size_t)kb * gk_kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
+        const double2 *u = Uk + ((size_t)kb * nb + m) * nb;
+        for (int mp = 0; mp < nb; ++mp) cfma(acc, u[mp], g[(size_t)mp * nb]);
+        const double d = (double)degq[rp];
+        acc.x /= d;
+        acc.y /= d;
+    }
+    h[idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Fused pair kernel (FP64): Rp contraction on MFMA + band/mode rotations +
+// |g|^2 epilogue.  Block = 4 waves; each wave owns 32 k' rows (2 row blocks)
+// and all CP = 16*CB columns (np, sp) of one IBZ state (k, m).
+// ---------------------------------------------------------------------------
+struct PairArgs {
+    const double2 *h;          // [st][Kp][CP]
+    int Kp, CP;
+    const int *st_kint;        // [st][3] IBZ k on the electron mesh (0-based ints)
+    const int64_t *st_slot;    // output row of the state
+    int nwv;                   // k' rows
+    const int *kp_int;         // [nwv][3]
+    int Nel0, Nel1, Nel2;      // electron mesh
+    int mesh_ref;
+    int Nph0, Nph1, Nph2;      // phonon mesh
+    int L, Lf0, Lf1, Lf2;      // phase table size and L / Nel
+    const double2 *T;          // [L] e^{i 2pi j / L}
+    const int *Rq;             // [Kp][4] cell (padded rows are 0)
+    const double2 *Vkp;        // [nwv][n][np] = evecs(ikp, n, np)
+    const int *nwin;           // [nwv] in-window band count
+    const int *wband;          // [nwv][nb] in-window bands (0-based)
+    const int64_t *offs;       // [nwv] offset of k' in a row
+    const double2 *u;          // [nq][s][sp] phonon eigvec / sqrt(mass)
+    const double *om;          // [nq][s] eV
+    const double2 *uf;         // fine-mesh phonons (mesh_ref > 1) [nq_el][s][sp]
+    const double *omf;
+    int nb, nbr;
+    double *out;
+    int64_t nprocs;
+};
+
+// Phase-contraction core shared by the ibzk and ibzq pair kernels:
+//   acc[rb][c] (16 rows x 16 cols) += sum_K T[(coef . R_K) mod L] * B[K][c]
+// B = hst ([Kp][CP] complex) is streamed through LDS in KC-row chunks with a
+// register prefetch of the next chunk; the phase of row r and cell R is the
+// exact root of unity e^{i 2 pi (coef_r . R) / L}.
+template <int CB, int RB, int KC>
+__device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int Kp, const int (&coef)[RB][3], int L,
+                                                const double2 *Tsh, const int4 *Rsh, double2 *buf0, double2 *buf1,
+                                                d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+    constexpr int CP = 16 * CB;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) { cr[rb][c] = d4{0, 0, 0, 0}; ci[rb][c] = d4{0, 0, 0, 0}; }
+    constexpr int NPF = (KC * CP + 255) / 256;
+    constexpr int chunk = KC * CP;
+    for (int i = tid; i < chunk; i += 256) buf0[i] = hst[i];
+    __syncthreads();
+    const int nchunks = Kp / KC;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        double2 pf[NPF];
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            const double2 *src = hst + (size_t)(ch + 1) * chunk;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < chunk) pf[j] = src[i];
+            }
+        }
+        const double2 *B = (ch & 1) ? buf1 : buf0;
+#pragma unroll 2
+        for (int ks = 0; ks < KC / 4; ++ks) {
+            const int kl = ks * 4 + lk;
+            const int4 R = Rsh[ch * KC + kl];
+            double ar[RB], ai[RB], nai[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
+                const double2 t = Tsh[pos_mod(x, L)];
+                ar[rb] = t.x;
+                ai[rb] = t.y;
+                nai[rb] = -t.y;
+            }
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const double2 b = B[kl * CP + c * 16 + li];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], b.x, b.y, cr[rb][c], ci[rb][c]);
+            }
+        }
+        if (more) {
+            double2 *dst = (ch & 1) ? buf0 : buf1;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < chunk) dst[i] = pf[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
+    constexpr int KC = 32;
+    constexpr int RB = 2;
+    const int CP = 16 * CB;
+    extern __shared__ double2 smem[];
+    double2 *buf0 = smem;
+    double2 *buf1 = smem + KC * CP;
+    double2 *Tsh = smem + 2 * KC * CP;
+    int4 *Rsh = reinterpret_cast<int4 *>(Tsh + a.L);
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int st = blockIdx.y;
+    const int row_base = blockIdx.x * 128 + wave * 32;
+    const double2 *hst = a.h + (size_t)st * a.Kp * CP;
+
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
+    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
+
+    const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
+    int coef[RB][3];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int r = row_base + rb * 16 + li;
+        coef[rb][0] = coef[rb][1] = coef[rb][2] = 0;
+        if (r < a.nwv) {
+            // q = k' - k folded onto the electron mesh (vector_allreps_sub, src/vector.f90:86-102)
+            const int q0 = pos_mod(a.kp_int[3 * r] - k0i, a.Nel0);
+            const int q1 = pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1);
+            const int q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
+            coef[rb][0] = (q0 * a.Lf0) % a.L;
+            coef[rb][1] = (q1 * a.Lf1) % a.L;
+            coef[rb][2] = (q2 * a.Lf2) % a.L;
+        }
+    }
+    d4 cr[RB][CB], ci[RB][CB];
+    phase_gemm_core<CB, RB, KC>(hst, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
+
+    // ---- epilogue: per row block, stage Y (16 x CP) of this wave in LDS ----
+    double2 *Y = smem + wave * 16 * CP;
+    const int64_t slot = a.st_slot[st];
+    double *outrow = a.out + slot * a.nprocs;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * CP + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
+        __syncthreads();
+        const int row = row_base + rb * 16 + li;
+        if (row < a.nwv) {
+            const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
+            const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
+            const int q2 = pos_mod(a.kp_int[3 * row + 2] - k2i, a.Nel2);
+            const double2 *uq;
+            const double *omq;
+            if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
+                const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+                uq = a.u + (size_t)iq * a.nbr * a.nbr;
+                omq = a.om + (size_t)iq * a.nbr;
+            } else {
+                const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+                uq = a.uf + (size_t)iq * a.nbr * a.nbr;
+                omq = a.omf + (size_t)iq * a.nbr;
+            }
+            const int nn = a.nwin[row];
+            const double2 *Yr = Y + li * CP;
+            for (int rank = lk; rank < nn; rank += 4) {
+                const int n = a.wband[row * a.nb + rank];
+                const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
+                double *o = outrow + a.offs[row] + (int64_t)rank * a.nbr;
+                for (int s0 = 0; s0 < a.nbr; s0 += 6) {
+                    double2 gb[6];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) gb[j] = make_double2(0.0, 0.0);
+                    for (int sp = 0; sp < a.nbr; ++sp) {
+                        double2 z = make_double2(0.0, 0.0);
+                        for (int np = 0; np < a.nb; ++np) {
+                            const double2 v = V[np];
+                            const double2 y = Yr[np + a.nb * sp];
+                            // conj(v) * y
+                            z.x = fma(v.x, y.x, z.x);
+                            z.x = fma(v.y, y.y, z.x);
+                            z.y = fma(v.x, y.y, z.y);
+                            z.y = fma(-v.y, y.x, z.y);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 6; ++j)
+                            if (s0 + j < a.nbr) cfma(gb[j], uq[(s0 + j) * a.nbr + sp], z);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) {
+                        const int s = s0 + j;
+                        if (s < a.nbr) {
+                            const double w = omq[s];
+                            double g2 = 0.0;
+                            if (w != 0.0) g2 = 0.5 * (gb[j].x * gb[j].x + gb[j].y * gb[j].y) / w * kG2Unit;
+                            o[s] = g2;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Rp contraction of the IBZ-q side (gReq) as an FP64 MFMA GEMM per branch s:
+//   out[q][(i, s, Re)] = sum_Rp gwann[(i,Re)][s][Rp] * phiT[Rp][q]
+// rows X = i + nb2 * Re; one wave per 16-row block.
+// ---------------------------------------------------------------------------
+template <int NCB>
+__global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int nwsg,
+                                              const double2 *__restrict__ phiT, int ldphi, int nq,
+                                              double2 *__restrict__ out, int64_t out_qstride) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int s = blockIdx.y;
+    const int64_t nrows = (int64_t)nb2 * nwsk;
+    const int64_t rb = (int64_t)blockIdx.x * 4 + wave;
+    if (rb * 16 >= nrows) return;
+    const int64_t sstride = nrows * nbr;  // stride of Rp in gwann
+    const double2 *A = gw + (size_t)s * nrows;
+    d4 cr[NCB], ci[NCB];
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
+    const int64_t row = rb * 16 + li;
+    for (int k0 = 0; k0 < nwsg; k0 += 4) {
+        const int kk = k0 + lk;
+        double2 a = make_double2(0.0, 0.0);
+        if (row < nrows && kk < nwsg) a = A[(size_t)kk * sstride + row];
+        const double nai = -a.y;
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            double2 b = make_double2(0.0, 0.0);
+            if (kk < nwsg) b = phiT[(size_t)kk * ldphi + c * 16 + li];
+            cmfma(a.x, a.y, nai, b.x, b.y, cr[c], ci[c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+        const int col = c * 16 + li;
+        if (col >= nq) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t X = rb * 16 + lk + 4 * r;
+            if (X < nrows) {
+                const int64_t i = X % nb2, re = X / nb2;
+                out[(size_t)col * out_qstride + i + (size_t)nb2 * (s + (size_t)nbr * re)] = make_double2(cr[c][r], ci[c][r]);
+            }
+        }
+    }
+}
+
+// Phonon-branch rotation of gReq for phonon state (q, s):
+//   hq[st][Re][c] = sum_sp u(s,sp) gReq[q](np,mp,sp,Re) / elwsdeg(Re),  c = np + nb mp
+__global__ void k_rotate_hq(int nst, int nb, int nbr, int nwsk, int Kp, int CP, const double2 *__restrict__ gq,
+                            int64_t gq_stride, const int *__restrict__ st_qb, const double2 *__restrict__ st_u,
+                            const int *__restrict__ degk, double2 *__restrict__ h) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = (int64_t)Kp * CP;
+    if (idx >= per * nst) return;
+    const int st = (int)(idx / per);
+    const int rem = (int)(idx % per);
+    const int re = rem / CP, c = rem % CP;
+    const int nb2 = nb * nb;
+    double2 acc = make_double2(0.0, 0.0);
+    if (re < nwsk && c < nb2) {
+        const double2 *g = gq + (size_t)st_qb[st] * gq_stride + c + (size_t)nb2 * nbr * re;
+        const double2 *u = st_u + (size_t)st * nbr;
+        for (int sp = 0; sp < nbr; ++sp) cfma(acc, u[sp], g[(size_t)sp * nb2]);
+        const double d = (double)degk[re];
+        acc.x /= d;
+        acc.y /= d;
+    }
+    h[idx] = acc;
+}
+
+// Per state and k row: k' = k + q on the electron mesh (vector_allreps_add,
+// src/vector.f90:68-84), its position in the window list (binsearch of
+// src/interactions.f90:1817, here a mesh lookup table) and the number of
+// vertices the row contributes (in-window m at k times in-window n at k').
+__global__ void k_ibzq_rows(int nst, int nwv, const int *__restrict__ st_qe, const int *__restrict__ kint, int N0, int N1,
+                            int N2, const int *__restrict__ lut, const int *__restrict__ nwin, int *__restrict__ rowmap,
+                            int64_t *__restrict__ rowcnt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int st = blockIdx.y;
+    if (r >= nwv || st >= nst) return;
+    const int a0 = (kint[3 * r] + st_qe[3 * st]) % N0;
+    const int a1 = (kint[3 * r + 1] + st_qe[3 * st + 1]) % N1;
+    const int a2 = (kint[3 * r + 2] + st_qe[3 * st + 2]) % N2;
+    const int w = lut[(a2 * N1 + a1) * N0 + a0];
+    rowmap[(size_t)st * nwv + r] = w;
+    rowcnt[(size_t)st * nwv + r] = (w < 0) ? 0 : (int64_t)nwin[r] * nwin[w];
+}
+
+// Exclusive scan of rowcnt per state (one block per state); total -> counts.
+__global__ __launch_bounds__(1024) void k_scan_rows(int nwv, int64_t *__restrict__ rowcnt, int64_t *__restrict__ total) {
+    __shared__ int64_t part[1024];
+    const int st = blockIdx.x, t = threadIdx.x;
+    int64_t *v = rowcnt + (size_t)st * nwv;
+    const int per = (nwv + 1023) / 1024;
+    const int b = t * per, e = min(nwv, b + per);
+    int64_t sum = 0;
+    for (int i = b; i < e; ++i) sum += v[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t add = (t >= o) ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int64_t run = (t == 0) ? 0 : part[t - 1];
+    for (int i = b; i < e; ++i) {
+        const int64_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+    if (t == 1023) total[st] = part[1023];
+}
+
+struct PairQArgs {
+    const double2 *h;          // [st][Kp][CP]
+    int Kp, CP;
+    int nwv;
+    const int *kint;           // [nwv][3]
+    int L, Lf0, Lf1, Lf2;
+    const double2 *T;
+    const int *Rk;             // [Kp][4]
+    const double2 *Vk;         // [nwv][n][np]
+    const int *nwin, *wband;
+    const int *rowmap;         // [st][nwv]
+    const int64_t *rowoff;     // [st][nwv]
+    const double *st_om;       // [st] phonon energy (eV)
+    const int64_t *st_slot;
+    int nb;
+    double *out;
+    int64_t stride;
+};
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
+    constexpr int KC = 32;
+    constexpr int RB = 2;
+    const int CP = 16 * CB;
+    extern __shared__ double2 smem[];
+    double2 *buf0 = smem;
+    double2 *buf1 = smem + KC * CP;
+    double2 *Tsh = smem + 2 * KC * CP;
+    int4 *Rsh = reinterpret_cast<int4 *>(Tsh + a.L);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int st = blockIdx.y;
+    const int row_base = blockIdx.x * 128 + wave * 32;
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
+    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rk[4 * i], a.Rk[4 * i + 1], a.Rk[4 * i + 2], 0);
+    int coef[RB][3];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int r = row_base + rb * 16 + li;
+        coef[rb][0] = coef[rb][1] = coef[rb][2] = 0;
+        if (r < a.nwv) {
+            coef[rb][0] = (a.kint[3 * r] * a.Lf0) % a.L;
+            coef[rb][1] = (a.kint[3 * r + 1] * a.Lf1) % a.L;
+            coef[rb][2] = (a.kint[3 * r + 2] * a.Lf2) % a.L;
+        }
+    }
+    d4 cr[RB][CB], ci[RB][CB];
+    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * CP, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
+
+    double2 *Y = smem + wave * 16 * CP;
+    double *outrow = a.out + a.st_slot[st] * a.stride;
+    const double w = a.st_om[st];
+    const int nb = a.nb;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * CP + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
+        __syncthreads();
+        const int row = row_base + rb * 16 + li;
+        if (row < a.nwv) {
+            const int kp = a.rowmap[(size_t)st * a.nwv + row];
+            if (kp >= 0) {
+                const int nm = a.nwin[row], nn = a.nwin[kp];
+                const int64_t base = a.rowoff[(size_t)st * a.nwv + row];
+                const double2 *Yr = Y + li * CP;
+                for (int t = lk; t < nm * nn; t += 4) {
+                    const int mr = t / nn, nr = t % nn;
+                    const int m = a.wband[row * nb + mr], n = a.wband[kp * nb + nr];
+                    const double2 *Uk = a.Vk + ((size_t)row * nb + m) * nb;
+                    const double2 *Ukp = a.Vk + ((size_t)kp * nb + n) * nb;
+                    double2 g = make_double2(0.0, 0.0);
+                    for (int mp = 0; mp < nb; ++mp) {
+                        double2 z = make_double2(0.0, 0.0);
+                        for (int np = 0; np < nb; ++np) {
+                            const double2 v = Ukp[np];
+                            const double2 y = Yr[np + nb * mp];
+                            z.x = fma(v.x, y.x, z.x);
+                            z.x = fma(v.y, y.y, z.x);
+                            z.y = fma(v.x, y.y, z.y);
+                            z.y = fma(-v.y, y.x, z.y);
+                        }
+                        cfma(g, Uk[mp], z);
+                    }
+                    double g2 = 0.0;
+                    if (w != 0.0) g2 = 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
+                    outrow[base + (int64_t)mr * nn + nr] = g2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian / dynamical matrix assembly and the batched eigensolver
+// ---------------------------------------------------------------------------
+
+// H[k][ij] = sum_R phi[k][R] Hw[R][ij]; dH[k][p][ij] = sum_R i rcart_p(R) phi Hw
+// (src/wannier.f90:462-478).
+__global__ void k_build_H(int nk, int nb2, int nws, const double2 *__restrict__ phi, const double2 *__restrict__ HwT,
+                          const double *__restrict__ rcart, double2 *__restrict__ H, double2 *__restrict__ dH) {
+    const int k = blockIdx.y;
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= nb2 || k >= nk) return;
+    double2 acc = make_double2(0.0, 0.0);
+    double2 d0 = acc, d1 = acc, d2 = acc;
+    const double2 *ph = phi + (size_t)k * nws;
+    for (int r = 0; r < nws; ++r) {
+        const double2 t = cmul(ph[r], HwT[(size_t)r * nb2 + ij]);
+        acc.x += t.x;
+        acc.y += t.y;
+        if (dH) {
+            const double x = rcart[3 * r], y = rcart[3 * r + 1], z = rcart[3 * r + 2];
+            d0.x -= x * t.y; d0.y += x * t.x;
+            d1.x -= y * t.y; d1.y += y * t.x;
+            d2.x -= z * t.y; d2.y += z * t.x;
+        }
+    }
+    H[(size_t)k * nb2 + ij] = acc;
+    if (dH) {
+        dH[((size_t)k * 3 + 0) * nb2 + ij] = d0;
+        dH[((size_t)k * 3 + 1) * nb2 + ij] = d1;
+        dH[((size_t)k * 3 + 2) * nb2 + ij] = d2;
+    }
+}
+
+// D(q) of src/wannier.f90:631-665: for each cell, the WS-wrapped images of the
+// atom pair, weighted 1 / (phwsdeg * num_wrap), times Dphwann.
+__global__ void k_build_D(int nq, int nbr, int nat, const double *__restrict__ qv, const int *__restrict__ toff,
+                          const int *__restrict__ tiuc, const int *__restrict__ tT, const int *__restrict__ tdiv,
+                          const double2 *__restrict__ DphT, const double *__restrict__ rcart, double2 *__restrict__ D,
+                          double2 *__restrict__ dD) {
+    const int q = blockIdx.y;
+    const int nbr2 = nbr * nbr;
+    const int ij = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= nbr2 || q >= nq) return;
+    const int i = ij % nbr, j = ij / nbr;
+    const int ia = i / 3, ja = j / 3;
+    const double qx = qv[q], qy = qv[q + nq], qz = qv[q + 2 * nq];
+    double2 acc = make_double2(0.0, 0.0), d0 = acc, d1 = acc, d2 = acc;
+    const int pair = ia + nat * ja;
+    int t = toff[pair];
+    const int tend = toff[pair + 1];
+    while (t < tend) {
+        const int iuc = tiuc[t];
+        double2 cm = make_double2(0.0, 0.0);
+        for (; t < tend && tiuc[t] == iuc; ++t) {
+            const double dot = qx * (double)tT[3 * t] + qy * (double)tT[3 * t + 1] + qz * (double)tT[3 * t + 2];
+            double s, c;
+            sincos(kTwoPi * dot, &s, &c);
+            const double dv = (double)tdiv[t];
+            cm.x += c / dv;
+            cm.y += s / dv;
+        }
+        const double2 v = cmul(cm, DphT[(size_t)iuc * nbr2 + ij]);
+        acc.x += v.x;
+        acc.y += v.y;
+        if (dD) {
+            const double x = rcart[3 * iuc], y = rcart[3 * iuc + 1], z = rcart[3 * iuc + 2];
+            d0.x -= x * v.y; d0.y += x * v.x;
+            d1.x -= y * v.y; d1.y += y * v.x;
+            d2.x -= z * v.y; d2.y += z * v.x;
+        }
+    }
+    D[(size_t)q * nbr2 + ij] = acc;
+    if (dD) {
+        dD[((size_t)q * 3 + 0) * nbr2 + ij] = d0;
+        dD[((size_t)q * 3 + 1) * nbr2 + ij] = d1;
+        dD[((size_t)q * 3 + 2) * nbr2 + ij] = d2;
+    }
+}
+
+// Batched Hermitian eigensolver, one wavefront per matrix: parallel cyclic
+// Jacobi (round-robin pairing, disjoint rotations applied together) in FP64.
+// Input A (column-major n x n); the matrix used is the Hermitian one whose
+// upper triangle is the force-Hermitised upper triangle of A (zheev('V','U')
+// of src/wannier.f90:481-497, 708-738), optionally scaled elementwise by
+// scale (real symmetric, e.g. the mass normalisation of 716-729).
+// Output: W ascending, V column j = eigenvector j.
+template <int NMAX>
+__global__ __launch_bounds__(64) void k_jacobi(int nmat, int n, const double2 *__restrict__ Ain,
+                                               const double *__restrict__ scale, double *__restrict__ W,
+                                               double2 *__restrict__ Vout) {
+    __shared__ double2 A[NMAX * NMAX];
+    __shared__ double2 V[NMAX * NMAX];
+    __shared__ double rc[NMAX / 2 + 1], rs[NMAX / 2 + 1];
+    __shared__ double2 rph[NMAX / 2 + 1];
+    __shared__ int rp[NMAX / 2 + 1], rq[NMAX / 2 + 1];
+    __shared__ double sred[2];
+    const int mat = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (mat >= nmat) return;
+    const double2 *a = Ain + (size_t)mat * n * n;
+    for (int idx = lane; idx < NMAX * NMAX; idx += 64) {
+        const int i = idx % NMAX, j = idx / NMAX;
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n && j < n) {
+            if (i < j) {
+                const double2 u = a[i + n * j], l = a[j + n * i];
+                v = make_double2((u.x + l.x) * 0.5, (u.y - l.y) * 0.5);
+            } else if (i == j) {
+                v = make_double2(a[i + n * i].x, 0.0);
+            } else {
+                const double2 u = a[j + n * i], l = a[i + n * j];
+                v = make_double2((u.x + l.x) * 0.5, -(u.y - l.y) * 0.5);
+            }
+            if (scale) {
+                const double sc = scale[i + n * j];
+                v.x *= sc;
+                v.y *= sc;
+            }
+        }
+        A[idx] = v;
+        V[idx] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    const int m = n + (n & 1);
+    const int npair = m / 2;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int idx = lane; idx < n * n; idx += 64) {
+            const int i = idx % n, j = idx / n;
+            const double2 v = A[i + NMAX * j];
+            const double mag = v.x * v.x + v.y * v.y;
+            tot += mag;
+            if (i != j) off += mag;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            off += __shfl_xor(off, o);
+            tot += __shfl_xor(tot, o);
+        }
+        if (lane == 0) { sred[0] = off; sred[1] = tot; }
+        __syncthreads();
+        off = sred[0];
+        tot = sred[1];
+        __syncthreads();
+        if (off <= 1e-32 * tot || off == 0.0) break;
+        for (int step = 0; step < m - 1; ++step) {
+            if (lane < npair) {
+                int p, q;
+                if (lane == 0) { p = step; q = m - 1; }
+                else { p = (step + lane) % (m - 1); q = (step - lane + (m - 1)) % (m - 1); }
+                if (p > q) { int t = p; p = q; q = t; }
+                double cs = 1.0, sn = 0.0;
+                double2 ph = make_double2(1.0, 0.0);
+                if (q < n) {
+                    const double2 c = A[p + NMAX * q];
+                    const double ac = sqrt(c.x * c.x + c.y * c.y);
+                    if (ac > 0.0) {
+                        const double app = A[p + NMAX * p].x, aqq = A[q + NMAX * q].x;
+                        const double th = (aqq - app) / (2.0 * ac);
+                        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        cs = 1.0 / sqrt(t * t + 1.0);
+                        sn = t * cs;
+                        ph = make_double2(c.x / ac, c.y / ac);
+                    }
+                } else {
+                    q = -1;
+                }
+                rp[lane] = p; rq[lane] = q; rc[lane] = cs; rs[lane] = sn; rph[lane] = ph;
+            }
+            __syncthreads();
+            // columns: A <- A U, V <- V U
+            for (int w = lane; w < npair * n; w += 64) {
+                const int k = w / n, row = w % n;
+                const int p = rp[k], q = rq[k];
+                if (q < 0) continue;
+                const double cs = rc[k], sn = rs[k];
+                const double2 cph = make_double2(rph[k].x, -rph[k].y);  // conj(ph)
+                const double2 uqp = make_double2(-sn * cph.x, -sn * cph.y), uqq = make_double2(cs * cph.x, cs * cph.y);
+                double2 xp = A[row + NMAX * p], xq = A[row + NMAX * q];
+                double2 t1 = cmul(xq, uqp), t2 = cmul(xq, uqq);
+                A[row + NMAX * p] = make_double2(xp.x * cs + t1.x, xp.y * cs + t1.y);
+                A[row + NMAX * q] = make_double2(xp.x * sn + t2.x, xp.y * sn + t2.y);
+                xp = V[row + NMAX * p];
+                xq = V[row + NMAX * q];
+                t1 = cmul(xq, uqp);
+                t2 = cmul(xq, uqq);
+                V[row + NMAX * p] = make_double2(xp.x * cs + t1.x, xp.y * cs + t1.y);
+                V[row + NMAX * q] = make_double2(xp.x * sn + t2.x, xp.y * sn + t2.y);
+            }
+            __syncthreads();
+            // rows: A <- U^H A
+            for (int w = lane; w < npair * n; w += 64) {
+                const int k = w / n, col = w % n;
+                const int p = rp[k], q = rq[k];
+                if (q < 0) continue;
+                const double cs = rc[k], sn = rs[k];
+                const double2 ph = rph[k];
+                // conj(u_qp) = -sn ph, conj(u_qq) = cs ph
+                const double2 xp = A[p + NMAX * col], xq = A[q + NMAX * col];
+                const double2 t1 = cmul(make_double2(-sn * ph.x, -sn * ph.y), xq);
+                const double2 t2 = cmul(make_double2(cs * ph.x, cs * ph.y), xq);
+                A[p + NMAX * col] = make_double2(cs * xp.x + t1.x, cs * xp.y + t1.y);
+                A[q + NMAX * col] = make_double2(sn * xp.x + t2.x, sn * xp.y + t2.y);
+            }
+            __syncthreads();
+            if (lane < npair && rq[lane] >= 0) {
+                const int p = rp[lane], q = rq[lane];
+                A[p + NMAX * q] = make_double2(0.0, 0.0);
+                A[q + NMAX * p] = make_double2(0.0, 0.0);
+                A[p + NMAX * p].y = 0.0;
+                A[q + NMAX * q].y = 0.0;
+            }
+            __syncthreads();
+        }
+    }
+    // sort ascending, ties by index
+    if (lane < n) {
+        const double wi = A[lane + NMAX * lane].x;
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const double wj = A[j + NMAX * j].x;
+            if (wj < wi || (wj == wi && j < lane)) rank++;
+        }
+        W[(size_t)mat * n + rank] = wi;
+        for (int r = 0; r < n; ++r) Vout[(size_t)mat * n * n + r + (size_t)n * rank] = V[r + NMAX * lane];
+    }
+}
+
+// v(k,b,p) = Re(V_b^H dH_p V_b) (Hellmann-Feynman, src/wannier.f90:504-510).
+__global__ void k_quadform(int nk, int n, const double2 *__restrict__ V, const double2 *__restrict__ dH,
+                           double *__restrict__ out /*[k][b][3]*/) {
+    const int k = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t >= 3 * n) return;
+    const int b = t / 3, p = t % 3;
+    const double2 *v = V + (size_t)k * n * n + (size_t)b * n;
+    const double2 *M = dH + ((size_t)k * 3 + p) * n * n;
+    double acc = 0.0;
+    for (int r = 0; r < n; ++r) {
+        double2 mv = make_double2(0.0, 0.0);
+        for (int c = 0; c < n; ++c) cfma(mv, M[r + n * c], v[c]);
+        const double2 z = cmulc(v[r], mv);
+        acc += z.x;
+    }
+    out[((size_t)k * n + b) * 3 + p] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Generic vertex list (Wannier%g2, src/wannier.f90:903-1011): one workgroup
+// per vertex, literal order of contractions.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_g2_list(int n, int nb, int nbr, int nws, const double2 *__restrict__ phase,
+                                                 const double2 *__restrict__ ek, const double2 *__restrict__ ekp,
+                                                 const double2 *__restrict__ uq, const double *__restrict__ en,
+                                                 const double2 *__restrict__ gm, const int64_t *__restrict__ gidx,
+                                                 double *__restrict__ out) {
+    const int v = blockIdx.x;
+    if (v >= n) return;
+    __shared__ double2 ov[32 * 32];
+    __shared__ double red[2][256];
+    const int t = threadIdx.x;
+    for (int i = t; i < nb * nb; i += 256) {
+        const int mp = i % nb, np = i / nb;
+        ov[i] = cmulc(ekp[(size_t)v * nb + np], ek[(size_t)v * nb + mp]);  // overlap(mp,np)
+    }
+    __syncthreads();
+    const double2 *g = gm + (size_t)gidx[v] * nb * nb * nbr * nws;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int iws = t; iws < nws; iws += 256) {
+        double2 B = make_double2(0.0, 0.0);
+        for (int sp = 0; sp < nbr; ++sp) {
+            double2 caux = make_double2(0.0, 0.0);
+            for (int np = 0; np < nb; ++np)
+                for (int mp = 0; mp < nb; ++mp)
+                    cfma(caux, ov[mp + nb * np], g[np + nb * (mp + nb * (sp + (size_t)nbr * iws))]);
+            cfma(B, uq[(size_t)v * nbr + sp], caux);
+        }
+        cfma(acc, phase[(size_t)v * nws + iws], B);
+    }
+    red[0][t] = acc.x;
+    red[1][t] = acc.y;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) { red[0][t] += red[0][t + o]; red[1][t] += red[1][t + o]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double w = en[v];
+        out[v] = (w == 0.0) ? 0.0 : 0.5 * (red[0][0] * red[0][0] + red[1][0] * red[1][0]) / w * kG2Unit;
+    }
+}
+
+// Fixed-layout probe of the fp64 16x16x4 MFMA operand mapping.
+__global__ void k_mfma_probe(const double *A /*16x4 row-major*/, const double *B /*4x16*/, double *D /*16x16*/) {
+    const int lane = threadIdx.x;
+    const double a = A[(lane & 15) * 4 + (lane >> 4)];
+    const double b = B[(lane >> 4) * 16 + (lane & 15)];
+    d4 c = {0, 0, 0, 0};
+    c = mfma_f64(a, b, c);
+    for (int r = 0; r < 4; ++r) D[((lane >> 4) + 4 * r) * 16 + (lane & 15)] = c[r];
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+int ret_error(elph_ctx *ctx, const Error &e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+}
+
+int64_t lcm64(int64_t a, int64_t b) { return a / std::gcd(a, b) * b; }
+
+void upload_cells(elph_ctx *c, const std::vector<int64_t> &cells, int64_t n, DBuf<int> &dst, DBuf<double> *cart) {
+    std::vector<int> v(3 * n);
+    std::vector<double> cc(3 * n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d) v[3 * i + d] = (int)cells[i + n * d];
+        double R[3] = {(double)cells[i], (double)cells[i + n], (double)cells[i + 2 * n]};
+        for (int r = 0; r < 3; ++r)
+            cc[3 * i + r] = c->lattvecs[r] * R[0] + c->lattvecs[r + 3] * R[1] + c->lattvecs[r + 6] * R[2];
+    }
+    dst.upload(v, c->stream);
+    if (cart) cart->upload(cc, c->stream);
+}
+
+void build_phonon_terms(elph_ctx *c) {
+    const int64_t nat = c->natoms, nws = c->nwsq;
+    std::vector<int> off(nat * nat + 1, 0), tiuc, tT, tdiv;
+    std::vector<int> wr;
+    for (int64_t ja = 0; ja < nat; ++ja)
+        for (int64_t ia = 0; ia < nat; ++ia) {
+            off[ia + nat * ja] = (int)tiuc.size();
+            for (int64_t iuc = 0; iuc < nws; ++iuc) {
+                double dist[3];
+                for (int d = 0; d < 3; ++d) {
+                    dist[d] = (double)c->rcells_q[iuc + nws * d] - c->basis[3 * ia + d] + c->basis[3 * ja + d];
+                    dist[d] = dist[d] / (double)c->coarse_qmesh[d];
+                }
+                const int nw = ws_wrapping_vectors(c->lattvecs, dist, 1.0e-6, wr);
+                for (int w = 0; w < nw; ++w) {
+                    tiuc.push_back((int)iuc);
+                    for (int d = 0; d < 3; ++d)
+                        tT.push_back((int)(c->rcells_q[iuc + nws * d] + (int64_t)wr[3 * w + d] * c->coarse_qmesh[d]));
+                    tdiv.push_back((int)(c->phwsdeg[iuc] * nw));
+                }
+            }
+        }
+    off[nat * nat] = (int)tiuc.size();
+    // pairs are laid out (ia + nat*ja); reorder offsets to that index
+    std::vector<int> off2(nat * nat + 1);
+    for (int64_t p = 0; p <= nat * nat; ++p) off2[p] = off[p];
+    c->term_off.upload(off2, c->stream);
+    c->term_iuc.upload(tiuc, c->stream);
+    c->term_T.upload(tT, c->stream);
+    c->term_div.upload(tdiv, c->stream);
+}
+
+void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, const double *gwann) {
+    HCHECK(hipSetDevice(c->device));
+    HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HCHECK(hipEventCreate(&c->ev_a));
+    HCHECK(hipEventCreate(&c->ev_b));
+    HCHECK(hipEventCreate(&c->ev_c));
+    HCHECK(hipEventCreate(&c->ev_d));
+    reciprocal_lattice(c->lattvecs, c->reclattvecs, &c->volume);
+    const int64_t nb = c->nb, nbr = c->nbr;
+    // Hwann (nwsk,nb,nb) -> [R][i + nb j]
+    std::vector<double2> HwT(c->nwsk * nb * nb);
+    for (int64_t j = 0; j < nb; ++j)
+        for (int64_t i = 0; i < nb; ++i)
+            for (int64_t r = 0; r < c->nwsk; ++r) {
+                const int64_t src = r + c->nwsk * (i + nb * j);
+                HwT[r * nb * nb + i + nb * j] = make_double2(Hwann[2 * src], Hwann[2 * src + 1]);
+            }
+    c->HwT.upload(HwT, c->stream);
+    std::vector<double2> DT(c->nwsq * nbr * nbr);
+    for (int64_t j = 0; j < nbr; ++j)
+        for (int64_t i = 0; i < nbr; ++i)
+            for (int64_t r = 0; r < c->nwsq; ++r) {
+                const int64_t src = r + c->nwsq * (i + nbr * j);
+                DT[r * nbr * nbr + i + nbr * j] = make_double2(Dphwann[2 * src], Dphwann[2 * src + 1]);
+            }
+    c->DphT.upload(DT, c->stream);
+    const size_t ng = (size_t)nb * nb * c->nwsk * nbr * c->nwsg;
+    c->gwann.alloc(ng);
+    HCHECK(hipMemcpyAsync(c->gwann.p, gwann, ng * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    upload_cells(c, c->rcells_k, c->nwsk, c->rk, &c->rcart_k);
+    upload_cells(c, c->rcells_q, c->nwsq, c->rq, &c->rcart_q);
+    upload_cells(c, c->rcells_g, c->nwsg, c->rg, nullptr);
+    auto deg32 = [](const std::vector<int64_t> &d) { return std::vector<int>(d.begin(), d.end()); };
+    c->degk.upload(deg32(c->elwsdeg), c->stream);
+    c->degq.upload(deg32(c->phwsdeg), c->stream);
+    c->degg.upload(deg32(c->gwsdeg), c->stream);
+    build_phonon_terms(c);
+    HCHECK(hipStreamSynchronize(c->stream));
+}
+
+void phase_list(elph_ctx *c, int64_t n, const double *d_vecs, const DBuf<int> &cells, const DBuf<int> &deg,
+                int64_t ncell, DBuf<double2> &out) {
+    out.alloc((size_t)n * ncell);
+    dim3 grid((unsigned)((ncell + 127) / 128), (unsigned)n);
+    hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, c->stream, (int)n, (int)ncell, d_vecs, (int64_t)n, cells.p,
+                       deg.p, out.p, (int64_t)ncell, (int64_t)1);
+}
+
+template <int NMAX>
+void launch_jacobi_n(elph_ctx *c, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V) {
+    const int64_t chunk = 65535;
+    for (int64_t m0 = 0; m0 < nmat; m0 += chunk) {
+        const int64_t cnt = std::min(chunk, nmat - m0);
+        hipLaunchKernelGGL(k_jacobi<NMAX>, dim3((unsigned)cnt), dim3(64), 0, c->stream, (int)cnt, n,
+                           A + (size_t)m0 * n * n, scale, W + (size_t)m0 * n, V + (size_t)m0 * n * n);
+    }
+}
+
+void launch_jacobi(elph_ctx *c, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V) {
+    if (n <= 8) launch_jacobi_n<8>(c, nmat, n, A, scale, W, V);
+    else if (n <= 16) launch_jacobi_n<16>(c, nmat, n, A, scale, W, V);
+    else if (n <= 32) launch_jacobi_n<32>(c, nmat, n, A, scale, W, V);
+    else throw Error{ELPH_EINVAL, "eigensolver supports matrices up to 32x32"};
+}
+
+// Device el_wann into device outputs W [k][b] (Ry), V [k][col b][row], vel [k][b][3] (raw).
+void el_wann_device(elph_ctx *c, int64_t nk, const double *d_kvecs, DBuf<double> &W, DBuf<double2> &V,
+                    DBuf<double> *vel) {
+    const int64_t nb = c->nb, nb2 = nb * nb;
+    DBuf<double2> phi, H, dH;
+    phase_list(c, nk, d_kvecs, c->rk, c->degk, c->nwsk, phi);
+    H.alloc(nk * nb2);
+    if (vel) dH.alloc(nk * 3 * nb2);
+    dim3 grid((unsigned)((nb2 + 63) / 64), (unsigned)nk);
+    hipLaunchKernelGGL(k_build_H, grid, dim3(64), 0, c->stream, (int)nk, (int)nb2, (int)c->nwsk, phi.p, c->HwT.p,
+                       c->rcart_k.p, H.p, vel ? dH.p : (double2 *)nullptr);
+    W.alloc(nk * nb);
+    V.alloc(nk * nb2);
+    launch_jacobi(c, nk, (int)nb, H.p, nullptr, W.p, V.p);
+    if (vel) {
+        vel->alloc(nk * nb * 3);
+        hipLaunchKernelGGL(k_quadform, dim3((unsigned)nk), dim3((unsigned)(((3 * nb) + 63) / 64 * 64)), 0, c->stream,
+                           (int)nk, (int)nb, V.p, dH.p, vel->p);
+    }
+    HCHECK(hipGetLastError());
+}
+
+// Device ph_wann (non-polar part + mass normalisation + eigensolve).
+void ph_wann_device(elph_ctx *c, int64_t nq, const double *d_qvecs, DBuf<double> &W, DBuf<double2> &V,
+                    DBuf<double> *vel) {
+    const int64_t nbr = c->nbr, nbr2 = nbr * nbr;
+    if (c->polar) throw Error{ELPH_EINVAL, "polar (non-analytic) phonon correction is not available on the device"};
+    DBuf<double2> D, dD;
+    D.alloc(nq * nbr2);
+    if (vel) dD.alloc(nq * 3 * nbr2);
+    dim3 grid((unsigned)((nbr2 + 63) / 64), (unsigned)nq);
+    hipLaunchKernelGGL(k_build_D, grid, dim3(64), 0, c->stream, (int)nq, (int)nbr, (int)c->natoms, d_qvecs,
+                       c->term_off.p, c->term_iuc.p, c->term_T.p, c->term_div.p, c->DphT.p, c->rcart_q.p, D.p,
+                       vel ? dD.p : (double2 *)nullptr);
+    std::vector<double> scale(nbr2);
+    for (int64_t j = 0; j < nbr; ++j)
+        for (int64_t i = 0; i < nbr; ++i)
+            scale[i + nbr * j] = 1.0 / std::sqrt(c->masses[i / 3] * c->masses[j / 3]) * kRyd2amu;
+    DBuf<double> dscale;
+    dscale.upload(scale, c->stream);
+    W.alloc(nq * nbr);
+    V.alloc(nq * nbr2);
+    launch_jacobi(c, nq, (int)nbr, D.p, dscale.p, W.p, V.p);
+    if (vel) {
+        // mass-normalise dD on the host path below (small); the quadratic form
+        // uses the mass-normalised derivative, as the reference scales ddynmat
+        // by the same factor (src/wannier.f90:724-727).
+        std::vector<double2> hdD(nq * 3 * nbr2);
+        HCHECK(hipMemcpyAsync(hdD.data(), dD.p, hdD.size() * sizeof(double2), hipMemcpyDeviceToHost, c->stream));
+        HCHECK(hipStreamSynchronize(c->stream));
+        for (int64_t q = 0; q < nq; ++q)
+            for (int p = 0; p < 3; ++p)
+                for (int64_t ij = 0; ij < nbr2; ++ij) {
+                    double2 &v = hdD[(q * 3 + p) * nbr2 + ij];
+                    v.x *= scale[ij];
+                    v.y *= scale[ij];
+                }
+        dD.upload(hdD, c->stream);
+        vel->alloc(nq * nbr * 3);
+        hipLaunchKernelGGL(k_quadform, dim3((unsigned)nq), dim3((unsigned)(((3 * nbr) + 63) / 64 * 64)), 0,
+                           c->stream, (int)nq, (int)nbr, V.p, dD.p, vel->p);
+    }
+    HCHECK(hipGetLastError());
+}
+
+// Post-processing of ph_wann eigenvalues (src/wannier.f90:740-780), host side.
+int ph_postprocess(int64_t nq, int64_t nbr, const double *qvecs_colmajor, std::vector<double> &w,
+                   std::vector<double> *vel) {
+    int status = ELPH_OK;
+    for (int64_t q = 0; q < nq; ++q) {
+        for (int64_t s = 0; s < nbr; ++s) {
+            const double o2 = w[q * nbr + s];
+            const double a = std::sqrt(std::fabs(o2));
+            w[q * nbr + s] = std::signbit(o2) ? -a : a;
+        }
+        if (vel)
+            for (int64_t s = 0; s < nbr; ++s)
+                for (int p = 0; p < 3; ++p) (*vel)[(q * nbr + s) * 3 + p] /= (2.0 * w[q * nbr + s]);
+        for (int64_t s = 0; s < nbr; ++s) w[q * nbr + s] *= kRyd2eV;
+        if (vel)
+            for (int64_t i = 0; i < nbr * 3; ++i) (*vel)[q * nbr * 3 + i] *= kRyd2radTHz;
+        if (qvecs_colmajor[q] == 0 && qvecs_colmajor[q + nq] == 0 && qvecs_colmajor[q + 2 * nq] == 0) {
+            for (int64_t s = 0; s < 3 && s < nbr; ++s) w[q * nbr + s] = 0.0;
+            if (vel)
+                for (int64_t i = 0; i < nbr * 3; ++i) (*vel)[q * nbr * 3 + i] = 0.0;
+        }
+        for (int64_t s = 0; s < nbr; ++s) {
+            const double e = w[q * nbr + s];
+            if (e < -0.005) status = ELPH_ENEGPHONON;
+            else if (e < 0 && e > -0.005) w[q * nbr + s] = 0.0;
+        }
+    }
+    return status;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int elph_ctx_create(elph_ctx **out, int device, const elph_dims *d, const int64_t *rcells_k, const int64_t *rcells_q,
+                    const int64_t *rcells_g, const int64_t *elwsdeg, const int64_t *phwsdeg, const int64_t *gwsdeg,
+                    const double *Hwann, const double *Dphwann, const double *gwann, const double *basis,
+                    const double *masses, const double *born) {
+    if (!out || !d) return ELPH_EINVAL;
+    *out = nullptr;
+    auto c = std::make_unique<elph_ctx>();
+    try {
+        if (d->numwannbands <= 0 || d->numbranches <= 0 || d->numbranches != 3 * d->natoms || d->nwsk <= 0 ||
+            d->nwsq <= 0 || d->nwsg <= 0)
+            throw Error{ELPH_EINVAL, "inconsistent dimensions (numbranches must be 3*natoms, all sizes > 0)"};
+        if (d->numwannbands > 32 || d->numbranches > 32)
+            throw Error{ELPH_EINVAL, "numwannbands and numbranches must be <= 32"};
+        if (d->polar && !born) throw Error{ELPH_EINVAL, "polar crystal needs Born charges"};
+        c->device = device;
+        c->nb = d->numwannbands;
+        c->nbr = d->numbranches;
+        c->nwsk = d->nwsk;
+        c->nwsq = d->nwsq;
+        c->nwsg = d->nwsg;
+        c->natoms = d->natoms;
+        for (int i = 0; i < 3; ++i) c->coarse_qmesh[i] = d->coarse_qmesh[i];
+        std::memcpy(c->lattvecs, d->lattvecs, sizeof(c->lattvecs));
+        std::memcpy(c->epsilon, d->epsilon, sizeof(c->epsilon));
+        c->polar = d->polar;
+        c->basis.assign(basis, basis + 3 * c->natoms);
+        c->masses.assign(masses, masses + c->natoms);
+        if (d->polar) c->born.assign(born, born + 9 * c->natoms);
+        c->rcells_k.assign(rcells_k, rcells_k + 3 * c->nwsk);
+        c->rcells_q.assign(rcells_q, rcells_q + 3 * c->nwsq);
+        c->rcells_g.assign(rcells_g, rcells_g + 3 * c->nwsg);
+        c->elwsdeg.assign(elwsdeg, elwsdeg + c->nwsk);
+        c->phwsdeg.assign(phwsdeg, phwsdeg + c->nwsq);
+        c->gwsdeg.assign(gwsdeg, gwsdeg + c->nwsg);
+        ctx_init_device(c.get(), Hwann, Dphwann, gwann);
+    } catch (const Error &e) {
+        std::fprintf(stderr, "elph_ctx_create: %s\n", e.msg.c_str());
+        return e.code;
+    }
+    *out = c.release();
+    return ELPH_OK;
+}
+
+int elph_ctx_create_epw(elph_ctx **out, int device, const char *dir, int64_t natoms, const double *lattvecs,
+                        const double *basis, const double *masses, const int64_t *coarse_qmesh, int32_t polar,
+                        const double *epsilon, const double *born) {
+    EpwData e;
+    std::string err;
+    int st = read_epw(dir, e, err);
+    if (st != ELPH_OK) {
+        std::fprintf(stderr, "elph_ctx_create_epw: %s\n", err.c_str());
+        return st;
+    }
+    elph_dims d{};
+    d.numwannbands = e.nb;
+    d.numbranches = e.nbr;
+    d.nwsk = e.nwsk;
+    d.nwsq = e.nwsq;
+    d.nwsg = e.nwsg;
+    d.natoms = natoms;
+    for (int i = 0; i < 3; ++i) d.coarse_qmesh[i] = coarse_qmesh[i];
+    std::memcpy(d.lattvecs, lattvecs, sizeof(d.lattvecs));
+    d.polar = polar;
+    if (epsilon) std::memcpy(d.epsilon, epsilon, sizeof(d.epsilon));
+    return elph_ctx_create(out, device, &d, e.rcells_k.data(), e.rcells_q.data(), e.rcells_g.data(), e.elwsdeg.data(),
+                           e.phwsdeg.data(), e.gwsdeg.data(), e.Hwann.data(), e.Dphwann.data(), e.gwann.data(), basis,
+                           masses, born);
+}
+
+int elph_ctx_destroy(elph_ctx *ctx) {
+    if (!ctx) return ELPH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
+    if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    if (ctx->ev_c) (void)hipEventDestroy(ctx->ev_c);
+    if (ctx->ev_d) (void)hipEventDestroy(ctx->ev_d);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    if (s) (void)hipStreamDestroy(s);
+    return ELPH_OK;
+}
+
+const char *elph_last_error(const elph_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int elph_ctx_dims(const elph_ctx *ctx, int64_t out[6]) {
+    if (!ctx) return ELPH_EINVAL;
+    out[0] = ctx->nb; out[1] = ctx->nbr; out[2] = ctx->nwsk; out[3] = ctx->nwsq; out[4] = ctx->nwsg; out[5] = ctx->natoms;
+    return ELPH_OK;
+}
+
+int elph_last_timing(const elph_ctx *ctx, double out[3]) {
+    if (!ctx) return ELPH_EINVAL;
+    out[0] = ctx->t_dominant_ms;
+    out[1] = (double)ctx->n_dominant;
+    out[2] = ctx->t_all_ms;
+    return ELPH_OK;
+}
+
+int elph_el_wann(elph_ctx *ctx, int64_t nk, const double *kvecs, double *energies, double *velocities, double *evecs,
+                 const double *scissor) {
+    if (!ctx || nk < 0) return ELPH_EINVAL;
+    if (velocities && !evecs) {
+        ctx->err = "In el_wann, velocity is present but not eigenvecs.";
+        return ELPH_EINVAL;
+    }
+    if (nk == 0) return ELPH_OK;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t nb = ctx->nb;
+        DBuf<double> dk, W, vel;
+        DBuf<double2> V;
+        dk.upload(kvecs, 3 * nk, ctx->stream);
+        el_wann_device(ctx, nk, dk.p, W, V, velocities ? &vel : nullptr);
+        std::vector<double> w(nk * nb), hv;
+        std::vector<double2> v(nk * nb * nb);
+        HCHECK(hipMemcpyAsync(w.data(), W.p, w.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HCHECK(hipMemcpyAsync(v.data(), V.p, v.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        if (velocities) {
+            hv.resize(nk * nb * 3);
+            HCHECK(hipMemcpyAsync(hv.data(), vel.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        HCHECK(hipStreamSynchronize(ctx->stream));
+        for (int64_t k = 0; k < nk; ++k)
+            for (int64_t b = 0; b < nb; ++b) {
+                double e = w[k * nb + b] * kRyd2eV;
+                if (scissor) e += scissor[b];
+                energies[k + nk * b] = e;
+                if (evecs)
+                    for (int64_t j = 0; j < nb; ++j) {
+                        const double2 z = v[k * nb * nb + b * nb + j];  // component j of eigenvector b
+                        const int64_t o = k + nk * (b + nb * j);
+                        evecs[2 * o] = z.x;
+                        evecs[2 * o + 1] = z.y;
+                    }
+                if (velocities)
+                    for (int p = 0; p < 3; ++p) velocities[k + nk * (b + nb * p)] = hv[(k * nb + b) * 3 + p] * kRyd2radTHz;
+            }
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
+}
+
+int elph_ph_wann(elph_ctx *ctx, int64_t nq, const double *qvecs, double *energies, double *evecs, double *velocities) {
+    if (!ctx || nq < 0) return ELPH_EINVAL;
+    if (velocities && !evecs) {
+        ctx->err = "In ph_wann, velocity is present but not eigenvecs.";
+        return ELPH_EINVAL;
+    }
+    if (nq == 0) return ELPH_OK;
+    int status = ELPH_OK;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t nbr = ctx->nbr;
+        DBuf<double> dq, W, vel;
+        DBuf<double2> V;
+        dq.upload(qvecs, 3 * nq, ctx->stream);
+        ph_wann_device(ctx, nq, dq.p, W, V, velocities ? &vel : nullptr);
+        std::vector<double> w(nq * nbr), hv;
+        std::vector<double2> v(nq * nbr * nbr);
+        HCHECK(hipMemcpyAsync(w.data(), W.p, w.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HCHECK(hipMemcpyAsync(v.data(), V.p, v.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        if (velocities) {
+            hv.resize(nq * nbr * 3);
+            HCHECK(hipMemcpyAsync(hv.data(), vel.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        HCHECK(hipStreamSynchronize(ctx->stream));
+        status = ph_postprocess(nq, nbr, qvecs, w, velocities ? &hv : nullptr);
+        for (int64_t q = 0; q < nq; ++q)
+            for (int64_t s = 0; s < nbr; ++s) {
+                energies[q + nq * s] = w[q * nbr + s];
+                if (evecs)
+                    for (int64_t j = 0; j < nbr; ++j) {
+                        const double2 z = v[q * nbr * nbr + s * nbr + j];
+                        const int64_t o = q + nq * (s + nbr * j);
+                        evecs[2 * o] = z.x;
+                        evecs[2 * o + 1] = z.y;
+                    }
+                if (velocities)
+                    for (int p = 0; p < 3; ++p) velocities[q + nq * (s + nbr * p)] = hv[(q * nbr + s) * 3 + p];
+            }
+        if (status == ELPH_ENEGPHONON) ctx->err = "Large negative phonon energy found! Stopping!";
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return status;
+}
+
+// Batched gkRp into device memory: out_dev[k] (nb*nb, nbr*nwsg) for kvecs on device.
+static void gkRp_device(elph_ctx *c, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride) {
+    const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
+    const int64_t KT = 64;
+    DBuf<double2> phiT;
+    phiT.alloc(c->nwsk * KT);
+    for (int64_t k0 = 0; k0 < nk; k0 += KT) {
+        const int64_t n = std::min(KT, nk - k0);
+        // phases in [Re][k] layout (64 columns, unused columns zero)
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsk * KT + 255) / 256)), dim3(256), 0, c->stream, phiT.p,
+                           c->nwsk * KT);
+        dim3 grid((unsigned)((c->nwsk + 127) / 128), (unsigned)n);
+        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, c->stream, (int)n, (int)c->nwsk, d_kvecs + k0, nk,
+                           c->rk.p, c->degk.p, phiT.p, (int64_t)1, KT);
+        hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), 0, c->stream, c->gwann.p, (int)nb2,
+                           (int)c->nwsk, (int)nslices, phiT.p, (int)KT, (int)n, out + (size_t)k0 * out_kstride,
+                           out_kstride);
+        HCHECK(hipGetLastError());
+    }
+}
+
+int elph_gkRp(elph_ctx *ctx, int64_t nk, const double *kvecs, double *out) {
+    if (!ctx || nk < 0) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t nb2 = ctx->nb * ctx->nb, nslices = ctx->nbr * ctx->nwsg;
+        const int64_t blk = nb2 * ctx->nbr * ctx->nwsq;  // per-k output (nb,nb,nbr,nwsq)
+        DBuf<double> dk;
+        DBuf<double2> dout;
+        dk.upload(kvecs, 3 * nk, ctx->stream);
+        dout.alloc(nk * nb2 * nslices);
+        gkRp_device(ctx, nk, dk.p, dout.p, nb2 * nslices);
+        std::vector<double2> h(nk * nb2 * nslices);
+        HCHECK(hipMemcpyAsync(h.data(), dout.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        HCHECK(hipStreamSynchronize(ctx->stream));
+        for (int64_t k = 0; k < nk; ++k) {
+            std::memcpy(out + 2 * k * blk, &h[k * nb2 * nslices], nb2 * nslices * sizeof(double2));
+            if (blk > nb2 * nslices)
+                std::memset(out + 2 * (k * blk + nb2 * nslices), 0, (blk - nb2 * nslices) * sizeof(double2));
+        }
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
+}
+
+int elph_g2(elph_ctx *ctx, int64_t n, const double *kvecs, const double *qvecs, const double *el_evec_k,
+            const double *el_evec_kp, const double *ph_evec_q, const double *ph_en, const double *gmixed,
+            int64_t ngmixed, const int64_t *gidx, char wannspace, double *out) {
+    if (!ctx) return ELPH_EINVAL;
+    if (wannspace != 'e' && wannspace != 'p') {
+        ctx->err = "Invalid value of wannspace in call to g2_epw. Exiting.";
+        return ELPH_EINVAL;
+    }
+    if (n == 0) return ELPH_OK;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t nb = ctx->nb, nbr = ctx->nbr;
+        if (ctx->polar) throw Error{ELPH_EINVAL, "polar long-range vertex term is not available on the device"};
+        const bool ph = wannspace == 'p';
+        const int64_t nws = ph ? ctx->nwsg : ctx->nwsk;
+        for (int64_t i = 0; i < n; ++i)
+            if (gidx[i] < 0 || gidx[i] >= ngmixed) throw Error{ELPH_EINVAL, "gidx out of range"};
+        if (ph && ctx->nwsg > ctx->nwsq) throw Error{ELPH_EINVAL, "g2 'ph' sums nwsg cells of rcells_q: nwsg > nwsq"};
+        DBuf<double> dv, den;
+        DBuf<double2> phase, ek, ekp, uq, gm;
+        DBuf<int64_t> dg;
+        dv.upload(ph ? qvecs : kvecs, 3 * n, ctx->stream);
+        if (ph) phase_list(ctx, n, dv.p, ctx->rq, ctx->degq, nws, phase);
+        else phase_list(ctx, n, dv.p, ctx->rk, ctx->degk, nws, phase);
+        // inputs are (n,nb) column-major; the kernel reads [v][i]
+        auto rows = [&](const double *src, int64_t w) {
+            std::vector<double2> r(n * w);
+            for (int64_t v = 0; v < n; ++v)
+                for (int64_t i = 0; i < w; ++i) r[v * w + i] = make_double2(src[2 * (v + n * i)], src[2 * (v + n * i) + 1]);
+            return r;
+        };
+        ek.upload(rows(el_evec_k, nb), ctx->stream);
+        ekp.upload(rows(el_evec_kp, nb), ctx->stream);
+        std::vector<double2> u = rows(ph_evec_q, nbr);
+        for (int64_t v = 0; v < n; ++v)
+            for (int64_t p = 0; p < nbr; ++p) {
+                const double s = 1.0 / std::sqrt(ctx->masses[p / 3]);
+                u[v * nbr + p].x *= s;
+                u[v * nbr + p].y *= s;
+            }
+        uq.upload(u, ctx->stream);
+        den.upload(ph_en, n, ctx->stream);
+        gm.upload(reinterpret_cast<const double2 *>(gmixed), (size_t)ngmixed * nb * nb * nbr * nws, ctx->stream);
+        dg.upload(gidx, n, ctx->stream);
+        DBuf<double> dout;
+        dout.alloc(n);
+        hipLaunchKernelGGL(k_g2_list, dim3((unsigned)n), dim3(256), 0, ctx->stream, (int)n, (int)nb, (int)nbr, (int)nws,
+                           phase.p, ek.p, ekp.p, uq.p, den.p, gm.p, dg.p, dout.p);
+        HCHECK(hipGetLastError());
+        HCHECK(hipMemcpyAsync(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
+}
+
+int elph_selftest_mfma(int device, double *max_err) {
+    try {
+        HCHECK(hipSetDevice(device));
+        double A[64], B[64], D[256], R[256];
+        for (int i = 0; i < 64; ++i) { A[i] = 0.25 * (i % 7) - 0.5 + 0.01 * i; B[i] = 0.125 * (i % 5) - 0.3 + 0.02 * i; }
+        for (int i = 0; i < 16; ++i)
+            for (int j = 0; j < 16; ++j) {
+                double s = 0;
+                for (int k = 0; k < 4; ++k) s += A[i * 4 + k] * B[k * 16 + j];
+                R[i * 16 + j] = s;
+            }
+        DBuf<double> dA, dB, dD;
+        dA.upload(A, 64, nullptr);
+        dB.upload(B, 64, nullptr);
+        dD.alloc(256);
+        hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, nullptr, dA.p, dB.p, dD.p);
+        HCHECK(hipGetLastError());
+        HCHECK(hipMemcpy(D, dD.p, sizeof(D), hipMemcpyDeviceToHost));
+        double e = 0;
+        for (int i = 0; i < 256; ++i) e = std::max(e, std::fabs(D[i] - R[i]));
+        *max_err = e;
+    } catch (const Error &e) {
+        std::fprintf(stderr, "elph_selftest_mfma: %s\n", e.msg.c_str());
+        return e.code;
+    }
+    return ELPH_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// ibzk plan (calculate_eph_interaction_ibzk, key 'g')
+// ===========================================================================
+struct elph_ibzk_plan {
+    elph_ctx *ctx = nullptr;
+    int32_t precision = ELPH_FP64;
+    int64_t mesh_ref = 1;
+    int64_t nb = 0, nbr = 0, nwv = 0, nirr = 0, nprocs = 0;
+    int64_t Nel[3], Nph[3];
+    int64_t L = 1, Lf[3];
+    int64_t Kp = 0, CP = 0;
+    std::vector<int> kint_irr;     // [nirr][3]
+    std::vector<double> ens_irred; // host (nirr,nb)
+    double enref = 0, fsthick = 0;
+    std::vector<double2> Uk_host;  // [nirr][m][mp]
+    DBuf<int> kp_int, nwin, wband, Rq;
+    DBuf<int64_t> offs;
+    DBuf<double2> Vkp, u, uf, T;
+    DBuf<double> om, omf;
+    // per-run scratch
+    DBuf<double2> gk, h, Ukb;
+    DBuf<double> kv, g2stage;
+    DBuf<int> st_kb, st_m, st_kint;
+    DBuf<int64_t> st_slot;
+};
+
+namespace {
+
+void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t nprocs) {
+    elph_ctx *c = p->ctx;
+    PairArgs a{};
+    a.h = p->h.p;
+    a.Kp = (int)p->Kp;
+    a.CP = (int)p->CP;
+    a.st_kint = p->st_kint.p;
+    a.st_slot = p->st_slot.p;
+    a.nwv = (int)p->nwv;
+    a.kp_int = p->kp_int.p;
+    a.Nel0 = (int)p->Nel[0]; a.Nel1 = (int)p->Nel[1]; a.Nel2 = (int)p->Nel[2];
+    a.mesh_ref = (int)p->mesh_ref;
+    a.Nph0 = (int)p->Nph[0]; a.Nph1 = (int)p->Nph[1]; a.Nph2 = (int)p->Nph[2];
+    a.L = (int)p->L;
+    a.Lf0 = (int)p->Lf[0]; a.Lf1 = (int)p->Lf[1]; a.Lf2 = (int)p->Lf[2];
+    a.T = p->T.p;
+    a.Rq = p->Rq.p;
+    a.Vkp = p->Vkp.p;
+    a.nwin = p->nwin.p;
+    a.wband = p->wband.p;
+    a.offs = p->offs.p;
+    a.u = p->u.p;
+    a.om = p->om.p;
+    a.uf = p->uf.p;
+    a.omf = p->omf.p;
+    a.nb = (int)p->nb;
+    a.nbr = (int)p->nbr;
+    a.out = out;
+    a.nprocs = nprocs;
+    const int CB = (int)(p->CP / 16);
+    const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+    dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
+    HCHECK(hipEventRecord(c->ev_a, c->stream));
+    switch (CB) {
+        case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, c->stream, a); break;
+        case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, c->stream, a); break;
+        case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, c->stream, a); break;
+        case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, c->stream, a); break;
+        case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, c->stream, a); break;
+        case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, c->stream, a); break;
+        default: throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported by the fused kernel"};
+    }
+    HCHECK(hipGetLastError());
+    HCHECK(hipEventRecord(c->ev_b, c->stream));
+    HCHECK(hipEventSynchronize(c->ev_b));
+    float ms = 0;
+    HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    c->t_dominant_ms += ms;
+    c->n_dominant += 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int64_t mesh_ref,
+                          int32_t precision, elph_ibzk_plan **out) {
+    if (!ctx || !el || !ph || !out) return ELPH_EINVAL;
+    *out = nullptr;
+    auto p = std::make_unique<elph_ibzk_plan>();
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        if (precision != ELPH_FP64 && precision != ELPH_MIXED) throw Error{ELPH_EINVAL, "unknown precision"};
+        if (mesh_ref < 1) throw Error{ELPH_EINVAL, "mesh_ref must be >= 1"};
+        for (int d = 0; d < 3; ++d)
+            if (el->wvmesh[d] != ph->wvmesh[d] * mesh_ref)
+                throw Error{ELPH_EINVAL, "electron mesh must equal phonon mesh * mesh_ref"};
+        if (ctx->nwsg > ctx->nwsq)
+            throw Error{ELPH_EINVAL, "g2 'ph' sums nwsg cells with rcells_q/phwsdeg: needs nwsg <= nwsq"};
+        if (el->nwv <= 0) throw Error{ELPH_EWINDOW, "No states found within Fermi window."};
+        p->ctx = ctx;
+        p->precision = precision;
+        p->mesh_ref = mesh_ref;
+        const int64_t nb = ctx->nb, nbr = ctx->nbr, nwv = el->nwv;
+        p->nb = nb;
+        p->nbr = nbr;
+        p->nwv = nwv;
+        p->nirr = el->nwv_irred;
+        p->enref = el->enref;
+        p->fsthick = el->fsthick;
+        for (int d = 0; d < 3; ++d) { p->Nel[d] = el->wvmesh[d]; p->Nph[d] = ph->wvmesh[d]; }
+        p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
+        for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
+        if (nb * nbr > 96) throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported"};
+        p->CP = ((nb * nbr + 15) / 16) * 16;
+        p->Kp = ((ctx->nwsg + 31) / 32) * 32;
+        hipStream_t s = ctx->stream;
+        // k' rows
+        std::vector<int> kpi(3 * nwv), nw(nwv), wb(nwv * nb, 0);
+        std::vector<int64_t> offs(nwv);
+        std::vector<double2> V(nwv * nb * nb);
+        int64_t acc = 0, total_win = 0;
+        for (int64_t r = 0; r < nwv; ++r) {
+            int64_t v[3];
+            demux_vector0(el->indexlist[r], v, el->wvmesh);
+            for (int d = 0; d < 3; ++d) kpi[3 * r + d] = (int)v[d];
+            int cnt = 0;
+            for (int64_t n = 0; n < nb; ++n)
+                if (std::fabs(el->ens[r + nwv * n] - el->enref) <= el->fsthick) wb[r * nb + cnt++] = (int)n;
+            nw[r] = cnt;
+            offs[r] = acc;
+            acc += (int64_t)cnt * nbr;
+            total_win += cnt;
+            for (int64_t n = 0; n < nb; ++n)
+                for (int64_t j = 0; j < nb; ++j) {
+                    const int64_t o = r + nwv * (n + nb * j);
+                    V[(r * nb + n) * nb + j] = make_double2(el->evecs[2 * o], el->evecs[2 * o + 1]);
+                }
+        }
+        if (total_win != el->nstates_inwindow)
+            throw Error{ELPH_EINVAL, "nstates_inwindow does not match the in-window bands of ens"};
+        p->nprocs = el->nstates_inwindow * nbr;
+        p->kp_int.upload(kpi, s);
+        p->nwin.upload(nw, s);
+        p->wband.upload(wb, s);
+        p->offs.upload(offs, s);
+        p->Vkp.upload(V, s);
+        // phonons on the phonon mesh: u = e / sqrt(M), omega
+        const int64_t nq = ph->nwv;
+        std::vector<double2> u(nq * nbr * nbr);
+        std::vector<double> om(nq * nbr);
+        for (int64_t q = 0; q < nq; ++q)
+            for (int64_t sb = 0; sb < nbr; ++sb) {
+                om[q * nbr + sb] = ph->ens[q + nq * sb];
+                for (int64_t sp = 0; sp < nbr; ++sp) {
+                    const int64_t o = q + nq * (sb + nbr * sp);
+                    const double sc = 1.0 / std::sqrt(ctx->masses[sp / 3]);
+                    u[(q * nbr + sb) * nbr + sp] = make_double2(ph->evecs[2 * o] * sc, ph->evecs[2 * o + 1] * sc);
+                }
+            }
+        p->u.upload(u, s);
+        p->om.upload(om, s);
+        if (mesh_ref > 1) {
+            // phonons at every electron-mesh q, interpolated by ph_wann on the device
+            // (the reference calls ph_wann per k', src/interactions.f90:2241-2247)
+            const int64_t nqf = p->Nel[0] * p->Nel[1] * p->Nel[2];
+            std::vector<double> qv(3 * nqf);
+            for (int64_t i = 0; i < nqf; ++i) {
+                int64_t v[3];
+                demux_vector0(i + 1, v, el->wvmesh);
+                for (int d = 0; d < 3; ++d) qv[i + nqf * d] = (double)v[d] / (double)el->wvmesh[d];
+            }
+            std::vector<double> en(nqf * nbr), ev(2 * nqf * nbr * nbr);
+            int st = elph_ph_wann(ctx, nqf, qv.data(), en.data(), ev.data(), nullptr);
+            if (st != ELPH_OK) throw Error{st, ctx->err};
+            std::vector<double2> uf(nqf * nbr * nbr);
+            std::vector<double> omf(nqf * nbr);
+            for (int64_t q = 0; q < nqf; ++q)
+                for (int64_t sb = 0; sb < nbr; ++sb) {
+                    omf[q * nbr + sb] = en[q + nqf * sb];
+                    for (int64_t sp = 0; sp < nbr; ++sp) {
+                        const int64_t o = q + nqf * (sb + nbr * sp);
+                        const double sc = 1.0 / std::sqrt(ctx->masses[sp / 3]);
+                        uf[(q * nbr + sb) * nbr + sp] = make_double2(ev[2 * o] * sc, ev[2 * o + 1] * sc);
+                    }
+                }
+            p->uf.upload(uf, s);
+            p->omf.upload(omf, s);
+        } else {
+            p->uf.alloc(1);
+            p->omf.alloc(1);
+        }
+        // phase table e^{i 2 pi j / L} and the padded Rp cell list (rcells_q(1:nwsg))
+        std::vector<double2> T(p->L);
+        for (int64_t j = 0; j < p->L; ++j) {
+            const double x = kTwoPi * (double)j / (double)p->L;
+            T[j] = make_double2(std::cos(x), std::sin(x));
+        }
+        p->T.upload(T, s);
+        std::vector<int> R(4 * p->Kp, 0);
+        for (int64_t i = 0; i < ctx->nwsg; ++i)
+            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_q[i + ctx->nwsq * d];
+        p->Rq.upload(R, s);
+        // IBZ data
+        p->kint_irr.resize(3 * p->nirr);
+        p->ens_irred.assign(el->ens_irred, el->ens_irred + p->nirr * nb);
+        p->Uk_host.resize(p->nirr * nb * nb);
+        for (int64_t i = 0; i < p->nirr; ++i) {
+            int64_t v[3];
+            demux_vector0(el->indexlist_irred[i], v, el->wvmesh);
+            for (int d = 0; d < 3; ++d) p->kint_irr[3 * i + d] = (int)v[d];
+            for (int64_t m = 0; m < nb; ++m)
+                for (int64_t j = 0; j < nb; ++j) {
+                    const int64_t o = i + p->nirr * (m + nb * j);
+                    p->Uk_host[(i * nb + m) * nb + j] = make_double2(el->evecs_irred[2 * o], el->evecs_irred[2 * o + 1]);
+                }
+        }
+        HCHECK(hipStreamSynchronize(s));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    *out = p.release();
+    return ELPH_OK;
+}
+
+int64_t elph_ibzk_plan_nprocs(const elph_ibzk_plan *p) { return p ? p->nprocs : -1; }
+
+int elph_ibzk_plan_destroy(elph_ibzk_plan *p) {
+    delete p;
+    return ELPH_OK;
+}
+
+int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *counts, double *g2, int32_t on_device) {
+    if (!p) return ELPH_EINVAL;
+    elph_ctx *c = p->ctx;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        const int64_t nb = p->nb, nbr = p->nbr;
+        if (first < 1 || last > p->nirr * nb || last < first - 1)
+            throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
+        hipStream_t s = c->stream;
+        c->t_dominant_ms = 0;
+        c->n_dominant = 0;
+        HCHECK(hipEventRecord(c->ev_c, s));
+        // active states grouped by IBZ k (istate order keeps k ascending)
+        struct St { int64_t ik, m, slot; };
+        std::vector<St> act;
+        for (int64_t istate = first; istate <= last; ++istate) {
+            int64_t m, ik;
+            demux_state(istate, nb, &m, &ik);
+            const double e = p->ens_irred[(ik - 1) + p->nirr * (m - 1)];
+            const int64_t t = istate - first;
+            if (std::fabs(e - p->enref) > p->fsthick) { counts[t] = -1; continue; }
+            counts[t] = p->nprocs;
+            act.push_back({ik - 1, m - 1, t});
+        }
+        const int64_t nb2 = nb * nb, nslices = nbr * c->nwsg;
+        const int64_t KB = 32;  // IBZ k per gkRp batch
+        size_t i0 = 0;
+        while (i0 < act.size()) {
+            // collect up to KB distinct k
+            std::vector<int64_t> ks;
+            size_t i1 = i0;
+            while (i1 < act.size()) {
+                if (ks.empty() || ks.back() != act[i1].ik) {
+                    if ((int64_t)ks.size() == KB) break;
+                    ks.push_back(act[i1].ik);
+                }
+                ++i1;
+            }
+            const int64_t nkb = (int64_t)ks.size(), nst = (int64_t)(i1 - i0);
+            // kvecs of the batch (nkb,3) column-major, crystal coordinates int/mesh
+            std::vector<double> kv(3 * nkb);
+            std::vector<double2> Ukb(nkb * nb2);
+            for (int64_t b = 0; b < nkb; ++b) {
+                for (int d = 0; d < 3; ++d) kv[b + nkb * d] = (double)p->kint_irr[3 * ks[b] + d] / (double)p->Nel[d];
+                std::memcpy(&Ukb[b * nb2], &p->Uk_host[ks[b] * nb2], nb2 * sizeof(double2));
+            }
+            p->kv.upload(kv, s);
+            p->Ukb.upload(Ukb, s);
+            p->gk.alloc(nkb * nb2 * nslices);
+            gkRp_device(c, nkb, p->kv.p, p->gk.p, nb2 * nslices);
+            std::vector<int> skb(nst), sm(nst), ski(3 * nst);
+            std::vector<int64_t> sslot(nst);
+            int64_t bidx = -1, prev = -1;
+            for (int64_t j = 0; j < nst; ++j) {
+                const St &a = act[i0 + j];
+                if (a.ik != prev) { ++bidx; prev = a.ik; }
+                skb[j] = (int)bidx;
+                sm[j] = (int)a.m;
+                for (int d = 0; d < 3; ++d) ski[3 * j + d] = p->kint_irr[3 * a.ik + d];
+                sslot[j] = on_device ? a.slot : j;
+            }
+            p->st_kb.upload(skb, s);
+            p->st_m.upload(sm, s);
+            p->st_kint.upload(ski, s);
+            p->st_slot.upload(sslot, s);
+            p->h.alloc(nst * p->Kp * p->CP);
+            const int64_t tot = nst * p->Kp * p->CP;
+            hipLaunchKernelGGL(k_rotate_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nst, (int)nb,
+                               (int)nbr, (int)c->nwsg, (int)p->Kp, (int)p->CP, p->gk.p, nb2 * nslices, p->st_kb.p,
+                               p->st_m.p, p->Ukb.p, c->degq.p, p->h.p);
+            HCHECK(hipGetLastError());
+            double *dst = g2;
+            if (!on_device) {
+                p->g2stage.alloc(nst * p->nprocs);
+                dst = p->g2stage.p;
+            }
+            pair_launch(p, nst, dst, p->nprocs);
+            if (!on_device) {
+                for (int64_t j = 0; j < nst; ++j)
+                    HCHECK(hipMemcpyAsync(g2 + act[i0 + j].slot * p->nprocs, p->g2stage.p + j * p->nprocs,
+                                          p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
+                HCHECK(hipStreamSynchronize(s));
+            }
+            i0 = i1;
+        }
+        HCHECK(hipEventRecord(c->ev_d, s));
+        HCHECK(hipEventSynchronize(c->ev_d));
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->ev_c, c->ev_d));
+        c->t_all_ms = ms;
+    } catch (const Error &e) {
+        return ret_error(c, e);
+    }
+    return ELPH_OK;
+}
+
+int elph_ibzk_g2(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int64_t mesh_ref, int32_t precision,
+                 int64_t first, int64_t last, int64_t *counts, double *g2) {
+    elph_ibzk_plan *p = nullptr;
+    int st = elph_ibzk_plan_create(ctx, el, ph, mesh_ref, precision, &p);
+    if (st != ELPH_OK) return st;
+    st = elph_ibzk_plan_run(p, first, last, counts, g2, 0);
+    elph_ibzk_plan_destroy(p);
+    return st;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// gReq and the ibzq plan (calculate_eph_interaction_ibzq, key 'g')
+// ===========================================================================
+namespace {
+
+// Batched gReq into device memory: out[q] (nb,nb,nbr,nwsk) for q vectors on the device.
+void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, int64_t out_qstride) {
+    const int64_t nb2 = c->nb * c->nb;
+    const int64_t KT = 64;
+    DBuf<double2> phiT;
+    phiT.alloc(c->nwsg * KT);
+    const int64_t nrb = (nb2 * c->nwsk + 15) / 16;
+    for (int64_t q0 = 0; q0 < nq; q0 += KT) {
+        const int64_t n = std::min(KT, nq - q0);
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsg * KT + 255) / 256)), dim3(256), 0, c->stream, phiT.p,
+                           c->nwsg * KT);
+        dim3 grid((unsigned)((c->nwsg + 127) / 128), (unsigned)n);
+        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, c->stream, (int)n, (int)c->nwsg, d_qvecs + q0, nq,
+                           c->rg.p, c->degg.p, phiT.p, (int64_t)1, KT);
+        hipLaunchKernelGGL(k_gReq<4>, dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256), 0, c->stream,
+                           c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->nwsg, phiT.p, (int)KT, (int)n,
+                           out + (size_t)q0 * out_qstride, out_qstride);
+        HCHECK(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+struct elph_ibzq_plan {
+    elph_ctx *ctx = nullptr;
+    int32_t precision = ELPH_FP64;
+    int64_t nb = 0, nbr = 0, nwv = 0, stride = 0;
+    int64_t Nel[3], Nph[3], mesh_ref = 1;
+    int64_t L = 1, Lf[3];
+    int64_t Kp = 0, CP = 0;
+    int64_t nirr_q = 0, nq = 0;
+    std::vector<int64_t> q_irr;   // IBZ q muxed (phonon mesh)
+    std::vector<double> ph_ens;   // (nq, nbr)
+    std::vector<double> ph_evecs; // (nq, nbr, nbr) c128
+    DBuf<int> kint, nwin, wband, lut, Rk;
+    DBuf<double2> Vk, T;
+    DBuf<double2> gq, h, st_u;
+    DBuf<double> qv, st_om;
+    DBuf<int> st_qb, st_qe, rowmap;
+    DBuf<int64_t> rowoff, st_tot, st_slot;
+    DBuf<double> g2stage;
+};
+
+extern "C" {
+
+int elph_gReq(elph_ctx *ctx, int64_t nq, const double *qvecs, double *out) {
+    if (!ctx || nq < 0) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t blk = ctx->nb * ctx->nb * ctx->nbr * ctx->nwsk;
+        DBuf<double> dq;
+        DBuf<double2> dout;
+        dq.upload(qvecs, 3 * nq, ctx->stream);
+        dout.alloc(nq * blk);
+        gReq_device(ctx, nq, dq.p, dout.p, blk);
+        HCHECK(hipMemcpyAsync(out, dout.p, nq * blk * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+        HCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
+}
+
+int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int32_t precision,
+                          elph_ibzq_plan **out) {
+    if (!ctx || !el || !ph || !out) return ELPH_EINVAL;
+    *out = nullptr;
+    auto p = std::make_unique<elph_ibzq_plan>();
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        if (precision != ELPH_FP64 && precision != ELPH_MIXED) throw Error{ELPH_EINVAL, "unknown precision"};
+        const int64_t nb = ctx->nb, nbr = ctx->nbr, nwv = el->nwv;
+        if (nwv <= 0) throw Error{ELPH_EWINDOW, "No states found within Fermi window."};
+        for (int d = 0; d < 3; ++d) {
+            if (ph->wvmesh[d] <= 0 || el->wvmesh[d] % ph->wvmesh[d] != 0)
+                throw Error{ELPH_EINVAL, "electron mesh must be a multiple of the phonon mesh"};
+            p->Nel[d] = el->wvmesh[d];
+            p->Nph[d] = ph->wvmesh[d];
+        }
+        p->mesh_ref = p->Nel[0] / p->Nph[0];
+        p->ctx = ctx;
+        p->precision = precision;
+        p->nb = nb;
+        p->nbr = nbr;
+        p->nwv = nwv;
+        p->stride = el->nstates_inwindow * nb;
+        if (nb * nb > 96) throw Error{ELPH_EINVAL, "numwannbands^2 > 96 is not supported by the fused ibzq kernel"};
+        p->CP = ((nb * nb + 15) / 16) * 16;
+        p->Kp = ((ctx->nwsk + 31) / 32) * 32;
+        p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
+        for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
+        hipStream_t s = ctx->stream;
+        const int64_t nmesh = p->Nel[0] * p->Nel[1] * p->Nel[2];
+        std::vector<int> ki(3 * nwv), nw(nwv), wb(nwv * nb, 0), lut(nmesh, -1);
+        std::vector<double2> V(nwv * nb * nb);
+        for (int64_t r = 0; r < nwv; ++r) {
+            int64_t v[3];
+            demux_vector0(el->indexlist[r], v, el->wvmesh);
+            for (int d = 0; d < 3; ++d) ki[3 * r + d] = (int)v[d];
+            lut[el->indexlist[r] - 1] = (int)r;
+            int cnt = 0;
+            for (int64_t n = 0; n < nb; ++n)
+                if (std::fabs(el->ens[r + nwv * n] - el->enref) <= el->fsthick) wb[r * nb + cnt++] = (int)n;
+            nw[r] = cnt;
+            for (int64_t n = 0; n < nb; ++n)
+                for (int64_t j = 0; j < nb; ++j) {
+                    const int64_t o = r + nwv * (n + nb * j);
+                    V[(r * nb + n) * nb + j] = make_double2(el->evecs[2 * o], el->evecs[2 * o + 1]);
+                }
+        }
+        p->kint.upload(ki, s);
+        p->nwin.upload(nw, s);
+        p->wband.upload(wb, s);
+        p->lut.upload(lut, s);
+        p->Vk.upload(V, s);
+        std::vector<double2> T(p->L);
+        for (int64_t j = 0; j < p->L; ++j) {
+            const double x = kTwoPi * (double)j / (double)p->L;
+            T[j] = make_double2(std::cos(x), std::sin(x));
+        }
+        p->T.upload(T, s);
+        std::vector<int> R(4 * p->Kp, 0);
+        for (int64_t i = 0; i < ctx->nwsk; ++i)
+            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_k[i + ctx->nwsk * d];
+        p->Rk.upload(R, s);
+        p->nirr_q = ph->nwv_irred;
+        p->nq = ph->nwv;
+        p->q_irr.assign(ph->indexlist_irred, ph->indexlist_irred + ph->nwv_irred);
+        p->ph_ens.assign(ph->ens, ph->ens + ph->nwv * nbr);
+        p->ph_evecs.assign(ph->evecs, ph->evecs + 2 * ph->nwv * nbr * nbr);
+        HCHECK(hipStreamSynchronize(s));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    *out = p.release();
+    return ELPH_OK;
+}
+
+int64_t elph_ibzq_plan_stride(const elph_ibzq_plan *p) { return p ? p->stride : -1; }
+
+int elph_ibzq_plan_destroy(elph_ibzq_plan *p) {
+    delete p;
+    return ELPH_OK;
+}
+
+int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *counts, double *g2, int32_t on_device) {
+    if (!p) return ELPH_EINVAL;
+    elph_ctx *c = p->ctx;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb;
+        if (first < 1 || last > p->nirr_q * nbr || last < first - 1)
+            throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numbranches"};
+        hipStream_t s = c->stream;
+        c->t_dominant_ms = 0;
+        c->n_dominant = 0;
+        HCHECK(hipEventRecord(c->ev_c, s));
+        const int64_t QB = 32;
+        const int64_t gblk = nb2 * nbr * c->nwsk;
+        int64_t istate = first;
+        while (istate <= last) {
+            // states of up to QB distinct IBZ q
+            std::vector<int64_t> qs, sts;
+            int64_t j = istate;
+            for (; j <= last; ++j) {
+                int64_t sb, iq;
+                demux_state(j, nbr, &sb, &iq);
+                if (qs.empty() || qs.back() != iq - 1) {
+                    if ((int64_t)qs.size() == QB) break;
+                    qs.push_back(iq - 1);
+                }
+                sts.push_back(j);
+            }
+            const int64_t nqb = (int64_t)qs.size(), nst = (int64_t)sts.size();
+            std::vector<double> qv(3 * nqb);
+            for (int64_t b = 0; b < nqb; ++b) {
+                int64_t v[3];
+                demux_vector0(p->q_irr[qs[b]], v, p->Nph);
+                for (int d = 0; d < 3; ++d) qv[b + nqb * d] = (double)v[d] / (double)p->Nph[d];
+            }
+            p->qv.upload(qv, s);
+            p->gq.alloc(nqb * gblk);
+            gReq_device(c, nqb, p->qv.p, p->gq.p, gblk);
+            std::vector<int> sqb(nst), sqe(3 * nst);
+            std::vector<double2> su(nst * nbr);
+            std::vector<double> som(nst);
+            std::vector<int64_t> sslot(nst);
+            int64_t bidx = -1, prev = -1;
+            for (int64_t t = 0; t < nst; ++t) {
+                int64_t sb, iq;
+                demux_state(sts[t], nbr, &sb, &iq);
+                if (iq - 1 != prev) { ++bidx; prev = iq - 1; }
+                sqb[t] = (int)bidx;
+                const int64_t iqf = p->q_irr[iq - 1];  // muxed, phonon mesh
+                int64_t v[3];
+                demux_vector0(iqf, v, p->Nph);
+                for (int d = 0; d < 3; ++d) sqe[3 * t + d] = (int)(v[d] * p->mesh_ref);
+                som[t] = p->ph_ens[(iqf - 1) + p->nq * (sb - 1)];
+                for (int64_t sp = 0; sp < nbr; ++sp) {
+                    const int64_t o = (iqf - 1) + p->nq * ((sb - 1) + nbr * sp);
+                    const double sc = 1.0 / std::sqrt(c->masses[sp / 3]);
+                    su[t * nbr + sp] = make_double2(p->ph_evecs[2 * o] * sc, p->ph_evecs[2 * o + 1] * sc);
+                }
+                sslot[t] = on_device ? (sts[t] - first) : t;
+            }
+            p->st_qb.upload(sqb, s);
+            p->st_qe.upload(sqe, s);
+            p->st_u.upload(su, s);
+            p->st_om.upload(som, s);
+            p->st_slot.upload(sslot, s);
+            p->h.alloc(nst * p->Kp * p->CP);
+            const int64_t tot = nst * p->Kp * p->CP;
+            hipLaunchKernelGGL(k_rotate_hq, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nst, (int)nb,
+                               (int)nbr, (int)c->nwsk, (int)p->Kp, (int)p->CP, p->gq.p, gblk, p->st_qb.p, p->st_u.p,
+                               c->degk.p, p->h.p);
+            p->rowmap.alloc(nst * p->nwv);
+            p->rowoff.alloc(nst * p->nwv);
+            p->st_tot.alloc(nst);
+            hipLaunchKernelGGL(k_ibzq_rows, dim3((unsigned)((p->nwv + 255) / 256), (unsigned)nst), dim3(256), 0, s,
+                               (int)nst, (int)p->nwv, p->st_qe.p, p->kint.p, (int)p->Nel[0], (int)p->Nel[1],
+                               (int)p->Nel[2], p->lut.p, p->nwin.p, p->rowmap.p, p->rowoff.p);
+            hipLaunchKernelGGL(k_scan_rows, dim3((unsigned)nst), dim3(1024), 0, s, (int)p->nwv, p->rowoff.p, p->st_tot.p);
+            HCHECK(hipGetLastError());
+            double *dst = g2;
+            if (!on_device) {
+                p->g2stage.alloc(nst * p->stride);
+                dst = p->g2stage.p;
+            }
+            PairQArgs a{};
+            a.h = p->h.p;
+            a.Kp = (int)p->Kp;
+            a.CP = (int)p->CP;
+            a.nwv = (int)p->nwv;
+            a.kint = p->kint.p;
+            a.L = (int)p->L;
+            a.Lf0 = (int)p->Lf[0]; a.Lf1 = (int)p->Lf[1]; a.Lf2 = (int)p->Lf[2];
+            a.T = p->T.p;
+            a.Rk = p->Rk.p;
+            a.Vk = p->Vk.p;
+            a.nwin = p->nwin.p;
+            a.wband = p->wband.p;
+            a.rowmap = p->rowmap.p;
+            a.rowoff = p->rowoff.p;
+            a.st_om = p->st_om.p;
+            a.st_slot = p->st_slot.p;
+            a.nb = (int)nb;
+            a.out = dst;
+            a.stride = p->stride;
+            const int CB = (int)(p->CP / 16);
+            const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+            dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst);
+            HCHECK(hipEventRecord(c->ev_a, s));
+            switch (CB) {
+                case 1: hipLaunchKernelGGL(k_pair_q_fp64<1>, grid, dim3(256), lds, s, a); break;
+                case 2: hipLaunchKernelGGL(k_pair_q_fp64<2>, grid, dim3(256), lds, s, a); break;
+                case 3: hipLaunchKernelGGL(k_pair_q_fp64<3>, grid, dim3(256), lds, s, a); break;
+                case 4: hipLaunchKernelGGL(k_pair_q_fp64<4>, grid, dim3(256), lds, s, a); break;
+                case 5: hipLaunchKernelGGL(k_pair_q_fp64<5>, grid, dim3(256), lds, s, a); break;
+                case 6: hipLaunchKernelGGL(k_pair_q_fp64<6>, grid, dim3(256), lds, s, a); break;
+                default: throw Error{ELPH_EINVAL, "unsupported column count"};
+            }
+            HCHECK(hipGetLastError());
+            HCHECK(hipEventRecord(c->ev_b, s));
+            std::vector<int64_t> tot_h(nst);
+            HCHECK(hipMemcpyAsync(tot_h.data(), p->st_tot.p, nst * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            HCHECK(hipStreamSynchronize(s));
+            float ms = 0;
+            HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+            c->t_dominant_ms += ms;
+            c->n_dominant += 1;
+            for (int64_t t = 0; t < nst; ++t) {
+                counts[sts[t] - first] = tot_h[t];
+                if (tot_h[t] > p->stride) throw Error{ELPH_EINVAL, "ibzq count exceeds the row stride"};
+            }
+            if (!on_device) {
+                for (int64_t t = 0; t < nst; ++t)
+                    HCHECK(hipMemcpyAsync(g2 + (sts[t] - first) * p->stride, p->g2stage.p + t * p->stride,
+                                          tot_h[t] * sizeof(double), hipMemcpyDeviceToHost, s));
+                HCHECK(hipStreamSynchronize(s));
+            }
+            istate = j;
+        }
+        HCHECK(hipEventRecord(c->ev_d, s));
+        HCHECK(hipEventSynchronize(c->ev_d));
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->ev_c, c->ev_d));
+        c->t_all_ms = ms;
+    } catch (const Error &e) {
+        return ret_error(c, e);
+    }
+    return ELPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,365 @@
+// Host-side pieces of the vertex path: index search, crystal geometry,
+// symmetry operations and the irreducible wedge, the EPW reader, WS wrapping
+// vectors and the g2 file writer.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <unordered_set>
+
+#include "common.h"
+
+namespace elph {
+
+// src/misc.f90:738-763: bisection on a sorted list; the midpoint rule picks
+// index 3 of [-105,0,105,105,105] for 105 (test/test_misc.f90:110-116).
+int64_t binsearch(const int64_t *array, int64_t n, int64_t e) {
+    int64_t a = 1, b = n, m = (b + a) / 2;
+    int64_t mid = array[m - 1];
+    while (mid != e) {
+        if (e > mid) a = m + 1;
+        else if (e < mid) b = m - 1;
+        if (a > b) return -1;
+        m = (b + a) / 2;
+        mid = array[m - 1];
+    }
+    return m;
+}
+
+void reciprocal_lattice(const double L[9], double rec[9], double *volume) {
+    for (int i = 0; i < 3; ++i) {
+        int j = (i + 1) % 3, k = (j + 1) % 3;
+        const double *a = &L[3 * j], *b = &L[3 * k];
+        rec[3 * i + 0] = a[1] * b[2] - a[2] * b[1];
+        rec[3 * i + 1] = a[2] * b[0] - a[0] * b[2];
+        rec[3 * i + 2] = a[0] * b[1] - a[1] * b[0];
+    }
+    double vol = std::fabs(L[0] * rec[0] + L[1] * rec[1] + L[2] * rec[2]);
+    double vbz = 2.0 * M_PI / vol;
+    for (int i = 0; i < 9; ++i) rec[i] *= vbz;
+    *volume = vol;
+}
+
+static void matvec3(const double m[9], const double v[3], double out[3]) {
+    for (int r = 0; r < 3; ++r) out[r] = m[r] * v[0] + m[r + 3] * v[1] + m[r + 6] * v[2];
+}
+
+// Operations x -> R x + t (fractional coordinates) that keep the lattice metric
+// and carry every atom onto an atom of its species.  spglib's dataset
+// (src/symmetry.f90:97) lists the same operations for these structures; only
+// the set matters downstream because the irreducible wedge keeps the first
+// mesh point of each orbit.  Rotation entries of a lattice-preserving matrix
+// in a reduced basis of a cubic/fcc/bcc lattice lie in {-1,0,1}.
+int find_point_group(const double L[9], const double *basis, const int64_t *types, int64_t natoms, double symprec,
+                     std::vector<int> &rot) {
+    double G[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int c = 0; c < 3; ++c) s += L[3 * i + c] * L[3 * j + c];
+            G[i + 3 * j] = s;
+        }
+    const double gscale = (G[0] + G[4] + G[8]) / 3.0;
+    rot.clear();
+    int e[9];
+    for (int code = 0; code < 19683; ++code) {
+        int c = code;
+        for (int i = 0; i < 9; ++i) { e[i] = c % 3 - 1; c /= 3; }
+        const int det = e[0] * (e[4] * e[8] - e[7] * e[5]) - e[3] * (e[1] * e[8] - e[7] * e[2]) +
+                        e[6] * (e[1] * e[5] - e[4] * e[2]);
+        if (det != 1 && det != -1) continue;
+        bool ok = true;
+        for (int i = 0; i < 3 && ok; ++i)
+            for (int j = 0; j < 3 && ok; ++j) {
+                double s = 0;
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) s += e[a + 3 * i] * G[a + 3 * b] * e[b + 3 * j];
+                if (std::fabs(s - G[i + 3 * j]) > 1e-8 * gscale) ok = false;
+            }
+        if (!ok) continue;
+        bool found = false;
+        for (int64_t j0 = 0; j0 < natoms && !found; ++j0) {
+            if (types[j0] != types[0]) continue;
+            double t[3];
+            for (int r = 0; r < 3; ++r)
+                t[r] = basis[3 * j0 + r] - (e[r] * basis[0] + e[r + 3] * basis[1] + e[r + 6] * basis[2]);
+            bool all = true;
+            for (int64_t a = 0; a < natoms && all; ++a) {
+                double x[3];
+                for (int r = 0; r < 3; ++r)
+                    x[r] = e[r] * basis[3 * a] + e[r + 3] * basis[3 * a + 1] + e[r + 6] * basis[3 * a + 2] + t[r];
+                bool hit = false;
+                for (int64_t b = 0; b < natoms && !hit; ++b) {
+                    if (types[b] != types[a]) continue;
+                    double d[3], dc[3];
+                    for (int r = 0; r < 3; ++r) { d[r] = x[r] - basis[3 * b + r]; d[r] -= std::nearbyint(d[r]); }
+                    matvec3(L, d, dc);
+                    if (std::sqrt(dc[0] * dc[0] + dc[1] * dc[1] + dc[2] * dc[2]) < symprec) hit = true;
+                }
+                all = hit;
+            }
+            found = all;
+        }
+        if (!found) continue;
+        rot.insert(rot.end(), e, e + 9);
+    }
+    return (int)(rot.size() / 9);
+}
+
+static void inv3(const double a[9], double r[9]) {
+    double det = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
+                 a[6] * (a[1] * a[5] - a[4] * a[2]);
+    r[0] = (a[4] * a[8] - a[5] * a[7]) / det; r[3] = (a[6] * a[5] - a[3] * a[8]) / det;
+    r[6] = (a[3] * a[7] - a[6] * a[4]) / det; r[1] = (a[7] * a[2] - a[1] * a[8]) / det;
+    r[4] = (a[0] * a[8] - a[6] * a[2]) / det; r[7] = (a[6] * a[1] - a[0] * a[7]) / det;
+    r[2] = (a[1] * a[5] - a[4] * a[2]) / det; r[5] = (a[3] * a[2] - a[0] * a[5]) / det;
+    r[8] = (a[0] * a[4] - a[3] * a[1]) / det;
+}
+
+// Image of the 0-based mesh point ijk under qrot: the muxed index, or -1 when
+// it is off the mesh (src/symmetry.f90:214-229, 277-286).
+static int64_t star_image(const double *qr, const int64_t ijk[3], const int64_t mesh[3]) {
+    double f[3] = {(double)ijk[0] / (double)mesh[0], (double)ijk[1] / (double)mesh[1], (double)ijk[2] / (double)mesh[2]};
+    double v[3];
+    matvec3(qr, f, v);
+    int64_t iv[3];
+    double d2 = 0;
+    for (int c = 0; c < 3; ++c) {
+        v[c] = (double)mesh[c] * v[c];
+        iv[c] = fnint(v[c]);
+        double d = std::fabs(v[c] - (double)iv[c]);
+        d2 += d * d;
+    }
+    if (std::sqrt(d2) >= 1e-2) return -1;
+    int64_t m[3] = {fmodulo(iv[0], mesh[0]), fmodulo(iv[1], mesh[1]), fmodulo(iv[2], mesh[2])};
+    return mux_vector0(m, mesh);
+}
+
+// src/symmetry.f90:134-211: q-space rotations G R G^-1 (G = L^T L), time
+// reversal appended, mesh-incompatible rotations and duplicates of the
+// real-space matrices dropped.
+int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_t mesh[3], std::vector<double> &qrot) {
+    const int64_t ns = (int64_t)(rot.size() / 9), n2 = 2 * ns;
+    double G[9], Gi[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0;
+            for (int c = 0; c < 3; ++c) s += L[3 * i + c] * L[3 * j + c];
+            G[i + 3 * j] = s;
+        }
+    inv3(G, Gi);
+    std::vector<double> qr(9 * n2);
+    std::vector<int> rr(9 * n2);
+    for (int64_t s = 0; s < ns; ++s) {
+        double GR[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0;
+                for (int a = 0; a < 3; ++a) v += G[i + 3 * a] * rot[9 * s + a + 3 * j];
+                GR[i + 3 * j] = v;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0;
+                for (int a = 0; a < 3; ++a) v += GR[i + 3 * a] * Gi[a + 3 * j];
+                qr[9 * s + i + 3 * j] = v;
+                qr[9 * (s + ns) + i + 3 * j] = -v;
+            }
+        for (int i = 0; i < 9; ++i) { rr[9 * s + i] = rot[9 * s + i]; rr[9 * (s + ns) + i] = -rot[9 * s + i]; }
+    }
+    std::vector<char> valid(n2, 1);
+    const int64_t nq = mesh[0] * mesh[1] * mesh[2];
+    for (int64_t s = 0; s < n2; ++s)
+        for (int64_t i = 1; i <= nq && valid[s]; ++i) {
+            int64_t ijk[3];
+            demux_vector0(i, ijk, mesh);
+            if (star_image(&qr[9 * s], ijk, mesh) < 0) valid[s] = 0;
+        }
+    for (int64_t s = 1; s < n2; ++s)
+        for (int64_t t = 0; t < s; ++t) {
+            if (!valid[t]) continue;
+            if (std::memcmp(&rr[9 * s], &rr[9 * t], 9 * sizeof(int)) == 0) { valid[s] = 0; break; }
+        }
+    qrot.clear();
+    for (int64_t s = 0; s < n2; ++s)
+        if (valid[s]) qrot.insert(qrot.end(), &qr[9 * s], &qr[9 * s] + 9);
+    return (int64_t)(qrot.size() / 9);
+}
+
+// src/symmetry.f90:303-451 find_irred_wedge.  A point starts a new orbit when
+// no earlier orbit reached it; membership is a flag array over the mesh instead
+// of the reference's distributed running-list scan (same result).
+int64_t irred_wedge(const int64_t mesh[3], int64_t nsr, const double *qrot, const int64_t *indexlist, int64_t nlist,
+                    int64_t *irr, int64_t *nequiv, int64_t *ibz2fbz) {
+    const int64_t nmesh = mesh[0] * mesh[1] * mesh[2];
+    std::vector<char> seen(nmesh + 1, 0);
+    std::vector<int64_t> images(nsr);
+    int64_t nirr = 0, counter = 0;
+    for (int64_t i = 0; i < nlist; ++i) {
+        const int64_t imux = indexlist ? indexlist[i] : i + 1;
+        if (seen[imux]) continue;
+        int64_t ijk[3];
+        demux_vector0(imux, ijk, mesh);
+        irr[nirr] = imux;
+        int64_t neq = 0;
+        for (int64_t s = 0; s < nsr; ++s) {
+            const int64_t image = star_image(&qrot[9 * s], ijk, mesh);
+            bool present = false;
+            for (int64_t l = 0; l < neq; ++l)
+                if (images[l] == image) { present = true; break; }
+            if (present) continue;
+            images[neq] = image;
+            if (ibz2fbz) {
+                ibz2fbz[neq + nsr * (nirr + nlist * 0)] = s + 1;
+                ibz2fbz[neq + nsr * (nirr + nlist * 1)] = image;
+            }
+            neq++;
+            if (image >= 1 && image <= nmesh) seen[image] = 1;
+        }
+        nequiv[nirr] = neq;
+        counter += neq;
+        nirr++;
+    }
+    if (counter != nlist) return -1;
+    return nirr;
+}
+
+// src/wannier.f90:532-582: lattice vectors S with |lvec (frac(vl) + S')| minimal
+// (within eps) over a 7^3 search box; S = S' - int(vl) (truncation toward zero).
+int ws_wrapping_vectors(const double lvec[9], const double vl[3], double eps, std::vector<int> &wrappers) {
+    int iv0[3];
+    double wl[3];
+    for (int c = 0; c < 3; ++c) { iv0[c] = (int)vl[c]; wl[c] = vl[c] - (double)iv0[c]; }
+    double d = 1.7976931348623157e308;
+    int n = 0;
+    double norms[343];
+    int idx = 0;
+    for (int k = -3; k <= 3; ++k)
+        for (int j = -3; j <= 3; ++j)
+            for (int i = -3; i <= 3; ++i, ++idx) {
+                double v[3] = {wl[0] + i, wl[1] + j, wl[2] + k}, wc[3];
+                matvec3(lvec, v, wc);
+                double nrm = std::sqrt(wc[0] * wc[0] + wc[1] * wc[1] + wc[2] * wc[2]);
+                norms[idx] = nrm;
+                if (std::fabs(nrm - d) < eps) n = n + 1;
+                else if (nrm < d) { n = 1; d = nrm; }
+            }
+    wrappers.clear();
+    idx = 0;
+    for (int k = -3; k <= 3; ++k)
+        for (int j = -3; j <= 3; ++j)
+            for (int i = -3; i <= 3; ++i, ++idx)
+                if (std::fabs(norms[idx] - d) < eps) {
+                    wrappers.push_back(i - iv0[0]);
+                    wrappers.push_back(j - iv0[1]);
+                    wrappers.push_back(k - iv0[2]);
+                }
+    return (int)(wrappers.size() / 3);
+}
+
+// Fortran list-directed complex "(re,im)".
+static bool parse_complex(const std::string &s, double &re, double &im) {
+    size_t a = s.find('('), b = s.find(','), c = s.find(')');
+    if (a == std::string::npos || b == std::string::npos || c == std::string::npos) return false;
+    re = std::strtod(s.c_str() + a + 1, nullptr);
+    im = std::strtod(s.c_str() + b + 1, nullptr);
+    return true;
+}
+
+int read_epw(const std::string &dir, EpwData &o, std::string &err) {
+    std::ifstream f(dir + "/epwdata.fmt");
+    if (!f) { err = "cannot open " + dir + "/epwdata.fmt"; return ELPH_EIO; }
+    double ef;
+    f >> ef;
+    f >> o.nb >> o.nwsk >> o.nbr >> o.nwsq >> o.nwsg;
+    if (!f || o.nb <= 0 || o.nbr <= 0 || o.nwsk <= 0 || o.nwsq <= 0 || o.nwsg <= 0) {
+        err = "bad header in epwdata.fmt"; return ELPH_EIO;
+    }
+    const int64_t ndummy = (o.nbr / 3 + 1) * 9;  // Born charges + epsilon, read and ignored
+    for (int64_t i = 0; i < ndummy; ++i) { double x; f >> x; }
+    std::string tok;
+    auto read_block = [&](std::vector<double> &dst, int64_t nws, int64_t n) -> bool {
+        dst.assign(2 * nws * n * n, 0.0);
+        for (int64_t ib = 0; ib < n; ++ib)
+            for (int64_t jb = 0; jb < n; ++jb)
+                for (int64_t iuc = 0; iuc < nws; ++iuc) {
+                    if (!(f >> tok)) return false;
+                    double re, im;
+                    if (!parse_complex(tok, re, im)) return false;
+                    const int64_t off = iuc + nws * (ib + n * jb);
+                    dst[2 * off] = re;
+                    dst[2 * off + 1] = im;
+                }
+        return true;
+    };
+    if (!read_block(o.Hwann, o.nwsk, o.nb)) { err = "bad Hwann records in epwdata.fmt"; return ELPH_EIO; }
+    if (!read_block(o.Dphwann, o.nwsq, o.nbr)) { err = "bad Dphwann records in epwdata.fmt"; return ELPH_EIO; }
+    const int64_t ng = o.nb * o.nb * o.nwsk * o.nbr * o.nwsg;
+    o.gwann.resize(2 * ng);
+    FILE *fb = std::fopen((dir + "/epmatwp1").c_str(), "rb");
+    if (!fb) { err = "cannot open " + dir + "/epmatwp1"; return ELPH_EIO; }
+    size_t got = std::fread(o.gwann.data(), sizeof(double), 2 * ng, fb);
+    std::fclose(fb);
+    if ((int64_t)got != 2 * ng) { err = "epmatwp1 is shorter than nb*nb*nwsk*nbr*nwsg complex values"; return ELPH_EIO; }
+    auto read_cells = [&](const char *tag, int64_t n, std::vector<int64_t> &cells, std::vector<int64_t> &deg) -> bool {
+        std::ifstream fc(dir + "/rcells_" + tag), fd(dir + "/wsdeg_" + tag);
+        if (!fc || !fd) return false;
+        cells.assign(3 * n, 0);
+        deg.assign(n, 0);
+        for (int64_t i = 0; i < n; ++i) {  // (nws,3) column-major
+            if (!(fc >> cells[i] >> cells[i + n] >> cells[i + 2 * n])) return false;
+            if (!(fd >> deg[i])) return false;
+        }
+        return true;
+    };
+    if (!read_cells("k", o.nwsk, o.rcells_k, o.elwsdeg)) { err = "bad rcells_k/wsdeg_k"; return ELPH_EIO; }
+    if (!read_cells("q", o.nwsq, o.rcells_q, o.phwsdeg)) { err = "bad rcells_q/wsdeg_q"; return ELPH_EIO; }
+    if (!read_cells("g", o.nwsg, o.rcells_g, o.gwsdeg)) { err = "bad rcells_g/wsdeg_g"; return ELPH_EIO; }
+    return ELPH_OK;
+}
+
+}  // namespace elph
+
+extern "C" int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates,
+                                   const int64_t *counts, const double *g2, int64_t row_stride) {
+    if (kind != 'k' && kind != 'q') return ELPH_EINVAL;
+    for (int64_t t = 0; t < nstates; ++t) {
+        if (counts[t] < 0) continue;
+        char name[64];
+        std::snprintf(name, sizeof(name), "%s.istate%lld", kind == 'k' ? "gk2" : "gq2",
+                      (long long)(istate_first + t));
+        FILE *f = std::fopen((std::string(dir) + "/" + name).c_str(), "wb");
+        if (!f) return ELPH_EIO;
+        int64_t c = counts[t];
+        std::fwrite(&c, sizeof(int64_t), 1, f);
+        const int64_t nwrite = kind == 'k' ? row_stride : c;
+        std::fwrite(g2 + t * row_stride, sizeof(double), nwrite, f);
+        std::fclose(f);
+    }
+    return ELPH_OK;
+}
+
+extern "C" int elph_symmetry(const double *lattvecs, int64_t natoms, const double *basis, const int64_t *atomtypes,
+                             double symprec, const int64_t *mesh, int64_t *nsymm, int64_t *nsymm_rot,
+                             double *qrotations) {
+    std::vector<int> rot;
+    int ns = elph::find_point_group(lattvecs, basis, atomtypes, natoms, symprec, rot);
+    if (ns <= 0) return ELPH_EINVAL;
+    std::vector<double> qr;
+    int64_t nsr = elph::q_rotations(lattvecs, rot, mesh, qr);
+    *nsymm = ns;
+    *nsymm_rot = nsr;
+    std::memcpy(qrotations, qr.data(), sizeof(double) * qr.size());
+    return ELPH_OK;
+}
+
+extern "C" int elph_irred_wedge(const int64_t *mesh, int64_t nsymm_rot, const double *qrotations, int64_t nlist,
+                                const int64_t *indexlist, int64_t *nirr, int64_t *indexlist_irred, int64_t *nequiv,
+                                int64_t *ibz2fbz) {
+    int64_t n = elph::irred_wedge(mesh, nsymm_rot, qrotations, indexlist, nlist, indexlist_irred, nequiv, ibz2fbz);
+    if (n < 0) return ELPH_EWEDGE;
+    *nirr = n;
+    return ELPH_OK;
+}

@@ -1,0 +1,205 @@
+"""Host mirror of the vertex drivers of src/interactions.f90:
+calculate_eph_interaction_ibzk (2095-2400) and calculate_eph_interaction_ibzq
+(1673-1930), key 'g', plus calculate_gkRp / calculate_gReq (1603-1671).
+
+The reference streams gkRp.ik* / gReq.iq* files between the two phases; here
+the mixed-representation tensors stay in HBM inside one plan run.  Outputs
+are the per-state vertex lists in the reference's order and, on request, the
+reference's gk2.istate* / gq2.istate* files.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import ElphElectrons, ElphError, ElphPhonons, c_dblp, c_i64p
+from .wannier import Wannier, _p, c128, f64, i64
+
+PRECISION = {"fp64": _lib.ELPH_FP64, "mixed": _lib.ELPH_MIXED}
+
+
+def _el_struct(el):
+    s = ElphElectrons()
+    keep = []
+    for i in range(3):
+        s.wvmesh[i] = int(el.wvmesh[i])
+    s.enref, s.fsthick = float(el.enref), float(el.fsthick)
+    s.nwv_irred, s.nwv, s.nstates_inwindow = int(el.nwv_irred), int(el.nwv), int(el.nstates_inwindow)
+    for name, conv, t in (("indexlist_irred", i64, c_i64p), ("ens_irred", f64, c_dblp), ("evecs_irred", c128, c_dblp),
+                          ("indexlist", i64, c_i64p), ("ens", f64, c_dblp), ("evecs", c128, c_dblp)):
+        a = conv(getattr(el, name))
+        keep.append(a)
+        setattr(s, name, _p(a, t))
+    return s, keep
+
+
+def _ph_struct(ph):
+    s = ElphPhonons()
+    for i in range(3):
+        s.wvmesh[i] = int(ph.wvmesh[i])
+    s.nwv = int(ph.nwv)
+    e, v, irr = f64(ph.ens), c128(ph.evecs), i64(ph.indexlist_irred)
+    s.ens, s.evecs = _p(e), _p(v)
+    s.nwv_irred = len(irr)
+    s.indexlist_irred = _p(irr, c_i64p)
+    return s, [e, v, irr]
+
+
+def _dev_ptr(out):
+    """(pointer, on_device) of an output buffer: a CUDA/HIP torch tensor (device)
+    or a numpy array (host)."""
+    if hasattr(out, "data_ptr") and getattr(out, "is_cuda", False):
+        return ct.c_void_p(out.data_ptr()), 1
+    return out.ctypes.data_as(ct.c_void_p), 0
+
+
+class IbzkPlan:
+    """calculate_eph_interaction_ibzk('g') with HBM-resident inputs."""
+
+    def __init__(self, wann: Wannier, el, ph, mesh_ref: int = 1, precision: str = "fp64"):
+        self.wann, self.lib = wann, wann.lib
+        self.nb, self.nbr = wann.numwannbands, wann.numbranches
+        self.nwv_irred = int(el.nwv_irred)
+        es, k1 = _el_struct(el)
+        ps, k2 = _ph_struct(ph)
+        plan = ct.c_void_p()
+        st = self.lib.elph_ibzk_plan_create(wann.ctx, ct.byref(es), ct.byref(ps), int(mesh_ref),
+                                            PRECISION[precision], ct.byref(plan))
+        wann._check(st, "ibzk plan")
+        self.plan = plan
+        self.nprocs = int(self.lib.elph_ibzk_plan_nprocs(plan))
+
+    def run(self, istate_first=1, istate_last=None, out=None):
+        if istate_last is None:
+            istate_last = self.nwv_irred * self.nb
+        nst = istate_last - istate_first + 1
+        counts = np.zeros(max(nst, 1), np.int64)
+        if out is None:
+            out = np.zeros((nst, self.nprocs))
+        ptr, dev = _dev_ptr(out)
+        st = self.lib.elph_ibzk_plan_run(self.plan, istate_first, istate_last, _p(counts, c_i64p), ptr, dev)
+        self.wann._check(st, "ibzk run")
+        return counts[:nst], out
+
+    def timing(self):
+        t = np.zeros(3)
+        self.lib.elph_last_timing(self.wann.ctx, _p(t))
+        return {"dominant_ms": t[0], "launches": int(t[1]), "all_ms": t[2]}
+
+    def close(self):
+        if getattr(self, "plan", None):
+            self.lib.elph_ibzk_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IbzqPlan:
+    """calculate_eph_interaction_ibzq('g') with HBM-resident inputs."""
+
+    def __init__(self, wann: Wannier, el, ph, precision: str = "fp64"):
+        self.wann, self.lib = wann, wann.lib
+        self.nb, self.nbr = wann.numwannbands, wann.numbranches
+        self.nwv_irred_q = int(ph.nwv_irred)
+        es, k1 = _el_struct(el)
+        ps, k2 = _ph_struct(ph)
+        plan = ct.c_void_p()
+        st = self.lib.elph_ibzq_plan_create(wann.ctx, ct.byref(es), ct.byref(ps), PRECISION[precision],
+                                            ct.byref(plan))
+        wann._check(st, "ibzq plan")
+        self.plan = plan
+        self.stride = int(self.lib.elph_ibzq_plan_stride(plan))
+
+    def run(self, istate_first=1, istate_last=None, out=None):
+        if istate_last is None:
+            istate_last = self.nwv_irred_q * self.nbr
+        nst = istate_last - istate_first + 1
+        counts = np.zeros(max(nst, 1), np.int64)
+        if out is None:
+            out = np.zeros((nst, self.stride))
+        ptr, dev = _dev_ptr(out)
+        st = self.lib.elph_ibzq_plan_run(self.plan, istate_first, istate_last, _p(counts, c_i64p), ptr, dev)
+        self.wann._check(st, "ibzq run")
+        return counts[:nst], out
+
+    def timing(self):
+        t = np.zeros(3)
+        self.lib.elph_last_timing(self.wann.ctx, _p(t))
+        return {"dominant_ms": t[0], "launches": int(t[1]), "all_ms": t[2]}
+
+    def close(self):
+        if getattr(self, "plan", None):
+            self.lib.elph_ibzq_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_g2_files(g2dir, kind, istate_first, counts, g2):
+    """gk2.istate<i> ('k') / gq2.istate<i> ('q') in the reference's byte layout
+    (src/interactions.f90:2337-2349, 1877-1889)."""
+    lib = _lib.load()
+    os.makedirs(g2dir, exist_ok=True)
+    c, g = i64(counts), np.ascontiguousarray(g2, dtype=np.float64)
+    st = lib.elph_write_g2_files(str(g2dir).encode(), kind.encode(), istate_first, len(c), _p(c, c_i64p), _p(g),
+                                 g.shape[1])
+    if st != 0:
+        raise ElphError(st, f"writing g2 files into {g2dir} failed")
+
+
+def read_g2_file(path):
+    """Reads one gk2/gq2 file back: (count, values) (the reader of
+    src/interactions.f90:2202-2212 / 3065-3091 takes the first `count`)."""
+    with open(path, "rb") as f:
+        count = int(np.frombuffer(f.read(8), np.int64)[0])
+        vals = np.frombuffer(f.read(), np.float64)
+    return count, vals
+
+
+def calculate_eph_interaction_ibzk(wann: Wannier, el, ph, key="g", mesh_ref=1, precision="fp64",
+                                   istate_first=1, istate_last=None, g2dir=None):
+    if key not in ("g",):
+        raise ElphError(_lib.ELPH_EINVAL, "Invalid value of key in call to calculate_eph_interaction_ibzk. Exiting.")
+    plan = IbzkPlan(wann, el, ph, mesh_ref=mesh_ref, precision=precision)
+    counts, g2 = plan.run(istate_first, istate_last)
+    plan.close()
+    if g2dir is not None:
+        write_g2_files(g2dir, "k", istate_first, counts, g2)
+    return counts, g2
+
+
+def calculate_eph_interaction_ibzq(wann: Wannier, el, ph, key="g", precision="fp64", istate_first=1,
+                                   istate_last=None, g2dir=None):
+    if key not in ("g",):
+        raise ElphError(_lib.ELPH_EINVAL, "Invalid value of key in call to calculate_eph_interaction_ibzq. Exiting.")
+    plan = IbzqPlan(wann, el, ph, precision=precision)
+    counts, g2 = plan.run(istate_first, istate_last)
+    plan.close()
+    if g2dir is not None:
+        write_g2_files(g2dir, "q", istate_first, counts, g2)
+    return counts, g2
+
+
+def calculate_gkRp(wann: Wannier, el):
+    """g(k,Rp) for all IBZ k of `el` (calculate_gkRp, src/interactions.f90:1632-1671)."""
+    k = np.stack([np.array(np.unravel_index(i - 1, tuple(el.wvmesh[::-1]))[::-1], dtype=np.float64) / el.wvmesh
+                  for i in el.indexlist_irred])
+    return wann.gkRp(k)
+
+
+def calculate_gReq(wann: Wannier, ph):
+    """g(Re,q) for all IBZ q of `ph` (calculate_gReq, src/interactions.f90:1603-1630)."""
+    q = np.stack([np.array(np.unravel_index(i - 1, tuple(ph.wvmesh[::-1]))[::-1], dtype=np.float64) / ph.wvmesh
+                  for i in ph.indexlist_irred])
+    return wann.gReq(q)

@@ -1,0 +1,199 @@
+/* elph_vertex: MI355X (gfx950) electron-phonon vertex library, C ABI.
+ *
+ * Drop-in for the electron-phonon vertex path of elphbolt (Fortran, coarray).
+ * Every entry point takes plain pointers and sizes; arrays use the reference's
+ * column-major (Fortran) layouts; complex128 arrays are interleaved (re, im)
+ * doubles, i.e. `complex(r64)` memory.  Status 0 = success; any other value is
+ * an error whose text `elph_last_error()` returns (the Fortran wrapper passes
+ * it to `exit_with_message`, src/misc.f90:194-203).
+ *
+ * One context owns one GPU; its Wannier-space tensors stay resident in HBM
+ * until elph_ctx_destroy().  A context is not thread-safe (one coarray image
+ * drives it, as `resource%gpu_manager` does for the OpenACC path,
+ * src/resource.f90:75-77).
+ */
+#ifndef ELPH_VERTEX_H
+#define ELPH_VERTEX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct elph_ctx elph_ctx;
+typedef struct elph_ibzk_plan elph_ibzk_plan;
+typedef struct elph_ibzq_plan elph_ibzq_plan;
+
+enum { ELPH_OK = 0, ELPH_EINVAL = 1, ELPH_EDEVICE = 2, ELPH_EIO = 3, ELPH_ENEGPHONON = 4, ELPH_EWINDOW = 5,
+       ELPH_ENOMEM = 6, ELPH_EWEDGE = 7 };
+
+/* Arithmetic of the two Bloch-phase contractions. */
+enum { ELPH_FP64 = 0,  /* FP64 MFMA, FP64 everywhere */
+       ELPH_MIXED = 1  /* phase contraction as sliced low-precision MFMA products with exact
+                          integer accumulation, FP64 recombination; eigensolves and k.R
+                          reduction stay FP64 */ };
+
+/* Sizes of `type Wannier` (src/wannier.f90:46-93) and the crystal data the
+ * path reads (src/crystal.f90). */
+typedef struct {
+    int64_t numwannbands;   /* nb  */
+    int64_t numbranches;    /* nbr = 3*natoms */
+    int64_t nwsk, nwsq, nwsg;
+    int64_t natoms;
+    int64_t coarse_qmesh[3];
+    double lattvecs[9];     /* crys%lattvecs (3,3) nm, columns */
+    int32_t polar;          /* crys%polar */
+    double epsilon[9];      /* crys%epsilon (3,3) */
+} elph_dims;
+
+/* Replaces `wann%read` / `read_EPW_Wannier` (src/wannier.f90:289-428) when the
+ * caller already holds the arrays.  rcells_* are (nws,3) int64; *wsdeg (nws);
+ * Hwann (nwsk,nb,nb) c128 Ry; Dphwann (nwsq,nbr,nbr) c128; gwann
+ * (nb,nb,nwsk,nbr,nwsg) c128 (the un-sharded `gwann`, src/wannier.f90:41);
+ * basis (3,natoms) fractional; masses (natoms) amu, per atom (i.e.
+ * crys%masses(crys%atomtypes(:))); born (3,3,natoms) or NULL when !polar. */
+int elph_ctx_create(elph_ctx **ctx, int device, const elph_dims *dims, const int64_t *rcells_k,
+                    const int64_t *rcells_q, const int64_t *rcells_g, const int64_t *elwsdeg,
+                    const int64_t *phwsdeg, const int64_t *gwsdeg, const double *Hwann, const double *Dphwann,
+                    const double *gwann, const double *basis, const double *masses, const double *born);
+
+/* Replaces `read_EPW_Wannier` (src/wannier.f90:289-428): reads epwdata.fmt,
+ * epmatwp1, rcells_{k,q,g}, wsdeg_{k,q,g} from `dir`.  The crystal data comes
+ * from the caller (`crys`, src/crystal.f90). */
+int elph_ctx_create_epw(elph_ctx **ctx, int device, const char *dir, int64_t natoms, const double *lattvecs,
+                        const double *basis, const double *masses, const int64_t *coarse_qmesh, int32_t polar,
+                        const double *epsilon, const double *born);
+
+int elph_ctx_destroy(elph_ctx *ctx);
+const char *elph_last_error(const elph_ctx *ctx);
+/* Sizes read back from a context (nb, nbr, nwsk, nwsq, nwsg, natoms). */
+int elph_ctx_dims(const elph_ctx *ctx, int64_t out[6]);
+
+/* Replaces `Wannier%el_wann` (src/wannier.f90:430-524).  kvecs (nk,3) crystal
+ * coords; energies (nk,nb) eV; velocities (nk,nb,3) or NULL; evecs (nk,nb,nb)
+ * c128 or NULL (evecs(ik,ib,:) = eigenvector ib); scissor (nb) or NULL.
+ * velocities without evecs is an error, as in the reference (454-455). */
+int elph_el_wann(elph_ctx *ctx, int64_t nk, const double *kvecs, double *energies, double *velocities,
+                 double *evecs, const double *scissor);
+
+/* Replaces `Wannier%ph_wann` (src/wannier.f90:585-782).  qvecs (nq,3);
+ * energies (nq,nbr) eV; evecs (nq,nbr,nbr) c128 or NULL; velocities (nq,nbr,3)
+ * or NULL.  Returns ELPH_ENEGPHONON when an energy below -0.005 eV occurs. */
+int elph_ph_wann(elph_ctx *ctx, int64_t nq, const double *qvecs, double *energies, double *evecs,
+                 double *velocities);
+
+/* Replaces `Wannier%gkRp` (src/wannier.f90:1073-1132) for nk wave vectors at
+ * once.  out: nk consecutive (nb,nb,nbr,nwsq) c128 blocks, the byte layout of
+ * the reference's gkRp.ik<ik> files (entries beyond nwsg are zero). */
+int elph_gkRp(elph_ctx *ctx, int64_t nk, const double *kvecs, double *out);
+
+/* Replaces `Wannier%gReq` (src/wannier.f90:1134-1188) for nq wave vectors.
+ * out: nq consecutive (nb,nb,nbr,nwsk) c128 blocks (gReq.iq<iq> layout). */
+int elph_gReq(elph_ctx *ctx, int64_t nq, const double *qvecs, double *out);
+
+/* Replaces `Wannier%g2` (src/wannier.f90:903-1011) for a list of n vertices.
+ * Vertex i uses kvecs/qvecs row i ((n,3)), el_evec_k/el_evec_kp (n,nb) c128,
+ * ph_evec_q (n,nbr) c128, ph_en (n) eV and the mixed-representation block
+ * gmixed[gidx[i]] of shape (nb,nb,nbr,nws) with nws = nwsg for wannspace 'p'
+ * ('ph') or nwsk for 'e' ('el').  out (n) eV^2. */
+int elph_g2(elph_ctx *ctx, int64_t n, const double *kvecs, const double *qvecs, const double *el_evec_k,
+            const double *el_evec_kp, const double *ph_evec_q, const double *ph_en, const double *gmixed,
+            int64_t ngmixed, const int64_t *gidx, char wannspace, double *out);
+
+/* Electron and phonon data consumed by the vertex drivers: the fields of
+ * `type electron` / `type phonon` that calculate_eph_interaction_ibzk/ibzq
+ * read (src/interactions.f90:2095-2400, 1673-1930).  All host pointers. */
+typedef struct {
+    int64_t wvmesh[3];
+    double enref, fsthick;
+    int64_t nwv_irred;              /* IBZ blocks */
+    const int64_t *indexlist_irred; /* (nwv_irred) muxed FBZ index, 1-based */
+    const double *ens_irred;        /* (nwv_irred,nb) eV */
+    const double *evecs_irred;      /* (nwv_irred,nb,nb) c128 */
+    int64_t nwv;                    /* FBZ blocks */
+    const int64_t *indexlist;       /* (nwv) sorted, 1-based */
+    const double *ens;              /* (nwv,nb) */
+    const double *evecs;            /* (nwv,nb,nb) c128 */
+    int64_t nstates_inwindow;
+} elph_electrons;
+
+typedef struct {
+    int64_t wvmesh[3];
+    int64_t nwv;                    /* full phonon mesh */
+    const double *ens;              /* (nwv,nbr) eV */
+    const double *evecs;            /* (nwv,nbr,nbr) c128 */
+    int64_t nwv_irred;
+    const int64_t *indexlist_irred; /* (nwv_irred) */
+} elph_phonons;
+
+/* calculate_eph_interaction_ibzk, key 'g' (src/interactions.f90:2095-2400),
+ * split into plan (upload + window/offset tables, HBM-resident) and run.
+ * run() evaluates IBZ states istate_first..istate_last (1-based, state =
+ * (ik-1)*nb + m as demux_state, src/misc.f90:977-990): counts[i] = count of
+ * state istate_first+i, or -1 when the state is outside the window (the
+ * reference writes no file then); g2 row i (length nprocs =
+ * nstates_inwindow*nbr) holds the vertices in the reference's order (k'
+ * ascending, in-window n ascending, s ascending).  g2 is device memory when
+ * g2_on_device != 0, host memory otherwise.  mesh_ref > 1 (fine phonons off
+ * the phonon mesh, 2241-2251) interpolates those phonons with ph_wann on the
+ * device. */
+int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int64_t mesh_ref,
+                          int32_t precision, elph_ibzk_plan **plan);
+int elph_ibzk_plan_run(elph_ibzk_plan *plan, int64_t istate_first, int64_t istate_last, int64_t *counts,
+                       double *g2, int32_t g2_on_device);
+int64_t elph_ibzk_plan_nprocs(const elph_ibzk_plan *plan);
+int elph_ibzk_plan_destroy(elph_ibzk_plan *plan);
+/* One-shot form (plan + run + destroy). */
+int elph_ibzk_g2(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int64_t mesh_ref,
+                 int32_t precision, int64_t istate_first, int64_t istate_last, int64_t *counts, double *g2);
+
+/* calculate_eph_interaction_ibzq, key 'g' (src/interactions.f90:1673-1930).
+ * Phonon state = (iq-1)*nbr + s over IBZ q.  g2 row stride =
+ * nstates_inwindow*nb (the reference sizes its buffer nstates_inwindow*nbr,
+ * 1724, which a window with more than nbr band pairs per k would overrun). */
+int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int32_t precision,
+                          elph_ibzq_plan **plan);
+int elph_ibzq_plan_run(elph_ibzq_plan *plan, int64_t istate_first, int64_t istate_last, int64_t *counts,
+                       double *g2, int32_t g2_on_device);
+int64_t elph_ibzq_plan_stride(const elph_ibzq_plan *plan);
+int elph_ibzq_plan_destroy(elph_ibzq_plan *plan);
+
+/* Writes gk2.istate<i> (kind 'k': int64 count + nprocs doubles, the whole
+ * buffer as src/interactions.f90:2337-2349 does) or gq2.istate<i> (kind 'q':
+ * int64 count + count doubles, 1877-1889) for states with counts >= 0. */
+int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, const int64_t *counts,
+                        const double *g2, int64_t row_stride);
+
+/* Setup the vertex path consumes (host + device): symmetry operations
+ * (src/symmetry.f90:66-212, found by a lattice/basis search in place of
+ * spglib), the irreducible wedge (303-451), calculate_electrons
+ * (src/electron.f90:321-595) and calculate_phonons (src/phonon.f90:159-310).
+ * Outputs are library-owned until elph_state_free. */
+int elph_symmetry(const double *lattvecs, int64_t natoms, const double *basis, const int64_t *atomtypes,
+                  double symprec, const int64_t *mesh, int64_t *nsymm, int64_t *nsymm_rot,
+                  double *qrotations /* (3,3,96) */);
+int elph_irred_wedge(const int64_t *mesh, int64_t nsymm_rot, const double *qrotations, int64_t nlist,
+                     const int64_t *indexlist /* NULL = full mesh */, int64_t *nirr,
+                     int64_t *indexlist_irred /* (nlist) */, int64_t *nequiv /* (nlist) */,
+                     int64_t *ibz2fbz /* (nsymm_rot,nlist,2) or NULL */);
+
+typedef struct elph_state elph_state;
+int elph_calculate_electrons(elph_ctx *ctx, int64_t nsymm_rot, const double *qrotations, const int64_t *wvmesh,
+                             double enref, double fsthick, elph_state **state, elph_electrons *view);
+int elph_calculate_phonons(elph_ctx *ctx, int64_t nsymm_rot, const double *qrotations, const int64_t *wvmesh,
+                           elph_state **state, elph_phonons *view);
+int elph_state_free(elph_state *state);
+
+/* Checks the fp64 MFMA operand layout the kernels assume (returns max abs
+ * error of a 16x16x4 product against the host; the device must be present). */
+int elph_selftest_mfma(int device, double *max_err);
+
+/* Device time of the last run()'s dominant kernel launches in ms, measured with
+ * HIP events on the stream the kernels ran on: [0] sum over launches, [1]
+ * number of launches, [2] sum of all kernels of run(). */
+int elph_last_timing(const elph_ctx *ctx, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,432 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
+the checker; the product never imports it.  Array conventions follow the
+reference (column-major, 1-based indices where the reference returns them).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+c_i64p = ct.POINTER(ct.c_int64)
+c_dblp = ct.POINTER(ct.c_double)
+c_i32p = ct.POINTER(ct.c_int32)
+
+
+class OrcWannier(ct.Structure):
+    _fields_ = [
+        ("nb", ct.c_int64), ("nbr", ct.c_int64), ("nwsk", ct.c_int64), ("nwsq", ct.c_int64),
+        ("nwsg", ct.c_int64), ("natoms", ct.c_int64), ("coarse_qmesh", ct.c_int64 * 3),
+        ("rcells_k", c_i64p), ("rcells_q", c_i64p), ("rcells_g", c_i64p),
+        ("elwsdeg", c_i64p), ("phwsdeg", c_i64p), ("gwsdeg", c_i64p),
+        ("Hwann", c_dblp), ("Dphwann", c_dblp), ("gwann", c_dblp),
+        ("lattvecs", ct.c_double * 9), ("basis", c_dblp), ("masses", c_dblp),
+        ("polar", ct.c_int32), ("epsilon", ct.c_double * 9), ("born", c_dblp),
+    ]
+
+
+class OrcVec(ct.Structure):
+    _fields_ = [("frac", ct.c_double * 3), ("cart", ct.c_double * 3),
+                ("ints", ct.c_int64 * 3), ("muxed", ct.c_int64)]
+
+
+class OrcElectrons(ct.Structure):
+    _fields_ = [
+        ("wvmesh", ct.c_int64 * 3), ("enref", ct.c_double), ("fsthick", ct.c_double),
+        ("nwv_irred", ct.c_int64), ("nwv", ct.c_int64), ("nstates_inwindow", ct.c_int64),
+        ("indexlist_irred", c_i64p), ("ens_irred", c_dblp), ("evecs_irred", c_dblp),
+        ("indexlist", c_i64p), ("ens", c_dblp), ("evecs", c_dblp),
+    ]
+
+
+class OrcPhonons(ct.Structure):
+    _fields_ = [("wvmesh", ct.c_int64 * 3), ("nwv", ct.c_int64), ("ens", c_dblp), ("evecs", c_dblp),
+                ("nwv_irred", ct.c_int64), ("indexlist_irred", c_i64p)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ct.CDLL(_LIB_PATH)
+        L.orc_mux_vector.restype = ct.c_int64
+        L.orc_mux_state.restype = ct.c_int64
+        L.orc_binsearch.restype = ct.c_int64
+        L.orc_g2.restype = ct.c_double
+        L.orc_qrotations.restype = ct.c_int64
+        L.orc_find_irred_wedge.restype = ct.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a, t=c_dblp):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _i64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int64))
+
+
+def _c128(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+class Wannier:
+    """Holds the oracle's view of a synth.Material (keeps arrays alive)."""
+
+    def __init__(self, mat):
+        self.mat = mat
+        self._keep = []
+        s = OrcWannier()
+        s.nb, s.nbr, s.nwsk, s.nwsq, s.nwsg, s.natoms = (mat.nb, mat.nbr, mat.nwsk, mat.nwsq,
+                                                         mat.nwsg, mat.natoms)
+        for i in range(3):
+            s.coarse_qmesh[i] = int(mat.coarse_qmesh[i])
+        for name in ("rcells_k", "rcells_q", "rcells_g"):
+            a = _i64(getattr(mat, name))
+            self._keep.append(a)
+            setattr(s, name, _p(a, c_i64p))
+        for name in ("elwsdeg", "phwsdeg", "gwsdeg"):
+            a = _i64(getattr(mat, name))
+            self._keep.append(a)
+            setattr(s, name, _p(a, c_i64p))
+        for name in ("Hwann", "Dphwann", "gwann"):
+            a = _c128(getattr(mat, name))
+            self._keep.append(a)
+            setattr(s, name, _p(a))
+        L = _f64(mat.lattvecs)
+        for i in range(9):
+            s.lattvecs[i] = L.ravel(order="F")[i]
+        b = _f64(mat.basis)
+        m = _f64(mat.masses)
+        self._keep += [b, m]
+        s.basis, s.masses = _p(b), _p(m)
+        s.polar = 1 if mat.polar else 0
+        if mat.polar:
+            e = _f64(mat.epsilon)
+            for i in range(9):
+                s.epsilon[i] = e.ravel(order="F")[i]
+            bo = _f64(mat.born)
+            self._keep.append(bo)
+            s.born = _p(bo)
+        self.s = s
+
+    # ---- wannier.f90 ----
+    def el_wann(self, kvecs, velocities=True, scissor=None):
+        k = _f64(np.atleast_2d(kvecs))
+        nk, nb = k.shape[0], self.mat.nb
+        en = np.zeros((nk, nb), order="F")
+        ev = np.zeros((nk, nb, nb), np.complex128, order="F")
+        vel = np.zeros((nk, nb, 3), order="F") if velocities else None
+        sc = _f64(scissor) if scissor is not None else None
+        st = lib().orc_el_wann(ct.byref(self.s), ct.c_int64(nk), _p(k), _p(en),
+                               _p(vel) if vel is not None else None, _p(ev),
+                               _p(sc) if sc is not None else None)
+        assert st == 0
+        return en, ev, vel
+
+    def ph_wann(self, qvecs, velocities=False):
+        q = _f64(np.atleast_2d(qvecs))
+        nq, nbr = q.shape[0], self.mat.nbr
+        en = np.zeros((nq, nbr), order="F")
+        ev = np.zeros((nq, nbr, nbr), np.complex128, order="F")
+        vel = np.zeros((nq, nbr, 3), order="F") if velocities else None
+        st = lib().orc_ph_wann(ct.byref(self.s), ct.c_int64(nq), _p(q), _p(en), _p(ev),
+                               _p(vel) if vel is not None else None)
+        if st == 2:
+            raise RuntimeError("Large negative phonon energy found! Stopping!")
+        assert st == 0
+        return en, ev, vel
+
+    def gkRp(self, kvec):
+        m = self.mat
+        out = np.zeros((m.nb, m.nb, m.nbr, m.nwsq), np.complex128, order="F")
+        k = _f64(kvec)
+        lib().orc_gkRp(ct.byref(self.s), _p(k), _p(out))
+        return out
+
+    def gReq(self, qvec):
+        m = self.mat
+        out = np.zeros((m.nb, m.nb, m.nbr, m.nwsk), np.complex128, order="F")
+        q = _f64(qvec)
+        lib().orc_gReq(ct.byref(self.s), _p(q), _p(out))
+        return out
+
+    def g2(self, kvec, qvec, el_evec_k, el_evec_kp, ph_evec_q, ph_en, gmixed, wannspace):
+        if wannspace not in ("el", "ph"):
+            raise ValueError("Invalid value of wannspace in call to g2_epw. Exiting.")
+        k, q = _f64(kvec), _f64(qvec)
+        a, b, c = _c128(el_evec_k), _c128(el_evec_kp), _c128(ph_evec_q)
+        g = _c128(gmixed)
+        return lib().orc_g2(ct.byref(self.s), _p(k), _p(q), _p(a), ct.c_int64(1), _p(b), ct.c_int64(1),
+                            _p(c), ct.c_int64(1), ct.c_double(ph_en), _p(g),
+                            ct.c_int(1 if wannspace == "ph" else 0))
+
+
+# ---- symmetry ----
+def find_rotations(mat, symprec=1e-5):
+    types = np.array(atom_types(mat.masses), np.int64)
+    rot = np.zeros(9 * 48, np.int32)
+    b = _f64(mat.basis)
+    L = _f64(mat.lattvecs)
+    n = lib().orc_find_rotations(_p(L), _p(b), _p(types, c_i64p), ct.c_int64(mat.natoms),
+                                 ct.c_double(symprec), _p(rot, c_i32p))
+    return rot[:9 * n].reshape(n, 9)
+
+
+def atom_types(masses):
+    types, seen = [], {}
+    for m in masses:
+        key = round(float(m), 6)
+        if key not in seen:
+            seen[key] = len(seen) + 1
+        types.append(seen[key])
+    return types
+
+
+def qrotations(mat, mesh):
+    rot = find_rotations(mat)
+    L = _f64(mat.lattvecs)
+    q = np.zeros(9 * 2 * rot.shape[0])
+    r = np.ascontiguousarray(rot.ravel(), np.int32)
+    mesh = _i64(mesh)
+    n = lib().orc_qrotations(_p(L), _p(r, c_i32p), ct.c_int64(rot.shape[0]), _p(mesh, c_i64p), _p(q))
+    return q[:9 * n].copy(), int(n), int(rot.shape[0])
+
+
+def irred_wedge(mesh, qrot, nsr, indexlist=None):
+    mesh = _i64(mesh)
+    nlist = int(np.prod(mesh)) if indexlist is None else len(indexlist)
+    irr = np.zeros(nlist, np.int64)
+    neq = np.zeros(nlist, np.int64)
+    il = _i64(indexlist) if indexlist is not None else None
+    n = lib().orc_find_irred_wedge(_p(mesh, c_i64p), ct.c_int64(nsr), _p(qrot),
+                                   _p(il, c_i64p) if il is not None else None, ct.c_int64(nlist),
+                                   _p(irr, c_i64p), _p(neq, c_i64p), None)
+    return irr[:n].copy(), neq[:n].copy()
+
+
+class Electrons:
+    def __init__(self, w: Wannier, qrot, nsr, wvmesh, enref, fsthick):
+        self.s = OrcElectrons()
+        mesh = _i64(wvmesh)
+        st = lib().orc_calculate_electrons(ct.byref(w.s), _p(qrot), ct.c_int64(nsr), _p(mesh, c_i64p),
+                                           ct.c_double(enref), ct.c_double(fsthick), ct.byref(self.s))
+        if st != 0:
+            raise RuntimeError(f"orc_calculate_electrons failed ({st})")
+        s, nb = self.s, w.mat.nb
+        self.nb = nb
+        self.wvmesh = np.array(wvmesh, np.int64)
+        self.enref, self.fsthick = enref, fsthick
+        self.nwv, self.nwv_irred, self.nstates_inwindow = s.nwv, s.nwv_irred, s.nstates_inwindow
+
+        def arr(ptr, shape, dt):
+            n = int(np.prod(shape))
+            buf = np.ctypeslib.as_array(ptr, shape=(n * (2 if dt == np.complex128 else 1),))
+            if dt == np.complex128:
+                buf = buf.view(np.complex128)
+            return np.array(buf.reshape(shape, order="F"), order="F")
+        self.indexlist = arr(s.indexlist, (s.nwv,), np.int64)
+        self.indexlist_irred = arr(s.indexlist_irred, (s.nwv_irred,), np.int64)
+        self.ens = arr(s.ens, (s.nwv, nb), np.float64)
+        self.ens_irred = arr(s.ens_irred, (s.nwv_irred, nb), np.float64)
+        self.evecs = arr(s.evecs, (s.nwv, nb, nb), np.complex128)
+        self.evecs_irred = arr(s.evecs_irred, (s.nwv_irred, nb, nb), np.complex128)
+
+    def __del__(self):
+        try:
+            lib().orc_free_electrons(ct.byref(self.s))
+        except Exception:
+            pass
+
+
+class Phonons:
+    def __init__(self, w: Wannier, qrot, nsr, wvmesh):
+        self.s = OrcPhonons()
+        mesh = _i64(wvmesh)
+        st = lib().orc_calculate_phonons(ct.byref(w.s), _p(qrot), ct.c_int64(nsr), _p(mesh, c_i64p),
+                                         ct.byref(self.s))
+        if st != 0:
+            raise RuntimeError(f"orc_calculate_phonons failed ({st})")
+        s, nbr = self.s, w.mat.nbr
+        self.wvmesh = np.array(wvmesh, np.int64)
+        self.nwv = s.nwv
+        self.nwv_irred = s.nwv_irred
+        self.ens = np.array(np.ctypeslib.as_array(s.ens, shape=(s.nwv * nbr,)).reshape((s.nwv, nbr), order="F"),
+                            order="F")
+        ev = np.ctypeslib.as_array(s.evecs, shape=(2 * s.nwv * nbr * nbr,)).view(np.complex128)
+        self.evecs = np.array(ev.reshape((s.nwv, nbr, nbr), order="F"), order="F")
+        self.indexlist_irred = np.array(np.ctypeslib.as_array(s.indexlist_irred, shape=(s.nwv_irred,)))
+
+    def __del__(self):
+        try:
+            lib().orc_free_phonons(ct.byref(self.s))
+        except Exception:
+            pass
+
+
+def _electrons_struct(w, el):
+    """Build an OrcElectrons pointing at numpy arrays of an electron-state object
+    (oracle-made or product-made), so both sides can be driven by the same data."""
+    s = OrcElectrons()
+    keep = []
+    for i in range(3):
+        s.wvmesh[i] = int(el.wvmesh[i])
+    s.enref, s.fsthick = el.enref, el.fsthick
+    s.nwv, s.nwv_irred, s.nstates_inwindow = el.nwv, el.nwv_irred, el.nstates_inwindow
+    for name, conv, t in (("indexlist", _i64, c_i64p), ("indexlist_irred", _i64, c_i64p),
+                          ("ens", _f64, c_dblp), ("ens_irred", _f64, c_dblp),
+                          ("evecs", _c128, c_dblp), ("evecs_irred", _c128, c_dblp)):
+        a = conv(getattr(el, name))
+        keep.append(a)
+        setattr(s, name, _p(a, t))
+    return s, keep
+
+
+def _phonons_struct(ph):
+    s = OrcPhonons()
+    for i in range(3):
+        s.wvmesh[i] = int(ph.wvmesh[i])
+    s.nwv = ph.nwv
+    e, v, irr = _f64(ph.ens), _c128(ph.evecs), _i64(ph.indexlist_irred)
+    s.ens, s.evecs = _p(e), _p(v)
+    s.nwv_irred = len(irr)
+    s.indexlist_irred = _p(irr, c_i64p)
+    return s, [e, v, irr]
+
+
+def ibzk_g2(w: Wannier, el, ph, mesh_ref=1, istate_first=1, istate_last=None, nthreads=0):
+    """calculate_eph_interaction_ibzk('g') for IBZ states istate_first..istate_last.
+    Returns (counts (-1 = skipped), g2 rows of length nstates_inwindow*nbr)."""
+    nb, nbr = w.mat.nb, w.mat.nbr
+    if istate_last is None:
+        istate_last = el.nwv_irred * nb
+    nst = istate_last - istate_first + 1
+    nprocs = el.nstates_inwindow * nbr
+    counts = np.zeros(nst, np.int64)
+    out = np.zeros((nst, nprocs))
+    es, k1 = _electrons_struct(w, el)
+    ps, k2 = _phonons_struct(ph)
+    st = lib().orc_ibzk_g2(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref),
+                           ct.c_int64(istate_first), ct.c_int64(istate_last), _p(counts, c_i64p), _p(out),
+                           ct.c_int(nthreads))
+    if st == 2:
+        raise RuntimeError("Large negative phonon energy found! Stopping!")
+    assert st == 0, st
+    return counts, out
+
+
+def ibzq_g2(w: Wannier, el, ph, istate_first=1, istate_last=None, nthreads=0):
+    nb, nbr = w.mat.nb, w.mat.nbr
+    if istate_last is None:
+        istate_last = ph.nwv_irred * nbr
+    nst = istate_last - istate_first + 1
+    stride = el.nstates_inwindow * nb
+    counts = np.zeros(nst, np.int64)
+    out = np.zeros((nst, stride))
+    es, k1 = _electrons_struct(w, el)
+    ps, k2 = _phonons_struct(ph)
+    st = lib().orc_ibzq_g2(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(istate_first),
+                           ct.c_int64(istate_last), _p(counts, c_i64p), _p(out), ct.c_int(nthreads))
+    assert st == 0, st
+    return counts, out
+
+
+# ---- index algebra (misc.f90, vector.f90) ----
+def mux_vector(v, mesh, base):
+    v, mesh = _i64(v), _i64(mesh)
+    return int(lib().orc_mux_vector(_p(v, c_i64p), _p(mesh, c_i64p), ct.c_int64(base)))
+
+
+def demux_vector(i, mesh, base):
+    v = np.zeros(3, np.int64)
+    mesh = _i64(mesh)
+    lib().orc_demux_vector(ct.c_int64(i), _p(v, c_i64p), _p(mesh, c_i64p), ct.c_int64(base))
+    return v
+
+
+def mux_state(nbands, iband, ik):
+    return int(lib().orc_mux_state(ct.c_int64(nbands), ct.c_int64(iband), ct.c_int64(ik)))
+
+
+def demux_state(m, nbands):
+    a, b = ct.c_int64(), ct.c_int64()
+    lib().orc_demux_state(ct.c_int64(m), ct.c_int64(nbands), ct.byref(a), ct.byref(b))
+    return a.value, b.value
+
+
+def binsearch(array, e):
+    a = _i64(array)
+    return int(lib().orc_binsearch(_p(a, c_i64p), ct.c_int64(len(a)), ct.c_int64(e)))
+
+
+def add_and_fold(q1, q2):
+    a, b, c = _f64(q1), _f64(q2), np.zeros(3)
+    lib().orc_add_and_fold(_p(a), _p(b), _p(c))
+    return c
+
+
+def expi(x):
+    out = np.zeros(2)
+    lib().orc_expi(ct.c_double(x), _p(out))
+    return complex(out[0], out[1])
+
+
+def distribute_points(npts, num_images, image):
+    c, s, e, na = ct.c_int64(), ct.c_int64(), ct.c_int64(), ct.c_int64()
+    lib().orc_distribute_points(ct.c_int64(npts), ct.c_int64(num_images), ct.c_int64(image),
+                                ct.byref(c), ct.byref(s), ct.byref(e), ct.byref(na))
+    return c.value, s.value, e.value, na.value
+
+
+def _vec(v: OrcVec):
+    return dict(frac=np.array(v.frac[:]), cart=np.array(v.cart[:]), int=np.array(v.ints[:]), muxed=v.muxed)
+
+
+def vec_create(ind, grid, prim):
+    v = OrcVec()
+    g, p = _i64(grid), _f64(prim)
+    lib().orc_vec_create(ct.c_int64(ind), _p(g, c_i64p), _p(p), ct.byref(v))
+    return v
+
+
+def vec_add(v1, v2, grid, prim):
+    v = OrcVec()
+    g, p = _i64(grid), _f64(prim)
+    lib().orc_vec_add(ct.byref(v1), ct.byref(v2), _p(g, c_i64p), _p(p), ct.byref(v))
+    return v
+
+
+def vec_sub(v1, v2, grid, prim):
+    v = OrcVec()
+    g, p = _i64(grid), _f64(prim)
+    lib().orc_vec_sub(ct.byref(v1), ct.byref(v2), _p(g, c_i64p), _p(p), ct.byref(v))
+    return v
+
+
+def vec_change_grid(v1, grid):
+    v = OrcVec()
+    g = _i64(grid)
+    lib().orc_vec_change_grid(ct.byref(v1), _p(g, c_i64p), ct.byref(v))
+    return v
+
+
+vec_dict = _vec
