@@ -1,0 +1,179 @@
+"""Benchmark of the electron-phonon vertex path on one MI355X.
+
+Workload: the 3C-SiC-like synthetic material (fcc a = 4.36 A, 2 atoms, 8
+Wannier bands, 6 branches, Wigner-Seitz sets of an 8^3 coarse grid, 617 cells)
+on a 24^3 k = q mesh with a window keeping ~4 of 8 bands, driven through
+calculate_eph_interaction_ibzk('g').  One step = all in-window IBZ states of a
+block of IBZ k points (every k' of the window, every branch), inputs resident
+in HBM, |g|^2 written to HBM.  Reported: |g|^2 vertices/s, the roofline of the
+dominant kernel (fused phase-contraction + rotation + |g|^2 kernel) from HIP
+events on its stream, and the CPU reference method (the oracle) timed on a
+bounded sample on this host.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64|mixed]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "|g|² vertices/sec, 1 MI355X, mixed vs FP64 at ≤1e-5 rel err; p50/p99 latency"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops(nst, nwv, nstates_inwindow, nb, nbr, nwsg):
+    """Flops of one pair-kernel launch (nst IBZ states x nwv k' rows): the Rp
+    phase contraction of the band-rotated g(k,Rp) (8 nwsg nb nbr per state and
+    row), the final-band and mode rotations (8 nb nbr + 8 nbr^2 per vertex
+    column block) and |g|^2/omega (4 per vertex)."""
+    gemm = 8.0 * nwsg * nb * nbr * nwv * nst
+    epi = nst * nstates_inwindow * (8.0 * nbr * nb + 8.0 * nbr * nbr + 4.0 * nbr)
+    return gemm + epi
+
+
+def survey_flops(nst, nwv, nstates_inwindow, nb, nbr, nwsg, nm):
+    """The same launch counted with the unrotated per-pair form (Rp contraction
+    over all nb^2 nbr columns of g(k,Rp), then rotations), shared over the nm
+    in-window initial bands of the IBZ point."""
+    pairs = nst * nwv / max(nm, 1)
+    rot = 8.0 * (nb * nb * nbr * nm) + 8.0 * nb * nm * nbr * (nstates_inwindow / nwv) \
+        + 8.0 * nm * (nstates_inwindow / nwv) * nbr * nbr
+    return pairs * (8.0 * nwsg * nb * nb * nbr + rot) + 4.0 * nst * nstates_inwindow * nbr
+
+
+def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
+    """The oracle (literal C restatement of the reference's per-vertex g2 in the
+    ibzk loop order, OpenMP over states) on a bounded sample: the given IBZ
+    states, each over the first `ikp_limit` final k' of the window."""
+    from oracle import oracle as O
+    o = O.Wannier(mat)
+    first, last = states
+    pilot = 4
+    t0 = time.perf_counter()
+    c, _ = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=pilot)
+    t_pilot = time.perf_counter() - t0
+    nact = int((c >= 0).sum())
+    per_row = max(t_pilot / pilot, 1e-6)
+    ikp_limit = int(max(pilot, min(el.nwv, target_s / per_row)))
+    t0 = time.perf_counter()
+    c, g = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=ikp_limit)
+    dt = time.perf_counter() - t0
+    nvert = int(c[c >= 0].sum())
+    return {"value": nvert / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
+            "sample": f"ibzk states {first}..{last} ({nact} in window), first {ikp_limit} of {el.nwv} k' each, "
+                      f"{nvert} vertices in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--kblock", type=int, default=8, help="IBZ k points per step")
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not run)"}))
+        sys.exit(2)
+
+    import torch
+    from elphbolt_amd import bz, interactions, synth
+    from elphbolt_amd.wannier import Wannier
+
+    t0 = time.perf_counter()
+    mat = synth.config(args.config)
+    log(f"[bench] material {mat.name}: nb={mat.nb} nbr={mat.nbr} nws={mat.nwsk} mesh={list(mat.wvmesh)} "
+        f"enref={mat.enref:.3f} fsthick={mat.fsthick:.3f} ({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    w = Wannier(mat, device=0)
+    sym = bz.Symmetry(mat.lattvecs, mat.basis, mat.masses, mat.wvmesh)
+    el = bz.Electrons(w, sym, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = bz.Phonons(w, sym, mat.phmesh)
+    log(f"[bench] setup: nsymm={sym.nsymm} nwv={el.nwv} nwv_irred={el.nwv_irred} "
+        f"nstates_inwindow={el.nstates_inwindow} ({time.perf_counter() - t0:.1f} s)")
+    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=args.precision)
+    nb, nbr = mat.nb, mat.nbr
+    kb = min(args.kblock, el.nwv_irred)
+    nblocks = max(1, el.nwv_irred // kb)
+    out = torch.empty((kb * nb, plan.nprocs), dtype=torch.float64, device=dev)
+
+    def step(i):
+        b = i % nblocks
+        first, last = b * kb * nb + 1, (b + 1) * kb * nb
+        counts, _ = plan.run(first, last, out=out)
+        return counts
+
+    for i in range(args.warmup):
+        step(i)
+        log(f"[bench] warmup step {i} done")
+    torch.cuda.synchronize()
+    vertices = 0
+    dom_ms, dom_n, flops_launch, survey_launch = 0.0, 0, 0.0, 0.0
+    nm_avg = float((np.abs(el.ens_irred - el.enref) <= el.fsthick).sum()) / max(1, el.nwv_irred)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        counts = step(args.warmup + i)
+        nact = int((counts >= 0).sum())
+        vertices += int(counts[counts >= 0].sum())
+        tm = plan.timing()
+        dom_ms += tm["dominant_ms"]
+        dom_n += tm["launches"]
+        flops_launch += algorithmic_flops(nact, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg)
+        survey_launch += survey_flops(nact, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg, nm_avg)
+        log(f"[bench] step {i}: {nact} states, {tm['dominant_ms']:.1f} ms in {tm['launches']} pair launches, "
+            f"{tm['all_ms']:.1f} ms device total")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    value = vertices / elapsed
+    avg_launch_s = dom_ms / max(dom_n, 1) * 1e-3
+    achieved = (flops_launch / max(dom_n, 1)) / avg_launch_s / 1e12
+    achieved_survey = (survey_launch / max(dom_n, 1)) / avg_launch_s / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "k_pair_fp64<3>", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
+                "flops_per_launch": flops_launch / max(dom_n, 1),
+                "achieved_unrotated_count": round(achieved_survey, 3)}
+    res = {"metric": METRIC, "value": value, "unit": "vertices/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
+           "data": "synthetic",
+           "config": {"workload": f"{mat.name}: calculate_eph_interaction_ibzk('g'), {kb} IBZ k per step",
+                      "precision": args.precision, "nb": nb, "nbr": nbr, "nws": mat.nwsg,
+                      "mesh": [int(x) for x in mat.wvmesh], "nwv_window": el.nwv,
+                      "nwv_irred": el.nwv_irred, "nstates_inwindow": el.nstates_inwindow,
+                      "vertices_timed": vertices},
+           "roofline": roofline}
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) ...")
+        first_active = None
+        for s in range(1, el.nwv_irred * nb + 1):
+            ik, m = (s - 1) // nb, (s - 1) % nb
+            if abs(el.ens_irred[ik, m] - el.enref) <= el.fsthick:
+                first_active = s
+                break
+        res["cpu_baseline"] = cpu_baseline(mat, el, ph, (first_active, min(first_active + 15, el.nwv_irred * nb)),
+                                           target_s=args.cpu_seconds, threads=16)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -230,16 +230,26 @@ def make_vertex(cells_k, cells_g, lattvecs, nb, nbr, seed, chunk_rp: int = 64) -
     amp = 0.1 / C.RYD2EV
     # C-order array with reversed axes == Fortran order of (nb,nb,nwsk,nbr,nwsg)
     out = np.empty((nwsg, nbr, nwsk, nb, nb), np.complex128)
-    for r0 in range(0, nwsg, chunk_rp):
-        r1 = min(nwsg, r0 + chunk_rp)
-        rng = np.random.default_rng([seed, _TID_G, r0])
-        th = rng.random((r1 - r0, nbr, nwsk, nb, nb))
+    shape = (nbr, nwsk, nb, nb)
+    th = np.empty(shape)
+    tmp = np.empty(shape)
+    env = (amp * ek)[None, :, None, None] * np.ones(shape)
+    for rp in range(nwsg):
+        # one stream per Rp cell (chunk_rp cells share a key, as a block)
+        r0 = (rp // chunk_rp) * chunk_rp
+        if rp == r0:
+            rng = np.random.default_rng([seed, _TID_G, r0])
+        rng.random(out=th)
         th *= 2.0 * np.pi
-        blk = np.cos(th) + 1j * np.sin(th)
-        del th
-        blk *= (amp * eg[r0:r1])[:, None, None, None, None]
-        blk *= ek[None, None, :, None, None]
-        out[r0:r1] = blk
+        o = out[rp]
+        np.cos(th, out=tmp)
+        np.multiply(tmp, env, out=tmp)
+        tmp *= eg[rp]
+        o.real = tmp
+        np.sin(th, out=tmp)
+        np.multiply(tmp, env, out=tmp)
+        tmp *= eg[rp]
+        o.imag = tmp
     return out.T  # shape (nb, nb, nwsk, nbr, nwsg), F-contiguous
 
 

@@ -1020,7 +1020,16 @@ void orc_free_phonons(orc_phonons *ph) {
  * re-reads it as gkRp.ik<ik>, 1120-1131 / 2173-2182). */
 int orc_ibzk_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                 int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads) {
-    const int64_t nb = w->nb, nbr = w->nbr, nwv = el->nwv, nirr = el->nwv_irred;
+    return orc_ibzk_g2_sample(w, el, ph, mesh_ref, istate_first, istate_last, out_count, out_g2, nthreads, -1);
+}
+
+/* Same driver restricted to the first ikp_limit final wave vectors of each
+ * state when ikp_limit >= 0 (a bounded sample for CPU timing). */
+int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+                       int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
+                       int64_t ikp_limit) {
+    const int64_t nb = w->nb, nbr = w->nbr, nirr = el->nwv_irred;
+    const int64_t nwv = (ikp_limit >= 0 && ikp_limit < el->nwv) ? ikp_limit : el->nwv;
     const int64_t nprocs = el->nstates_inwindow * nbr;
     const int64_t gsz = nb * nb * nbr * w->nwsq;
     double rec[9], vol;

@@ -119,6 +119,9 @@ void orc_free_phonons(orc_phonons *ph);
 int orc_ibzk_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                 int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2,
                 int nthreads);
+int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+                       int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
+                       int64_t ikp_limit);
 /* calculate_eph_interaction_ibzq, key 'g'; out row stride nprocs = nstates_inwindow*nb. */
 int orc_ibzq_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph,
                 int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2,

@@ -313,7 +313,7 @@ def _phonons_struct(ph):
     return s, [e, v, irr]
 
 
-def ibzk_g2(w: Wannier, el, ph, mesh_ref=1, istate_first=1, istate_last=None, nthreads=0):
+def ibzk_g2(w: Wannier, el, ph, mesh_ref=1, istate_first=1, istate_last=None, nthreads=0, ikp_limit=-1):
     """calculate_eph_interaction_ibzk('g') for IBZ states istate_first..istate_last.
     Returns (counts (-1 = skipped), g2 rows of length nstates_inwindow*nbr)."""
     nb, nbr = w.mat.nb, w.mat.nbr
@@ -325,9 +325,9 @@ def ibzk_g2(w: Wannier, el, ph, mesh_ref=1, istate_first=1, istate_last=None, nt
     out = np.zeros((nst, nprocs))
     es, k1 = _electrons_struct(w, el)
     ps, k2 = _phonons_struct(ph)
-    st = lib().orc_ibzk_g2(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref),
-                           ct.c_int64(istate_first), ct.c_int64(istate_last), _p(counts, c_i64p), _p(out),
-                           ct.c_int(nthreads))
+    st = lib().orc_ibzk_g2_sample(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref),
+                                  ct.c_int64(istate_first), ct.c_int64(istate_last), _p(counts, c_i64p), _p(out),
+                                  ct.c_int(nthreads), ct.c_int64(ikp_limit))
     if st == 2:
         raise RuntimeError("Large negative phonon energy found! Stopping!")
     assert st == 0, st

@@ -1,0 +1,68 @@
+"""The C-ABI library loads (no GPU needed) and exports every entry point that
+include/elph_vertex.h declares; the ctypes prototypes cover the same set."""
+import ctypes as ct
+import os
+import re
+
+import pytest
+
+from elphbolt_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "elph_vertex.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(elph_[A-Za-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_prototypes_match_header():
+    assert sorted(_lib.PROTOTYPES) == _declared()
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.load()
+
+
+def test_write_and_read_g2_files(tmp_path):
+    import numpy as np
+
+    from elphbolt_amd import interactions
+    counts = np.array([3, -1, 2], np.int64)
+    g2 = np.arange(12, dtype=np.float64).reshape(3, 4)
+    interactions.write_g2_files(str(tmp_path), "k", 5, counts, g2)
+    c, v = interactions.read_g2_file(str(tmp_path / "gk2.istate5"))
+    assert c == 3 and np.array_equal(v, g2[0])        # 'k': count + whole nprocs buffer
+    assert not (tmp_path / "gk2.istate6").exists()     # out-of-window state: no file
+    interactions.write_g2_files(str(tmp_path), "q", 1, counts, g2)
+    c, v = interactions.read_g2_file(str(tmp_path / "gq2.istate3"))
+    assert c == 2 and np.array_equal(v, g2[2, :2])    # 'q': count + count values
+
+
+def test_symmetry_host_path():
+    """elph_symmetry / elph_irred_wedge are host code: same counts as the
+    reference's published values."""
+    import numpy as np
+
+    from elphbolt_amd import bz, synth
+    L = synth.fcc_lattice(5.43)
+    basis = np.array([[0, 0.25], [0, 0.25], [0, 0.25]], float)
+    s = bz.Symmetry(L, basis, [28.0, 28.0], [24, 24, 24])
+    assert s.nsymm == 48
+    irr, neq = bz.irred_wedge([24, 24, 24], s)
+    assert len(irr) == 413 and neq.sum() == 24 ** 3
+    s2 = bz.Symmetry(L, basis, [28.0, 12.0], [6, 6, 6])
+    assert s2.nsymm == 24
+    assert len(bz.irred_wedge([6, 6, 6], s2)[0]) == 16
