@@ -3,9 +3,9 @@
 Workload: the 3C-SiC-like synthetic material (fcc a = 4.36 A, 2 atoms, 8
 Wannier bands, 6 branches, Wigner-Seitz sets of an 8^3 coarse grid, 617 cells)
 on a 24^3 k = q mesh with a window keeping ~4 of 8 bands, driven through
-calculate_eph_interaction_ibzk('g').  One step = all in-window IBZ states of a
-block of IBZ k points (every k' of the window, every branch), inputs resident
-in HBM, |g|^2 written to HBM.  Reported: |g|^2 vertices/s, the roofline of the
+calculate_eph_interaction_ibzk('g').  One step = the whole job: every in-window
+IBZ state (all 413 IBZ k) against every k' of the window and every branch,
+inputs resident in HBM, |g|^2 written to HBM.  Reported: |g|^2 vertices/s, the roofline of the
 dominant kernel (fused phase-contraction + rotation + |g|^2 kernel) from HIP
 events on its stream, and the CPU reference method (the oracle) timed on a
 bounded sample on this host.
@@ -80,10 +80,10 @@ def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--kblock", type=int, default=8, help="IBZ k points per step")
+    ap.add_argument("--kblock", type=int, default=0, help="IBZ k points per step (0 = all: the whole job)")
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -111,7 +111,7 @@ def main():
         f"nstates_inwindow={el.nstates_inwindow} ({time.perf_counter() - t0:.1f} s)")
     plan = interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=args.precision)
     nb, nbr = mat.nb, mat.nbr
-    kb = min(args.kblock, el.nwv_irred)
+    kb = el.nwv_irred if args.kblock <= 0 else min(args.kblock, el.nwv_irred)
     nblocks = max(1, el.nwv_irred // kb)
     out = torch.empty((kb * nb, plan.nprocs), dtype=torch.float64, device=dev)
 

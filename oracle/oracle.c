@@ -1029,7 +1029,8 @@ int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_
                        int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
                        int64_t ikp_limit) {
     const int64_t nb = w->nb, nbr = w->nbr, nirr = el->nwv_irred;
-    const int64_t nwv = (ikp_limit >= 0 && ikp_limit < el->nwv) ? ikp_limit : el->nwv;
+    const int64_t nwv = el->nwv;
+    const int64_t nkp = (ikp_limit >= 0 && ikp_limit < el->nwv) ? ikp_limit : el->nwv;
     const int64_t nprocs = el->nstates_inwindow * nbr;
     const int64_t gsz = nb * nb * nbr * w->nwsq;
     double rec[9], vol;
@@ -1057,7 +1058,7 @@ int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_
             if (cur_ik != ik) { orc_gkRp(w, kv.frac, gk); cur_ik = ik; }
             int64_t count = 0;
             double *row = &out_g2[t * nprocs];
-            for (int64_t ikp = 1; ikp <= nwv; ++ikp) {
+            for (int64_t ikp = 1; ikp <= nkp; ++ikp) {
                 orc_vec_create(el->indexlist[ikp - 1], el->wvmesh, rec, &kpv);
                 orc_vec_sub(&kpv, &kv, el->wvmesh, rec, &qv);
                 int need_fine = 0;

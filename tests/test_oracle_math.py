@@ -267,3 +267,20 @@ def test_ibzk_driver_counts_and_order():
                                 ph.evecs[iq, s], ph.ens[iq, s], gk, "ph"))
     assert len(row) == nprocs
     assert np.allclose(g2[t], row, rtol=1e-13, atol=0)
+
+
+def test_oracle_sample_is_prefix():
+    """The bounded CPU-timing sample (first ikp_limit k' per state) reproduces
+    the leading entries of the full rows exactly."""
+    mat = synth.tiny(nb=4, natoms=2, ws=3, wv=6, seed=12)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    c0, g0 = O.ibzk_g2(o, el, ph, 1, 1, 8)
+    for lim in (3, 40):
+        c, g = O.ibzk_g2(o, el, ph, 1, 1, 8, ikp_limit=lim)
+        assert np.array_equal(c < 0, c0 < 0)
+        for t in np.where(c >= 0)[0]:
+            assert 0 < c[t] < c0[t]
+            assert np.array_equal(g[t, :c[t]], g0[t, :c[t]])
