@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "|g|² vertices/sec, 1 MI355X, mixed vs FP64 at ≤1e-5 rel err; p50/p99 latency"
 FP64_PEAK_TFLOPS = 78.6
+INT8_PEAK_TOPS = 5000.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -61,13 +62,15 @@ def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
     from oracle import oracle as O
     o = O.Wannier(mat)
     first, last = states
-    pilot = 4
-    t0 = time.perf_counter()
-    c, _ = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=pilot)
-    t_pilot = time.perf_counter() - t0
+    # two pilot sizes: the difference removes the per-state g(k,Rp) setup cost
+    t = []
+    for lim in (2, 10):
+        t0 = time.perf_counter()
+        c, _ = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=lim)
+        t.append(time.perf_counter() - t0)
     nact = int((c >= 0).sum())
-    per_row = max(t_pilot / pilot, 1e-6)
-    ikp_limit = int(max(pilot, min(el.nwv, target_s / per_row)))
+    per_row = max((t[1] - t[0]) / 8.0, 1e-6)
+    ikp_limit = int(max(10, min(el.nwv, (target_s - t[0]) / per_row)))
     t0 = time.perf_counter()
     c, g = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=ikp_limit)
     dt = time.perf_counter() - t0
@@ -82,7 +85,6 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--kblock", type=int, default=0, help="IBZ k points per step (0 = all: the whole job)")
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -109,59 +111,92 @@ def main():
     ph = bz.Phonons(w, sym, mat.phmesh)
     log(f"[bench] setup: nsymm={sym.nsymm} nwv={el.nwv} nwv_irred={el.nwv_irred} "
         f"nstates_inwindow={el.nstates_inwindow} ({time.perf_counter() - t0:.1f} s)")
-    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=args.precision)
     nb, nbr = mat.nb, mat.nbr
     kb = el.nwv_irred if args.kblock <= 0 else min(args.kblock, el.nwv_irred)
     nblocks = max(1, el.nwv_irred // kb)
-    out = torch.empty((kb * nb, plan.nprocs), dtype=torch.float64, device=dev)
-
-    def step(i):
-        b = i % nblocks
-        first, last = b * kb * nb + 1, (b + 1) * kb * nb
-        counts, _ = plan.run(first, last, out=out)
-        return counts
-
-    for i in range(args.warmup):
-        step(i)
-        log(f"[bench] warmup step {i} done")
-    torch.cuda.synchronize()
-    vertices = 0
-    dom_ms, dom_n, flops_launch, survey_launch = 0.0, 0, 0.0, 0.0
     nm_avg = float((np.abs(el.ens_irred - el.enref) <= el.fsthick).sum()) / max(1, el.nwv_irred)
+    plans = {pr: interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=pr) for pr in ("mixed", "fp64")}
+    nprocs = plans["fp64"].nprocs
+    out = torch.empty((kb * nb, nprocs), dtype=torch.float64, device=dev)
+
+    def timed(plan, tag):
+        def step(i):
+            b = i % nblocks
+            counts, _ = plan.run(b * kb * nb + 1, (b + 1) * kb * nb, out=out)
+            return counts
+        for i in range(args.warmup):
+            step(i)
+        log(f"[bench] {tag}: {args.warmup} warmup steps done")
+        vertices, dom_ms, dom_n, nst_launch = 0, 0.0, 0, 0
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for i in range(args.steps):
+            counts = step(args.warmup + i)
+            nact = int((counts >= 0).sum())
+            vertices += int(counts[counts >= 0].sum())
+            tm = plan.timing()
+            dom_ms += tm["dominant_ms"]
+            dom_n += tm["launches"]
+            nst_launch += nact
+            log(f"[bench] {tag} step {i}: {nact} states, {tm['dominant_ms']:.2f} ms in {tm['launches']} pair "
+                f"launches, {tm['all_ms']:.2f} ms device total")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        avg_states = nst_launch / max(dom_n, 1)
+        return dict(value=vertices / elapsed, elapsed=elapsed, vertices=vertices,
+                    avg_launch_s=dom_ms / max(dom_n, 1) * 1e-3, states_per_launch=avg_states)
+
+    res_mx = timed(plans["mixed"], "mixed")
+    res_64 = timed(plans["fp64"], "fp64")
+
+    # accuracy of the mixed path against the FP64 path on the same states
+    ncmp = min(el.nwv_irred, 32) * nb
+    a = torch.empty((ncmp, nprocs), dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    c64, _ = plans["fp64"].run(1, ncmp, out=a)
+    cmx, _ = plans["mixed"].run(1, ncmp, out=b)
     torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        counts = step(args.warmup + i)
-        nact = int((counts >= 0).sum())
-        vertices += int(counts[counts >= 0].sum())
-        tm = plan.timing()
-        dom_ms += tm["dominant_ms"]
-        dom_n += tm["launches"]
-        flops_launch += algorithmic_flops(nact, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg)
-        survey_launch += survey_flops(nact, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg, nm_avg)
-        log(f"[bench] step {i}: {nact} states, {tm['dominant_ms']:.1f} ms in {tm['launches']} pair launches, "
-            f"{tm['all_ms']:.1f} ms device total")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    value = vertices / elapsed
-    avg_launch_s = dom_ms / max(dom_n, 1) * 1e-3
-    achieved = (flops_launch / max(dom_n, 1)) / avg_launch_s / 1e12
-    achieved_survey = (survey_launch / max(dom_n, 1)) / avg_launch_s / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "k_pair_fp64<3>", "avg_launch_ms": round(avg_launch_s * 1e3, 3),
-                "flops_per_launch": flops_launch / max(dom_n, 1),
-                "achieved_unrotated_count": round(achieved_survey, 3)}
-    res = {"metric": METRIC, "value": value, "unit": "vertices/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "mixed",
-           "data": "synthetic",
-           "config": {"workload": f"{mat.name}: calculate_eph_interaction_ibzk('g'), {kb} IBZ k per step",
-                      "precision": args.precision, "nb": nb, "nbr": nbr, "nws": mat.nwsg,
-                      "mesh": [int(x) for x in mat.wvmesh], "nwv_window": el.nwv,
-                      "nwv_irred": el.nwv_irred, "nstates_inwindow": el.nstates_inwindow,
-                      "vertices_timed": vertices},
-           "roofline": roofline}
+    act = c64 >= 0
+    assert np.array_equal(c64, cmx)
+    ref = a.cpu().numpy()[act].ravel()
+    got = b.cpu().numpy()[act].ravel()
+    sel = np.abs(ref) > 1e-6 * np.abs(ref).max()
+    rel = np.abs(got[sel] - ref[sel]) / np.abs(ref[sel])
+    exact_zero_match = bool(np.array_equal(ref == 0.0, got == 0.0))
+    err = {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)), "floor": "1e-6*max|g|^2",
+           "entries_compared": int(sel.sum()), "reference": "FP64 path (itself within 1e-12 of the oracle)",
+           "zeros_match": exact_zero_match, "bound_met": bool(rel.max() <= 1e-5)}
+    log(f"[bench] mixed vs fp64: max rel {err['max_rel']:.2e} median {err['median_rel']:.2e}")
+
+    kp64 = ((mat.nwsg + 63) // 64) * 64
+    spl = res_mx["states_per_launch"]
+    i8_ops = spl * el.nwv * 10 * 4 * 2.0 * mat.nwsg * nb * nbr   # 10 digit products x 4 real products
+    ach_i8 = i8_ops / res_mx["avg_launch_s"] / 1e12
+    f64eq_mx = algorithmic_flops(spl, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg) / res_mx["avg_launch_s"] / 1e12
+    spl64 = res_64["states_per_launch"]
+    fl64 = algorithmic_flops(spl64, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg)
+    ach64 = fl64 / res_64["avg_launch_s"] / 1e12
+    roofline = {"bound": "mfma", "achieved": round(ach_i8, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(ach_i8 / INT8_PEAK_TOPS, 4), "traffic": None, "kernel": "k_pair_i8<3>",
+                "avg_launch_ms": round(res_mx["avg_launch_s"] * 1e3, 3),
+                "ops_per_launch": i8_ops, "fp64_equivalent_tflops": round(f64eq_mx, 2)}
+    fp64 = {"value": res_64["value"], "ms_per_step": res_64["elapsed"] / args.steps * 1e3,
+            "roofline": {"bound": "mfma", "achieved": round(ach64, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(ach64 / FP64_PEAK_TFLOPS, 4), "kernel": "k_pair_fp64<3>",
+                         "avg_launch_ms": round(res_64["avg_launch_s"] * 1e3, 3), "flops_per_launch": fl64,
+                         "achieved_unrotated_count": round(
+                             survey_flops(spl64, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg, nm_avg)
+                             / res_64["avg_launch_s"] / 1e12, 2)}}
+    res = {"metric": METRIC, "value": res_mx["value"], "unit": "vertices/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res_mx["elapsed"] / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int8+f64", "data": "synthetic",
+           "config": {"workload": f"{mat.name}: calculate_eph_interaction_ibzk('g'), all {el.nwv_irred} IBZ k "
+                                  f"per step; mixed precision (config 2) timed first, plain FP64 (config 1) after",
+                      "nb": nb, "nbr": nbr, "nws": mat.nwsg, "mesh": [int(x) for x in mat.wvmesh],
+                      "nwv_window": el.nwv, "nwv_irred": el.nwv_irred, "nstates_inwindow": el.nstates_inwindow,
+                      "vertices_per_step": res_mx["vertices"] // args.steps},
+           "mixed_vs_fp64": round(res_mx["value"] / res_64["value"], 3), "error_vs_fp64": err,
+           "fp64": fp64, "roofline": roofline}
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         first_active = None

@@ -237,6 +237,66 @@ struct PairArgs {
     int64_t nprocs;
 };
 
+// ibzk epilogue of one k' row (row `row`, Y row `yi` of the staged 16 x CP
+// tile): for every in-window final band n (ranks lk, lk+4, ...) and branch s,
+//   g = sum_sp u_q(s,sp) sum_np conj(U_k'(n,np)) Y(np + nb sp),
+//   |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu   (0 when omega == 0),
+// written at the reference's position (k' block, rank of n, s).
+__device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int CP, int row, int yi, int lk,
+                                                  int k0i, int k1i, int k2i, double *outrow) {
+    if (row >= a.nwv) return;
+    const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
+    const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
+    const int q2 = pos_mod(a.kp_int[3 * row + 2] - k2i, a.Nel2);
+    const double2 *uq;
+    const double *omq;
+    if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
+        const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+        uq = a.u + (size_t)iq * a.nbr * a.nbr;
+        omq = a.om + (size_t)iq * a.nbr;
+    } else {
+        const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+        uq = a.uf + (size_t)iq * a.nbr * a.nbr;
+        omq = a.omf + (size_t)iq * a.nbr;
+    }
+    const int nn = a.nwin[row];
+    const double2 *Yr = Y + yi * CP;
+    for (int rank = lk; rank < nn; rank += 4) {
+        const int n = a.wband[row * a.nb + rank];
+        const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
+        double *o = outrow + a.offs[row] + (int64_t)rank * a.nbr;
+        for (int s0 = 0; s0 < a.nbr; s0 += 6) {
+            double2 gb[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) gb[j] = make_double2(0.0, 0.0);
+            for (int sp = 0; sp < a.nbr; ++sp) {
+                double2 z = make_double2(0.0, 0.0);
+                for (int np = 0; np < a.nb; ++np) {
+                    const double2 v = V[np];
+                    const double2 y = Yr[np + a.nb * sp];
+                    z.x = fma(v.x, y.x, z.x);  // conj(v) * y
+                    z.x = fma(v.y, y.y, z.x);
+                    z.y = fma(v.x, y.y, z.y);
+                    z.y = fma(-v.y, y.x, z.y);
+                }
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    if (s0 + j < a.nbr) cfma(gb[j], uq[(s0 + j) * a.nbr + sp], z);
+            }
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+                const int sb = s0 + j;
+                if (sb < a.nbr) {
+                    const double w = omq[sb];
+                    double g2 = 0.0;
+                    if (w != 0.0) g2 = 0.5 * (gb[j].x * gb[j].x + gb[j].y * gb[j].y) / w * kG2Unit;
+                    o[sb] = g2;
+                }
+            }
+        }
+    }
+}
+
 // Phase-contraction core shared by the ibzk and ibzq pair kernels:
 //   acc[rb][c] (16 rows x 16 cols) += sum_K T[(coef . R_K) mod L] * B[K][c]
 // B = hst ([Kp][CP] complex) is streamed through LDS in KC-row chunks with a
@@ -343,8 +403,7 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 
     // ---- epilogue: per row block, stage Y (16 x CP) of this wave in LDS ----
     double2 *Y = smem + wave * 16 * CP;
-    const int64_t slot = a.st_slot[st];
-    double *outrow = a.out + slot * a.nprocs;
+    double *outrow = a.out + a.st_slot[st] * a.nprocs;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
@@ -352,60 +411,233 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * CP + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
         __syncthreads();
-        const int row = row_base + rb * 16 + li;
-        if (row < a.nwv) {
-            const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
-            const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
-            const int q2 = pos_mod(a.kp_int[3 * row + 2] - k2i, a.Nel2);
-            const double2 *uq;
-            const double *omq;
-            if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
-                const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-                uq = a.u + (size_t)iq * a.nbr * a.nbr;
-                omq = a.om + (size_t)iq * a.nbr;
-            } else {
-                const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
-                uq = a.uf + (size_t)iq * a.nbr * a.nbr;
-                omq = a.omf + (size_t)iq * a.nbr;
+        ibzk_epilogue_row(a, Y, CP, row_base + rb * 16 + li, li, lk, k0i, k1i, k2i, outrow);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision phase contraction.  Both operands are split into kNS int8
+// signed digits (base 254, split_digits); the kNS (kNS+1)/2 digit products with
+// i + j < kNS run on the int8 matrix cores with exact int32 accumulation over
+// the whole K, one accumulator per level i + j, recombined in FP64 with the
+// level weights 1 / (127^2 254^l) and the column scale of B.  The band/mode
+// rotations and |g|^2 stay FP64.
+// ---------------------------------------------------------------------------
+constexpr int kNS = 4;
+
+// Atab[q][kc][p][s][g] (16 bytes): digit s of Re (p = 0) / Im (p = 1) of
+// e^{i 2 pi q.R_k} for k = 64 kc + 16 g + b, q over the whole electron mesh.
+__global__ void k_build_atab(int nq, int nkc, int nws, int N0, int N1, int L, int Lf0, int Lf1, int Lf2,
+                             const int *__restrict__ R, const signed char *__restrict__ Tsl, v4i *__restrict__ atab) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nq * nkc * 2 * kNS * 4;
+    if (t >= total) return;
+    const int g = (int)(t % 4);
+    int64_t r = t / 4;
+    const int sd = (int)(r % kNS);
+    r /= kNS;
+    const int p = (int)(r % 2);
+    r /= 2;
+    const int kc = (int)(r % nkc);
+    const int q = (int)(r / nkc);
+    const int q0 = q % N0, q1 = (q / N0) % N1, q2 = q / (N0 * N1);
+    const int c0 = (q0 * Lf0) % L, c1 = (q1 * Lf1) % L, c2 = (q2 * Lf2) % L;
+    union { signed char b[16]; v4i v; } u;
+    for (int b = 0; b < 16; ++b) {
+        const int k = kc * 64 + g * 16 + b;
+        signed char v = 0;
+        if (k < nws) {
+            const int x = c0 * R[4 * k] + c1 * R[4 * k + 1] + c2 * R[4 * k + 2];
+            v = Tsl[(pos_mod(x, L) * 2 + p) * kNS + sd];
+        }
+        u.b[b] = v;
+    }
+    atab[t] = u.v;
+}
+
+// Column scale of B = h[st] (max over K of max(|Re|, |Im|); 1 for zero columns).
+__global__ void k_colscale(int nst, int Kp, int CP, const double2 *__restrict__ h, double *__restrict__ scale) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nst * CP) return;
+    const int st = t / CP, c = t % CP;
+    const double2 *p = h + (size_t)st * Kp * CP + c;
+    double m = 0.0;
+    for (int k = 0; k < Kp; ++k) {
+        const double2 v = p[(size_t)k * CP];
+        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+    scale[t] = m > 0.0 ? m : 1.0;
+}
+
+// Bsl[st][kc][cb][p][s][lane] (16 bytes): digits of B / scale in the int8 MFMA
+// B-operand order, p = Re, Im, -Im.
+__global__ void k_slice_b(int nst, int nkc, int CB, int Kp, int CP, const double2 *__restrict__ h,
+                          const double *__restrict__ scale, v4i *__restrict__ bsl) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nst * nkc * CB * 64;
+    if (t >= total) return;
+    const int lane = (int)(t % 64);
+    int64_t r = t / 64;
+    const int cb = (int)(r % CB);
+    r /= CB;
+    const int kc = (int)(r % nkc);
+    const int st = (int)(r / nkc);
+    const int col = cb * 16 + (lane & 15), g = lane >> 4;
+    const double inv = 1.0 / scale[(size_t)st * CP + col];
+    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS], unim[kNS];
+    for (int b = 0; b < 16; ++b) {
+        const int k = kc * 64 + g * 16 + b;
+        const double2 x = h[((size_t)st * Kp + k) * CP + col];
+        signed char dr[kNS], di[kNS];
+        split_digits<kNS>(x.x * inv, dr);
+        split_digits<kNS>(x.y * inv, di);
+#pragma unroll
+        for (int sd = 0; sd < kNS; ++sd) {
+            ure[sd].b[b] = dr[sd];
+            uim[sd].b[b] = di[sd];
+            unim[sd].b[b] = (signed char)(-di[sd]);
+        }
+    }
+    v4i *o = bsl + ((((size_t)st * nkc + kc) * CB + cb) * 3) * kNS * 64 + lane;
+#pragma unroll
+    for (int sd = 0; sd < kNS; ++sd) {
+        o[(0 * kNS + sd) * 64] = ure[sd].v;
+        o[(1 * kNS + sd) * 64] = uim[sd].v;
+        o[(2 * kNS + sd) * 64] = unim[sd].v;
+    }
+}
+
+template <int CB>
+__global__ __launch_bounds__(256) void k_pair_i8(PairArgs a, const v4i *__restrict__ atab,
+                                                 const v4i *__restrict__ bsl, const double *__restrict__ bscale,
+                                                 int nkc) {
+    constexpr int RB = 2;
+    constexpr int NS = kNS;
+    constexpr int CP = 16 * CB;
+    constexpr int BCH = CB * 3 * NS * 64;  // v4i per K chunk of one state
+    constexpr int NPF = (BCH + 255) / 256;
+    extern __shared__ v4i smem8[];
+    v4i *buf0 = smem8, *buf1 = smem8 + BCH;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    const int st = blockIdx.y;
+    const int row_base = blockIdx.x * 128 + wave * 32;
+    const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
+    const int qstride = nkc * 2 * NS * 4;  // v4i per q row of atab
+    const v4i *arow[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int r = row_base + rb * 16 + li;
+        int q = 0;
+        if (r < a.nwv) {
+            const int q0 = pos_mod(a.kp_int[3 * r] - k0i, a.Nel0);
+            const int q1 = pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1);
+            const int q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
+            q = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+        }
+        arow[rb] = atab + (size_t)q * qstride + lg;
+    }
+    v4i acc[RB][CB][NS][2];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int l = 0; l < NS; ++l) { acc[rb][c][l][0] = v4i{0, 0, 0, 0}; acc[rb][c][l][1] = v4i{0, 0, 0, 0}; }
+    const v4i *bst = bsl + (size_t)st * nkc * BCH;
+    for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
+    v4i Ar[RB][NS], Ai[RB][NS];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int sd = 0; sd < NS; ++sd) {
+            Ar[rb][sd] = arow[rb][(0 * NS + sd) * 4];
+            Ai[rb][sd] = arow[rb][(1 * NS + sd) * 4];
+        }
+    __syncthreads();
+    for (int kc = 0; kc < nkc; ++kc) {
+        const bool more = kc + 1 < nkc;
+        v4i pf[NPF];
+        v4i nAr[RB][NS], nAi[RB][NS];
+        if (more) {
+            const v4i *src = bst + (size_t)(kc + 1) * BCH;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < BCH) pf[j] = src[i];
             }
-            const int nn = a.nwin[row];
-            const double2 *Yr = Y + li * CP;
-            for (int rank = lk; rank < nn; rank += 4) {
-                const int n = a.wband[row * a.nb + rank];
-                const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
-                double *o = outrow + a.offs[row] + (int64_t)rank * a.nbr;
-                for (int s0 = 0; s0 < a.nbr; s0 += 6) {
-                    double2 gb[6];
 #pragma unroll
-                    for (int j = 0; j < 6; ++j) gb[j] = make_double2(0.0, 0.0);
-                    for (int sp = 0; sp < a.nbr; ++sp) {
-                        double2 z = make_double2(0.0, 0.0);
-                        for (int np = 0; np < a.nb; ++np) {
-                            const double2 v = V[np];
-                            const double2 y = Yr[np + a.nb * sp];
-                            // conj(v) * y
-                            z.x = fma(v.x, y.x, z.x);
-                            z.x = fma(v.y, y.y, z.x);
-                            z.y = fma(v.x, y.y, z.y);
-                            z.y = fma(-v.y, y.x, z.y);
-                        }
+            for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                        for (int j = 0; j < 6; ++j)
-                            if (s0 + j < a.nbr) cfma(gb[j], uq[(s0 + j) * a.nbr + sp], z);
-                    }
-#pragma unroll
-                    for (int j = 0; j < 6; ++j) {
-                        const int s = s0 + j;
-                        if (s < a.nbr) {
-                            const double w = omq[s];
-                            double g2 = 0.0;
-                            if (w != 0.0) g2 = 0.5 * (gb[j].x * gb[j].x + gb[j].y * gb[j].y) / w * kG2Unit;
-                            o[s] = g2;
-                        }
-                    }
+                for (int sd = 0; sd < NS; ++sd) {
+                    nAr[rb][sd] = arow[rb][((size_t)(kc + 1) * 2 * NS + 0 * NS + sd) * 4];
+                    nAi[rb][sd] = arow[rb][((size_t)(kc + 1) * 2 * NS + 1 * NS + sd) * 4];
                 }
+        }
+        const v4i *B = (kc & 1) ? buf1 : buf0;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            v4i Bre[NS], Bim[NS], Bnim[NS];
+#pragma unroll
+            for (int sd = 0; sd < NS; ++sd) {
+                Bre[sd] = B[((c * 3 + 0) * NS + sd) * 64 + lane];
+                Bim[sd] = B[((c * 3 + 1) * NS + sd) * 64 + lane];
+                Bnim[sd] = B[((c * 3 + 2) * NS + sd) * 64 + lane];
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int j = 0; j < NS - i; ++j) {
+                        const int l = i + j;
+                        acc[rb][c][l][0] = mfma_i8(Ar[rb][i], Bre[j], acc[rb][c][l][0]);
+                        acc[rb][c][l][0] = mfma_i8(Ai[rb][i], Bnim[j], acc[rb][c][l][0]);
+                        acc[rb][c][l][1] = mfma_i8(Ar[rb][i], Bim[j], acc[rb][c][l][1]);
+                        acc[rb][c][l][1] = mfma_i8(Ai[rb][i], Bre[j], acc[rb][c][l][1]);
+                    }
+        }
+        if (more) {
+            v4i *dst = (kc & 1) ? buf0 : buf1;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < BCH) dst[i] = pf[j];
+            }
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+                for (int sd = 0; sd < NS; ++sd) { Ar[rb][sd] = nAr[rb][sd]; Ai[rb][sd] = nAi[rb][sd]; }
+        }
+        __syncthreads();
+    }
+    // ---- recombine levels in FP64 and run the FP64 epilogue ----
+    double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * CP;
+    double *outrow = a.out + a.st_slot[st] * a.nprocs;
+    double wl[NS];
+    wl[0] = 1.0 / (127.0 * 127.0);
+#pragma unroll
+    for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
+    const double *sc = bscale + (size_t)st * CP;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const double s = sc[c * 16 + li];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for (int l = NS - 1; l >= 0; --l) {
+                    re += wl[l] * (double)acc[rb][c][l][0][r];
+                    im += wl[l] * (double)acc[rb][c][l][1][r];
+                }
+                Y[(4 * lg + r) * CP + c * 16 + li] = make_double2(re * s, im * s);
             }
         }
+        __syncthreads();
+        ibzk_epilogue_row(a, Y, CP, row_base + rb * 16 + li, li, lg, k0i, k1i, k2i, outrow);
         __syncthreads();
     }
 }
@@ -1490,6 +1722,12 @@ struct elph_ibzk_plan {
     DBuf<double> kv, g2stage;
     DBuf<int> st_kb, st_m, st_kint;
     DBuf<int64_t> st_slot;
+    // mixed precision: digit table of the phases (all q of the electron mesh),
+    // per-batch digits and column scales of B
+    int64_t nkc = 0;
+    DBuf<signed char> Tsl;
+    DBuf<v4i> atab, bsl;
+    DBuf<double> bscale;
 };
 
 namespace {
@@ -1526,15 +1764,30 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     const int CB = (int)(p->CP / 16);
     const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
-    HCHECK(hipEventRecord(c->ev_a, c->stream));
-    switch (CB) {
-        case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, c->stream, a); break;
-        case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, c->stream, a); break;
-        case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, c->stream, a); break;
-        case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, c->stream, a); break;
-        case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, c->stream, a); break;
-        case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, c->stream, a); break;
-        default: throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported by the fused kernel"};
+    if (p->precision == ELPH_MIXED) {
+        const size_t lds8 = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i);
+        const v4i *at = p->atab.p, *bs = p->bsl.p;
+        const double *sc = p->bscale.p;
+        const int nkc = (int)p->nkc;
+        HCHECK(hipEventRecord(c->ev_a, c->stream));
+        switch (CB) {
+            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
+            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
+            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
+            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
+            default: throw Error{ELPH_EINVAL, "mixed precision supports numwannbands*numbranches <= 64"};
+        }
+    } else {
+        HCHECK(hipEventRecord(c->ev_a, c->stream));
+        switch (CB) {
+            case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, c->stream, a); break;
+            case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, c->stream, a); break;
+            case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, c->stream, a); break;
+            case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, c->stream, a); break;
+            case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, c->stream, a); break;
+            case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, c->stream, a); break;
+            default: throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported by the fused kernel"};
+        }
     }
     HCHECK(hipGetLastError());
     HCHECK(hipEventRecord(c->ev_b, c->stream));
@@ -1579,7 +1832,9 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
         if (nb * nbr > 96) throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported"};
         p->CP = ((nb * nbr + 15) / 16) * 16;
-        p->Kp = ((ctx->nwsg + 31) / 32) * 32;
+        p->Kp = ((ctx->nwsg + 63) / 64) * 64;
+        if (precision == ELPH_MIXED && p->CP > 64)
+            throw Error{ELPH_EINVAL, "mixed precision supports numwannbands*numbranches <= 64"};
         hipStream_t s = ctx->stream;
         // k' rows
         std::vector<int> kpi(3 * nwv), nw(nwv), wb(nwv * nb, 0);
@@ -1667,6 +1922,27 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         for (int64_t i = 0; i < ctx->nwsg; ++i)
             for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_q[i + ctx->nwsq * d];
         p->Rq.upload(R, s);
+        if (precision == ELPH_MIXED) {
+            p->nkc = p->Kp / 64;
+            std::vector<signed char> tsl(p->L * 2 * kNS);
+            for (int64_t j = 0; j < p->L; ++j) {
+                signed char dr[kNS], di[kNS];
+                split_digits<kNS>(T[j].x, dr);
+                split_digits<kNS>(T[j].y, di);
+                for (int sd = 0; sd < kNS; ++sd) {
+                    tsl[(j * 2 + 0) * kNS + sd] = dr[sd];
+                    tsl[(j * 2 + 1) * kNS + sd] = di[sd];
+                }
+            }
+            p->Tsl.upload(tsl, s);
+            const int64_t nq = p->Nel[0] * p->Nel[1] * p->Nel[2];
+            const int64_t tot = nq * p->nkc * 2 * kNS * 4;
+            p->atab.alloc(tot);
+            hipLaunchKernelGGL(k_build_atab, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nq, (int)p->nkc,
+                               (int)ctx->nwsg, (int)p->Nel[0], (int)p->Nel[1], (int)p->L, (int)p->Lf[0],
+                               (int)p->Lf[1], (int)p->Lf[2], p->Rq.p, p->Tsl.p, p->atab.p);
+            HCHECK(hipGetLastError());
+        }
         // IBZ data
         p->kint_irr.resize(3 * p->nirr);
         p->ens_irred.assign(el->ens_irred, el->ens_irred + p->nirr * nb);
@@ -1767,6 +2043,17 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
                                (int)nbr, (int)c->nwsg, (int)p->Kp, (int)p->CP, p->gk.p, nb2 * nslices, p->st_kb.p,
                                p->st_m.p, p->Ukb.p, c->degq.p, p->h.p);
             HCHECK(hipGetLastError());
+            if (p->precision == ELPH_MIXED) {
+                const int CB = (int)(p->CP / 16);
+                p->bscale.alloc(nst * p->CP);
+                p->bsl.alloc(nst * p->nkc * CB * 3 * kNS * 64);
+                hipLaunchKernelGGL(k_colscale, dim3((unsigned)((nst * p->CP + 255) / 256)), dim3(256), 0, s, (int)nst,
+                                   (int)p->Kp, (int)p->CP, p->h.p, p->bscale.p);
+                const int64_t nt = nst * p->nkc * CB * 64;
+                hipLaunchKernelGGL(k_slice_b, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, (int)nst,
+                                   (int)p->nkc, CB, (int)p->Kp, (int)p->CP, p->h.p, p->bscale.p, p->bsl.p);
+                HCHECK(hipGetLastError());
+            }
             double *dst = g2;
             if (!on_device) {
                 p->g2stage.alloc(nst * p->nprocs);

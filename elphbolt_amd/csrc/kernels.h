@@ -38,6 +38,29 @@ __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
     acc.y = fma(a.y, b.x, acc.y);
 }
 
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// D(16x16, int32) += A(16x64, int8) B(64x16, int8), exact.  Operand layout
+// (checked on gfx950 by a layout probe): lane i + 16 g holds A[i][16 g .. 16 g + 15]
+// (16 bytes) and B[16 g .. 16 g + 15][i]; D[i][j] is element i % 4 of lane
+// j + 16 (i / 4).
+__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+// Signed-digit split of x in [-1, 1] into NS int8 digits d_s in [-127, 127]:
+// x ~= sum_s d_s / (127 * 254^s), error below 1 / (2 * 127 * 254^(NS-1)).
+template <int NS>
+__host__ __device__ inline void split_digits(double x, signed char (&d)[NS]) {
+    double r = x, scale = 127.0;
+    for (int s = 0; s < NS; ++s) {
+        const double q = rint(r * scale);
+        d[s] = (signed char)q;
+        r -= q / scale;
+        scale *= 254.0;
+    }
+}
+
 __device__ __forceinline__ int pos_mod(int x, int L) {
     int m = x % L;
     return m < 0 ? m + L : m;

@@ -1,0 +1,86 @@
+"""Mixed-precision vertex path (int8 signed-digit products with exact int32
+accumulation, FP64 recombination and rotations) against the FP64 oracle.
+
+Bound (stated for this path): max relative error of |g|^2 <= 1e-5 on entries
+above 1e-6 * max|g|^2; index maps, counts and exact zeros (omega = 0) as in
+the FP64 path."""
+import numpy as np
+import pytest
+
+from elphbolt_amd import interactions, synth
+from elphbolt_amd.wannier import Wannier
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL_MIXED = 1e-5
+FLOOR = 1e-6
+
+
+def _rel(x, ref):
+    x, ref = np.asarray(x).ravel(), np.asarray(ref).ravel()
+    sel = np.abs(ref) > FLOOR * np.abs(ref).max()
+    e = np.abs(x[sel] - ref[sel]) / np.abs(ref[sel])
+    return float(e.max()), float(np.median(e))
+
+
+def _state(mat):
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    return o, el, ph
+
+
+@pytest.mark.parametrize("kw", [dict(nb=2, natoms=1, ws=3, wv=4, seed=11),
+                                dict(nb=4, natoms=2, ws=3, wv=6, seed=12),
+                                dict(nb=8, natoms=2, ws=4, wv=6, seed=13, bands_in_window=4)])
+def test_ibzk_mixed_vs_oracle(kw):
+    mat = synth.tiny(**kw)
+    o, el, ph = _state(mat)
+    last = min(el.nwv_irred * mat.nb, 24)
+    print(f"mixed ibzk {mat.name}: states 1..{last}", flush=True)
+    c_o, g_o = O.ibzk_g2(o, el, ph, 1, 1, last)
+    w = Wannier(mat)
+    c_g, g_g = interactions.IbzkPlan(w, el, ph, precision="mixed").run(1, last)
+    assert np.array_equal(c_o, c_g)
+    act = c_o >= 0
+    assert np.array_equal(g_o[act] == 0.0, g_g[act] == 0.0)
+    mx, med = _rel(g_g[act], g_o[act])
+    print(f"  max rel {mx:.2e} median {med:.2e}", flush=True)
+    assert mx < REL_MIXED
+
+
+def test_ibzk_mixed_mesh_ref2():
+    mat = synth.tiny(nb=4, natoms=1, ws=3, wv=4, seed=21, mesh_ref=2)
+    o, el, ph = _state(mat)
+    last = min(el.nwv_irred * mat.nb, 12)
+    w = Wannier(mat)
+    c64, g64 = interactions.IbzkPlan(w, el, ph, mesh_ref=2, precision="fp64").run(1, last)
+    cmx, gmx = interactions.IbzkPlan(w, el, ph, mesh_ref=2, precision="mixed").run(1, last)
+    assert np.array_equal(c64, cmx)
+    act = c64 >= 0
+    assert _rel(gmx[act], g64[act])[0] < REL_MIXED
+
+
+def test_mixed_equals_fp64_path_on_device_buffer():
+    """Mixed and FP64 plans on the same material, output into device memory
+    (torch tensor) as bench.py does."""
+    torch = pytest.importorskip("torch")
+    mat = synth.tiny(nb=4, natoms=2, ws=3, wv=6, seed=12)
+    o, el, ph = _state(mat)
+    w = Wannier(mat)
+    last = el.nwv_irred * mat.nb
+    p64 = interactions.IbzkPlan(w, el, ph, precision="fp64")
+    pmx = interactions.IbzkPlan(w, el, ph, precision="mixed")
+    d64 = torch.zeros((last, p64.nprocs), dtype=torch.float64, device="cuda:0")
+    dmx = torch.zeros_like(d64)
+    c64, _ = p64.run(1, last, out=d64)
+    cmx, _ = pmx.run(1, last, out=dmx)
+    torch.cuda.synchronize()
+    assert np.array_equal(c64, cmx)
+    act = c64 >= 0
+    a, b = d64.cpu().numpy()[act], dmx.cpu().numpy()[act]
+    assert _rel(b, a)[0] < REL_MIXED
+    c_host, g_host = p64.run(1, last)
+    assert np.array_equal(g_host[act], a)
