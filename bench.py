@@ -95,7 +95,6 @@ def main():
                                    "described in DESIGN.md, not run)"}))
         sys.exit(2)
 
-    import torch
     from elphbolt_amd import bz, interactions, synth
     from elphbolt_amd.wannier import Wannier
 
@@ -103,8 +102,6 @@ def main():
     mat = synth.config(args.config)
     log(f"[bench] material {mat.name}: nb={mat.nb} nbr={mat.nbr} nws={mat.nwsk} mesh={list(mat.wvmesh)} "
         f"enref={mat.enref:.3f} fsthick={mat.fsthick:.3f} ({time.perf_counter() - t0:.1f} s)")
-    dev = torch.device("cuda:0")
-    torch.cuda.init()
     w = Wannier(mat, device=0)
     sym = bz.Symmetry(mat.lattvecs, mat.basis, mat.masses, mat.wvmesh)
     el = bz.Electrons(w, sym, mat.wvmesh, mat.enref, mat.fsthick)
@@ -117,7 +114,7 @@ def main():
     nm_avg = float((np.abs(el.ens_irred - el.enref) <= el.fsthick).sum()) / max(1, el.nwv_irred)
     plans = {pr: interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=pr) for pr in ("mixed", "fp64")}
     nprocs = plans["fp64"].nprocs
-    out = torch.empty((kb * nb, nprocs), dtype=torch.float64, device=dev)
+    out = w.device_buffer((kb * nb, nprocs))
 
     def timed(plan, tag):
         def step(i):
@@ -128,7 +125,7 @@ def main():
             step(i)
         log(f"[bench] {tag}: {args.warmup} warmup steps done")
         vertices, dom_ms, dom_n, nst_launch = 0, 0.0, 0, 0
-        torch.cuda.synchronize()
+        w.synchronize()
         t_start = time.perf_counter()
         for i in range(args.steps):
             counts = step(args.warmup + i)
@@ -140,7 +137,7 @@ def main():
             nst_launch += nact
             log(f"[bench] {tag} step {i}: {nact} states, {tm['dominant_ms']:.2f} ms in {tm['launches']} pair "
                 f"launches, {tm['all_ms']:.2f} ms device total")
-        torch.cuda.synchronize()
+        w.synchronize()
         elapsed = time.perf_counter() - t_start
         avg_states = nst_launch / max(dom_n, 1)
         return dict(value=vertices / elapsed, elapsed=elapsed, vertices=vertices,
@@ -151,15 +148,17 @@ def main():
 
     # accuracy of the mixed path against the FP64 path on the same states
     ncmp = min(el.nwv_irred, 32) * nb
-    a = torch.empty((ncmp, nprocs), dtype=torch.float64, device=dev)
-    b = torch.empty_like(a)
+    a = w.device_buffer((ncmp, nprocs))
+    b = w.device_buffer((ncmp, nprocs))
     c64, _ = plans["fp64"].run(1, ncmp, out=a)
     cmx, _ = plans["mixed"].run(1, ncmp, out=b)
-    torch.cuda.synchronize()
+    w.synchronize()
     act = c64 >= 0
     assert np.array_equal(c64, cmx)
-    ref = a.cpu().numpy()[act].ravel()
-    got = b.cpu().numpy()[act].ravel()
+    ref = a.numpy()[act].ravel()
+    got = b.numpy()[act].ravel()
+    a.free()
+    b.free()
     sel = np.abs(ref) > 1e-6 * np.abs(ref).max()
     rel = np.abs(got[sel] - ref[sel]) / np.abs(ref[sel])
     exact_zero_match = bool(np.array_equal(ref == 0.0, got == 0.0))

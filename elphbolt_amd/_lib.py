@@ -73,6 +73,10 @@ PROTOTYPES = {
                                           ct.POINTER(ElphPhonons)]),
     "elph_state_free": (ct.c_int, [ct.c_void_p]),
     "elph_selftest_mfma": (ct.c_int, [ct.c_int, c_dblp]),
+    "elph_device_alloc": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_void_p)]),
+    "elph_device_free": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "elph_device_to_host": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]),
+    "elph_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
 }
 

@@ -1668,6 +1668,38 @@ int elph_g2(elph_ctx *ctx, int64_t n, const double *kvecs, const double *qvecs, 
     return ELPH_OK;
 }
 
+int elph_device_alloc(elph_ctx *ctx, int64_t bytes, void **dptr) {
+    if (!ctx || !dptr || bytes < 0) return ELPH_EINVAL;
+    *dptr = nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
+    if (hipMalloc(dptr, (size_t)std::max<int64_t>(bytes, 1)) != hipSuccess) {
+        ctx->err = "hipMalloc of " + std::to_string(bytes) + " bytes failed";
+        *dptr = nullptr;
+        return ELPH_ENOMEM;
+    }
+    return ELPH_OK;
+}
+
+int elph_device_free(elph_ctx *ctx, void *dptr) {
+    if (!ctx) return ELPH_EINVAL;
+    if (!dptr) return ELPH_OK;
+    if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
+    return hipFree(dptr) == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
+}
+
+int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t bytes) {
+    if (!ctx || bytes < 0) return ELPH_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
+    if (hipMemcpyAsync(host, dptr, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return ELPH_EDEVICE;
+    return hipStreamSynchronize(ctx->stream) == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
+}
+
+int elph_ctx_synchronize(elph_ctx *ctx) {
+    if (!ctx) return ELPH_EINVAL;
+    if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
+    return hipDeviceSynchronize() == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
+}
+
 int elph_selftest_mfma(int device, double *max_err) {
     try {
         HCHECK(hipSetDevice(device));

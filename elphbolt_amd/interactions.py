@@ -49,8 +49,11 @@ def _ph_struct(ph):
 
 
 def _dev_ptr(out):
-    """(pointer, on_device) of an output buffer: a CUDA/HIP torch tensor (device)
-    or a numpy array (host)."""
+    """(pointer, on_device) of an output buffer: a wannier.DeviceBuffer or any
+    object exposing a device pointer as `.ptr`/`data_ptr()` (device), or a numpy
+    array (host)."""
+    if hasattr(out, "ptr") and not isinstance(out, np.ndarray):
+        return out.ptr, 1
     if hasattr(out, "data_ptr") and getattr(out, "is_cuda", False):
         return ct.c_void_p(out.data_ptr()), 1
     return out.ctypes.data_as(ct.c_void_p), 0

@@ -96,6 +96,13 @@ class Wannier:
         except Exception:
             pass
 
+    def device_buffer(self, shape, dtype=np.float64):
+        """HBM buffer on this context's GPU (freed when the object is collected)."""
+        return DeviceBuffer(self, shape, dtype)
+
+    def synchronize(self):
+        self._check(self.lib.elph_ctx_synchronize(self.ctx), "synchronize")
+
     def _check(self, st, what):
         if st != 0:
             msg = self.lib.elph_last_error(self.ctx)
@@ -165,3 +172,32 @@ class Wannier:
         self._check(self.lib.elph_g2(self.ctx, n, _p(k), _p(q), _p(a), _p(b), _p(c), _p(en), _p(gm), len(gm),
                                      _p(gi, c_i64p), b"p" if wannspace == "ph" else b"e", _p(out)), "g2")
         return out
+
+
+class DeviceBuffer:
+    """Device memory allocated through the library (no torch in the process:
+    torch wheels bundle their own HIP runtime)."""
+
+    def __init__(self, wann: Wannier, shape, dtype=np.float64):
+        self.wann, self.shape, self.dtype = wann, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ct.c_void_p()
+        wann._check(wann.lib.elph_device_alloc(wann.ctx, self.nbytes, ct.byref(p)), "device_alloc")
+        self.ptr = p
+
+    def numpy(self):
+        host = np.empty(self.shape, self.dtype)
+        self.wann._check(self.wann.lib.elph_device_to_host(self.wann.ctx, host.ctypes.data_as(ct.c_void_p), self.ptr,
+                                                           self.nbytes), "device_to_host")
+        return host
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.wann.ctx:
+            self.wann.lib.elph_device_free(self.wann.ctx, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

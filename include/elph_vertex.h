@@ -184,6 +184,16 @@ int elph_calculate_phonons(elph_ctx *ctx, int64_t nsymm_rot, const double *qrota
                            elph_state **state, elph_phonons *view);
 int elph_state_free(elph_state *state);
 
+/* Device memory of the context's GPU for callers without their own allocator
+ * (e.g. to keep run() outputs in HBM: pass the pointer with g2_on_device = 1).
+ * No counterpart in the reference (its outputs go to files). */
+int elph_device_alloc(elph_ctx *ctx, int64_t bytes, void **dptr);
+int elph_device_free(elph_ctx *ctx, void *dptr);
+int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t bytes);
+/* Waits for all work of the context's device (the reference's `sync all`
+ * boundary after a driver, src/interactions.f90:2387). */
+int elph_ctx_synchronize(elph_ctx *ctx);
+
 /* Checks the fp64 MFMA operand layout the kernels assume (returns max abs
  * error of a 16x16x4 product against the host; the device must be present). */
 int elph_selftest_mfma(int device, double *max_err);

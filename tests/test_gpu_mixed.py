@@ -64,23 +64,22 @@ def test_ibzk_mixed_mesh_ref2():
 
 
 def test_mixed_equals_fp64_path_on_device_buffer():
-    """Mixed and FP64 plans on the same material, output into device memory
-    (torch tensor) as bench.py does."""
-    torch = pytest.importorskip("torch")
+    """Mixed and FP64 plans on the same material, output into HBM buffers as
+    bench.py does, then compared with the host-output path."""
     mat = synth.tiny(nb=4, natoms=2, ws=3, wv=6, seed=12)
     o, el, ph = _state(mat)
     w = Wannier(mat)
     last = el.nwv_irred * mat.nb
     p64 = interactions.IbzkPlan(w, el, ph, precision="fp64")
     pmx = interactions.IbzkPlan(w, el, ph, precision="mixed")
-    d64 = torch.zeros((last, p64.nprocs), dtype=torch.float64, device="cuda:0")
-    dmx = torch.zeros_like(d64)
+    d64 = w.device_buffer((last, p64.nprocs))
+    dmx = w.device_buffer((last, p64.nprocs))
     c64, _ = p64.run(1, last, out=d64)
     cmx, _ = pmx.run(1, last, out=dmx)
-    torch.cuda.synchronize()
+    w.synchronize()
     assert np.array_equal(c64, cmx)
     act = c64 >= 0
-    a, b = d64.cpu().numpy()[act], dmx.cpu().numpy()[act]
+    a, b = d64.numpy()[act], dmx.numpy()[act]
     assert _rel(b, a)[0] < REL_MIXED
     c_host, g_host = p64.run(1, last)
     assert np.array_equal(g_host[act], a)
