@@ -426,36 +426,6 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int kNS = 4;
 
-// Atab[q][kc][p][s][g] (16 bytes): digit s of Re (p = 0) / Im (p = 1) of
-// e^{i 2 pi q.R_k} for k = 64 kc + 16 g + b, q over the whole electron mesh.
-__global__ void k_build_atab(int nq, int nkc, int nws, int N0, int N1, int L, int Lf0, int Lf1, int Lf2,
-                             const int *__restrict__ R, const signed char *__restrict__ Tsl, v4i *__restrict__ atab) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = (int64_t)nq * nkc * 2 * kNS * 4;
-    if (t >= total) return;
-    const int g = (int)(t % 4);
-    int64_t r = t / 4;
-    const int sd = (int)(r % kNS);
-    r /= kNS;
-    const int p = (int)(r % 2);
-    r /= 2;
-    const int kc = (int)(r % nkc);
-    const int q = (int)(r / nkc);
-    const int q0 = q % N0, q1 = (q / N0) % N1, q2 = q / (N0 * N1);
-    const int c0 = (q0 * Lf0) % L, c1 = (q1 * Lf1) % L, c2 = (q2 * Lf2) % L;
-    union { signed char b[16]; v4i v; } u;
-    for (int b = 0; b < 16; ++b) {
-        const int k = kc * 64 + g * 16 + b;
-        signed char v = 0;
-        if (k < nws) {
-            const int x = c0 * R[4 * k] + c1 * R[4 * k + 1] + c2 * R[4 * k + 2];
-            v = Tsl[(pos_mod(x, L) * 2 + p) * kNS + sd];
-        }
-        u.b[b] = v;
-    }
-    atab[t] = u.v;
-}
-
 // Column scale of B = h[st] (max over K of max(|Re|, |Im|); 1 for zero columns).
 __global__ void k_colscale(int nst, int Kp, int CP, const double2 *__restrict__ h, double *__restrict__ scale) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -508,107 +478,106 @@ __global__ void k_slice_b(int nst, int nkc, int CB, int Kp, int CP, const double
     }
 }
 
+// A digits are generated on the fly: Tdig[j] holds the kNS digits of Re (bytes
+// 0-3) and Im (bytes 4-7) of e^{i 2 pi j / L}.  Rpk[k] packs the cell R_k
+// reduced modulo the electron mesh (bytes 0-2), so the phase index of row q
+// is (c . Rpk) mod L with c_d = q_d L / N_d, all in 24-bit products.  A lane
+// gathers the 16 indices of its row and K slice and byte-transposes the
+// digits into the 2 x kNS operand registers.  One wave owns 16 k' rows and
+// all CP columns of one state; 4 waves share the B chunk staged in LDS.
 template <int CB>
-__global__ __launch_bounds__(256) void k_pair_i8(PairArgs a, const v4i *__restrict__ atab,
-                                                 const v4i *__restrict__ bsl, const double *__restrict__ bscale,
-                                                 int nkc) {
-    constexpr int RB = 2;
+__global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
+                                                    const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
+                                                    const double *__restrict__ bscale, int nkc) {
     constexpr int NS = kNS;
+    static_assert(NS == 4, "digit packing assumes 4 digits per part");
     constexpr int CP = 16 * CB;
     constexpr int BCH = CB * 3 * NS * 64;  // v4i per K chunk of one state
     constexpr int NPF = (BCH + 255) / 256;
     extern __shared__ v4i smem8[];
     v4i *buf0 = smem8, *buf1 = smem8 + BCH;
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
+    unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int st = blockIdx.y;
-    const int row_base = blockIdx.x * 128 + wave * 32;
+    const int row_base = blockIdx.x * 64 + wave * 16;
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
-    const int qstride = nkc * 2 * NS * 4;  // v4i per q row of atab
-    const v4i *arow[RB];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-        const int r = row_base + rb * 16 + li;
-        int q = 0;
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
+    for (int i = tid; i < nkc * 64; i += 256) Rsh[i] = Rpk[i];
+    int c0 = 0, c1 = 0, c2 = 0;
+    {
+        const int r = row_base + li;
         if (r < a.nwv) {
-            const int q0 = pos_mod(a.kp_int[3 * r] - k0i, a.Nel0);
-            const int q1 = pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1);
-            const int q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
-            q = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+            c0 = (pos_mod(a.kp_int[3 * r] - k0i, a.Nel0) * a.Lf0) % a.L;
+            c1 = (pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1) * a.Lf1) % a.L;
+            c2 = (pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2) * a.Lf2) % a.L;
         }
-        arow[rb] = atab + (size_t)q * qstride + lg;
     }
-    v4i acc[RB][CB][NS][2];
+    const float invL = 1.0f / (float)a.L;
+    const int L = a.L;
+    v4i acc[CB][NS][2];
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
+    for (int c = 0; c < CB; ++c)
 #pragma unroll
-        for (int c = 0; c < CB; ++c)
-#pragma unroll
-            for (int l = 0; l < NS; ++l) { acc[rb][c][l][0] = v4i{0, 0, 0, 0}; acc[rb][c][l][1] = v4i{0, 0, 0, 0}; }
+        for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
     const v4i *bst = bsl + (size_t)st * nkc * BCH;
     for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
-    v4i Ar[RB][NS], Ai[RB][NS];
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int sd = 0; sd < NS; ++sd) {
-            Ar[rb][sd] = arow[rb][(0 * NS + sd) * 4];
-            Ai[rb][sd] = arow[rb][(1 * NS + sd) * 4];
-        }
     __syncthreads();
     for (int kc = 0; kc < nkc; ++kc) {
-        const bool more = kc + 1 < nkc;
+        // prefetch the next B chunk (the last chunk re-reads itself: no branch)
+        const v4i *src = bst + (size_t)min(kc + 1, nkc - 1) * BCH;
         v4i pf[NPF];
-        v4i nAr[RB][NS], nAi[RB][NS];
-        if (more) {
-            const v4i *src = bst + (size_t)(kc + 1) * BCH;
 #pragma unroll
-            for (int j = 0; j < NPF; ++j) {
-                const int i = tid + 256 * j;
-                if (i < BCH) pf[j] = src[i];
+        for (int j = 0; j < NPF; ++j) {
+            const int i = tid + 256 * j;
+            if (i < BCH) pf[j] = src[i];
+        }
+        // A operand digits of this chunk
+        v4i Ar[NS], Ai[NS];
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            unsigned tx[4], ty[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const unsigned rp = Rsh[kc * 64 + lg * 16 + 4 * w + b];
+                const int x = __mul24(c0, (int)(rp & 255u)) + __mul24(c1, (int)((rp >> 8) & 255u)) +
+                              __mul24(c2, (int)((rp >> 16) & 255u));
+                int m = x - __mul24(L, (int)((float)x * invL));  // x < 2^18: quotient off by at most 1
+                m = (m >= L) ? m - L : m;
+                m = (m < 0) ? m + L : m;
+                const uint2 t = Tsh[m];
+                tx[b] = t.x;
+                ty[b] = t.y;
             }
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int sd = 0; sd < NS; ++sd) {
-                    nAr[rb][sd] = arow[rb][((size_t)(kc + 1) * 2 * NS + 0 * NS + sd) * 4];
-                    nAi[rb][sd] = arow[rb][((size_t)(kc + 1) * 2 * NS + 1 * NS + sd) * 4];
-                }
+            unsigned o0, o1, o2, o3;
+            transpose4x4_bytes(tx[0], tx[1], tx[2], tx[3], o0, o1, o2, o3);
+            Ar[0][w] = (int)o0; Ar[1][w] = (int)o1; Ar[2][w] = (int)o2; Ar[3][w] = (int)o3;
+            transpose4x4_bytes(ty[0], ty[1], ty[2], ty[3], o0, o1, o2, o3);
+            Ai[0][w] = (int)o0; Ai[1][w] = (int)o1; Ai[2][w] = (int)o2; Ai[3][w] = (int)o3;
         }
         const v4i *B = (kc & 1) ? buf1 : buf0;
 #pragma unroll
-        for (int c = 0; c < CB; ++c) {
-            v4i Bre[NS], Bim[NS], Bnim[NS];
+        for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int sd = 0; sd < NS; ++sd) {
-                Bre[sd] = B[((c * 3 + 0) * NS + sd) * 64 + lane];
-                Bim[sd] = B[((c * 3 + 1) * NS + sd) * 64 + lane];
-                Bnim[sd] = B[((c * 3 + 2) * NS + sd) * 64 + lane];
+            for (int j = 0; j < NS; ++j) {
+                const v4i Bre = B[((c * 3 + 0) * NS + j) * 64 + lane];
+                const v4i Bim = B[((c * 3 + 1) * NS + j) * 64 + lane];
+                const v4i Bnim = B[((c * 3 + 2) * NS + j) * 64 + lane];
+#pragma unroll
+                for (int i = 0; i < NS - j; ++i) {
+                    const int l = i + j;
+                    acc[c][l][0] = mfma_i8(Ar[i], Bre, acc[c][l][0]);
+                    acc[c][l][1] = mfma_i8(Ar[i], Bim, acc[c][l][1]);
+                    acc[c][l][0] = mfma_i8(Ai[i], Bnim, acc[c][l][0]);
+                    acc[c][l][1] = mfma_i8(Ai[i], Bre, acc[c][l][1]);
+                }
             }
+        v4i *dst = (kc & 1) ? buf0 : buf1;
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS - i; ++j) {
-                        const int l = i + j;
-                        acc[rb][c][l][0] = mfma_i8(Ar[rb][i], Bre[j], acc[rb][c][l][0]);
-                        acc[rb][c][l][0] = mfma_i8(Ai[rb][i], Bnim[j], acc[rb][c][l][0]);
-                        acc[rb][c][l][1] = mfma_i8(Ar[rb][i], Bim[j], acc[rb][c][l][1]);
-                        acc[rb][c][l][1] = mfma_i8(Ai[rb][i], Bre[j], acc[rb][c][l][1]);
-                    }
-        }
-        if (more) {
-            v4i *dst = (kc & 1) ? buf0 : buf1;
-#pragma unroll
-            for (int j = 0; j < NPF; ++j) {
-                const int i = tid + 256 * j;
-                if (i < BCH) dst[i] = pf[j];
-            }
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-                for (int sd = 0; sd < NS; ++sd) { Ar[rb][sd] = nAr[rb][sd]; Ai[rb][sd] = nAi[rb][sd]; }
+        for (int j = 0; j < NPF; ++j) {
+            const int i = tid + 256 * j;
+            if (i < BCH) dst[i] = pf[j];
         }
         __syncthreads();
     }
@@ -621,25 +590,21 @@ __global__ __launch_bounds__(256) void k_pair_i8(PairArgs a, const v4i *__restri
     for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
     const double *sc = bscale + (size_t)st * CP;
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
+    for (int c = 0; c < CB; ++c) {
+        const double s = sc[c * 16 + li];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) {
-            const double s = sc[c * 16 + li];
+        for (int r = 0; r < 4; ++r) {
+            double re = 0.0, im = 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                double re = 0.0, im = 0.0;
-#pragma unroll
-                for (int l = NS - 1; l >= 0; --l) {
-                    re += wl[l] * (double)acc[rb][c][l][0][r];
-                    im += wl[l] * (double)acc[rb][c][l][1][r];
-                }
-                Y[(4 * lg + r) * CP + c * 16 + li] = make_double2(re * s, im * s);
+            for (int l = NS - 1; l >= 0; --l) {
+                re += wl[l] * (double)acc[c][l][0][r];
+                im += wl[l] * (double)acc[c][l][1][r];
             }
+            Y[(4 * lg + r) * CP + c * 16 + li] = make_double2(re * s, im * s);
         }
-        __syncthreads();
-        ibzk_epilogue_row(a, Y, CP, row_base + rb * 16 + li, li, lg, k0i, k1i, k2i, outrow);
-        __syncthreads();
     }
+    __syncthreads();
+    ibzk_epilogue_row(a, Y, CP, row_base + li, li, lg, k0i, k1i, k2i, outrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -1757,8 +1722,9 @@ struct elph_ibzk_plan {
     // mixed precision: digit table of the phases (all q of the electron mesh),
     // per-batch digits and column scales of B
     int64_t nkc = 0;
-    DBuf<signed char> Tsl;
-    DBuf<v4i> atab, bsl;
+    DBuf<uint2> Tdig;
+    DBuf<unsigned> Rpk;
+    DBuf<v4i> bsl;
     DBuf<double> bscale;
 };
 
@@ -1797,16 +1763,20 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
     if (p->precision == ELPH_MIXED) {
-        const size_t lds8 = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i);
-        const v4i *at = p->atab.p, *bs = p->bsl.p;
+        const size_t lds8 = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
+                            (size_t)p->nkc * 64 * sizeof(unsigned);
+        const uint2 *at = p->Tdig.p;
+        const unsigned *rpk = p->Rpk.p;
+        const v4i *bs = p->bsl.p;
+        grid.x = (unsigned)((p->nwv + 63) / 64);
         const double *sc = p->bscale.p;
         const int nkc = (int)p->nkc;
         HCHECK(hipEventRecord(c->ev_a, c->stream));
         switch (CB) {
-            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
-            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
-            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
-            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, c->stream, a, at, bs, sc, nkc); break;
+            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
+            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
+            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
+            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
             default: throw Error{ELPH_EINVAL, "mixed precision supports numwannbands*numbranches <= 64"};
         }
     } else {
@@ -1956,23 +1926,30 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         p->Rq.upload(R, s);
         if (precision == ELPH_MIXED) {
             p->nkc = p->Kp / 64;
-            std::vector<signed char> tsl(p->L * 2 * kNS);
+            std::vector<uint2> tdig(p->L);
             for (int64_t j = 0; j < p->L; ++j) {
                 signed char dr[kNS], di[kNS];
                 split_digits<kNS>(T[j].x, dr);
                 split_digits<kNS>(T[j].y, di);
+                unsigned lo = 0, hi = 0;
                 for (int sd = 0; sd < kNS; ++sd) {
-                    tsl[(j * 2 + 0) * kNS + sd] = dr[sd];
-                    tsl[(j * 2 + 1) * kNS + sd] = di[sd];
+                    lo |= (unsigned)(unsigned char)dr[sd] << (8 * sd);
+                    hi |= (unsigned)(unsigned char)di[sd] << (8 * sd);
                 }
+                tdig[j] = make_uint2(lo, hi);
             }
-            p->Tsl.upload(tsl, s);
-            const int64_t nq = p->Nel[0] * p->Nel[1] * p->Nel[2];
-            const int64_t tot = nq * p->nkc * 2 * kNS * 4;
-            p->atab.alloc(tot);
-            hipLaunchKernelGGL(k_build_atab, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nq, (int)p->nkc,
-                               (int)ctx->nwsg, (int)p->Nel[0], (int)p->Nel[1], (int)p->L, (int)p->Lf[0],
-                               (int)p->Lf[1], (int)p->Lf[2], p->Rq.p, p->Tsl.p, p->atab.p);
+            p->Tdig.upload(tdig, s);
+            // cells reduced modulo the electron mesh, packed one byte per component
+            std::vector<unsigned> rpk(p->Kp, 0u);
+            for (int64_t i = 0; i < ctx->nwsg; ++i) {
+                unsigned v = 0;
+                for (int d = 0; d < 3; ++d)
+                    v |= (unsigned)fmodulo(ctx->rcells_q[i + ctx->nwsq * d], p->Nel[d]) << (8 * d);
+                rpk[i] = v;
+            }
+            for (int d = 0; d < 3; ++d)
+                if (p->Nel[d] > 255 || p->L > 255) throw Error{ELPH_EINVAL, "mixed precision needs meshes up to 255"};
+            p->Rpk.upload(rpk, s);
             HCHECK(hipGetLastError());
         }
         // IBZ data
@@ -2029,7 +2006,7 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
             act.push_back({ik - 1, m - 1, t});
         }
         const int64_t nb2 = nb * nb, nslices = nbr * c->nwsg;
-        const int64_t KB = 32;  // IBZ k per gkRp batch
+        const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
         size_t i0 = 0;
         while (i0 < act.size()) {
             // collect up to KB distinct k
