@@ -242,7 +242,7 @@ struct PairArgs {
 //   g = sum_sp u_q(s,sp) sum_np conj(U_k'(n,np)) Y(np + nb sp),
 //   |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu   (0 when omega == 0),
 // written at the reference's position (k' block, rank of n, s).
-__device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int CP, int row, int yi, int lk,
+__device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int ys, int row, int yi, int lk,
                                                   int k0i, int k1i, int k2i, double *outrow) {
     if (row >= a.nwv) return;
     const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
@@ -260,7 +260,7 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
         omq = a.omf + (size_t)iq * a.nbr;
     }
     const int nn = a.nwin[row];
-    const double2 *Yr = Y + yi * CP;
+    const double2 *Yr = Y + yi * ys;
     for (int rank = lk; rank < nn; rank += 4) {
         const int n = a.wband[row * a.nb + rank];
         const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
@@ -401,17 +401,19 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
     d4 cr[RB][CB], ci[RB][CB];
     phase_gemm_core<CB, RB, KC>(hst, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
 
-    // ---- epilogue: per row block, stage Y (16 x CP) of this wave in LDS ----
-    double2 *Y = smem + wave * 16 * CP;
+    // ---- epilogue: per row block, stage Y (16 x CP) of this wave in LDS; the
+    // odd row stride YS keeps the 16 rows on distinct LDS banks ----
+    const int YS = CP + 1;
+    double2 *Y = smem + wave * 16 * YS;
     double *outrow = a.out + a.st_slot[st] * a.nprocs;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * CP + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
+            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * YS + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
         __syncthreads();
-        ibzk_epilogue_row(a, Y, CP, row_base + rb * 16 + li, li, lk, k0i, k1i, k2i, outrow);
+        ibzk_epilogue_row(a, Y, YS, row_base + rb * 16 + li, li, lk, k0i, k1i, k2i, outrow);
         __syncthreads();
     }
 }
@@ -524,23 +526,14 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
     const v4i *bst = bsl + (size_t)st * nkc * BCH;
     for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
     __syncthreads();
-    for (int kc = 0; kc < nkc; ++kc) {
-        // prefetch the next B chunk (the last chunk re-reads itself: no branch)
-        const v4i *src = bst + (size_t)min(kc + 1, nkc - 1) * BCH;
-        v4i pf[NPF];
-#pragma unroll
-        for (int j = 0; j < NPF; ++j) {
-            const int i = tid + 256 * j;
-            if (i < BCH) pf[j] = src[i];
-        }
-        // A operand digits of this chunk
-        v4i Ar[NS], Ai[NS];
+    // digits of the phases of K chunk kk into (Ar, Ai)
+    auto make_a = [&](int kk, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
             unsigned tx[4], ty[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const unsigned rp = Rsh[kc * 64 + lg * 16 + 4 * w + b];
+                const unsigned rp = Rsh[kk * 64 + lg * 16 + 4 * w + b];
                 const int x = __mul24(c0, (int)(rp & 255u)) + __mul24(c1, (int)((rp >> 8) & 255u)) +
                               __mul24(c2, (int)((rp >> 16) & 255u));
                 int m = x - __mul24(L, (int)((float)x * invL));  // x < 2^18: quotient off by at most 1
@@ -555,6 +548,19 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
             Ar[0][w] = (int)o0; Ar[1][w] = (int)o1; Ar[2][w] = (int)o2; Ar[3][w] = (int)o3;
             transpose4x4_bytes(ty[0], ty[1], ty[2], ty[3], o0, o1, o2, o3);
             Ai[0][w] = (int)o0; Ai[1][w] = (int)o1; Ai[2][w] = (int)o2; Ai[3][w] = (int)o3;
+        }
+    };
+    v4i Ar[NS], Ai[NS];
+    make_a(0, Ar, Ai);
+    for (int kc = 0; kc < nkc; ++kc) {
+        // prefetch the next B chunk (the last chunk re-reads itself: no branch)
+        const int kn = min(kc + 1, nkc - 1);
+        const v4i *src = bst + (size_t)kn * BCH;
+        v4i pf[NPF];
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            const int i = tid + 256 * j;
+            if (i < BCH) pf[j] = src[i];
         }
         const v4i *B = (kc & 1) ? buf1 : buf0;
 #pragma unroll
@@ -573,6 +579,11 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
                     acc[c][l][1] = mfma_i8(Ai[i], Bre, acc[c][l][1]);
                 }
             }
+        // next chunk's phase digits (independent of the MFMAs above)
+        v4i nAr[NS], nAi[NS];
+        make_a(kn, nAr, nAi);
+#pragma unroll
+        for (int i = 0; i < NS; ++i) { Ar[i] = nAr[i]; Ai[i] = nAi[i]; }
         v4i *dst = (kc & 1) ? buf0 : buf1;
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
@@ -582,7 +593,8 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
         __syncthreads();
     }
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
-    double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * CP;
+    constexpr int YS = CP + 1;
+    double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * YS;
     double *outrow = a.out + a.st_slot[st] * a.nprocs;
     double wl[NS];
     wl[0] = 1.0 / (127.0 * 127.0);
@@ -600,11 +612,11 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
                 re += wl[l] * (double)acc[c][l][0][r];
                 im += wl[l] * (double)acc[c][l][1][r];
             }
-            Y[(4 * lg + r) * CP + c * 16 + li] = make_double2(re * s, im * s);
+            Y[(4 * lg + r) * YS + c * 16 + li] = make_double2(re * s, im * s);
         }
     }
     __syncthreads();
-    ibzk_epilogue_row(a, Y, CP, row_base + li, li, lg, k0i, k1i, k2i, outrow);
+    ibzk_epilogue_row(a, Y, YS, row_base + li, li, lg, k0i, k1i, k2i, outrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -772,7 +784,8 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     d4 cr[RB][CB], ci[RB][CB];
     phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * CP, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
 
-    double2 *Y = smem + wave * 16 * CP;
+    const int YS = CP + 1;
+    double2 *Y = smem + wave * 16 * YS;
     double *outrow = a.out + a.st_slot[st] * a.stride;
     const double w = a.st_om[st];
     const int nb = a.nb;
@@ -781,7 +794,7 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
 #pragma unroll
         for (int c = 0; c < CB; ++c)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * CP + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
+            for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * YS + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
         __syncthreads();
         const int row = row_base + rb * 16 + li;
         if (row < a.nwv) {
@@ -789,7 +802,7 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
             if (kp >= 0) {
                 const int nm = a.nwin[row], nn = a.nwin[kp];
                 const int64_t base = a.rowoff[(size_t)st * a.nwv + row];
-                const double2 *Yr = Y + li * CP;
+                const double2 *Yr = Y + li * YS;
                 for (int t = lk; t < nm * nn; t += 4) {
                     const int mr = t / nn, nr = t % nn;
                     const int m = a.wband[row * nb + mr], n = a.wband[kp * nb + nr];
@@ -1760,11 +1773,13 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     a.out = out;
     a.nprocs = nprocs;
     const int CB = (int)(p->CP / 16);
-    const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+    const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
+                                (size_t)64 * (p->CP + 1) * sizeof(double2));
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
     if (p->precision == ELPH_MIXED) {
-        const size_t lds8 = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
-                            (size_t)p->nkc * 64 * sizeof(unsigned);
+        const size_t lds8 = std::max((size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) +
+                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 64 * sizeof(unsigned),
+                                     (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;
         const v4i *bs = p->bsl.p;
@@ -2354,7 +2369,8 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             a.out = dst;
             a.stride = p->stride;
             const int CB = (int)(p->CP / 16);
-            const size_t lds = (size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+            const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
+                                        (size_t)64 * (p->CP + 1) * sizeof(double2));
             dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst);
             HCHECK(hipEventRecord(c->ev_a, s));
             switch (CB) {
