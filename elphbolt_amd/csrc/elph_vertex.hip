@@ -563,22 +563,33 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
             if (i < BCH) pf[j] = src[i];
         }
         const v4i *B = (kc & 1) ? buf1 : buf0;
+        // per B digit j: first the Re(A) products of every (level, column
+        // block), then the Im(A) products, so two updates of one accumulator
+        // are at least 2*CB matrix instructions apart
 #pragma unroll
-        for (int c = 0; c < CB; ++c)
+        for (int j = 0; j < NS; ++j) {
+            v4i Bre[CB], Bim[CB], Bnim[CB];
 #pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                const v4i Bre = B[((c * 3 + 0) * NS + j) * 64 + lane];
-                const v4i Bim = B[((c * 3 + 1) * NS + j) * 64 + lane];
-                const v4i Bnim = B[((c * 3 + 2) * NS + j) * 64 + lane];
-#pragma unroll
-                for (int i = 0; i < NS - j; ++i) {
-                    const int l = i + j;
-                    acc[c][l][0] = mfma_i8(Ar[i], Bre, acc[c][l][0]);
-                    acc[c][l][1] = mfma_i8(Ar[i], Bim, acc[c][l][1]);
-                    acc[c][l][0] = mfma_i8(Ai[i], Bnim, acc[c][l][0]);
-                    acc[c][l][1] = mfma_i8(Ai[i], Bre, acc[c][l][1]);
-                }
+            for (int c = 0; c < CB; ++c) {
+                Bre[c] = B[((c * 3 + 0) * NS + j) * 64 + lane];
+                Bim[c] = B[((c * 3 + 1) * NS + j) * 64 + lane];
+                Bnim[c] = B[((c * 3 + 2) * NS + j) * 64 + lane];
             }
+#pragma unroll
+            for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
+                }
+#pragma unroll
+            for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
+                }
+        }
         // next chunk's phase digits (independent of the MFMAs above)
         v4i nAr[NS], nAi[NS];
         make_a(kn, nAr, nAi);
