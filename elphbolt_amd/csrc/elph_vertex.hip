@@ -96,6 +96,11 @@ struct elph_ctx {
     DBuf<int> rk, rq, rg;  // [cell][3]
     DBuf<int> degk, degq, degg;
     DBuf<double> rcart_k, rcart_q;  // [cell][3] Cartesian (nm)
+    // int8 digits of gwann for the mixed-precision Re contraction (built on
+    // first use): [slice][row block][K chunk][re/im][digit][lane]
+    DBuf<v4i> gdig;
+    DBuf<double> gscale;
+    int64_t gnrb = 0, gnkc = 0;
     // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
     DBuf<int> term_off, term_iuc, term_T, term_div;
     std::string err;
@@ -628,6 +633,173 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
     }
     __syncthreads();
     ibzk_epilogue_row(a, Y, YS, row_base + li, li, lg, k0i, k1i, k2i, outrow);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision Re contraction (gkRp): gwann split once per context into
+// kNS int8 digits per (row i = (np,mp), slice = (sp,Rp)) with a per-row scale,
+// phases split per batch; 10 digit products on the int8 matrix cores with
+// exact int32 accumulation, recombined in FP64 into the FP64 g(k,Rp) layout.
+// ---------------------------------------------------------------------------
+
+// gscale[slice][i] = max over Re of max(|Re g|, |Im g|) (1 for zero rows).
+__global__ void k_gwann_scale(int nb2, int nwsk, int nslices, int nrb, const double2 *__restrict__ gw,
+                              double *__restrict__ gscale) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nslices * nrb * 16) return;
+    const int i = (int)(t % (nrb * 16));
+    const int64_t slice = t / (nrb * 16);
+    double m = 0.0;
+    if (i < nb2) {
+        const double2 *p = gw + (size_t)slice * nb2 * nwsk + i;
+        for (int r = 0; r < nwsk; ++r) {
+            const double2 v = p[(size_t)r * nb2];
+            m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+        }
+    }
+    gscale[t] = m > 0.0 ? m : 1.0;
+}
+
+// Gdig[slice][rb][kc][p][s][lane]: digits of gwann / scale in the int8 MFMA
+// A-operand order (lane i + 16 g holds rows rb*16 + i, K = 64 kc + 16 g .. +15).
+__global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc, const double2 *__restrict__ gw,
+                               const double *__restrict__ gscale, v4i *__restrict__ gdig) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nslices * nrb * nkc * 64;
+    if (t >= total) return;
+    const int lane = (int)(t % 64);
+    int64_t r = t / 64;
+    const int kc = (int)(r % nkc);
+    r /= nkc;
+    const int rb = (int)(r % nrb);
+    const int64_t slice = r / nrb;
+    const int i = rb * 16 + (lane & 15), g = lane >> 4;
+    const double inv = 1.0 / gscale[slice * nrb * 16 + i];
+    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS];
+    for (int b = 0; b < 16; ++b) {
+        const int k = kc * 64 + g * 16 + b;
+        double2 x = make_double2(0.0, 0.0);
+        if (i < nb2 && k < nwsk) x = gw[((size_t)slice * nwsk + k) * nb2 + i];
+        signed char dr[kNS], di[kNS];
+        split_digits<kNS>(x.x * inv, dr);
+        split_digits<kNS>(x.y * inv, di);
+#pragma unroll
+        for (int sd = 0; sd < kNS; ++sd) { ure[sd].b[b] = dr[sd]; uim[sd].b[b] = di[sd]; }
+    }
+    v4i *o = gdig + ((((size_t)slice * nrb + rb) * nkc + kc) * 2) * kNS * 64 + lane;
+#pragma unroll
+    for (int sd = 0; sd < kNS; ++sd) {
+        o[(0 * kNS + sd) * 64] = ure[sd].v;
+        o[(1 * kNS + sd) * 64] = uim[sd].v;
+    }
+}
+
+// Bphi[kc][cb][p][s][lane], p = Re, Im, -Im: digits of the phases phiT[Re][col]
+// (|phi| <= 1, no scale) in the int8 MFMA B-operand order.
+__global__ void k_phase_digits(int nwsk, int nkc, int ncb, const double2 *__restrict__ phiT, int ldphi,
+                               v4i *__restrict__ bphi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nkc * ncb * 64) return;
+    const int lane = t % 64, cb = (t / 64) % ncb, kc = t / (64 * ncb);
+    const int col = cb * 16 + (lane & 15), g = lane >> 4;
+    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS], unim[kNS];
+    for (int b = 0; b < 16; ++b) {
+        const int k = kc * 64 + g * 16 + b;
+        double2 x = make_double2(0.0, 0.0);
+        if (k < nwsk) x = phiT[(size_t)k * ldphi + col];
+        signed char dr[kNS], di[kNS];
+        split_digits<kNS>(x.x, dr);
+        split_digits<kNS>(x.y, di);
+#pragma unroll
+        for (int sd = 0; sd < kNS; ++sd) {
+            ure[sd].b[b] = dr[sd];
+            uim[sd].b[b] = di[sd];
+            unim[sd].b[b] = (signed char)(-di[sd]);
+        }
+    }
+    v4i *o = bphi + (((size_t)kc * ncb + cb) * 3) * kNS * 64 + lane;
+#pragma unroll
+    for (int sd = 0; sd < kNS; ++sd) {
+        o[(0 * kNS + sd) * 64] = ure[sd].v;
+        o[(1 * kNS + sd) * 64] = uim[sd].v;
+        o[(2 * kNS + sd) * 64] = unim[sd].v;
+    }
+}
+
+// One workgroup per (sp,Rp) slice; wave w takes row blocks w, w+4, ...; all
+// NCB column blocks (16 k each).  out[k][slice][i] FP64 complex.
+template <int NCB>
+__global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig, const double *__restrict__ gscale,
+                                                    int nb2, int nrb, int nkc, int nslices,
+                                                    const v4i *__restrict__ bphi, int nk, double2 *__restrict__ out,
+                                                    int64_t out_kstride) {
+    constexpr int NS = kNS;
+    const int slice = blockIdx.x;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    for (int rb = wave; rb < nrb; rb += 4) {
+        v4i acc[NCB][NS][2];
+#pragma unroll
+        for (int c = 0; c < NCB; ++c)
+#pragma unroll
+            for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
+        const v4i *ga = gdig + (((size_t)slice * nrb + rb) * nkc) * 2 * NS * 64 + lane;
+        for (int kc = 0; kc < nkc; ++kc) {
+            v4i Ar[NS], Ai[NS];
+#pragma unroll
+            for (int sd = 0; sd < NS; ++sd) {
+                Ar[sd] = ga[((size_t)kc * 2 * NS + 0 * NS + sd) * 64];
+                Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
+            }
+            const v4i *bb = bphi + ((size_t)kc * NCB * 3) * NS * 64 + lane;
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                v4i Bre[NCB], Bim[NCB], Bnim[NCB];
+#pragma unroll
+                for (int c = 0; c < NCB; ++c) {
+                    Bre[c] = bb[((c * 3 + 0) * NS + j) * 64];
+                    Bim[c] = bb[((c * 3 + 1) * NS + j) * 64];
+                    Bnim[c] = bb[((c * 3 + 2) * NS + j) * 64];
+                }
+#pragma unroll
+                for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                    for (int c = 0; c < NCB; ++c) {
+                        acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
+                        acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
+                    }
+#pragma unroll
+                for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                    for (int c = 0; c < NCB; ++c) {
+                        acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
+                        acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
+                    }
+            }
+        }
+        double wl[NS];
+        wl[0] = 1.0 / (127.0 * 127.0);
+#pragma unroll
+        for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            const int col = c * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = rb * 16 + 4 * lg + r;
+                if (col < nk && i < nb2) {
+                    double re = 0.0, im = 0.0;
+#pragma unroll
+                    for (int l = NS - 1; l >= 0; --l) {
+                        re += wl[l] * (double)acc[c][l][0][r];
+                        im += wl[l] * (double)acc[c][l][1][r];
+                    }
+                    const double sc = gscale[(size_t)slice * nrb * 16 + i];
+                    out[(size_t)col * out_kstride + (size_t)slice * nb2 + i] = make_double2(re * sc, im * sc);
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1554,25 +1726,64 @@ int elph_ph_wann(elph_ctx *ctx, int64_t nq, const double *qvecs, double *energie
     return status;
 }
 
-// Batched gkRp into device memory: out_dev[k] (nb*nb, nbr*nwsg) for kvecs on device.
-static void gkRp_device(elph_ctx *c, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride) {
+// int8 digits of gwann (once per context).
+static void ensure_gwann_digits(elph_ctx *c) {
+    if (c->gdig.p) return;
+    const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
+    c->gnrb = (nb2 + 15) / 16;
+    c->gnkc = (c->nwsk + 63) / 64;
+    c->gscale.alloc(nslices * c->gnrb * 16);
+    const int64_t ns = nslices * c->gnrb * 16;
+    hipLaunchKernelGGL(k_gwann_scale, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c->stream, (int)nb2,
+                       (int)c->nwsk, (int)nslices, (int)c->gnrb, c->gwann.p, c->gscale.p);
+    const int64_t nt = nslices * c->gnrb * c->gnkc * 64;
+    c->gdig.alloc(nt * 2 * kNS);
+    hipLaunchKernelGGL(k_gwann_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, (int)nb2,
+                       (int)c->nwsk, (int)nslices, (int)c->gnrb, (int)c->gnkc, c->gwann.p, c->gscale.p, c->gdig.p);
+    HCHECK(hipGetLastError());
+}
+
+// Batched gkRp into device memory: out[k] (nb*nb, nbr*nwsg) for kvecs (nk,3)
+// column-major on the device; FP64 or int8-digit (mixed) contraction.
+// phiT (nwsk*64) and, for mixed, bphi are scratch buffers of the caller.
+static void gkRp_launch(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
+                        int64_t out_kstride, bool mixed, double2 *phiT, v4i *bphi) {
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
     const int64_t KT = 64;
-    DBuf<double2> phiT;
-    phiT.alloc(c->nwsk * KT);
     for (int64_t k0 = 0; k0 < nk; k0 += KT) {
         const int64_t n = std::min(KT, nk - k0);
         // phases in [Re][k] layout (64 columns, unused columns zero)
-        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsk * KT + 255) / 256)), dim3(256), 0, c->stream, phiT.p,
-                           c->nwsk * KT);
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsk * KT + 255) / 256)), dim3(256), 0, st, phiT, c->nwsk * KT);
         dim3 grid((unsigned)((c->nwsk + 127) / 128), (unsigned)n);
-        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, c->stream, (int)n, (int)c->nwsk, d_kvecs + k0, nk,
-                           c->rk.p, c->degk.p, phiT.p, (int64_t)1, KT);
-        hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), 0, c->stream, c->gwann.p, (int)nb2,
-                           (int)c->nwsk, (int)nslices, phiT.p, (int)KT, (int)n, out + (size_t)k0 * out_kstride,
-                           out_kstride);
+        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, st, (int)n, (int)c->nwsk, d_kvecs + k0, nk, c->rk.p,
+                           c->degk.p, phiT, (int64_t)1, KT);
+        if (mixed) {
+            const int nt = (int)(c->gnkc * (KT / 16) * 64);
+            hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->nwsk,
+                               (int)c->gnkc, (int)(KT / 16), phiT, (int)KT, bphi);
+            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)nslices), dim3(256), 0, st, c->gdig.p, c->gscale.p,
+                               (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
+                               out + (size_t)k0 * out_kstride, out_kstride);
+        } else {
+            hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), 0, st, c->gwann.p, (int)nb2,
+                               (int)c->nwsk, (int)nslices, phiT, (int)KT, (int)n, out + (size_t)k0 * out_kstride,
+                               out_kstride);
+        }
         HCHECK(hipGetLastError());
     }
+}
+
+static void gkRp_device(elph_ctx *c, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride,
+                        bool mixed = false) {
+    DBuf<double2> phiT;
+    DBuf<v4i> bphi;
+    phiT.alloc(c->nwsk * 64);
+    if (mixed) {
+        ensure_gwann_digits(c);
+        bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
+    }
+    gkRp_launch(c, c->stream, nk, d_kvecs, out, out_kstride, mixed, phiT.p, bphi.p);
+    HCHECK(hipStreamSynchronize(c->stream));
 }
 
 int elph_gkRp(elph_ctx *ctx, int64_t nk, const double *kvecs, double *out) {
@@ -2056,7 +2267,7 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
             p->kv.upload(kv, s);
             p->Ukb.upload(Ukb, s);
             p->gk.alloc(nkb * nb2 * nslices);
-            gkRp_device(c, nkb, p->kv.p, p->gk.p, nb2 * nslices);
+            gkRp_device(c, nkb, p->kv.p, p->gk.p, nb2 * nslices, p->precision == ELPH_MIXED);
             std::vector<int> skb(nst), sm(nst), ski(3 * nst);
             std::vector<int64_t> sslot(nst);
             int64_t bidx = -1, prev = -1;
