@@ -103,6 +103,12 @@ struct elph_ctx {
     int64_t gnrb = 0, gnkc = 0;
     // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
     DBuf<int> term_off, term_iuc, term_T, term_div;
+    // polar crystals: reciprocal vectors of the (2N+1)^3 box (1/bohr), Born
+    // charges, atom positions (bohr) and the q-independent part of D_L
+    DBuf<double> pol_G, pol_Z, pol_tau;
+    DBuf<double2> pol_D0;
+    int64_t pol_nG = 0;
+    double pol_alpha = 0, pol_fac = 0;
     std::string err;
     double t_dominant_ms = 0, t_all_ms = 0;
     int64_t n_dominant = 0;
@@ -237,6 +243,11 @@ struct PairArgs {
     const double *om;          // [nq][s] eV
     const double2 *uf;         // fine-mesh phonons (mesh_ref > 1) [nq_el][s][sp]
     const double *omf;
+    // polar crystals: long-range prefactor per mode (null otherwise), [nq][s] and
+    // [nq_el][s], and the IBZ eigenvector U_k(m,:) of each state
+    const double2 *lp, *lpf;
+    const double2 *Ukb;        // [k of batch][m][mp]
+    const int *st_kb, *st_m;
     int nb, nbr;
     double *out;
     int64_t nprocs;
@@ -246,30 +257,42 @@ struct PairArgs {
 // tile): for every in-window final band n (ranks lk, lk+4, ...) and branch s,
 //   g = sum_sp u_q(s,sp) sum_np conj(U_k'(n,np)) Y(np + nb sp),
 //   |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu   (0 when omega == 0),
-// written at the reference's position (k' block, rank of n, s).
+// written at the reference's position (k' block, rank of n, s).  Polar
+// crystals add lp(q,s) * sum_i conj(U_k'(n,i)) U_k(m,i) to g first
+// (src/wannier.f90:998-1006).
 __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int ys, int row, int yi, int lk,
-                                                  int k0i, int k1i, int k2i, double *outrow) {
+                                                  int st, int k0i, int k1i, int k2i, double *outrow) {
     if (row >= a.nwv) return;
     const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
     const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
     const int q2 = pos_mod(a.kp_int[3 * row + 2] - k2i, a.Nel2);
-    const double2 *uq;
+    const double2 *uq, *lq = nullptr;
     const double *omq;
     if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
         const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
         uq = a.u + (size_t)iq * a.nbr * a.nbr;
         omq = a.om + (size_t)iq * a.nbr;
+        if (a.lp) lq = a.lp + (size_t)iq * a.nbr;
     } else {
         const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
         uq = a.uf + (size_t)iq * a.nbr * a.nbr;
         omq = a.omf + (size_t)iq * a.nbr;
+        if (a.lpf) lq = a.lpf + (size_t)iq * a.nbr;
     }
+    const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb : nullptr;
     const int nn = a.nwin[row];
     const double2 *Yr = Y + yi * ys;
     for (int rank = lk; rank < nn; rank += 4) {
         const int n = a.wband[row * a.nb + rank];
         const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
         double *o = outrow + a.offs[row] + (int64_t)rank * a.nbr;
+        double2 unm = make_double2(0.0, 0.0);
+        if (lq)
+            for (int i = 0; i < a.nb; ++i) {
+                const double2 z = cmulc(V[i], Ukm[i]);
+                unm.x += z.x;
+                unm.y += z.y;
+            }
         for (int s0 = 0; s0 < a.nbr; s0 += 6) {
             double2 gb[6];
 #pragma unroll
@@ -288,6 +311,10 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                 for (int j = 0; j < 6; ++j)
                     if (s0 + j < a.nbr) cfma(gb[j], uq[(s0 + j) * a.nbr + sp], z);
             }
+            if (lq)
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+                    if (s0 + j < a.nbr) cfma(gb[j], lq[s0 + j], unm);
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
                 const int sb = s0 + j;
@@ -418,7 +445,7 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * YS + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
         __syncthreads();
-        ibzk_epilogue_row(a, Y, YS, row_base + rb * 16 + li, li, lk, k0i, k1i, k2i, outrow);
+        ibzk_epilogue_row(a, Y, YS, row_base + rb * 16 + li, li, lk, st, k0i, k1i, k2i, outrow);
         __syncthreads();
     }
 }
@@ -632,7 +659,7 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
         }
     }
     __syncthreads();
-    ibzk_epilogue_row(a, Y, YS, row_base + li, li, lg, k0i, k1i, k2i, outrow);
+    ibzk_epilogue_row(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -931,6 +958,7 @@ struct PairQArgs {
     const int *rowmap;         // [st][nwv]
     const int64_t *rowoff;     // [st][nwv]
     const double *st_om;       // [st] phonon energy (eV)
+    const double2 *st_lp;      // [st] polar long-range prefactor, or null
     const int64_t *st_slot;
     int nb;
     double *out;
@@ -1003,6 +1031,15 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
                             z.y = fma(-v.y, y.x, z.y);
                         }
                         cfma(g, Uk[mp], z);
+                    }
+                    if (a.st_lp) {  // src/wannier.f90:998-1006
+                        double2 unm = make_double2(0.0, 0.0);
+                        for (int i = 0; i < nb; ++i) {
+                            const double2 z = cmulc(Ukp[i], Uk[i]);
+                            unm.x += z.x;
+                            unm.y += z.y;
+                        }
+                        cfma(g, a.st_lp[st], unm);
                     }
                     double g2 = 0.0;
                     if (w != 0.0) g2 = 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
@@ -1091,6 +1128,166 @@ __global__ void k_build_D(int nq, int nbr, int nat, const double *__restrict__ q
         dD[((size_t)q * 3 + 1) * nbr2 + ij] = d1;
         dD[((size_t)q * 3 + 2) * nbr2 + ij] = d2;
     }
+}
+
+// Polar (Froehlich) long-range terms at a list of q, src/wannier.f90:784-901
+// (dyn_nonanalytic) and 1013-1071 (long_range_prefac).  With g = G + q in
+// 1/bohr, G over the (2N+1)^3 box of reciprocal vectors (N = coarse q mesh),
+// kept where g.eps.g > 0 and g.eps.g / (4 alpha) < gmax, f(g) =
+// exp(-g.eps.g / (4 alpha)) / g.eps.g and zg_a = g.Z_a:
+//   D_L(3a+i, 3b+j) = D0 + fac sum_g f e^{i g.(tau_a - tau_b)} zg_a(i) zg_b(j)
+//   dD_L(., ., c)   = dscale fac sum_g f e^{i g.(tau_a - tau_b)} [zg_b(j) Z_a(c,i)
+//                     + zg_a(i) Z_b(c,j) + zg_a(i) zg_b(j) (i (tau_a - tau_b)_c
+//                     - (1/(4 alpha) + 1/g.eps.g) ((eps + eps^T) g)_c)]
+//   Lq(3a+i)        = i fac sum_g f e^{-i g.tau_a} zg_a(i)
+// D and dD are accumulated into (the short-range D(q) is already there); D0 is
+// the q-independent self term (the G-only sum of dyn_nonanalytic).
+constexpr int kPolMaxAt = 10, kPolChunk = 64;
+
+struct PolarArgs {
+    int nq;
+    const double *qv;     // (nq,3) column-major, crystal coordinates
+    int nat, nbr, nG;
+    const double *G;      // [nG][3] (1/bohr)
+    const double *Z;      // Born charges (3,3,nat), Fortran order: Z[c + 3 i + 9 a]
+    const double *tau;    // [nat][3] (bohr)
+    double eps[9], rec[9];  // column-major; rec in 1/bohr, 2 pi included
+    double alpha, fac, gmax, dscale;
+    const double2 *D0;    // [nbr^2]
+    double2 *D;           // [q][nbr^2] or null
+    double2 *dD;          // [q][3][nbr^2] or null
+    double2 *Lq;          // [q][nbr] or null
+};
+
+__global__ __launch_bounds__(256) void k_polar(PolarArgs a) {
+    const int q = blockIdx.x;
+    if (q >= a.nq) return;
+    __shared__ double sf[kPolChunk], sdc[kPolChunk], sdg[kPolChunk][3];
+    __shared__ double2 sph[kPolChunk][kPolMaxAt];
+    __shared__ double sz[kPolChunk][kPolMaxAt][3];
+    __shared__ double sZ[9 * kPolMaxAt], stau[3 * kPolMaxAt];
+    const int t = threadIdx.x, nbr = a.nbr, nbr2 = nbr * nbr, nat = a.nat;
+    for (int i = t; i < 9 * nat; i += 256) sZ[i] = a.Z[i];
+    for (int i = t; i < 3 * nat; i += 256) stau[i] = a.tau[i];
+    const double q0 = a.qv[q], q1 = a.qv[q + a.nq], q2 = a.qv[q + 2 * a.nq];
+    double qc[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) qc[r] = a.rec[r] * q0 + a.rec[r + 3] * q1 + a.rec[r + 6] * q2;
+    const bool wantd = a.dD != nullptr;
+    double2 accD[4], accd[4][3], accL = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int e4 = 0; e4 < 4; ++e4) {
+        accD[e4] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) accd[e4][c] = make_double2(0.0, 0.0);
+    }
+    for (int g0 = 0; g0 < a.nG; g0 += kPolChunk) {
+        __syncthreads();
+        if (t < kPolChunk) {
+            const int gi = g0 + t;
+            double f = 0.0, g[3] = {0.0, 0.0, 0.0};
+            if (gi < a.nG) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) g[r] = a.G[3 * gi + r] + qc[r];
+                double geg = 0.0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) geg += g[r] * (a.eps[r] * g[0] + a.eps[r + 3] * g[1] + a.eps[r + 6] * g[2]);
+                if (geg > 0.0 && geg / a.alpha / 4.0 < a.gmax) {
+                    f = exp(-geg / a.alpha / 4.0) / geg;
+                    sdc[t] = 0.25 / a.alpha + 1.0 / geg;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+                        sdg[t][r] = (a.eps[r] + a.eps[3 * r]) * g[0] + (a.eps[r + 3] + a.eps[3 * r + 1]) * g[1] +
+                                    (a.eps[r + 6] + a.eps[3 * r + 2]) * g[2];
+                }
+            }
+            sf[t] = f;
+            for (int at = 0; at < nat; ++at) {
+                double s = 0.0, c = 0.0;
+                if (f != 0.0) sincos(g[0] * stau[3 * at] + g[1] * stau[3 * at + 1] + g[2] * stau[3 * at + 2], &s, &c);
+                sph[t][at] = make_double2(c, s);
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    sz[t][at][i] = g[0] * sZ[9 * at + 3 * i] + g[1] * sZ[9 * at + 3 * i + 1] + g[2] * sZ[9 * at + 3 * i + 2];
+            }
+        }
+        __syncthreads();
+        const int cnt = min(kPolChunk, a.nG - g0);
+        if (a.D) {
+#pragma unroll
+            for (int e4 = 0; e4 < 4; ++e4) {
+                const int e = t + 256 * e4;
+                if (e < nbr2) {
+                    const int ii = e % nbr, jj = e / nbr;
+                    const int ia = ii / 3, ip = ii % 3, ja = jj / 3, jp = jj % 3;
+                    double rr[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) rr[c] = stau[3 * ia + c] - stau[3 * ja + c];
+                    for (int k = 0; k < cnt; ++k) {
+                        const double f = sf[k];
+                        if (f == 0.0) continue;
+                        const double2 ph = cmulc(sph[k][ja], sph[k][ia]);  // e^{i g.(tau_a - tau_b)}
+                        const double2 fq = make_double2(f * ph.x, f * ph.y);
+                        const double zi = sz[k][ia][ip], zj = sz[k][ja][jp];
+                        accD[e4].x = fma(fq.x, zi * zj, accD[e4].x);
+                        accD[e4].y = fma(fq.y, zi * zj, accD[e4].y);
+                        if (wantd) {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                // zj Z_a(c,i) + zi Z_b(c,j) + zi zj (i rr_c - dcoef dgeg_c)
+                                const double re = zj * sZ[9 * ia + 3 * ip + c] + zi * sZ[9 * ja + 3 * jp + c] -
+                                                  zi * zj * sdc[k] * sdg[k][c];
+                                const double im = zi * zj * rr[c];
+                                cfma(accd[e4][c], fq, make_double2(re, im));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (a.Lq && t < nbr) {
+            const int at = t / 3, ip = t % 3;
+            for (int k = 0; k < cnt; ++k) {
+                const double f = sf[k];
+                if (f == 0.0) continue;
+                const double2 ph = sph[k][at];
+                const double w = f * sz[k][at][ip];
+                accL.x = fma(w, ph.x, accL.x);   // f zg e^{-i g.tau}
+                accL.y = fma(-w, ph.y, accL.y);
+            }
+        }
+    }
+    if (a.D) {
+#pragma unroll
+        for (int e4 = 0; e4 < 4; ++e4) {
+            const int e = t + 256 * e4;
+            if (e < nbr2) {
+                double2 &d = a.D[(size_t)q * nbr2 + e];
+                d.x += a.fac * accD[e4].x + a.D0[e].x;
+                d.y += a.fac * accD[e4].y + a.D0[e].y;
+                if (wantd)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double2 &dd = a.dD[((size_t)q * 3 + c) * nbr2 + e];
+                        dd.x += a.dscale * a.fac * accd[e4][c].x;
+                        dd.y += a.dscale * a.fac * accd[e4][c].y;
+                    }
+            }
+        }
+    }
+    if (a.Lq && t < nbr) a.Lq[(size_t)q * nbr + t] = make_double2(-a.fac * accL.y, a.fac * accL.x);  // i fac acc
+}
+
+// lp[i] = sum_p Lq[r][p] u[i][p] with r = ridx[i] (or i / div): the long-range
+// vertex prefactor of one phonon mode, u = e / sqrt(M) (src/wannier.f90:1002-1003).
+__global__ void k_polar_lp(int n, int nbr, int div, const int *__restrict__ ridx, const double2 *__restrict__ Lq,
+                           const double2 *__restrict__ u, double2 *__restrict__ lp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = ridx ? ridx[i] : i / div;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = 0; p < nbr; ++p) cfma(acc, Lq[(size_t)r * nbr + p], u[(size_t)i * nbr + p]);
+    lp[i] = acc;
 }
 
 // Batched Hermitian eigensolver, one wavefront per matrix: parallel cyclic
@@ -1268,7 +1465,7 @@ __global__ __launch_bounds__(256) void k_g2_list(int n, int nb, int nbr, int nws
                                                  const double2 *__restrict__ ek, const double2 *__restrict__ ekp,
                                                  const double2 *__restrict__ uq, const double *__restrict__ en,
                                                  const double2 *__restrict__ gm, const int64_t *__restrict__ gidx,
-                                                 double *__restrict__ out) {
+                                                 const double2 *__restrict__ lp, double *__restrict__ out) {
     const int v = blockIdx.x;
     if (v >= n) return;
     __shared__ double2 ov[32 * 32];
@@ -1300,8 +1497,17 @@ __global__ __launch_bounds__(256) void k_g2_list(int n, int nb, int nbr, int nws
         __syncthreads();
     }
     if (t == 0) {
+        double2 g = make_double2(red[0][0], red[1][0]);
+        if (lp) {  // polar: + glprefac * dot_product(U_k', U_k) (src/wannier.f90:998-1006)
+            double2 unm = make_double2(0.0, 0.0);
+            for (int i = 0; i < nb; ++i) {
+                unm.x += ov[i + nb * i].x;
+                unm.y += ov[i + nb * i].y;
+            }
+            cfma(g, lp[v], unm);
+        }
         const double w = en[v];
-        out[v] = (w == 0.0) ? 0.0 : 0.5 * (red[0][0] * red[0][0] + red[1][0] * red[1][0]) / w * kG2Unit;
+        out[v] = (w == 0.0) ? 0.0 : 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
     }
 }
 
@@ -1370,6 +1576,113 @@ void build_phonon_terms(elph_ctx *c) {
     c->term_div.upload(tdiv, c->stream);
 }
 
+// Constants of the polar terms (src/wannier.f90:795-800, 1027-1033) and the
+// G-only self term of dyn_nonanalytic (src/wannier.f90:814-846), times fac.
+void build_polar_terms(elph_ctx *c) {
+    const int64_t nat = c->natoms, nbr = c->nbr;
+    const double *L = c->lattvecs, *rec = c->reclattvecs, *eps = c->epsilon;
+    const double a1 = std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+    const double tpiba = kTwoPi / a1 * kBohr2nm;
+    c->pol_alpha = tpiba * tpiba;
+    c->pol_fac = 4.0 * kTwoPi / (c->volume / (kBohr2nm * kBohr2nm * kBohr2nm));
+    const double gmax = 14.0, alph = c->pol_alpha;
+    std::vector<double> G, tau(3 * nat);
+    for (int64_t a = 0; a < nat; ++a)
+        for (int r = 0; r < 3; ++r)
+            tau[3 * a + r] = (L[r] * c->basis[3 * a] + L[r + 3] * c->basis[3 * a + 1] + L[r + 6] * c->basis[3 * a + 2]) /
+                             kBohr2nm;
+    const double *Z = c->born.data();
+    std::vector<double2> D0(nbr * nbr, make_double2(0.0, 0.0));
+    const int64_t *N = c->coarse_qmesh;
+    for (int64_t m1 = -N[0]; m1 <= N[0]; ++m1)
+        for (int64_t m2 = -N[1]; m2 <= N[1]; ++m2)
+            for (int64_t m3 = -N[2]; m3 <= N[2]; ++m3) {
+                double g[3];
+                for (int r = 0; r < 3; ++r) g[r] = (m1 * rec[r] + m2 * rec[3 + r] + m3 * rec[6 + r]) * kBohr2nm;
+                G.insert(G.end(), g, g + 3);
+                double geg = 0.0;
+                for (int r = 0; r < 3; ++r) geg += g[r] * (eps[r] * g[0] + eps[r + 3] * g[1] + eps[r + 6] * g[2]);
+                if (!(geg > 0.0 && geg / alph / 4.0 < gmax)) continue;
+                const double f = std::exp(-geg / alph / 4.0) / geg;
+                for (int64_t ia = 0; ia < nat; ++ia) {
+                    double zi[3];
+                    for (int i = 0; i < 3; ++i)
+                        zi[i] = g[0] * Z[9 * ia + 3 * i] + g[1] * Z[9 * ia + 3 * i + 1] + g[2] * Z[9 * ia + 3 * i + 2];
+                    double2 fn[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+                    for (int64_t ja = 0; ja < nat; ++ja) {
+                        double arg = 0.0;
+                        for (int r = 0; r < 3; ++r) arg += g[r] * (tau[3 * ia + r] - tau[3 * ja + r]);
+                        const double cs = std::cos(arg), sn = std::sin(arg);
+                        for (int j = 0; j < 3; ++j) {
+                            const double zj = g[0] * Z[9 * ja + 3 * j] + g[1] * Z[9 * ja + 3 * j + 1] +
+                                              g[2] * Z[9 * ja + 3 * j + 2];
+                            fn[j].x += zj * cs;
+                            fn[j].y += zj * sn;
+                        }
+                    }
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) {
+                            double2 &d = D0[(3 * ia + i) + nbr * (3 * ia + j)];
+                            d.x -= f * zi[i] * fn[j].x;
+                            d.y -= f * zi[i] * fn[j].y;
+                        }
+                }
+            }
+    for (auto &d : D0) { d.x *= c->pol_fac; d.y *= c->pol_fac; }
+    c->pol_nG = (int64_t)G.size() / 3;
+    c->pol_G.upload(G, c->stream);
+    c->pol_Z.upload(c->born, c->stream);
+    c->pol_tau.upload(tau, c->stream);
+    c->pol_D0.upload(D0, c->stream);
+}
+
+// Polar terms at nq device q vectors (crystal coordinates, (nq,3) column-major):
+// D_L added into D (and dD_L into dD), and/or Lq written.
+void polar_launch(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D, double2 *dD,
+                  double2 *Lq) {
+    if (c->natoms > kPolMaxAt) throw Error{ELPH_EINVAL, "polar terms support up to 10 atoms"};
+    PolarArgs a{};
+    a.nq = (int)nq;
+    a.qv = d_qvecs;
+    a.nat = (int)c->natoms;
+    a.nbr = (int)c->nbr;
+    a.nG = (int)c->pol_nG;
+    a.G = c->pol_G.p;
+    a.Z = c->pol_Z.p;
+    a.tau = c->pol_tau.p;
+    for (int i = 0; i < 9; ++i) {
+        a.eps[i] = c->epsilon[i];
+        a.rec[i] = c->reclattvecs[i] * kBohr2nm;
+    }
+    a.alpha = c->pol_alpha;
+    a.fac = c->pol_fac;
+    a.gmax = 14.0;
+    a.dscale = kBohr2nm;
+    a.D0 = c->pol_D0.p;
+    a.D = D;
+    a.dD = dD;
+    a.Lq = Lq;
+    if (nq > 0) hipLaunchKernelGGL(k_polar, dim3((unsigned)nq), dim3(256), 0, st, a);
+    HCHECK(hipGetLastError());
+}
+
+// lp[q][s] = sum_p Lq(q)[p] u[q][s][p] for nq host q vectors and device u [q][s][p].
+void polar_prefactors(elph_ctx *c, int64_t nq, const std::vector<double> &qv, const DBuf<double2> &u,
+                      DBuf<double2> &lp) {
+    const int64_t nbr = c->nbr;
+    DBuf<double> dq;
+    DBuf<double2> Lq;
+    dq.upload(qv, c->stream);
+    Lq.alloc(nq * nbr);
+    polar_launch(c, c->stream, nq, dq.p, nullptr, nullptr, Lq.p);
+    lp.alloc(nq * nbr);
+    const int64_t n = nq * nbr;
+    hipLaunchKernelGGL(k_polar_lp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (int)n, (int)nbr,
+                       (int)nbr, (const int *)nullptr, Lq.p, u.p, lp.p);
+    HCHECK(hipGetLastError());
+    HCHECK(hipStreamSynchronize(c->stream));
+}
+
 void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, const double *gwann) {
     HCHECK(hipSetDevice(c->device));
     HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -1407,6 +1720,7 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     c->degq.upload(deg32(c->phwsdeg), c->stream);
     c->degg.upload(deg32(c->gwsdeg), c->stream);
     build_phonon_terms(c);
+    if (c->polar) build_polar_terms(c);
     HCHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -1457,11 +1771,10 @@ void el_wann_device(elph_ctx *c, int64_t nk, const double *d_kvecs, DBuf<double>
     HCHECK(hipGetLastError());
 }
 
-// Device ph_wann (non-polar part + mass normalisation + eigensolve).
+// Device ph_wann (short-range D(q), polar D_L, mass normalisation, eigensolve).
 void ph_wann_device(elph_ctx *c, int64_t nq, const double *d_qvecs, DBuf<double> &W, DBuf<double2> &V,
                     DBuf<double> *vel) {
     const int64_t nbr = c->nbr, nbr2 = nbr * nbr;
-    if (c->polar) throw Error{ELPH_EINVAL, "polar (non-analytic) phonon correction is not available on the device"};
     DBuf<double2> D, dD;
     D.alloc(nq * nbr2);
     if (vel) dD.alloc(nq * 3 * nbr2);
@@ -1469,6 +1782,7 @@ void ph_wann_device(elph_ctx *c, int64_t nq, const double *d_qvecs, DBuf<double>
     hipLaunchKernelGGL(k_build_D, grid, dim3(64), 0, c->stream, (int)nq, (int)nbr, (int)c->natoms, d_qvecs,
                        c->term_off.p, c->term_iuc.p, c->term_T.p, c->term_div.p, c->DphT.p, c->rcart_q.p, D.p,
                        vel ? dD.p : (double2 *)nullptr);
+    if (c->polar) polar_launch(c, c->stream, nq, d_qvecs, D.p, vel ? dD.p : nullptr, nullptr);
     std::vector<double> scale(nbr2);
     for (int64_t j = 0; j < nbr; ++j)
         for (int64_t i = 0; i < nbr; ++i)
@@ -1823,7 +2137,6 @@ int elph_g2(elph_ctx *ctx, int64_t n, const double *kvecs, const double *qvecs, 
     try {
         HCHECK(hipSetDevice(ctx->device));
         const int64_t nb = ctx->nb, nbr = ctx->nbr;
-        if (ctx->polar) throw Error{ELPH_EINVAL, "polar long-range vertex term is not available on the device"};
         const bool ph = wannspace == 'p';
         const int64_t nws = ph ? ctx->nwsg : ctx->nwsk;
         for (int64_t i = 0; i < n; ++i)
@@ -1856,9 +2169,22 @@ int elph_g2(elph_ctx *ctx, int64_t n, const double *kvecs, const double *qvecs, 
         gm.upload(reinterpret_cast<const double2 *>(gmixed), (size_t)ngmixed * nb * nb * nbr * nws, ctx->stream);
         dg.upload(gidx, n, ctx->stream);
         DBuf<double> dout;
+        DBuf<double2> Lq, lp;
+        if (ctx->polar) {
+            DBuf<double> dqv;
+            if (ph) dqv.alloc(1);
+            else dqv.upload(qvecs, 3 * n, ctx->stream);
+            Lq.alloc(n * nbr);
+            lp.alloc(n);
+            polar_launch(ctx, ctx->stream, n, ph ? dv.p : dqv.p, nullptr, nullptr, Lq.p);
+            hipLaunchKernelGGL(k_polar_lp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, (int)n,
+                               (int)nbr, 1, (const int *)nullptr, Lq.p, uq.p, lp.p);
+            HCHECK(hipStreamSynchronize(ctx->stream));
+        }
         dout.alloc(n);
         hipLaunchKernelGGL(k_g2_list, dim3((unsigned)n), dim3(256), 0, ctx->stream, (int)n, (int)nb, (int)nbr, (int)nws,
-                           phase.p, ek.p, ekp.p, uq.p, den.p, gm.p, dg.p, dout.p);
+                           phase.p, ek.p, ekp.p, uq.p, den.p, gm.p, dg.p, ctx->polar ? lp.p : (const double2 *)nullptr,
+                           dout.p);
         HCHECK(hipGetLastError());
         HCHECK(hipMemcpyAsync(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HCHECK(hipStreamSynchronize(ctx->stream));
@@ -1947,7 +2273,7 @@ struct elph_ibzk_plan {
     std::vector<double2> Uk_host;  // [nirr][m][mp]
     DBuf<int> kp_int, nwin, wband, Rq;
     DBuf<int64_t> offs;
-    DBuf<double2> Vkp, u, uf, T;
+    DBuf<double2> Vkp, u, uf, T, lp, lpf;
     DBuf<double> om, omf;
     // per-run scratch
     DBuf<double2> gk, h, Ukb;
@@ -1990,6 +2316,11 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     a.om = p->om.p;
     a.uf = p->uf.p;
     a.omf = p->omf.p;
+    a.lp = c->polar ? p->lp.p : nullptr;
+    a.lpf = (c->polar && p->mesh_ref > 1) ? p->lpf.p : nullptr;
+    a.Ukb = p->Ukb.p;
+    a.st_kb = p->st_kb.p;
+    a.st_m = p->st_m.p;
     a.nb = (int)p->nb;
     a.nbr = (int)p->nbr;
     a.out = out;
@@ -2120,6 +2451,16 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             }
         p->u.upload(u, s);
         p->om.upload(om, s);
+        if (ctx->polar) {
+            // long-range vertex prefactor of every phonon-mesh mode, q = ints / mesh
+            std::vector<double> qv(3 * nq);
+            for (int64_t i = 0; i < nq; ++i) {
+                int64_t v[3];
+                demux_vector0(i + 1, v, ph->wvmesh);
+                for (int d = 0; d < 3; ++d) qv[i + nq * d] = (double)v[d] / (double)ph->wvmesh[d];
+            }
+            polar_prefactors(ctx, nq, qv, p->u, p->lp);
+        }
         if (mesh_ref > 1) {
             // phonons at every electron-mesh q, interpolated by ph_wann on the device
             // (the reference calls ph_wann per k', src/interactions.f90:2241-2247)
@@ -2146,6 +2487,7 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
                 }
             p->uf.upload(uf, s);
             p->omf.upload(omf, s);
+            if (ctx->polar) polar_prefactors(ctx, nqf, qv, p->uf, p->lpf);
         } else {
             p->uf.alloc(1);
             p->omf.alloc(1);
@@ -2378,7 +2720,7 @@ struct elph_ibzq_plan {
     std::vector<double> ph_evecs; // (nq, nbr, nbr) c128
     DBuf<int> kint, nwin, wband, lut, Rk;
     DBuf<double2> Vk, T;
-    DBuf<double2> gq, h, st_u;
+    DBuf<double2> gq, h, st_u, Lq, st_lp;
     DBuf<double> qv, st_om;
     DBuf<int> st_qb, st_qe, rowmap;
     DBuf<int64_t> rowoff, st_tot, st_slot;
@@ -2552,6 +2894,13 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             p->st_u.upload(su, s);
             p->st_om.upload(som, s);
             p->st_slot.upload(sslot, s);
+            if (c->polar) {
+                p->Lq.alloc(nqb * nbr);
+                p->st_lp.alloc(nst);
+                polar_launch(c, s, nqb, p->qv.p, nullptr, nullptr, p->Lq.p);
+                hipLaunchKernelGGL(k_polar_lp, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, s, (int)nst,
+                                   (int)nbr, 1, p->st_qb.p, p->Lq.p, p->st_u.p, p->st_lp.p);
+            }
             p->h.alloc(nst * p->Kp * p->CP);
             const int64_t tot = nst * p->Kp * p->CP;
             hipLaunchKernelGGL(k_rotate_hq, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nst, (int)nb,
@@ -2586,6 +2935,7 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             a.rowmap = p->rowmap.p;
             a.rowoff = p->rowoff.p;
             a.st_om = p->st_om.p;
+            a.st_lp = c->polar ? p->st_lp.p : nullptr;
             a.st_slot = p->st_slot.p;
             a.nb = (int)nb;
             a.out = dst;

@@ -323,6 +323,34 @@ def tiny(nb=2, natoms=1, ws=3, wv=4, seed=7, bands_in_window=None, mesh_ref=1) -
                          [ws] * 3, [wv] * 3, seed, mesh_ref=mesh_ref, bands_in_window=biw)
 
 
+def with_polar(mat: Material, kind: str = "sic", seed: int = 0) -> Material:
+    """A polar copy of `mat` (Born effective charges and dielectric tensor).
+    kind "sic": the 3C-SiC values of the reference's test/3C-SiC/input.nml
+    (epsilon = 7.4190074 I, Z*_1 = -Z*_2 = 2.7514963 I; two atoms).
+    kind "random": a symmetric positive-definite anisotropic epsilon and
+    non-symmetric Born tensors with sum_a Z*_a = 0 (charge neutrality)."""
+    nat = mat.natoms
+    if kind == "sic":
+        if nat != 2:
+            raise ValueError("the 3C-SiC charges need two atoms")
+        eps = 7.4190074 * np.eye(3)
+        born = np.zeros((3, 3, 2))
+        born[:, :, 0] = 2.7514963 * np.eye(3)
+        born[:, :, 1] = -2.7514963 * np.eye(3)
+    elif kind == "random":
+        rng = np.random.default_rng(seed)
+        a = rng.standard_normal((3, 3)) * 0.4
+        eps = 6.0 * np.eye(3) + a @ a.T
+        born = rng.standard_normal((3, 3, nat)) * 0.5
+        for i in range(nat):
+            born[:, :, i] += (2.0 if i % 2 == 0 else -2.0) * np.eye(3)
+        born -= born.mean(axis=2, keepdims=True)
+    else:
+        raise ValueError(f"unknown polar kind {kind!r}")
+    return dataclasses.replace(mat, name=mat.name + f"_polar_{kind}", polar=True,
+                               epsilon=np.asfortranarray(eps), born=np.asfortranarray(born))
+
+
 # --------------------------------------------------------------------------
 # EPW file format writer (the reader is the product's, in the C++ library)
 # --------------------------------------------------------------------------

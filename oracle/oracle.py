@@ -156,6 +156,22 @@ class Wannier:
         assert st == 0
         return en, ev, vel
 
+    def dyn_nonanalytic(self, qcart, derivative=False):
+        """D_L(q) (nbr,nbr) and optionally dD_L/dq (nbr,nbr,3); q Cartesian, 1/bohr."""
+        nbr = self.mat.nbr
+        q = _f64(qcart)
+        d = np.zeros((nbr, nbr), np.complex128, order="F")
+        dd = np.zeros((nbr, nbr, 3), np.complex128, order="F") if derivative else None
+        lib().orc_dyn_nonanalytic(ct.byref(self.s), _p(q), _p(d), _p(dd) if dd is not None else None)
+        return (d, dd) if derivative else d
+
+    def long_range_prefac(self, qcart, u):
+        """glprefac for q Cartesian (1/bohr) and u = e_q / sqrt(M) (nbr,)."""
+        q, uu = _f64(qcart), _c128(u)
+        out = np.zeros(1, np.complex128)
+        lib().orc_long_range_prefac(ct.byref(self.s), _p(q), _p(uu), _p(out))
+        return complex(out[0])
+
     def gkRp(self, kvec):
         m = self.mat
         out = np.zeros((m.nb, m.nb, m.nbr, m.nwsq), np.complex128, order="F")
