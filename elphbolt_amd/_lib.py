@@ -37,6 +37,12 @@ class ElphPhonons(ct.Structure):
                 ("nwv_irred", ct.c_int64), ("indexlist_irred", c_i64p)]
 
 
+class ElphServerStats(ct.Structure):
+    _fields_ = [("completed", ct.c_int64), ("failed", ct.c_int64), ("pending", ct.c_int64), ("active", ct.c_int64),
+                ("vertices", ct.c_int64), ("units", ct.c_int64), ("pool_bytes", ct.c_int64),
+                ("pool_used", ct.c_int64), ("max_slots_busy", ct.c_int64), ("busy_span_ms", ct.c_double)]
+
+
 # exported symbol -> (restype, argtypes)
 PROTOTYPES = {
     "elph_ctx_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.POINTER(ElphDims), c_i64p, c_i64p, c_i64p,
@@ -78,6 +84,18 @@ PROTOTYPES = {
     "elph_device_to_host": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]),
     "elph_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
+    "elph_synth_vertex": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "elph_server_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int32, ct.c_int64, ct.c_int32]),
+    "elph_server_add_material": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "elph_server_submit": (ct.c_int64, [ct.c_void_p, ct.c_int32, ct.c_int64, c_dblp, ct.c_int64, c_dblp]),
+    "elph_server_wait": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp]),
+    "elph_server_test": (ct.c_int, [ct.c_void_p, ct.c_int64]),
+    "elph_server_fetch": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_float)]),
+    "elph_server_result_device": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_void_p), c_i64p]),
+    "elph_server_release": (ct.c_int, [ct.c_void_p, ct.c_int64]),
+    "elph_server_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ElphServerStats)]),
+    "elph_server_error": (ct.c_char_p, [ct.c_void_p]),
+    "elph_server_destroy": (ct.c_int, [ct.c_void_p]),
 }
 
 _lib = None

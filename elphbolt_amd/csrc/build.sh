@@ -6,7 +6,7 @@ OUT="$HERE/../libelph_vertex.so"
 HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 "$HIPCC" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
   -Xclang -target-feature -Xclang -packed-fp32-ops \
-  -Wall -Wno-unused-result \
-  -o "$OUT.tmp" "$HERE/elph_vertex.hip" "$HERE/host_setup.cpp" "$HERE/host_state.cpp"
+  -Wall -Wno-unused-result -pthread \
+  -o "$OUT.tmp" "$HERE/elph_vertex.hip" "$HERE/serve.hip" "$HERE/host_setup.cpp" "$HERE/host_state.cpp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"

@@ -1,0 +1,136 @@
+// Internal declarations shared by the library's translation units: device
+// buffers, the context (HBM-resident Wannier tensors) and stream-parameterised
+// launchers of kernels that more than one driver uses.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+#define HCHECK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            throw Error{ELPH_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+        }                                                                               \
+    } while (0)
+
+namespace elph {
+
+constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{ELPH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+        }
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        HCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+}  // namespace elph
+
+using elph::DBuf;
+using elph::v4i;
+
+struct elph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
+    int64_t nb = 0, nbr = 0, nwsk = 0, nwsq = 0, nwsg = 0, natoms = 0;
+    int64_t coarse_qmesh[3] = {0, 0, 0};
+    double lattvecs[9], reclattvecs[9], volume = 0;
+    int polar = 0;
+    double epsilon[9];
+    std::vector<double> basis, masses, born;
+    std::vector<int64_t> rcells_k, rcells_q, rcells_g, elwsdeg, phwsdeg, gwsdeg;
+    // HBM-resident tensors
+    DBuf<double2> gwann;   // (nb*nb, nwsk, nbr*nwsg)
+    DBuf<double2> HwT;     // [Re][i + nb j]
+    DBuf<double2> DphT;    // [iuc][i + nbr j]
+    DBuf<int> rk, rq, rg;  // [cell][3]
+    DBuf<int> degk, degq, degg;
+    DBuf<double> rcart_k, rcart_q;  // [cell][3] Cartesian (nm)
+    // int8 digits of gwann for the mixed-precision Re contraction (built on
+    // first use): [slice][row block][K chunk][re/im][digit][lane]
+    DBuf<v4i> gdig;
+    DBuf<double> gscale;
+    int64_t gnrb = 0, gnkc = 0;
+    // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
+    DBuf<int> term_off, term_iuc, term_T, term_div;
+    // polar crystals: reciprocal vectors of the (2N+1)^3 box (1/bohr), Born
+    // charges, atom positions (bohr) and the q-independent part of D_L
+    DBuf<double> pol_G, pol_Z, pol_tau;
+    DBuf<double2> pol_D0;
+    int64_t pol_nG = 0;
+    double pol_alpha = 0, pol_fac = 0;
+    std::string err;
+    double t_dominant_ms = 0, t_all_ms = 0;
+    int64_t n_dominant = 0;
+};
+
+
+namespace elph {
+
+int ret_error(elph_ctx *ctx, const Error &e);
+int64_t lcm64(int64_t a, int64_t b);
+
+// Launchers (all enqueue on `st`; none synchronises or allocates).
+// gkRp for nk device k vectors ((nk,3) column-major) into out[k] (nb*nb, nbr*nwsg);
+// phiT: nwsk*64 complex scratch; bphi (mixed only): gnkc*4*3*kNS*64 v4i scratch.
+void launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride,
+                 bool mixed, double2 *phiT, v4i *bphi);
+void ensure_gwann_digits(elph_ctx *c);  // int8 digits of gwann, once per context (synchronous)
+// h[st][Kp][CP] = sum_mp U(kb,m,mp) gk[kb](np,mp,sp,Rp) / phwsdeg(Rp)
+void launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *gk,
+                     int64_t gk_kstride, const int *st_kb, const int *st_m, const double2 *Uk, const int *degq,
+                     double2 *h);
+void launch_colscale(hipStream_t st, int nst, int Kp, int CP, const double2 *h, double *scale);
+void launch_slice_b(hipStream_t st, int nst, int nkc, int CB, int Kp, int CP, const double2 *h, const double *scale,
+                    v4i *bsl);
+// Batched zheev('V','U') (optionally on A .* scale): W ascending, V[mat][col j][row]
+void launch_jacobi(hipStream_t st, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V);
+// Short-range D(q) (plus the polar D_L for polar contexts) at nq device q vectors
+void launch_dynmat(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D);
+// Mass normalisation factors Ryd2amu / sqrt(M_i M_j) of the context, (nbr, nbr)
+std::vector<double> mass_scale(const elph_ctx *c);
+// Polar terms (see k_polar): D_L into D, dD_L into dD, long-range vector into Lq
+void polar_launch(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D, double2 *dD,
+                  double2 *Lq);
+// H(k) at n device k vectors (k_d of point v at d_kvecs[v + vstride d]) into
+// H[k][i + nb j]; phi: n * nwsk complex scratch
+void launch_build_H(elph_ctx *c, hipStream_t st, int64_t n, const double *d_kvecs, int64_t vstride, double2 *phi,
+                    double2 *H);
+// lp[i] = sum_p Lq[r][p] u[i][p], r = ridx[i] (or i / div)
+void launch_polar_lp(hipStream_t st, int n, int nbr, int div, const int *ridx, const double2 *Lq, const double2 *u,
+                     double2 *lp);
+
+}  // namespace elph

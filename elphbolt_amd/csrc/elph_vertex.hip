@@ -26,93 +26,10 @@
 #include <vector>
 
 #include "common.h"
+#include "ctx.h"
 #include "kernels.h"
 
 using namespace elph;
-
-#define HCHECK(call)                                                                    \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess) {                                                         \
-            throw Error{ELPH_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
-        }                                                                               \
-    } while (0)
-
-namespace {
-
-constexpr double kTwoPi = 2.0 * 3.14159265358979323846;
-
-// ---------------------------------------------------------------------------
-// Device buffers
-// ---------------------------------------------------------------------------
-template <typename T>
-struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        if (count <= n && p) return;
-        release();
-        if (count == 0) count = 1;
-        hipError_t e = hipMalloc(&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw Error{ELPH_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
-        }
-        n = count;
-    }
-    void upload(const T *h, size_t count, hipStream_t s) {
-        alloc(count);
-        HCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-    void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
-};
-
-}  // namespace
-
-struct elph_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
-    int64_t nb = 0, nbr = 0, nwsk = 0, nwsq = 0, nwsg = 0, natoms = 0;
-    int64_t coarse_qmesh[3] = {0, 0, 0};
-    double lattvecs[9], reclattvecs[9], volume = 0;
-    int polar = 0;
-    double epsilon[9];
-    std::vector<double> basis, masses, born;
-    std::vector<int64_t> rcells_k, rcells_q, rcells_g, elwsdeg, phwsdeg, gwsdeg;
-    // HBM-resident tensors
-    DBuf<double2> gwann;   // (nb*nb, nwsk, nbr*nwsg)
-    DBuf<double2> HwT;     // [Re][i + nb j]
-    DBuf<double2> DphT;    // [iuc][i + nbr j]
-    DBuf<int> rk, rq, rg;  // [cell][3]
-    DBuf<int> degk, degq, degg;
-    DBuf<double> rcart_k, rcart_q;  // [cell][3] Cartesian (nm)
-    // int8 digits of gwann for the mixed-precision Re contraction (built on
-    // first use): [slice][row block][K chunk][re/im][digit][lane]
-    DBuf<v4i> gdig;
-    DBuf<double> gscale;
-    int64_t gnrb = 0, gnkc = 0;
-    // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
-    DBuf<int> term_off, term_iuc, term_T, term_div;
-    // polar crystals: reciprocal vectors of the (2N+1)^3 box (1/bohr), Born
-    // charges, atom positions (bohr) and the q-independent part of D_L
-    DBuf<double> pol_G, pol_Z, pol_tau;
-    DBuf<double2> pol_D0;
-    int64_t pol_nG = 0;
-    double pol_alpha = 0, pol_fac = 0;
-    std::string err;
-    double t_dominant_ms = 0, t_all_ms = 0;
-    int64_t n_dominant = 0;
-};
 
 namespace {
 
@@ -329,69 +246,18 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
     }
 }
 
-// Phase-contraction core shared by the ibzk and ibzq pair kernels:
-//   acc[rb][c] (16 rows x 16 cols) += sum_K T[(coef . R_K) mod L] * B[K][c]
-// B = hst ([Kp][CP] complex) is streamed through LDS in KC-row chunks with a
-// register prefetch of the next chunk; the phase of row r and cell R is the
-// exact root of unity e^{i 2 pi (coef_r . R) / L}.
+// Phase-contraction core of the mesh drivers: the phase of row r and cell R_K
+// is the exact root of unity T[(coef_r . R_K) mod L] (see gemm_core_f64).
 template <int CB, int RB, int KC>
 __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int Kp, const int (&coef)[RB][3], int L,
                                                 const double2 *Tsh, const int4 *Rsh, double2 *buf0, double2 *buf1,
                                                 d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
-    constexpr int CP = 16 * CB;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int c = 0; c < CB; ++c) { cr[rb][c] = d4{0, 0, 0, 0}; ci[rb][c] = d4{0, 0, 0, 0}; }
-    constexpr int NPF = (KC * CP + 255) / 256;
-    constexpr int chunk = KC * CP;
-    for (int i = tid; i < chunk; i += 256) buf0[i] = hst[i];
-    __syncthreads();
-    const int nchunks = Kp / KC;
-    for (int ch = 0; ch < nchunks; ++ch) {
-        double2 pf[NPF];
-        const bool more = ch + 1 < nchunks;
-        if (more) {
-            const double2 *src = hst + (size_t)(ch + 1) * chunk;
-#pragma unroll
-            for (int j = 0; j < NPF; ++j) {
-                const int i = tid + 256 * j;
-                if (i < chunk) pf[j] = src[i];
-            }
-        }
-        const double2 *B = (ch & 1) ? buf1 : buf0;
-#pragma unroll 2
-        for (int ks = 0; ks < KC / 4; ++ks) {
-            const int kl = ks * 4 + lk;
-            const int4 R = Rsh[ch * KC + kl];
-            double ar[RB], ai[RB], nai[RB];
-#pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
-                const double2 t = Tsh[pos_mod(x, L)];
-                ar[rb] = t.x;
-                ai[rb] = t.y;
-                nai[rb] = -t.y;
-            }
-#pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                const double2 b = B[kl * CP + c * 16 + li];
-#pragma unroll
-                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], b.x, b.y, cr[rb][c], ci[rb][c]);
-            }
-        }
-        if (more) {
-            double2 *dst = (ch & 1) ? buf0 : buf1;
-#pragma unroll
-            for (int j = 0; j < NPF; ++j) {
-                const int i = tid + 256 * j;
-                if (i < chunk) dst[i] = pf[j];
-            }
-        }
-        __syncthreads();
-    }
+    auto aop = [&](int rb, int k) {
+        const int4 R = Rsh[k];
+        const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
+        return Tsh[pos_mod(x, L)];
+    };
+    gemm_core_f64<CB, RB, KC>(hst, 16 * CB, Kp, aop, buf0, buf1, cr, ci);
 }
 
 template <int CB>
@@ -458,8 +324,6 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 // level weights 1 / (127^2 254^l) and the column scale of B.  The band/mode
 // rotations and |g|^2 stay FP64.
 // ---------------------------------------------------------------------------
-constexpr int kNS = 4;
-
 // Column scale of B = h[st] (max over K of max(|Re|, |Im|); 1 for zero columns).
 __global__ void k_colscale(int nst, int Kp, int CP, const double2 *__restrict__ h, double *__restrict__ scale) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1524,12 +1388,16 @@ __global__ void k_mfma_probe(const double *A /*16x4 row-major*/, const double *B
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
-int ret_error(elph_ctx *ctx, const Error &e) {
+}  // namespace
+
+int elph::ret_error(elph_ctx *ctx, const Error &e) {
     if (ctx) ctx->err = e.msg;
     return e.code;
 }
 
-int64_t lcm64(int64_t a, int64_t b) { return a / std::gcd(a, b) * b; }
+int64_t elph::lcm64(int64_t a, int64_t b) { return a / std::gcd(a, b) * b; }
+
+namespace {
 
 void upload_cells(elph_ctx *c, const std::vector<int64_t> &cells, int64_t n, DBuf<int> &dst, DBuf<double> *cart) {
     std::vector<int> v(3 * n);
@@ -1638,8 +1506,10 @@ void build_polar_terms(elph_ctx *c) {
 
 // Polar terms at nq device q vectors (crystal coordinates, (nq,3) column-major):
 // D_L added into D (and dD_L into dD), and/or Lq written.
-void polar_launch(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D, double2 *dD,
-                  double2 *Lq) {
+}  // namespace
+
+void elph::polar_launch(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D, double2 *dD,
+                        double2 *Lq) {
     if (c->natoms > kPolMaxAt) throw Error{ELPH_EINVAL, "polar terms support up to 10 atoms"};
     PolarArgs a{};
     a.nq = (int)nq;
@@ -1666,6 +1536,8 @@ void polar_launch(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs
     HCHECK(hipGetLastError());
 }
 
+namespace {
+
 // lp[q][s] = sum_p Lq(q)[p] u[q][s][p] for nq host q vectors and device u [q][s][p].
 void polar_prefactors(elph_ctx *c, int64_t nq, const std::vector<double> &qv, const DBuf<double2> &u,
                       DBuf<double2> &lp) {
@@ -1681,6 +1553,35 @@ void polar_prefactors(elph_ctx *c, int64_t nq, const std::vector<double> &qv, co
                        (int)nbr, (const int *)nullptr, Lq.p, u.p, lp.p);
     HCHECK(hipGetLastError());
     HCHECK(hipStreamSynchronize(c->stream));
+}
+
+// Synthetic Wannier vertex of the benchmark workloads, generated in HBM:
+// g(np,mp,Re,sp,Rp) = A e^{-|Re|/2} e^{-|Rp|/2} e^{i 2 pi u}, |R| in Angstrom,
+// A = 0.1 eV in Ry, u uniform in [0,1) from splitmix64 of (seed, element index).
+__global__ void k_synth_vertex(int64_t n, int nb2, int nwsk, int nbr, const double *__restrict__ rk_nm,
+                               const int *__restrict__ cells_g, double L0, double L1, double L2, double L3, double L4,
+                               double L5, double L6, double L7, double L8, double amp, uint64_t seed,
+                               double2 *__restrict__ g) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int64_t r = t / nb2;
+    const int re = (int)(r % nwsk);
+    r /= nwsk;
+    const int rp = (int)(r / nbr);
+    const double ke = sqrt(rk_nm[3 * re] * rk_nm[3 * re] + rk_nm[3 * re + 1] * rk_nm[3 * re + 1] +
+                           rk_nm[3 * re + 2] * rk_nm[3 * re + 2]) * 10.0;
+    const double x = (double)cells_g[3 * rp], y = (double)cells_g[3 * rp + 1], z = (double)cells_g[3 * rp + 2];
+    const double cx = L0 * x + L3 * y + L6 * z, cy = L1 * x + L4 * y + L7 * z, cz = L2 * x + L5 * y + L8 * z;
+    const double kg = sqrt(cx * cx + cy * cy + cz * cz) * 10.0;
+    uint64_t h = seed + (uint64_t)t * 0x9E3779B97F4A7C15ull;
+    h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+    h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+    h ^= h >> 31;
+    const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    double sn, cs;
+    sincos(kTwoPi * u, &sn, &cs);
+    const double a = amp * exp(-0.5 * ke - 0.5 * kg);
+    g[t] = make_double2(a * cs, a * sn);
 }
 
 void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, const double *gwann) {
@@ -1711,7 +1612,11 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     c->DphT.upload(DT, c->stream);
     const size_t ng = (size_t)nb * nb * c->nwsk * nbr * c->nwsg;
     c->gwann.alloc(ng);
-    HCHECK(hipMemcpyAsync(c->gwann.p, gwann, ng * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    if (gwann)
+        HCHECK(hipMemcpyAsync(c->gwann.p, gwann, ng * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    else
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream, c->gwann.p,
+                           (int64_t)ng);
     upload_cells(c, c->rcells_k, c->nwsk, c->rk, &c->rcart_k);
     upload_cells(c, c->rcells_q, c->nwsq, c->rq, &c->rcart_q);
     upload_cells(c, c->rcells_g, c->nwsg, c->rg, nullptr);
@@ -1733,21 +1638,89 @@ void phase_list(elph_ctx *c, int64_t n, const double *d_vecs, const DBuf<int> &c
 }
 
 template <int NMAX>
-void launch_jacobi_n(elph_ctx *c, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V) {
+void launch_jacobi_n(hipStream_t st, int64_t nmat, int n, const double2 *A, const double *scale, double *W,
+                     double2 *V) {
     const int64_t chunk = 65535;
     for (int64_t m0 = 0; m0 < nmat; m0 += chunk) {
         const int64_t cnt = std::min(chunk, nmat - m0);
-        hipLaunchKernelGGL(k_jacobi<NMAX>, dim3((unsigned)cnt), dim3(64), 0, c->stream, (int)cnt, n,
+        hipLaunchKernelGGL(k_jacobi<NMAX>, dim3((unsigned)cnt), dim3(64), 0, st, (int)cnt, n,
                            A + (size_t)m0 * n * n, scale, W + (size_t)m0 * n, V + (size_t)m0 * n * n);
     }
 }
 
-void launch_jacobi(elph_ctx *c, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V) {
-    if (n <= 8) launch_jacobi_n<8>(c, nmat, n, A, scale, W, V);
-    else if (n <= 16) launch_jacobi_n<16>(c, nmat, n, A, scale, W, V);
-    else if (n <= 32) launch_jacobi_n<32>(c, nmat, n, A, scale, W, V);
+}  // namespace
+
+void elph::launch_jacobi(hipStream_t st, int64_t nmat, int n, const double2 *A, const double *scale, double *W,
+                         double2 *V) {
+    if (n <= 8) launch_jacobi_n<8>(st, nmat, n, A, scale, W, V);
+    else if (n <= 16) launch_jacobi_n<16>(st, nmat, n, A, scale, W, V);
+    else if (n <= 32) launch_jacobi_n<32>(st, nmat, n, A, scale, W, V);
     else throw Error{ELPH_EINVAL, "eigensolver supports matrices up to 32x32"};
 }
+
+void elph::launch_build_H(elph_ctx *c, hipStream_t st, int64_t n, const double *d_kvecs, int64_t vstride, double2 *phi,
+                          double2 *H) {
+    const int64_t nb2 = c->nb * c->nb;
+    dim3 gp((unsigned)((c->nwsk + 127) / 128), (unsigned)n);
+    hipLaunchKernelGGL(k_phase_list, gp, dim3(128), 0, st, (int)n, (int)c->nwsk, d_kvecs, vstride, c->rk.p, c->degk.p,
+                       phi, c->nwsk, (int64_t)1);
+    dim3 grid((unsigned)((nb2 + 63) / 64), (unsigned)n);
+    hipLaunchKernelGGL(k_build_H, grid, dim3(64), 0, st, (int)n, (int)nb2, (int)c->nwsk, phi, c->HwT.p, c->rcart_k.p,
+                       H, (double2 *)nullptr);
+    HCHECK(hipGetLastError());
+}
+
+void elph::launch_dynmat(elph_ctx *c, hipStream_t st, int64_t nq, const double *d_qvecs, double2 *D) {
+    const int64_t nbr2 = c->nbr * c->nbr;
+    dim3 grid((unsigned)((nbr2 + 63) / 64), (unsigned)nq);
+    hipLaunchKernelGGL(k_build_D, grid, dim3(64), 0, st, (int)nq, (int)c->nbr, (int)c->natoms, d_qvecs,
+                       c->term_off.p, c->term_iuc.p, c->term_T.p, c->term_div.p, c->DphT.p, c->rcart_q.p, D,
+                       (double2 *)nullptr);
+    if (c->polar) polar_launch(c, st, nq, d_qvecs, D, nullptr, nullptr);
+    HCHECK(hipGetLastError());
+}
+
+std::vector<double> elph::mass_scale(const elph_ctx *c) {
+    const int64_t nbr = c->nbr;
+    std::vector<double> scale(nbr * nbr);
+    for (int64_t j = 0; j < nbr; ++j)
+        for (int64_t i = 0; i < nbr; ++i)
+            scale[i + nbr * j] = 1.0 / std::sqrt(c->masses[i / 3] * c->masses[j / 3]) * kRyd2amu;
+    return scale;
+}
+
+void elph::launch_polar_lp(hipStream_t st, int n, int nbr, int div, const int *ridx, const double2 *Lq,
+                           const double2 *u, double2 *lp) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_polar_lp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, nbr, div, ridx, Lq, u,
+                           lp);
+    HCHECK(hipGetLastError());
+}
+
+void elph::launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *gk,
+                           int64_t gk_kstride, const int *st_kb, const int *st_m, const double2 *Uk, const int *degq,
+                           double2 *h) {
+    const int64_t tot = (int64_t)nst * Kp * CP;
+    hipLaunchKernelGGL(k_rotate_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nst, nb, nbr, nwsg, Kp, CP,
+                       gk, gk_kstride, st_kb, st_m, Uk, degq, h);
+    HCHECK(hipGetLastError());
+}
+
+void elph::launch_colscale(hipStream_t st, int nst, int Kp, int CP, const double2 *h, double *scale) {
+    hipLaunchKernelGGL(k_colscale, dim3((unsigned)(((int64_t)nst * CP + 255) / 256)), dim3(256), 0, st, nst, Kp, CP,
+                       h, scale);
+    HCHECK(hipGetLastError());
+}
+
+void elph::launch_slice_b(hipStream_t st, int nst, int nkc, int CB, int Kp, int CP, const double2 *h,
+                          const double *scale, v4i *bsl) {
+    const int64_t nt = (int64_t)nst * nkc * CB * 64;
+    hipLaunchKernelGGL(k_slice_b, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, nst, nkc, CB, Kp, CP, h, scale,
+                       bsl);
+    HCHECK(hipGetLastError());
+}
+
+namespace {
 
 // Device el_wann into device outputs W [k][b] (Ry), V [k][col b][row], vel [k][b][3] (raw).
 void el_wann_device(elph_ctx *c, int64_t nk, const double *d_kvecs, DBuf<double> &W, DBuf<double2> &V,
@@ -1762,7 +1735,7 @@ void el_wann_device(elph_ctx *c, int64_t nk, const double *d_kvecs, DBuf<double>
                        c->rcart_k.p, H.p, vel ? dH.p : (double2 *)nullptr);
     W.alloc(nk * nb);
     V.alloc(nk * nb2);
-    launch_jacobi(c, nk, (int)nb, H.p, nullptr, W.p, V.p);
+    launch_jacobi(c->stream, nk, (int)nb, H.p, nullptr, W.p, V.p);
     if (vel) {
         vel->alloc(nk * nb * 3);
         hipLaunchKernelGGL(k_quadform, dim3((unsigned)nk), dim3((unsigned)(((3 * nb) + 63) / 64 * 64)), 0, c->stream,
@@ -1791,7 +1764,7 @@ void ph_wann_device(elph_ctx *c, int64_t nq, const double *d_qvecs, DBuf<double>
     dscale.upload(scale, c->stream);
     W.alloc(nq * nbr);
     V.alloc(nq * nbr2);
-    launch_jacobi(c, nq, (int)nbr, D.p, dscale.p, W.p, V.p);
+    launch_jacobi(c->stream, nq, (int)nbr, D.p, dscale.p, W.p, V.p);
     if (vel) {
         // mass-normalise dD on the host path below (small); the quadratic form
         // uses the mass-normalised derivative, as the reference scales ddynmat
@@ -2040,8 +2013,10 @@ int elph_ph_wann(elph_ctx *ctx, int64_t nq, const double *qvecs, double *energie
     return status;
 }
 
+}  // extern "C"
+
 // int8 digits of gwann (once per context).
-static void ensure_gwann_digits(elph_ctx *c) {
+void elph::ensure_gwann_digits(elph_ctx *c) {
     if (c->gdig.p) return;
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
     c->gnrb = (nb2 + 15) / 16;
@@ -2060,7 +2035,7 @@ static void ensure_gwann_digits(elph_ctx *c) {
 // Batched gkRp into device memory: out[k] (nb*nb, nbr*nwsg) for kvecs (nk,3)
 // column-major on the device; FP64 or int8-digit (mixed) contraction.
 // phiT (nwsk*64) and, for mixed, bphi are scratch buffers of the caller.
-static void gkRp_launch(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
+void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
                         int64_t out_kstride, bool mixed, double2 *phiT, v4i *bphi) {
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
     const int64_t KT = 64;
@@ -2096,9 +2071,11 @@ static void gkRp_device(elph_ctx *c, int64_t nk, const double *d_kvecs, double2 
         ensure_gwann_digits(c);
         bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
     }
-    gkRp_launch(c, c->stream, nk, d_kvecs, out, out_kstride, mixed, phiT.p, bphi.p);
+    launch_gkRp(c, c->stream, nk, d_kvecs, out, out_kstride, mixed, phiT.p, bphi.p);
     HCHECK(hipStreamSynchronize(c->stream));
 }
+
+extern "C" {
 
 int elph_gkRp(elph_ctx *ctx, int64_t nk, const double *kvecs, double *out) {
     if (!ctx || nk < 0) return ELPH_EINVAL;
@@ -2224,6 +2201,24 @@ int elph_ctx_synchronize(elph_ctx *ctx) {
     if (!ctx) return ELPH_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
     return hipDeviceSynchronize() == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
+}
+
+int elph_synth_vertex(elph_ctx *ctx, uint64_t seed) {
+    if (!ctx) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        const int64_t n = ctx->nb * ctx->nb * ctx->nwsk * ctx->nbr * ctx->nwsg;
+        const double *L = ctx->lattvecs;
+        hipLaunchKernelGGL(k_synth_vertex, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, n,
+                           (int)(ctx->nb * ctx->nb), (int)ctx->nwsk, (int)ctx->nbr, ctx->rcart_k.p, ctx->rg.p, L[0],
+                           L[1], L[2], L[3], L[4], L[5], L[6], L[7], L[8], 0.1 / kRyd2eV, seed, ctx->gwann.p);
+        HCHECK(hipGetLastError());
+        ctx->gdig.release();  // digits of the previous vertex, if any
+        HCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
 }
 
 int elph_selftest_mfma(int device, double *max_err) {

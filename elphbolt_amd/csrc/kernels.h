@@ -48,6 +48,9 @@ __device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
+// Digits per part in the mixed-precision products (base 254, see split_digits).
+constexpr int kNS = 4;
+
 // Signed-digit split of x in [-1, 1] into NS int8 digits d_s in [-127, 127]:
 // x ~= sum_s d_s / (127 * 254^s), error below 1 / (2 * 127 * 254^(NS-1)).
 template <int NS>
@@ -73,6 +76,70 @@ __device__ __forceinline__ void transpose4x4_bytes(unsigned a0, unsigned a1, uns
     o1 = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
     o2 = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
     o3 = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
+// Complex GEMM core on the FP64 matrix cores, one 256-thread block (4 waves):
+//   acc[rb][c] (16 rows x 16 cols) += sum_K A(rb, K) * B[K][c]
+// for the wave's RB row blocks (A supplied by aop(rb, K), K the global row
+// index of B, evaluated by lane (li, lk) for its row li of block rb and its K
+// slot) and the CB column blocks of the tile.  B (Kp rows of 16*CB complex,
+// row stride ldb) is streamed through LDS in KC-row chunks (buf0/buf1, KC*16*CB
+// complex each) with a register prefetch of the next chunk.  Kp % KC == 0.
+template <int CB, int RB, int KC, typename AOp>
+__device__ __forceinline__ void gemm_core_f64(const double2 *__restrict__ B, int ldb, int Kp, AOp aop,
+                                              double2 *buf0, double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+    constexpr int CP = 16 * CB;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) { cr[rb][c] = d4{0, 0, 0, 0}; ci[rb][c] = d4{0, 0, 0, 0}; }
+    constexpr int NPF = (KC * CP + 255) / 256;
+    constexpr int chunk = KC * CP;
+    for (int i = tid; i < chunk; i += 256) buf0[i] = B[(size_t)(i / CP) * ldb + i % CP];
+    __syncthreads();
+    const int nchunks = Kp / KC;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        double2 pf[NPF];
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            const double2 *src = B + (size_t)(ch + 1) * KC * ldb;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < chunk) pf[j] = src[(size_t)(i / CP) * ldb + i % CP];
+            }
+        }
+        const double2 *Bs = (ch & 1) ? buf1 : buf0;
+#pragma unroll 2
+        for (int ks = 0; ks < KC / 4; ++ks) {
+            const int kl = ks * 4 + lk;
+            double ar[RB], ai[RB], nai[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const double2 t = aop(rb, ch * KC + kl);
+                ar[rb] = t.x;
+                ai[rb] = t.y;
+                nai[rb] = -t.y;
+            }
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const double2 b = Bs[kl * CP + c * 16 + li];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], b.x, b.y, cr[rb][c], ci[rb][c]);
+            }
+        }
+        if (more) {
+            double2 *dst = (ch & 1) ? buf0 : buf1;
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int i = tid + 256 * j;
+                if (i < chunk) dst[i] = pf[j];
+            }
+        }
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ int pos_mod(int x, int L) {

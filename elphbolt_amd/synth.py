@@ -60,7 +60,7 @@ class Material:
     gwsdeg: np.ndarray            # (nwsg,) int64
     Hwann: np.ndarray             # (nwsk, nb, nb) complex128, Ry
     Dphwann: np.ndarray           # (nwsq, nbr, nbr) complex128
-    gwann: np.ndarray             # (nb, nb, nwsk, nbr, nwsg) complex128, Fortran order
+    gwann: Optional[np.ndarray]   # (nb, nb, nwsk, nbr, nwsg) complex128, Fortran order; None: made in HBM
     # window / mesh settings that come with the config
     wvmesh: np.ndarray = dataclasses.field(default_factory=lambda: np.array([4, 4, 4], np.int64))
     mesh_ref: int = 1
@@ -69,6 +69,11 @@ class Material:
     polar: bool = False
     epsilon: Optional[np.ndarray] = None      # (3,3)
     born: Optional[np.ndarray] = None         # (3,3,natoms)
+    vertex_seed: Optional[int] = None         # seed of the device-generated vertex (gwann None)
+
+    @property
+    def gwann_bytes(self) -> int:
+        return 16 * self.nb * self.nb * self.nbr * self.nwsk * self.nwsg
 
     @property
     def nb(self) -> int:
@@ -278,7 +283,10 @@ def choose_window(mat: Material, bands_in_window: float, rng_seed: int, fsthick=
 
 def make_material(name: str, a_angstrom: float, masses, nb: int, ws_mesh, wvmesh,
                   seed: int, mesh_ref: int = 1, fsthick=None, bands_in_window=4.0,
-                  basis=None) -> Material:
+                  basis=None, vertex: str = "host", window: bool = True) -> Material:
+    """vertex "host": gwann generated here (make_vertex); "device": left None
+    and generated in HBM by the library (elph_synth_vertex, same amplitude law,
+    different random phases).  window False skips choose_window."""
     lat = fcc_lattice(a_angstrom)
     masses = np.asarray(masses, np.float64)
     nat = masses.shape[0]
@@ -290,13 +298,15 @@ def make_material(name: str, a_angstrom: float, masses, nb: int, ws_mesh, wvmesh
     cells, degs = ws_cells(lat, ws_mesh)
     H = make_hamiltonian(cells, lat, nb, seed)
     D = make_force_constants(cells, lat, basis, masses, ws_mesh)
-    g = make_vertex(cells, cells, lat, nb, 3 * nat, seed)
+    g = make_vertex(cells, cells, lat, nb, 3 * nat, seed) if vertex == "host" else None
     mat = Material(name=name, lattvecs=lat, basis=basis, masses=masses,
                    coarse_qmesh=ws_mesh.copy(),
                    rcells_k=cells, elwsdeg=degs, rcells_q=cells.copy(), phwsdeg=degs.copy(),
                    rcells_g=cells.copy(), gwsdeg=degs.copy(), Hwann=H, Dphwann=D, gwann=g,
-                   wvmesh=np.asarray(wvmesh, np.int64), mesh_ref=mesh_ref)
-    mat.enref, mat.fsthick = choose_window(mat, bands_in_window, seed, fsthick)
+                   wvmesh=np.asarray(wvmesh, np.int64), mesh_ref=mesh_ref,
+                   vertex_seed=None if g is not None else seed)
+    if window:
+        mat.enref, mat.fsthick = choose_window(mat, bands_in_window, seed, fsthick)
     return mat
 
 
@@ -313,6 +323,39 @@ def config(idx: int) -> Material:
         return make_material("sic_like_ws8_k24", 4.36, [28.0855, 12.0107], 8,
                              [8, 8, 8], [24, 24, 24], SEED_BASE + 1, bands_in_window=4.0)
     raise ValueError(f"unknown config {idx}")
+
+
+MULTI_SEED_BASE = 0xE1B03000
+_MASSES = [28.0855, 12.0107, 15.999, 24.305, 69.723, 74.92, 10.81, 30.97, 26.98, 14.007, 72.63, 118.71]
+
+
+def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", window: bool = True,
+                   seed_base: int = MULTI_SEED_BASE) -> Material:
+    """Material i of the multi-material generator: atoms per cell uniform in
+    1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
+    uniform in 4..9 (about 150..900 cells), fine mesh from {12, 16, 20, 24}^3.
+    Draws are repeated (same stream) until 16 nb^2 nbr nws^2 <= budget_bytes.
+    The vertex follows the law of config 0 (device-generated by default)."""
+    rng = np.random.default_rng(seed_base + i)
+    for _ in range(10000):
+        nat = int(rng.integers(1, 9))
+        nb = int(rng.integers(4, 27))
+        n = int(rng.integers(4, 10))
+        wv = int(rng.choice([12, 16, 20, 24]))
+        a = float(rng.uniform(4.0, 6.0))
+        cells, _ = ws_cells(fcc_lattice(a), np.array([n, n, n], np.int64))
+        nws = cells.shape[0]
+        if 16 * nb * nb * 3 * nat * nws * nws <= budget_bytes:
+            break
+    else:
+        raise ValueError("no draw within the budget")
+    masses = [_MASSES[j % len(_MASSES)] for j in range(nat)]
+    basis = np.zeros((3, nat))
+    for j in range(nat):
+        basis[:, j] = (j / nat) * np.ones(3) + 0.05 * rng.standard_normal(3)
+    return make_material(f"multi{i}_nb{nb}_na{nat}_ws{n}_k{wv}", a, masses, nb, [n] * 3, [wv] * 3,
+                         seed_base + i, basis=basis, vertex=vertex, window=window,
+                         bands_in_window=nb / 2.0)
 
 
 def tiny(nb=2, natoms=1, ws=3, wv=4, seed=7, bands_in_window=None, mesh_ref=1) -> Material:

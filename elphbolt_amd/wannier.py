@@ -56,8 +56,9 @@ class Wannier:
                 e = f64(mat.epsilon).ravel(order="F")
                 for i in range(9):
                     d.epsilon[i] = e[i]
+            gw = None if mat.gwann is None else c128(mat.gwann)
             keep = [i64(mat.rcells_k), i64(mat.rcells_q), i64(mat.rcells_g), i64(mat.elwsdeg), i64(mat.phwsdeg),
-                    i64(mat.gwsdeg), c128(mat.Hwann), c128(mat.Dphwann), c128(mat.gwann), f64(mat.basis),
+                    i64(mat.gwsdeg), c128(mat.Hwann), c128(mat.Dphwann), gw, f64(mat.basis),
                     f64(mat.masses), f64(mat.born) if d.polar else None]
             ctx = ct.c_void_p()
             st = self.lib.elph_ctx_create(ct.byref(ctx), device, ct.byref(d), *[_p(a, c_i64p) for a in keep[:6]],
@@ -65,6 +66,9 @@ class Wannier:
             if st != 0:
                 raise ElphError(st, "elph_ctx_create failed")
             self.ctx = ctx
+            if mat.gwann is None:
+                # vertex generated in HBM (synth.make_material(..., vertex="device"))
+                self._check(self.lib.elph_synth_vertex(ctx, int(mat.vertex_seed) & (2 ** 64 - 1)), "synth_vertex")
         dims = np.zeros(6, np.int64)
         self.lib.elph_ctx_dims(self.ctx, _p(dims, c_i64p))
         (self.numwannbands, self.numbranches, self.nwsk, self.nwsq, self.nwsg, self.natoms) = map(int, dims)

@@ -194,6 +194,12 @@ int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t byt
  * boundary after a driver, src/interactions.f90:2387). */
 int elph_ctx_synchronize(elph_ctx *ctx);
 
+/* Fills the context's Wannier vertex in HBM with the synthetic law of the
+ * benchmark workloads (|g| = 0.1 eV e^{-|Re|/2 A - |Rp|/2 A}, phase uniform in
+ * [0, 2 pi) from a counter-based hash of (seed, element)); pass gwann = NULL to
+ * elph_ctx_create to skip the host copy.  No counterpart in the reference. */
+int elph_synth_vertex(elph_ctx *ctx, uint64_t seed);
+
 /* Checks the fp64 MFMA operand layout the kernels assume (returns max abs
  * error of a 16x16x4 product against the host; the device must be present). */
 int elph_selftest_mfma(int device, double *max_err);
@@ -202,6 +208,49 @@ int elph_selftest_mfma(int device, double *max_err);
  * HIP events on the stream the kernels ran on: [0] sum over launches, [1]
  * number of launches, [2] sum of all kernels of run(). */
 int elph_last_timing(const elph_ctx *ctx, double out[3]);
+
+/* ------------------------------------------------------------------------
+ * Serving (no counterpart in the reference; not called by the Fortran side).
+ * A server keeps the Wannier tensors of its materials (contexts) resident and
+ * answers requests for |g|^2 at arbitrary k- and q-lists: for every k of the
+ * list, every q of the list and all m, n, s (the g2 'ph' form of
+ * src/wannier.f90:903-1011 with U_k, U_k+q from el_wann, e_q from ph_wann and
+ * g(k,Rp) from gkRp), as FP32 eV^2 in the order
+ *   out[(((ik * nq + iq) * nb + m) * nb + n) * nbr + s].
+ * Requests are admitted first come first served while their device data
+ * (lists, phonons, phases, output) fits in the pool given at creation, and
+ * are executed by a scheduler thread as work units of up to 64 k-points of
+ * one material taken from one or more requests, on nslots streams at once.
+ * precision: ELPH_FP64 or ELPH_MIXED (int8 signed-digit phase contractions).
+ * All calls are thread safe.  Materials are added before the first submit and
+ * the contexts must outlive the server.
+ * ------------------------------------------------------------------------ */
+typedef struct elph_server elph_server;
+typedef struct {
+    int64_t completed, failed, pending, active;
+    int64_t vertices;        /* |g|^2 values of completed requests */
+    int64_t units;           /* work units executed */
+    int64_t pool_bytes, pool_used;
+    int64_t max_slots_busy;
+    double busy_span_ms;     /* first submit to last completion */
+} elph_server_stats_t;
+int elph_server_create(elph_server **out, int device, int32_t precision, int64_t pool_bytes, int32_t nslots);
+/* returns the material id (>= 0) or -status */
+int elph_server_add_material(elph_server *s, elph_ctx *ctx);
+/* kvecs (nk,3), qvecs (nq,3) column-major crystal coordinates; returns the
+ * request id (>= 0) or -status (ELPH_ENOMEM: larger than the pool) */
+int64_t elph_server_submit(elph_server *s, int32_t material, int64_t nk, const double *kvecs, int64_t nq,
+                           const double *qvecs);
+/* blocks until the request is done; returns its status (ELPH_ENEGPHONON when
+ * ph_wann at its q-list found omega < -0.005 eV); latency = completion - submit */
+int elph_server_wait(elph_server *s, int64_t id, double *latency_ms);
+int elph_server_test(elph_server *s, int64_t id);  /* 1 done, 0 not yet, <0 error */
+int elph_server_fetch(elph_server *s, int64_t id, float *out);  /* copy the result to host */
+int elph_server_result_device(elph_server *s, int64_t id, const float **dptr, int64_t *count);
+int elph_server_release(elph_server *s, int64_t id);  /* frees the request's device data */
+int elph_server_stats(elph_server *s, elph_server_stats_t *stats);
+const char *elph_server_error(const elph_server *s);
+int elph_server_destroy(elph_server *s);
 
 #ifdef __cplusplus
 }
