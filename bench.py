@@ -10,7 +10,12 @@ dominant kernel (fused phase-contraction + rotation + |g|^2 kernel) from HIP
 events on its stream, and the CPU reference method (the oracle) timed on a
 bounded sample on this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision fp64|mixed]
+Serving (north-star pattern 3): a Poisson stream of |g|^2 requests at arbitrary
+k- and q-lists over resident materials of the multi-material generator,
+served with continuous batching by the library's server in mixed precision
+and in plain FP64; p50/p99 latency and sustained vertices/s in "serving".
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-serving] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -80,6 +85,130 @@ def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
                       f"{nvert} vertices in {dt:.1f} s"}
 
 
+SERVING_SEED = 0xE1B04000
+
+
+def serving_requests(nreq, rate, sigma, nmat, seed=SERVING_SEED):
+    """Seed-fixed request stream: Poisson arrivals at `rate`/s, material id
+    uniform, k- and q-list lengths independent lognormal (median 500, log-sd
+    sigma, clamped to [1, 20000]), coordinates uniform in [0, 1)^3."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    mid = rng.integers(0, nmat, nreq)
+    nk = np.clip(np.rint(np.exp(np.log(500.0) + sigma * rng.standard_normal(nreq))), 1, 20000).astype(int)
+    nq = np.clip(np.rint(np.exp(np.log(500.0) + sigma * rng.standard_normal(nreq))), 1, 20000).astype(int)
+    return [dict(t=float(t[i]), mat=int(mid[i]), k=rng.random((nk[i], 3)), q=rng.random((nq[i], 3)))
+            for i in range(nreq)]
+
+
+def serve_stream(srv, mids, reqs):
+    """Submits `reqs` on their arrival schedule from a thread; waits for each
+    in order and releases it (the result is consumed in HBM).  Returns the
+    latencies (arrival to completion) of completed requests and the number
+    refused at admission (larger than the pool)."""
+    import threading
+    from elphbolt_amd._lib import ElphError
+    ids = [None] * len(reqs)
+    ready = [threading.Event() for _ in reqs]
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            dt = t0 + r["t"] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            try:
+                ids[i] = srv.submit(mids[r["mat"]], r["k"], r["q"])
+            except ElphError:
+                ids[i] = -1
+            ready[i].set()
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lat, refused, failed = [], 0, 0
+    for i in range(len(reqs)):
+        ready[i].wait()
+        if ids[i] < 0:
+            refused += 1
+            continue
+        st, ms = srv.wait(ids[i])
+        srv.release(ids[i])
+        if st == 0:
+            lat.append(ms)
+        else:
+            failed += 1
+    th.join()
+    return lat, refused, failed
+
+
+def serving_bench(args):
+    """Serving form: resident materials from the multi-material generator, a
+    Poisson request stream, continuous batching in the library's server; the
+    same stream through the mixed and the plain FP64 server."""
+    from elphbolt_amd import synth
+    from elphbolt_amd.serving import Server
+    from elphbolt_amd.wannier import Wannier
+    t0 = time.perf_counter()
+    mats = [synth.multi_material(i, budget_bytes=args.serving_budget_gb * 1e9, window=False)
+            for i in range(args.serving_materials)]
+    ws = [Wannier(m) for m in mats]
+    gb = sum(m.gwann_bytes for m in mats) / 1e9
+    log(f"[bench] serving: {len(mats)} materials, {gb:.1f} GB of Wannier vertex in HBM "
+        f"({time.perf_counter() - t0:.1f} s)")
+    reqs = serving_requests(args.serving_requests, args.serving_rate, args.serving_sigma, len(mats))
+    probe = [(m, np.random.default_rng(7 + m).random((8, 3)), np.random.default_rng(9 + m).random((8, 3)))
+             for m in range(min(3, len(mats)))]
+    out = {"workload": f"{len(mats)} resident materials of the multi-material generator (seeds "
+                       f"{synth.MULTI_SEED_BASE:#x}+i, vertex cap {args.serving_budget_gb:g} GB each): "
+                       + ", ".join(m.name for m in mats),
+           "resident_vertex_gb": round(gb, 2), "requests": len(reqs), "arrival_rate_per_s": args.serving_rate,
+           "list_lengths": f"k and q independent lognormal, median 500, log-sd {args.serving_sigma:g}, "
+                           f"clamped [1, 20000]; every (k, q) pair, all m, n, s; |g|^2 FP32",
+           "pool_gb": args.serving_pool_gb, "slots": args.serving_slots}
+    results = {}
+    for prec in ("mixed", "fp64"):
+        with Server(prec, pool_bytes=int(args.serving_pool_gb * (1 << 30)), nslots=args.serving_slots) as srv:
+            mids = [srv.add_material(w) for w in ws]
+            pr = []
+            for m, k, q in probe:   # warm-up and accuracy probe
+                rid = srv.submit(mids[m], k, q)
+                srv.wait(rid)
+                pr.append(srv.fetch(rid, release=True))
+            base = srv.stats()
+            t1 = time.perf_counter()
+            lat, refused, failed = serve_stream(srv, mids, reqs)
+            wall = time.perf_counter() - t1
+            st = srv.stats()
+        vert = st["vertices"] - base["vertices"]
+        span = reqs[-1]["t"] if reqs else 0.0
+        lat = np.asarray(lat)
+        res = {"completed": int(lat.size), "refused_larger_than_pool": refused, "failed": failed,
+               "p50_latency_ms": float(np.percentile(lat, 50)) if lat.size else None,
+               "p99_latency_ms": float(np.percentile(lat, 99)) if lat.size else None,
+               "mean_latency_ms": float(lat.mean()) if lat.size else None,
+               "sustained_vertices_per_s": vert / wall, "wall_s": wall, "vertices": vert,
+               "achieved_requests_per_s": lat.size / wall, "offered_span_s": span,
+               "max_slots_busy": st["max_slots_busy"], "units": st["units"] - base["units"]}
+        results[prec] = (res, pr)
+        log(f"[bench] serving {prec}: {res['completed']} done, {refused} refused, p50 {res['p50_latency_ms']:.0f} ms "
+            f"p99 {res['p99_latency_ms']:.0f} ms, {res['sustained_vertices_per_s']:.3e} vertices/s, "
+            f"wall {wall:.1f} s")
+    rel = []
+    for a, b in zip(results["mixed"][1], results["fp64"][1]):
+        a, b = a.astype(np.float64).ravel(), b.astype(np.float64).ravel()
+        sel = b > 1e-6 * b.max()
+        rel.append(np.abs(a[sel] - b[sel]) / b[sel])
+    rel = np.concatenate(rel)
+    out["mixed"] = results["mixed"][0]
+    out["fp64"] = results["fp64"][0]
+    out["mixed_vs_fp64_sustained"] = round(out["mixed"]["sustained_vertices_per_s"]
+                                           / out["fp64"]["sustained_vertices_per_s"], 3)
+    out["error_mixed_vs_fp64"] = {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
+                                  "entries": int(rel.size), "floor": "1e-6*max|g|^2",
+                                  "probe": f"{len(probe)} requests of 8 k x 8 q", "bound_met": bool(rel.max() <= 1e-5)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +218,14 @@ def main():
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-sigma", type=float, default=1.0)
+    ap.add_argument("--serving-materials", type=int, default=8)
+    ap.add_argument("--serving-budget-gb", type=float, default=8.0)
+    ap.add_argument("--serving-pool-gb", type=float, default=128.0)
+    ap.add_argument("--serving-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -196,6 +333,8 @@ def main():
                       "vertices_per_step": res_mx["vertices"] // args.steps},
            "mixed_vs_fp64": round(res_mx["value"] / res_64["value"], 3), "error_vs_fp64": err,
            "fp64": fp64, "roofline": roofline}
+    if not args.no_serving:
+        res["serving"] = serving_bench(args)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         first_active = None

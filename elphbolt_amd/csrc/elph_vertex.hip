@@ -1652,8 +1652,12 @@ void launch_jacobi_n(hipStream_t st, int64_t nmat, int n, const double2 *A, cons
 
 void elph::launch_jacobi(hipStream_t st, int64_t nmat, int n, const double2 *A, const double *scale, double *W,
                          double2 *V) {
+    // NMAX sizes the LDS copies of A and V (one wave per matrix), so the
+    // smallest fitting size keeps the most waves resident per CU
     if (n <= 8) launch_jacobi_n<8>(st, nmat, n, A, scale, W, V);
+    else if (n <= 12) launch_jacobi_n<12>(st, nmat, n, A, scale, W, V);
     else if (n <= 16) launch_jacobi_n<16>(st, nmat, n, A, scale, W, V);
+    else if (n <= 24) launch_jacobi_n<24>(st, nmat, n, A, scale, W, V);
     else if (n <= 32) launch_jacobi_n<32>(st, nmat, n, A, scale, W, V);
     else throw Error{ELPH_EINVAL, "eigensolver supports matrices up to 32x32"};
 }

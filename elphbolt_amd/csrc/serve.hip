@@ -102,12 +102,17 @@ __global__ void k_ph_post(int nq, int nbr, const double *__restrict__ qv, int ld
     if (bad) *flag = 1;  // vector store; every writer stores the same value
 }
 
-// Phi[q][K] = e^{i 2 pi (q . R_K - nint(q . R_K))} for K < nwsg, 0 for the padding.
-__global__ void k_srv_phase(int nq, int Kp, int nwsg, const double *__restrict__ qv, int ldq,
+// PhiT[K][q] = e^{i 2 pi (q . R_K - nint(q . R_K))} for K < nwsg, 0 for the
+// padding; K-major with row stride ldp (columns q >= nq are 0).
+__global__ void k_srv_phase(int nq, int ldp, int Kp, int nwsg, const double *__restrict__ qv, int ldq,
                             const int *__restrict__ cells, double2 *__restrict__ phi) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nq * Kp) return;
-    const int q = (int)(t / Kp), k = (int)(t % Kp);
+    if (t >= (int64_t)ldp * Kp) return;
+    const int k = (int)(t / ldp), q = (int)(t % ldp);
+    if (q >= nq) {
+        phi[t] = make_double2(0.0, 0.0);
+        return;
+    }
     double2 v = make_double2(0.0, 0.0);
     if (k < nwsg) {
         const double x = qv[q] * (double)cells[3 * k] + qv[q + ldq] * (double)cells[3 * k + 1] +
@@ -122,7 +127,7 @@ __global__ void k_srv_phase(int nq, int Kp, int nwsg, const double *__restrict__
 // int8 digits of Phi in the A-operand order of the 16x16x64 int8 MFMA:
 // adig[((qb * nkc + kc) * 2 + part) * kNS + d][lane], lane = i + 16 g holding
 // rows qb*16 + i and K = kc*64 + 16 g .. +15; part 0 = Re, 1 = Im; rows >= nq are 0.
-__global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int Kp, const double2 *__restrict__ phi,
+__global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const double2 *__restrict__ phi,
                                    v4i *__restrict__ adig) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nqb * nkc * 64) return;
@@ -133,7 +138,7 @@ __global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int Kp, const doubl
     union { signed char b[16]; v4i v; } ur[kNS], ui[kNS];
     for (int b = 0; b < 16; ++b) {
         double2 x = make_double2(0.0, 0.0);
-        if (q < nq) x = phi[(size_t)q * Kp + kc * 64 + g * 16 + b];
+        if (q < nq) x = phi[(size_t)(kc * 64 + g * 16 + b) * ldp + q];
         signed char dr[kNS], di[kNS];
         split_digits<kNS>(x.x, dr);
         split_digits<kNS>(x.y, di);
@@ -151,28 +156,83 @@ __global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int Kp, const doubl
     }
 }
 
-// Y[st][r][col] = sum_K Phi[q0 + r][K] h[st0 + st][K][col] (FP64 matrix cores).
+// Y[st][r][col] = sum_K PhiT[K][q0 + r] h[st0 + st][K][col] (FP64 matrix cores).
 // Block: 4 waves x 32 rows; grid (rows / 128, states, column tiles of 16*CB).
+// Each K chunk of both operands (128 phase rows, K-major, and the 16*CB
+// columns of h) is staged in LDS; the next chunk is loaded into registers
+// while the current one computes.
 template <int CB>
 __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ h, int Kp, int CPT, int st0,
-                                                    const double2 *__restrict__ phi, int q0, int nqs,
+                                                    const double2 *__restrict__ phiT, int ldp, int q0, int nqs,
                                                     double2 *__restrict__ Y) {
-    constexpr int KC = 32, RB = 2;
+    constexpr int KC = 16, RB = 2, CP = 16 * CB, ROWS = 128;
+    constexpr int NA = KC * ROWS / 256, NB = (KC * CP + 255) / 256;
     extern __shared__ double2 smem[];
-    double2 *buf0 = smem, *buf1 = smem + KC * 16 * CB;
+    double2 *As = smem, *Bs = smem + KC * ROWS;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const int st = blockIdx.y, col0 = blockIdx.z * 16 * CB;
-    const int row_base = blockIdx.x * 128 + wave * 32;
-    const double2 *arow[RB];
+    const int st = blockIdx.y, col0 = blockIdx.z * CP;
+    const int row0 = blockIdx.x * ROWS;
+    const double2 *B = h + (size_t)(st0 + st) * Kp * CPT + col0;
+    double2 pa[NA], pb[NB];
+    auto load = [&](int ch) {
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
-        const int r = row_base + rb * 16 + li;
-        arow[rb] = r < nqs ? phi + (size_t)(q0 + r) * Kp : nullptr;
-    }
-    auto aop = [&](int rb, int k) { return arow[rb] ? arow[rb][k] : make_double2(0.0, 0.0); };
+        for (int j = 0; j < NA; ++j) {
+            const int i = tid + 256 * j, k = i / ROWS, r = i % ROWS;
+            pa[j] = (row0 + r < nqs) ? phiT[(size_t)(ch * KC + k) * ldp + q0 + row0 + r] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int i = tid + 256 * j;
+            if (i < KC * CP) pb[j] = B[(size_t)(ch * KC + i / CP) * CPT + i % CP];
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < NA; ++j) As[tid + 256 * j] = pa[j];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int i = tid + 256 * j;
+            if (i < KC * CP) Bs[i] = pb[j];
+        }
+    };
     d4 cr[RB][CB], ci[RB][CB];
-    gemm_core_f64<CB, RB, KC>(h + (size_t)(st0 + st) * Kp * CPT + col0, CPT, Kp, aop, buf0, buf1, cr, ci);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c) { cr[rb][c] = d4{0, 0, 0, 0}; ci[rb][c] = d4{0, 0, 0, 0}; }
+    load(0);
+    store();
+    __syncthreads();
+    const int nchunks = Kp / KC;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const bool more = ch + 1 < nchunks;
+        if (more) load(ch + 1);
+#pragma unroll
+        for (int ks = 0; ks < KC / 4; ++ks) {
+            const int kl = ks * 4 + lk;
+            double ar[RB], ai[RB], nai[RB];
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) {
+                const double2 t = As[kl * ROWS + wave * 32 + rb * 16 + li];
+                ar[rb] = t.x;
+                ai[rb] = t.y;
+                nai[rb] = -t.y;
+            }
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const double2 bv = Bs[kl * CP + c * 16 + li];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], bv.x, bv.y, cr[rb][c], ci[rb][c]);
+            }
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
+    }
+    const int row_base = row0 + wave * 32;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -183,6 +243,45 @@ __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ 
                 if (row < nqs)
                     Y[((size_t)st * nqs + row) * CPT + col0 + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
             }
+}
+
+// h[(k * nb + m)][rp][c] = sum_mp U_k(m,mp) gk[k](np, mp, sp, rp) / phwsdeg(rp) for
+// all bands m (c = np + nb sp; zero for rp >= nwsg or c >= nb nbr).  One
+// thread per (k, rp, c) reads its g column once and writes all nb rows.
+template <int NB>
+__global__ __launch_bounds__(256) void k_rotate_all(int nb, int nbr, int nwsg, int Kp, int CPT,
+                                                    const double2 *__restrict__ gk, int64_t kstride,
+                                                    const double2 *__restrict__ Uk, const int *__restrict__ degq,
+                                                    double2 *__restrict__ h) {
+    __shared__ double2 U[NB * NB];
+    const int k = blockIdx.y;
+    for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) U[i] = Uk[(size_t)k * nb * nb + i];
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)Kp * CPT) return;
+    const int rp = (int)(t / CPT), c = (int)(t % CPT);
+    double2 g[NB];
+    const bool live = rp < nwsg && c < nb * nbr;
+    if (live) {
+        const int np = c % nb, sp = c / nb;
+        const double2 *src = gk + (size_t)k * kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
+        const double inv = 1.0 / (double)degq[rp];
+#pragma unroll
+        for (int mp = 0; mp < NB; ++mp) {
+            g[mp] = mp < nb ? src[(size_t)mp * nb] : make_double2(0.0, 0.0);
+            g[mp].x *= inv;
+            g[mp].y *= inv;
+        }
+    }
+    double2 *dst = h + ((size_t)k * nb * Kp + rp) * CPT + c;
+    for (int m = 0; m < nb; ++m) {
+        double2 acc = make_double2(0.0, 0.0);
+        if (live)
+#pragma unroll
+            for (int mp = 0; mp < NB; ++mp)
+                if (mp < nb) cfma(acc, U[m * nb + mp], g[mp]);
+        dst[(size_t)m * Kp * CPT] = acc;
+    }
 }
 
 // Mixed-precision form of k_srv_y_fp64: kNS int8 digits of Phi (adig) and of
@@ -422,7 +521,6 @@ struct Material {
     int64_t nb = 0, nbr = 0, nb2 = 0, Kp = 0, nkc = 0, CB = 1, CPT = 16, KT = 1;
     int64_t gk_k = 0;  // complex per k of g(k,Rp)
     DBuf<double> mscale, invsqrtm;
-    DBuf<int> st_kb, st_m;  // unit states st = k * nb + m
 };
 
 struct Request {
@@ -612,13 +710,13 @@ bool admit(elph_server *s) {
         polar_launch(c, st, nq, r.d_qv, nullptr, nullptr, r.d_Lq);
         launch_polar_lp(st, (int)(nq * nbr), (int)nbr, (int)nbr, nullptr, r.d_Lq, r.d_V, r.d_lp);
     }
-    const int64_t np = nq * m.Kp;
-    hipLaunchKernelGGL(k_srv_phase, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, (int)nq, (int)m.Kp,
-                       (int)c->nwsg, r.d_qv, (int)nq, c->rq.p, r.d_phi);
+    const int64_t np = r.nq64 * m.Kp;
+    hipLaunchKernelGGL(k_srv_phase, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, (int)nq, (int)r.nq64,
+                       (int)m.Kp, (int)c->nwsg, r.d_qv, (int)nq, c->rq.p, r.d_phi);
     if (s->precision == ELPH_MIXED) {
         const int64_t nt = (r.nq64 / 16) * m.nkc * 64;
         hipLaunchKernelGGL(k_srv_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)nq,
-                           (int)(r.nq64 / 16), (int)m.nkc, (int)m.Kp, r.d_phi, r.d_adig);
+                           (int)(r.nq64 / 16), (int)m.nkc, (int)r.nq64, r.d_phi, r.d_adig);
     }
     HCHECK(hipMemcpyAsync(r.h_flag, r.d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
     HCHECK(hipGetLastError());
@@ -670,11 +768,20 @@ bool form_unit(elph_server *s, int &mat, std::vector<Segment> &segs) {
 }
 
 template <int CB>
-void launch_y_fp64(hipStream_t st, const Material &m, const double2 *h, int st0, int nst, const double2 *phi, int q0,
-                   int nqs, double2 *Y) {
+void launch_y_fp64(hipStream_t st, const Material &m, const double2 *h, int st0, int nst, const double2 *phi, int ldp,
+                   int q0, int nqs, double2 *Y) {
     dim3 grid((unsigned)((nqs + 127) / 128), (unsigned)nst, (unsigned)(m.CPT / (16 * CB)));
-    const size_t lds = (size_t)2 * 32 * 16 * CB * sizeof(double2);
-    hipLaunchKernelGGL(k_srv_y_fp64<CB>, grid, dim3(256), lds, st, h, (int)m.Kp, (int)m.CPT, st0, phi, q0, nqs, Y);
+    const size_t lds = (size_t)16 * (128 + 16 * CB) * sizeof(double2);
+    hipLaunchKernelGGL(k_srv_y_fp64<CB>, grid, dim3(256), lds, st, h, (int)m.Kp, (int)m.CPT, st0, phi, ldp, q0, nqs,
+                       Y);
+}
+
+template <int NB>
+void launch_rotate_all(hipStream_t st, const Material &m, int nk, const double2 *gk, const double2 *Uk, double2 *h) {
+    const int64_t per = m.Kp * m.CPT;
+    dim3 grid((unsigned)((per + 255) / 256), (unsigned)nk);
+    hipLaunchKernelGGL(k_rotate_all<NB>, grid, dim3(256), 0, st, (int)m.nb, (int)m.nbr, (int)m.c->nwsg, (int)m.Kp,
+                       (int)m.CPT, gk, m.gk_k, Uk, m.c->degq.p, h);
 }
 
 template <int CB>
@@ -706,8 +813,10 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
     launch_jacobi(st, nk, (int)nb, sl.H.p, nullptr, sl.W.p, sl.Uk.p);
     launch_gkRp(c, st, nk, sl.kv.p, sl.gk.p, m.gk_k, mixed, sl.phiT.p, sl.bphi.p);
     const int nst = (int)(nk * nb);
-    launch_rotate_h(st, nst, (int)nb, (int)nbr, (int)c->nwsg, (int)m.Kp, (int)m.CPT, sl.gk.p, m.gk_k, m.st_kb.p,
-                    m.st_m.p, sl.Uk.p, c->degq.p, sl.h.p);
+    if (nb <= 8) launch_rotate_all<8>(st, m, (int)nk, sl.gk.p, sl.Uk.p, sl.h.p);
+    else if (nb <= 16) launch_rotate_all<16>(st, m, (int)nk, sl.gk.p, sl.Uk.p, sl.h.p);
+    else if (nb <= 24) launch_rotate_all<24>(st, m, (int)nk, sl.gk.p, sl.Uk.p, sl.h.p);
+    else launch_rotate_all<32>(st, m, (int)nk, sl.gk.p, sl.Uk.p, sl.h.p);
     if (mixed) {
         launch_colscale(st, nst, (int)m.Kp, (int)m.CPT, sl.h.p, sl.bscale.p);
         launch_slice_b(st, nst, (int)m.nkc, (int)(m.CPT / 16), (int)m.Kp, (int)m.CPT, sl.h.p, sl.bscale.p, sl.bsl.p);
@@ -737,10 +846,10 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
                 }
             } else {
                 switch (m.CB) {
-                    case 1: launch_y_fp64<1>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
-                    case 2: launch_y_fp64<2>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
-                    case 3: launch_y_fp64<3>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
-                    default: launch_y_fp64<4>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
+                    case 1: launch_y_fp64<1>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)r.nq64, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
+                    case 2: launch_y_fp64<2>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)r.nq64, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
+                    case 3: launch_y_fp64<3>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)r.nq64, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
+                    default: launch_y_fp64<4>(st, m, sl.h.p, (int)(st_seg0 + s0), ns, r.d_phi, (int)r.nq64, (int)sg.q0, (int)sg.nqs, sl.Y.p); break;
                 }
             }
             EpiArgs a{};
@@ -912,7 +1021,7 @@ int elph_server_add_material(elph_server *s, elph_ctx *ctx) {
         m->Kp = rup(ctx->nwsg, 64);
         m->nkc = m->Kp / 64;
         const int64_t cp16 = rup(m->nb * m->nbr, 16) / 16;
-        m->CB = std::min<int64_t>(s->precision == ELPH_MIXED ? 3 : 4, cp16);
+        m->CB = std::min<int64_t>(3, cp16);
         m->CPT = rup(m->nb * m->nbr, 16 * m->CB);
         m->gk_k = m->nb2 * m->nbr * ctx->nwsg;
         const int64_t per_k = std::max(m->gk_k, m->nb * m->Kp * m->CPT) * (int64_t)sizeof(double2);
@@ -921,13 +1030,6 @@ int elph_server_add_material(elph_server *s, elph_ctx *ctx) {
         std::vector<double> ism(m->nbr);
         for (int64_t p = 0; p < m->nbr; ++p) ism[p] = 1.0 / std::sqrt(ctx->masses[p / 3]);
         m->invsqrtm.upload(ism, ctx->stream);
-        std::vector<int> kb(m->KT * m->nb), mm(m->KT * m->nb);
-        for (int64_t i = 0; i < m->KT * m->nb; ++i) {
-            kb[i] = (int)(i / m->nb);
-            mm[i] = (int)(i % m->nb);
-        }
-        m->st_kb.upload(kb, ctx->stream);
-        m->st_m.upload(mm, ctx->stream);
         if (s->precision == ELPH_MIXED) ensure_gwann_digits(ctx);
         HCHECK(hipStreamSynchronize(ctx->stream));
         s->mats.push_back(std::move(m));
