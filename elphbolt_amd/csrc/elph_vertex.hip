@@ -527,6 +527,264 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
 }
 
 // ---------------------------------------------------------------------------
+// General ibzk path for any numwannbands * numbranches: the Rp contraction
+// runs in column tiles of 16*CB (grid.z) and writes Y[st][k'][CPT] to HBM for
+// a chunk of states; k_ibzk_epi then applies the rotations and |g|^2.  Used
+// when nb*nbr exceeds what the fused kernels hold in registers.
+// ---------------------------------------------------------------------------
+
+// phase coefficients of row r for state st: (q . e_d) L / N_d, q = k' - k
+__device__ __forceinline__ void ibzk_coef(const PairArgs &a, int st, int r, int &c0, int &c1, int &c2) {
+    c0 = c1 = c2 = 0;
+    if (r < a.nwv) {
+        c0 = (pos_mod(a.kp_int[3 * r] - a.st_kint[3 * st], a.Nel0) * a.Lf0) % a.L;
+        c1 = (pos_mod(a.kp_int[3 * r + 1] - a.st_kint[3 * st + 1], a.Nel1) * a.Lf1) % a.L;
+        c2 = (pos_mod(a.kp_int[3 * r + 2] - a.st_kint[3 * st + 2], a.Nel2) * a.Lf2) % a.L;
+    }
+}
+
+// FP64: block = 4 waves x 16*RB rows, grid (rows, states of the chunk, column tiles)
+template <int CB, int RB>
+__global__ __launch_bounds__(256) void k_ibzk_y_fp64(PairArgs a, int st0, double2 *__restrict__ Y) {
+    constexpr int KC = 32, CPt = 16 * CB;
+    extern __shared__ double2 smem[];
+    double2 *buf0 = smem, *buf1 = smem + KC * CPt;
+    double2 *Tsh = smem + 2 * KC * CPt;
+    int4 *Rsh = reinterpret_cast<int4 *>(Tsh + a.L);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int stl = blockIdx.y, st = st0 + stl, col0 = blockIdx.z * CPt;
+    const int row_base = blockIdx.x * 64 * RB + wave * 16 * RB;
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
+    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
+    int coef[RB][3];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) ibzk_coef(a, st, row_base + rb * 16 + li, coef[rb][0], coef[rb][1], coef[rb][2]);
+    __syncthreads();
+    d4 cr[RB][CB], ci[RB][CB];
+    const int L = a.L;
+    auto aop = [&](int rb, int k) {
+        const int4 R = Rsh[k];
+        return Tsh[pos_mod(coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z, L)];
+    };
+    gemm_core_f64<CB, RB, KC>(a.h + (size_t)st * a.Kp * a.CP + col0, a.CP, a.Kp, aop, buf0, buf1, cr, ci);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row_base + rb * 16 + lk + 4 * r;
+                if (row < a.nwv)
+                    Y[((size_t)stl * a.nwv + row) * a.CP + col0 + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
+            }
+}
+
+// Mixed: block = 4 waves x 16 rows; A digits generated from the digit table
+// of the roots of unity as in k_pair_i8, B digits of the column tile from bsl
+// ([st][kc][CBT col blocks][Re, Im, -Im][digit][lane]).
+template <int CB>
+__global__ __launch_bounds__(256, 2) void k_ibzk_y_i8(PairArgs a, int st0, const uint2 *__restrict__ Tdig,
+                                                      const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
+                                                      const double *__restrict__ bscale, int nkc, int CBT,
+                                                      double2 *__restrict__ Y) {
+    constexpr int NS = kNS;
+    constexpr int BCH = CB * 3 * NS * 64;
+    constexpr int NPF = (BCH + 255) / 256;
+    extern __shared__ v4i smem8[];
+    v4i *buf0 = smem8, *buf1 = smem8 + BCH;
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
+    unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lg = lane >> 4;
+    const int stl = blockIdx.y, st = st0 + stl, cb0 = blockIdx.z * CB;
+    const int row_base = blockIdx.x * 64 + wave * 16;
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
+    for (int i = tid; i < nkc * 64; i += 256) Rsh[i] = Rpk[i];
+    int c0, c1, c2;
+    ibzk_coef(a, st, row_base + li, c0, c1, c2);
+    const float invL = 1.0f / (float)a.L;
+    const int L = a.L;
+    v4i acc[CB][NS][2];
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
+    const size_t bkc = (size_t)CBT * 3 * NS * 64;
+    const v4i *bst = bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 3 * NS * 64;
+    for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
+    __syncthreads();
+    auto make_a = [&](int kk, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            unsigned tx[4], ty[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const unsigned rp = Rsh[kk * 64 + lg * 16 + 4 * w + b];
+                const int x = __mul24(c0, (int)(rp & 255u)) + __mul24(c1, (int)((rp >> 8) & 255u)) +
+                              __mul24(c2, (int)((rp >> 16) & 255u));
+                int m = x - __mul24(L, (int)((float)x * invL));
+                m = (m >= L) ? m - L : m;
+                m = (m < 0) ? m + L : m;
+                const uint2 t = Tsh[m];
+                tx[b] = t.x;
+                ty[b] = t.y;
+            }
+            unsigned o0, o1, o2, o3;
+            transpose4x4_bytes(tx[0], tx[1], tx[2], tx[3], o0, o1, o2, o3);
+            Ar[0][w] = (int)o0; Ar[1][w] = (int)o1; Ar[2][w] = (int)o2; Ar[3][w] = (int)o3;
+            transpose4x4_bytes(ty[0], ty[1], ty[2], ty[3], o0, o1, o2, o3);
+            Ai[0][w] = (int)o0; Ai[1][w] = (int)o1; Ai[2][w] = (int)o2; Ai[3][w] = (int)o3;
+        }
+    };
+    v4i Ar[NS], Ai[NS];
+    make_a(0, Ar, Ai);
+    for (int kc = 0; kc < nkc; ++kc) {
+        const int kn = min(kc + 1, nkc - 1);
+        const v4i *src = bst + (size_t)kn * bkc;
+        v4i pf[NPF];
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            const int i = tid + 256 * j;
+            if (i < BCH) pf[j] = src[i];
+        }
+        const v4i *B = (kc & 1) ? buf1 : buf0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            v4i Bre[CB], Bim[CB], Bnim[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                Bre[c] = B[((c * 3 + 0) * NS + j) * 64 + lane];
+                Bim[c] = B[((c * 3 + 1) * NS + j) * 64 + lane];
+                Bnim[c] = B[((c * 3 + 2) * NS + j) * 64 + lane];
+            }
+#pragma unroll
+            for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
+                }
+#pragma unroll
+            for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
+                }
+        }
+        v4i nAr[NS], nAi[NS];
+        make_a(kn, nAr, nAi);
+#pragma unroll
+        for (int i = 0; i < NS; ++i) { Ar[i] = nAr[i]; Ai[i] = nAi[i]; }
+        v4i *dst = (kc & 1) ? buf0 : buf1;
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            const int i = tid + 256 * j;
+            if (i < BCH) dst[i] = pf[j];
+        }
+        __syncthreads();
+    }
+    double wl[NS];
+    wl[0] = 1.0 / (127.0 * 127.0);
+#pragma unroll
+    for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
+    const double *sc = bscale + (size_t)st * a.CP + cb0 * 16;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const double s = sc[c * 16 + li];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = row_base + 4 * lg + r;
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int l = NS - 1; l >= 0; --l) {
+                re += wl[l] * (double)acc[c][l][0][r];
+                im += wl[l] * (double)acc[c][l][1][r];
+            }
+            if (row < a.nwv) Y[((size_t)stl * a.nwv + row) * a.CP + (cb0 + c) * 16 + li] = make_double2(re * s, im * s);
+        }
+    }
+}
+
+// Epilogue of the general path: one state and G k' rows per block,
+//   Z(rank, sp) = sum_np conj(U_k'(n_rank, np)) Y(np + nb sp)
+//   g(rank, s)  = sum_sp u_q(s, sp) Z(rank, sp) [+ lp(q,s) sum_i conj(U_k'(n,i)) U_k(m,i)]
+// written like ibzk_epilogue_row.
+__global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const double2 *__restrict__ Y, int G) {
+    extern __shared__ double2 sm[];
+    const int nb = a.nb, nbr = a.nbr, nbnr = nb * nbr;
+    const int TPQ = 256 / G;
+    const int tid = threadIdx.x, sub = tid / TPQ, tt = tid % TPQ;
+    const int stl = blockIdx.y, st = st0 + stl;
+    const int row = blockIdx.x * G + sub;
+    const int per = 2 * nbnr + nbr * nbr + nb;
+    double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *us = Zs + nbnr, *un = us + nbr * nbr;
+    const bool live = sub < G && row < a.nwv;
+    const double2 *uq = nullptr, *lq = nullptr;
+    const double *omq = nullptr;
+    int nn = 0;
+    if (live) {
+        const int q0 = pos_mod(a.kp_int[3 * row] - a.st_kint[3 * st], a.Nel0);
+        const int q1 = pos_mod(a.kp_int[3 * row + 1] - a.st_kint[3 * st + 1], a.Nel1);
+        const int q2 = pos_mod(a.kp_int[3 * row + 2] - a.st_kint[3 * st + 2], a.Nel2);
+        if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
+            const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+            uq = a.u + (size_t)iq * nbr * nbr;
+            omq = a.om + (size_t)iq * nbr;
+            if (a.lp) lq = a.lp + (size_t)iq * nbr;
+        } else {
+            const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+            uq = a.uf + (size_t)iq * nbr * nbr;
+            omq = a.omf + (size_t)iq * nbr;
+            if (a.lpf) lq = a.lpf + (size_t)iq * nbr;
+        }
+        nn = a.nwin[row];
+        const double2 *Yr = Y + ((size_t)stl * a.nwv + row) * a.CP;
+        for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
+        for (int i = tt; i < nbr * nbr; i += TPQ) us[i] = uq[i];
+    }
+    __syncthreads();
+    if (live) {
+        const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * nb + a.st_m[st]) * nb : nullptr;
+        for (int i = tt; i < nn * nbr; i += TPQ) {
+            const int rank = i % nn, sp = i / nn;
+            const double2 *V = a.Vkp + ((size_t)row * nb + a.wband[row * nb + rank]) * nb;
+            double2 z = make_double2(0.0, 0.0);
+            for (int np = 0; np < nb; ++np) {
+                const double2 t = cmulc(V[np], Ys[np + nb * sp]);
+                z.x += t.x;
+                z.y += t.y;
+            }
+            Zs[rank + nb * sp] = z;
+        }
+        if (lq)
+            for (int rank = tt; rank < nn; rank += TPQ) {
+                const double2 *V = a.Vkp + ((size_t)row * nb + a.wband[row * nb + rank]) * nb;
+                double2 z = make_double2(0.0, 0.0);
+                for (int i = 0; i < nb; ++i) {
+                    const double2 t = cmulc(V[i], Ukm[i]);
+                    z.x += t.x;
+                    z.y += t.y;
+                }
+                un[rank] = z;
+            }
+    }
+    __syncthreads();
+    if (live) {
+        double *o = a.out + a.st_slot[st] * a.nprocs + a.offs[row];
+        for (int i = tt; i < nn * nbr; i += TPQ) {
+            const int rank = i / nbr, s = i % nbr;
+            double2 g = make_double2(0.0, 0.0);
+            for (int sp = 0; sp < nbr; ++sp) cfma(g, us[s * nbr + sp], Zs[rank + nb * sp]);
+            if (lq) cfma(g, lq[s], un[rank]);
+            const double w = omq[s];
+            o[(int64_t)rank * nbr + s] = (w == 0.0) ? 0.0 : 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Mixed-precision Re contraction (gkRp): gwann split once per context into
 // kNS int8 digits per (row i = (np,mp), slice = (sp,Rp)) with a per-row scale,
 // phases split per batch; 10 digit products on the int8 matrix cores with
@@ -2286,9 +2544,73 @@ struct elph_ibzk_plan {
     DBuf<unsigned> Rpk;
     DBuf<v4i> bsl;
     DBuf<double> bscale;
+    // general path (nb*nbr beyond the fused kernels): column tile, row tile
+    // (FP64: 64*rb rows per block) and the Y buffer of one chunk of states
+    bool general = false;
+    int gcb = 3, grb = 2;
+    DBuf<double2> Y;
 };
 
 namespace {
+
+template <int CB, int RB>
+void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, double2 *Y) {
+    const size_t lds = (size_t)(2 * 32 * 16 * CB + a.L) * sizeof(double2) + (size_t)a.Kp * sizeof(int4);
+    dim3 grid((unsigned)((a.nwv + 64 * RB - 1) / (64 * RB)), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
+    hipLaunchKernelGGL((k_ibzk_y_fp64<CB, RB>), grid, dim3(256), lds, s, a, st0, Y);
+}
+
+template <int CB>
+void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p, int st0, int ns, double2 *Y) {
+    const size_t lds = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) + (size_t)((a.L + 1) & ~1) * sizeof(uint2) +
+                       (size_t)p->nkc * 64 * sizeof(unsigned);
+    dim3 grid((unsigned)((a.nwv + 63) / 64), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
+    hipLaunchKernelGGL(k_ibzk_y_i8<CB>, grid, dim3(256), lds, s, a, st0, p->Tdig.p, p->Rpk.p, p->bsl.p,
+                       p->bscale.p, (int)p->nkc, (int)(a.CP / 16), Y);
+}
+
+// General path: Y of a chunk of states into HBM, then the epilogue.
+void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst) {
+    elph_ctx *c = p->ctx;
+    hipStream_t s = c->stream;
+    const int64_t per_state = p->nwv * p->CP * (int64_t)sizeof(double2);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per_state));
+    p->Y.alloc(chunk * p->nwv * p->CP);
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
+    const size_t elds = (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb) * sizeof(double2);
+    HCHECK(hipEventRecord(c->ev_a, s));
+    for (int64_t s0 = 0; s0 < nst; s0 += chunk) {
+        const int ns = (int)std::min(chunk, nst - s0);
+        if (p->precision == ELPH_MIXED) {
+            switch (p->gcb) {
+                case 1: launch_ibzk_y_i8<1>(s, a, p, (int)s0, ns, p->Y.p); break;
+                case 2: launch_ibzk_y_i8<2>(s, a, p, (int)s0, ns, p->Y.p); break;
+                default: launch_ibzk_y_i8<3>(s, a, p, (int)s0, ns, p->Y.p); break;
+            }
+        } else {
+            const int key = p->gcb * 10 + p->grb;
+            switch (key) {
+                case 11: launch_ibzk_y_fp64<1, 1>(s, a, (int)s0, ns, p->Y.p); break;
+                case 12: launch_ibzk_y_fp64<1, 2>(s, a, (int)s0, ns, p->Y.p); break;
+                case 21: launch_ibzk_y_fp64<2, 1>(s, a, (int)s0, ns, p->Y.p); break;
+                case 22: launch_ibzk_y_fp64<2, 2>(s, a, (int)s0, ns, p->Y.p); break;
+                case 31: launch_ibzk_y_fp64<3, 1>(s, a, (int)s0, ns, p->Y.p); break;
+                case 32: launch_ibzk_y_fp64<3, 2>(s, a, (int)s0, ns, p->Y.p); break;
+                case 41: launch_ibzk_y_fp64<4, 1>(s, a, (int)s0, ns, p->Y.p); break;
+                default: launch_ibzk_y_fp64<4, 2>(s, a, (int)s0, ns, p->Y.p); break;
+            }
+        }
+        dim3 eg((unsigned)((p->nwv + G - 1) / G), (unsigned)ns);
+        hipLaunchKernelGGL(k_ibzk_epi, eg, dim3(256), elds, s, a, (int)s0, p->Y.p, G);
+        HCHECK(hipGetLastError());
+    }
+    HCHECK(hipEventRecord(c->ev_b, s));
+    HCHECK(hipEventSynchronize(c->ev_b));
+    float ms = 0;
+    HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+    c->t_dominant_ms += ms;
+    c->n_dominant += 1;
+}
 
 void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t nprocs) {
     elph_ctx *c = p->ctx;
@@ -2325,6 +2647,10 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     a.out = out;
     a.nprocs = nprocs;
     const int CB = (int)(p->CP / 16);
+    if (p->general) {
+        general_launch(p, a, nst_batch);
+        return;
+    }
     const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
                                 (size_t)64 * (p->CP + 1) * sizeof(double2));
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
@@ -2399,11 +2725,15 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         for (int d = 0; d < 3; ++d) { p->Nel[d] = el->wvmesh[d]; p->Nph[d] = ph->wvmesh[d]; }
         p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
-        if (nb * nbr > 96) throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported"};
         p->CP = ((nb * nbr + 15) / 16) * 16;
         p->Kp = ((ctx->nwsg + 63) / 64) * 64;
-        if (precision == ELPH_MIXED && p->CP > 64)
-            throw Error{ELPH_EINVAL, "mixed precision supports numwannbands*numbranches <= 64"};
+        // fused kernels hold all nb*nbr columns of a state in registers up to
+        // 96 (FP64) / 64 (mixed); beyond, the general column-tiled path
+        p->general = p->CP > (precision == ELPH_MIXED ? 64 : 96);
+        if (p->general) {
+            p->gcb = std::min<int>(3, (int)(p->CP / 16));
+            p->CP = ((nb * nbr + 16 * p->gcb - 1) / (16 * p->gcb)) * (16 * p->gcb);
+        }
         hipStream_t s = ctx->stream;
         // k' rows
         std::vector<int> kpi(3 * nwv), nw(nwv), wb(nwv * nb, 0);
