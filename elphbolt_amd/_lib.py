@@ -43,6 +43,12 @@ class ElphServerStats(ct.Structure):
                 ("pool_used", ct.c_int64), ("max_slots_busy", ct.c_int64), ("busy_span_ms", ct.c_double)]
 
 
+class ElphBatchClass(ct.Structure):
+    _fields_ = [("precision", ct.c_int32), ("path", ct.c_int32), ("cb", ct.c_int32), ("rb", ct.c_int32),
+                ("candidates", ct.c_int32), ("cols", ct.c_int64), ("kp", ct.c_int64), ("members", ct.c_int64),
+                ("ms", ct.c_double)]
+
+
 # exported symbol -> (restype, argtypes)
 PROTOTYPES = {
     "elph_ctx_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.POINTER(ElphDims), c_i64p, c_i64p, c_i64p,
@@ -85,6 +91,13 @@ PROTOTYPES = {
     "elph_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
     "elph_synth_vertex": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "elph_batch_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int32]),
+    "elph_batch_add": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, c_i64p]),
+    "elph_batch_prepare": (ct.c_int, [ct.c_void_p, ct.c_int32]),
+    "elph_batch_run": (ct.c_int, [ct.c_void_p, c_dblp]),
+    "elph_batch_classes": (ct.c_int64, [ct.c_void_p, ct.POINTER(ElphBatchClass), ct.c_int64]),
+    "elph_batch_error": (ct.c_char_p, [ct.c_void_p]),
+    "elph_batch_destroy": (ct.c_int, [ct.c_void_p]),
     "elph_server_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int32, ct.c_int64, ct.c_int32]),
     "elph_server_add_material": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "elph_server_submit": (ct.c_int64, [ct.c_void_p, ct.c_int32, ct.c_int64, c_dblp, ct.c_int64, c_dblp]),

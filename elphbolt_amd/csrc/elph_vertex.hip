@@ -20,9 +20,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "common.h"
@@ -2516,6 +2518,28 @@ int elph_selftest_mfma(int device, double *max_err) {
 // ===========================================================================
 // ibzk plan (calculate_eph_interaction_ibzk, key 'g')
 // ===========================================================================
+namespace {
+
+// Device tables of one group of up to 64 IBZ k of a run and of their in-window
+// states (built once; kernels then run from them without host work).
+struct IbzkGroup {
+    int64_t nkb = 0, nst = 0;
+    std::vector<int64_t> slot;  // output row of each state (relative to first)
+    DBuf<double> kv;            // (nkb,3) column-major
+    DBuf<double2> Ukb;          // [kb][m][mp]
+    DBuf<int> st_kb, st_m, st_kint;
+    DBuf<int64_t> st_slot;
+};
+
+// Scratch of one stream running groups (sized by ibzk_reserve).
+struct IbzkScratch {
+    DBuf<double2> gk, h, phiT, Y;
+    DBuf<v4i> bphi, bsl;
+    DBuf<double> bscale;
+};
+
+}  // namespace
+
 struct elph_ibzk_plan {
     elph_ctx *ctx = nullptr;
     int32_t precision = ELPH_FP64;
@@ -2532,23 +2556,19 @@ struct elph_ibzk_plan {
     DBuf<int64_t> offs;
     DBuf<double2> Vkp, u, uf, T, lp, lpf;
     DBuf<double> om, omf;
-    // per-run scratch
-    DBuf<double2> gk, h, Ukb;
-    DBuf<double> kv, g2stage;
-    DBuf<int> st_kb, st_m, st_kint;
-    DBuf<int64_t> st_slot;
-    // mixed precision: digit table of the phases (all q of the electron mesh),
-    // per-batch digits and column scales of B
+    // per-run scratch of plan_run
+    IbzkScratch scr;
+    DBuf<double> g2stage;
+    // mixed precision: digit table of the phases (all q of the electron mesh)
     int64_t nkc = 0;
     DBuf<uint2> Tdig;
     DBuf<unsigned> Rpk;
-    DBuf<v4i> bsl;
-    DBuf<double> bscale;
-    // general path (nb*nbr beyond the fused kernels): column tile, row tile
-    // (FP64: 64*rb rows per block) and the Y buffer of one chunk of states
+    // pair stage: fused kernels, or the general path (nb*nbr beyond the fused
+    // kernels, or chosen by the batch autotuner) with column tile 16*gcb and
+    // (FP64) 64*grb rows per block
     bool general = false;
     int gcb = 3, grb = 2;
-    DBuf<double2> Y;
+    int64_t CPf = 0;  // column count of the fused layout (16-rounded nb*nbr)
 };
 
 namespace {
@@ -2561,65 +2581,65 @@ void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, doubl
 }
 
 template <int CB>
-void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p, int st0, int ns, double2 *Y) {
+void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p, const IbzkScratch &sc, int st0,
+                      int ns, double2 *Y) {
     const size_t lds = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) + (size_t)((a.L + 1) & ~1) * sizeof(uint2) +
                        (size_t)p->nkc * 64 * sizeof(unsigned);
     dim3 grid((unsigned)((a.nwv + 63) / 64), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
-    hipLaunchKernelGGL(k_ibzk_y_i8<CB>, grid, dim3(256), lds, s, a, st0, p->Tdig.p, p->Rpk.p, p->bsl.p,
-                       p->bscale.p, (int)p->nkc, (int)(a.CP / 16), Y);
+    hipLaunchKernelGGL(k_ibzk_y_i8<CB>, grid, dim3(256), lds, s, a, st0, p->Tdig.p, p->Rpk.p, sc.bsl.p,
+                       sc.bscale.p, (int)p->nkc, (int)(a.CP / 16), Y);
+}
+
+int64_t general_chunk(const elph_ibzk_plan *p, int64_t nst) {
+    const int64_t per_state = p->nwv * p->CP * (int64_t)sizeof(double2);
+    return std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per_state));
 }
 
 // General path: Y of a chunk of states into HBM, then the epilogue.
-void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst) {
-    elph_ctx *c = p->ctx;
-    hipStream_t s = c->stream;
-    const int64_t per_state = p->nwv * p->CP * (int64_t)sizeof(double2);
-    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per_state));
-    p->Y.alloc(chunk * p->nwv * p->CP);
+void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScratch &sc, hipStream_t s) {
+    const int64_t chunk = general_chunk(p, nst);
     const int G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
     const size_t elds = (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb) * sizeof(double2);
-    HCHECK(hipEventRecord(c->ev_a, s));
     for (int64_t s0 = 0; s0 < nst; s0 += chunk) {
         const int ns = (int)std::min(chunk, nst - s0);
         if (p->precision == ELPH_MIXED) {
             switch (p->gcb) {
-                case 1: launch_ibzk_y_i8<1>(s, a, p, (int)s0, ns, p->Y.p); break;
-                case 2: launch_ibzk_y_i8<2>(s, a, p, (int)s0, ns, p->Y.p); break;
-                default: launch_ibzk_y_i8<3>(s, a, p, (int)s0, ns, p->Y.p); break;
+                case 1: launch_ibzk_y_i8<1>(s, a, p, sc, (int)s0, ns, sc.Y.p); break;
+                case 2: launch_ibzk_y_i8<2>(s, a, p, sc, (int)s0, ns, sc.Y.p); break;
+                default: launch_ibzk_y_i8<3>(s, a, p, sc, (int)s0, ns, sc.Y.p); break;
             }
         } else {
             const int key = p->gcb * 10 + p->grb;
             switch (key) {
-                case 11: launch_ibzk_y_fp64<1, 1>(s, a, (int)s0, ns, p->Y.p); break;
-                case 12: launch_ibzk_y_fp64<1, 2>(s, a, (int)s0, ns, p->Y.p); break;
-                case 21: launch_ibzk_y_fp64<2, 1>(s, a, (int)s0, ns, p->Y.p); break;
-                case 22: launch_ibzk_y_fp64<2, 2>(s, a, (int)s0, ns, p->Y.p); break;
-                case 31: launch_ibzk_y_fp64<3, 1>(s, a, (int)s0, ns, p->Y.p); break;
-                case 32: launch_ibzk_y_fp64<3, 2>(s, a, (int)s0, ns, p->Y.p); break;
-                case 41: launch_ibzk_y_fp64<4, 1>(s, a, (int)s0, ns, p->Y.p); break;
-                default: launch_ibzk_y_fp64<4, 2>(s, a, (int)s0, ns, p->Y.p); break;
+                case 11: launch_ibzk_y_fp64<1, 1>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 12: launch_ibzk_y_fp64<1, 2>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 21: launch_ibzk_y_fp64<2, 1>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 22: launch_ibzk_y_fp64<2, 2>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 31: launch_ibzk_y_fp64<3, 1>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 32: launch_ibzk_y_fp64<3, 2>(s, a, (int)s0, ns, sc.Y.p); break;
+                case 41: launch_ibzk_y_fp64<4, 1>(s, a, (int)s0, ns, sc.Y.p); break;
+                default: launch_ibzk_y_fp64<4, 2>(s, a, (int)s0, ns, sc.Y.p); break;
             }
         }
         dim3 eg((unsigned)((p->nwv + G - 1) / G), (unsigned)ns);
-        hipLaunchKernelGGL(k_ibzk_epi, eg, dim3(256), elds, s, a, (int)s0, p->Y.p, G);
+        hipLaunchKernelGGL(k_ibzk_epi, eg, dim3(256), elds, s, a, (int)s0, sc.Y.p, G);
         HCHECK(hipGetLastError());
     }
-    HCHECK(hipEventRecord(c->ev_b, s));
-    HCHECK(hipEventSynchronize(c->ev_b));
-    float ms = 0;
-    HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
-    c->t_dominant_ms += ms;
-    c->n_dominant += 1;
 }
 
-void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t nprocs) {
+void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkScratch &sc, hipStream_t s, int CB);
+
+// Pair stage of one group on stream s; with `timed`, HIP events bracket it on
+// s and the time is added to the context's dominant-kernel total (synchronous).
+void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
     elph_ctx *c = p->ctx;
+    const int64_t nst_batch = gr.nst, nprocs = p->nprocs;
     PairArgs a{};
-    a.h = p->h.p;
+    a.h = sc.h.p;
     a.Kp = (int)p->Kp;
     a.CP = (int)p->CP;
-    a.st_kint = p->st_kint.p;
-    a.st_slot = p->st_slot.p;
+    a.st_kint = gr.st_kint.p;
+    a.st_slot = gr.st_slot.p;
     a.nwv = (int)p->nwv;
     a.kp_int = p->kp_int.p;
     a.Nel0 = (int)p->Nel[0]; a.Nel1 = (int)p->Nel[1]; a.Nel2 = (int)p->Nel[2];
@@ -2639,18 +2659,32 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
     a.omf = p->omf.p;
     a.lp = c->polar ? p->lp.p : nullptr;
     a.lpf = (c->polar && p->mesh_ref > 1) ? p->lpf.p : nullptr;
-    a.Ukb = p->Ukb.p;
-    a.st_kb = p->st_kb.p;
-    a.st_m = p->st_m.p;
+    a.Ukb = gr.Ukb.p;
+    a.st_kb = gr.st_kb.p;
+    a.st_m = gr.st_m.p;
     a.nb = (int)p->nb;
     a.nbr = (int)p->nbr;
     a.out = out;
     a.nprocs = nprocs;
     const int CB = (int)(p->CP / 16);
+    if (timed) HCHECK(hipEventRecord(c->ev_a, s));
     if (p->general) {
-        general_launch(p, a, nst_batch);
-        return;
+        general_launch(p, a, nst_batch, sc, s);
+    } else {
+        fused_launch(p, a, nst_batch, sc, s, CB);
     }
+    HCHECK(hipGetLastError());
+    if (timed) {
+        HCHECK(hipEventRecord(c->ev_b, s));
+        HCHECK(hipEventSynchronize(c->ev_b));
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+        c->t_dominant_ms += ms;
+        c->n_dominant += 1;
+    }
+}
+
+void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkScratch &sc, hipStream_t s, int CB) {
     const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
                                 (size_t)64 * (p->CP + 1) * sizeof(double2));
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
@@ -2660,37 +2694,121 @@ void pair_launch(elph_ibzk_plan *p, int64_t nst_batch, double *out, int64_t npro
                                      (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;
-        const v4i *bs = p->bsl.p;
+        const v4i *bs = sc.bsl.p;
         grid.x = (unsigned)((p->nwv + 63) / 64);
-        const double *sc = p->bscale.p;
+        const double *bsc = sc.bscale.p;
         const int nkc = (int)p->nkc;
-        HCHECK(hipEventRecord(c->ev_a, c->stream));
         switch (CB) {
-            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
-            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
-            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
-            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, c->stream, a, at, rpk, bs, sc, nkc); break;
-            default: throw Error{ELPH_EINVAL, "mixed precision supports numwannbands*numbranches <= 64"};
+            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
+            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
+            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
+            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
+            default: throw Error{ELPH_EINVAL, "mixed precision fused kernel holds numwannbands*numbranches <= 64"};
         }
     } else {
-        HCHECK(hipEventRecord(c->ev_a, c->stream));
         switch (CB) {
-            case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, c->stream, a); break;
-            case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, c->stream, a); break;
-            case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, c->stream, a); break;
-            case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, c->stream, a); break;
-            case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, c->stream, a); break;
-            case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, c->stream, a); break;
-            default: throw Error{ELPH_EINVAL, "numwannbands*numbranches > 96 is not supported by the fused kernel"};
+            case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, s, a); break;
+            case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, s, a); break;
+            case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, s, a); break;
+            case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, s, a); break;
+            case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, s, a); break;
+            case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, s, a); break;
+            default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
         }
     }
-    HCHECK(hipGetLastError());
-    HCHECK(hipEventRecord(c->ev_b, c->stream));
-    HCHECK(hipEventSynchronize(c->ev_b));
-    float ms = 0;
-    HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
-    c->t_dominant_ms += ms;
-    c->n_dominant += 1;
+}
+
+// Active states of first..last (counts -1 for out-of-window states) grouped by
+// up to 64 IBZ k, with their device tables; st_slot is the output row relative
+// to `first` when on_device, else the row within the group.
+void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device, int64_t *counts, hipStream_t s,
+                 std::vector<std::unique_ptr<IbzkGroup>> &groups) {
+    const int64_t nb = p->nb, nb2 = nb * nb;
+    struct St { int64_t ik, m, slot; };
+    std::vector<St> act;
+    for (int64_t istate = first; istate <= last; ++istate) {
+        int64_t m, ik;
+        demux_state(istate, nb, &m, &ik);
+        const double e = p->ens_irred[(ik - 1) + p->nirr * (m - 1)];
+        const int64_t t = istate - first;
+        if (std::fabs(e - p->enref) > p->fsthick) { counts[t] = -1; continue; }
+        counts[t] = p->nprocs;
+        act.push_back({ik - 1, m - 1, t});
+    }
+    const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
+    size_t i0 = 0;
+    while (i0 < act.size()) {
+        std::vector<int64_t> ks;
+        size_t i1 = i0;
+        while (i1 < act.size()) {
+            if (ks.empty() || ks.back() != act[i1].ik) {
+                if ((int64_t)ks.size() == KB) break;
+                ks.push_back(act[i1].ik);
+            }
+            ++i1;
+        }
+        auto g = std::make_unique<IbzkGroup>();
+        const int64_t nkb = (int64_t)ks.size(), nst = (int64_t)(i1 - i0);
+        g->nkb = nkb;
+        g->nst = nst;
+        std::vector<double> kv(3 * nkb);
+        std::vector<double2> Ukb(nkb * nb2);
+        for (int64_t b = 0; b < nkb; ++b) {
+            for (int d = 0; d < 3; ++d) kv[b + nkb * d] = (double)p->kint_irr[3 * ks[b] + d] / (double)p->Nel[d];
+            std::memcpy(&Ukb[b * nb2], &p->Uk_host[ks[b] * nb2], nb2 * sizeof(double2));
+        }
+        std::vector<int> skb(nst), sm(nst), ski(3 * nst);
+        std::vector<int64_t> sslot(nst);
+        int64_t bidx = -1, prev = -1;
+        for (int64_t j = 0; j < nst; ++j) {
+            const St &a = act[i0 + j];
+            if (a.ik != prev) { ++bidx; prev = a.ik; }
+            skb[j] = (int)bidx;
+            sm[j] = (int)a.m;
+            for (int d = 0; d < 3; ++d) ski[3 * j + d] = p->kint_irr[3 * a.ik + d];
+            sslot[j] = on_device ? a.slot : j;
+            g->slot.push_back(a.slot);
+        }
+        g->kv.upload(kv, s);
+        g->Ukb.upload(Ukb, s);
+        g->st_kb.upload(skb, s);
+        g->st_m.upload(sm, s);
+        g->st_kint.upload(ski, s);
+        g->st_slot.upload(sslot, s);
+        HCHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
+        groups.push_back(std::move(g));
+        i0 = i1;
+    }
+}
+
+// Scratch for groups of up to nkb IBZ k and nst states.
+void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t nst) {
+    elph_ctx *c = p->ctx;
+    const int64_t nb2 = p->nb * p->nb, nslices = p->nbr * c->nwsg;
+    sc.gk.alloc(nkb * nb2 * nslices);
+    sc.h.alloc(nst * p->Kp * p->CP);
+    sc.phiT.alloc(c->nwsk * 64);
+    if (p->precision == ELPH_MIXED) {
+        sc.bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
+        sc.bscale.alloc(nst * p->CP);
+        sc.bsl.alloc(nst * p->nkc * (p->CP / 16) * 3 * kNS * 64);
+    }
+    if (p->general) sc.Y.alloc(general_chunk(p, nst) * p->nwv * p->CP);
+}
+
+// One group: g(k,Rp) of its k, h = U_k g / phwsdeg, (mixed) digits, pair stage.
+void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
+    elph_ctx *c = p->ctx;
+    const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb, nslices = nbr * c->nwsg;
+    launch_gkRp(c, s, g.nkb, g.kv.p, sc.gk.p, nb2 * nslices, p->precision == ELPH_MIXED, sc.phiT.p, sc.bphi.p);
+    launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)c->nwsg, (int)p->Kp, (int)p->CP, sc.gk.p, nb2 * nslices,
+                    g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, sc.h.p);
+    if (p->precision == ELPH_MIXED) {
+        launch_colscale(s, (int)g.nst, (int)p->Kp, (int)p->CP, sc.h.p, sc.bscale.p);
+        launch_slice_b(s, (int)g.nst, (int)p->nkc, (int)(p->CP / 16), (int)p->Kp, (int)p->CP, sc.h.p, sc.bscale.p,
+                       sc.bsl.p);
+    }
+    pair_launch(p, g, sc, s, out, timed);
 }
 
 }  // namespace
@@ -2726,6 +2844,7 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
         p->CP = ((nb * nbr + 15) / 16) * 16;
+        p->CPf = p->CP;
         p->Kp = ((ctx->nwsg + 63) / 64) * 64;
         // fused kernels hold all nb*nbr columns of a state in registers up to
         // 96 (FP64) / 64 (mixed); beyond, the general column-tiled path
@@ -2894,96 +3013,30 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
     elph_ctx *c = p->ctx;
     try {
         HCHECK(hipSetDevice(c->device));
-        const int64_t nb = p->nb, nbr = p->nbr;
-        if (first < 1 || last > p->nirr * nb || last < first - 1)
+        if (first < 1 || last > p->nirr * p->nb || last < first - 1)
             throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
         hipStream_t s = c->stream;
         c->t_dominant_ms = 0;
         c->n_dominant = 0;
-        HCHECK(hipEventRecord(c->ev_c, s));
-        // active states grouped by IBZ k (istate order keeps k ascending)
-        struct St { int64_t ik, m, slot; };
-        std::vector<St> act;
-        for (int64_t istate = first; istate <= last; ++istate) {
-            int64_t m, ik;
-            demux_state(istate, nb, &m, &ik);
-            const double e = p->ens_irred[(ik - 1) + p->nirr * (m - 1)];
-            const int64_t t = istate - first;
-            if (std::fabs(e - p->enref) > p->fsthick) { counts[t] = -1; continue; }
-            counts[t] = p->nprocs;
-            act.push_back({ik - 1, m - 1, t});
+        std::vector<std::unique_ptr<IbzkGroup>> groups;
+        ibzk_groups(p, first, last, on_device != 0, counts, s, groups);
+        int64_t nkb = 1, nst = 1;
+        for (auto &g : groups) {
+            nkb = std::max(nkb, g->nkb);
+            nst = std::max(nst, g->nst);
         }
-        const int64_t nb2 = nb * nb, nslices = nbr * c->nwsg;
-        const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
-        size_t i0 = 0;
-        while (i0 < act.size()) {
-            // collect up to KB distinct k
-            std::vector<int64_t> ks;
-            size_t i1 = i0;
-            while (i1 < act.size()) {
-                if (ks.empty() || ks.back() != act[i1].ik) {
-                    if ((int64_t)ks.size() == KB) break;
-                    ks.push_back(act[i1].ik);
-                }
-                ++i1;
-            }
-            const int64_t nkb = (int64_t)ks.size(), nst = (int64_t)(i1 - i0);
-            // kvecs of the batch (nkb,3) column-major, crystal coordinates int/mesh
-            std::vector<double> kv(3 * nkb);
-            std::vector<double2> Ukb(nkb * nb2);
-            for (int64_t b = 0; b < nkb; ++b) {
-                for (int d = 0; d < 3; ++d) kv[b + nkb * d] = (double)p->kint_irr[3 * ks[b] + d] / (double)p->Nel[d];
-                std::memcpy(&Ukb[b * nb2], &p->Uk_host[ks[b] * nb2], nb2 * sizeof(double2));
-            }
-            p->kv.upload(kv, s);
-            p->Ukb.upload(Ukb, s);
-            p->gk.alloc(nkb * nb2 * nslices);
-            gkRp_device(c, nkb, p->kv.p, p->gk.p, nb2 * nslices, p->precision == ELPH_MIXED);
-            std::vector<int> skb(nst), sm(nst), ski(3 * nst);
-            std::vector<int64_t> sslot(nst);
-            int64_t bidx = -1, prev = -1;
-            for (int64_t j = 0; j < nst; ++j) {
-                const St &a = act[i0 + j];
-                if (a.ik != prev) { ++bidx; prev = a.ik; }
-                skb[j] = (int)bidx;
-                sm[j] = (int)a.m;
-                for (int d = 0; d < 3; ++d) ski[3 * j + d] = p->kint_irr[3 * a.ik + d];
-                sslot[j] = on_device ? a.slot : j;
-            }
-            p->st_kb.upload(skb, s);
-            p->st_m.upload(sm, s);
-            p->st_kint.upload(ski, s);
-            p->st_slot.upload(sslot, s);
-            p->h.alloc(nst * p->Kp * p->CP);
-            const int64_t tot = nst * p->Kp * p->CP;
-            hipLaunchKernelGGL(k_rotate_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nst, (int)nb,
-                               (int)nbr, (int)c->nwsg, (int)p->Kp, (int)p->CP, p->gk.p, nb2 * nslices, p->st_kb.p,
-                               p->st_m.p, p->Ukb.p, c->degq.p, p->h.p);
-            HCHECK(hipGetLastError());
-            if (p->precision == ELPH_MIXED) {
-                const int CB = (int)(p->CP / 16);
-                p->bscale.alloc(nst * p->CP);
-                p->bsl.alloc(nst * p->nkc * CB * 3 * kNS * 64);
-                hipLaunchKernelGGL(k_colscale, dim3((unsigned)((nst * p->CP + 255) / 256)), dim3(256), 0, s, (int)nst,
-                                   (int)p->Kp, (int)p->CP, p->h.p, p->bscale.p);
-                const int64_t nt = nst * p->nkc * CB * 64;
-                hipLaunchKernelGGL(k_slice_b, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, (int)nst,
-                                   (int)p->nkc, CB, (int)p->Kp, (int)p->CP, p->h.p, p->bscale.p, p->bsl.p);
-                HCHECK(hipGetLastError());
-            }
-            double *dst = g2;
+        if (p->precision == ELPH_MIXED) ensure_gwann_digits(c);
+        ibzk_reserve(p, p->scr, nkb, nst);
+        if (!on_device) p->g2stage.alloc(nst * p->nprocs);
+        HCHECK(hipEventRecord(c->ev_c, s));
+        for (auto &g : groups) {
+            ibzk_enqueue(p, *g, p->scr, s, on_device ? g2 : p->g2stage.p, true);
             if (!on_device) {
-                p->g2stage.alloc(nst * p->nprocs);
-                dst = p->g2stage.p;
-            }
-            pair_launch(p, nst, dst, p->nprocs);
-            if (!on_device) {
-                for (int64_t j = 0; j < nst; ++j)
-                    HCHECK(hipMemcpyAsync(g2 + act[i0 + j].slot * p->nprocs, p->g2stage.p + j * p->nprocs,
+                for (int64_t j = 0; j < g->nst; ++j)
+                    HCHECK(hipMemcpyAsync(g2 + g->slot[j] * p->nprocs, p->g2stage.p + j * p->nprocs,
                                           p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
                 HCHECK(hipStreamSynchronize(s));
             }
-            i0 = i1;
         }
         HCHECK(hipEventRecord(c->ev_d, s));
         HCHECK(hipEventSynchronize(c->ev_d));
@@ -2993,6 +3046,280 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
     } catch (const Error &e) {
         return ret_error(c, e);
     }
+    return ELPH_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched multi-material driver: the ibzk work of many plans (materials of
+// different band, mode and WS counts) captured once into a hipGraph whose
+// branches run on several streams; the pair-stage tile shape of each size
+// class is picked by timing the candidates on the class's first group.
+// ===========================================================================
+struct elph_batch {
+    struct Item {
+        elph_ibzk_plan *p = nullptr;
+        int64_t first = 0, last = 0;
+        double *out = nullptr;
+        std::vector<std::unique_ptr<IbzkGroup>> groups;
+        int stream = 0, cls = -1;
+        double work = 0;
+    };
+    struct Class {
+        int32_t precision = 0;
+        int cb = 0, kb = 0;  // size buckets of nb*nbr and Kp
+        int64_t members = 0;
+        elph_batch_class_t rep{};
+    };
+    int device = 0;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    std::vector<Item> items;
+    std::vector<Class> classes;
+    std::vector<std::unique_ptr<IbzkScratch>> scr;  // per stream
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int col_bucket(int64_t cp) {
+    const int64_t edges[] = {32, 64, 96, 192, 384};
+    for (int i = 0; i < 5; ++i)
+        if (cp <= edges[i]) return i;
+    return 5;
+}
+int kp_bucket(int64_t kp) {
+    const int64_t edges[] = {128, 320, 640};
+    for (int i = 0; i < 3; ++i)
+        if (kp <= edges[i]) return i;
+    return 3;
+}
+
+struct TileCand {
+    bool general;
+    int gcb, grb;
+};
+
+void apply_cand(elph_ibzk_plan *p, const TileCand &t) {
+    p->general = t.general;
+    p->gcb = t.gcb;
+    p->grb = t.grb;
+    p->CP = t.general ? ((p->nb * p->nbr + 16 * t.gcb - 1) / (16 * t.gcb)) * (16 * t.gcb) : p->CPf;
+}
+
+// Times the candidates on the first group of `it` (stream st) and applies the best.
+elph_batch_class_t autotune(elph_batch *b, elph_batch::Item &it, hipStream_t st, bool tune) {
+    elph_ibzk_plan *p = it.p;
+    const bool mixed = p->precision == ELPH_MIXED;
+    std::vector<TileCand> cands;
+    if (p->CPf <= (mixed ? 64 : 96)) cands.push_back({false, 0, 0});
+    if (mixed) {
+        for (int cb = 1; cb <= 3; ++cb) cands.push_back({true, cb, 1});
+    } else {
+        for (int cb = 1; cb <= 4; ++cb)
+            for (int rb = 1; rb <= 2; ++rb) cands.push_back({true, cb, rb});
+    }
+    if (!tune) cands.resize(1);
+    const IbzkGroup &g = *it.groups.front();
+    IbzkScratch sc;
+    double best = 1e300;
+    TileCand bc = cands.front();
+    for (const TileCand &t : cands) {
+        apply_cand(p, t);
+        ibzk_reserve(p, sc, g.nkb, g.nst);
+        ibzk_enqueue(p, g, sc, st, it.out, false);  // warm-up (also builds h for this layout)
+        float tmin = 1e30f;
+        for (int r = 0; r < 3; ++r) {
+            HCHECK(hipEventRecord(b->ev0, st));
+            pair_launch(p, g, sc, st, it.out, false);
+            HCHECK(hipEventRecord(b->ev1, st));
+            HCHECK(hipEventSynchronize(b->ev1));
+            float ms = 0;
+            HCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            tmin = std::min(tmin, ms);
+        }
+        if (tmin < best) {
+            best = tmin;
+            bc = t;
+        }
+    }
+    apply_cand(p, bc);
+    elph_batch_class_t r{};
+    r.precision = p->precision;
+    r.path = bc.general ? 1 : 0;
+    r.cb = bc.general ? bc.gcb : (int)(p->CPf / 16);
+    r.rb = bc.general ? bc.grb : (mixed ? 1 : 2);
+    r.candidates = (int32_t)cands.size();
+    r.cols = p->nb * p->nbr;
+    r.kp = p->Kp;
+    r.ms = best;
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int elph_batch_create(elph_batch **out, int device, int32_t nstreams) {
+    if (!out || nstreams < 1 || nstreams > 16) return ELPH_EINVAL;
+    *out = nullptr;
+    auto b = std::make_unique<elph_batch>();
+    try {
+        HCHECK(hipSetDevice(device));
+        b->device = device;
+        for (int i = 0; i < nstreams; ++i) {
+            hipStream_t st;
+            HCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            b->streams.push_back(st);
+            hipEvent_t e;
+            HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->joins.push_back(e);
+            b->scr.push_back(std::make_unique<IbzkScratch>());
+        }
+        HCHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HCHECK(hipEventCreate(&b->ev0));
+        HCHECK(hipEventCreate(&b->ev1));
+    } catch (const Error &e) {
+        return e.code;
+    }
+    *out = b.release();
+    return ELPH_OK;
+}
+
+int elph_batch_add(elph_batch *b, elph_ibzk_plan *plan, int64_t first, int64_t last, double *g2_device,
+                   int64_t *counts) {
+    if (!b || !plan || !g2_device || !counts) return ELPH_EINVAL;
+    if (b->exec) {
+        b->err = "plans must be added before elph_batch_prepare";
+        return ELPH_EINVAL;
+    }
+    try {
+        if (plan->ctx->device != b->device) throw Error{ELPH_EINVAL, "plan and batch are on different devices"};
+        if (first < 1 || last > plan->nirr * plan->nb || last < first)
+            throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
+        HCHECK(hipSetDevice(b->device));
+        elph_batch::Item it;
+        it.p = plan;
+        it.first = first;
+        it.last = last;
+        it.out = g2_device;
+        ibzk_groups(plan, first, last, true, counts, b->streams[0], it.groups);
+        int64_t nst = 0;
+        for (auto &g : it.groups) nst += g->nst;
+        it.work = (double)nst * plan->nwv * plan->Kp * plan->nb * plan->nbr;
+        if (plan->precision == ELPH_MIXED) ensure_gwann_digits(plan->ctx);
+        b->items.push_back(std::move(it));
+    } catch (const Error &e) {
+        b->err = e.msg;
+        return e.code;
+    }
+    return ELPH_OK;
+}
+
+int elph_batch_prepare(elph_batch *b, int32_t autotune_tiles) {
+    if (!b) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(b->device));
+        // size classes: (precision, bucket of nb*nbr, bucket of Kp)
+        std::map<std::tuple<int, int, int>, int> key2cls;
+        for (auto &it : b->items) {
+            if (it.groups.empty()) continue;
+            const auto key = std::make_tuple((int)it.p->precision, col_bucket(it.p->CPf), kp_bucket(it.p->Kp));
+            auto f = key2cls.find(key);
+            if (f == key2cls.end()) {
+                elph_batch::Class c;
+                c.precision = it.p->precision;
+                c.cb = std::get<1>(key);
+                c.kb = std::get<2>(key);
+                c.rep = autotune(b, it, b->streams[0], autotune_tiles != 0);
+                key2cls[key] = (int)b->classes.size();
+                b->classes.push_back(c);
+                f = key2cls.find(key);
+            } else {
+                const elph_batch_class_t &r = b->classes[f->second].rep;
+                apply_cand(it.p, {r.path == 1, r.cb, r.rb});
+            }
+            it.cls = f->second;
+            b->classes[f->second].members += 1;
+            b->classes[f->second].rep.members = b->classes[f->second].members;
+        }
+        // longest work first onto the least-loaded stream
+        std::vector<size_t> order(b->items.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return b->items[x].work > b->items[y].work; });
+        std::vector<double> load(b->streams.size(), 0.0);
+        for (size_t i : order) {
+            const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            b->items[i].stream = s;
+            load[s] += b->items[i].work;
+        }
+        for (auto &it : b->items)
+            for (auto &g : it.groups) ibzk_reserve(it.p, *b->scr[it.stream], g->nkb, g->nst);
+        HCHECK(hipDeviceSynchronize());
+        // capture: fork from stream 0, every stream runs its items, join on stream 0
+        hipStream_t s0 = b->streams[0];
+        HCHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+        HCHECK(hipEventRecord(b->fork, s0));
+        for (size_t i = 1; i < b->streams.size(); ++i) HCHECK(hipStreamWaitEvent(b->streams[i], b->fork, 0));
+        for (auto &it : b->items)
+            for (auto &g : it.groups)
+                ibzk_enqueue(it.p, *g, *b->scr[it.stream], b->streams[it.stream], it.out, false);
+        for (size_t i = 1; i < b->streams.size(); ++i) {
+            HCHECK(hipEventRecord(b->joins[i], b->streams[i]));
+            HCHECK(hipStreamWaitEvent(s0, b->joins[i], 0));
+        }
+        HCHECK(hipStreamEndCapture(s0, &b->graph));
+        HCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    } catch (const Error &e) {
+        b->err = e.msg;
+        return e.code;
+    }
+    return ELPH_OK;
+}
+
+int elph_batch_run(elph_batch *b, double *ms) {
+    if (!b || !b->exec) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(b->device));
+        hipStream_t s0 = b->streams[0];
+        HCHECK(hipEventRecord(b->ev0, s0));
+        HCHECK(hipGraphLaunch(b->exec, s0));
+        HCHECK(hipEventRecord(b->ev1, s0));
+        HCHECK(hipEventSynchronize(b->ev1));
+        float t = 0;
+        HCHECK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+        if (ms) *ms = t;
+    } catch (const Error &e) {
+        b->err = e.msg;
+        return e.code;
+    }
+    return ELPH_OK;
+}
+
+int64_t elph_batch_classes(const elph_batch *b, elph_batch_class_t *out, int64_t max) {
+    if (!b) return -ELPH_EINVAL;
+    for (int64_t i = 0; i < (int64_t)b->classes.size() && i < max; ++i) out[i] = b->classes[i].rep;
+    return (int64_t)b->classes.size();
+}
+
+const char *elph_batch_error(const elph_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int elph_batch_destroy(elph_batch *b) {
+    if (!b) return ELPH_EINVAL;
+    (void)hipSetDevice(b->device);
+    for (auto st : b->streams) (void)hipStreamSynchronize(st);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto e : b->joins) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (auto st : b->streams) (void)hipStreamDestroy(st);
+    delete b;
     return ELPH_OK;
 }
 

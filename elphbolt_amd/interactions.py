@@ -104,6 +104,66 @@ class IbzkPlan:
             pass
 
 
+class Batch:
+    """Several materials' calculate_eph_interaction_ibzk('g') work in one step:
+    captured once into a hipGraph with branches on `nstreams` streams, the
+    pair-stage tile shape of each size class chosen by timing (autotune).
+    Outputs stay in HBM (one DeviceBuffer per added plan)."""
+
+    def __init__(self, nstreams: int = 4, device: int = 0):
+        from . import _lib
+        self.lib = _lib.load()
+        h = ct.c_void_p()
+        st = self.lib.elph_batch_create(ct.byref(h), device, int(nstreams))
+        if st != 0:
+            raise _lib.ElphError(st, "elph_batch_create failed")
+        self.h = h
+        self.entries = []
+
+    def _check(self, st, what):
+        if st != 0:
+            from ._lib import ElphError
+            msg = self.lib.elph_batch_error(self.h)
+            raise ElphError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def add(self, plan: IbzkPlan, istate_first: int = 1, istate_last=None):
+        """Adds a plan's states; returns (counts, device output (nst, nprocs))."""
+        if istate_last is None:
+            istate_last = plan.nwv_irred * plan.nb
+        nst = istate_last - istate_first + 1
+        counts = np.zeros(nst, np.int64)
+        out = plan.wann.device_buffer((nst, plan.nprocs))
+        self._check(self.lib.elph_batch_add(self.h, plan.plan, istate_first, istate_last, out.ptr,
+                                            _p(counts, c_i64p)), "batch add")
+        self.entries.append((plan, counts, out))
+        return counts, out
+
+    def prepare(self, autotune: bool = True):
+        self._check(self.lib.elph_batch_prepare(self.h, 1 if autotune else 0), "batch prepare")
+
+    def run(self) -> float:
+        ms = ct.c_double(0)
+        self._check(self.lib.elph_batch_run(self.h, ct.byref(ms)), "batch run")
+        return ms.value
+
+    def classes(self):
+        from ._lib import ElphBatchClass
+        buf = (ElphBatchClass * 64)()
+        n = int(self.lib.elph_batch_classes(self.h, buf, 64))
+        return [{f: getattr(buf[i], f) for f, _ in ElphBatchClass._fields_} for i in range(min(n, 64))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.elph_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class IbzqPlan:
     """calculate_eph_interaction_ibzq('g') with HBM-resident inputs."""
 

@@ -166,8 +166,11 @@ def make_hamiltonian(cells, lattvecs, nb, seed) -> np.ndarray:
     return H / C.RYD2EV
 
 
+_T_GRID = np.stack(np.meshgrid(*[np.arange(-2, 3)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)
+
+
 def _min_image(lattvecs, cell, dvec_frac, mesh):
-    t = np.stack(np.meshgrid(*[np.arange(-2, 3)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)
+    t = _T_GRID
     cand = cell[None, :] + t * mesh[None, :]
     d = (cand + dvec_frac[None, :]) @ lattvecs.T
     lens = np.linalg.norm(d, axis=1)
@@ -207,18 +210,18 @@ def make_force_constants(cells, lattvecs, basis, masses, mesh, omega_max_ev=0.06
     if not springs:
         raise ValueError("no spring shorter than 0.45 x the shortest supercell vector: use a larger WS grid")
     # scale so that max omega (on a 4^3 q sample) equals omega_max_ev
-    def dq(q):
-        D = np.zeros((nbr, nbr), np.complex128)
-        D += classes[key0]
-        for (i, j, T, e, k) in springs:
-            D[3 * i:3 * i + 3, 3 * j:3 * j + 3] += -k * np.outer(e, e) * np.exp(2j * np.pi * q @ T)
-        return D
+    qs = np.stack(np.meshgrid(*[np.arange(4) / 4.0] * 3, indexing="ij"), -1).reshape(-1, 3)
+    si = np.array([sp[0] for sp in springs])
+    sj = np.array([sp[1] for sp in springs])
+    sT = np.array([sp[2] for sp in springs], np.float64)
+    sblk = np.array([-sp[4] * np.outer(sp[3], sp[3]) for sp in springs])
+    ph = np.exp(2j * np.pi * (qs @ sT.T))                        # (nq, nspring)
+    Dq = np.zeros((qs.shape[0], nat, nat, 3, 3), np.complex128)
+    np.add.at(Dq, (slice(None), si, sj), ph[:, :, None, None] * sblk[None])
+    Dq = Dq.transpose(0, 1, 3, 2, 4).reshape(qs.shape[0], nbr, nbr) + classes[key0][None]
     mvec = np.repeat(masses, 3)
     mnorm = C.RYD2AMU / np.sqrt(np.outer(mvec, mvec))
-    wmax2 = 0.0
-    for q in np.stack(np.meshgrid(*[np.arange(4) / 4.0] * 3, indexing="ij"), -1).reshape(-1, 3):
-        w2 = np.linalg.eigvalsh(dq(q) * mnorm)
-        wmax2 = max(wmax2, float(w2.max()))
+    wmax2 = float(np.linalg.eigvalsh(Dq * mnorm[None]).max())
     scale = (omega_max_ev / C.RYD2EV) ** 2 / wmax2
     D = np.empty((cells.shape[0], nbr, nbr), np.complex128)
     for iuc, cell in enumerate(cells):
@@ -326,6 +329,7 @@ def config(idx: int) -> Material:
 
 
 MULTI_SEED_BASE = 0xE1B03000
+_WS_COUNT: dict = {}
 _MASSES = [28.0855, 12.0107, 15.999, 24.305, 69.723, 74.92, 10.81, 30.97, 26.98, 14.007, 72.63, 118.71]
 
 
@@ -343,8 +347,9 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
         n = int(rng.integers(4, 10))
         wv = int(rng.choice([12, 16, 20, 24]))
         a = float(rng.uniform(4.0, 6.0))
-        cells, _ = ws_cells(fcc_lattice(a), np.array([n, n, n], np.int64))
-        nws = cells.shape[0]
+        if n not in _WS_COUNT:  # the WS set of an fcc n^3 grid does not depend on a
+            _WS_COUNT[n] = ws_cells(fcc_lattice(4.36), np.array([n, n, n], np.int64))[0].shape[0]
+        nws = _WS_COUNT[n]
         if 16 * nb * nb * 3 * nat * nws * nws <= budget_bytes:
             break
     else:

@@ -210,6 +210,36 @@ int elph_selftest_mfma(int device, double *max_err);
 int elph_last_timing(const elph_ctx *ctx, double out[3]);
 
 /* ------------------------------------------------------------------------
+ * Batched multi-material driver (no counterpart in the reference; it runs the
+ * work of several calculate_eph_interaction_ibzk('g') calls at once).  Each
+ * added plan contributes the in-window IBZ states first..last of its material,
+ * |g|^2 rows written to g2_device (device memory, row stride = the plan's
+ * nprocs) and counts filled as elph_ibzk_plan_run does.  prepare() picks the
+ * pair-stage tile shape of each size class (nb*nbr and Kp buckets) by timing
+ * the candidates on the class's first group when autotune != 0, assigns
+ * plans to streams and captures the whole step into one hipGraph; run()
+ * replays it (ms = device time of the replay).  Plans are tuned in place:
+ * their later runs use the chosen shapes.
+ * ------------------------------------------------------------------------ */
+typedef struct elph_batch elph_batch;
+typedef struct {
+    int32_t precision, path;  /* path 0 fused kernel, 1 general column-tiled */
+    int32_t cb, rb;           /* column blocks of 16, row blocks of 16 per wave */
+    int32_t candidates;       /* shapes timed */
+    int64_t cols, kp;         /* nb*nbr and Kp of the class's tuning material */
+    int64_t members;
+    double ms;                /* pair stage of the tuning group with the chosen shape */
+} elph_batch_class_t;
+int elph_batch_create(elph_batch **out, int device, int32_t nstreams);
+int elph_batch_add(elph_batch *b, elph_ibzk_plan *plan, int64_t first, int64_t last, double *g2_device,
+                   int64_t *counts);
+int elph_batch_prepare(elph_batch *b, int32_t autotune);
+int elph_batch_run(elph_batch *b, double *ms);
+int64_t elph_batch_classes(const elph_batch *b, elph_batch_class_t *out, int64_t max);
+const char *elph_batch_error(const elph_batch *b);
+int elph_batch_destroy(elph_batch *b);
+
+/* ------------------------------------------------------------------------
  * Serving (no counterpart in the reference; not called by the Fortran side).
  * A server keeps the Wannier tensors of its materials (contexts) resident and
  * answers requests for |g|^2 at arbitrary k- and q-lists: for every k of the
