@@ -40,6 +40,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def log_mem(tag):
+    from elphbolt_amd import _lib
+    f, t = _lib.device_meminfo(0)
+    log(f"[bench] device memory {tag}: {f / 2**30:.1f} GiB free of {t / 2**30:.1f} GiB")
+
+
 def algorithmic_flops(nst, nwv, nstates_inwindow, nb, nbr, nwsg):
     """Flops of one pair-kernel launch (nst IBZ states x nwv k' rows): the Rp
     phase contraction of the band-rotated g(k,Rp) (8 nwsg nb nbr per state and
@@ -167,8 +173,10 @@ def serving_bench(args):
            "pool_gb": args.serving_pool_gb, "slots": args.serving_slots}
     results = {}
     for prec in ("mixed", "fp64"):
+        log_mem(f"serving {prec}: before server")
         with Server(prec, pool_bytes=int(args.serving_pool_gb * (1 << 30)), nslots=args.serving_slots) as srv:
             mids = [srv.add_material(w) for w in ws]
+            log_mem(f"serving {prec}: server ready")
             pr = []
             for m, k, q in probe:   # warm-up and accuracy probe
                 rid = srv.submit(mids[m], k, q)
@@ -179,6 +187,7 @@ def serving_bench(args):
             lat, refused, failed = serve_stream(srv, mids, reqs)
             wall = time.perf_counter() - t1
             st = srv.stats()
+            log_mem(f"serving {prec}: stream done")
         vert = st["vertices"] - base["vertices"]
         span = reqs[-1]["t"] if reqs else 0.0
         lat = np.asarray(lat)
@@ -190,6 +199,7 @@ def serving_bench(args):
                "achieved_requests_per_s": lat.size / wall, "offered_span_s": span,
                "max_slots_busy": st["max_slots_busy"], "units": st["units"] - base["units"]}
         results[prec] = (res, pr)
+        log_mem(f"after serving {prec}")
         log(f"[bench] serving {prec}: {res['completed']} done, {refused} refused, p50 {res['p50_latency_ms']:.0f} ms "
             f"p99 {res['p99_latency_ms']:.0f} ms, {res['sustained_vertices_per_s']:.3e} vertices/s, "
             f"wall {wall:.1f} s")
@@ -206,6 +216,122 @@ def serving_bench(args):
     out["error_mixed_vs_fp64"] = {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
                                   "entries": int(rel.size), "floor": "1e-6*max|g|^2",
                                   "probe": f"{len(probe)} requests of 8 k x 8 q", "bound_met": bool(rel.max() <= 1e-5)}
+    return out
+
+
+BATCH_SEED = 0xE1B05000
+
+
+def _batched_material(i, budget):
+    from elphbolt_amd import synth
+    return synth.multi_material(i, budget_bytes=budget, bands_in_window=4.0)
+
+
+def start_batched_generation(args):
+    """Generates the batched materials in worker processes started with the
+    spawn method before this process touches the GPU (they overlap the rest
+    of the benchmark); returns (executor, futures)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    n = args.batched_materials
+    ex = cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1), mp_context=mp.get_context("spawn"))
+    futs = [ex.submit(_batched_material, i, args.batched_budget_gb * 1e9) for i in range(n)]
+    return ex, futs
+
+
+def batched_bench(args, gen):
+    """Batched form: materials of the multi-material generator (atoms 1-8,
+    bands 4-26, WS 4^3-9^3, fine meshes 12^3-24^3, window keeping ~4 bands),
+    each with 0.5-2e6 k-q pairs of calculate_eph_interaction_ibzk('g'), all in
+    one captured hipGraph per precision with tile shapes from the autotuner."""
+    import gc
+    from elphbolt_amd import bz, interactions
+    from elphbolt_amd._lib import ElphError
+    from elphbolt_amd.wannier import Wannier
+    t0 = time.perf_counter()
+    ex, futs = gen
+    mats = [f.result() for f in futs]
+    ex.shutdown()
+    log(f"[bench] batched: {len(mats)} materials generated (waited {time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(BATCH_SEED)
+    entries, skipped = [], []
+    for m in mats:
+        target = float(rng.uniform(0.5e6, 2e6))
+        try:
+            w = Wannier(m)
+            sym = bz.Symmetry(m.lattvecs, m.basis, m.masses, m.wvmesh)
+            el = bz.Electrons(w, sym, m.wvmesh, m.enref, m.fsthick)
+            ph = bz.Phonons(w, sym, m.phmesh)
+        except ElphError as e:
+            skipped.append(f"{m.name}: {e}")
+            continue
+        plans = {pr: interactions.IbzkPlan(w, el, ph, precision=pr) for pr in ("mixed", "fp64")}
+        nprocs = plans["fp64"].nprocs
+        # IBZ k (in istate order) until the pair target or the output budget
+        act_k = plans["fp64"].active.sum(axis=1)
+        rows = np.cumsum(act_k)
+        nk_pairs = int(round(target / el.nwv))
+        nk_out = int(np.searchsorted(rows, args.batched_out_gb * 1e9 / (nprocs * 8), side="right"))
+        nk = int(max(1, min(el.nwv_irred, nk_pairs, nk_out)))
+        entries.append(dict(mat=m, w=w, el=el, plans=plans, last=nk * m.nb, pairs=nk * el.nwv, target=target))
+    log(f"[bench] batched: {len(entries)} plans ready, {len(skipped)} skipped ({time.perf_counter() - t0:.1f} s)")
+    gb = sum(e["mat"].gwann_bytes for e in entries) / 1e9
+    pairs = np.array([e["pairs"] for e in entries])
+    out = {"workload": f"{len(entries)} materials of the multi-material generator (seeds "
+                       f"{0xE1B03000:#x}+i, vertex cap {args.batched_budget_gb:g} GB, window ~4 bands): "
+                       f"calculate_eph_interaction_ibzk('g') on the first IBZ k of each, up to a target of "
+                       f"U[0.5e6, 2e6] k-q pairs or {args.batched_out_gb:g} GB of |g|^2 output per material",
+           "materials": len(entries), "skipped": skipped, "resident_vertex_gb": round(gb, 2),
+           "pairs_total": int(pairs.sum()), "pairs_per_material": [int(pairs.min()), int(np.median(pairs)),
+                                                                   int(pairs.max())],
+           "materials_at_pair_target": int(sum(1 for e in entries if e["pairs"] >= 0.98 * e["target"]
+                                               or e["last"] == e["el"].nwv_irred * e["mat"].nb)),
+           "nb_range": [min(e["mat"].nb for e in entries), max(e["mat"].nb for e in entries)],
+           "nbr_range": [min(e["mat"].nbr for e in entries), max(e["mat"].nbr for e in entries)],
+           "nws_range": [min(e["mat"].nwsk for e in entries), max(e["mat"].nwsk for e in entries)],
+           "streams": args.batched_streams}
+    bufs = [None] * len(entries)
+    probe = {}
+    for prec in ("mixed", "fp64"):
+        b = interactions.Batch(nstreams=args.batched_streams)
+        res = []
+        for i, e in enumerate(entries):
+            c, buf = b.add(e["plans"][prec], 1, e["last"], out=bufs[i])
+            bufs[i] = buf
+            res.append(c)
+        t1 = time.perf_counter()
+        b.prepare(autotune=True)
+        t_prep = time.perf_counter() - t1
+        vert = sum(int(c[c >= 0].sum()) for c in res)
+        b.run()
+        ms = [b.run() for _ in range(args.batched_steps)]
+        tw = time.perf_counter()
+        b.run()
+        wall_ms = (time.perf_counter() - tw) * 1e3
+        cls = b.classes()
+        b.close()
+        probe[prec] = [[bufs[i].row(r) for r in range(min(2, bufs[i].shape[0]))] for i in range(min(16, len(entries)))]
+        out[prec] = {"vertices_per_step": vert, "ms_per_step": float(np.mean(ms)), "ms_steps": [round(x, 2) for x in ms],
+                     "vertices_per_s": vert / (np.mean(ms) * 1e-3), "wall_ms_one_replay": round(wall_ms, 2),
+                     "prepare_s": round(t_prep, 2),
+                     "tile_classes": [{"cols": c["cols"], "kp": c["kp"], "members": c["members"],
+                                       "path": "fused" if c["path"] == 0 else "column-tiled",
+                                       "cb": c["cb"], "rb": c["rb"], "candidates": c["candidates"],
+                                       "tuned_ms": round(c["ms"], 3)} for c in cls]}
+        log(f"[bench] batched {prec}: {vert:.3e} vertices per step, {np.mean(ms):.1f} ms per graph replay, "
+            f"{vert / (np.mean(ms) * 1e-3):.3e} vertices/s, {len(cls)} size classes (prepare {t_prep:.1f} s)")
+    rel = []
+    for rm, r6 in zip(probe["mixed"], probe["fp64"]):
+        for a, bb in zip(r6, rm):
+            sel = np.abs(a) > 1e-6 * np.abs(a).max()
+            rel.append(np.abs(bb[sel] - a[sel]) / np.abs(a[sel]))
+    rel = np.concatenate(rel) if rel else np.zeros(1)
+    out["mixed_vs_fp64"] = round(out["mixed"]["vertices_per_s"] / out["fp64"]["vertices_per_s"], 3)
+    out["error_mixed_vs_fp64"] = {"max_rel": float(rel.max()), "median_rel": float(np.median(rel)),
+                                  "entries": int(rel.size), "rows": "first 2 in-window states of 16 materials",
+                                  "bound_met": bool(rel.max() <= 1e-5)}
+    del bufs, entries
+    gc.collect()
     return out
 
 
@@ -226,11 +352,18 @@ def main():
     ap.add_argument("--serving-budget-gb", type=float, default=8.0)
     ap.add_argument("--serving-pool-gb", type=float, default=128.0)
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-materials", type=int, default=64)
+    ap.add_argument("--batched-budget-gb", type=float, default=2.0)
+    ap.add_argument("--batched-out-gb", type=float, default=1.5)
+    ap.add_argument("--batched-streams", type=int, default=4)
+    ap.add_argument("--batched-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not run)"}))
         sys.exit(2)
+    gen = None if args.no_batched else start_batched_generation(args)
 
     from elphbolt_amd import bz, interactions, synth
     from elphbolt_amd.wannier import Wannier
@@ -333,8 +466,13 @@ def main():
                       "vertices_per_step": res_mx["vertices"] // args.steps},
            "mixed_vs_fp64": round(res_mx["value"] / res_64["value"], 3), "error_vs_fp64": err,
            "fp64": fp64, "roofline": roofline}
+    if not args.no_batched:
+        log_mem("before batched")
+        res["batched"] = batched_bench(args, gen)
+        log_mem("after batched")
     if not args.no_serving:
         res["serving"] = serving_bench(args)
+        log_mem("after serving")
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) ...")
         first_active = None

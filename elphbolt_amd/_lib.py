@@ -91,6 +91,7 @@ PROTOTYPES = {
     "elph_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
     "elph_synth_vertex": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
+    "elph_device_meminfo": (ct.c_int, [ct.c_int, c_i64p, c_i64p]),
     "elph_batch_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int32]),
     "elph_batch_add": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, c_i64p]),
     "elph_batch_prepare": (ct.c_int, [ct.c_void_p, ct.c_int32]),
@@ -135,3 +136,10 @@ class ElphError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[elph {code}] {msg}")
         self.code = code
+
+
+def device_meminfo(device: int = 0):
+    """(free, total) bytes of device memory."""
+    f, t = ct.c_int64(0), ct.c_int64(0)
+    load().elph_device_meminfo(device, ct.byref(f), ct.byref(t))
+    return f.value, t.value

@@ -2467,6 +2467,14 @@ int elph_ctx_synchronize(elph_ctx *ctx) {
     return hipDeviceSynchronize() == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
 }
 
+int elph_device_meminfo(int device, int64_t *free_bytes, int64_t *total_bytes) {
+    size_t f = 0, t = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&f, &t) != hipSuccess) return ELPH_EDEVICE;
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+    return ELPH_OK;
+}
+
 int elph_synth_vertex(elph_ctx *ctx, uint64_t seed) {
     if (!ctx) return ELPH_EINVAL;
     try {
@@ -2719,10 +2727,11 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
 }
 
 // Active states of first..last (counts -1 for out-of-window states) grouped by
-// up to 64 IBZ k, with their device tables; st_slot is the output row relative
-// to `first` when on_device, else the row within the group.
+// up to 64 IBZ k, with their device tables; st_slot is the output row: relative
+// to `first` when on_device (or, compact, the index among active states), else
+// the row within the group.
 void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device, int64_t *counts, hipStream_t s,
-                 std::vector<std::unique_ptr<IbzkGroup>> &groups) {
+                 std::vector<std::unique_ptr<IbzkGroup>> &groups, bool compact = false) {
     const int64_t nb = p->nb, nb2 = nb * nb;
     struct St { int64_t ik, m, slot; };
     std::vector<St> act;
@@ -2733,7 +2742,7 @@ void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device,
         const int64_t t = istate - first;
         if (std::fabs(e - p->enref) > p->fsthick) { counts[t] = -1; continue; }
         counts[t] = p->nprocs;
-        act.push_back({ik - 1, m - 1, t});
+        act.push_back({ik - 1, m - 1, compact ? (int64_t)act.size() : t});
     }
     const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
     size_t i0 = 0;
@@ -3207,7 +3216,7 @@ int elph_batch_add(elph_batch *b, elph_ibzk_plan *plan, int64_t first, int64_t l
         it.first = first;
         it.last = last;
         it.out = g2_device;
-        ibzk_groups(plan, first, last, true, counts, b->streams[0], it.groups);
+        ibzk_groups(plan, first, last, true, counts, b->streams[0], it.groups, true);
         int64_t nst = 0;
         for (auto &g : it.groups) nst += g->nst;
         it.work = (double)nst * plan->nwv * plan->Kp * plan->nb * plan->nbr;

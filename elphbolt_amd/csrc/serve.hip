@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -474,9 +475,15 @@ struct Pool {
     std::map<int64_t, int64_t> free_;  // offset -> bytes
     void init(int64_t bytes) {
         bytes = rup(bytes, 256);
-        if (hipMalloc(&base, bytes) != hipSuccess) {
+        // memory freed just before (e.g. by another server) is returned by the
+        // driver asynchronously: retry for up to 5 s
+        for (int attempt = 0;; ++attempt) {
+            if (hipMalloc(&base, bytes) == hipSuccess) break;
             base = nullptr;
-            throw Error{ELPH_ENOMEM, "server pool: hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+            (void)hipGetLastError();
+            if (attempt == 50)
+                throw Error{ELPH_ENOMEM, "server pool: hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+            std::this_thread::sleep_for(std::chrono::milliseconds(100));
         }
         size = bytes;
         free_.clear();
@@ -575,6 +582,7 @@ struct elph_server {
     std::vector<std::unique_ptr<Slot>> slots;
     hipStream_t setup_st = nullptr, copy_st = nullptr;
     int *h_flags = nullptr;        // pinned, kFlagRing entries
+    std::vector<hipEvent_t> free_events;  // set-up events for reuse (created by the scheduler)
     std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
     std::deque<int64_t> pending;   // submitted, not admitted (FIFO)
     std::vector<int64_t> active;   // admitted, not done (arrival order)
@@ -696,6 +704,12 @@ bool admit(elph_server *s) {
     r.d_flag = reinterpret_cast<int *>(b + L.flag);
     r.nq64 = rup(r.nq, 64);
     r.admitted = true;
+    if (!s->free_events.empty()) {
+        r.ev_setup = s->free_events.back();
+        s->free_events.pop_back();
+    } else {
+        HCHECK(hipEventCreateWithFlags(&r.ev_setup, hipEventDisableTiming));
+    }
     // set-up on the set-up stream: phonons at the q-list, phases, digits
     hipStream_t st = s->setup_st;
     const int64_t nq = r.nq, nbr = m.nbr;
@@ -1058,8 +1072,8 @@ int64_t elph_server_submit(elph_server *s, int32_t material, int64_t nk, const d
         r->nq = nq;
         r->kv.assign(kvecs, kvecs + 3 * nk);
         r->qv.assign(qvecs, qvecs + 3 * nq);
-        HCHECK(hipSetDevice(s->device));
-        HCHECK(hipEventCreateWithFlags(&r->ev_setup, hipEventDisableTiming));
+        // no HIP call on the caller's thread: device objects of a request are
+        // made by the scheduler thread
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
         r->h_flag = s->h_flags + (r->id % kFlagRing);
@@ -1147,9 +1161,9 @@ int elph_server_release(elph_server *s, int64_t id) {
         r = std::move(it->second);
         s->reqs.erase(it);
         if (r->off >= 0) s->pool.release(r->off, r->bytes);
+        if (r->ev_setup) s->free_events.push_back(r->ev_setup);
         s->cv_work.notify_one();
     }
-    if (r->ev_setup) (void)hipEventDestroy(r->ev_setup);
     return ELPH_OK;
 }
 
@@ -1186,6 +1200,7 @@ int elph_server_destroy(elph_server *s) {
     if (s->setup_st) (void)hipStreamSynchronize(s->setup_st);
     for (auto &kv : s->reqs)
         if (kv.second->ev_setup) (void)hipEventDestroy(kv.second->ev_setup);
+    for (auto e : s->free_events) (void)hipEventDestroy(e);
     if (s->h_flags) (void)hipHostFree(s->h_flags);
     for (auto &sp : s->slots) {
         if (sp->h_kv) (void)hipHostFree(sp->h_kv);
@@ -1194,7 +1209,16 @@ int elph_server_destroy(elph_server *s) {
     }
     if (s->setup_st) (void)hipStreamDestroy(s->setup_st);
     if (s->copy_st) (void)hipStreamDestroy(s->copy_st);
+    hipError_t fe = hipSuccess;
+    if (s->pool.base) {
+        fe = hipFree(s->pool.base);
+        s->pool.base = nullptr;
+    }
     delete s;
+    if (fe != hipSuccess) {
+        std::fprintf(stderr, "elph_server_destroy: hipFree of the pool: %s\n", hipGetErrorString(fe));
+        return ELPH_EDEVICE;
+    }
     return ELPH_OK;
 }
 

@@ -66,6 +66,7 @@ class IbzkPlan:
         self.wann, self.lib = wann, wann.lib
         self.nb, self.nbr = wann.numwannbands, wann.numbranches
         self.nwv_irred = int(el.nwv_irred)
+        self.active = (np.abs(np.asarray(el.ens_irred) - el.enref) <= el.fsthick)  # (nirr, nb) in-window states
         es, k1 = _el_struct(el)
         ps, k2 = _ph_struct(ph)
         plan = ct.c_void_p()
@@ -126,13 +127,19 @@ class Batch:
             msg = self.lib.elph_batch_error(self.h)
             raise ElphError(st, f"{what}: {msg.decode() if msg else ''}")
 
-    def add(self, plan: IbzkPlan, istate_first: int = 1, istate_last=None):
-        """Adds a plan's states; returns (counts, device output (nst, nprocs))."""
+    def n_active(self, plan: IbzkPlan, istate_first: int, istate_last: int) -> int:
+        s = np.arange(istate_first - 1, istate_last)
+        return int(plan.active[s // plan.nb, s % plan.nb].sum())
+
+    def add(self, plan: IbzkPlan, istate_first: int = 1, istate_last=None, out=None):
+        """Adds a plan's states; returns (counts, device output (n_active, nprocs)):
+        one row per in-window state in istate order (counts -1 elsewhere)."""
         if istate_last is None:
             istate_last = plan.nwv_irred * plan.nb
         nst = istate_last - istate_first + 1
         counts = np.zeros(nst, np.int64)
-        out = plan.wann.device_buffer((nst, plan.nprocs))
+        if out is None:
+            out = plan.wann.device_buffer((max(1, self.n_active(plan, istate_first, istate_last)), plan.nprocs))
         self._check(self.lib.elph_batch_add(self.h, plan.plan, istate_first, istate_last, out.ptr,
                                             _p(counts, c_i64p)), "batch add")
         self.entries.append((plan, counts, out))

@@ -334,7 +334,7 @@ _MASSES = [28.0855, 12.0107, 15.999, 24.305, 69.723, 74.92, 10.81, 30.97, 26.98,
 
 
 def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", window: bool = True,
-                   seed_base: int = MULTI_SEED_BASE) -> Material:
+                   seed_base: int = MULTI_SEED_BASE, bands_in_window=None) -> Material:
     """Material i of the multi-material generator: atoms per cell uniform in
     1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
     uniform in 4..9 (about 150..900 cells), fine mesh from {12, 16, 20, 24}^3.
@@ -360,7 +360,7 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
         basis[:, j] = (j / nat) * np.ones(3) + 0.05 * rng.standard_normal(3)
     return make_material(f"multi{i}_nb{nb}_na{nat}_ws{n}_k{wv}", a, masses, nb, [n] * 3, [wv] * 3,
                          seed_base + i, basis=basis, vertex=vertex, window=window,
-                         bands_in_window=nb / 2.0)
+                         bands_in_window=nb / 2.0 if bands_in_window is None else bands_in_window)
 
 
 def tiny(nb=2, natoms=1, ws=3, wv=4, seed=7, bands_in_window=None, mesh_ref=1) -> Material:

@@ -189,6 +189,15 @@ class DeviceBuffer:
         wann._check(wann.lib.elph_device_alloc(wann.ctx, self.nbytes, ct.byref(p)), "device_alloc")
         self.ptr = p
 
+    def row(self, i: int):
+        """Row i of a 2-D buffer (one device-to-host copy of that row)."""
+        n = int(np.prod(self.shape[1:]))
+        host = np.empty(n, self.dtype)
+        off = ct.c_void_p(int(self.ptr.value) + int(i) * n * self.dtype.itemsize)
+        self.wann._check(self.wann.lib.elph_device_to_host(self.wann.ctx, host.ctypes.data_as(ct.c_void_p), off,
+                                                           n * self.dtype.itemsize), "device_to_host")
+        return host
+
     def numpy(self):
         host = np.empty(self.shape, self.dtype)
         self.wann._check(self.wann.lib.elph_device_to_host(self.wann.ctx, host.ctypes.data_as(ct.c_void_p), self.ptr,

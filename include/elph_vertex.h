@@ -194,6 +194,9 @@ int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t byt
  * boundary after a driver, src/interactions.f90:2387). */
 int elph_ctx_synchronize(elph_ctx *ctx);
 
+/* Free and total device memory of a GPU (bytes).  No counterpart in the reference. */
+int elph_device_meminfo(int device, int64_t *free_bytes, int64_t *total_bytes);
+
 /* Fills the context's Wannier vertex in HBM with the synthetic law of the
  * benchmark workloads (|g| = 0.1 eV e^{-|Re|/2 A - |Rp|/2 A}, phase uniform in
  * [0, 2 pi) from a counter-based hash of (seed, element)); pass gwann = NULL to
@@ -214,7 +217,8 @@ int elph_last_timing(const elph_ctx *ctx, double out[3]);
  * work of several calculate_eph_interaction_ibzk('g') calls at once).  Each
  * added plan contributes the in-window IBZ states first..last of its material,
  * |g|^2 rows written to g2_device (device memory, row stride = the plan's
- * nprocs) and counts filled as elph_ibzk_plan_run does.  prepare() picks the
+ * nprocs; one row per in-window state, in istate order) and counts filled as
+ * elph_ibzk_plan_run does (-1 for out-of-window states).  prepare() picks the
  * pair-stage tile shape of each size class (nb*nbr and Kp buckets) by timing
  * the candidates on the class's first group when autotune != 0, assigns
  * plans to streams and captures the whole step into one hipGraph; run()
