@@ -352,6 +352,10 @@ def main():
     ap.add_argument("--serving-budget-gb", type=float, default=8.0)
     ap.add_argument("--serving-pool-gb", type=float, default=128.0)
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--profile", action="store_true",
+                    help="config-1 pair kernels only (no serving, batched or CPU sections), after a "
+                         "device-to-device copy of --calibration-copy-gb GiB, for hardware-counter runs")
+    ap.add_argument("--calibration-copy-gb", type=float, default=1.0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-materials", type=int, default=64)
     ap.add_argument("--batched-budget-gb", type=float, default=2.0)
@@ -363,6 +367,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not run)"}))
         sys.exit(2)
+    if args.profile:
+        args.no_batched = args.no_serving = args.no_cpu_baseline = True
     gen = None if args.no_batched else start_batched_generation(args)
 
     from elphbolt_amd import bz, interactions, synth
@@ -382,6 +388,14 @@ def main():
     kb = el.nwv_irred if args.kblock <= 0 else min(args.kblock, el.nwv_irred)
     nblocks = max(1, el.nwv_irred // kb)
     nm_avg = float((np.abs(el.ens_irred - el.enref) <= el.fsthick).sum()) / max(1, el.nwv_irred)
+    if args.profile:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE calibrate the counters
+        nbytes = int(args.calibration_copy_gb * (1 << 30))
+        src, dst = w.device_buffer((nbytes // 8,)), w.device_buffer((nbytes // 8,))
+        dst.copy_from(src)
+        log(f"[bench] calibration copy of {nbytes} bytes done")
+        src.free()
+        dst.free()
     plans = {pr: interactions.IbzkPlan(w, el, ph, mesh_ref=1, precision=pr) for pr in ("mixed", "fp64")}
     nprocs = plans["fp64"].nprocs
     out = w.device_buffer((kb * nb, nprocs))

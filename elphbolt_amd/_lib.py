@@ -92,6 +92,7 @@ PROTOTYPES = {
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),
     "elph_synth_vertex": (ct.c_int, [ct.c_void_p, ct.c_uint64]),
     "elph_device_meminfo": (ct.c_int, [ct.c_int, c_i64p, c_i64p]),
+    "elph_device_copy": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]),
     "elph_batch_create": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_int32]),
     "elph_batch_add": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int64, ct.c_void_p, c_i64p]),
     "elph_batch_prepare": (ct.c_int, [ct.c_void_p, ct.c_int32]),

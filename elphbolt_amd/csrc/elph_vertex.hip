@@ -355,7 +355,8 @@ __global__ void k_slice_b(int nst, int nkc, int CB, int Kp, int CP, const double
     const int st = (int)(r / nkc);
     const int col = cb * 16 + (lane & 15), g = lane >> 4;
     const double inv = 1.0 / scale[(size_t)st * CP + col];
-    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS], unim[kNS];
+    unsigned ure[kNS][4] = {}, uim[kNS][4] = {}, unim[kNS][4] = {};
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
         const int k = kc * 64 + g * 16 + b;
         const double2 x = h[((size_t)st * Kp + k) * CP + col];
@@ -364,17 +365,17 @@ __global__ void k_slice_b(int nst, int nkc, int CB, int Kp, int CP, const double
         split_digits<kNS>(x.y * inv, di);
 #pragma unroll
         for (int sd = 0; sd < kNS; ++sd) {
-            ure[sd].b[b] = dr[sd];
-            uim[sd].b[b] = di[sd];
-            unim[sd].b[b] = (signed char)(-di[sd]);
+            put_byte(ure[sd], b, dr[sd]);
+            put_byte(uim[sd], b, di[sd]);
+            put_byte(unim[sd], b, (signed char)(-di[sd]));
         }
     }
     v4i *o = bsl + ((((size_t)st * nkc + kc) * CB + cb) * 3) * kNS * 64 + lane;
 #pragma unroll
     for (int sd = 0; sd < kNS; ++sd) {
-        o[(0 * kNS + sd) * 64] = ure[sd].v;
-        o[(1 * kNS + sd) * 64] = uim[sd].v;
-        o[(2 * kNS + sd) * 64] = unim[sd].v;
+        o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
+        o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
+        o[(2 * kNS + sd) * 64] = as_v4i(unim[sd]);
     }
 }
 
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) pf[j] = src[i];
+            if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
         }
         const v4i *B = (kc & 1) ? buf1 : buf0;
         // per B digit j: first the Re(A) products of every (level, column
@@ -497,7 +498,7 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) dst[i] = pf[j];
+            if (BCH % 256 == 0 || i < BCH) dst[i] = pf[j];
         }
         __syncthreads();
     }
@@ -648,7 +649,7 @@ __global__ __launch_bounds__(256, 2) void k_ibzk_y_i8(PairArgs a, int st0, const
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) pf[j] = src[i];
+            if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
         }
         const v4i *B = (kc & 1) ? buf1 : buf0;
 #pragma unroll
@@ -683,7 +684,7 @@ __global__ __launch_bounds__(256, 2) void k_ibzk_y_i8(PairArgs a, int st0, const
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) dst[i] = pf[j];
+            if (BCH % 256 == 0 || i < BCH) dst[i] = pf[j];
         }
         __syncthreads();
     }
@@ -826,7 +827,8 @@ __global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc,
     const int64_t slice = r / nrb;
     const int i = rb * 16 + (lane & 15), g = lane >> 4;
     const double inv = 1.0 / gscale[slice * nrb * 16 + i];
-    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS];
+    unsigned ure[kNS][4] = {}, uim[kNS][4] = {};
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
         const int k = kc * 64 + g * 16 + b;
         double2 x = make_double2(0.0, 0.0);
@@ -835,13 +837,13 @@ __global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc,
         split_digits<kNS>(x.x * inv, dr);
         split_digits<kNS>(x.y * inv, di);
 #pragma unroll
-        for (int sd = 0; sd < kNS; ++sd) { ure[sd].b[b] = dr[sd]; uim[sd].b[b] = di[sd]; }
+        for (int sd = 0; sd < kNS; ++sd) { put_byte(ure[sd], b, dr[sd]); put_byte(uim[sd], b, di[sd]); }
     }
     v4i *o = gdig + ((((size_t)slice * nrb + rb) * nkc + kc) * 2) * kNS * 64 + lane;
 #pragma unroll
     for (int sd = 0; sd < kNS; ++sd) {
-        o[(0 * kNS + sd) * 64] = ure[sd].v;
-        o[(1 * kNS + sd) * 64] = uim[sd].v;
+        o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
+        o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
     }
 }
 
@@ -853,7 +855,8 @@ __global__ void k_phase_digits(int nwsk, int nkc, int ncb, const double2 *__rest
     if (t >= nkc * ncb * 64) return;
     const int lane = t % 64, cb = (t / 64) % ncb, kc = t / (64 * ncb);
     const int col = cb * 16 + (lane & 15), g = lane >> 4;
-    union { signed char b[16]; v4i v; } ure[kNS], uim[kNS], unim[kNS];
+    unsigned ure[kNS][4] = {}, uim[kNS][4] = {}, unim[kNS][4] = {};
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
         const int k = kc * 64 + g * 16 + b;
         double2 x = make_double2(0.0, 0.0);
@@ -863,17 +866,17 @@ __global__ void k_phase_digits(int nwsk, int nkc, int ncb, const double2 *__rest
         split_digits<kNS>(x.y, di);
 #pragma unroll
         for (int sd = 0; sd < kNS; ++sd) {
-            ure[sd].b[b] = dr[sd];
-            uim[sd].b[b] = di[sd];
-            unim[sd].b[b] = (signed char)(-di[sd]);
+            put_byte(ure[sd], b, dr[sd]);
+            put_byte(uim[sd], b, di[sd]);
+            put_byte(unim[sd], b, (signed char)(-di[sd]));
         }
     }
     v4i *o = bphi + (((size_t)kc * ncb + cb) * 3) * kNS * 64 + lane;
 #pragma unroll
     for (int sd = 0; sd < kNS; ++sd) {
-        o[(0 * kNS + sd) * 64] = ure[sd].v;
-        o[(1 * kNS + sd) * 64] = uim[sd].v;
-        o[(2 * kNS + sd) * 64] = unim[sd].v;
+        o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
+        o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
+        o[(2 * kNS + sd) * 64] = as_v4i(unim[sd]);
     }
 }
 
@@ -2465,6 +2468,18 @@ int elph_ctx_synchronize(elph_ctx *ctx) {
     if (!ctx) return ELPH_EINVAL;
     if (hipSetDevice(ctx->device) != hipSuccess) return ELPH_EDEVICE;
     return hipDeviceSynchronize() == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
+}
+
+int elph_device_copy(elph_ctx *ctx, void *dst, const void *src, int64_t bytes) {
+    if (!ctx || bytes < 0) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        HCHECK(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        HCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
 }
 
 int elph_device_meminfo(int device, int64_t *free_bytes, int64_t *total_bytes) {

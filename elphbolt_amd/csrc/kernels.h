@@ -9,6 +9,10 @@
 namespace elph {
 
 typedef double __attribute__((ext_vector_type(4))) d4;
+// native 2-double vector for staging copies (HIP's double2 is a union-based
+// struct that the compiler does not keep in registers when arrays of it are
+// carried across branches)
+typedef double __attribute__((ext_vector_type(2))) dv2;
 
 // D(16x16) += A(16x4) B(4x16) in FP64 on the matrix cores.  Operand layout
 // (checked by elph_selftest_mfma): A[i][k] in lane i + 16 k, B[k][j] in lane
@@ -64,6 +68,15 @@ __host__ __device__ inline void split_digits(double x, signed char (&d)[NS]) {
     }
 }
 
+// Packs 16 signed bytes (one per b of an unrolled loop) into a v4i: byte b of
+// word b / 4 (little endian), as the int8 MFMA operand registers hold them.
+__device__ __forceinline__ void put_byte(unsigned (&w)[4], int b, signed char v) {
+    w[b >> 2] |= (unsigned)(unsigned char)v << (8 * (b & 3));
+}
+__device__ __forceinline__ v4i as_v4i(const unsigned (&w)[4]) {
+    return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+}
+
 // 4x4 byte transpose: o_e byte b = a_b byte e (v_perm_b32 selects from the
 // 8 bytes {S0:S1}, S1 = bytes 0-3).
 __device__ __forceinline__ void transpose4x4_bytes(unsigned a0, unsigned a1, unsigned a2, unsigned a3, unsigned &o0,
@@ -97,18 +110,21 @@ __device__ __forceinline__ void gemm_core_f64(const double2 *__restrict__ B, int
         for (int c = 0; c < CB; ++c) { cr[rb][c] = d4{0, 0, 0, 0}; ci[rb][c] = d4{0, 0, 0, 0}; }
     constexpr int NPF = (KC * CP + 255) / 256;
     constexpr int chunk = KC * CP;
+    // with chunk a multiple of the block size every prefetch slot is live; the
+    // guard would make the compiler keep pf[] in scratch memory
+    constexpr bool exact = chunk % 256 == 0;
     for (int i = tid; i < chunk; i += 256) buf0[i] = B[(size_t)(i / CP) * ldb + i % CP];
     __syncthreads();
     const int nchunks = Kp / KC;
     for (int ch = 0; ch < nchunks; ++ch) {
-        double2 pf[NPF];
+        dv2 pf[NPF];
         const bool more = ch + 1 < nchunks;
         if (more) {
-            const double2 *src = B + (size_t)(ch + 1) * KC * ldb;
+            const dv2 *src = reinterpret_cast<const dv2 *>(B + (size_t)(ch + 1) * KC * ldb);
 #pragma unroll
             for (int j = 0; j < NPF; ++j) {
                 const int i = tid + 256 * j;
-                if (i < chunk) pf[j] = src[(size_t)(i / CP) * ldb + i % CP];
+                if (exact || i < chunk) pf[j] = src[(size_t)(i / CP) * ldb + i % CP];
             }
         }
         const double2 *Bs = (ch & 1) ? buf1 : buf0;
@@ -131,11 +147,11 @@ __device__ __forceinline__ void gemm_core_f64(const double2 *__restrict__ B, int
             }
         }
         if (more) {
-            double2 *dst = (ch & 1) ? buf0 : buf1;
+            dv2 *dst = reinterpret_cast<dv2 *>((ch & 1) ? buf0 : buf1);
 #pragma unroll
             for (int j = 0; j < NPF; ++j) {
                 const int i = tid + 256 * j;
-                if (i < chunk) dst[i] = pf[j];
+                if (exact || i < chunk) dst[i] = pf[j];
             }
         }
         __syncthreads();

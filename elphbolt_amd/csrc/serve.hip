@@ -136,7 +136,8 @@ __global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const doub
     const int kc = (int)((t / 64) % nkc);
     const int qb = (int)(t / (64 * nkc));
     const int q = qb * 16 + (lane & 15), g = lane >> 4;
-    union { signed char b[16]; v4i v; } ur[kNS], ui[kNS];
+    unsigned ur[kNS][4] = {}, ui[kNS][4] = {};
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
         double2 x = make_double2(0.0, 0.0);
         if (q < nq) x = phi[(size_t)(kc * 64 + g * 16 + b) * ldp + q];
@@ -145,15 +146,15 @@ __global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const doub
         split_digits<kNS>(x.y, di);
 #pragma unroll
         for (int d = 0; d < kNS; ++d) {
-            ur[d].b[b] = dr[d];
-            ui[d].b[b] = di[d];
+            put_byte(ur[d], b, dr[d]);
+            put_byte(ui[d], b, di[d]);
         }
     }
     v4i *o = adig + (((size_t)qb * nkc + kc) * 2) * kNS * 64 + lane;
 #pragma unroll
     for (int d = 0; d < kNS; ++d) {
-        o[d * 64] = ur[d].v;
-        o[(kNS + d) * 64] = ui[d].v;
+        o[d * 64] = as_v4i(ur[d]);
+        o[(kNS + d) * 64] = as_v4i(ui[d]);
     }
 }
 
@@ -175,26 +176,29 @@ __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ 
     const int st = blockIdx.y, col0 = blockIdx.z * CP;
     const int row0 = blockIdx.x * ROWS;
     const double2 *B = h + (size_t)(st0 + st) * Kp * CPT + col0;
-    double2 pa[NA], pb[NB];
+    dv2 pa[NA], pb[NB];
+    const dv2 *phv = reinterpret_cast<const dv2 *>(phiT);
+    const dv2 *Bv = reinterpret_cast<const dv2 *>(B);
+    dv2 *Asv = reinterpret_cast<dv2 *>(As), *Bsv = reinterpret_cast<dv2 *>(Bs);
     auto load = [&](int ch) {
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int i = tid + 256 * j, k = i / ROWS, r = i % ROWS;
-            pa[j] = (row0 + r < nqs) ? phiT[(size_t)(ch * KC + k) * ldp + q0 + row0 + r] : make_double2(0.0, 0.0);
+            pa[j] = (row0 + r < nqs) ? phv[(size_t)(ch * KC + k) * ldp + q0 + row0 + r] : dv2{0.0, 0.0};
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const int i = tid + 256 * j;
-            if (i < KC * CP) pb[j] = B[(size_t)(ch * KC + i / CP) * CPT + i % CP];
+            if ((KC * CP) % 256 == 0 || i < KC * CP) pb[j] = Bv[(size_t)(ch * KC + i / CP) * CPT + i % CP];
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int j = 0; j < NA; ++j) As[tid + 256 * j] = pa[j];
+        for (int j = 0; j < NA; ++j) Asv[tid + 256 * j] = pa[j];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const int i = tid + 256 * j;
-            if (i < KC * CP) Bs[i] = pb[j];
+            if ((KC * CP) % 256 == 0 || i < KC * CP) Bsv[i] = pb[j];
         }
     };
     d4 cr[RB][CB], ci[RB][CB];
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) pf[j] = src[i];
+            if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
         }
         v4i nAr[NS], nAi[NS];
         const v4i *an = ast + (size_t)kn * 2 * NS * 64;
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
             const int i = tid + 256 * j;
-            if (i < BCH) dst[i] = pf[j];
+            if (BCH % 256 == 0 || i < BCH) dst[i] = pf[j];
         }
         __syncthreads();
     }

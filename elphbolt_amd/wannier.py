@@ -189,6 +189,11 @@ class DeviceBuffer:
         wann._check(wann.lib.elph_device_alloc(wann.ctx, self.nbytes, ct.byref(p)), "device_alloc")
         self.ptr = p
 
+    def copy_from(self, other: "DeviceBuffer"):
+        """Device-to-device copy of other's bytes into this buffer."""
+        n = min(self.nbytes, other.nbytes)
+        self.wann._check(self.wann.lib.elph_device_copy(self.wann.ctx, self.ptr, other.ptr, n), "device_copy")
+
     def row(self, i: int):
         """Row i of a 2-D buffer (one device-to-host copy of that row)."""
         n = int(np.prod(self.shape[1:]))

@@ -194,6 +194,9 @@ int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t byt
  * boundary after a driver, src/interactions.f90:2387). */
 int elph_ctx_synchronize(elph_ctx *ctx);
 
+/* Device-to-device copy on the context's stream (synchronous).  No counterpart
+ * in the reference; used to calibrate hardware memory counters. */
+int elph_device_copy(elph_ctx *ctx, void *dst, const void *src, int64_t bytes);
 /* Free and total device memory of a GPU (bytes).  No counterpart in the reference. */
 int elph_device_meminfo(int device, int64_t *free_bytes, int64_t *total_bytes);
 
