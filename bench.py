@@ -459,8 +459,12 @@ def main():
     spl64 = res_64["states_per_launch"]
     fl64 = algorithmic_flops(spl64, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg)
     ach64 = fl64 / res_64["avg_launch_s"] / 1e12
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "ibzk_pair_i8_sic_ws8_k24_traffic.json")
+    if mat.name == "sic_like_ws8_k24" and os.path.exists(tfile):
+        traffic = json.load(open(tfile))["bytes_per_launch"]   # rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
     roofline = {"bound": "mfma", "achieved": round(ach_i8, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-                "frac": round(ach_i8 / INT8_PEAK_TOPS, 4), "traffic": None, "kernel": "k_pair_i8<3>",
+                "frac": round(ach_i8 / INT8_PEAK_TOPS, 4), "traffic": traffic, "kernel": "k_pair_i8<3>",
                 "avg_launch_ms": round(res_mx["avg_launch_s"] * 1e3, 3),
                 "ops_per_launch": i8_ops, "fp64_equivalent_tflops": round(f64eq_mx, 2)}
     fp64 = {"value": res_64["value"], "ms_per_step": res_64["elapsed"] / args.steps * 1e3,
