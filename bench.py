@@ -46,12 +46,30 @@ def log_mem(tag):
     log(f"[bench] device memory {tag}: {f / 2**30:.1f} GiB free of {t / 2**30:.1f} GiB")
 
 
-def algorithmic_flops(nst, nwv, nstates_inwindow, nb, nbr, nwsg):
+def pair_rows(cells):
+    """Rows of the cell-pair form of the Rp contraction: the cells (R, -R) are
+    folded into one cos row and one sin row (a cell without its opposite, R = 0,
+    still takes two), so the phase operand is real."""
+    cells = [tuple(c) for c in np.asarray(cells, np.int64)]
+    present, used, npair = set(cells), set(), 0
+    for c in cells:
+        if c in used:
+            continue
+        used.add(c)
+        neg = tuple(-x for x in c)
+        if neg in present and neg not in used:
+            used.add(neg)
+        npair += 1
+    return 2 * npair
+
+
+def algorithmic_flops(nst, nwv, nstates_inwindow, nb, nbr, k2):
     """Flops of one pair-kernel launch (nst IBZ states x nwv k' rows): the Rp
-    phase contraction of the band-rotated g(k,Rp) (8 nwsg nb nbr per state and
-    row), the final-band and mode rotations (8 nb nbr + 8 nbr^2 per vertex
-    column block) and |g|^2/omega (4 per vertex)."""
-    gemm = 8.0 * nwsg * nb * nbr * nwv * nst
+    phase contraction of the band-rotated g(k,Rp) in cell-pair form (k2 real
+    phase rows times complex columns: 4 k2 nb nbr per state and row), the
+    final-band and mode rotations (8 nb nbr + 8 nbr^2 per vertex column block)
+    and |g|^2/omega (4 per vertex)."""
+    gemm = 4.0 * k2 * nb * nbr * nwv * nst
     epi = nst * nstates_inwindow * (8.0 * nbr * nb + 8.0 * nbr * nbr + 4.0 * nbr)
     return gemm + epi
 
@@ -451,13 +469,13 @@ def main():
            "zeros_match": exact_zero_match, "bound_met": bool(rel.max() <= 1e-5)}
     log(f"[bench] mixed vs fp64: max rel {err['max_rel']:.2e} median {err['median_rel']:.2e}")
 
-    kp64 = ((mat.nwsg + 63) // 64) * 64
+    k2 = pair_rows(mat.rcells_q[:mat.nwsg])
     spl = res_mx["states_per_launch"]
-    i8_ops = spl * el.nwv * 10 * 4 * 2.0 * mat.nwsg * nb * nbr   # 10 digit products x 4 real products
+    i8_ops = spl * el.nwv * 10 * 2 * 2.0 * k2 * nb * nbr   # 10 digit products x (Re, Im) columns
     ach_i8 = i8_ops / res_mx["avg_launch_s"] / 1e12
-    f64eq_mx = algorithmic_flops(spl, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg) / res_mx["avg_launch_s"] / 1e12
+    f64eq_mx = algorithmic_flops(spl, el.nwv, el.nstates_inwindow, nb, nbr, k2) / res_mx["avg_launch_s"] / 1e12
     spl64 = res_64["states_per_launch"]
-    fl64 = algorithmic_flops(spl64, el.nwv, el.nstates_inwindow, nb, nbr, mat.nwsg)
+    fl64 = algorithmic_flops(spl64, el.nwv, el.nstates_inwindow, nb, nbr, k2)
     ach64 = fl64 / res_64["avg_launch_s"] / 1e12
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "ibzk_pair_i8_sic_ws8_k24_traffic.json")

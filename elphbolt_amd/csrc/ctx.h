@@ -102,6 +102,10 @@ namespace elph {
 
 int ret_error(elph_ctx *ctx, const Error &e);
 int64_t lcm64(int64_t a, int64_t b);
+// Pairs (i, j) of opposite cells R_j = -R_i among the first n columns of the
+// 3 x ld column-major cell list, in order of first appearance; j = -1 when -R_i
+// is not in the list (R_i = 0).  Every cell is in exactly one pair.
+std::vector<int2> cell_pairs(const int64_t *cells, int64_t ld, int64_t n);
 
 // Launchers (all enqueue on `st`; none synchronises or allocates).
 // gkRp for nk device k vectors ((nk,3) column-major) into out[k] (nb*nb, nbr*nwsg);
@@ -109,13 +113,19 @@ int64_t lcm64(int64_t a, int64_t b);
 void launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride,
                  bool mixed, double2 *phiT, v4i *bphi);
 void ensure_gwann_digits(elph_ctx *c);  // int8 digits of gwann, once per context (synchronous)
-// h[st][Kp][CP] = sum_mp U(kb,m,mp) gk[kb](np,mp,sp,Rp) / phwsdeg(Rp)
-void launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *gk,
+// h(Rp) = sum_mp U(kb,m,mp) gk[kb](np,mp,sp,Rp) / phwsdeg(Rp), folded over the
+// cell pairs (R_p, -R_p): h[st][2p][CP] = h(R_p) + h(-R_p), h[st][2p+1][CP] =
+// i (h(R_p) - h(-R_p)); pairs[p] = (index of R_p, index of -R_p or -1), Kp even.
+void launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int npair, int Kp, int CP, const double2 *gk,
                      int64_t gk_kstride, const int *st_kb, const int *st_m, const double2 *Uk, const int *degq,
-                     double2 *h);
+                     const int2 *pairs, double2 *h);
 void launch_colscale(hipStream_t st, int nst, int Kp, int CP, const double2 *h, double *scale);
+// int8 B digits [st][kc][cb][Re, Im, -Im][digit][lane] (complex phase operand)
 void launch_slice_b(hipStream_t st, int nst, int nkc, int CB, int Kp, int CP, const double2 *h, const double *scale,
                     v4i *bsl);
+// int8 B digits [st][kc][cb][Re, Im][digit][lane] (real phase operand, cell pairs)
+void launch_slice_b_ri(hipStream_t st, int nst, int nkc, int CB, int Kp, int CP, const double2 *h,
+                       const double *scale, v4i *bsl);
 // Batched zheev('V','U') (optionally on A .* scale): W ascending, V[mat][col j][row]
 void launch_jacobi(hipStream_t st, int64_t nmat, int n, const double2 *A, const double *scale, double *W, double2 *V);
 // Short-range D(q) (plus the polar D_L for polar contexts) at nq device q vectors

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -108,32 +109,47 @@ __global__ __launch_bounds__(256) void k_gkRp(const double2 *__restrict__ gw, in
 }
 
 // ---------------------------------------------------------------------------
-// Band rotation of gkRp for IBZ state (k, m):
-//   h[st][Rp][c] = sum_mp U_k(m,mp) gkRp[k](np,mp,sp,Rp) / phwsdeg(Rp),  c = np + nb sp
-// (the initial-band half of the overlap of src/wannier.f90:960-965).
+// Band rotation of gkRp for IBZ state (k, m), folded over pairs of opposite
+// cells.  With
+//   h(R) = sum_mp U_k(m,mp) gkRp[k](np,mp,sp,R) / phwsdeg(R),  c = np + nb sp
+// (the initial-band half of the overlap of src/wannier.f90:960-965) and the
+// pair (R_p, -R_p) of the Rp list,
+//   F[st][2p][c] = h(R_p) + h(-R_p),   F[st][2p+1][c] = i (h(R_p) - h(-R_p)),
+// so that sum_R e^{i theta(R)} h(R) = sum_p cos(theta_p) F[2p] + sin(theta_p) F[2p+1],
+// theta_p = theta(R_p): the phase operand of the contraction is real.  A cell
+// whose opposite is not in the list (R = 0) has h(-R_p) = 0.
 // ---------------------------------------------------------------------------
-__global__ void k_rotate_h(int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *__restrict__ gk,
+__global__ void k_rotate_h(int nst, int nb, int nbr, int npair, int Kp, int CP, const double2 *__restrict__ gk,
                            int64_t gk_kstride, const int *__restrict__ st_kb, const int *__restrict__ st_m,
                            const double2 *__restrict__ Uk /*[kb][m][mp]*/, const int *__restrict__ degq,
-                           double2 *__restrict__ h) {
+                           const int2 *__restrict__ pairs, double2 *__restrict__ h) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t per = (int64_t)Kp * CP;
+    const int64_t per = (int64_t)(Kp / 2) * CP;
     if (idx >= per * nst) return;
     const int st = (int)(idx / per);
     const int rem = (int)(idx % per);
-    const int rp = rem / CP, c = rem % CP;
-    double2 acc = make_double2(0.0, 0.0);
-    if (rp < nwsg && c < nb * nbr) {
+    const int p = rem / CP, c = rem % CP;
+    double2 f0 = make_double2(0.0, 0.0), f1 = make_double2(0.0, 0.0);
+    if (p < npair && c < nb * nbr) {
         const int np = c % nb, sp = c / nb;
         const int kb = st_kb[st], m = st_m[st];
-        const double2 *g = gk + (size_t)kb * gk_kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
         const double2 *u = Uk + ((size_t)kb * nb + m) * nb;
-        for (int mp = 0; mp < nb; ++mp) cfma(acc, u[mp], g[(size_t)mp * nb]);
-        const double d = (double)degq[rp];
-        acc.x /= d;
-        acc.y /= d;
+        auto rot = [&](int rp) {
+            const double2 *g = gk + (size_t)kb * gk_kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
+            double2 acc = make_double2(0.0, 0.0);
+            for (int mp = 0; mp < nb; ++mp) cfma(acc, u[mp], g[(size_t)mp * nb]);
+            const double d = (double)degq[rp];
+            return make_double2(acc.x / d, acc.y / d);
+        };
+        const int2 pr = pairs[p];
+        const double2 hi = rot(pr.x);
+        const double2 hj = pr.y >= 0 ? rot(pr.y) : make_double2(0.0, 0.0);
+        f0 = make_double2(hi.x + hj.x, hi.y + hj.y);
+        f1 = make_double2(hj.y - hi.y, hi.x - hj.x);
     }
-    h[idx] = acc;
+    double2 *o = h + ((size_t)st * Kp + 2 * p) * CP + c;
+    o[0] = f0;
+    o[CP] = f1;
 }
 
 // ---------------------------------------------------------------------------
@@ -152,6 +168,7 @@ struct PairArgs {
     int mesh_ref;
     int Nph0, Nph1, Nph2;      // phonon mesh
     int L, Lf0, Lf1, Lf2;      // phase table size and L / Nel
+    unsigned Lmul;             // ceil(2^32 / L) (0 for L = 1): index mod L by multiply-high
     const double2 *T;          // [L] e^{i 2pi j / L}
     const int *Rq;             // [Kp][4] cell (padded rows are 0)
     const double2 *Vkp;        // [nwv][n][np] = evecs(ikp, n, np)
@@ -179,6 +196,11 @@ struct PairArgs {
 // written at the reference's position (k' block, rank of n, s).  Polar
 // crystals add lp(q,s) * sum_i conj(U_k'(n,i)) U_k(m,i) to g first
 // (src/wannier.f90:998-1006).
+// NBM > 0: nb <= NBM and nbr <= NBRM; the loops over bands and branches are
+// unrolled to those bounds (guarded by the runtime counts), the row of U_k'
+// is held in registers and every load of a rank is issued before its
+// arithmetic.  NBM == 0: loops over the runtime counts.
+template <int NBM, int NBRM>
 __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int ys, int row, int yi, int lk,
                                                   int st, int k0i, int k1i, int k2i, double *outrow) {
     if (row >= a.nwv) return;
@@ -201,6 +223,62 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
     const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb : nullptr;
     const int nn = a.nwin[row];
     const double2 *Yr = Y + yi * ys;
+    if constexpr (NBM > 0) {
+        const int nb = a.nb, nbr = a.nbr;
+        for (int rank = lk; rank < nn; rank += 4) {
+            const int n = a.wband[row * nb + rank];
+            const double2 *V = a.Vkp + ((size_t)row * nb + n) * nb;
+            double2 v[NBM];
+#pragma unroll
+            for (int np = 0; np < NBM; ++np) v[np] = np < nb ? V[np] : make_double2(0.0, 0.0);
+            // z(sp) = sum_np conj(U_k'(n,np)) Y(np + nb sp)
+            double2 z[NBRM];
+#pragma unroll
+            for (int sp = 0; sp < NBRM; ++sp) {
+                double2 t = make_double2(0.0, 0.0);
+                if (sp < nbr)
+#pragma unroll
+                    for (int np = 0; np < NBM; ++np)
+                        if (np < nb) {
+                            const double2 y = Yr[np + nb * sp];
+                            t.x = fma(v[np].x, y.x, t.x);
+                            t.x = fma(v[np].y, y.y, t.x);
+                            t.y = fma(v[np].x, y.y, t.y);
+                            t.y = fma(-v[np].y, y.x, t.y);
+                        }
+                z[sp] = t;
+            }
+            double2 unm = make_double2(0.0, 0.0);
+            if (lq)
+#pragma unroll
+                for (int i = 0; i < NBM; ++i)
+                    if (i < nb) {
+                        const double2 t = cmulc(v[i], Ukm[i]);
+                        unm.x += t.x;
+                        unm.y += t.y;
+                    }
+            // g(s) = sum_sp u_q(s,sp) z(sp) (+ polar); the stores follow all loads
+            double g2[NBRM];
+#pragma unroll
+            for (int s = 0; s < NBRM; ++s) {
+                g2[s] = 0.0;
+                if (s < nbr) {
+                    double2 gb = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int sp = 0; sp < NBRM; ++sp)
+                        if (sp < nbr) cfma(gb, uq[s * nbr + sp], z[sp]);
+                    if (lq) cfma(gb, lq[s], unm);
+                    const double w = omq[s];
+                    if (w != 0.0) g2[s] = 0.5 * (gb.x * gb.x + gb.y * gb.y) / w * kG2Unit;
+                }
+            }
+            double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
+#pragma unroll
+            for (int s = 0; s < NBRM; ++s)
+                if (s < nbr) o[s] = g2[s];
+        }
+        return;
+    }
     for (int rank = lk; rank < nn; rank += 4) {
         const int n = a.wband[row * a.nb + rank];
         const double2 *V = a.Vkp + ((size_t)row * a.nb + n) * a.nb;
@@ -248,6 +326,15 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
     }
 }
 
+// Real phase operand of the cell-pair rows (k_rotate_h): row K2 = 2p + f of
+// pair p is cos (f = 0) or sin (f = 1) of 2 pi (coef . R_p) / L, read from the
+// table T[j] = e^{i 2 pi j / L}; Rsh[p] holds R_p.
+__device__ __forceinline__ double pair_phase(const double2 *Tsh, const int4 *Rsh, const int (&coef)[3], int L, int k) {
+    const int4 R = Rsh[k >> 1];
+    const double2 t = Tsh[pos_mod(coef[0] * R.x + coef[1] * R.y + coef[2] * R.z, L)];
+    return (k & 1) ? t.y : t.x;
+}
+
 // Phase-contraction core of the mesh drivers: the phase of row r and cell R_K
 // is the exact root of unity T[(coef_r . R_K) mod L] (see gemm_core_f64).
 template <int CB, int RB, int KC>
@@ -262,7 +349,7 @@ __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst,
     gemm_core_f64<CB, RB, KC>(hst, 16 * CB, Kp, aop, buf0, buf1, cr, ci);
 }
 
-template <int CB>
+template <int CB, int NBM, int NBRM>
 __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
     constexpr int KC = 32;
     constexpr int RB = 2;
@@ -280,7 +367,7 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
     const double2 *hst = a.h + (size_t)st * a.Kp * CP;
 
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
+    for (int i = tid; i < a.Kp / 2; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
 
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
     int coef[RB][3];
@@ -299,7 +386,11 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
         }
     }
     d4 cr[RB][CB], ci[RB][CB];
-    phase_gemm_core<CB, RB, KC>(hst, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
+    {
+        const int L = a.L;
+        auto aop = [&](int rb, int k) { return pair_phase(Tsh, Rsh, coef[rb], L, k); };
+        gemm_core_f64<CB, RB, KC>(hst, CP, a.Kp, aop, buf0, buf1, cr, ci);
+    }
 
     // ---- epilogue: per row block, stage Y (16 x CP) of this wave in LDS; the
     // odd row stride YS keeps the 16 rows on distinct LDS banks ----
@@ -313,7 +404,7 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) Y[(lk + 4 * r) * YS + c * 16 + li] = make_double2(cr[rb][c][r], ci[rb][c][r]);
         __syncthreads();
-        ibzk_epilogue_row(a, Y, YS, row_base + rb * 16 + li, li, lk, st, k0i, k1i, k2i, outrow);
+        ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + rb * 16 + li, li, lk, st, k0i, k1i, k2i, outrow);
         __syncthreads();
     }
 }
@@ -379,25 +470,162 @@ __global__ void k_slice_b(int nst, int nkc, int CB, int Kp, int CP, const double
     }
 }
 
-// A digits are generated on the fly: Tdig[j] holds the kNS digits of Re (bytes
-// 0-3) and Im (bytes 4-7) of e^{i 2 pi j / L}.  Rpk[k] packs the cell R_k
-// reduced modulo the electron mesh (bytes 0-2), so the phase index of row q
-// is (c . Rpk) mod L with c_d = q_d L / N_d, all in 24-bit products.  A lane
-// gathers the 16 indices of its row and K slice and byte-transposes the
-// digits into the 2 x kNS operand registers.  One wave owns 16 k' rows and
-// all CP columns of one state; 4 waves share the B chunk staged in LDS.
+// Bsl[st][kc][cb][p][s][lane] (16 bytes): digits of B / scale in the int8 MFMA
+// B-operand order, p = Re, Im (real phase operand: the cell-pair layout of
+// k_rotate_h).
+__global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const double2 *__restrict__ h,
+                             const double *__restrict__ scale, v4i *__restrict__ bsl) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)nst * nkc * CB * 64;
+    if (t >= total) return;
+    const int lane = (int)(t % 64);
+    int64_t r = t / 64;
+    const int cb = (int)(r % CB);
+    r /= CB;
+    const int kc = (int)(r % nkc);
+    const int st = (int)(r / nkc);
+    const int col = cb * 16 + (lane & 15), g = lane >> 4;
+    const double inv = 1.0 / scale[(size_t)st * CP + col];
+    unsigned ure[kNS][4] = {}, uim[kNS][4] = {};
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+        const int k = kc * 64 + g * 16 + b;
+        const double2 x = h[((size_t)st * Kp + k) * CP + col];
+        signed char dr[kNS], di[kNS];
+        split_digits<kNS>(x.x * inv, dr);
+        split_digits<kNS>(x.y * inv, di);
+#pragma unroll
+        for (int sd = 0; sd < kNS; ++sd) {
+            put_byte(ure[sd], b, dr[sd]);
+            put_byte(uim[sd], b, di[sd]);
+        }
+    }
+    v4i *o = bsl + ((((size_t)st * nkc + kc) * CB + cb) * 2) * kNS * 64 + lane;
+#pragma unroll
+    for (int sd = 0; sd < kNS; ++sd) {
+        o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
+        o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
+    }
+}
+
+// Mixed-precision contraction over the cell-pair rows (K2 = 2p: cos, 2p+1:
+// sin; k_rotate_h) for one block of 4 waves:
+//   acc[c][l][0|1] (the wave's 16 rows x column block c) = sum over K2 and
+//   digit pairs i + j = l of A_i(row, K2) B_j(K2, col) for Re|Im of B,
+// exact in int32.  A digits are gathered from the digit table of the roots of
+// unity: Tsh[j] holds the kNS digits of cos (bytes 0-3) and sin (bytes 4-7)
+// of 2 pi j / L, Rsh[p] the cell of pair p reduced modulo the electron mesh
+// (bytes 0-2), so the phase index of row q is (c . Rsh[p]) mod L with
+// c_d = q_d L / N_d (cpk, byte-packed; phase_index).  Lane (li, lg) holds
+// K2 = 64 kk + 16 lg + 4 w + b of chunk kk: pair 32 kk + 8 lg + 2 w + b / 2.
+// B: chunk kc of the block's CB column blocks at bst + kc * bkc
+// ([c][Re, Im][digit][lane] v4i), copied global -> LDS asynchronously two
+// chunks ahead into kPairBufs rotating buffers of BCH v4i (buf); the next
+// chunk's A digits are gathered between the matrix instructions of the current
+// one, into the second of two register sets (the K loop is unrolled by two).
+// The caller fills Tsh and Rsh before the call (the first barrier here
+// publishes them) and owns buf again after it.
+constexpr int kPairBufs = 3;
 template <int CB>
+__device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t bkc, int nkc, const uint2 *Tsh,
+                                             const unsigned *Rsh, unsigned cpk, unsigned lmul, int negL, v4i *buf,
+                                             v4i (&acc)[CB][kNS][2]) {
+    constexpr int NS = kNS;
+    static_assert(NS == 4, "digit packing assumes 4 digits per part");
+    constexpr int BCH = CB * 2 * NS * 64;
+    constexpr int NR = BCH / 256;  // copy instructions per wave and chunk
+    static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = lane >> 4;
+    auto bufp = [&](int kc) { return buf + (kc % kPairBufs) * BCH; };
+    auto copy_b = [&](int kn) {
+        const v4i *src = bst + (size_t)kn * bkc;
+        v4i *dst = bufp(kn);
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+            const int r = 4 * j + wave;
+            lds_dma16(src + r * 64 + lane, dst + r * 64);
+        }
+    };
+    // chunk kc + 1 complete in LDS for the whole block; with `ahead` the copy
+    // of chunk kc + 2 (the wave's last NR copies) stays in flight.  No LDS
+    // stores are pending here, so the barrier needs no fence; the memory
+    // clobber keeps the compiler's LDS reads on their side of it.
+    auto next_chunk_ready = [&](bool ahead) {
+        if (ahead)
+            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(NR) : "memory");
+        else
+            asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+    };
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
+    copy_b(0);
+    if (nkc > 1) copy_b(1);
+    lds_dma_wait();
+    __syncthreads();
+    // dword w of the A operands of chunk kk (4 levels)
+    auto gather = [&](int kk, int w, v4i (&A)[NS]) {
+        const uint2 rp = *reinterpret_cast<const uint2 *>(Rsh + kk * 32 + lg * 8 + 2 * w);
+        const uint2 t0 = Tsh[phase_index(cpk, rp.x, lmul, negL)];
+        const uint2 t1 = Tsh[phase_index(cpk, rp.y, lmul, negL)];
+        unsigned o0, o1, o2, o3;
+        transpose4x4_bytes(t0.x, t0.y, t1.x, t1.y, o0, o1, o2, o3);
+        A[0][w] = (int)o0; A[1][w] = (int)o1; A[2][w] = (int)o2; A[3][w] = (int)o3;
+    };
+    // all products of chunk B with A; dword j of chunk kn's operands is
+    // gathered after the products of B digit j
+    auto chunk = [&](const v4i *B, const v4i (&A)[NS], int kn, v4i (&nA)[NS]) {
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            v4i Bre[CB], Bim[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                Bre[c] = B[((c * 2 + 0) * NS + j) * 64 + lane];
+                Bim[c] = B[((c * 2 + 1) * NS + j) * 64 + lane];
+            }
+#pragma unroll
+            for (int i = 0; i < NS - j; ++i)
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(A[i], Bre[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(A[i], Bim[c], acc[c][i + j][1]);
+                }
+            gather(kn, j, nA);
+        }
+    };
+    v4i A0[NS], A1[NS];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) gather(0, w, A0);
+    // chunk kc: the copy of chunk kc + 2 goes out first (its buffer was last
+    // read in chunk kc - 1, before the preceding barrier); past the last chunk
+    // the gather re-reads it (no branch in the chunk body)
+    for (int kc = 0;; kc += 2) {
+        if (kc + 2 < nkc) copy_b(kc + 2);
+        chunk(bufp(kc), A0, min(kc + 1, nkc - 1), A1);
+        if (kc + 1 >= nkc) break;
+        next_chunk_ready(kc + 2 < nkc);
+        if (kc + 3 < nkc) copy_b(kc + 3);
+        chunk(bufp(kc + 1), A1, min(kc + 2, nkc - 1), A0);
+        if (kc + 2 >= nkc) break;
+        next_chunk_ready(kc + 3 < nkc);
+    }
+    __syncthreads();
+}
+
+// Fused mixed-precision pair kernel: the cell-pair contraction of i8_pair_core,
+// FP64 recombination of the digit levels with the column scale of B, and the
+// FP64 rotation / |g|^2 epilogue.  One wave owns 16 k' rows and all CP
+// columns of one state.
+template <int CB, int NBM, int NBRM>
 __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
                                                     const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
                                                     const double *__restrict__ bscale, int nkc) {
     constexpr int NS = kNS;
-    static_assert(NS == 4, "digit packing assumes 4 digits per part");
     constexpr int CP = 16 * CB;
-    constexpr int BCH = CB * 3 * NS * 64;  // v4i per K chunk of one state
-    constexpr int NPF = (BCH + 255) / 256;
+    constexpr int BCH = CB * 2 * NS * 64;  // v4i per K2 chunk of one state
     extern __shared__ v4i smem8[];
-    v4i *buf0 = smem8, *buf1 = smem8 + BCH;
-    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + kPairBufs * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
@@ -405,103 +633,19 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
     const int row_base = blockIdx.x * 64 + wave * 16;
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
-    for (int i = tid; i < nkc * 64; i += 256) Rsh[i] = Rpk[i];
-    int c0 = 0, c1 = 0, c2 = 0;
+    for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
+    unsigned cpk = 0;
     {
         const int r = row_base + li;
         if (r < a.nwv) {
-            c0 = (pos_mod(a.kp_int[3 * r] - k0i, a.Nel0) * a.Lf0) % a.L;
-            c1 = (pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1) * a.Lf1) % a.L;
-            c2 = (pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2) * a.Lf2) % a.L;
+            const unsigned c0 = (pos_mod(a.kp_int[3 * r] - k0i, a.Nel0) * a.Lf0) % a.L;
+            const unsigned c1 = (pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1) * a.Lf1) % a.L;
+            const unsigned c2 = (pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2) * a.Lf2) % a.L;
+            cpk = c0 | (c1 << 8) | (c2 << 16);
         }
     }
-    const float invL = 1.0f / (float)a.L;
-    const int L = a.L;
     v4i acc[CB][NS][2];
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-#pragma unroll
-        for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
-    const v4i *bst = bsl + (size_t)st * nkc * BCH;
-    for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
-    __syncthreads();
-    // digits of the phases of K chunk kk into (Ar, Ai)
-    auto make_a = [&](int kk, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            unsigned tx[4], ty[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const unsigned rp = Rsh[kk * 64 + lg * 16 + 4 * w + b];
-                const int x = __mul24(c0, (int)(rp & 255u)) + __mul24(c1, (int)((rp >> 8) & 255u)) +
-                              __mul24(c2, (int)((rp >> 16) & 255u));
-                int m = x - __mul24(L, (int)((float)x * invL));  // x < 2^18: quotient off by at most 1
-                m = (m >= L) ? m - L : m;
-                m = (m < 0) ? m + L : m;
-                const uint2 t = Tsh[m];
-                tx[b] = t.x;
-                ty[b] = t.y;
-            }
-            unsigned o0, o1, o2, o3;
-            transpose4x4_bytes(tx[0], tx[1], tx[2], tx[3], o0, o1, o2, o3);
-            Ar[0][w] = (int)o0; Ar[1][w] = (int)o1; Ar[2][w] = (int)o2; Ar[3][w] = (int)o3;
-            transpose4x4_bytes(ty[0], ty[1], ty[2], ty[3], o0, o1, o2, o3);
-            Ai[0][w] = (int)o0; Ai[1][w] = (int)o1; Ai[2][w] = (int)o2; Ai[3][w] = (int)o3;
-        }
-    };
-    v4i Ar[NS], Ai[NS];
-    make_a(0, Ar, Ai);
-    for (int kc = 0; kc < nkc; ++kc) {
-        // prefetch the next B chunk (the last chunk re-reads itself: no branch)
-        const int kn = min(kc + 1, nkc - 1);
-        const v4i *src = bst + (size_t)kn * BCH;
-        v4i pf[NPF];
-#pragma unroll
-        for (int j = 0; j < NPF; ++j) {
-            const int i = tid + 256 * j;
-            if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
-        }
-        const v4i *B = (kc & 1) ? buf1 : buf0;
-        // per B digit j: first the Re(A) products of every (level, column
-        // block), then the Im(A) products, so two updates of one accumulator
-        // are at least 2*CB matrix instructions apart
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            v4i Bre[CB], Bim[CB], Bnim[CB];
-#pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                Bre[c] = B[((c * 3 + 0) * NS + j) * 64 + lane];
-                Bim[c] = B[((c * 3 + 1) * NS + j) * 64 + lane];
-                Bnim[c] = B[((c * 3 + 2) * NS + j) * 64 + lane];
-            }
-#pragma unroll
-            for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
-                }
-#pragma unroll
-            for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
-                }
-        }
-        // next chunk's phase digits (independent of the MFMAs above)
-        v4i nAr[NS], nAi[NS];
-        make_a(kn, nAr, nAi);
-#pragma unroll
-        for (int i = 0; i < NS; ++i) { Ar[i] = nAr[i]; Ai[i] = nAi[i]; }
-        v4i *dst = (kc & 1) ? buf0 : buf1;
-#pragma unroll
-        for (int j = 0; j < NPF; ++j) {
-            const int i = tid + 256 * j;
-            if (BCH % 256 == 0 || i < BCH) dst[i] = pf[j];
-        }
-        __syncthreads();
-    }
+    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
     constexpr int YS = CP + 1;
     double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * YS;
@@ -526,7 +670,7 @@ __global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__r
         }
     }
     __syncthreads();
-    ibzk_epilogue_row(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
+    ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
 }
 
 // ---------------------------------------------------------------------------
@@ -559,17 +703,14 @@ __global__ __launch_bounds__(256) void k_ibzk_y_fp64(PairArgs a, int st0, double
     const int stl = blockIdx.y, st = st0 + stl, col0 = blockIdx.z * CPt;
     const int row_base = blockIdx.x * 64 * RB + wave * 16 * RB;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
+    for (int i = tid; i < a.Kp / 2; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
     int coef[RB][3];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) ibzk_coef(a, st, row_base + rb * 16 + li, coef[rb][0], coef[rb][1], coef[rb][2]);
     __syncthreads();
     d4 cr[RB][CB], ci[RB][CB];
     const int L = a.L;
-    auto aop = [&](int rb, int k) {
-        const int4 R = Rsh[k];
-        return Tsh[pos_mod(coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z, L)];
-    };
+    auto aop = [&](int rb, int k) { return pair_phase(Tsh, Rsh, coef[rb], L, k); };
     gemm_core_f64<CB, RB, KC>(a.h + (size_t)st * a.Kp * a.CP + col0, a.CP, a.Kp, aop, buf0, buf1, cr, ci);
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
@@ -583,111 +724,32 @@ __global__ __launch_bounds__(256) void k_ibzk_y_fp64(PairArgs a, int st0, double
             }
 }
 
-// Mixed: block = 4 waves x 16 rows; A digits generated from the digit table
-// of the roots of unity as in k_pair_i8, B digits of the column tile from bsl
-// ([st][kc][CBT col blocks][Re, Im, -Im][digit][lane]).
+// Mixed: block = 4 waves x 16 rows, the cell-pair contraction of i8_pair_core
+// over the block's CB column blocks of B digits in bsl
+// ([st][kc][CBT col blocks][Re, Im][digit][lane]).
 template <int CB>
 __global__ __launch_bounds__(256, 2) void k_ibzk_y_i8(PairArgs a, int st0, const uint2 *__restrict__ Tdig,
                                                       const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
                                                       const double *__restrict__ bscale, int nkc, int CBT,
                                                       double2 *__restrict__ Y) {
     constexpr int NS = kNS;
-    constexpr int BCH = CB * 3 * NS * 64;
-    constexpr int NPF = (BCH + 255) / 256;
+    constexpr int BCH = CB * 2 * NS * 64;
     extern __shared__ v4i smem8[];
-    v4i *buf0 = smem8, *buf1 = smem8 + BCH;
-    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + kPairBufs * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int stl = blockIdx.y, st = st0 + stl, cb0 = blockIdx.z * CB;
     const int row_base = blockIdx.x * 64 + wave * 16;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
-    for (int i = tid; i < nkc * 64; i += 256) Rsh[i] = Rpk[i];
+    for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
     int c0, c1, c2;
     ibzk_coef(a, st, row_base + li, c0, c1, c2);
-    const float invL = 1.0f / (float)a.L;
-    const int L = a.L;
+    const unsigned cpk = (unsigned)c0 | ((unsigned)c1 << 8) | ((unsigned)c2 << 16);
     v4i acc[CB][NS][2];
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-#pragma unroll
-        for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
-    const size_t bkc = (size_t)CBT * 3 * NS * 64;
-    const v4i *bst = bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 3 * NS * 64;
-    for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
-    __syncthreads();
-    auto make_a = [&](int kk, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            unsigned tx[4], ty[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const unsigned rp = Rsh[kk * 64 + lg * 16 + 4 * w + b];
-                const int x = __mul24(c0, (int)(rp & 255u)) + __mul24(c1, (int)((rp >> 8) & 255u)) +
-                              __mul24(c2, (int)((rp >> 16) & 255u));
-                int m = x - __mul24(L, (int)((float)x * invL));
-                m = (m >= L) ? m - L : m;
-                m = (m < 0) ? m + L : m;
-                const uint2 t = Tsh[m];
-                tx[b] = t.x;
-                ty[b] = t.y;
-            }
-            unsigned o0, o1, o2, o3;
-            transpose4x4_bytes(tx[0], tx[1], tx[2], tx[3], o0, o1, o2, o3);
-            Ar[0][w] = (int)o0; Ar[1][w] = (int)o1; Ar[2][w] = (int)o2; Ar[3][w] = (int)o3;
-            transpose4x4_bytes(ty[0], ty[1], ty[2], ty[3], o0, o1, o2, o3);
-            Ai[0][w] = (int)o0; Ai[1][w] = (int)o1; Ai[2][w] = (int)o2; Ai[3][w] = (int)o3;
-        }
-    };
-    v4i Ar[NS], Ai[NS];
-    make_a(0, Ar, Ai);
-    for (int kc = 0; kc < nkc; ++kc) {
-        const int kn = min(kc + 1, nkc - 1);
-        const v4i *src = bst + (size_t)kn * bkc;
-        v4i pf[NPF];
-#pragma unroll
-        for (int j = 0; j < NPF; ++j) {
-            const int i = tid + 256 * j;
-            if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
-        }
-        const v4i *B = (kc & 1) ? buf1 : buf0;
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            v4i Bre[CB], Bim[CB], Bnim[CB];
-#pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                Bre[c] = B[((c * 3 + 0) * NS + j) * 64 + lane];
-                Bim[c] = B[((c * 3 + 1) * NS + j) * 64 + lane];
-                Bnim[c] = B[((c * 3 + 2) * NS + j) * 64 + lane];
-            }
-#pragma unroll
-            for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
-                }
-#pragma unroll
-            for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
-                }
-        }
-        v4i nAr[NS], nAi[NS];
-        make_a(kn, nAr, nAi);
-#pragma unroll
-        for (int i = 0; i < NS; ++i) { Ar[i] = nAr[i]; Ai[i] = nAi[i]; }
-        v4i *dst = (kc & 1) ? buf0 : buf1;
-#pragma unroll
-        for (int j = 0; j < NPF; ++j) {
-            const int i = tid + 256 * j;
-            if (BCH % 256 == 0 || i < BCH) dst[i] = pf[j];
-        }
-        __syncthreads();
-    }
+    const size_t bkc = (size_t)CBT * 2 * NS * 64;
+    i8_pair_core<CB>(bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 2 * NS * 64, bkc, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L,
+                     smem8, acc);
     double wl[NS];
     wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
@@ -1660,6 +1722,25 @@ int elph::ret_error(elph_ctx *ctx, const Error &e) {
 
 int64_t elph::lcm64(int64_t a, int64_t b) { return a / std::gcd(a, b) * b; }
 
+std::vector<int2> elph::cell_pairs(const int64_t *cells, int64_t ld, int64_t n) {
+    std::map<std::array<int64_t, 3>, int64_t> at;
+    for (int64_t i = n - 1; i >= 0; --i) at[{cells[i], cells[i + ld], cells[i + 2 * ld]}] = i;
+    std::vector<char> used(n, 0);
+    std::vector<int2> out;
+    for (int64_t i = 0; i < n; ++i) {
+        if (used[i]) continue;
+        used[i] = 1;
+        int j = -1;
+        const auto it = at.find({-cells[i], -cells[i + ld], -cells[i + 2 * ld]});
+        if (it != at.end() && !used[it->second]) {
+            j = (int)it->second;
+            used[j] = 1;
+        }
+        out.push_back(make_int2((int)i, j));
+    }
+    return out;
+}
+
 namespace {
 
 void upload_cells(elph_ctx *c, const std::vector<int64_t> &cells, int64_t n, DBuf<int> &dst, DBuf<double> *cart) {
@@ -1964,12 +2045,12 @@ void elph::launch_polar_lp(hipStream_t st, int n, int nbr, int div, const int *r
     HCHECK(hipGetLastError());
 }
 
-void elph::launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int nwsg, int Kp, int CP, const double2 *gk,
+void elph::launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int npair, int Kp, int CP, const double2 *gk,
                            int64_t gk_kstride, const int *st_kb, const int *st_m, const double2 *Uk, const int *degq,
-                           double2 *h) {
-    const int64_t tot = (int64_t)nst * Kp * CP;
-    hipLaunchKernelGGL(k_rotate_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nst, nb, nbr, nwsg, Kp, CP,
-                       gk, gk_kstride, st_kb, st_m, Uk, degq, h);
+                           const int2 *pairs, double2 *h) {
+    const int64_t tot = (int64_t)nst * (Kp / 2) * CP;
+    hipLaunchKernelGGL(k_rotate_h, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nst, nb, nbr, npair, Kp, CP,
+                       gk, gk_kstride, st_kb, st_m, Uk, degq, pairs, h);
     HCHECK(hipGetLastError());
 }
 
@@ -1984,6 +2065,14 @@ void elph::launch_slice_b(hipStream_t st, int nst, int nkc, int CB, int Kp, int 
     const int64_t nt = (int64_t)nst * nkc * CB * 64;
     hipLaunchKernelGGL(k_slice_b, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, nst, nkc, CB, Kp, CP, h, scale,
                        bsl);
+    HCHECK(hipGetLastError());
+}
+
+void elph::launch_slice_b_ri(hipStream_t st, int nst, int nkc, int CB, int Kp, int CP, const double2 *h,
+                             const double *scale, v4i *bsl) {
+    const int64_t nt = (int64_t)nst * nkc * CB * 64;
+    hipLaunchKernelGGL(k_slice_b_ri, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, nst, nkc, CB, Kp, CP, h,
+                       scale, bsl);
     HCHECK(hipGetLastError());
 }
 
@@ -2570,7 +2659,9 @@ struct elph_ibzk_plan {
     int64_t nb = 0, nbr = 0, nwv = 0, nirr = 0, nprocs = 0;
     int64_t Nel[3], Nph[3];
     int64_t L = 1, Lf[3];
-    int64_t Kp = 0, CP = 0;
+    // Rp cells in pairs (R_p, -R_p): Kp = 2 npair rounded up to 64 rows (cos, sin)
+    int64_t npair = 0, Kp = 0, CP = 0;
+    DBuf<int2> pairs;
     std::vector<int> kint_irr;     // [nirr][3]
     std::vector<double> ens_irred; // host (nirr,nb)
     double enref = 0, fsthick = 0;
@@ -2598,7 +2689,7 @@ namespace {
 
 template <int CB, int RB>
 void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, double2 *Y) {
-    const size_t lds = (size_t)(2 * 32 * 16 * CB + a.L) * sizeof(double2) + (size_t)a.Kp * sizeof(int4);
+    const size_t lds = (size_t)(2 * 32 * 16 * CB + a.L) * sizeof(double2) + (size_t)(a.Kp / 2) * sizeof(int4);
     dim3 grid((unsigned)((a.nwv + 64 * RB - 1) / (64 * RB)), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
     hipLaunchKernelGGL((k_ibzk_y_fp64<CB, RB>), grid, dim3(256), lds, s, a, st0, Y);
 }
@@ -2606,8 +2697,8 @@ void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, doubl
 template <int CB>
 void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p, const IbzkScratch &sc, int st0,
                       int ns, double2 *Y) {
-    const size_t lds = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) + (size_t)((a.L + 1) & ~1) * sizeof(uint2) +
-                       (size_t)p->nkc * 64 * sizeof(unsigned);
+    const size_t lds = (size_t)kPairBufs * CB * 2 * kNS * 64 * sizeof(v4i) +
+                       (size_t)((a.L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 32 * sizeof(unsigned);
     dim3 grid((unsigned)((a.nwv + 63) / 64), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
     hipLaunchKernelGGL(k_ibzk_y_i8<CB>, grid, dim3(256), lds, s, a, st0, p->Tdig.p, p->Rpk.p, sc.bsl.p,
                        sc.bscale.p, (int)p->nkc, (int)(a.CP / 16), Y);
@@ -2670,6 +2761,7 @@ void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStr
     a.Nph0 = (int)p->Nph[0]; a.Nph1 = (int)p->Nph[1]; a.Nph2 = (int)p->Nph[2];
     a.L = (int)p->L;
     a.Lf0 = (int)p->Lf[0]; a.Lf1 = (int)p->Lf[1]; a.Lf2 = (int)p->Lf[2];
+    a.Lmul = phase_lmul(p->L);
     a.T = p->T.p;
     a.Rq = p->Rq.p;
     a.Vkp = p->Vkp.p;
@@ -2707,13 +2799,21 @@ void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStr
     }
 }
 
+// Epilogue variant of the fused kernels: 1 for nb <= 8 and nbr <= 6, 2 for
+// nb <= 16 and nbr <= 12 (loops unrolled to those bounds), 0 otherwise.
+int epilogue_class(int64_t nb, int64_t nbr) {
+    if (nb <= 8 && nbr <= 6) return 1;
+    if (nb <= 16 && nbr <= 12) return 2;
+    return 0;
+}
+
 void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkScratch &sc, hipStream_t s, int CB) {
-    const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
+    const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
                                 (size_t)64 * (p->CP + 1) * sizeof(double2));
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
     if (p->precision == ELPH_MIXED) {
-        const size_t lds8 = std::max((size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i) +
-                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 64 * sizeof(unsigned),
+        const size_t lds8 = std::max((size_t)kPairBufs * CB * 2 * kNS * 64 * sizeof(v4i) +
+                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 32 * sizeof(unsigned),
                                      (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;
@@ -2721,21 +2821,40 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
         grid.x = (unsigned)((p->nwv + 63) / 64);
         const double *bsc = sc.bscale.p;
         const int nkc = (int)p->nkc;
-        switch (CB) {
-            case 1: hipLaunchKernelGGL(k_pair_i8<1>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
-            case 2: hipLaunchKernelGGL(k_pair_i8<2>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
-            case 3: hipLaunchKernelGGL(k_pair_i8<3>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
-            case 4: hipLaunchKernelGGL(k_pair_i8<4>, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); break;
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); };
+        // (nb <= 8 and nbr <= 6 implies CB <= 3)
+        switch (CB * 4 + epilogue_class(p->nb, p->nbr)) {
+            case 4 + 1: go(k_pair_i8<1, 8, 6>); break;
+            case 8 + 1: go(k_pair_i8<2, 8, 6>); break;
+            case 12 + 1: go(k_pair_i8<3, 8, 6>); break;
+            case 4 + 2: go(k_pair_i8<1, 16, 12>); break;
+            case 8 + 2: go(k_pair_i8<2, 16, 12>); break;
+            case 12 + 2: go(k_pair_i8<3, 16, 12>); break;
+            case 16 + 2: go(k_pair_i8<4, 16, 12>); break;
+            case 4: go(k_pair_i8<1, 0, 0>); break;
+            case 8: go(k_pair_i8<2, 0, 0>); break;
+            case 12: go(k_pair_i8<3, 0, 0>); break;
+            case 16: go(k_pair_i8<4, 0, 0>); break;
             default: throw Error{ELPH_EINVAL, "mixed precision fused kernel holds numwannbands*numbranches <= 64"};
         }
     } else {
-        switch (CB) {
-            case 1: hipLaunchKernelGGL(k_pair_fp64<1>, grid, dim3(256), lds, s, a); break;
-            case 2: hipLaunchKernelGGL(k_pair_fp64<2>, grid, dim3(256), lds, s, a); break;
-            case 3: hipLaunchKernelGGL(k_pair_fp64<3>, grid, dim3(256), lds, s, a); break;
-            case 4: hipLaunchKernelGGL(k_pair_fp64<4>, grid, dim3(256), lds, s, a); break;
-            case 5: hipLaunchKernelGGL(k_pair_fp64<5>, grid, dim3(256), lds, s, a); break;
-            case 6: hipLaunchKernelGGL(k_pair_fp64<6>, grid, dim3(256), lds, s, a); break;
+        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a); };
+        // the unrolled epilogue of nb <= 16, nbr <= 12 only where it keeps two
+        // waves per SIMD (CB <= 2)
+        int e = epilogue_class(p->nb, p->nbr);
+        if (e == 2 && CB > 2) e = 0;
+        switch (CB * 4 + e) {
+            case 4 + 1: go(k_pair_fp64<1, 8, 6>); break;
+            case 8 + 1: go(k_pair_fp64<2, 8, 6>); break;
+            case 12 + 1: go(k_pair_fp64<3, 8, 6>); break;
+            case 4 + 2: go(k_pair_fp64<1, 16, 12>); break;
+            case 8 + 2: go(k_pair_fp64<2, 16, 12>); break;
+            case 4: go(k_pair_fp64<1, 0, 0>); break;
+            case 8: go(k_pair_fp64<2, 0, 0>); break;
+            case 12: go(k_pair_fp64<3, 0, 0>); break;
+            case 16: go(k_pair_fp64<4, 0, 0>); break;
+            case 20: go(k_pair_fp64<5, 0, 0>); break;
+            case 24: go(k_pair_fp64<6, 0, 0>); break;
             default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
         }
     }
@@ -2815,7 +2934,7 @@ void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t
     if (p->precision == ELPH_MIXED) {
         sc.bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
         sc.bscale.alloc(nst * p->CP);
-        sc.bsl.alloc(nst * p->nkc * (p->CP / 16) * 3 * kNS * 64);
+        sc.bsl.alloc(nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
     }
     if (p->general) sc.Y.alloc(general_chunk(p, nst) * p->nwv * p->CP);
 }
@@ -2825,12 +2944,12 @@ void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStr
     elph_ctx *c = p->ctx;
     const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb, nslices = nbr * c->nwsg;
     launch_gkRp(c, s, g.nkb, g.kv.p, sc.gk.p, nb2 * nslices, p->precision == ELPH_MIXED, sc.phiT.p, sc.bphi.p);
-    launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)c->nwsg, (int)p->Kp, (int)p->CP, sc.gk.p, nb2 * nslices,
-                    g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, sc.h.p);
+    launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)p->npair, (int)p->Kp, (int)p->CP, sc.gk.p, nb2 * nslices,
+                    g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.h.p);
     if (p->precision == ELPH_MIXED) {
         launch_colscale(s, (int)g.nst, (int)p->Kp, (int)p->CP, sc.h.p, sc.bscale.p);
-        launch_slice_b(s, (int)g.nst, (int)p->nkc, (int)(p->CP / 16), (int)p->Kp, (int)p->CP, sc.h.p, sc.bscale.p,
-                       sc.bsl.p);
+        launch_slice_b_ri(s, (int)g.nst, (int)p->nkc, (int)(p->CP / 16), (int)p->Kp, (int)p->CP, sc.h.p,
+                          sc.bscale.p, sc.bsl.p);
     }
     pair_launch(p, g, sc, s, out, timed);
 }
@@ -2869,7 +2988,9 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
         p->CP = ((nb * nbr + 15) / 16) * 16;
         p->CPf = p->CP;
-        p->Kp = ((ctx->nwsg + 63) / 64) * 64;
+        const std::vector<int2> pairs = cell_pairs(ctx->rcells_q.data(), ctx->nwsq, ctx->nwsg);
+        p->npair = (int64_t)pairs.size();
+        p->Kp = ((2 * p->npair + 63) / 64) * 64;
         // fused kernels hold all nb*nbr columns of a state in registers up to
         // 96 (FP64) / 64 (mixed); beyond, the general column-tiled path
         p->general = p->CP > (precision == ELPH_MIXED ? 64 : 96);
@@ -2971,10 +3092,12 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             T[j] = make_double2(std::cos(x), std::sin(x));
         }
         p->T.upload(T, s);
-        std::vector<int> R(4 * p->Kp, 0);
-        for (int64_t i = 0; i < ctx->nwsg; ++i)
-            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_q[i + ctx->nwsq * d];
+        // first cell of each pair (rcells_q(1:nwsg) in pairs, padded with 0)
+        std::vector<int> R(4 * (p->Kp / 2), 0);
+        for (int64_t i = 0; i < p->npair; ++i)
+            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_q[pairs[i].x + ctx->nwsq * d];
         p->Rq.upload(R, s);
+        p->pairs.upload(pairs, s);
         if (precision == ELPH_MIXED) {
             p->nkc = p->Kp / 64;
             std::vector<uint2> tdig(p->L);
@@ -2991,11 +3114,11 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             }
             p->Tdig.upload(tdig, s);
             // cells reduced modulo the electron mesh, packed one byte per component
-            std::vector<unsigned> rpk(p->Kp, 0u);
-            for (int64_t i = 0; i < ctx->nwsg; ++i) {
+            std::vector<unsigned> rpk(p->Kp / 2, 0u);
+            for (int64_t i = 0; i < p->npair; ++i) {
                 unsigned v = 0;
                 for (int d = 0; d < 3; ++d)
-                    v |= (unsigned)fmodulo(ctx->rcells_q[i + ctx->nwsq * d], p->Nel[d]) << (8 * d);
+                    v |= (unsigned)fmodulo(ctx->rcells_q[pairs[i].x + ctx->nwsq * d], p->Nel[d]) << (8 * d);
                 rpk[i] = v;
             }
             for (int d = 0; d < 3; ++d)

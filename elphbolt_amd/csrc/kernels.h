@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace elph {
 
@@ -91,13 +92,37 @@ __device__ __forceinline__ void transpose4x4_bytes(unsigned a0, unsigned a1, uns
     o3 = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
 }
 
-// Complex GEMM core on the FP64 matrix cores, one 256-thread block (4 waves):
-//   acc[rb][c] (16 rows x 16 cols) += sum_K A(rb, K) * B[K][c]
+// Phase-table index (c . R) mod L of byte-packed coefficients c (c_d < L <= 255)
+// and cell R (R_d < 256): one 4-way byte dot product, x <= 3 * 255 * 255 < 2^18,
+// then the quotient floor(x / L) as the high word of x * ceil(2^32 / L), which
+// is exact for x < 2^32 / (L - 1) > 2^24.
+__host__ __device__ inline unsigned phase_lmul(int64_t L) {
+    return L <= 1 ? 0u : (unsigned)(((1ull << 32) + (uint64_t)L - 1) / (uint64_t)L);
+}
+__device__ __forceinline__ int phase_index(unsigned cpk, unsigned rp, unsigned lmul, int negL) {
+    const unsigned x = __builtin_amdgcn_udot4(cpk, rp, 0u, false);
+    return __mul24((int)__umulhi(x, lmul), negL) + (int)x;
+}
+
+// Asynchronous copy global -> LDS of one 1 KiB row per wave (16 bytes per lane,
+// lane i to dst + 16 i); dst must be the same for all lanes of the wave.  The
+// data is visible to the wave after lds_dma_wait() and to the block after a
+// following barrier.
+__device__ __forceinline__ void lds_dma16(const void *src, void *dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
+}
+__device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// GEMM core on the FP64 matrix cores, one 256-thread block (4 waves):
+//   (cr + i ci)[rb][c] (16 rows x 16 cols) += sum_K A(rb, K) * B[K][c]
 // for the wave's RB row blocks (A supplied by aop(rb, K), K the global row
 // index of B, evaluated by lane (li, lk) for its row li of block rb and its K
 // slot) and the CB column blocks of the tile.  B (Kp rows of 16*CB complex,
 // row stride ldb) is streamed through LDS in KC-row chunks (buf0/buf1, KC*16*CB
 // complex each) with a register prefetch of the next chunk.  Kp % KC == 0.
+// aop returns double2 (complex A: 4 real products per K) or double (real A:
+// 2 real products per K).
 template <int CB, int RB, int KC, typename AOp>
 __device__ __forceinline__ void gemm_core_f64(const double2 *__restrict__ B, int ldb, int Kp, AOp aop,
                                               double2 *buf0, double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
@@ -131,19 +156,34 @@ __device__ __forceinline__ void gemm_core_f64(const double2 *__restrict__ B, int
 #pragma unroll 2
         for (int ks = 0; ks < KC / 4; ++ks) {
             const int kl = ks * 4 + lk;
-            double ar[RB], ai[RB], nai[RB];
+            if constexpr (std::is_same<decltype(aop(0, 0)), double>::value) {
+                double av[RB];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                const double2 t = aop(rb, ch * KC + kl);
-                ar[rb] = t.x;
-                ai[rb] = t.y;
-                nai[rb] = -t.y;
-            }
+                for (int rb = 0; rb < RB; ++rb) av[rb] = aop(rb, ch * KC + kl);
 #pragma unroll
-            for (int c = 0; c < CB; ++c) {
-                const double2 b = Bs[kl * CP + c * 16 + li];
+                for (int c = 0; c < CB; ++c) {
+                    const double2 b = Bs[kl * CP + c * 16 + li];
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], b.x, b.y, cr[rb][c], ci[rb][c]);
+                    for (int rb = 0; rb < RB; ++rb) {
+                        cr[rb][c] = mfma_f64(av[rb], b.x, cr[rb][c]);
+                        ci[rb][c] = mfma_f64(av[rb], b.y, ci[rb][c]);
+                    }
+                }
+            } else {
+                double ar[RB], ai[RB], nai[RB];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    const double2 t = aop(rb, ch * KC + kl);
+                    ar[rb] = t.x;
+                    ai[rb] = t.y;
+                    nai[rb] = -t.y;
+                }
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    const double2 b = Bs[kl * CP + c * 16 + li];
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], b.x, b.y, cr[rb][c], ci[rb][c]);
+                }
             }
         }
         if (more) {
