@@ -519,13 +519,15 @@ __global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const dou
 // c_d = q_d L / N_d (cpk, byte-packed; phase_index).  Lane (li, lg) holds
 // K2 = 64 kk + 16 lg + 4 w + b of chunk kk: pair 32 kk + 8 lg + 2 w + b / 2.
 // B: chunk kc of the block's CB column blocks at bst + kc * bkc
-// ([c][Re, Im][digit][lane] v4i), copied global -> LDS asynchronously two
-// chunks ahead into kPairBufs rotating buffers of BCH v4i (buf); the next
-// chunk's A digits are gathered between the matrix instructions of the current
-// one, into the second of two register sets (the K loop is unrolled by two).
-// The caller fills Tsh and Rsh before the call (the first barrier here
-// publishes them) and owns buf again after it.
-constexpr int kPairBufs = 3;
+// ([c][Re, Im][digit][lane] v4i), copied global -> LDS asynchronously one
+// chunk ahead into two buffers of BCH v4i (buf); the next chunk's A digits
+// are gathered between the matrix instructions of the current one, into the
+// second of two register sets (the K loop is unrolled by two).  The caller
+// fills Tsh and Rsh before the call (the first barrier here publishes them)
+// and owns buf again after it.  Two buffers keep the LDS of a block small
+// enough for three blocks per CU (CB <= 3), which hides the copy latency
+// better than a deeper prefetch with two blocks.
+__host__ __device__ constexpr int pair_blocks_per_cu(int CB) { return CB <= 3 ? 3 : 2; }
 template <int CB>
 __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t bkc, int nkc, const uint2 *Tsh,
                                              const unsigned *Rsh, unsigned cpk, unsigned lmul, int negL, v4i *buf,
@@ -536,7 +538,7 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
     constexpr int NR = BCH / 256;  // copy instructions per wave and chunk
     static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = lane >> 4;
-    auto bufp = [&](int kc) { return buf + (kc % kPairBufs) * BCH; };
+    auto bufp = [&](int kc) { return buf + (kc & 1) * BCH; };
     auto copy_b = [&](int kn) {
         const v4i *src = bst + (size_t)kn * bkc;
         v4i *dst = bufp(kn);
@@ -546,22 +548,15 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
             lds_dma16(src + r * 64 + lane, dst + r * 64);
         }
     };
-    // chunk kc + 1 complete in LDS for the whole block; with `ahead` the copy
-    // of chunk kc + 2 (the wave's last NR copies) stays in flight.  No LDS
-    // stores are pending here, so the barrier needs no fence; the memory
-    // clobber keeps the compiler's LDS reads on their side of it.
-    auto next_chunk_ready = [&](bool ahead) {
-        if (ahead)
-            asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"(NR) : "memory");
-        else
-            asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
-    };
+    // chunk kc + 1 complete in LDS for the whole block.  No LDS stores are
+    // pending here, so the barrier needs no fence; the memory clobber keeps the
+    // compiler's LDS reads on their side of it.
+    auto next_chunk_ready = [&]() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); };
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
         for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
     copy_b(0);
-    if (nkc > 1) copy_b(1);
     lds_dma_wait();
     __syncthreads();
     // dword w of the A operands of chunk kk (4 levels)
@@ -597,18 +592,18 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
     v4i A0[NS], A1[NS];
 #pragma unroll
     for (int w = 0; w < 4; ++w) gather(0, w, A0);
-    // chunk kc: the copy of chunk kc + 2 goes out first (its buffer was last
+    // chunk kc: the copy of chunk kc + 1 goes out first (its buffer was last
     // read in chunk kc - 1, before the preceding barrier); past the last chunk
     // the gather re-reads it (no branch in the chunk body)
     for (int kc = 0;; kc += 2) {
-        if (kc + 2 < nkc) copy_b(kc + 2);
+        if (kc + 1 < nkc) copy_b(kc + 1);
         chunk(bufp(kc), A0, min(kc + 1, nkc - 1), A1);
         if (kc + 1 >= nkc) break;
-        next_chunk_ready(kc + 2 < nkc);
-        if (kc + 3 < nkc) copy_b(kc + 3);
+        next_chunk_ready();
+        if (kc + 2 < nkc) copy_b(kc + 2);
         chunk(bufp(kc + 1), A1, min(kc + 2, nkc - 1), A0);
         if (kc + 2 >= nkc) break;
-        next_chunk_ready(kc + 3 < nkc);
+        next_chunk_ready();
     }
     __syncthreads();
 }
@@ -618,14 +613,14 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
 // FP64 rotation / |g|^2 epilogue.  One wave owns 16 k' rows and all CP
 // columns of one state.
 template <int CB, int NBM, int NBRM>
-__global__ __launch_bounds__(256, 2) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
+__global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
                                                     const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
                                                     const double *__restrict__ bscale, int nkc) {
     constexpr int NS = kNS;
     constexpr int CP = 16 * CB;
     constexpr int BCH = CB * 2 * NS * 64;  // v4i per K2 chunk of one state
     extern __shared__ v4i smem8[];
-    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + kPairBufs * BCH);
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
@@ -728,14 +723,14 @@ __global__ __launch_bounds__(256) void k_ibzk_y_fp64(PairArgs a, int st0, double
 // over the block's CB column blocks of B digits in bsl
 // ([st][kc][CBT col blocks][Re, Im][digit][lane]).
 template <int CB>
-__global__ __launch_bounds__(256, 2) void k_ibzk_y_i8(PairArgs a, int st0, const uint2 *__restrict__ Tdig,
+__global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_ibzk_y_i8(PairArgs a, int st0, const uint2 *__restrict__ Tdig,
                                                       const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
                                                       const double *__restrict__ bscale, int nkc, int CBT,
                                                       double2 *__restrict__ Y) {
     constexpr int NS = kNS;
     constexpr int BCH = CB * 2 * NS * 64;
     extern __shared__ v4i smem8[];
-    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + kPairBufs * BCH);
+    uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
@@ -2697,7 +2692,7 @@ void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, doubl
 template <int CB>
 void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p, const IbzkScratch &sc, int st0,
                       int ns, double2 *Y) {
-    const size_t lds = (size_t)kPairBufs * CB * 2 * kNS * 64 * sizeof(v4i) +
+    const size_t lds = (size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) +
                        (size_t)((a.L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 32 * sizeof(unsigned);
     dim3 grid((unsigned)((a.nwv + 63) / 64), (unsigned)ns, (unsigned)(a.CP / (16 * CB)));
     hipLaunchKernelGGL(k_ibzk_y_i8<CB>, grid, dim3(256), lds, s, a, st0, p->Tdig.p, p->Rpk.p, sc.bsl.p,
@@ -2812,7 +2807,7 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
                                 (size_t)64 * (p->CP + 1) * sizeof(double2));
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
     if (p->precision == ELPH_MIXED) {
-        const size_t lds8 = std::max((size_t)kPairBufs * CB * 2 * kNS * 64 * sizeof(v4i) +
+        const size_t lds8 = std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) +
                                          (size_t)((p->L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 32 * sizeof(unsigned),
                                      (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
