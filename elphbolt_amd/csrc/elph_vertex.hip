@@ -418,17 +418,27 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 // rotations and |g|^2 stay FP64.
 // ---------------------------------------------------------------------------
 // Column scale of B = h[st] (max over K of max(|Re|, |Im|); 1 for zero columns).
-__global__ void k_colscale(int nst, int Kp, int CP, const double2 *__restrict__ h, double *__restrict__ scale) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nst * CP) return;
-    const int st = t / CP, c = t % CP;
-    const double2 *p = h + (size_t)st * Kp * CP + c;
+// Block (state, 64-column tile): 4 row groups of 64 lanes read rows k = g,
+// g + 4, ... (one coalesced row segment per wave), reduced in LDS.
+__global__ __launch_bounds__(256) void k_colscale(int nst, int Kp, int CP, const double2 *__restrict__ h,
+                                                  double *__restrict__ scale) {
+    __shared__ double red[4][64];
+    const int st = blockIdx.x, lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.y * 64 + lane;
     double m = 0.0;
-    for (int k = 0; k < Kp; ++k) {
-        const double2 v = p[(size_t)k * CP];
-        m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    if (c < CP) {
+        const double2 *p = h + (size_t)st * Kp * CP + c;
+        for (int k = g; k < Kp; k += 4) {
+            const double2 v = p[(size_t)k * CP];
+            m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+        }
     }
-    scale[t] = m > 0.0 ? m : 1.0;
+    red[g][lane] = m;
+    __syncthreads();
+    if (g == 0 && c < CP) {
+        m = fmax(fmax(red[0][lane], red[1][lane]), fmax(red[2][lane], red[3][lane]));
+        scale[(size_t)st * CP + c] = m > 0.0 ? m : 1.0;
+    }
 }
 
 // Bsl[st][kc][cb][p][s][lane] (16 bytes): digits of B / scale in the int8 MFMA
@@ -526,12 +536,16 @@ __global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const dou
 // fills Tsh and Rsh before the call (the first barrier here publishes them)
 // and owns buf again after it.  Two buffers keep the LDS of a block small
 // enough for three blocks per CU (CB <= 3), which hides the copy latency
-// better than a deeper prefetch with two blocks.
-__host__ __device__ constexpr int pair_blocks_per_cu(int CB) { return CB <= 3 ? 3 : 2; }
-template <int CB>
+// better than a deeper prefetch with two blocks.  after_setup() runs once
+// after the first barrier (its global -> LDS copies complete within the first
+// chunk).
+// (NBM: the epilogue's unrolled band bound, whose registers fit three blocks
+// per CU up to 8 bands)
+__host__ __device__ constexpr int pair_blocks_per_cu(int CB, int NBM = 0) { return CB <= 3 && NBM <= 8 ? 3 : 2; }
+template <int CB, typename F>
 __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t bkc, int nkc, const uint2 *Tsh,
                                              const unsigned *Rsh, unsigned cpk, unsigned lmul, int negL, v4i *buf,
-                                             v4i (&acc)[CB][kNS][2]) {
+                                             v4i (&acc)[CB][kNS][2], F after_setup) {
     constexpr int NS = kNS;
     static_assert(NS == 4, "digit packing assumes 4 digits per part");
     constexpr int BCH = CB * 2 * NS * 64;
@@ -559,6 +573,7 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
     copy_b(0);
     lds_dma_wait();
     __syncthreads();
+    after_setup();
     // dword w of the A operands of chunk kk (4 levels)
     auto gather = [&](int kk, int w, v4i (&A)[NS]) {
         const uint2 rp = *reinterpret_cast<const uint2 *>(Rsh + kk * 32 + lg * 8 + 2 * w);
@@ -605,6 +620,7 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
         if (kc + 2 >= nkc) break;
         next_chunk_ready();
     }
+    lds_dma_wait();
     __syncthreads();
 }
 
@@ -613,7 +629,7 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
 // FP64 rotation / |g|^2 epilogue.  One wave owns 16 k' rows and all CP
 // columns of one state.
 template <int CB, int NBM, int NBRM>
-__global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
+__global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
                                                     const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
                                                     const double *__restrict__ bscale, int nkc) {
     constexpr int NS = kNS;
@@ -622,6 +638,7 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_pair_i8(PairArg
     extern __shared__ v4i smem8[];
     uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
+    unsigned *sink = Rsh + nkc * 32;  // 256 bytes per wave, never read
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int st = blockIdx.y;
@@ -630,17 +647,39 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_pair_i8(PairArg
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
     for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
     unsigned cpk = 0;
-    {
-        const int r = row_base + li;
-        if (r < a.nwv) {
-            const unsigned c0 = (pos_mod(a.kp_int[3 * r] - k0i, a.Nel0) * a.Lf0) % a.L;
-            const unsigned c1 = (pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1) * a.Lf1) % a.L;
-            const unsigned c2 = (pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2) * a.Lf2) % a.L;
-            cpk = c0 | (c1 << 8) | (c2 << 16);
-        }
+    int q0 = 0, q1 = 0, q2 = 0;
+    const int r = row_base + li;
+    if (r < a.nwv) {
+        q0 = pos_mod(a.kp_int[3 * r] - k0i, a.Nel0);
+        q1 = pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1);
+        q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
+        cpk = (unsigned)((q0 * a.Lf0) % a.L) | ((unsigned)((q1 * a.Lf1) % a.L) << 8) |
+              ((unsigned)((q2 * a.Lf2) % a.L) << 16);
     }
+    // the epilogue's per-row data (rows of U_k', u_q) pulled into the caches
+    // while the contraction runs: 4-byte global -> LDS copies into a sink
+    auto warm = [&]() {
+        const int nrow = min(16, a.nwv - row_base);
+        if (nrow <= 0) return;
+        const char *v0 = reinterpret_cast<const char *>(a.Vkp + (size_t)row_base * a.nb * a.nb);
+        const int vlines = (nrow * a.nb * a.nb * 16 + 127) / 128;
+        for (int l = lane; l < vlines; l += 64) lds_dma4(v0 + (size_t)l * 128, sink + wave * 64);
+        if (li < nrow) {
+            const bool coarse = q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0;
+            const char *u0;
+            if (coarse) {
+                const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+                u0 = reinterpret_cast<const char *>(a.u + (size_t)iq * a.nbr * a.nbr);
+            } else {
+                const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+                u0 = reinterpret_cast<const char *>(a.uf + (size_t)iq * a.nbr * a.nbr);
+            }
+            const int ubytes = a.nbr * a.nbr * 16;
+            for (int l = lg; l * 128 < ubytes; l += 4) lds_dma4(u0 + l * 128, sink + wave * 64);
+        }
+    };
     v4i acc[CB][NS][2];
-    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
+    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc, warm);
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
     constexpr int YS = CP + 1;
     double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * YS;
@@ -744,7 +783,7 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_ibzk_y_i8(PairA
     v4i acc[CB][NS][2];
     const size_t bkc = (size_t)CBT * 2 * NS * 64;
     i8_pair_core<CB>(bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 2 * NS * 64, bkc, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L,
-                     smem8, acc);
+                     smem8, acc, [] {});
     double wl[NS];
     wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
@@ -2050,7 +2089,8 @@ void elph::launch_rotate_h(hipStream_t st, int nst, int nb, int nbr, int npair, 
 }
 
 void elph::launch_colscale(hipStream_t st, int nst, int Kp, int CP, const double2 *h, double *scale) {
-    hipLaunchKernelGGL(k_colscale, dim3((unsigned)(((int64_t)nst * CP + 255) / 256)), dim3(256), 0, st, nst, Kp, CP,
+    if (nst <= 0) return;
+    hipLaunchKernelGGL(k_colscale, dim3((unsigned)nst, (unsigned)((CP + 63) / 64)), dim3(256), 0, st, nst, Kp, CP,
                        h, scale);
     HCHECK(hipGetLastError());
 }
@@ -2808,7 +2848,8 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
     dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
     if (p->precision == ELPH_MIXED) {
         const size_t lds8 = std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) +
-                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) + (size_t)p->nkc * 32 * sizeof(unsigned),
+                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
+                                         (size_t)p->nkc * 32 * sizeof(unsigned) + 1024,
                                      (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;

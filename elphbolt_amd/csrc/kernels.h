@@ -112,6 +112,11 @@ __device__ __forceinline__ void lds_dma16(const void *src, void *dst) {
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
                                      (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
 }
+// Same with 4 bytes per lane (lane i to dst + 4 i).
+__device__ __forceinline__ void lds_dma4(const void *src, void *dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                     (__attribute__((address_space(3))) void *)dst, 4, 0, 0);
+}
 __device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // GEMM core on the FP64 matrix cores, one 256-thread block (4 waves):
