@@ -84,6 +84,14 @@ struct elph_ctx {
     DBuf<v4i> gdig;
     DBuf<double> gscale;
     int64_t gnrb = 0, gnkc = 0;
+    // Re cells in pairs (R_p, -R_p) for the cell-pair form of the Re
+    // contraction: gpair[p] = (index of R_p, index of -R_p or -1), gpcell[p] =
+    // R_p, ginvdeg[cell] = 1 / elwsdeg; K2 = 2 npairk cos/sin rows, padded to
+    // gk2p (a multiple of 64)
+    DBuf<int2> gpair;
+    DBuf<int> gpcell;
+    DBuf<double> ginvdeg;
+    int64_t npairk = 0, gk2p = 0;
     // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
     DBuf<int> term_off, term_iuc, term_T, term_div;
     // polar crystals: reciprocal vectors of the (2N+1)^3 box (1/bohr), Born
@@ -109,10 +117,12 @@ std::vector<int2> cell_pairs(const int64_t *cells, int64_t ld, int64_t n);
 
 // Launchers (all enqueue on `st`; none synchronises or allocates).
 // gkRp for nk device k vectors ((nk,3) column-major) into out[k] (nb*nb, nbr*nwsg);
-// phiT: nwsk*64 complex scratch; bphi (mixed only): gnkc*4*3*kNS*64 v4i scratch.
+// phiT: gk_phi_count complex scratch; bphi (mixed only): gk_bphi_count v4i scratch.
 void launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride,
                  bool mixed, double2 *phiT, v4i *bphi);
 void ensure_gwann_digits(elph_ctx *c);  // int8 digits of gwann, once per context (synchronous)
+int64_t gk_phi_count(const elph_ctx *c);
+int64_t gk_bphi_count(const elph_ctx *c);
 // h(Rp) = sum_mp U(kb,m,mp) gk[kb](np,mp,sp,Rp) / phwsdeg(Rp), folded over the
 // cell pairs (R_p, -R_p): h[st][2p][CP] = h(R_p) + h(-R_p), h[st][2p+1][CP] =
 // i (h(R_p) - h(-R_p)); pairs[p] = (index of R_p, index of -R_p or -1), Kp even.

@@ -57,6 +57,44 @@ __global__ void k_phase_list(int n, int ncell, const double *__restrict__ vecs, 
     out[(size_t)v * ld_v + (size_t)c * ld_c] = make_double2(co / d, s / d);
 }
 
+// Cell-pair phase table of the Re contraction: out[v * ld_v + K2 * ld_c] is
+// cos (K2 = 2p) or sin (K2 = 2p + 1) of 2 pi vecs(v,:) . R_p, the factor of
+// src/wannier.f90:466-467 without the degeneracy (carried by the folded
+// gwann, gw_pair).  vecs is (n,3) column-major with stride vstride.
+__global__ void k_phase_pairs(int n, int npair, const double *__restrict__ vecs, int64_t vstride,
+                              const int *__restrict__ pcell, double *__restrict__ out, int64_t ld_v, int64_t ld_c) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (p >= npair || v >= n) return;
+    const double dot = vecs[v] * (double)pcell[3 * p] + vecs[v + vstride] * (double)pcell[3 * p + 1] +
+                       vecs[v + 2 * vstride] * (double)pcell[3 * p + 2];
+    double sn, co;
+    sincos(kTwoPi * dot, &sn, &co);
+    out[(size_t)v * ld_v + (size_t)(2 * p) * ld_c] = co;
+    out[(size_t)v * ld_v + (size_t)(2 * p + 1) * ld_c] = sn;
+}
+
+// Row K2 of the folded gwann of one (sp,Rp) slice (gs = slice base, [Re][i]):
+//   G(2p) = g(R_p)/d(R_p) + g(-R_p)/d(-R_p),  G(2p+1) = i (g(R_p)/d(R_p) - g(-R_p)/d(-R_p)),
+// so that sum_Re e^{i theta(Re)} g(Re) / d(Re) = sum_p cos(theta_p) G(2p) + sin(theta_p) G(2p+1)
+// (zero past the pairs; a cell without its opposite has g(-R_p) = 0).  invdeg
+// holds 1/d per cell.
+__device__ __forceinline__ double2 gw_pair(const double2 *__restrict__ gs, int nb2, int i, int k2, int npair,
+                                           const int2 *__restrict__ pairs, const double *__restrict__ invdeg) {
+    const int p = k2 >> 1;
+    if (p >= npair) return make_double2(0.0, 0.0);
+    const int2 pr = pairs[p];
+    const double2 ga = gs[(size_t)pr.x * nb2 + i];
+    const double da = invdeg[pr.x];
+    double2 a = make_double2(ga.x * da, ga.y * da), b = make_double2(0.0, 0.0);
+    if (pr.y >= 0) {
+        const double2 gb = gs[(size_t)pr.y * nb2 + i];
+        const double db = invdeg[pr.y];
+        b = make_double2(gb.x * db, gb.y * db);
+    }
+    return (k2 & 1) ? make_double2(b.y - a.y, a.x - b.x) : make_double2(a.x + b.x, a.y + b.y);
+}
+
 // Zero fill (used instead of hipMemsetAsync so the launches stay graph-safe).
 __global__ void k_zero2(double2 *__restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -64,35 +102,54 @@ __global__ void k_zero2(double2 *__restrict__ p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Re contraction (gkRp) as an FP64 MFMA GEMM per (sp,Rp) slice:
-//   out[k][slice][i] = sum_Re gwann[i][Re][slice] * phiT[Re][k]
+// Re contraction (gkRp) as an FP64 MFMA GEMM per (sp,Rp) slice in cell-pair
+// form (real phase operand, gwann folded on the fly by gw_pair):
+//   out[k][slice][i] = sum_K2 G[i][K2][slice] * phi[K2][k]
 // One workgroup per slice; wave w takes row blocks w, w+4, ...
 // ---------------------------------------------------------------------------
 template <int NCB>
-__global__ __launch_bounds__(256) void k_gkRp(const double2 *__restrict__ gw, int nb2, int nwsk, int nslices,
-                                              const double2 *__restrict__ phiT, int ldphi, int nk,
+__global__ __launch_bounds__(256) void k_gkRp(const double2 *__restrict__ gw, int nb2, int nwsk, int nslices, int npair,
+                                              const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
+                                              const double *__restrict__ phi, int ldphi, int nk,
                                               double2 *__restrict__ out, int64_t out_kstride) {
+    // pair table with the cells' 1/deg in LDS: (index, 1/deg) of R_p and -R_p
+    extern __shared__ double4 ptab[];
     const int slice = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
+    for (int q = threadIdx.x; q < npair; q += 256) {
+        const int2 pr = pairs[q];
+        ptab[q] = make_double4((double)pr.x, invdeg[pr.x], (double)pr.y, pr.y >= 0 ? invdeg[pr.y] : 0.0);
+    }
+    __syncthreads();
     const double2 *A = gw + (size_t)slice * nb2 * nwsk;
-    const int nrb = (nb2 + 15) >> 4;
+    const int nrb = (nb2 + 15) >> 4, K2 = 2 * npair;
     for (int rb = wave; rb < nrb; rb += 4) {
         d4 cr[NCB], ci[NCB];
 #pragma unroll
         for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
         const int row = rb * 16 + li;
-        for (int k0 = 0; k0 < nwsk; k0 += 4) {
+#pragma unroll 4
+        for (int k0 = 0; k0 < K2; k0 += 4) {
             const int kk = k0 + lk;
+            // G(kk) of gw_pair from the LDS pair table
             double2 a = make_double2(0.0, 0.0);
-            if (row < nb2 && kk < nwsk) a = A[(size_t)kk * nb2 + row];
-            const double nai = -a.y;
+            if (row < nb2 && kk < K2) {
+                const double4 t = ptab[kk >> 1];
+                const double2 ga = A[(size_t)(int)t.x * nb2 + row];
+                double2 u = make_double2(ga.x * t.y, ga.y * t.y), v = make_double2(0.0, 0.0);
+                if (t.z >= 0.0) {
+                    const double2 gb = A[(size_t)(int)t.z * nb2 + row];
+                    v = make_double2(gb.x * t.w, gb.y * t.w);
+                }
+                a = (kk & 1) ? make_double2(v.y - u.y, u.x - v.x) : make_double2(u.x + v.x, u.y + v.y);
+            }
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
                 const int col = c * 16 + li;
-                double2 b = make_double2(0.0, 0.0);
-                if (kk < nwsk) b = phiT[(size_t)kk * ldphi + col];
-                cmfma(a.x, a.y, nai, b.x, b.y, cr[c], ci[c]);
+                const double b = kk < K2 ? phi[(size_t)kk * ldphi + col] : 0.0;
+                cr[c] = mfma_f64(a.x, b, cr[c]);
+                ci[c] = mfma_f64(a.y, b, ci[c]);
             }
         }
 #pragma unroll
@@ -884,14 +941,16 @@ __global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const dou
 }
 
 // ---------------------------------------------------------------------------
-// Mixed-precision Re contraction (gkRp): gwann split once per context into
-// kNS int8 digits per (row i = (np,mp), slice = (sp,Rp)) with a per-row scale,
-// phases split per batch; 10 digit products on the int8 matrix cores with
-// exact int32 accumulation, recombined in FP64 into the FP64 g(k,Rp) layout.
+// Mixed-precision Re contraction (gkRp) in cell-pair form: the folded gwann
+// (gw_pair) split once per context into kNS int8 digits per (row i = (np,mp),
+// slice = (sp,Rp)) with a per-row scale, the real cos/sin phases split per
+// batch; 10 digit products x (Re, Im) of G on the int8 matrix cores with exact
+// int32 accumulation, recombined in FP64 into the FP64 g(k,Rp) layout.
 // ---------------------------------------------------------------------------
 
-// gscale[slice][i] = max over Re of max(|Re g|, |Im g|) (1 for zero rows).
-__global__ void k_gwann_scale(int nb2, int nwsk, int nslices, int nrb, const double2 *__restrict__ gw,
+// gscale[slice][i] = max over K2 of max(|Re G|, |Im G|) (1 for zero rows).
+__global__ void k_gwann_scale(int nb2, int nwsk, int nslices, int nrb, int npair, const int2 *__restrict__ pairs,
+                              const double *__restrict__ invdeg, const double2 *__restrict__ gw,
                               double *__restrict__ gscale) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nslices * nrb * 16) return;
@@ -899,19 +958,22 @@ __global__ void k_gwann_scale(int nb2, int nwsk, int nslices, int nrb, const dou
     const int64_t slice = t / (nrb * 16);
     double m = 0.0;
     if (i < nb2) {
-        const double2 *p = gw + (size_t)slice * nb2 * nwsk + i;
-        for (int r = 0; r < nwsk; ++r) {
-            const double2 v = p[(size_t)r * nb2];
+        const double2 *gs = gw + (size_t)slice * nb2 * nwsk;
+        for (int k2 = 0; k2 < 2 * npair; ++k2) {
+            const double2 v = gw_pair(gs, nb2, i, k2, npair, pairs, invdeg);
             m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
         }
     }
     gscale[t] = m > 0.0 ? m : 1.0;
 }
 
-// Gdig[slice][rb][kc][p][s][lane]: digits of gwann / scale in the int8 MFMA
-// A-operand order (lane i + 16 g holds rows rb*16 + i, K = 64 kc + 16 g .. +15).
-__global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc, const double2 *__restrict__ gw,
-                               const double *__restrict__ gscale, v4i *__restrict__ gdig) {
+// Gdig[slice][rb][kc][p][s][lane]: digits of G / scale in the int8 MFMA
+// A-operand order (lane i + 16 g holds rows rb*16 + i, K2 = 64 kc + 16 g .. +15),
+// p = Re, Im.
+__global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc, int npair,
+                               const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
+                               const double2 *__restrict__ gw, const double *__restrict__ gscale,
+                               v4i *__restrict__ gdig) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)nslices * nrb * nkc * 64;
     if (t >= total) return;
@@ -923,12 +985,13 @@ __global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc,
     const int64_t slice = r / nrb;
     const int i = rb * 16 + (lane & 15), g = lane >> 4;
     const double inv = 1.0 / gscale[slice * nrb * 16 + i];
+    const double2 *gs = gw + (size_t)slice * nb2 * nwsk;
     unsigned ure[kNS][4] = {}, uim[kNS][4] = {};
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
-        const int k = kc * 64 + g * 16 + b;
+        const int k2 = kc * 64 + g * 16 + b;
         double2 x = make_double2(0.0, 0.0);
-        if (i < nb2 && k < nwsk) x = gw[((size_t)slice * nwsk + k) * nb2 + i];
+        if (i < nb2) x = gw_pair(gs, nb2, i, k2, npair, pairs, invdeg);
         signed char dr[kNS], di[kNS];
         split_digits<kNS>(x.x * inv, dr);
         split_digits<kNS>(x.y * inv, di);
@@ -943,37 +1006,25 @@ __global__ void k_gwann_digits(int nb2, int nwsk, int nslices, int nrb, int nkc,
     }
 }
 
-// Bphi[kc][cb][p][s][lane], p = Re, Im, -Im: digits of the phases phiT[Re][col]
+// Bphi[kc][cb][s][lane]: digits of the real cell-pair phases phi[K2][col]
 // (|phi| <= 1, no scale) in the int8 MFMA B-operand order.
-__global__ void k_phase_digits(int nwsk, int nkc, int ncb, const double2 *__restrict__ phiT, int ldphi,
-                               v4i *__restrict__ bphi) {
+__global__ void k_phase_digits(int nkc, int ncb, const double *__restrict__ phi, int ldphi, v4i *__restrict__ bphi) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nkc * ncb * 64) return;
     const int lane = t % 64, cb = (t / 64) % ncb, kc = t / (64 * ncb);
     const int col = cb * 16 + (lane & 15), g = lane >> 4;
-    unsigned ure[kNS][4] = {}, uim[kNS][4] = {}, unim[kNS][4] = {};
+    unsigned u[kNS][4] = {};
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
         const int k = kc * 64 + g * 16 + b;
-        double2 x = make_double2(0.0, 0.0);
-        if (k < nwsk) x = phiT[(size_t)k * ldphi + col];
-        signed char dr[kNS], di[kNS];
-        split_digits<kNS>(x.x, dr);
-        split_digits<kNS>(x.y, di);
+        signed char d[kNS];
+        split_digits<kNS>(phi[(size_t)k * ldphi + col], d);
 #pragma unroll
-        for (int sd = 0; sd < kNS; ++sd) {
-            put_byte(ure[sd], b, dr[sd]);
-            put_byte(uim[sd], b, di[sd]);
-            put_byte(unim[sd], b, (signed char)(-di[sd]));
-        }
+        for (int sd = 0; sd < kNS; ++sd) put_byte(u[sd], b, d[sd]);
     }
-    v4i *o = bphi + (((size_t)kc * ncb + cb) * 3) * kNS * 64 + lane;
+    v4i *o = bphi + ((size_t)kc * ncb + cb) * kNS * 64 + lane;
 #pragma unroll
-    for (int sd = 0; sd < kNS; ++sd) {
-        o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
-        o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
-        o[(2 * kNS + sd) * 64] = as_v4i(unim[sd]);
-    }
+    for (int sd = 0; sd < kNS; ++sd) o[sd * 64] = as_v4i(u[sd]);
 }
 
 // One workgroup per (sp,Rp) slice; wave w takes row blocks w, w+4, ...; all
@@ -1001,29 +1052,18 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
                 Ar[sd] = ga[((size_t)kc * 2 * NS + 0 * NS + sd) * 64];
                 Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
             }
-            const v4i *bb = bphi + ((size_t)kc * NCB * 3) * NS * 64 + lane;
+            const v4i *bb = bphi + (size_t)kc * NCB * NS * 64 + lane;
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
-                v4i Bre[NCB], Bim[NCB], Bnim[NCB];
+                v4i B[NCB];
 #pragma unroll
-                for (int c = 0; c < NCB; ++c) {
-                    Bre[c] = bb[((c * 3 + 0) * NS + j) * 64];
-                    Bim[c] = bb[((c * 3 + 1) * NS + j) * 64];
-                    Bnim[c] = bb[((c * 3 + 2) * NS + j) * 64];
-                }
+                for (int c = 0; c < NCB; ++c) B[c] = bb[(c * NS + j) * 64];
 #pragma unroll
                 for (int i = 0; i < NS - j; ++i)
 #pragma unroll
                     for (int c = 0; c < NCB; ++c) {
-                        acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
-                        acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
-                    }
-#pragma unroll
-                for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                    for (int c = 0; c < NCB; ++c) {
-                        acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
-                        acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
+                        acc[c][i + j][0] = mfma_i8(Ar[i], B[c], acc[c][i + j][0]);
+                        acc[c][i + j][1] = mfma_i8(Ai[i], B[c], acc[c][i + j][1]);
                     }
             }
         }
@@ -2002,6 +2042,20 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     c->degk.upload(deg32(c->elwsdeg), c->stream);
     c->degq.upload(deg32(c->phwsdeg), c->stream);
     c->degg.upload(deg32(c->gwsdeg), c->stream);
+    {
+        // Re cells in pairs (R, -R) for the cell-pair form of the Re contraction
+        const std::vector<int2> pr = cell_pairs(c->rcells_k.data(), c->nwsk, c->nwsk);
+        c->npairk = (int64_t)pr.size();
+        c->gk2p = ((2 * c->npairk + 63) / 64) * 64;
+        std::vector<int> pc(3 * c->npairk);
+        for (int64_t q = 0; q < c->npairk; ++q)
+            for (int d = 0; d < 3; ++d) pc[3 * q + d] = (int)c->rcells_k[pr[q].x + c->nwsk * d];
+        c->gpair.upload(pr, c->stream);
+        c->gpcell.upload(pc, c->stream);
+        std::vector<double> inv(c->nwsk);
+        for (int64_t q = 0; q < c->nwsk; ++q) inv[q] = 1.0 / (double)c->elwsdeg[q];
+        c->ginvdeg.upload(inv, c->stream);
+    }
     build_phonon_terms(c);
     if (c->polar) build_polar_terms(c);
     HCHECK(hipStreamSynchronize(c->stream));
@@ -2411,43 +2465,54 @@ void elph::ensure_gwann_digits(elph_ctx *c) {
     if (c->gdig.p) return;
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
     c->gnrb = (nb2 + 15) / 16;
-    c->gnkc = (c->nwsk + 63) / 64;
+    c->gnkc = c->gk2p / 64;
     c->gscale.alloc(nslices * c->gnrb * 16);
     const int64_t ns = nslices * c->gnrb * 16;
     hipLaunchKernelGGL(k_gwann_scale, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c->stream, (int)nb2,
-                       (int)c->nwsk, (int)nslices, (int)c->gnrb, c->gwann.p, c->gscale.p);
+                       (int)c->nwsk, (int)nslices, (int)c->gnrb, (int)c->npairk, c->gpair.p, c->ginvdeg.p, c->gwann.p,
+                       c->gscale.p);
     const int64_t nt = nslices * c->gnrb * c->gnkc * 64;
     c->gdig.alloc(nt * 2 * kNS);
     hipLaunchKernelGGL(k_gwann_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, (int)nb2,
-                       (int)c->nwsk, (int)nslices, (int)c->gnrb, (int)c->gnkc, c->gwann.p, c->gscale.p, c->gdig.p);
+                       (int)c->nwsk, (int)nslices, (int)c->gnrb, (int)c->gnkc, (int)c->npairk, c->gpair.p, c->ginvdeg.p,
+                       c->gwann.p, c->gscale.p, c->gdig.p);
     HCHECK(hipGetLastError());
 }
 
+int64_t elph::gk_phi_count(const elph_ctx *c) { return c->gk2p * 64 / 2; }
+int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * 4 * kNS * 64; }
+
 // Batched gkRp into device memory: out[k] (nb*nb, nbr*nwsg) for kvecs (nk,3)
-// column-major on the device; FP64 or int8-digit (mixed) contraction.
-// phiT (nwsk*64) and, for mixed, bphi are scratch buffers of the caller.
+// column-major on the device; FP64 or int8-digit (mixed) contraction, both in
+// cell-pair form.  phiT (gk_phi_count) and, for mixed, bphi (gk_bphi_count)
+// are scratch buffers of the caller.
 void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
                         int64_t out_kstride, bool mixed, double2 *phiT, v4i *bphi) {
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
     const int64_t KT = 64;
+    double *phi = reinterpret_cast<double *>(phiT);
+    if (!mixed && c->npairk * (int64_t)sizeof(double4) > 160 * 1024)
+        throw Error{ELPH_EINVAL, "FP64 Re contraction holds up to 5120 cell pairs (10240 electron WS cells)"};
     for (int64_t k0 = 0; k0 < nk; k0 += KT) {
         const int64_t n = std::min(KT, nk - k0);
-        // phases in [Re][k] layout (64 columns, unused columns zero)
-        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsk * KT + 255) / 256)), dim3(256), 0, st, phiT, c->nwsk * KT);
-        dim3 grid((unsigned)((c->nwsk + 127) / 128), (unsigned)n);
-        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, st, (int)n, (int)c->nwsk, d_kvecs + k0, nk, c->rk.p,
-                           c->degk.p, phiT, (int64_t)1, KT);
+        // real phases in [K2][k] layout (64 columns; unused rows and columns zero)
+        const int64_t nz = gk_phi_count(c);
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, phiT, nz);
+        dim3 grid((unsigned)((c->npairk + 127) / 128), (unsigned)n);
+        hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, st, (int)n, (int)c->npairk, d_kvecs + k0, nk,
+                           c->gpcell.p, phi, (int64_t)1, KT);
         if (mixed) {
             const int nt = (int)(c->gnkc * (KT / 16) * 64);
-            hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->nwsk,
-                               (int)c->gnkc, (int)(KT / 16), phiT, (int)KT, bphi);
+            hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->gnkc,
+                               (int)(KT / 16), phi, (int)KT, bphi);
             hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)nslices), dim3(256), 0, st, c->gdig.p, c->gscale.p,
                                (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
                                out + (size_t)k0 * out_kstride, out_kstride);
         } else {
-            hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), 0, st, c->gwann.p, (int)nb2,
-                               (int)c->nwsk, (int)nslices, phiT, (int)KT, (int)n, out + (size_t)k0 * out_kstride,
-                               out_kstride);
+            hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), c->npairk * sizeof(double4), st,
+                               c->gwann.p, (int)nb2,
+                               (int)c->nwsk, (int)nslices, (int)c->npairk, c->gpair.p, c->ginvdeg.p, phi, (int)KT,
+                               (int)n, out + (size_t)k0 * out_kstride, out_kstride);
         }
         HCHECK(hipGetLastError());
     }
@@ -2457,10 +2522,10 @@ static void gkRp_device(elph_ctx *c, int64_t nk, const double *d_kvecs, double2 
                         bool mixed = false) {
     DBuf<double2> phiT;
     DBuf<v4i> bphi;
-    phiT.alloc(c->nwsk * 64);
+    phiT.alloc(gk_phi_count(c));
     if (mixed) {
         ensure_gwann_digits(c);
-        bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
+        bphi.alloc(gk_bphi_count(c));
     }
     launch_gkRp(c, c->stream, nk, d_kvecs, out, out_kstride, mixed, phiT.p, bphi.p);
     HCHECK(hipStreamSynchronize(c->stream));
@@ -2966,9 +3031,9 @@ void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t
     const int64_t nb2 = p->nb * p->nb, nslices = p->nbr * c->nwsg;
     sc.gk.alloc(nkb * nb2 * nslices);
     sc.h.alloc(nst * p->Kp * p->CP);
-    sc.phiT.alloc(c->nwsk * 64);
+    sc.phiT.alloc(gk_phi_count(c));
     if (p->precision == ELPH_MIXED) {
-        sc.bphi.alloc(c->gnkc * 4 * 3 * kNS * 64);
+        sc.bphi.alloc(gk_bphi_count(c));
         sc.bscale.alloc(nst * p->CP);
         sc.bsl.alloc(nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
     }

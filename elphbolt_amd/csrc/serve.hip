@@ -621,9 +621,9 @@ void size_slots(elph_server *s) {
         Ukq = std::max(Ukq, kPairMax * m.nb2);
         gk = std::max(gk, m.KT * m.gk_k);
         hh = std::max(hh, m.KT * m.nb * m.Kp * m.CPT);
-        phiT = std::max(phiT, c->nwsk * 64);
+        phiT = std::max(phiT, gk_phi_count(c));
         if (s->precision == ELPH_MIXED) {
-            bphi = std::max(bphi, c->gnkc * 4 * 3 * kNS * 64);
+            bphi = std::max(bphi, gk_bphi_count(c));
             bsl = std::max(bsl, m.KT * m.nb * m.nkc * (m.CPT / 16) * 3 * kNS * 64);
             bscale = std::max(bscale, m.KT * m.nb * m.CPT);
         }
