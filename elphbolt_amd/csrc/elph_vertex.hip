@@ -391,6 +391,14 @@ __device__ __forceinline__ double pair_phase(const double2 *Tsh, const int4 *Rsh
     const double2 t = Tsh[pos_mod(coef[0] * R.x + coef[1] * R.y + coef[2] * R.z, L)];
     return (k & 1) ? t.y : t.x;
 }
+// The same for L <= 255 and electron meshes up to 255: Rsh[p].w holds R_p
+// reduced modulo the mesh, one byte per component, and cpk the byte-packed
+// coefficients (phase_index).
+__device__ __forceinline__ double pair_phase_packed(const double2 *Tsh, const int4 *Rsh, unsigned cpk, unsigned lmul,
+                                                    int negL, int k) {
+    const double2 t = Tsh[phase_index(cpk, (unsigned)Rsh[k >> 1].w, lmul, negL)];
+    return (k & 1) ? t.y : t.x;
+}
 
 // Phase-contraction core of the mesh drivers: the phase of row r and cell R_K
 // is the exact root of unity T[(coef_r . R_K) mod L] (see gemm_core_f64).
@@ -406,7 +414,8 @@ __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst,
     gemm_core_f64<CB, RB, KC>(hst, 16 * CB, Kp, aop, buf0, buf1, cr, ci);
 }
 
-template <int CB, int NBM, int NBRM>
+// PACKED: phase indices by pair_phase_packed (L and the mesh up to 255).
+template <int CB, int NBM, int NBRM, bool PACKED>
 __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
     constexpr int KC = 32;
     constexpr int RB = 2;
@@ -424,7 +433,8 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
     const double2 *hst = a.h + (size_t)st * a.Kp * CP;
 
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp / 2; i += 256) Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], 0);
+    for (int i = tid; i < a.Kp / 2; i += 256)
+        Rsh[i] = make_int4(a.Rq[4 * i], a.Rq[4 * i + 1], a.Rq[4 * i + 2], a.Rq[4 * i + 3]);
 
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
     int coef[RB][3];
@@ -443,7 +453,16 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
         }
     }
     d4 cr[RB][CB], ci[RB][CB];
-    {
+    if constexpr (PACKED) {
+        unsigned cpk[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+            cpk[rb] = (unsigned)coef[rb][0] | ((unsigned)coef[rb][1] << 8) | ((unsigned)coef[rb][2] << 16);
+        const unsigned lmul = a.Lmul;
+        const int negL = -a.L;
+        auto aop = [&](int rb, int k) { return pair_phase_packed(Tsh, Rsh, cpk[rb], lmul, negL, k); };
+        gemm_core_f64<CB, RB, KC>(hst, CP, a.Kp, aop, buf0, buf1, cr, ci);
+    } else {
         const int L = a.L;
         auto aop = [&](int rb, int k) { return pair_phase(Tsh, Rsh, coef[rb], L, k); };
         gemm_core_f64<CB, RB, KC>(hst, CP, a.Kp, aop, buf0, buf1, cr, ci);
@@ -2762,6 +2781,7 @@ struct elph_ibzk_plan {
     // Rp cells in pairs (R_p, -R_p): Kp = 2 npair rounded up to 64 rows (cos, sin)
     int64_t npair = 0, Kp = 0, CP = 0;
     DBuf<int2> pairs;
+    bool packed = false;  // L and the electron mesh up to 255 (byte-packed phase indices)
     std::vector<int> kint_irr;     // [nirr][3]
     std::vector<double> ens_irred; // host (nirr,nb)
     double enref = 0, fsthick = 0;
@@ -2944,20 +2964,25 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
         // waves per SIMD (CB <= 2)
         int e = epilogue_class(p->nb, p->nbr);
         if (e == 2 && CB > 2) e = 0;
-        switch (CB * 4 + e) {
-            case 4 + 1: go(k_pair_fp64<1, 8, 6>); break;
-            case 8 + 1: go(k_pair_fp64<2, 8, 6>); break;
-            case 12 + 1: go(k_pair_fp64<3, 8, 6>); break;
-            case 4 + 2: go(k_pair_fp64<1, 16, 12>); break;
-            case 8 + 2: go(k_pair_fp64<2, 16, 12>); break;
-            case 4: go(k_pair_fp64<1, 0, 0>); break;
-            case 8: go(k_pair_fp64<2, 0, 0>); break;
-            case 12: go(k_pair_fp64<3, 0, 0>); break;
-            case 16: go(k_pair_fp64<4, 0, 0>); break;
-            case 20: go(k_pair_fp64<5, 0, 0>); break;
-            case 24: go(k_pair_fp64<6, 0, 0>); break;
-            default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
-        }
+        auto launch = [&](auto packed) {
+            constexpr bool P = decltype(packed)::value;
+            switch (CB * 4 + e) {
+                case 4 + 1: go(k_pair_fp64<1, 8, 6, P>); break;
+                case 8 + 1: go(k_pair_fp64<2, 8, 6, P>); break;
+                case 12 + 1: go(k_pair_fp64<3, 8, 6, P>); break;
+                case 4 + 2: go(k_pair_fp64<1, 16, 12, P>); break;
+                case 8 + 2: go(k_pair_fp64<2, 16, 12, P>); break;
+                case 4: go(k_pair_fp64<1, 0, 0, P>); break;
+                case 8: go(k_pair_fp64<2, 0, 0, P>); break;
+                case 12: go(k_pair_fp64<3, 0, 0, P>); break;
+                case 16: go(k_pair_fp64<4, 0, 0, P>); break;
+                case 20: go(k_pair_fp64<5, 0, 0, P>); break;
+                case 24: go(k_pair_fp64<6, 0, 0, P>); break;
+                default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
+            }
+        };
+        if (p->packed) launch(std::true_type{});
+        else launch(std::false_type{});
     }
 }
 
@@ -3193,10 +3218,20 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             T[j] = make_double2(std::cos(x), std::sin(x));
         }
         p->T.upload(T, s);
-        // first cell of each pair (rcells_q(1:nwsg) in pairs, padded with 0)
+        // first cell of each pair (rcells_q(1:nwsg) in pairs, padded with 0),
+        // and (meshes and L up to 255) that cell reduced modulo the electron
+        // mesh, one byte per component
+        p->packed = p->L <= 255 && p->Nel[0] <= 255 && p->Nel[1] <= 255 && p->Nel[2] <= 255;
         std::vector<int> R(4 * (p->Kp / 2), 0);
-        for (int64_t i = 0; i < p->npair; ++i)
-            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_q[pairs[i].x + ctx->nwsq * d];
+        for (int64_t i = 0; i < p->npair; ++i) {
+            unsigned v = 0;
+            for (int d = 0; d < 3; ++d) {
+                const int64_t c = ctx->rcells_q[pairs[i].x + ctx->nwsq * d];
+                R[4 * i + d] = (int)c;
+                if (p->packed) v |= (unsigned)fmodulo(c, p->Nel[d]) << (8 * d);
+            }
+            R[4 * i + 3] = (int)v;
+        }
         p->Rq.upload(R, s);
         p->pairs.upload(pairs, s);
         if (precision == ELPH_MIXED) {
