@@ -103,32 +103,29 @@ __global__ void k_ph_post(int nq, int nbr, const double *__restrict__ qv, int ld
     if (bad) *flag = 1;  // vector store; every writer stores the same value
 }
 
-// PhiT[K][q] = e^{i 2 pi (q . R_K - nint(q . R_K))} for K < nwsg, 0 for the
-// padding; K-major with row stride ldp (columns q >= nq are 0).
-__global__ void k_srv_phase(int nq, int ldp, int Kp, int nwsg, const double *__restrict__ qv, int ldq,
-                            const int *__restrict__ cells, double2 *__restrict__ phi) {
+// PhiT[K2][q] (real, row stride ldp) of the cell pairs (R_p, -R_p): cos (K2 =
+// 2p) and sin (K2 = 2p + 1) of 2 pi (q . R_p - nint(q . R_p)); 0 past the
+// pairs and for columns q >= nq.  With the folded h of k_rotate_all,
+// sum_K2 PhiT[K2][q] h[K2] = sum_R e^{i 2 pi q . R} h(R).
+__global__ void k_srv_phase(int nq, int ldp, int Kp, int npair, const double *__restrict__ qv, int ldq,
+                            const int *__restrict__ pcell, double *__restrict__ phi) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)ldp * Kp) return;
-    const int k = (int)(t / ldp), q = (int)(t % ldp);
-    if (q >= nq) {
-        phi[t] = make_double2(0.0, 0.0);
-        return;
+    if (t >= (int64_t)ldp * (Kp / 2)) return;
+    const int p = (int)(t / ldp), q = (int)(t % ldp);
+    double c = 0.0, sn = 0.0;
+    if (q < nq && p < npair) {
+        const double x = qv[q] * (double)pcell[3 * p] + qv[q + ldq] * (double)pcell[3 * p + 1] +
+                         qv[q + 2 * (size_t)ldq] * (double)pcell[3 * p + 2];
+        sincos(kTwoPi * (x - rint(x)), &sn, &c);
     }
-    double2 v = make_double2(0.0, 0.0);
-    if (k < nwsg) {
-        const double x = qv[q] * (double)cells[3 * k] + qv[q + ldq] * (double)cells[3 * k + 1] +
-                         qv[q + 2 * (size_t)ldq] * (double)cells[3 * k + 2];
-        double s, c;
-        sincos(kTwoPi * (x - rint(x)), &s, &c);
-        v = make_double2(c, s);
-    }
-    phi[t] = v;
+    phi[(size_t)(2 * p) * ldp + q] = c;
+    phi[(size_t)(2 * p + 1) * ldp + q] = sn;
 }
 
 // int8 digits of Phi in the A-operand order of the 16x16x64 int8 MFMA:
-// adig[((qb * nkc + kc) * 2 + part) * kNS + d][lane], lane = i + 16 g holding
-// rows qb*16 + i and K = kc*64 + 16 g .. +15; part 0 = Re, 1 = Im; rows >= nq are 0.
-__global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const double2 *__restrict__ phi,
+// adig[((qb * nkc + kc) * kNS + d][lane], lane = i + 16 g holding rows
+// qb*16 + i and K2 = kc*64 + 16 g .. +15; rows >= nq are 0.
+__global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const double *__restrict__ phi,
                                    v4i *__restrict__ adig) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nqb * nkc * 64) return;
@@ -136,55 +133,49 @@ __global__ void k_srv_phase_digits(int nq, int nqb, int nkc, int ldp, const doub
     const int kc = (int)((t / 64) % nkc);
     const int qb = (int)(t / (64 * nkc));
     const int q = qb * 16 + (lane & 15), g = lane >> 4;
-    unsigned ur[kNS][4] = {}, ui[kNS][4] = {};
+    unsigned u[kNS][4] = {};
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
-        double2 x = make_double2(0.0, 0.0);
-        if (q < nq) x = phi[(size_t)(kc * 64 + g * 16 + b) * ldp + q];
-        signed char dr[kNS], di[kNS];
-        split_digits<kNS>(x.x, dr);
-        split_digits<kNS>(x.y, di);
+        const double x = q < nq ? phi[(size_t)(kc * 64 + g * 16 + b) * ldp + q] : 0.0;
+        signed char d[kNS];
+        split_digits<kNS>(x, d);
 #pragma unroll
-        for (int d = 0; d < kNS; ++d) {
-            put_byte(ur[d], b, dr[d]);
-            put_byte(ui[d], b, di[d]);
-        }
+        for (int e = 0; e < kNS; ++e) put_byte(u[e], b, d[e]);
     }
-    v4i *o = adig + (((size_t)qb * nkc + kc) * 2) * kNS * 64 + lane;
+    v4i *o = adig + ((size_t)qb * nkc + kc) * kNS * 64 + lane;
 #pragma unroll
-    for (int d = 0; d < kNS; ++d) {
-        o[d * 64] = as_v4i(ur[d]);
-        o[(kNS + d) * 64] = as_v4i(ui[d]);
-    }
+    for (int e = 0; e < kNS; ++e) o[e * 64] = as_v4i(u[e]);
 }
 
-// Y[st][r][col] = sum_K PhiT[K][q0 + r] h[st0 + st][K][col] (FP64 matrix cores).
+// Y[st][r][col] = sum_K2 PhiT[K2][q0 + r] h[st0 + st][K2][col] (FP64 matrix
+// cores; real phase operand, two products per K2).
 // Block: 4 waves x 32 rows; grid (rows / 128, states, column tiles of 16*CB).
 // Each K chunk of both operands (128 phase rows, K-major, and the 16*CB
 // columns of h) is staged in LDS; the next chunk is loaded into registers
 // while the current one computes.
 template <int CB>
 __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ h, int Kp, int CPT, int st0,
-                                                    const double2 *__restrict__ phiT, int ldp, int q0, int nqs,
+                                                    const double *__restrict__ phiT, int ldp, int q0, int nqs,
                                                     double2 *__restrict__ Y) {
     constexpr int KC = 16, RB = 2, CP = 16 * CB, ROWS = 128;
     constexpr int NA = KC * ROWS / 256, NB = (KC * CP + 255) / 256;
     extern __shared__ double2 smem[];
-    double2 *As = smem, *Bs = smem + KC * ROWS;
+    double *As = reinterpret_cast<double *>(smem);
+    double2 *Bs = smem + KC * ROWS / 2;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int st = blockIdx.y, col0 = blockIdx.z * CP;
     const int row0 = blockIdx.x * ROWS;
     const double2 *B = h + (size_t)(st0 + st) * Kp * CPT + col0;
-    dv2 pa[NA], pb[NB];
-    const dv2 *phv = reinterpret_cast<const dv2 *>(phiT);
+    double pa[NA];
+    dv2 pb[NB];
     const dv2 *Bv = reinterpret_cast<const dv2 *>(B);
-    dv2 *Asv = reinterpret_cast<dv2 *>(As), *Bsv = reinterpret_cast<dv2 *>(Bs);
+    dv2 *Bsv = reinterpret_cast<dv2 *>(Bs);
     auto load = [&](int ch) {
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int i = tid + 256 * j, k = i / ROWS, r = i % ROWS;
-            pa[j] = (row0 + r < nqs) ? phv[(size_t)(ch * KC + k) * ldp + q0 + row0 + r] : dv2{0.0, 0.0};
+            pa[j] = (row0 + r < nqs) ? phiT[(size_t)(ch * KC + k) * ldp + q0 + row0 + r] : 0.0;
         }
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
@@ -194,7 +185,7 @@ __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ 
     };
     auto store = [&]() {
 #pragma unroll
-        for (int j = 0; j < NA; ++j) Asv[tid + 256 * j] = pa[j];
+        for (int j = 0; j < NA; ++j) As[tid + 256 * j] = pa[j];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const int i = tid + 256 * j;
@@ -216,19 +207,17 @@ __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ 
 #pragma unroll
         for (int ks = 0; ks < KC / 4; ++ks) {
             const int kl = ks * 4 + lk;
-            double ar[RB], ai[RB], nai[RB];
+            double av[RB];
 #pragma unroll
-            for (int rb = 0; rb < RB; ++rb) {
-                const double2 t = As[kl * ROWS + wave * 32 + rb * 16 + li];
-                ar[rb] = t.x;
-                ai[rb] = t.y;
-                nai[rb] = -t.y;
-            }
+            for (int rb = 0; rb < RB; ++rb) av[rb] = As[kl * ROWS + wave * 32 + rb * 16 + li];
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
                 const double2 bv = Bs[kl * CP + c * 16 + li];
 #pragma unroll
-                for (int rb = 0; rb < RB; ++rb) cmfma(ar[rb], ai[rb], nai[rb], bv.x, bv.y, cr[rb][c], ci[rb][c]);
+                for (int rb = 0; rb < RB; ++rb) {
+                    cr[rb][c] = mfma_f64(av[rb], bv.x, cr[rb][c]);
+                    ci[rb][c] = mfma_f64(av[rb], bv.y, ci[rb][c]);
+                }
             }
         }
         __syncthreads();
@@ -250,27 +239,55 @@ __global__ __launch_bounds__(256) void k_srv_y_fp64(const double2 *__restrict__ 
             }
 }
 
-// h[(k * nb + m)][rp][c] = sum_mp U_k(m,mp) gk[k](np, mp, sp, rp) / phwsdeg(rp) for
-// all bands m (c = np + nb sp; zero for rp >= nwsg or c >= nb nbr).  One
-// thread per (k, rp, c) reads its g column once and writes all nb rows.
+// h(R) = sum_mp U_k(m,mp) gk[k](np, mp, sp, R) / phwsdeg(R) (c = np + nb sp),
+// folded over the cell pairs (R_p, -R_p) of the Rp list for all bands m:
+//   h[(k * nb + m)][2p][c] = h(R_p) + h(-R_p),  h[(k * nb + m)][2p + 1][c] = i (h(R_p) - h(-R_p))
+// (zero past the pairs or for c >= nb nbr; -R_p absent: h(-R_p) = 0).  One
+// thread per (k, p, c) reads its two g columns once and writes all nb bands.
 template <int NB>
-__global__ __launch_bounds__(256) void k_rotate_all(int nb, int nbr, int nwsg, int Kp, int CPT,
+__global__ __launch_bounds__(256) void k_rotate_all(int nb, int nbr, int npair, int Kp, int CPT,
                                                     const double2 *__restrict__ gk, int64_t kstride,
                                                     const double2 *__restrict__ Uk, const int *__restrict__ degq,
-                                                    double2 *__restrict__ h) {
+                                                    const int2 *__restrict__ pairs, double2 *__restrict__ h) {
     __shared__ double2 U[NB * NB];
     const int k = blockIdx.y;
     for (int i = threadIdx.x; i < nb * nb; i += blockDim.x) U[i] = Uk[(size_t)k * nb * nb + i];
     __syncthreads();
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)Kp * CPT) return;
-    const int rp = (int)(t / CPT), c = (int)(t % CPT);
+    if (t >= (int64_t)(Kp / 2) * CPT) return;
+    const int p = (int)(t / CPT), c = (int)(t % CPT);
+    const bool live = p < npair && c < nb * nbr;
+    const int np = c % nb, sp = c / nb;
+    const int2 pr = live ? pairs[p] : make_int2(0, -1);
+    // h(R_p) for every band m
+    double2 hi[NB];
+    {
+        double2 g[NB];
+        if (live) {
+            const double2 *src = gk + (size_t)k * kstride + ((size_t)(sp + nbr * pr.x)) * nb * nb + np;
+            const double inv = 1.0 / (double)degq[pr.x];
+#pragma unroll
+            for (int mp = 0; mp < NB; ++mp) {
+                g[mp] = mp < nb ? src[(size_t)mp * nb] : make_double2(0.0, 0.0);
+                g[mp].x *= inv;
+                g[mp].y *= inv;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < NB; ++m) {
+            double2 acc = make_double2(0.0, 0.0);
+            if (live && m < nb)
+#pragma unroll
+                for (int mp = 0; mp < NB; ++mp)
+                    if (mp < nb) cfma(acc, U[m * nb + mp], g[mp]);
+            hi[m] = acc;
+        }
+    }
     double2 g[NB];
-    const bool live = rp < nwsg && c < nb * nbr;
-    if (live) {
-        const int np = c % nb, sp = c / nb;
-        const double2 *src = gk + (size_t)k * kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
-        const double inv = 1.0 / (double)degq[rp];
+    const bool partner = live && pr.y >= 0;
+    if (partner) {
+        const double2 *src = gk + (size_t)k * kstride + ((size_t)(sp + nbr * pr.y)) * nb * nb + np;
+        const double inv = 1.0 / (double)degq[pr.y];
 #pragma unroll
         for (int mp = 0; mp < NB; ++mp) {
             g[mp] = mp < nb ? src[(size_t)mp * nb] : make_double2(0.0, 0.0);
@@ -278,29 +295,33 @@ __global__ __launch_bounds__(256) void k_rotate_all(int nb, int nbr, int nwsg, i
             g[mp].y *= inv;
         }
     }
-    double2 *dst = h + ((size_t)k * nb * Kp + rp) * CPT + c;
-    for (int m = 0; m < nb; ++m) {
-        double2 acc = make_double2(0.0, 0.0);
-        if (live)
+    double2 *dst = h + ((size_t)k * nb * Kp + 2 * p) * CPT + c;
 #pragma unroll
-            for (int mp = 0; mp < NB; ++mp)
-                if (mp < nb) cfma(acc, U[m * nb + mp], g[mp]);
-        dst[(size_t)m * Kp * CPT] = acc;
+    for (int m = 0; m < NB; ++m) {
+        if (m < nb) {
+            double2 hj = make_double2(0.0, 0.0);
+            if (partner)
+#pragma unroll
+                for (int mp = 0; mp < NB; ++mp)
+                    if (mp < nb) cfma(hj, U[m * nb + mp], g[mp]);
+            dst[(size_t)m * Kp * CPT] = make_double2(hi[m].x + hj.x, hi[m].y + hj.y);
+            dst[(size_t)m * Kp * CPT + CPT] = make_double2(hj.y - hi[m].y, hi[m].x - hj.x);
+        }
     }
 }
 
-// Mixed-precision form of k_srv_y_fp64: kNS int8 digits of Phi (adig) and of
-// h / column scale (bsl, [st][kc][cb][Re, Im, -Im][digit][lane]); the digit
-// products with i + j < kNS accumulate exactly in int32 per level i + j and
-// are recombined in FP64 with the weights 1 / (127^2 254^l).  Block: 4 waves x
-// 16 rows; grid (rows / 64, states, column tiles of 16*CB).
+// Mixed-precision form of k_srv_y_fp64: kNS int8 digits of the real Phi
+// (adig) and of h / column scale (bsl, [st][kc][cb][Re, Im][digit][lane]);
+// the digit products with i + j < kNS accumulate exactly in int32 per level
+// i + j and are recombined in FP64 with the weights 1 / (127^2 254^l).
+// Block: 4 waves x 16 rows; grid (rows / 64, states, column tiles of 16*CB).
 template <int CB>
 __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl, int nkc, int CBT, int st0,
                                                      const double *__restrict__ bscale, int CPT,
                                                      const v4i *__restrict__ adig, int qb0, int nqs,
                                                      double2 *__restrict__ Y) {
     constexpr int NS = kNS;
-    constexpr int BCH = CB * 3 * NS * 64;
+    constexpr int BCH = CB * 2 * NS * 64;
     constexpr int NPF = (BCH + 255) / 256;
     extern __shared__ v4i smem8[];
     v4i *buf0 = smem8, *buf1 = smem8 + BCH;
@@ -309,18 +330,18 @@ __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl
     const int st = blockIdx.y, cb0 = blockIdx.z * CB;
     const int row_base = blockIdx.x * 64 + wave * 16;
     const int qb = qb0 + row_base / 16;
-    const v4i *ast = adig + (size_t)qb * nkc * 2 * NS * 64 + lane;
-    const v4i *bst = bsl + (size_t)(st0 + st) * nkc * CBT * 3 * NS * 64 + (size_t)cb0 * 3 * NS * 64;
-    const size_t bkc = (size_t)CBT * 3 * NS * 64;  // v4i per K chunk of one state
+    const v4i *ast = adig + (size_t)qb * nkc * NS * 64 + lane;
+    const v4i *bst = bsl + (size_t)(st0 + st) * nkc * CBT * 2 * NS * 64 + (size_t)cb0 * 2 * NS * 64;
+    const size_t bkc = (size_t)CBT * 2 * NS * 64;  // v4i per K chunk of one state
     v4i acc[CB][NS][2];
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
         for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
     for (int i = tid; i < BCH; i += 256) buf0[i] = bst[i];
-    v4i Ar[NS], Ai[NS];
+    v4i A[NS];
 #pragma unroll
-    for (int d = 0; d < NS; ++d) { Ar[d] = ast[d * 64]; Ai[d] = ast[(NS + d) * 64]; }
+    for (int d = 0; d < NS; ++d) A[d] = ast[d * 64];
     __syncthreads();
     for (int kc = 0; kc < nkc; ++kc) {
         const int kn = min(kc + 1, nkc - 1);
@@ -331,37 +352,29 @@ __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl
             const int i = tid + 256 * j;
             if (BCH % 256 == 0 || i < BCH) pf[j] = src[i];
         }
-        v4i nAr[NS], nAi[NS];
-        const v4i *an = ast + (size_t)kn * 2 * NS * 64;
+        v4i nA[NS];
+        const v4i *an = ast + (size_t)kn * NS * 64;
 #pragma unroll
-        for (int d = 0; d < NS; ++d) { nAr[d] = an[d * 64]; nAi[d] = an[(NS + d) * 64]; }
+        for (int d = 0; d < NS; ++d) nA[d] = an[d * 64];
         const v4i *B = (kc & 1) ? buf1 : buf0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            v4i Bre[CB], Bim[CB], Bnim[CB];
+            v4i Bre[CB], Bim[CB];
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
-                Bre[c] = B[((c * 3 + 0) * NS + j) * 64 + lane];
-                Bim[c] = B[((c * 3 + 1) * NS + j) * 64 + lane];
-                Bnim[c] = B[((c * 3 + 2) * NS + j) * 64 + lane];
+                Bre[c] = B[((c * 2 + 0) * NS + j) * 64 + lane];
+                Bim[c] = B[((c * 2 + 1) * NS + j) * 64 + lane];
             }
 #pragma unroll
             for (int i = 0; i < NS - j; ++i)
 #pragma unroll
                 for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ar[i], Bre[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ar[i], Bim[c], acc[c][i + j][1]);
-                }
-#pragma unroll
-            for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    acc[c][i + j][0] = mfma_i8(Ai[i], Bnim[c], acc[c][i + j][0]);
-                    acc[c][i + j][1] = mfma_i8(Ai[i], Bre[c], acc[c][i + j][1]);
+                    acc[c][i + j][0] = mfma_i8(A[i], Bre[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(A[i], Bim[c], acc[c][i + j][1]);
                 }
         }
 #pragma unroll
-        for (int d = 0; d < NS; ++d) { Ar[d] = nAr[d]; Ai[d] = nAi[d]; }
+        for (int d = 0; d < NS; ++d) A[d] = nA[d];
         v4i *dst = (kc & 1) ? buf0 : buf1;
 #pragma unroll
         for (int j = 0; j < NPF; ++j) {
@@ -529,6 +542,11 @@ struct Pool {
 
 struct Material {
     elph_ctx *c = nullptr;
+    // Rp cells (rcells_q(1:nwsg)) in pairs (R_p, -R_p): Kp = 2 npair cos/sin
+    // rows rounded up to 64; pcell[p] = R_p
+    int64_t npair = 0;
+    DBuf<int2> pairs;
+    DBuf<int> pcell;
     int64_t nb = 0, nbr = 0, nb2 = 0, Kp = 0, nkc = 0, CB = 1, CPT = 16, KT = 1;
     int64_t gk_k = 0;  // complex per k of g(k,Rp)
     DBuf<double> mscale, invsqrtm;
@@ -545,7 +563,8 @@ struct Request {
     int64_t off = -1, bytes = 0;
     // device views into the pool
     double *d_kv = nullptr, *d_qv = nullptr, *d_W = nullptr, *d_om = nullptr;
-    double2 *d_D = nullptr, *d_V = nullptr, *d_phi = nullptr, *d_Lq = nullptr, *d_lp = nullptr;
+    double2 *d_D = nullptr, *d_V = nullptr, *d_Lq = nullptr, *d_lp = nullptr;
+    double *d_phi = nullptr;  // real cell-pair phases [K2][nq64]
     v4i *d_adig = nullptr;
     float *d_out = nullptr;
     int *d_flag = nullptr;
@@ -624,7 +643,7 @@ void size_slots(elph_server *s) {
         phiT = std::max(phiT, gk_phi_count(c));
         if (s->precision == ELPH_MIXED) {
             bphi = std::max(bphi, gk_bphi_count(c));
-            bsl = std::max(bsl, m.KT * m.nb * m.nkc * (m.CPT / 16) * 3 * kNS * 64);
+            bsl = std::max(bsl, m.KT * m.nb * m.nkc * (m.CPT / 16) * 2 * kNS * 64);
             bscale = std::max(bscale, m.KT * m.nb * m.CPT);
         }
     }
@@ -669,8 +688,8 @@ ReqLayout req_layout(const elph_server *s, const Material &m, int64_t nk, int64_
     L.V = take(nq * nbr2 * 16);
     L.W = take(nq * m.nbr * 8);
     L.om = take(nq * m.nbr * 8);
-    L.phi = take(nq64 * m.Kp * 16);
-    L.adig = s->precision == ELPH_MIXED ? take(nq64 * m.Kp * 8) : -1;
+    L.phi = take(nq64 * m.Kp * 8);
+    L.adig = s->precision == ELPH_MIXED ? take(nq64 * m.Kp * kNS) : -1;
     L.Lq = m.c->polar ? take(nq * m.nbr * 16) : -1;
     L.lp = m.c->polar ? take(nq * m.nbr * 16) : -1;
     L.out = take(nk * nq * m.nb2 * m.nbr * 4);
@@ -700,7 +719,7 @@ bool admit(elph_server *s) {
     r.d_V = reinterpret_cast<double2 *>(b + L.V);
     r.d_W = reinterpret_cast<double *>(b + L.W);
     r.d_om = reinterpret_cast<double *>(b + L.om);
-    r.d_phi = reinterpret_cast<double2 *>(b + L.phi);
+    r.d_phi = reinterpret_cast<double *>(b + L.phi);
     r.d_adig = L.adig >= 0 ? reinterpret_cast<v4i *>(b + L.adig) : nullptr;
     r.d_Lq = L.Lq >= 0 ? reinterpret_cast<double2 *>(b + L.Lq) : nullptr;
     r.d_lp = L.lp >= 0 ? reinterpret_cast<double2 *>(b + L.lp) : nullptr;
@@ -729,8 +748,8 @@ bool admit(elph_server *s) {
         launch_polar_lp(st, (int)(nq * nbr), (int)nbr, (int)nbr, nullptr, r.d_Lq, r.d_V, r.d_lp);
     }
     const int64_t np = r.nq64 * m.Kp;
-    hipLaunchKernelGGL(k_srv_phase, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, (int)nq, (int)r.nq64,
-                       (int)m.Kp, (int)c->nwsg, r.d_qv, (int)nq, c->rq.p, r.d_phi);
+    hipLaunchKernelGGL(k_srv_phase, dim3((unsigned)((np / 2 + 255) / 256)), dim3(256), 0, st, (int)nq, (int)r.nq64,
+                       (int)m.Kp, (int)m.npair, r.d_qv, (int)nq, m.pcell.p, r.d_phi);
     if (s->precision == ELPH_MIXED) {
         const int64_t nt = (r.nq64 / 16) * m.nkc * 64;
         hipLaunchKernelGGL(k_srv_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)nq,
@@ -786,27 +805,27 @@ bool form_unit(elph_server *s, int &mat, std::vector<Segment> &segs) {
 }
 
 template <int CB>
-void launch_y_fp64(hipStream_t st, const Material &m, const double2 *h, int st0, int nst, const double2 *phi, int ldp,
+void launch_y_fp64(hipStream_t st, const Material &m, const double2 *h, int st0, int nst, const double *phi, int ldp,
                    int q0, int nqs, double2 *Y) {
     dim3 grid((unsigned)((nqs + 127) / 128), (unsigned)nst, (unsigned)(m.CPT / (16 * CB)));
-    const size_t lds = (size_t)16 * (128 + 16 * CB) * sizeof(double2);
+    const size_t lds = (size_t)16 * 128 * sizeof(double) + (size_t)16 * 16 * CB * sizeof(double2);
     hipLaunchKernelGGL(k_srv_y_fp64<CB>, grid, dim3(256), lds, st, h, (int)m.Kp, (int)m.CPT, st0, phi, ldp, q0, nqs,
                        Y);
 }
 
 template <int NB>
 void launch_rotate_all(hipStream_t st, const Material &m, int nk, const double2 *gk, const double2 *Uk, double2 *h) {
-    const int64_t per = m.Kp * m.CPT;
+    const int64_t per = (m.Kp / 2) * m.CPT;
     dim3 grid((unsigned)((per + 255) / 256), (unsigned)nk);
-    hipLaunchKernelGGL(k_rotate_all<NB>, grid, dim3(256), 0, st, (int)m.nb, (int)m.nbr, (int)m.c->nwsg, (int)m.Kp,
-                       (int)m.CPT, gk, m.gk_k, Uk, m.c->degq.p, h);
+    hipLaunchKernelGGL(k_rotate_all<NB>, grid, dim3(256), 0, st, (int)m.nb, (int)m.nbr, (int)m.npair, (int)m.Kp,
+                       (int)m.CPT, gk, m.gk_k, Uk, m.c->degq.p, m.pairs.p, h);
 }
 
 template <int CB>
 void launch_y_i8(hipStream_t st, const Material &m, const v4i *bsl, int st0, int nst, const double *bscale,
                  const v4i *adig, int qb0, int nqs, double2 *Y) {
     dim3 grid((unsigned)((nqs + 63) / 64), (unsigned)nst, (unsigned)(m.CPT / (16 * CB)));
-    const size_t lds = (size_t)2 * CB * 3 * kNS * 64 * sizeof(v4i);
+    const size_t lds = (size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i);
     hipLaunchKernelGGL(k_srv_y_i8<CB>, grid, dim3(256), lds, st, bsl, (int)m.nkc, (int)(m.CPT / 16), st0, bscale,
                        (int)m.CPT, adig, qb0, nqs, Y);
 }
@@ -837,7 +856,8 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
     else launch_rotate_all<32>(st, m, (int)nk, sl.gk.p, sl.Uk.p, sl.h.p);
     if (mixed) {
         launch_colscale(st, nst, (int)m.Kp, (int)m.CPT, sl.h.p, sl.bscale.p);
-        launch_slice_b(st, nst, (int)m.nkc, (int)(m.CPT / 16), (int)m.Kp, (int)m.CPT, sl.h.p, sl.bscale.p, sl.bsl.p);
+        launch_slice_b_ri(st, nst, (int)m.nkc, (int)(m.CPT / 16), (int)m.Kp, (int)m.CPT, sl.h.p, sl.bscale.p,
+                          sl.bsl.p);
     }
     for (auto &sg : segs) {
         const Request &r = *s->reqs.at(sg.req);
@@ -1036,7 +1056,16 @@ int elph_server_add_material(elph_server *s, elph_ctx *ctx) {
         m->nb = ctx->nb;
         m->nbr = ctx->nbr;
         m->nb2 = ctx->nb * ctx->nb;
-        m->Kp = rup(ctx->nwsg, 64);
+        {
+            const std::vector<int2> pr = cell_pairs(ctx->rcells_q.data(), ctx->nwsq, ctx->nwsg);
+            m->npair = (int64_t)pr.size();
+            std::vector<int> pc(3 * m->npair);
+            for (int64_t q = 0; q < m->npair; ++q)
+                for (int d = 0; d < 3; ++d) pc[3 * q + d] = (int)ctx->rcells_q[pr[q].x + ctx->nwsq * d];
+            m->pairs.upload(pr, ctx->stream);
+            m->pcell.upload(pc, ctx->stream);
+        }
+        m->Kp = rup(2 * m->npair, 64);
         m->nkc = m->Kp / 64;
         const int64_t cp16 = rup(m->nb * m->nbr, 16) / 16;
         m->CB = std::min<int64_t>(3, cp16);
