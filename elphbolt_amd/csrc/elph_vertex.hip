@@ -882,19 +882,36 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_ibzk_y_i8(PairA
     }
 }
 
-// Epilogue of the general path: one state and G k' rows per block,
+// General-path epilogue for nb <= 16 and nbr <= 12: the fused kernels' row
+// epilogue (ibzk_epilogue_row) on Y rows read from HBM, Y[st - st0][k'][CP].
+// Block = 4 waves x 16 rows of one state, 4 lanes per row (final-band ranks
+// lk, lk + 4, ...); NBM / NBRM as in ibzk_epilogue_row.
+template <int NBM, int NBRM>
+__global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const double2 *__restrict__ Y) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int stl = blockIdx.y, st = st0 + stl;
+    const int row = blockIdx.x * 64 + wave * 16 + li;
+    double *outrow = a.out + a.st_slot[st] * a.nprocs;
+    ibzk_epilogue_row<NBM, NBRM>(a, Y + (size_t)stl * a.nwv * a.CP, a.CP, row, row, lk, st, a.st_kint[3 * st],
+                                 a.st_kint[3 * st + 1], a.st_kint[3 * st + 2], outrow);
+}
+
+// General-path epilogue for larger nb, nbr: one state and G k' rows per block
+// (256 / G threads per row), with the row's Y, u_q and in-window rows of
+// U_k' staged in LDS:
 //   Z(rank, sp) = sum_np conj(U_k'(n_rank, np)) Y(np + nb sp)
 //   g(rank, s)  = sum_sp u_q(s, sp) Z(rank, sp) [+ lp(q,s) sum_i conj(U_k'(n,i)) U_k(m,i)]
 // written like ibzk_epilogue_row.
-__global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const double2 *__restrict__ Y, int G) {
+__global__ __launch_bounds__(256) void k_ibzk_epi_lds(PairArgs a, int st0, const double2 *__restrict__ Y, int G) {
     extern __shared__ double2 sm[];
     const int nb = a.nb, nbr = a.nbr, nbnr = nb * nbr;
     const int TPQ = 256 / G;
     const int tid = threadIdx.x, sub = tid / TPQ, tt = tid % TPQ;
     const int stl = blockIdx.y, st = st0 + stl;
     const int row = blockIdx.x * G + sub;
-    const int per = 2 * nbnr + nbr * nbr + nb;
-    double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *us = Zs + nbnr, *un = us + nbr * nbr;
+    const int per = 2 * nbnr + nbr * nbr + nb + nb * nb;
+    double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *us = Zs + nbnr, *un = us + nbr * nbr, *Vs = un + nb;
     const bool live = sub < G && row < a.nwv;
     const double2 *uq = nullptr, *lq = nullptr;
     const double *omq = nullptr;
@@ -918,13 +935,15 @@ __global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const dou
         const double2 *Yr = Y + ((size_t)stl * a.nwv + row) * a.CP;
         for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
         for (int i = tt; i < nbr * nbr; i += TPQ) us[i] = uq[i];
+        const double2 *Vr = a.Vkp + (size_t)row * nb * nb;
+        for (int i = tt; i < nn * nb; i += TPQ) Vs[i] = Vr[(size_t)a.wband[row * nb + i / nb] * nb + i % nb];
     }
     __syncthreads();
     if (live) {
         const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * nb + a.st_m[st]) * nb : nullptr;
         for (int i = tt; i < nn * nbr; i += TPQ) {
             const int rank = i % nn, sp = i / nn;
-            const double2 *V = a.Vkp + ((size_t)row * nb + a.wband[row * nb + rank]) * nb;
+            const double2 *V = Vs + rank * nb;
             double2 z = make_double2(0.0, 0.0);
             for (int np = 0; np < nb; ++np) {
                 const double2 t = cmulc(V[np], Ys[np + nb * sp]);
@@ -935,7 +954,7 @@ __global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const dou
         }
         if (lq)
             for (int rank = tt; rank < nn; rank += TPQ) {
-                const double2 *V = a.Vkp + ((size_t)row * nb + a.wband[row * nb + rank]) * nb;
+                const double2 *V = Vs + rank * nb;
                 double2 z = make_double2(0.0, 0.0);
                 for (int i = 0; i < nb; ++i) {
                     const double2 t = cmulc(V[i], Ukm[i]);
@@ -2832,8 +2851,6 @@ int64_t general_chunk(const elph_ibzk_plan *p, int64_t nst) {
 // General path: Y of a chunk of states into HBM, then the epilogue.
 void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScratch &sc, hipStream_t s) {
     const int64_t chunk = general_chunk(p, nst);
-    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
-    const size_t elds = (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb) * sizeof(double2);
     for (int64_t s0 = 0; s0 < nst; s0 += chunk) {
         const int ns = (int)std::min(chunk, nst - s0);
         if (p->precision == ELPH_MIXED) {
@@ -2855,8 +2872,18 @@ void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScrat
                 default: launch_ibzk_y_fp64<4, 2>(s, a, (int)s0, ns, sc.Y.p); break;
             }
         }
-        dim3 eg((unsigned)((p->nwv + G - 1) / G), (unsigned)ns);
-        hipLaunchKernelGGL(k_ibzk_epi, eg, dim3(256), elds, s, a, (int)s0, sc.Y.p, G);
+        dim3 eg((unsigned)((p->nwv + 63) / 64), (unsigned)ns);
+        if (p->nb <= 8 && p->nbr <= 6) {
+            hipLaunchKernelGGL((k_ibzk_epi<8, 6>), eg, dim3(256), 0, s, a, (int)s0, sc.Y.p);
+        } else if (p->nb <= 16 && p->nbr <= 12) {
+            hipLaunchKernelGGL((k_ibzk_epi<16, 12>), eg, dim3(256), 0, s, a, (int)s0, sc.Y.p);
+        } else {
+            const int G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
+            const size_t elds =
+                (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb + p->nb * p->nb) * sizeof(double2);
+            dim3 lg((unsigned)((p->nwv + G - 1) / G), (unsigned)ns);
+            hipLaunchKernelGGL(k_ibzk_epi_lds, lg, dim3(256), elds, s, a, (int)s0, sc.Y.p, G);
+        }
         HCHECK(hipGetLastError());
     }
 }
