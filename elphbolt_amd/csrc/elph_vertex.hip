@@ -1066,31 +1066,50 @@ __global__ void k_phase_digits(int nkc, int ncb, const double *__restrict__ phi,
 }
 
 // One workgroup per (sp,Rp) slice; wave w takes row blocks w, w+4, ...; all
-// NCB column blocks (16 k each).  out[k][slice][i] FP64 complex.
+// NCB column blocks (16 k each).  out[k][slice][i] FP64 complex.  The phase
+// digits (the same for every slice) are copied global -> LDS one chunk ahead
+// into two buffers shared by the 4 waves; the wave's A digits (its rows of
+// the folded gwann, streamed from HBM) are loaded one chunk ahead into the
+// second of two register sets.
 template <int NCB>
 __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig, const double *__restrict__ gscale,
                                                     int nb2, int nrb, int nkc, int nslices,
                                                     const v4i *__restrict__ bphi, int nk, double2 *__restrict__ out,
                                                     int64_t out_kstride) {
     constexpr int NS = kNS;
+    constexpr int BCH = NCB * NS * 64;  // v4i per chunk of phase digits
+    static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
+    extern __shared__ v4i bsm[];
     const int slice = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
-    for (int rb = wave; rb < nrb; rb += 4) {
+    auto copy_b = [&](int kc) {
+        const v4i *src = bphi + (size_t)kc * BCH;
+        v4i *dst = bsm + (kc & 1) * BCH;
+#pragma unroll
+        for (int q = 0; q < BCH / 256; ++q) {
+            const int r = 4 * q + wave;
+            lds_dma16(src + r * 64 + lane, dst + r * 64);
+        }
+    };
+    for (int rb0 = 0; rb0 < nrb; rb0 += 4) {
+        const int rb = rb0 + wave;
+        const bool live = rb < nrb;  // every wave joins the copies and barriers
         v4i acc[NCB][NS][2];
 #pragma unroll
         for (int c = 0; c < NCB; ++c)
 #pragma unroll
             for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
         const v4i *ga = gdig + (((size_t)slice * nrb + rb) * nkc) * 2 * NS * 64 + lane;
-        for (int kc = 0; kc < nkc; ++kc) {
-            v4i Ar[NS], Ai[NS];
+        auto load_a = [&](int kc, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
 #pragma unroll
             for (int sd = 0; sd < NS; ++sd) {
                 Ar[sd] = ga[((size_t)kc * 2 * NS + 0 * NS + sd) * 64];
                 Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
             }
-            const v4i *bb = bphi + (size_t)kc * NCB * NS * 64 + lane;
+        };
+        auto chunk = [&](int kc, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
+            const v4i *bb = bsm + (kc & 1) * BCH + lane;
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
                 v4i B[NCB];
@@ -1104,7 +1123,34 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
                         acc[c][i + j][1] = mfma_i8(Ai[i], B[c], acc[c][i + j][1]);
                     }
             }
+        };
+        // chunk kc + 1 in LDS for the whole block (no LDS stores pending: the
+        // barrier needs no fence; the clobber keeps LDS reads on their side)
+        auto next_ready = [&]() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); };
+        v4i Ar0[NS], Ai0[NS], Ar1[NS], Ai1[NS];
+        copy_b(0);
+        if (live) load_a(0, Ar0, Ai0);
+        lds_dma_wait();
+        __syncthreads();
+        for (int kc = 0;; kc += 2) {
+            if (kc + 1 < nkc) {
+                copy_b(kc + 1);
+                if (live) load_a(kc + 1, Ar1, Ai1);
+            }
+            if (live) chunk(kc, Ar0, Ai0);
+            if (kc + 1 >= nkc) break;
+            next_ready();
+            if (kc + 2 < nkc) {
+                copy_b(kc + 2);
+                if (live) load_a(kc + 2, Ar0, Ai0);
+            }
+            if (live) chunk(kc + 1, Ar1, Ai1);
+            if (kc + 2 >= nkc) break;
+            next_ready();
         }
+        lds_dma_wait();
+        __syncthreads();  // buffers free for the next row-block round
+        if (!live) continue;
         double wl[NS];
         wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
@@ -2543,7 +2589,8 @@ void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_
             const int nt = (int)(c->gnkc * (KT / 16) * 64);
             hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->gnkc,
                                (int)(KT / 16), phi, (int)KT, bphi);
-            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)nslices), dim3(256), 0, st, c->gdig.p, c->gscale.p,
+            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)nslices), dim3(256), 2 * 4 * kNS * 64 * sizeof(v4i), st,
+                               c->gdig.p, c->gscale.p,
                                (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
                                out + (size_t)k0 * out_kstride, out_kstride);
         } else {
