@@ -428,16 +428,18 @@ __global__ __launch_bounds__(256) void k_srv_epi(EpiArgs a) {
     const int nb = a.nb, nbr = a.nbr, nbnr = nb * nbr;
     const int G = a.G, TPQ = 256 / G;
     const int tid = threadIdx.x, sub = tid / TPQ, tt = tid % TPQ;
-    const int st = a.st0 + blockIdx.y;  // state within the segment
+    // grid (states, q-row groups): consecutive blocks are the states (k, m)
+    // of one q-row group, which share U_k+q and u_q while they are in L2
+    const int st = a.st0 + blockIdx.x;  // state within the segment
     const int kseg = st / nb, m = st % nb;
-    const int r = blockIdx.x * G + sub;
+    const int r = blockIdx.y * G + sub;
     const int per = 3 * nbnr + nb * nb + nbr * nbr;
     double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *Us = Zs + nbnr, *us = Us + nb * nb, *un = us + nbr * nbr;
     const bool live = sub < G && r < a.nqs;
     const int q = a.q0 + r;
     const double2 *Ukm = a.Uk + ((size_t)(a.k_unit0 + kseg) * nb + m) * nb;
     if (live) {
-        const double2 *Yr = a.Y + ((size_t)blockIdx.y * a.nqs + r) * a.CPT;
+        const double2 *Yr = a.Y + ((size_t)blockIdx.x * a.nqs + r) * a.CPT;
         const double2 *Ur = a.Ukq + ((size_t)kseg * a.nqs + r) * nb * nb;
         const double2 *ur = a.u + (size_t)q * nbr * nbr;
         for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
@@ -909,7 +911,7 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
             a.out = r.d_out;
             a.G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (nb * nbr)));
             const size_t lds = (size_t)a.G * (3 * nb * nbr + nb * nb + nbr * nbr) * sizeof(double2);
-            dim3 grid((unsigned)((sg.nqs + a.G - 1) / a.G), (unsigned)ns);
+            dim3 grid((unsigned)ns, (unsigned)((sg.nqs + a.G - 1) / a.G));
             hipLaunchKernelGGL(k_srv_epi, grid, dim3(256), lds, st, a);
         }
         HCHECK(hipGetLastError());
