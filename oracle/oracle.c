@@ -13,6 +13,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1149,4 +1150,274 @@ int orc_ibzq_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons
         free(gq);
     }
     return 0;
+}
+
+/* ======================================================================
+ * Consumers of g2: tetrahedron delta function, transition probabilities
+ * X+/- and the electron RTA scattering rates.
+ * ====================================================================== */
+
+/* delta.f90:494-605 form_tetrahedra_3d.  tetra (6 nk, 4), tetracount (nk),
+ * tetramap (2, nk, 24), column-major, 1-based indices; with blocks != 0 the
+ * wave vectors are indexlist(1:nk) (muxed, sorted) and a vertex outside the
+ * list is stored as -1 (binsearch's miss value). */
+void orc_form_tetrahedra_3d(int64_t nk, const int64_t mesh[3], int blocks, const int64_t *indexlist, int64_t *tetra,
+                            int64_t *tetracount, int64_t *tetramap) {
+    static const int64_t labels[6][4] = {{1, 2, 3, 6}, {1, 3, 5, 6}, {3, 5, 6, 7},
+                                         {3, 6, 7, 8}, {3, 4, 6, 8}, {2, 3, 4, 6}};
+    const int64_t ntet = 6 * nk;
+    memset(tetra, 0, sizeof(int64_t) * ntet * 4);
+    memset(tetracount, 0, sizeof(int64_t) * nk);
+    memset(tetramap, 0, sizeof(int64_t) * 2 * nk * 24);
+    int64_t count = 1;
+    for (int64_t ik = 1; ik <= nk; ++ik) {
+        int64_t ijk[3];
+        orc_demux_vector(blocks ? indexlist[ik - 1] : ik, ijk, mesh, 1);
+        const int64_t i = ijk[0], j = ijk[1], k = ijk[2];
+        const int64_t ip1 = i == mesh[0] ? 1 : i + 1;
+        const int64_t jp1 = j == mesh[1] ? 1 : j + 1;
+        const int64_t kp1 = k == mesh[2] ? 1 : k + 1;
+        const int64_t sc[8][3] = {{i, j, k},     {ip1, j, k},     {i, jp1, k},     {ip1, jp1, k},
+                                  {i, j, kp1},   {ip1, j, kp1},   {i, jp1, kp1},   {ip1, jp1, kp1}};
+        for (int tk = 0; tk < 6; ++tk) {
+            for (int tl = 1; tl <= 4; ++tl) {
+                const int64_t *v = sc[labels[tk][tl - 1] - 1];
+                const int64_t aux = orc_mux_vector(v, mesh, 1);
+                int64_t tmp = aux;
+                if (blocks) tmp = orc_binsearch(indexlist, nk, aux);
+                tetra[(count - 1) + ntet * (tl - 1)] = tmp;
+                if (tmp > 0) {
+                    const int64_t c = ++tetracount[tmp - 1];
+                    tetramap[0 + 2 * ((tmp - 1) + nk * (c - 1))] = count;
+                    tetramap[1 + 2 * ((tmp - 1) + nk * (c - 1))] = tl;
+                }
+            }
+            count++;
+        }
+    }
+}
+
+/* delta.f90:607-646 fill_tetrahedra_3d: tetra_evals (ntet, nb, 4), the band
+ * energies evals (nk, nb) at the four vertices (0 for a vertex outside the
+ * list), sorted ascending over the vertices. */
+void orc_fill_tetrahedra_3d(int64_t ntet, const int64_t *tetra, int64_t nk, int64_t nb, const double *evals,
+                            double *tetra_evals) {
+    memset(tetra_evals, 0, sizeof(double) * ntet * nb * 4);
+    for (int64_t it = 0; it < ntet; ++it)
+        for (int iv = 0; iv < 4; ++iv) {
+            const int64_t aux = tetra[it + ntet * iv];
+            if (aux > 0)
+                for (int64_t ib = 0; ib < nb; ++ib) tetra_evals[it + ntet * (ib + nb * iv)] = evals[(aux - 1) + nk * ib];
+        }
+    for (int64_t it = 0; it < ntet; ++it)
+        for (int64_t ib = 0; ib < nb; ++ib) {
+            double *e[4];
+            for (int iv = 0; iv < 4; ++iv) e[iv] = &tetra_evals[it + ntet * (ib + nb * iv)];
+            /* misc.f90:716-736 sort_real (exchange sort, ascending) */
+            for (int a = 0; a < 4; ++a) {
+                double aux = *e[a];
+                for (int b = a + 1; b < 4; ++b)
+                    if (aux > *e[b]) {
+                        const double t = *e[b];
+                        *e[b] = aux;
+                        *e[a] = t;
+                        aux = t;
+                    }
+            }
+        }
+}
+
+/* delta.f90:61-225 delta_fn_tetra, the vertex weight of the linear
+ * tetrahedron method for sample energy e at wave vector ik (1-based, of the
+ * list the tetrahedra were formed on) and band ib.  The case selector iv is
+ * the wave vector's position in the tetrahedron as formed (tetramap(2,:,:)),
+ * the energies e1 <= e2 <= e3 <= e4 are the sorted vertex energies: both as
+ * the reference has them.  numtetra = 6 product(mesh). */
+double orc_delta_fn_tetra(double e, int64_t ik, int64_t ib, const int64_t mesh[3], int64_t nk, int64_t ntet,
+                          int64_t nb, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals) {
+    double tmp = 0.0, d = 0.0;
+    const int64_t numtetra = mesh[0] * mesh[1] * mesh[2] * 6;
+    const int64_t num = tetracount[ik - 1];
+    for (int64_t itk = 1; itk <= num; ++itk) {
+        const int64_t it = tetramap[0 + 2 * ((ik - 1) + nk * (itk - 1))];
+        const int64_t iv = tetramap[1 + 2 * ((ik - 1) + nk * (itk - 1))];
+        const double e1 = tetra_evals[(it - 1) + ntet * ((ib - 1) + nb * 0)];
+        const double e2 = tetra_evals[(it - 1) + ntet * ((ib - 1) + nb * 1)];
+        const double e3 = tetra_evals[(it - 1) + ntet * ((ib - 1) + nb * 2)];
+        const double e4 = tetra_evals[(it - 1) + ntet * ((ib - 1) + nb * 3)];
+        const double e1e = e1 - e, e2e = e2 - e, e3e = e3 - e, e4e = e4 - e;
+        const double e21 = e2 - e1, e31 = e3 - e1, e41 = e4 - e1, e32 = e3 - e2, e42 = e4 - e2, e43 = e4 - e3;
+        const int c1 = e1 <= e && e <= e2, c2 = e2 <= e && e <= e3, c3 = e3 <= e && e <= e4;
+        if (e < e1 || e > e4) continue;
+        switch (iv) {
+        case 1:
+            if (c1) {
+                tmp = (e2e / e21 + e3e / e31 + e4e / e41) * (e1e * e1e) / e41 / e31 / e21;
+                if (e1 == e2) tmp = 0.0;
+            } else if (c2) {
+                tmp = -0.5 * (e3e / (e31 * e31) * (e3e * e2e / e42 / e32 + e4e * e1e / e41 / e42 + e3e * e1e / e32 / e41) +
+                              e4e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e * e2e / e42 / e32 + e3e * e1e / e31 / e32));
+                if (e2 == e3)
+                    tmp = -0.5 * (e4e * e1e / e41 / e42 + e1e / e41 +
+                                  e4e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e / e42 + e1e / e31));
+            } else if (c3) {
+                tmp = (e4e * e4e * e4e) / (e41 * e41) / e42 / e43;
+                if (e3 == e4) tmp = (e4e * e4e) / (e41 * e41) / e42;
+            }
+            break;
+        case 2:
+            if (c1) {
+                tmp = -(e1e * e1e * e1e) / (e21 * e21) / e31 / e41;
+                if (e1 == e2) tmp = 0.0;
+            } else if (c2) {
+                tmp = -0.5 * (e3e / (e32 * e32) * (e3e * e2e / e42 / e31 + e4e * e2e / e42 / e41 + e3e * e1e / e31 / e41) +
+                              e4e / (e42 * e42) * (e3e * e2e / e32 / e31 + e4e * e1e / e41 / e31 + e4e * e2e / e32 / e41));
+                if (e2 == e3) tmp = -0.5 * (0.0 + e4e / e42 / e41 + 0.0 + e4e / (e42 * e42) * (0.0 + e4e * e1e / e41 / e31 + 1.0));
+            } else if (c3) {
+                tmp = (e4e * e4e * e4e) / e41 / (e42 * e42) / e43;
+                if (e3 == e4) tmp = 0.0;
+            }
+            break;
+        case 3:
+            if (c1) {
+                tmp = -(e1e * e1e * e1e) / e21 / (e31 * e31) / e41;
+                if (e1 == e2) tmp = 0.0;
+            } else if (c2) {
+                tmp = 0.5 * (e2e / (e32 * e32) * (e3e * e2e / e42 / e31 + e4e * e2e / e42 / e41 + e3e * e1e / e31 / e41) +
+                             e1e / (e31 * e31) * (e3e * e2e / e42 / e32 + e4e * e1e / e41 / e42 + e3e * e1e / e32 / e41));
+                if (e2 == e3)
+                    tmp = 0.5 * (0.0 + e4e / e42 / e41 + e1e / e31 / e41 +
+                                 e1e / (e31 * e31) * (0.0 + e4e * e1e / e41 / e42 + e1e / e41));
+            } else if (c3) {
+                tmp = (e4e * e4e * e4e) / e41 / e42 / (e43 * e43);
+                if (e3 == e4) tmp = 0.0;
+            }
+            break;
+        case 4:
+            if (c1) {
+                tmp = -(e1e * e1e * e1e) / e21 / e31 / (e41 * e41);
+                if (e1 == e2) tmp = 0.0;
+            } else if (c2) {
+                tmp = 0.5 * (e2e / (e42 * e42) * (e3e * e2e / e32 / e31 + e4e * e1e / e41 / e31 + e4e * e2e / e32 / e41) +
+                             e1e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e * e2e / e42 / e32 + e3e * e1e / e31 / e32));
+                if (e2 == e3) tmp = 0.5 * (0.0 + e1e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e / e42 + e1e / e31));
+            } else if (c3) {
+                tmp = -(e3e / e43 + e2e / e42 + e1e / e41) * (e4e * e4e) / e41 / e42 / e43;
+                if (e3 == e4) tmp = 0.0;
+            }
+            break;
+        }
+        if (e1 == e2 && e1 == e3 && e1 == e4 && e == e1) tmp = 0.25;
+        d += tmp;
+    }
+    if (d < 1.0e-12) d = 0.0;
+    return d / (double)numtetra;
+}
+
+/* misc.f90:992-1019 Bose (with the quad-precision expm1) and Fermi; kB from
+ * params.f90:39. */
+static const double KB_EV = 1.380649e-23 / 1.602176634e-19;
+double orc_bose(double e, double T) {
+    const __float128 x = (__float128)(e / KB_EV / T);
+    const double em1 = (double)(expq(x) - (__float128)1.0);
+    return 1.0 / em1;
+}
+double orc_fermi(double e, double chempot, double T) { return 1.0 / (exp((e - chempot) / KB_EV / T) + 1.0); }
+
+/* interactions.f90:2190-2218, 2228-2251, 2278-2332, 2351-2354
+ * calculate_eph_interaction_ibzk, key 'X': for IBZ states
+ * istate_first..istate_last with in-window counts (count -1 = outside the
+ * window: rows untouched), the transition probabilities (THz)
+ *   X+ = 2pi/hbar g2 (n_B + f(e + w)) delta(e + w; k', n),
+ *   X- = 2pi/hbar g2 (1 + n_B - f(e - w)) delta(e - w; k', n),
+ * zero for phonon energies below 0.5 meV, and the final electron and
+ * phonon state of each process (fine-mesh phonons negative).  g2 rows have
+ * stride nprocs = nstates_inwindow nbr, as orc_ibzk_g2 writes them. */
+int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+               int64_t istate_first, int64_t istate_last, const int64_t *counts, const double *g2, double T,
+               double chempot, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals,
+               double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph, int nthreads) {
+    const int64_t nb = w->nb, nbr = w->nbr, nirr = el->nwv_irred, nwv = el->nwv;
+    const int64_t nprocs = el->nstates_inwindow * nbr, ntet = 6 * nwv;
+    const double cnst = TWOPI_ / (HBAR / QE);
+    double rec[9], vol;
+    orc_reclattvecs(w->lattvecs, rec, &vol);
+    const int64_t nst = istate_last - istate_first + 1;
+    int status = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        double phe[64];
+        zdouble phv[64 * 64];
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < nst; ++t) {
+            if (counts[t] < 0) continue;
+            const int64_t istate = istate_first + t;
+            int64_t m, ik;
+            orc_demux_state(istate, nb, &m, &ik);
+            const double en_el = el->ens_irred[(ik - 1) + nirr * (m - 1)];
+            orc_vec kv, kpv, qv, qc;
+            orc_vec_create(el->indexlist_irred[ik - 1], el->wvmesh, rec, &kv);
+            const double *g = &g2[t * nprocs];
+            double *xp = &Xplus[t * nprocs], *xm = &Xminus[t * nprocs];
+            int64_t *se = &istate_el[t * nprocs], *sp = &istate_ph[t * nprocs];
+            int64_t count = 0;
+            for (int64_t ikp = 1; ikp <= nwv; ++ikp) {
+                orc_vec_create(el->indexlist[ikp - 1], el->wvmesh, rec, &kpv);
+                orc_vec_sub(&kpv, &kv, el->wvmesh, rec, &qv);
+                int need_fine = 0;
+                for (int c = 0; c < 3; ++c)
+                    if (qv.ints[c] % mesh_ref != 0) need_fine = 1;
+                int64_t iqc = 0;
+                if (need_fine) {
+                    double ql[3] = {qv.frac[0], qv.frac[1], qv.frac[2]};
+                    if (orc_ph_wann(w, 1, ql, phe, phv, NULL) == 2) status = 2;
+                } else {
+                    orc_vec_change_grid(&qv, ph->wvmesh, &qc);
+                    iqc = qc.muxed;
+                }
+                for (int64_t n = 1; n <= nb; ++n) {
+                    if (fabs(el->ens[(ikp - 1) + nwv * (n - 1)] - el->enref) > el->fsthick) continue;
+                    for (int64_t s = 1; s <= nbr; ++s) {
+                        count++;
+                        if (count > nprocs) { status = 5; continue; }
+                        const double en_ph = need_fine ? phe[s - 1] : ph->ens[(iqc - 1) + ph->nwv * (s - 1)];
+                        const double bosefac = en_ph != 0.0 ? orc_bose(en_ph, T) : 0.0;
+                        const double fp = orc_fermi(en_el + en_ph, chempot, T);
+                        const double fm = orc_fermi(en_el - en_ph, chempot, T);
+                        double delta = orc_delta_fn_tetra(en_el + en_ph, ikp, n, el->wvmesh, nwv, ntet, nb, tetramap,
+                                                          tetracount, tetra_evals);
+                        double x = 0.0;
+                        if (en_ph >= 0.5e-3) x = g[count - 1] * (bosefac + fp) * delta;
+                        xp[count - 1] = cnst * x;
+                        delta = orc_delta_fn_tetra(en_el - en_ph, ikp, n, el->wvmesh, nwv, ntet, nb, tetramap,
+                                                   tetracount, tetra_evals);
+                        x = 0.0;
+                        if (en_ph >= 0.5e-3) x = g[count - 1] * (1.0 + bosefac - fm) * delta;
+                        xm[count - 1] = cnst * x;
+                        se[count - 1] = orc_mux_state(nb, n, ikp);
+                        sp[count - 1] = need_fine ? -orc_mux_state(nbr, s, qv.muxed) : orc_mux_state(nbr, s, iqc);
+                    }
+                }
+            }
+            if (count != counts[t]) status = 6;
+        }
+    }
+    return status;
+}
+
+/* interactions.f90:2957-3063 calculate_el_rta_rates, e-ph channel: the
+ * scattering rate of each in-window IBZ state is sum X+ + sum X- over its
+ * processes (THz), each sum in process order.  rates has one entry per
+ * state of istate_first..istate_last (0 outside the window). */
+void orc_el_rta_rates(int64_t nst, const int64_t *counts, int64_t nprocs, const double *Xplus, const double *Xminus,
+                      double *rates) {
+    for (int64_t t = 0; t < nst; ++t) {
+        double r = 0.0;
+        for (int64_t i = 0; i < counts[t]; ++i) r = r + Xplus[t * nprocs + i];
+        for (int64_t i = 0; i < counts[t]; ++i) r = r + Xminus[t * nprocs + i];
+        rates[t] = r;
+    }
 }

@@ -127,6 +127,22 @@ int orc_ibzq_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons
                 int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2,
                 int nthreads);
 
+/* ---- consumers of g2: delta.f90, interactions.f90 key 'X', RTA rates ---- */
+void orc_form_tetrahedra_3d(int64_t nk, const int64_t mesh[3], int blocks, const int64_t *indexlist, int64_t *tetra,
+                            int64_t *tetracount, int64_t *tetramap);
+void orc_fill_tetrahedra_3d(int64_t ntet, const int64_t *tetra, int64_t nk, int64_t nb, const double *evals,
+                            double *tetra_evals);
+double orc_delta_fn_tetra(double e, int64_t ik, int64_t ib, const int64_t mesh[3], int64_t nk, int64_t ntet,
+                          int64_t nb, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals);
+double orc_bose(double e, double T);
+double orc_fermi(double e, double chempot, double T);
+int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+               int64_t istate_first, int64_t istate_last, const int64_t *counts, const double *g2, double T,
+               double chempot, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals,
+               double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph, int nthreads);
+void orc_el_rta_rates(int64_t nst, const int64_t *counts, int64_t nprocs, const double *Xplus, const double *Xminus,
+                      double *rates);
+
 #ifdef __cplusplus
 }
 #endif

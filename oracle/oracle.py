@@ -69,6 +69,9 @@ def lib():
         L.orc_g2.restype = ct.c_double
         L.orc_qrotations.restype = ct.c_int64
         L.orc_find_irred_wedge.restype = ct.c_int64
+        L.orc_delta_fn_tetra.restype = ct.c_double
+        L.orc_bose.restype = ct.c_double
+        L.orc_fermi.restype = ct.c_double
         _lib = L
     return _lib
 
@@ -364,6 +367,86 @@ def ibzq_g2(w: Wannier, el, ph, istate_first=1, istate_last=None, nthreads=0):
                            ct.c_int64(istate_last), _p(counts, c_i64p), _p(out), ct.c_int(nthreads))
     assert st == 0, st
     return counts, out
+
+
+# ---- consumers of g2 (delta.f90, interactions.f90 key 'X', RTA rates) ----
+class Tetrahedra:
+    """form_tetrahedra_3d + fill_tetrahedra_3d (delta.f90:494-646) over a wave
+    vector list (blocks: muxed sorted indexlist, else the full mesh) with band
+    energies evals (nk, nb).  Arrays in the reference's column-major shapes:
+    tetra (6 nk, 4), tetracount (nk), tetramap (2, nk, 24), evals (6 nk, nb, 4)."""
+
+    def __init__(self, mesh, evals, indexlist=None):
+        mesh = _i64(mesh)
+        evals = np.asfortranarray(np.asarray(evals, np.float64))
+        nk, nb = evals.shape
+        blocks = indexlist is not None
+        il = _i64(indexlist if blocks else np.zeros(1))
+        self.mesh, self.nk, self.nb, self.ntet = mesh, nk, nb, 6 * nk
+        self.tetra = np.zeros((6 * nk, 4), np.int64, order="F")
+        self.tetracount = np.zeros(nk, np.int64)
+        self.tetramap = np.zeros((2, nk, 24), np.int64, order="F")
+        lib().orc_form_tetrahedra_3d(ct.c_int64(nk), _p(mesh, c_i64p), ct.c_int(int(blocks)), _p(il, c_i64p),
+                                     _p(self.tetra, c_i64p), _p(self.tetracount, c_i64p),
+                                     _p(self.tetramap, c_i64p))
+        self.evals = np.zeros((6 * nk, nb, 4), order="F")
+        lib().orc_fill_tetrahedra_3d(ct.c_int64(6 * nk), _p(self.tetra, c_i64p), ct.c_int64(nk), ct.c_int64(nb),
+                                     _p(evals), _p(self.evals))
+
+    def delta(self, e, ik, ib):
+        """delta_fn_tetra(e, ik, ib) (delta.f90:61-225), ik/ib 1-based."""
+        return lib().orc_delta_fn_tetra(ct.c_double(e), ct.c_int64(ik), ct.c_int64(ib), _p(self.mesh, c_i64p),
+                                        ct.c_int64(self.nk), ct.c_int64(self.ntet), ct.c_int64(self.nb),
+                                        _p(self.tetramap, c_i64p), _p(self.tetracount, c_i64p), _p(self.evals))
+
+
+def delta_fn_tetra_raw(e, ik, ib, mesh, tetramap, tetracount, tetra_evals):
+    """delta_fn_tetra on caller-made tables (shapes as in Tetrahedra)."""
+    mesh = _i64(mesh)
+    tm, tc, te = _i64(tetramap), _i64(tetracount), _f64(tetra_evals)
+    nk, ntet, nb = tm.shape[1], te.shape[0], te.shape[1]
+    return lib().orc_delta_fn_tetra(ct.c_double(e), ct.c_int64(ik), ct.c_int64(ib), _p(mesh, c_i64p),
+                                    ct.c_int64(nk), ct.c_int64(ntet), ct.c_int64(nb), _p(tm, c_i64p),
+                                    _p(tc, c_i64p), _p(te))
+
+
+def bose(e, T):
+    return lib().orc_bose(ct.c_double(e), ct.c_double(T))
+
+
+def fermi(e, chempot, T):
+    return lib().orc_fermi(ct.c_double(e), ct.c_double(chempot), ct.c_double(T))
+
+
+def ibzk_X(w: Wannier, el, ph, tet: Tetrahedra, counts, g2, T, chempot, mesh_ref=1, istate_first=1, nthreads=0):
+    """calculate_eph_interaction_ibzk('X') on g2 rows (counts, g2 as ibzk_g2
+    returns them).  Returns (Xplus, Xminus, istate_el, istate_ph), rows of
+    length nstates_inwindow*nbr (entries past each count are 0)."""
+    counts = _i64(counts)
+    g2 = np.ascontiguousarray(g2, np.float64)
+    nst, nprocs = g2.shape
+    xp, xm = np.zeros_like(g2), np.zeros_like(g2)
+    se, sp = np.zeros(g2.shape, np.int64), np.zeros(g2.shape, np.int64)
+    es, k1 = _electrons_struct(w, el)
+    ps, k2 = _phonons_struct(ph)
+    st = lib().orc_ibzk_X(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref), ct.c_int64(istate_first),
+                          ct.c_int64(istate_first + nst - 1), _p(counts, c_i64p), _p(g2), ct.c_double(T),
+                          ct.c_double(chempot), _p(tet.tetramap, c_i64p), _p(tet.tetracount, c_i64p), _p(tet.evals),
+                          _p(xp), _p(xm), _p(se, c_i64p), _p(sp, c_i64p), ct.c_int(nthreads))
+    if st == 2:
+        raise RuntimeError("Large negative phonon energy found! Stopping!")
+    assert st == 0, st
+    return xp, xm, se, sp
+
+
+def el_rta_rates(counts, Xplus, Xminus):
+    """calculate_el_rta_rates, e-ph channel (interactions.f90:2957-3063)."""
+    counts = _i64(counts)
+    xp, xm = np.ascontiguousarray(Xplus, np.float64), np.ascontiguousarray(Xminus, np.float64)
+    out = np.zeros(len(counts))
+    lib().orc_el_rta_rates(ct.c_int64(len(counts)), _p(counts, c_i64p), ct.c_int64(xp.shape[1]), _p(xp), _p(xm),
+                           _p(out))
+    return out
 
 
 # ---- index algebra (misc.f90, vector.f90) ----

@@ -1,0 +1,227 @@
+"""Pins of the oracle's g2 consumers: the tetrahedra of delta.f90:494-646, the
+tetrahedron delta function (delta.f90:61-225), Bose/Fermi (misc.f90:992-1019),
+the key-'X' driver (interactions.f90:2278-2354) and the RTA rates
+(interactions.f90:2957-3063).
+
+What fixes each one independently of the oracle's own formulas:
+  * the six tetrahedra of a subcell tile it: each has volume 1/6 and every
+    point of the cube lies in exactly one of them;
+  * the vertex weights of one tetrahedron are the energy densities of the
+    barycentric coordinates, so their moments int e^p w_i(e) de equal the
+    closed-form simplex integrals of eps(x)^p lambda_i(x) (Gauss-Legendre on
+    each piece is exact for these piecewise cubics);
+  * on a full mesh int sum_k delta(e, k) de = 1 and int delta(e, k) de = 1/nk;
+  * Bose and Fermi against numpy's expm1 and exp;
+  * X+/- and the process maps against a Python loop over the same
+    primitives, the rates against numpy sums.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from elphbolt_amd import synth
+from oracle import oracle as O
+
+KB = 1.380649e-23 / 1.602176634e-19
+HBAR_EVPS = 1.05457172647e-22 / 1.602176634e-19
+
+def _offsets(tet, mesh, c):
+    """Vertex offsets (0/1 per axis) of tetrahedron c of a full-mesh table."""
+    mesh = np.asarray(mesh)
+    ik = c // 6 + 1
+    base = np.array(O.demux_vector(ik, mesh, 1))
+    out = []
+    for iv in range(4):
+        v = np.array(O.demux_vector(int(tet.tetra[c, iv]), mesh, 1))
+        out.append((v - base) % mesh)
+    return np.array(out)
+
+
+def test_tetrahedra_tile_the_subcell():
+    mesh = [3, 4, 5]
+    nk = int(np.prod(mesh))
+    tet = O.Tetrahedra(mesh, np.zeros((nk, 1)))
+    assert np.all(tet.tetracount == 24)
+    assert np.all(tet.tetra > 0)
+    rng = np.random.default_rng(0)
+    pts = rng.random((4000, 3))
+    for ik in (0, 17, nk - 1):
+        inside = np.zeros(len(pts), int)
+        for tk in range(6):
+            off = _offsets(tet, mesh, 6 * ik + tk).astype(float)
+            assert np.all((off == 0) | (off == 1))
+            M = (off[1:] - off[0]).T
+            assert abs(abs(np.linalg.det(M)) - 1.0) < 1e-12  # volume 1/6 of the cube
+            lam = np.linalg.solve(M, (pts - off[0]).T)
+            bary = np.vstack([1 - lam.sum(0), lam])
+            inside += np.all(bary >= -1e-12, axis=0)
+        assert np.all(inside == 1)
+
+
+def test_tetramap_consistent_and_blocks():
+    mesh = [4, 3, 4]
+    nk = int(np.prod(mesh))
+    rng = np.random.default_rng(1)
+    il = np.sort(rng.choice(np.arange(1, nk + 1), size=29, replace=False))
+    full = O.Tetrahedra(mesh, np.zeros((nk, 1)))
+    blk = O.Tetrahedra(mesh, np.zeros((len(il), 1)), indexlist=il)
+    # vertices of the blocks table are positions in il (or -1), those of the
+    # full table muxed indices: the tetrahedra of a listed base point agree
+    for p, muxed in enumerate(il):
+        for tk in range(6):
+            rf, rb = full.tetra[6 * (muxed - 1) + tk], blk.tetra[6 * p + tk]
+            expect = [int(np.searchsorted(il, v)) + 1 if v in il else -1 for v in rf]
+            assert list(rb) == expect
+    # tetramap inverts tetra, tetracount counts the appearances
+    for t in (full, blk):
+        npos = np.bincount(t.tetra[t.tetra > 0] - 1, minlength=t.nk)
+        assert np.array_equal(npos, t.tetracount)
+        for ik in range(t.nk):
+            for j in range(t.tetracount[ik]):
+                it, iv = t.tetramap[0, ik, j], t.tetramap[1, ik, j]
+                assert t.tetra[it - 1, iv - 1] == ik + 1
+
+
+def _simplex_moment(es, i, p):
+    """int over a tetrahedron of eps(x)^p lambda_i(x) / volume, eps linear with
+    vertex values es: sum over vertex multi-indices of prod e_j times
+    3! a!b!c!d! / (a+b+c+d+3)!."""
+    tot = 0.0
+    for idx in np.ndindex(*([4] * p)):
+        expo = [0, 0, 0, 0]
+        expo[i] += 1
+        w = 1.0
+        for j in idx:
+            expo[j] += 1
+            w *= es[j]
+        tot += w * 6.0 * np.prod([math.factorial(a) for a in expo]) / math.factorial(sum(expo) + 3)
+    return tot
+
+
+def _single_tetra_weight(e, es, iv):
+    """delta_fn_tetra of a one-tetrahedron table (mesh 1x1x1, numtetra = 6)."""
+    tm = np.zeros((2, 1, 24), np.int64, order="F")
+    tm[0, 0, 0], tm[1, 0, 0] = 1, iv
+    te = np.asfortranarray(np.array(es, float).reshape(1, 1, 4))
+    return 6.0 * O.delta_fn_tetra_raw(e, 1, 1, [1, 1, 1], tm, np.array([1]), te)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_vertex_weight_moments(seed):
+    rng = np.random.default_rng(seed)
+    es = np.sort(rng.random(4) * 2 - 1)
+    xg, wg = np.polynomial.legendre.leggauss(6)
+    for iv in range(1, 5):
+        mom = np.zeros(4)
+        for a, b in ((es[0], es[1]), (es[1], es[2]), (es[2], es[3])):
+            for x, wq in zip(xg, wg):
+                e = 0.5 * (b - a) * x + 0.5 * (a + b)
+                w = _single_tetra_weight(e, es, iv)
+                mom += 0.5 * (b - a) * wq * w * e ** np.arange(4)
+        ref = [_simplex_moment(es, iv - 1, p) for p in range(4)]
+        assert np.allclose(mom, ref, rtol=1e-10, atol=1e-12), (iv, mom, ref)
+        # zero at the ends, continuous at the inner breakpoints, never negative
+        assert _single_tetra_weight(es[0], es, iv) == 0.0
+        assert _single_tetra_weight(es[3], es, iv) == 0.0
+        for eb in es[1:3]:
+            lo, hi = _single_tetra_weight(eb - 1e-9, es, iv), _single_tetra_weight(eb + 1e-9, es, iv)
+            assert abs(lo - hi) < 1e-6 * max(1.0, abs(lo))
+        grid = np.linspace(es[0], es[3], 101)
+        assert min(_single_tetra_weight(e, es, iv) for e in grid) >= 0.0
+
+
+def test_degenerate_tetrahedron():
+    # all four energies equal: weight 1/4 at that energy (delta.f90:212-214)
+    assert _single_tetra_weight(0.3, [0.3] * 4, 2) == pytest.approx(0.25, rel=1e-15)
+    assert _single_tetra_weight(0.31, [0.3] * 4, 2) == 0.0
+
+
+def test_dos_normalization_full_mesh():
+    mesh = [4, 4, 3]
+    nk = int(np.prod(mesh))
+    ijk = np.array([O.demux_vector(i, mesh, 0) for i in range(1, nk + 1)], float) / np.array(mesh)
+    ens = (np.cos(2 * np.pi * ijk[:, 0]) + 0.7 * np.cos(2 * np.pi * ijk[:, 1]) * np.sin(2 * np.pi * ijk[:, 2])
+           + 0.3 * np.sin(2 * np.pi * (ijk[:, 0] + ijk[:, 2])))[:, None]
+    tet = O.Tetrahedra(mesh, ens)
+    bps = np.unique(ens)
+    xg, wg = np.polynomial.legendre.leggauss(4)
+    per_k = np.zeros(nk)
+    for a, b in zip(bps[:-1], bps[1:]):
+        for x, wq in zip(xg, wg):
+            e = 0.5 * (b - a) * x + 0.5 * (a + b)
+            per_k += 0.5 * (b - a) * wq * np.array([tet.delta(e, ik, 1) for ik in range(1, nk + 1)])
+    assert abs(per_k.sum() - 1.0) < 1e-10
+    assert np.allclose(per_k, 1.0 / nk, rtol=1e-9)
+
+
+def test_bose_fermi():
+    for T in (77.0, 300.0, 1000.0):
+        for e in (1e-4, 0.01, 0.05, 0.3):
+            assert O.bose(e, T) == pytest.approx(1.0 / np.expm1(e / KB / T), rel=1e-14)
+        for e in (-0.4, -0.02, 0.0, 0.01, 0.5):
+            assert O.fermi(e, 0.05, T) == pytest.approx(1.0 / (np.exp((e - 0.05) / KB / T) + 1.0), rel=1e-14)
+
+
+def _x_by_loops(mat, o, el, ph, tet, counts, g2, T, mu, mesh_ref):
+    """X+/- and the process maps by Python loops over the oracle's primitives
+    (interactions.f90:2228-2332 loop order)."""
+    nb, nbr = mat.nb, mat.nbr
+    mesh = np.array(el.wvmesh)
+    xp, xm = np.zeros_like(g2), np.zeros_like(g2)
+    se, sp = np.zeros(g2.shape, np.int64), np.zeros(g2.shape, np.int64)
+    const = 2 * np.pi / HBAR_EVPS
+    for t in range(len(counts)):
+        if counts[t] < 0:
+            continue
+        m, ik = O.demux_state(t + 1, nb)
+        e_el = el.ens_irred[ik - 1, m - 1]
+        kint = np.array(O.demux_vector(el.indexlist_irred[ik - 1], mesh, 0))
+        c = 0
+        for ikp in range(1, el.nwv + 1):
+            kpint = np.array(O.demux_vector(el.indexlist[ikp - 1], mesh, 0))
+            qint = (kpint - kint) % mesh
+            fine = bool(np.any(qint % mesh_ref))
+            if fine:
+                pe, _, _ = o.ph_wann((qint / mesh)[None, :])
+                pe = np.asarray(pe).reshape(-1)
+                iq = O.mux_vector(qint, mesh, 0)
+            else:
+                iq = O.mux_vector(qint // mesh_ref, ph.wvmesh, 0)
+                pe = ph.ens[iq - 1]
+            for n in range(1, nb + 1):
+                if abs(el.ens[ikp - 1, n - 1] - el.enref) > el.fsthick:
+                    continue
+                for s in range(1, nbr + 1):
+                    w = pe[s - 1]
+                    nB = O.bose(w, T) if w != 0 else 0.0
+                    if w >= 0.5e-3:
+                        xp[t, c] = const * g2[t, c] * (nB + O.fermi(e_el + w, mu, T)) * tet.delta(e_el + w, ikp, n)
+                        xm[t, c] = const * g2[t, c] * (1 + nB - O.fermi(e_el - w, mu, T)) * tet.delta(e_el - w, ikp, n)
+                    se[t, c] = (ikp - 1) * nb + n
+                    sp[t, c] = -((iq - 1) * nbr + s) if fine else (iq - 1) * nbr + s
+                    c += 1
+        assert c == counts[t]
+    return xp, xm, se, sp
+
+
+@pytest.mark.parametrize("mesh_ref", [1, 2])
+def test_ibzk_X_against_loops(mesh_ref):
+    mat = synth.tiny(nb=3, natoms=1, ws=3, wv=4, seed=21, bands_in_window=2, mesh_ref=mesh_ref)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    last = min(el.nwv_irred * mat.nb, 12)
+    counts, g2 = O.ibzk_g2(o, el, ph, mesh_ref, 1, last)
+    tet = O.Tetrahedra(el.wvmesh, el.ens, indexlist=el.indexlist)
+    T, mu = 300.0, float(mat.enref)
+    xp, xm, se, sp = O.ibzk_X(o, el, ph, tet, counts, g2, T, mu, mesh_ref=mesh_ref)
+    rp, rm, re_, rs = _x_by_loops(mat, o, el, ph, tet, counts, g2, T, mu, mesh_ref)
+    assert np.array_equal(se, re_) and np.array_equal(sp, rs)
+    assert np.allclose(xp, rp, rtol=1e-13, atol=0) and np.allclose(xm, rm, rtol=1e-13, atol=0)
+    assert (xp > 0).any() and (xm > 0).any()
+    rates = O.el_rta_rates(counts, xp, xm)
+    for t in range(len(counts)):
+        c = max(counts[t], 0)
+        assert rates[t] == pytest.approx(xp[t, :c].sum() + xm[t, :c].sum(), rel=1e-13)
