@@ -153,4 +153,32 @@ void launch_build_H(elph_ctx *c, hipStream_t st, int64_t n, const double *d_kvec
 void launch_polar_lp(hipStream_t st, int n, int nbr, int div, const int *ridx, const double2 *Lq, const double2 *u,
                      double2 *lp);
 
+// Scratch of the consumers of an ibzk plan's g2 rows (transition
+// probabilities and rates), owned by the plan.
+struct ConsumerScratch {
+    DBuf<int> vrow;      // [nprocs] k' row of each process of a row
+    DBuf<int4> st_k;     // per state: k (electron mesh ints), in-window flag
+    DBuf<double> st_e;   // per state: electron energy
+    DBuf<double> partial;
+};
+
+// The k'-row tables of an ibzk plan, as its consumers read them.  A g2 row is
+// ordered k' ascending (row r), then in-window band rank, then branch s: the
+// process of rank j at row r is at offs[r] + j nbr + s, band wband[r nb + j].
+struct IbzkRows {
+    elph_ctx *ctx;
+    int nb, nbr, nwv, mesh_ref;
+    int Nel[3], Nph[3];
+    int64_t nirr, nprocs;
+    const int *kp_int, *wband;
+    const int64_t *offs;
+    const double *om, *omf;      // phonon energies [nq][s] on the phonon mesh / every electron-mesh q
+    const int *kint_irr;         // host [nirr][3]
+    const double *ens_irred;     // host (nirr, nb)
+    double enref, fsthick;
+    ConsumerScratch *scr;
+};
+// Builds the vrow table on first use (synchronous).
+IbzkRows ibzk_rows(elph_ibzk_plan *p);
+
 }  // namespace elph

@@ -2869,6 +2869,7 @@ struct elph_ibzk_plan {
     bool general = false;
     int gcb = 3, grb = 2;
     int64_t CPf = 0;  // column count of the fused layout (16-rounded nb*nbr)
+    ConsumerScratch cons;
 };
 
 namespace {
@@ -3407,6 +3408,46 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
 }
 
 }  // extern "C"
+
+namespace elph {
+IbzkRows ibzk_rows(elph_ibzk_plan *p) {
+    IbzkRows v;
+    v.ctx = p->ctx;
+    v.nb = (int)p->nb;
+    v.nbr = (int)p->nbr;
+    v.nwv = (int)p->nwv;
+    v.mesh_ref = (int)p->mesh_ref;
+    for (int d = 0; d < 3; ++d) {
+        v.Nel[d] = (int)p->Nel[d];
+        v.Nph[d] = (int)p->Nph[d];
+    }
+    v.nirr = p->nirr;
+    v.nprocs = p->nprocs;
+    v.kp_int = p->kp_int.p;
+    v.wband = p->wband.p;
+    v.offs = p->offs.p;
+    v.om = p->om.p;
+    v.omf = p->omf.p;
+    v.kint_irr = p->kint_irr.data();
+    v.ens_irred = p->ens_irred.data();
+    v.enref = p->enref;
+    v.fsthick = p->fsthick;
+    v.scr = &p->cons;
+    if (!p->cons.vrow.p && p->nprocs > 0) {
+        std::vector<int64_t> offs(p->nwv);
+        std::vector<int> nw(p->nwv);
+        HCHECK(hipStreamSynchronize(p->ctx->stream));
+        HCHECK(hipMemcpy(offs.data(), p->offs.p, p->nwv * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HCHECK(hipMemcpy(nw.data(), p->nwin.p, p->nwv * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int> vr(p->nprocs);
+        for (int64_t r = 0; r < p->nwv; ++r)
+            for (int64_t j = 0; j < (int64_t)nw[r] * p->nbr; ++j) vr[offs[r] + j] = (int)r;
+        p->cons.vrow.upload(vr, p->ctx->stream);
+        HCHECK(hipStreamSynchronize(p->ctx->stream));
+    }
+    return v;
+}
+}  // namespace elph
 
 // ===========================================================================
 // Batched multi-material driver: the ibzk work of many plans (materials of

@@ -1,0 +1,477 @@
+// Consumers of the vertex rows on the device: the linear tetrahedron delta
+// function (src/delta.f90:61-225) on the tetrahedra of src/delta.f90:494-646,
+// the transition probabilities X+/- of calculate_eph_interaction_ibzk key 'X'
+// (src/interactions.f90:2278-2354) and the e-ph RTA scattering rates of
+// calculate_el_rta_rates (src/interactions.f90:2957-3063).
+//
+// Layout in HBM:
+//   tetrahedra  tcount[nk], tmap[nk][24] = 4 (it - 1) + (iv - 1) (it the
+//               tetrahedron, iv the wave vector's vertex label in it), tev[ntet][nb]
+//               the four vertex energies sorted (32 bytes), erange[nk][nb] the lowest
+//               and highest vertex energy over the wave vector's tetrahedra;
+//   processes   one thread per process of a g2 row (state, k', band, branch): reads
+//               its |g|^2 (8 B), writes X+ and X- (16 B) and optionally the final
+//               electron and phonon state (16 B).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ctx.h"
+#include "kernels.h"
+
+using namespace elph;
+
+struct elph_tetra {
+    elph_ctx *ctx = nullptr;
+    int64_t nk = 0, nb = 0, ntet = 0, mesh[3] = {0, 0, 0};
+    // host copies in the reference's layouts (tetra (ntet,4), tetracount (nk),
+    // tetramap (2,nk,24), tetra_evals (ntet,nb,4))
+    std::vector<int64_t> tetra, tetracount, tetramap;
+    std::vector<double> tetra_evals;
+    DBuf<int> tcount, tmap;
+    DBuf<d4> tev;
+    DBuf<dv2> erange;
+};
+
+namespace {
+
+// Constants of src/params.f90:27-43.
+constexpr double kQe = 1.602176634e-19;
+constexpr double kHbar = 1.05457172647e-22;
+constexpr double kBoltz = 1.380649e-23 / kQe;  // eV/K
+
+// Weight of vertex label iv (1..4) of a tetrahedron with sorted energies
+// e1 <= e2 <= e3 <= e4 at energy e, for e1 <= e <= e4: the case expressions of
+// src/delta.f90:118-210, with their degenerate-energy replacements and the
+// equal-energies value 1/4.  Evaluated without contraction so the rounding
+// follows the expressions as written.
+__device__ __noinline__ double tetra_weight(int iv, double e, double e1, double e2, double e3, double e4) {
+#pragma clang fp contract(off)
+    const double e1e = e1 - e, e2e = e2 - e, e3e = e3 - e, e4e = e4 - e;
+    const double e21 = e2 - e1, e31 = e3 - e1, e41 = e4 - e1, e32 = e3 - e2, e42 = e4 - e2, e43 = e4 - e3;
+    const bool c1 = e1 <= e && e <= e2, c2 = e2 <= e && e <= e3, c3 = e3 <= e && e <= e4;
+    double w = 0.0;
+    if (iv == 1) {
+        if (c1) {
+            w = e1 == e2 ? 0.0 : (e2e / e21 + e3e / e31 + e4e / e41) * (e1e * e1e) / e41 / e31 / e21;
+        } else if (c2) {
+            if (e2 == e3)
+                w = -0.5 * (e4e * e1e / e41 / e42 + e1e / e41 + e4e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e / e42 + e1e / e31));
+            else
+                w = -0.5 * (e3e / (e31 * e31) * (e3e * e2e / e42 / e32 + e4e * e1e / e41 / e42 + e3e * e1e / e32 / e41) +
+                            e4e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e * e2e / e42 / e32 + e3e * e1e / e31 / e32));
+        } else if (c3) {
+            w = e3 == e4 ? (e4e * e4e) / (e41 * e41) / e42 : (e4e * e4e * e4e) / (e41 * e41) / e42 / e43;
+        }
+    } else if (iv == 2) {
+        if (c1) {
+            w = e1 == e2 ? 0.0 : -(e1e * e1e * e1e) / (e21 * e21) / e31 / e41;
+        } else if (c2) {
+            if (e2 == e3)
+                w = -0.5 * (0.0 + e4e / e42 / e41 + 0.0 + e4e / (e42 * e42) * (0.0 + e4e * e1e / e41 / e31 + 1.0));
+            else
+                w = -0.5 * (e3e / (e32 * e32) * (e3e * e2e / e42 / e31 + e4e * e2e / e42 / e41 + e3e * e1e / e31 / e41) +
+                            e4e / (e42 * e42) * (e3e * e2e / e32 / e31 + e4e * e1e / e41 / e31 + e4e * e2e / e32 / e41));
+        } else if (c3) {
+            w = e3 == e4 ? 0.0 : (e4e * e4e * e4e) / e41 / (e42 * e42) / e43;
+        }
+    } else if (iv == 3) {
+        if (c1) {
+            w = e1 == e2 ? 0.0 : -(e1e * e1e * e1e) / e21 / (e31 * e31) / e41;
+        } else if (c2) {
+            if (e2 == e3)
+                w = 0.5 * (0.0 + e4e / e42 / e41 + e1e / e31 / e41 + e1e / (e31 * e31) * (0.0 + e4e * e1e / e41 / e42 + e1e / e41));
+            else
+                w = 0.5 * (e2e / (e32 * e32) * (e3e * e2e / e42 / e31 + e4e * e2e / e42 / e41 + e3e * e1e / e31 / e41) +
+                           e1e / (e31 * e31) * (e3e * e2e / e42 / e32 + e4e * e1e / e41 / e42 + e3e * e1e / e32 / e41));
+        } else if (c3) {
+            w = e3 == e4 ? 0.0 : (e4e * e4e * e4e) / e41 / e42 / (e43 * e43);
+        }
+    } else {
+        if (c1) {
+            w = e1 == e2 ? 0.0 : -(e1e * e1e * e1e) / e21 / e31 / (e41 * e41);
+        } else if (c2) {
+            if (e2 == e3)
+                w = 0.5 * (0.0 + e1e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e / e42 + e1e / e31));
+            else
+                w = 0.5 * (e2e / (e42 * e42) * (e3e * e2e / e32 / e31 + e4e * e1e / e41 / e31 + e4e * e2e / e32 / e41) +
+                           e1e / (e41 * e41) * (e4e * e1e / e42 / e31 + e4e * e2e / e42 / e32 + e3e * e1e / e31 / e32));
+        } else if (c3) {
+            w = e3 == e4 ? 0.0 : -(e3e / e43 + e2e / e42 + e1e / e41) * (e4e * e4e) / e41 / e42 / e43;
+        }
+    }
+    if (e1 == e2 && e1 == e3 && e1 == e4 && e == e1) w = 0.25;
+    return w;
+}
+
+struct TetraView {
+    const int *tcount, *tmap;
+    const d4 *tev;
+    const dv2 *erange;
+    int nb;
+    double numtetra;
+};
+
+// delta_fn_tetra(e, r + 1, n + 1) (src/delta.f90:61-225): sum of the vertex
+// weights over the tetrahedra of list point r, zero below 1e-12, divided by
+// 6 prod(mesh).  Energies outside every tetrahedron of (r, n) return 0 at once.
+__device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int r, int n) {
+    const dv2 rg = t.erange[(size_t)r * t.nb + n];
+    if (e < rg.x || e > rg.y) return 0.0;
+    double d = 0.0;
+    const int num = t.tcount[r];
+    for (int j = 0; j < num; ++j) {
+        const int code = t.tmap[r * 24 + j];
+        const d4 v = t.tev[(size_t)(code >> 2) * t.nb + n];
+        if (e < v.x || e > v.w) continue;
+        d += tetra_weight((code & 3) + 1, e, v.x, v.y, v.z, v.w);
+    }
+    if (d < 1.0e-12) d = 0.0;
+    return d / t.numtetra;
+}
+
+__global__ void k_delta_samples(TetraView t, int n, const double *e, const int64_t *ik, const int64_t *ib,
+                                double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = delta_tetra(t, e[i], (int)ik[i] - 1, (int)ib[i] - 1);
+}
+
+struct XArgs {
+    TetraView t;
+    int nb, nbr, mesh_ref, Nel0, Nel1, Nel2, Nph0, Nph1, Nph2;
+    int64_t nprocs;
+    const int *vrow, *kp_int, *wband;
+    const int64_t *offs;
+    const double *om, *omf;
+    const int4 *st_k;
+    const double *st_e;
+    double T, chempot, cnst;
+    const double *g2;
+    double *xp, *xm;
+    int64_t *se, *sp;
+    double *partial;  // [state][block] sums of X+ + X-
+};
+
+// One thread per process of a g2 row (blockIdx.y = state of the chunk): the
+// loop body of src/interactions.f90:2278-2332 for that process, X scaled by
+// 2 pi / hbar (2351-2354), and the block's sum of X+ + X- for the rates.
+__global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
+    const int st = blockIdx.y;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int4 k = a.st_k[st];
+    double contrib = 0.0;
+    if (k.w && c < a.nprocs) {
+        const int r = a.vrow[c];
+        const int loc = (int)(c - a.offs[r]);
+        const int rank = loc / a.nbr, s = loc - rank * a.nbr;
+        const int n = a.wband[r * a.nb + rank];
+        const int q0 = pos_mod(a.kp_int[3 * r] - k.x, a.Nel0);
+        const int q1 = pos_mod(a.kp_int[3 * r + 1] - k.y, a.Nel1);
+        const int q2 = pos_mod(a.kp_int[3 * r + 2] - k.z, a.Nel2);
+        double en_ph;
+        int64_t sph;
+        if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
+            const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+            en_ph = a.om[(size_t)iq * a.nbr + s];
+            sph = (int64_t)iq * a.nbr + s + 1;
+        } else {
+            const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+            en_ph = a.omf[(size_t)iq * a.nbr + s];
+            sph = -((int64_t)iq * a.nbr + s + 1);
+        }
+        const double en_el = a.st_e[st];
+        double xp = 0.0, xm = 0.0;
+        if (en_ph >= 0.5e-3) {
+            const double g = a.g2[(size_t)st * a.nprocs + c];
+            const double bose = 1.0 / expm1(en_ph / kBoltz / a.T);
+            const double fp = 1.0 / (exp((en_el + en_ph - a.chempot) / kBoltz / a.T) + 1.0);
+            const double fm = 1.0 / (exp((en_el - en_ph - a.chempot) / kBoltz / a.T) + 1.0);
+            const double dp = delta_tetra(a.t, en_el + en_ph, r, n);
+            const double dm = delta_tetra(a.t, en_el - en_ph, r, n);
+            xp = a.cnst * (g * (bose + fp) * dp);
+            xm = a.cnst * (g * (1.0 + bose - fm) * dm);
+        }
+        const size_t o = (size_t)st * a.nprocs + c;
+        if (a.xp) {
+            a.xp[o] = xp;
+            a.xm[o] = xm;
+        }
+        if (a.se) {
+            a.se[o] = (int64_t)r * a.nb + n + 1;
+            a.sp[o] = sph;
+        }
+        contrib = xp + xm;
+    }
+    // block sum (fixed order: wave shuffles, then the 4 waves in order)
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = contrib;
+    __syncthreads();
+    if (threadIdx.x == 0) a.partial[(size_t)st * gridDim.x + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+// rates[st] = sum over the state's blocks of the partial sums (0 outside the
+// window), one block per state.
+__global__ __launch_bounds__(256) void k_rates(const double *partial, int nblk, const int4 *st_k, double *rates) {
+    const int st = blockIdx.x;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 256) v += partial[(size_t)st * nblk + i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) rates[st] = st_k[st].w ? ((ws[0] + ws[1]) + ws[2]) + ws[3] : 0.0;
+}
+
+TetraView view_of(const elph_tetra *t) {
+    TetraView v;
+    v.tcount = t->tcount.p;
+    v.tmap = t->tmap.p;
+    v.tev = t->tev.p;
+    v.erange = t->erange.p;
+    v.nb = (int)t->nb;
+    v.numtetra = (double)(t->mesh[0] * t->mesh[1] * t->mesh[2] * 6);
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                      const double *ens, elph_tetra **out) {
+    if (!ctx || !mesh || !ens || !out || nk < 1 || nb < 1) return ELPH_EINVAL;
+    *out = nullptr;
+    auto *t = new elph_tetra();
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        for (int d = 0; d < 3; ++d) {
+            if (mesh[d] < 1) throw Error{ELPH_EINVAL, "mesh components must be positive"};
+            t->mesh[d] = mesh[d];
+        }
+        const int64_t nmesh = mesh[0] * mesh[1] * mesh[2];
+        const bool blocks = indexlist != nullptr;
+        if (!blocks && nk != nmesh) throw Error{ELPH_EINVAL, "without an index list nk must be the mesh size"};
+        if (blocks) {
+            for (int64_t i = 0; i < nk; ++i) {
+                if (indexlist[i] < 1 || indexlist[i] > nmesh) throw Error{ELPH_EINVAL, "index list entry off the mesh"};
+                if (i > 0 && indexlist[i] <= indexlist[i - 1])
+                    throw Error{ELPH_EINVAL, "index list must be strictly ascending"};
+            }
+        }
+        if (6 * nk >= (int64_t)1 << 28) throw Error{ELPH_EINVAL, "too many tetrahedra"};
+        t->ctx = ctx;
+        t->nk = nk;
+        t->nb = nb;
+        t->ntet = 6 * nk;
+        const int64_t ntet = t->ntet;
+        t->tetra.assign(ntet * 4, 0);
+        t->tetracount.assign(nk, 0);
+        t->tetramap.assign(2 * nk * 24, 0);
+        // vertex offsets of a subcell (src/delta.f90:565-573) and the six
+        // tetrahedra as labels into them (522-529)
+        static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                         {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        static const int label[6][4] = {{1, 2, 3, 6}, {1, 3, 5, 6}, {3, 5, 6, 7}, {3, 6, 7, 8}, {3, 4, 6, 8}, {2, 3, 4, 6}};
+        std::vector<int> hcount(nk, 0), hmap(nk * 24, 0);
+        for (int64_t p = 0; p < nk; ++p) {
+            int64_t v[3];
+            demux_vector0(blocks ? indexlist[p] : p + 1, v, mesh);
+            for (int tk = 0; tk < 6; ++tk) {
+                const int64_t it = 6 * p + tk;  // 0-based tetrahedron
+                for (int tl = 0; tl < 4; ++tl) {
+                    const int *cn = corner[label[tk][tl] - 1];
+                    int64_t w[3];
+                    for (int d = 0; d < 3; ++d) w[d] = (v[d] + cn[d]) % mesh[d];
+                    const int64_t muxed = (w[2] * mesh[1] + w[1]) * mesh[0] + w[0] + 1;
+                    int64_t idx = muxed;
+                    if (blocks) {
+                        const int64_t *f = std::lower_bound(indexlist, indexlist + nk, muxed);
+                        idx = (f != indexlist + nk && *f == muxed) ? (f - indexlist) + 1 : -1;
+                    }
+                    t->tetra[it + ntet * tl] = idx;
+                    if (idx > 0) {
+                        const int64_t c = t->tetracount[idx - 1]++;
+                        if (c >= 24) throw Error{ELPH_EINVAL, "more than 24 tetrahedra at a wave vector"};
+                        t->tetramap[0 + 2 * ((idx - 1) + nk * c)] = it + 1;
+                        t->tetramap[1 + 2 * ((idx - 1) + nk * c)] = tl + 1;
+                        hmap[(idx - 1) * 24 + c] = (int)(4 * it + tl);
+                        hcount[idx - 1] = (int)(c + 1);
+                    }
+                }
+            }
+        }
+        // vertex energies (0 off the list), sorted per tetrahedron and band
+        // (src/delta.f90:607-646)
+        t->tetra_evals.assign(ntet * nb * 4, 0.0);
+        std::vector<d4> hev(ntet * nb);
+        for (int64_t it = 0; it < ntet; ++it)
+            for (int64_t b = 0; b < nb; ++b) {
+                double e[4];
+                for (int iv = 0; iv < 4; ++iv) {
+                    const int64_t idx = t->tetra[it + ntet * iv];
+                    e[iv] = idx > 0 ? ens[(idx - 1) + nk * b] : 0.0;
+                }
+                std::sort(e, e + 4);
+                for (int iv = 0; iv < 4; ++iv) t->tetra_evals[it + ntet * (b + nb * iv)] = e[iv];
+                hev[it * nb + b] = d4{e[0], e[1], e[2], e[3]};
+            }
+        std::vector<dv2> hr(nk * nb);
+        for (int64_t p = 0; p < nk; ++p)
+            for (int64_t b = 0; b < nb; ++b) {
+                double lo = INFINITY, hi = -INFINITY;
+                for (int j = 0; j < hcount[p]; ++j) {
+                    const d4 &v = hev[(size_t)(hmap[p * 24 + j] >> 2) * nb + b];
+                    lo = std::min(lo, v.x);
+                    hi = std::max(hi, v.w);
+                }
+                hr[p * nb + b] = dv2{lo, hi};
+            }
+        hipStream_t s = ctx->stream;
+        t->tcount.upload(hcount, s);
+        t->tmap.upload(hmap, s);
+        t->tev.upload(hev, s);
+        t->erange.upload(hr, s);
+        HCHECK(hipStreamSynchronize(s));
+    } catch (const Error &e) {
+        delete t;
+        return ret_error(ctx, e);
+    }
+    *out = t;
+    return ELPH_OK;
+}
+
+int elph_tetra_tables(const elph_tetra *t, int64_t *tetra, int64_t *tetracount, int64_t *tetramap,
+                      double *tetra_evals) {
+    if (!t) return ELPH_EINVAL;
+    if (tetra) std::memcpy(tetra, t->tetra.data(), t->tetra.size() * sizeof(int64_t));
+    if (tetracount) std::memcpy(tetracount, t->tetracount.data(), t->tetracount.size() * sizeof(int64_t));
+    if (tetramap) std::memcpy(tetramap, t->tetramap.data(), t->tetramap.size() * sizeof(int64_t));
+    if (tetra_evals) std::memcpy(tetra_evals, t->tetra_evals.data(), t->tetra_evals.size() * sizeof(double));
+    return ELPH_OK;
+}
+
+int elph_tetra_delta(elph_tetra *t, int64_t n, const double *e, const int64_t *ik, const int64_t *ib, double *out) {
+    if (!t || n < 0 || (n > 0 && (!e || !ik || !ib || !out))) return ELPH_EINVAL;
+    if (n == 0) return ELPH_OK;
+    elph_ctx *c = t->ctx;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        for (int64_t i = 0; i < n; ++i)
+            if (ik[i] < 1 || ik[i] > t->nk || ib[i] < 1 || ib[i] > t->nb)
+                throw Error{ELPH_EINVAL, "wave vector or band index out of range"};
+        if (n > INT32_MAX / 2) throw Error{ELPH_EINVAL, "too many samples"};
+        hipStream_t s = c->stream;
+        DBuf<double> de, dout;
+        DBuf<int64_t> dk, db;
+        de.upload(e, n, s);
+        dk.upload(ik, n, s);
+        db.upload(ib, n, s);
+        dout.alloc(n);
+        hipLaunchKernelGGL(k_delta_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, view_of(t), (int)n,
+                           de.p, dk.p, db.p, dout.p);
+        HCHECK(hipGetLastError());
+        HCHECK(hipMemcpyAsync(out, dout.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HCHECK(hipStreamSynchronize(s));
+    } catch (const Error &err) {
+        return ret_error(c, err);
+    }
+    return ELPH_OK;
+}
+
+int elph_tetra_destroy(elph_tetra *t) {
+    delete t;
+    return ELPH_OK;
+}
+
+int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, double chempot, int64_t first,
+                     int64_t last, const double *g2, double *xplus, double *xminus, int64_t *istate_el,
+                     int64_t *istate_ph, double *rates) {
+    if (!plan || !tet || !g2) return ELPH_EINVAL;
+    elph_ctx *c = tet->ctx;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        IbzkRows v = ibzk_rows(plan);
+        c = v.ctx;
+        if (!(T > 0.0)) throw Error{ELPH_EINVAL, "temperature must be positive"};
+        if ((xplus == nullptr) != (xminus == nullptr)) throw Error{ELPH_EINVAL, "pass both X+ and X- or neither"};
+        if ((istate_el == nullptr) != (istate_ph == nullptr))
+            throw Error{ELPH_EINVAL, "pass both process maps or neither"};
+        if (!xplus && !istate_el && !rates) throw Error{ELPH_EINVAL, "no output requested"};
+        if (tet->ctx->device != c->device) throw Error{ELPH_EINVAL, "tetrahedra and plan on different devices"};
+        if (tet->nk != v.nwv || tet->nb != v.nb)
+            throw Error{ELPH_EINVAL, "tetrahedra not formed on the plan's electron list and bands"};
+        for (int d = 0; d < 3; ++d)
+            if (tet->mesh[d] != v.Nel[d]) throw Error{ELPH_EINVAL, "tetrahedra mesh differs from the electron mesh"};
+        if (first < 1 || last > v.nirr * v.nb || last < first)
+            throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
+        const int64_t nst = last - first + 1;
+        const int64_t nblk = (v.nprocs + 255) / 256;
+        if (nst > 65535) throw Error{ELPH_EINVAL, "at most 65535 states per call"};
+        std::vector<int4> sk(nst);
+        std::vector<double> se(nst);
+        for (int64_t i = 0; i < nst; ++i) {
+            int64_t m, ik;
+            demux_state(first + i, v.nb, &m, &ik);
+            const double e = v.ens_irred[(ik - 1) + v.nirr * (m - 1)];
+            const int *kk = v.kint_irr + 3 * (ik - 1);
+            sk[i] = make_int4(kk[0], kk[1], kk[2], std::fabs(e - v.enref) > v.fsthick ? 0 : 1);
+            se[i] = e;
+        }
+        hipStream_t s = c->stream;
+        ConsumerScratch &scr = *v.scr;
+        scr.st_k.upload(sk, s);
+        scr.st_e.upload(se, s);
+        scr.partial.alloc(std::max<int64_t>(1, nst * nblk));
+        XArgs a;
+        a.t = view_of(tet);
+        a.nb = v.nb;
+        a.nbr = v.nbr;
+        a.mesh_ref = v.mesh_ref;
+        a.Nel0 = v.Nel[0];
+        a.Nel1 = v.Nel[1];
+        a.Nel2 = v.Nel[2];
+        a.Nph0 = v.Nph[0];
+        a.Nph1 = v.Nph[1];
+        a.Nph2 = v.Nph[2];
+        a.nprocs = v.nprocs;
+        a.vrow = scr.vrow.p;
+        a.kp_int = v.kp_int;
+        a.wband = v.wband;
+        a.offs = v.offs;
+        a.om = v.om;
+        a.omf = v.omf;
+        a.st_k = scr.st_k.p;
+        a.st_e = scr.st_e.p;
+        a.T = T;
+        a.chempot = chempot;
+        a.cnst = 2.0 * (4.0 * std::atan(1.0)) / (kHbar / kQe);
+        a.g2 = g2;
+        a.xp = xplus;
+        a.xm = xminus;
+        a.se = istate_el;
+        a.sp = istate_ph;
+        a.partial = scr.partial.p;
+        HCHECK(hipEventRecord(c->ev_a, s));
+        if (nblk > 0) {
+            hipLaunchKernelGGL(k_ibzk_X, dim3((unsigned)nblk, (unsigned)nst), dim3(256), 0, s, a);
+            HCHECK(hipGetLastError());
+        }
+        HCHECK(hipEventRecord(c->ev_b, s));
+        if (rates) {
+            hipLaunchKernelGGL(k_rates, dim3((unsigned)nst), dim3(256), 0, s, scr.partial.p, (int)nblk, scr.st_k.p,
+                               rates);
+            HCHECK(hipGetLastError());
+        }
+        HCHECK(hipStreamSynchronize(s));
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+        c->t_dominant_ms = ms;
+        c->n_dominant = 1;
+    } catch (const Error &err) {
+        return ret_error(c, err);
+    }
+    return ELPH_OK;
+}
+
+}  // extern "C"

@@ -88,6 +88,32 @@ class IbzkPlan:
         self.wann._check(st, "ibzk run")
         return counts[:nst], out
 
+    def run_X(self, tet: "Tetrahedra", T: float, chempot: float, istate_first: int, istate_last: int, g2,
+              maps: bool = True, probabilities: bool = True):
+        """calculate_eph_interaction_ibzk('X') (src/interactions.f90:2278-2354)
+        on the device g2 rows that run() wrote for the same states (a
+        DeviceBuffer of shape (nstates, nprocs)), plus the e-ph RTA rate of each
+        state (calculate_el_rta_rates, 2957-3063).  Returns a dict of
+        DeviceBuffers "Xplus", "Xminus" (THz), "istate_el", "istate_ph" (when
+        asked for) and the host array "rates" (THz, 0 outside the window)."""
+        from .wannier import DeviceBuffer
+        nst = istate_last - istate_first + 1
+        shape = (nst, self.nprocs)
+        out = {}
+        if probabilities:
+            out["Xplus"], out["Xminus"] = DeviceBuffer(self.wann, shape), DeviceBuffer(self.wann, shape)
+        if maps:
+            out["istate_el"] = DeviceBuffer(self.wann, shape, np.int64)
+            out["istate_ph"] = DeviceBuffer(self.wann, shape, np.int64)
+        rates = DeviceBuffer(self.wann, (nst,))
+        ptr = lambda k: out[k].ptr if k in out else None
+        g2p, _ = _dev_ptr(g2)
+        st = self.lib.elph_ibzk_plan_X(self.plan, tet.h, float(T), float(chempot), istate_first, istate_last, g2p,
+                                       ptr("Xplus"), ptr("Xminus"), ptr("istate_el"), ptr("istate_ph"), rates.ptr)
+        self.wann._check(st, "ibzk X")
+        out["rates"] = rates.numpy()
+        return out
+
     def timing(self):
         t = np.zeros(3)
         self.lib.elph_last_timing(self.wann.ctx, _p(t))
@@ -97,6 +123,56 @@ class IbzkPlan:
         if getattr(self, "plan", None):
             self.lib.elph_ibzk_plan_destroy(self.plan)
             self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tetrahedra:
+    """form_tetrahedra_3d + fill_tetrahedra_3d (src/delta.f90:494-646) on the
+    device: over el%indexlist (electron.f90:582-587) when indexlist is given,
+    else over the full mesh; ens (nk, nb) eV."""
+
+    def __init__(self, wann: Wannier, mesh, ens, indexlist=None):
+        self.wann, self.lib = wann, wann.lib
+        ens = np.asfortranarray(np.asarray(ens, np.float64))
+        self.nk, self.nb = ens.shape
+        self.mesh = np.asarray(mesh, np.int64)
+        il = None if indexlist is None else np.ascontiguousarray(indexlist, np.int64)
+        h = ct.c_void_p()
+        st = self.lib.elph_tetra_create(wann.ctx, self.nk, _p(self.mesh, c_i64p),
+                                        None if il is None else _p(il, c_i64p), self.nb, _p(ens), ct.byref(h))
+        wann._check(st, "tetrahedra")
+        self.h = h
+
+    def tables(self):
+        """(tetra, tetracount, tetramap, tetra_evals) in the reference's shapes."""
+        nk, nb = self.nk, self.nb
+        tetra = np.zeros((6 * nk, 4), np.int64, order="F")
+        count = np.zeros(nk, np.int64)
+        tmap = np.zeros((2, nk, 24), np.int64, order="F")
+        ev = np.zeros((6 * nk, nb, 4), order="F")
+        st = self.lib.elph_tetra_tables(self.h, _p(tetra, c_i64p), _p(count, c_i64p), _p(tmap, c_i64p), _p(ev))
+        self.wann._check(st, "tetrahedra tables")
+        return tetra, count, tmap, ev
+
+    def delta(self, e, ik, ib):
+        """delta_fn_tetra (src/delta.f90:61-225) on the device; ik, ib 1-based."""
+        e = np.ascontiguousarray(e, np.float64)
+        ik = np.ascontiguousarray(np.broadcast_to(ik, e.shape), np.int64)
+        ib = np.ascontiguousarray(np.broadcast_to(ib, e.shape), np.int64)
+        out = np.zeros(e.shape)
+        st = self.lib.elph_tetra_delta(self.h, e.size, _p(e), _p(ik, c_i64p), _p(ib, c_i64p), _p(out))
+        self.wann._check(st, "delta_fn_tetra")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.elph_tetra_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

@@ -22,6 +22,7 @@ extern "C" {
 
 typedef struct elph_ctx elph_ctx;
 typedef struct elph_ibzk_plan elph_ibzk_plan;
+typedef struct elph_tetra elph_tetra;
 typedef struct elph_ibzq_plan elph_ibzq_plan;
 
 enum { ELPH_OK = 0, ELPH_EINVAL = 1, ELPH_EDEVICE = 2, ELPH_EIO = 3, ELPH_ENEGPHONON = 4, ELPH_EWINDOW = 5,
@@ -157,6 +158,40 @@ int elph_ibzq_plan_run(elph_ibzq_plan *plan, int64_t istate_first, int64_t istat
                        double *g2, int32_t g2_on_device);
 int64_t elph_ibzq_plan_stride(const elph_ibzq_plan *plan);
 int elph_ibzq_plan_destroy(elph_ibzq_plan *plan);
+
+/* ---- Consumers of the g2 rows ----
+ * form_tetrahedra_3d + fill_tetrahedra_3d (src/delta.f90:494-646) over a wave
+ * vector list: indexlist (nk) muxed 1-based FBZ indices, strictly ascending,
+ * as electron.f90:582-587 passes el%indexlist (blocks = .true.), or NULL for
+ * the full mesh (nk = prod(mesh)); ens (nk, nb) column-major eV.  Vertices
+ * off the list carry energy 0, as in the reference.  Tables live on the
+ * context's device. */
+int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                      const double *ens, elph_tetra **out);
+/* Host copies in the reference's layouts, any pointer may be NULL: tetra
+ * (6 nk, 4) (-1 = vertex off the list), tetracount (nk), tetramap (2, nk, 24),
+ * tetra_evals (6 nk, nb, 4) sorted over the vertices. */
+int elph_tetra_tables(const elph_tetra *t, int64_t *tetra, int64_t *tetracount, int64_t *tetramap,
+                      double *tetra_evals);
+/* delta_fn_tetra (src/delta.f90:61-225) evaluated on the device for n host
+ * samples (energy e[i], 1-based list index ik[i], 1-based band ib[i]). */
+int elph_tetra_delta(elph_tetra *t, int64_t n, const double *e, const int64_t *ik, const int64_t *ib, double *out);
+int elph_tetra_destroy(elph_tetra *t);
+
+/* calculate_eph_interaction_ibzk, key 'X' (src/interactions.f90:2278-2354),
+ * on the device rows g2 (stride nprocs) that elph_ibzk_plan_run wrote for the
+ * same istate_first..istate_last, the tetrahedra formed on the plan's
+ * electron list: X+ and X- (THz, rows of stride nprocs; rows of states
+ * outside the window are left untouched), the final electron state
+ * mux_state(nb, n, ik') and phonon state mux_state(nbr, s, iq) (negative for
+ * a phonon off the phonon mesh), and the e-ph RTA rate of each state, sum X+
+ * + sum X- (calculate_el_rta_rates, 2957-3063; 0 outside the window).  All
+ * output pointers are device memory and may be NULL (X+ and X- together, the
+ * two state maps together); T in K, chempot in eV.  At most 65535 states per
+ * call. */
+int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, double chempot, int64_t istate_first,
+                     int64_t istate_last, const double *g2, double *xplus, double *xminus, int64_t *istate_el,
+                     int64_t *istate_ph, double *rates);
 
 /* Writes gk2.istate<i> (kind 'k': int64 count + nprocs doubles, the whole
  * buffer as src/interactions.f90:2337-2349 does) or gq2.istate<i> (kind 'q':

@@ -1,0 +1,117 @@
+"""GPU parity of the g2 consumers (through the C ABI) against the oracle:
+tetrahedra tables (exact), delta_fn_tetra on the device (1e-12 relative),
+X+/- of calculate_eph_interaction_ibzk('X') on the device g2 rows with the
+oracle fed the same rows (1e-11 relative above 1e-9 * max, maps exact), the
+RTA rates (1e-11), and X from mixed-precision g2 against X from the oracle's
+own g2 (1e-5 relative above 1e-6 * max, the tolerance of |g|^2 itself;
+branch sums of a (k', n) block where phonons are interpolated off the
+phonon mesh)."""
+import numpy as np
+import pytest
+
+from elphbolt_amd import interactions, synth
+from elphbolt_amd.wannier import Wannier
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+T_K = 300.0
+
+
+def _rel(x, ref, floor):
+    x, ref = np.asarray(x).ravel(), np.asarray(ref).ravel()
+    sel = np.abs(ref) > floor * np.abs(ref).max()
+    return float(np.max(np.abs(x[sel] - ref[sel]) / np.abs(ref[sel]))) if sel.any() else 0.0
+
+
+CASES = [
+    ("nb3_k6", dict(nb=3, natoms=1, ws=3, wv=6, seed=31, bands_in_window=2), 1),
+    ("nb4_na2_k4_ref2", dict(nb=4, natoms=2, ws=3, wv=4, seed=32, bands_in_window=2, mesh_ref=2), 2),
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def case(request):
+    name, kw, mesh_ref = request.param
+    mat = synth.tiny(**kw)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    tet_o = O.Tetrahedra(el.wvmesh, el.ens, indexlist=el.indexlist)
+    w = Wannier(mat)
+    tet_g = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
+    return dict(name=name, mat=mat, o=o, el=el, ph=ph, w=w, tet_o=tet_o, tet_g=tet_g, mesh_ref=mesh_ref)
+
+
+def test_tetra_tables(case):
+    print(f"tetra tables {case['name']}", flush=True)
+    t, c, m, ev = case["tet_g"].tables()
+    o = case["tet_o"]
+    assert np.array_equal(t, o.tetra) and np.array_equal(c, o.tetracount)
+    assert np.array_equal(m, o.tetramap) and np.array_equal(ev, o.evals)
+
+
+def test_delta_on_device(case):
+    print(f"delta_fn_tetra {case['name']}", flush=True)
+    el, o = case["el"], case["tet_o"]
+    rng = np.random.default_rng(3)
+    n = 4000
+    ik = rng.integers(1, el.nwv + 1, n)
+    ib = rng.integers(1, el.nb + 1, n)
+    e = el.ens[ik - 1, ib - 1] + rng.normal(0, 0.05, n)
+    e[:50] = el.ens[ik[:50] - 1, ib[:50] - 1]  # exactly at a vertex energy
+    d_g = case["tet_g"].delta(e, ik, ib)
+    d_o = np.array([o.delta(e[i], ik[i], ib[i]) for i in range(n)])
+    assert (d_o > 0).sum() > n // 10
+    assert np.array_equal(d_g == 0, d_o == 0)
+    assert _rel(d_g, d_o, 1e-9) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_X_and_rates(case, precision):
+    mat, el, ph, w = case["mat"], case["el"], case["ph"], case["w"]
+    mesh_ref = case["mesh_ref"]
+    last = min(el.nwv_irred * mat.nb, 24)
+    print(f"X {case['name']} {precision} states 1..{last}", flush=True)
+    from elphbolt_amd.wannier import DeviceBuffer
+    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=mesh_ref, precision=precision)
+    g2d = DeviceBuffer(w, (last, plan.nprocs))
+    counts, _ = plan.run(1, last, out=g2d)
+    mu = float(mat.enref) + 0.02
+    out = plan.run_X(case["tet_g"], T_K, mu, 1, last, g2d)
+    act = counts >= 0
+    assert act.any()
+    g2 = g2d.numpy()
+    xp_o, xm_o, se_o, sp_o = O.ibzk_X(case["o"], el, ph, case["tet_o"], counts, g2, T_K, mu, mesh_ref=mesh_ref)
+    xp, xm = out["Xplus"].numpy()[act], out["Xminus"].numpy()[act]
+    assert np.array_equal(out["istate_el"].numpy()[act], se_o[act])
+    assert np.array_equal(out["istate_ph"].numpy()[act], sp_o[act])
+    assert (xp_o[act] > 0).any() and (xm_o[act] > 0).any()
+    err = max(_rel(xp, xp_o[act], 1e-9), _rel(xm, xm_o[act], 1e-9))
+    print(f"  X same g2: max rel {err:.2e}", flush=True)
+    assert err < 1e-11
+    r_o = O.el_rta_rates(counts, xp_o, xm_o)
+    assert np.all(out["rates"][~act] == 0.0)
+    assert _rel(out["rates"], r_o, 1e-9) < 1e-11
+    # against the oracle's whole chain (its own g2)
+    c_o, g_o = O.ibzk_g2(case["o"], el, ph, mesh_ref, 1, last)
+    assert np.array_equal(c_o, counts)
+    xp_oo, xm_oo, _, _ = O.ibzk_X(case["o"], el, ph, case["tet_o"], c_o, g_o, T_K, mu, mesh_ref=mesh_ref)
+    # processes whose |g|^2 is above 1e-6 of the largest, where the vertex
+    # itself is held to the tolerance.  Phonons off the phonon mesh come from
+    # each side's own diagonalisation: within degenerate branches (same
+    # energy, so the same occupation and delta) only the sum over the
+    # branches of a (k', n) block is fixed, as for |g|^2 itself.
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    xp_r, xm_r, g_r = xp_oo[act], xm_oo[act], g_o[act]
+    if mesh_ref > 1:
+        tol = max(tol, 1e-8)
+        nbr = mat.nbr
+        xp, xm, xp_r, xm_r, g_r = (a.reshape(-1, nbr).sum(axis=1) for a in (xp, xm, xp_r, xm_r, g_r))
+    big = np.abs(g_r) > 1e-6 * np.abs(g_r).max()
+    err2 = max(_rel(xp[big], xp_r[big], 1e-6), _rel(xm[big], xm_r[big], 1e-6))
+    print(f"  X whole chain: max rel {err2:.2e}", flush=True)
+    assert err2 < tol
+    assert _rel(out["rates"], O.el_rta_rates(c_o, xp_oo, xm_oo), 1e-9) < tol
+    plan.close()
