@@ -1421,3 +1421,103 @@ void orc_el_rta_rates(int64_t nst, const int64_t *counts, int64_t nprocs, const 
         rates[t] = r;
     }
 }
+
+/* bte.f90:912-1011 calculate_field_term, electrons (species 'el'): field 'E'
+ * (A = qe, pow = 0) or 'T' (A = 1/T, pow = 1).  ibz2fbz_map (maxeq, nk_ibz, 2)
+ * column-major, (:, :, 2) the muxed FBZ index, located in indexlist (nk) by
+ * binsearch; rates_ibz (nk_ibz, nb); field_term (nk, nb, 3) is overwritten
+ * (zero where no IBZ image lands or the rate is 0). */
+void orc_field_term(char field, const int64_t *nequiv, int64_t nk_ibz, int64_t maxeq, const int64_t *ibz2fbz_map,
+                    double T, double chempot, int64_t nk, int64_t nb, const double *ens, const double *vels,
+                    const double *rates_ibz, const int64_t *indexlist, double *field_term) {
+    const double A = field == 'T' ? 1.0 / T : QE;
+    const int pow_ = field == 'T' ? 1 : 0;
+    memset(field_term, 0, sizeof(double) * nk * nb * 3);
+    for (int64_t i = 0; i < nk_ibz; ++i)
+        for (int64_t ieq = 0; ieq < nequiv[i]; ++ieq) {
+            const int64_t ik = orc_binsearch(indexlist, nk, ibz2fbz_map[ieq + maxeq * (i + nk_ibz * 1)]);
+            for (int64_t ib = 0; ib < nb; ++ib) {
+                const double r = rates_ibz[i + nk_ibz * ib];
+                if (r == 0.0) continue;
+                const double e = ens[(ik - 1) + nk * ib];
+                const double ep = pow_ == 1 ? (e - chempot) : 1.0;
+                for (int d = 0; d < 3; ++d)
+                    field_term[(ik - 1) + nk * (ib + nb * d)] = A * vels[(ik - 1) + nk * (ib + nb * d)] * ep / r;
+            }
+        }
+}
+
+/* symmetry.f90:496-513 symmetrize_3x3_tensor: t = sum_r C_r t C_r^T / nrot. */
+static void orc_symmetrize_3x3(double t[9], const double *crot, int64_t nrot) {
+    double aux[9] = {0};
+    for (int64_t r = 0; r < nrot; ++r) {
+        const double *C = &crot[9 * r];
+        double m[9];
+        for (int i = 0; i < 3; ++i) /* m = t C^T */
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += t[i + 3 * k] * C[j + 3 * k];
+                m[i + 3 * j] = s;
+            }
+        for (int i = 0; i < 3; ++i) /* C m */
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += C[i + 3 * k] * m[k + 3 * j];
+                aux[i + 3 * j] += s;
+            }
+    }
+    for (int i = 0; i < 9; ++i) t[i] = aux[i] / (double)nrot;
+}
+
+/* bz_sums.f90:484-611 calculate_transport_coeff for electrons ('el'), field
+ * 'E' or 'T': trans_coeff_hc and trans_coeff_cc (nb, 3, 3) column-major,
+ * symmetrized with the nrot Cartesian rotations crot (3,3,nrot) when nrot >
+ * 0.  The literal 1.0e21 of the source is a default-kind (single precision)
+ * real, so its value is the float nearest 1e21. */
+void orc_transport_coeff(char field, double T, int64_t deg, double chempot, int64_t nk, int64_t nb, const double *ens,
+                         const double *vels, double volume, const int64_t mesh[3], const double *response,
+                         int64_t nrot, const double *crot, double *hc, double *cc) {
+    const double fac = (double)1.0e21f / KB_EV / T / volume / (double)(mesh[0] * mesh[1] * mesh[2]);
+    double A_hc, A_cc;
+    int pow_hc, pow_cc;
+    if (field == 'T') {
+        A_cc = -deg * QE * fac;
+        pow_cc = 0;
+        A_hc = deg * QE * fac;
+        pow_hc = 1;
+    } else {
+        A_cc = deg * fac;
+        pow_cc = 0;
+        A_hc = -A_cc;
+        pow_hc = 1;
+    }
+    memset(hc, 0, sizeof(double) * nb * 9);
+    memset(cc, 0, sizeof(double) * nb * 9);
+    for (int64_t ik = 0; ik < nk; ++ik)
+        for (int64_t ib = 0; ib < nb; ++ib) {
+            const double e = ens[ik + nk * ib];
+            double dist = orc_fermi(e, chempot, T);
+            dist = dist * (1.0 - dist);
+            const double p_hc = pow_hc == 1 ? (e - chempot) : 1.0, p_cc = pow_cc == 1 ? (e - chempot) : 1.0;
+            for (int j = 0; j < 3; ++j) {
+                const double v = vels[ik + nk * (ib + nb * j)];
+                for (int l = 0; l < 3; ++l) {
+                    const double r = response[ik + nk * (ib + nb * l)];
+                    hc[ib + nb * (j + 3 * l)] += p_hc * dist * v * r;
+                    if (A_cc != 0.0) cc[ib + nb * (j + 3 * l)] += p_cc * dist * v * r;
+                }
+            }
+        }
+    for (int64_t i = 0; i < nb * 9; ++i) {
+        hc[i] = A_hc * hc[i];
+        if (A_cc != 0.0) cc[i] = A_cc * cc[i];
+    }
+    if (nrot > 0)
+        for (int64_t ib = 0; ib < nb; ++ib) {
+            double t[9], u[9];
+            for (int i = 0; i < 9; ++i) { t[i] = hc[ib + nb * i]; u[i] = cc[ib + nb * i]; }
+            orc_symmetrize_3x3(t, crot, nrot);
+            if (A_cc != 0.0) orc_symmetrize_3x3(u, crot, nrot);
+            for (int i = 0; i < 9; ++i) { hc[ib + nb * i] = t[i]; cc[ib + nb * i] = u[i]; }
+        }
+}

@@ -143,6 +143,13 @@ int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons 
 void orc_el_rta_rates(int64_t nst, const int64_t *counts, int64_t nprocs, const double *Xplus, const double *Xminus,
                       double *rates);
 
+void orc_field_term(char field, const int64_t *nequiv, int64_t nk_ibz, int64_t maxeq, const int64_t *ibz2fbz_map,
+                    double T, double chempot, int64_t nk, int64_t nb, const double *ens, const double *vels,
+                    const double *rates_ibz, const int64_t *indexlist, double *field_term);
+void orc_transport_coeff(char field, double T, int64_t deg, double chempot, int64_t nk, int64_t nb, const double *ens,
+                         const double *vels, double volume, const int64_t mesh[3], const double *response,
+                         int64_t nrot, const double *crot, double *hc, double *cc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -231,16 +231,48 @@ def qrotations(mat, mesh):
     return q[:9 * n].copy(), int(n), int(rot.shape[0])
 
 
-def irred_wedge(mesh, qrot, nsr, indexlist=None):
+def irred_wedge(mesh, qrot, nsr, indexlist=None, maps=False):
+    """find_irred_wedge: (indexlist_irred, nequiv) and, with maps, ibz2fbz_map
+    (nsr, nk_irred, 2) column-major."""
     mesh = _i64(mesh)
     nlist = int(np.prod(mesh)) if indexlist is None else len(indexlist)
     irr = np.zeros(nlist, np.int64)
     neq = np.zeros(nlist, np.int64)
     il = _i64(indexlist) if indexlist is not None else None
+    m = np.zeros((nsr, nlist, 2), np.int64, order="F") if maps else None
     n = lib().orc_find_irred_wedge(_p(mesh, c_i64p), ct.c_int64(nsr), _p(qrot),
                                    _p(il, c_i64p) if il is not None else None, ct.c_int64(nlist),
-                                   _p(irr, c_i64p), _p(neq, c_i64p), None)
+                                   _p(irr, c_i64p), _p(neq, c_i64p), _p(m, c_i64p) if maps else None)
+    if maps:
+        return irr[:n].copy(), neq[:n].copy(), np.asfortranarray(m[:, :n, :])
     return irr[:n].copy(), neq[:n].copy()
+
+
+def field_term(field, nequiv, ibz2fbz_map, T, chempot, ens, vels, rates_ibz, indexlist):
+    """calculate_field_term for electrons (bte.f90:912-1011): (nk, nb, 3)."""
+    ens, vels, rates = _f64(ens), _f64(vels), _f64(rates_ibz)
+    m, neq, il = _i64(ibz2fbz_map), _i64(nequiv), _i64(indexlist)
+    nk, nb = ens.shape
+    out = np.zeros((nk, nb, 3), order="F")
+    lib().orc_field_term(ct.c_char(field.encode()), _p(neq, c_i64p), ct.c_int64(len(neq)), ct.c_int64(m.shape[0]),
+                         _p(m, c_i64p), ct.c_double(T), ct.c_double(chempot), ct.c_int64(nk), ct.c_int64(nb),
+                         _p(ens), _p(vels), _p(rates), _p(il, c_i64p), _p(out))
+    return out
+
+
+def transport_coeff(field, T, deg, chempot, ens, vels, volume, mesh, response, crot=None):
+    """calculate_transport_coeff for electrons (bz_sums.f90:484-611):
+    (trans_coeff_hc, trans_coeff_cc), each (nb, 3, 3); crot (3, 3, nrot)."""
+    ens, vels, resp = _f64(ens), _f64(vels), _f64(response)
+    nk, nb = ens.shape
+    mesh = _i64(mesh)
+    cr = _f64(crot) if crot is not None else np.zeros((3, 3, 1), order="F")
+    nrot = 0 if crot is None else cr.shape[2]
+    hc, cc = np.zeros((nb, 3, 3), order="F"), np.zeros((nb, 3, 3), order="F")
+    lib().orc_transport_coeff(ct.c_char(field.encode()), ct.c_double(T), ct.c_int64(deg), ct.c_double(chempot),
+                              ct.c_int64(nk), ct.c_int64(nb), _p(ens), _p(vels), ct.c_double(volume),
+                              _p(mesh, c_i64p), _p(resp), ct.c_int64(nrot), _p(cr), _p(hc), _p(cc))
+    return hc, cc
 
 
 class Electrons:

@@ -225,3 +225,99 @@ def test_ibzk_X_against_loops(mesh_ref):
     for t in range(len(counts)):
         c = max(counts[t], 0)
         assert rates[t] == pytest.approx(xp[t, :c].sum() + xm[t, :c].sum(), rel=1e-13)
+
+
+# ---- RTA field term and transport coefficients (bte.f90:912-1011,
+# bz_sums.f90:484-611) ----
+def _rand_transport(seed, nk=60, nb=3):
+    rng = np.random.default_rng(seed)
+    ens = rng.normal(0.0, 0.08, (nk, nb))
+    vels = rng.normal(0.0, 300.0, (nk, nb, 3))
+    rates = rng.uniform(0.5, 20.0, (nk, nb))
+    il = np.arange(1, nk + 1, dtype=np.int64)
+    neq = np.ones(nk, np.int64)
+    m = np.zeros((1, nk, 2), np.int64, order="F")
+    m[0, :, 0], m[0, :, 1] = 1, il
+    return ens, vels, rates, il, neq, m
+
+
+def test_sigma_in_si_units():
+    """sigma = deg q^2 / (N V) sum (-df/dE) v v tau, from SI quantities (v in
+    km/s, tau = 1 / rate ps, V nm^3); the reference's 1.0e21 literal is a
+    single-precision real (value float32(1e21))."""
+    ens, vels, rates, il, neq, m = _rand_transport(4)
+    T, mu, deg, vol, mesh = 300.0, 0.01, 2, 0.04, [3, 4, 5]
+    resp = O.field_term("E", neq, m, T, mu, ens, vels, rates, il)
+    hc, cc = O.transport_coeff("E", T, deg, mu, ens, vels, vol, mesh, resp)
+    qe, kBJ = 1.602176634e-19, 1.380649e-23
+    f = 1.0 / (np.exp((ens - mu) * qe / (kBJ * T)) + 1.0)
+    mdf = f * (1 - f) / (kBJ * T)                       # -df/dE, 1/J
+    v = vels * 1e3                                       # m/s
+    tau = 1e-12 / rates                                  # s
+    sig = deg * qe ** 2 * np.einsum("kb,kbi,kbj,kb->bij", mdf, v, v, tau) / (np.prod(mesh) * vol * 1e-27)
+    lit = float(np.float32(1e21)) / 1e21
+    assert np.allclose(cc, sig * lit, rtol=1e-12, atol=0)
+    # the E-field heat coefficient: the same sum weighted by -(E - mu)
+    alpha = -deg * qe ** 2 * np.einsum("kb,kb,kbi,kbj,kb->bij", mdf, (ens - mu) * qe, v, v, tau) / (
+        np.prod(mesh) * vol * 1e-27) / qe
+    assert np.allclose(hc, alpha * lit, rtol=1e-11, atol=1e-12 * np.abs(alpha).max())
+
+
+def test_onsager_reciprocity():
+    """The gradT charge coefficient equals the E-field heat coefficient / T
+    (sigma S = alpha / T) in the RTA."""
+    ens, vels, rates, il, neq, m = _rand_transport(5)
+    T, mu, deg, vol, mesh = 450.0, -0.02, 2, 0.05, [4, 4, 4]
+    rE = O.field_term("E", neq, m, T, mu, ens, vels, rates, il)
+    rT = O.field_term("T", neq, m, T, mu, ens, vels, rates, il)
+    hcE, ccE = O.transport_coeff("E", T, deg, mu, ens, vels, vol, mesh, rE)
+    hcT, ccT = O.transport_coeff("T", T, deg, mu, ens, vels, vol, mesh, rT)
+    assert np.allclose(ccT, hcE / T, rtol=1e-12, atol=1e-12 * np.abs(ccT).max())
+    # kappa0 and sigma positive semi-definite per band
+    for b in range(ens.shape[1]):
+        assert np.all(np.linalg.eigvalsh(0.5 * (ccE[b] + ccE[b].T)) > 0)
+        assert np.all(np.linalg.eigvalsh(0.5 * (hcT[b] + hcT[b].T)) > 0)
+
+
+def _cubic_group():
+    import itertools
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product([1, -1], repeat=3):
+            M = np.zeros((3, 3))
+            for i, p in enumerate(perm):
+                M[i, p] = signs[i]
+            out.append(M)
+    return np.asfortranarray(np.stack(out, axis=2))
+
+
+def test_symmetrized_tensor_is_cubic():
+    ens, vels, rates, il, neq, m = _rand_transport(6)
+    rE = O.field_term("E", neq, m, 300.0, 0.0, ens, vels, rates, il)
+    hc, cc = O.transport_coeff("E", 300.0, 2, 0.0, ens, vels, 0.04, [4, 4, 4], rE, crot=_cubic_group())
+    raw_hc, raw_cc = O.transport_coeff("E", 300.0, 2, 0.0, ens, vels, 0.04, [4, 4, 4], rE)
+    for b in range(ens.shape[1]):
+        assert np.allclose(cc[b], np.trace(raw_cc[b]) / 3 * np.eye(3), rtol=1e-12, atol=1e-12 * abs(cc[b]).max())
+        assert np.allclose(hc[b], np.trace(raw_hc[b]) / 3 * np.eye(3), rtol=1e-12, atol=1e-12 * abs(hc[b]).max())
+
+
+def test_field_term_covers_the_wedge_images():
+    mat = synth.tiny(nb=2, natoms=1, ws=3, wv=4, seed=41)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    mesh = np.array(mat.wvmesh)
+    il = np.arange(1, int(np.prod(mesh)) + 1, dtype=np.int64)
+    irr, neq, m = O.irred_wedge(mesh, qr, nsr, maps=True)
+    rng = np.random.default_rng(7)
+    nk, nb = len(il), 2
+    ens, vels = rng.normal(0, 0.1, (nk, nb)), rng.normal(0, 100, (nk, nb, 3))
+    rates = rng.uniform(1, 2, (len(irr), nb))
+    rates[0, 1] = 0.0   # a zero rate leaves its images' field term at zero
+    F = O.field_term("E", neq, m, 300.0, 0.0, ens, vels, rates, il)
+    qe = 1.602176634e-19
+    for i in range(len(irr)):
+        for l in range(neq[i]):
+            ik = int(m[l, i, 1]) - 1
+            for b in range(nb):
+                want = 0.0 if rates[i, b] == 0 else 1.0
+                assert np.allclose(F[ik, b] * rates[i, b] / qe if want else F[ik, b], vels[ik, b] if want else 0.0,
+                                   rtol=1e-14)
