@@ -73,6 +73,12 @@ PROTOTYPES = {
     "elph_tetra_tables": (ct.c_int, [ct.c_void_p, c_i64p, c_i64p, c_i64p, c_dblp]),
     "elph_tetra_delta": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_i64p, c_i64p, c_dblp]),
     "elph_tetra_destroy": (ct.c_int, [ct.c_void_p]),
+    "elph_el_transport_create": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_dblp, c_dblp, c_i64p, ct.c_int64,
+                                            ct.c_int64, c_i64p, c_i64p, c_dblp, ct.c_int64, c_dblp, ct.c_double,
+                                            c_i64p, ct.c_int64, ct.POINTER(ct.c_void_p)]),
+    "elph_el_transport_rta": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_double, c_dblp, c_dblp, c_dblp,
+                                         c_dblp]),
+    "elph_el_transport_destroy": (ct.c_int, [ct.c_void_p]),
     "elph_ibzk_plan_X": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_double, ct.c_double, ct.c_int64, ct.c_int64,
                                     ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "elph_ibzk_plan_destroy": (ct.c_int, [ct.c_void_p]),
@@ -94,6 +100,7 @@ PROTOTYPES = {
     "elph_selftest_mfma": (ct.c_int, [ct.c_int, c_dblp]),
     "elph_device_alloc": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_void_p)]),
     "elph_device_free": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "elph_host_to_device": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]),
     "elph_device_to_host": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int64]),
     "elph_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "elph_last_timing": (ct.c_int, [ct.c_void_p, c_dblp]),

@@ -2744,6 +2744,18 @@ int elph_ctx_synchronize(elph_ctx *ctx) {
     return hipDeviceSynchronize() == hipSuccess ? ELPH_OK : ELPH_EDEVICE;
 }
 
+int elph_host_to_device(elph_ctx *ctx, void *dptr, const void *host, int64_t bytes) {
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dptr || !host))) return ELPH_EINVAL;
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        HCHECK(hipMemcpyAsync(dptr, host, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+        HCHECK(hipStreamSynchronize(ctx->stream));
+    } catch (const Error &e) {
+        return ret_error(ctx, e);
+    }
+    return ELPH_OK;
+}
+
 int elph_device_copy(elph_ctx *ctx, void *dst, const void *src, int64_t bytes) {
     if (!ctx || bytes < 0) return ELPH_EINVAL;
     try {

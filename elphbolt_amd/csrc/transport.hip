@@ -33,6 +33,20 @@ struct elph_tetra {
     DBuf<dv2> erange;
 };
 
+// Static inputs of the electron RTA transport on the device.
+struct elph_el_transport {
+    elph_ctx *ctx = nullptr;
+    int64_t nk = 0, nb = 0, nk_ibz = 0, deg = 0, mesh[3] = {0, 0, 0};
+    double volume = 0;
+    std::vector<double> crot;  // (3,3,nrot) Cartesian rotations, host
+    bool symmetrize_k = false;
+    DBuf<double> ens;          // [nk][nb]
+    DBuf<double> vels;         // [nk][nb][3]
+    DBuf<double> sym;          // [nk][9] symmetrizer (column-major 3x3)
+    DBuf<int> src;             // [nk] IBZ point whose rate feeds the list point (-1: none)
+    DBuf<double> part, sums;   // block partials [nb][nblk][36], totals [nb][36]
+};
+
 namespace {
 
 // Constants of src/params.f90:27-43.
@@ -221,6 +235,83 @@ __global__ __launch_bounds__(256) void k_rates(const double *partial, int nblk, 
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) rates[st] = st_k[st].w ? ((ws[0] + ws[1]) + ws[2]) + ws[3] : 0.0;
+}
+
+// Electron RTA transport sums (src/bte.f90:912-1011 field terms, then
+// src/bz_sums.f90:559-583): for list point ik and band b with IBZ rate r,
+//   F_E = qe v / r,  F_T = (1/T) v (e - mu) / r   (0 when r == 0 or no IBZ image),
+// optionally F <- S_k F (the per-k symmetrizer, src/bte.f90:241-244, 262-265),
+// and with d = f (1 - f) the 36 sums per band
+//   hcE(j,l) += (e - mu) d v_j F_E,l   ccE(j,l) += d v_j F_E,l
+//   hcT(j,l) += (e - mu) d v_j F_T,l   ccT(j,l) += d v_j F_T,l
+// (grid: blocks x bands; each block writes its 36 partial sums).
+__global__ __launch_bounds__(256) void k_el_transport(int nk, int nb, const double *ens, const double *vels,
+                                                      const double *sym, const int *src, const double *rates, double T,
+                                                      double chempot, double qe, double *part) {
+    const int b = blockIdx.y;
+    double acc[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) acc[i] = 0.0;
+    for (int ik = blockIdx.x * 256 + threadIdx.x; ik < nk; ik += gridDim.x * 256) {
+        const int i = src[ik];
+        const double r = i >= 0 ? rates[(size_t)i * nb + b] : 0.0;
+        if (r == 0.0) continue;
+        const double e = ens[(size_t)ik * nb + b];
+        const double *v = vels + ((size_t)ik * nb + b) * 3;
+        double fE[3], fT[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            fE[d] = qe * v[d] * 1.0 / r;
+            fT[d] = 1.0 / T * v[d] * (e - chempot) / r;
+        }
+        if (sym) {
+            const double *S = sym + (size_t)ik * 9;
+            double gE[3], gT[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                gE[d] = S[d] * fE[0] + S[d + 3] * fE[1] + S[d + 6] * fE[2];
+                gT[d] = S[d] * fT[0] + S[d + 3] * fT[1] + S[d + 6] * fT[2];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                fE[d] = gE[d];
+                fT[d] = gT[d];
+            }
+        }
+        double f = 1.0 / (exp((e - chempot) / kBoltz / T) + 1.0);
+        f = f * (1.0 - f);
+        const double w = e - chempot;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                acc[j + 3 * l] += w * f * v[j] * fE[l];
+                acc[9 + j + 3 * l] += f * v[j] * fE[l];
+                acc[18 + j + 3 * l] += w * f * v[j] * fT[l];
+                acc[27 + j + 3 * l] += f * v[j] * fT[l];
+            }
+    }
+    __shared__ double ws[4][36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) {
+        double x = acc[i];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][i] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 36)
+        part[((size_t)b * gridDim.x + blockIdx.x) * 36 + threadIdx.x] =
+            ((ws[0][threadIdx.x] + ws[1][threadIdx.x]) + ws[2][threadIdx.x]) + ws[3][threadIdx.x];
+}
+
+// sums[b][i] = sum over the blocks of part[b][blk][i] (one thread per (b, i)).
+__global__ void k_el_transport_sum(int nb, int nblk, const double *part, double *sums) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb * 36) return;
+    const int b = t / 36, i = t % 36;
+    double x = 0.0;
+    for (int k = 0; k < nblk; ++k) x += part[((size_t)b * nblk + k) * 36 + i];
+    sums[t] = x;
 }
 
 TetraView view_of(const elph_tetra *t) {
@@ -471,6 +562,129 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
     } catch (const Error &err) {
         return ret_error(c, err);
     }
+    return ELPH_OK;
+}
+
+int elph_el_transport_create(elph_ctx *ctx, int64_t nk, int64_t nb, const double *ens, const double *vels,
+                             const int64_t *indexlist, int64_t nk_ibz, int64_t maxeq, const int64_t *nequiv,
+                             const int64_t *ibz2fbz_map, const double *symmetrizers, int64_t nrot,
+                             const double *crotations, double volume, const int64_t *mesh, int64_t spindeg,
+                             elph_el_transport **out) {
+    if (!ctx || !out || !ens || !vels || !indexlist || !nequiv || !ibz2fbz_map || !mesh || nk < 1 || nb < 1 ||
+        nk_ibz < 1 || maxeq < 1 || nrot < 0 || (nrot > 0 && !crotations))
+        return ELPH_EINVAL;
+    *out = nullptr;
+    auto *t = new elph_el_transport();
+    try {
+        HCHECK(hipSetDevice(ctx->device));
+        if (!(volume > 0.0)) throw Error{ELPH_EINVAL, "volume must be positive"};
+        if (nk > INT32_MAX / 36 || nb > 65535) throw Error{ELPH_EINVAL, "list too large"};
+        t->ctx = ctx;
+        t->nk = nk;
+        t->nb = nb;
+        t->nk_ibz = nk_ibz;
+        t->deg = spindeg;
+        t->volume = volume;
+        for (int d = 0; d < 3; ++d) {
+            if (mesh[d] < 1) throw Error{ELPH_EINVAL, "mesh components must be positive"};
+            t->mesh[d] = mesh[d];
+        }
+        t->crot.assign(crotations, crotations + 9 * nrot);
+        // IBZ point feeding each list point: the images ibz2fbz_map(ieq, i, 2)
+        // located in the list (src/bte.f90:984-998), later entries win
+        std::vector<int> src(nk, -1);
+        for (int64_t i = 0; i < nk_ibz; ++i) {
+            if (nequiv[i] < 0 || nequiv[i] > maxeq) throw Error{ELPH_EINVAL, "nequiv out of range"};
+            for (int64_t l = 0; l < nequiv[i]; ++l) {
+                const int64_t muxed = ibz2fbz_map[l + maxeq * (i + nk_ibz * 1)];
+                const int64_t *f = std::lower_bound(indexlist, indexlist + nk, muxed);
+                if (f == indexlist + nk || *f != muxed) throw Error{ELPH_EINVAL, "IBZ image not in the index list"};
+                src[f - indexlist] = (int)i;
+            }
+        }
+        std::vector<double> he(nk * nb), hv(nk * nb * 3);
+        for (int64_t k = 0; k < nk; ++k)
+            for (int64_t b = 0; b < nb; ++b) {
+                he[k * nb + b] = ens[k + nk * b];
+                for (int d = 0; d < 3; ++d) hv[(k * nb + b) * 3 + d] = vels[k + nk * (b + nb * d)];
+            }
+        hipStream_t s = ctx->stream;
+        t->ens.upload(he, s);
+        t->vels.upload(hv, s);
+        t->src.upload(src, s);
+        if (symmetrizers) {
+            t->symmetrize_k = true;
+            t->sym.upload(symmetrizers, nk * 9, s);
+        }
+        HCHECK(hipStreamSynchronize(s));
+    } catch (const Error &e) {
+        delete t;
+        return ret_error(ctx, e);
+    }
+    *out = t;
+    return ELPH_OK;
+}
+
+int elph_el_transport_rta(elph_el_transport *t, const double *rates_ibz, double T, double chempot, double *kappa0,
+                          double *sigmaS, double *sigma, double *alphabyT) {
+    if (!t || !rates_ibz) return ELPH_EINVAL;
+    elph_ctx *c = t->ctx;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        if (!(T > 0.0)) throw Error{ELPH_EINVAL, "temperature must be positive"};
+        const int nb = (int)t->nb;
+        const int nblk = (int)std::min<int64_t>(64, (t->nk + 255) / 256);
+        hipStream_t s = c->stream;
+        t->part.alloc((size_t)nb * nblk * 36);
+        t->sums.alloc((size_t)nb * 36);
+        hipLaunchKernelGGL(k_el_transport, dim3(nblk, nb), dim3(256), 0, s, (int)t->nk, nb, t->ens.p, t->vels.p,
+                           t->symmetrize_k ? t->sym.p : nullptr, t->src.p, rates_ibz, T, chempot, kQe, t->part.p);
+        HCHECK(hipGetLastError());
+        hipLaunchKernelGGL(k_el_transport_sum, dim3((nb * 36 + 255) / 256), dim3(256), 0, s, nb, nblk, t->part.p,
+                           t->sums.p);
+        HCHECK(hipGetLastError());
+        std::vector<double> h((size_t)nb * 36);
+        HCHECK(hipMemcpyAsync(h.data(), t->sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HCHECK(hipStreamSynchronize(s));
+        // unit factors of src/bz_sums.f90:540-556 (the source's 1.0e21 is a
+        // single-precision literal) and the tensor symmetrization of 594-608
+        const double fac = (double)1.0e21f / kBoltz / T / t->volume / (double)(t->mesh[0] * t->mesh[1] * t->mesh[2]);
+        const double A_cc_E = t->deg * fac, A_hc_E = -A_cc_E;
+        const double A_cc_T = -t->deg * kQe * fac, A_hc_T = t->deg * kQe * fac;
+        const double A[4] = {A_hc_E, A_cc_E, A_hc_T, A_cc_T};
+        double *dst[4] = {alphabyT, sigma, kappa0, sigmaS};
+        const int64_t nrot = (int64_t)t->crot.size() / 9;
+        for (int q = 0; q < 4; ++q) {
+            if (!dst[q]) continue;
+            for (int b = 0; b < nb; ++b) {
+                double m[9];
+                for (int i = 0; i < 9; ++i) m[i] = A[q] * h[(size_t)b * 36 + 9 * q + i];
+                if (nrot > 0) {
+                    double aux[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int64_t r = 0; r < nrot; ++r) {
+                        const double *C = &t->crot[9 * r];
+                        double mc[9];
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                mc[i + 3 * j] = m[i] * C[j] + m[i + 3] * C[j + 3] + m[i + 6] * C[j + 6];
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                aux[i + 3 * j] += C[i] * mc[3 * j] + C[i + 3] * mc[1 + 3 * j] + C[i + 6] * mc[2 + 3 * j];
+                    }
+                    for (int i = 0; i < 9; ++i) m[i] = aux[i] / (double)nrot;
+                }
+                // alpha/T: the E-field heat coefficient divided by T (src/bte.f90:275)
+                for (int i = 0; i < 9; ++i) dst[q][b + nb * i] = q == 0 ? m[i] / T : m[i];
+            }
+        }
+    } catch (const Error &err) {
+        return ret_error(c, err);
+    }
+    return ELPH_OK;
+}
+
+int elph_el_transport_destroy(elph_el_transport *t) {
+    delete t;
     return ELPH_OK;
 }
 

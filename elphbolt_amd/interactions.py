@@ -95,7 +95,8 @@ class IbzkPlan:
         DeviceBuffer of shape (nstates, nprocs)), plus the e-ph RTA rate of each
         state (calculate_el_rta_rates, 2957-3063).  Returns a dict of
         DeviceBuffers "Xplus", "Xminus" (THz), "istate_el", "istate_ph" (when
-        asked for) and the host array "rates" (THz, 0 outside the window)."""
+        asked for), the host array "rates" (THz, 0 outside the window) and the
+        same rates as the DeviceBuffer "rates_device"."""
         from .wannier import DeviceBuffer
         nst = istate_last - istate_first + 1
         shape = (nst, self.nprocs)
@@ -112,6 +113,7 @@ class IbzkPlan:
                                        ptr("Xplus"), ptr("Xminus"), ptr("istate_el"), ptr("istate_ph"), rates.ptr)
         self.wann._check(st, "ibzk X")
         out["rates"] = rates.numpy()
+        out["rates_device"] = rates
         return out
 
     def timing(self):

@@ -194,6 +194,15 @@ class DeviceBuffer:
         n = min(self.nbytes, other.nbytes)
         self.wann._check(self.wann.lib.elph_device_copy(self.wann.ctx, self.ptr, other.ptr, n), "device_copy")
 
+    def upload(self, host):
+        """Host-to-device copy of a numpy array of this buffer's size."""
+        host = np.ascontiguousarray(host, self.dtype)
+        if host.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        self.wann._check(self.wann.lib.elph_host_to_device(self.wann.ctx, self.ptr, host.ctypes.data_as(ct.c_void_p),
+                                                           self.nbytes), "host_to_device")
+        return self
+
     def row(self, i: int):
         """Row i of a 2-D buffer (one device-to-host copy of that row)."""
         n = int(np.prod(self.shape[1:]))

@@ -23,6 +23,7 @@ extern "C" {
 typedef struct elph_ctx elph_ctx;
 typedef struct elph_ibzk_plan elph_ibzk_plan;
 typedef struct elph_tetra elph_tetra;
+typedef struct elph_el_transport elph_el_transport;
 typedef struct elph_ibzq_plan elph_ibzq_plan;
 
 enum { ELPH_OK = 0, ELPH_EINVAL = 1, ELPH_EDEVICE = 2, ELPH_EIO = 3, ELPH_ENEGPHONON = 4, ELPH_EWINDOW = 5,
@@ -193,6 +194,29 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
                      int64_t istate_last, const double *g2, double *xplus, double *xminus, int64_t *istate_el,
                      int64_t *istate_ph, double *rates);
 
+/* Electron RTA transport (dragless_ebte_RTA, src/bte.f90:200-277) on the
+ * device.  create() takes the static inputs, host memory, reference layouts:
+ * ens (nk, nb) eV and vels (nk, nb, 3) km/s on the list indexlist (nk,
+ * ascending muxed), the IBZ wedge nequiv (nk_ibz) and ibz2fbz_map (maxeq,
+ * nk_ibz, 2), the per-k symmetrizers (3, 3, nk) applied to the field terms
+ * (NULL: none), the Cartesian rotations (3, 3, nrot) that symmetrize each
+ * band's tensor (nrot = 0: none), the primitive cell volume (nm^3), the
+ * mesh and the spin degeneracy. */
+int elph_el_transport_create(elph_ctx *ctx, int64_t nk, int64_t nb, const double *ens, const double *vels,
+                             const int64_t *indexlist, int64_t nk_ibz, int64_t maxeq, const int64_t *nequiv,
+                             const int64_t *ibz2fbz_map, const double *symmetrizers, int64_t nrot,
+                             const double *crotations, double volume, const int64_t *mesh, int64_t spindeg,
+                             elph_el_transport **out);
+/* From the IBZ e-ph rates (device memory, THz, state (ik-1)*nb + m as
+ * elph_ibzk_plan_X writes them for istate 1..nk_ibz*nb): the field terms of
+ * calculate_field_term (src/bte.f90:912-1011) for gradT and E, RTA
+ * responses, and calculate_transport_coeff (src/bz_sums.f90:484-611).
+ * Outputs host (nb, 3, 3) column-major, any may be NULL: kappa0 (W/m/K),
+ * sigmaS (A/m/K), sigma (1/Ohm/m), alphabyT (A/m/K). */
+int elph_el_transport_rta(elph_el_transport *t, const double *rates_ibz, double T, double chempot, double *kappa0,
+                          double *sigmaS, double *sigma, double *alphabyT);
+int elph_el_transport_destroy(elph_el_transport *t);
+
 /* Writes gk2.istate<i> (kind 'k': int64 count + nprocs doubles, the whole
  * buffer as src/interactions.f90:2337-2349 does) or gq2.istate<i> (kind 'q':
  * int64 count + count doubles, 1877-1889) for states with counts >= 0. */
@@ -225,6 +249,7 @@ int elph_state_free(elph_state *state);
 int elph_device_alloc(elph_ctx *ctx, int64_t bytes, void **dptr);
 int elph_device_free(elph_ctx *ctx, void *dptr);
 int elph_device_to_host(elph_ctx *ctx, void *host, const void *dptr, int64_t bytes);
+int elph_host_to_device(elph_ctx *ctx, void *dptr, const void *host, int64_t bytes);
 /* Waits for all work of the context's device (the reference's `sync all`
  * boundary after a driver, src/interactions.f90:2387). */
 int elph_ctx_synchronize(elph_ctx *ctx);

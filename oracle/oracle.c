@@ -1521,3 +1521,18 @@ void orc_transport_coeff(char field, double T, int64_t deg, double chempot, int6
             for (int i = 0; i < 9; ++i) { hc[ib + nb * i] = t[i]; cc[ib + nb * i] = u[i]; }
         }
 }
+
+/* bte.f90:241-244 (and 262-265): field_term(ik, :, :) =
+ * transpose(matmul(symmetrizers(:, :, ik), transpose(field_term(ik, :, :)))),
+ * i.e. F(ik, ib, :) <- S_ik F(ik, ib, :); symmetrizers (3, 3, nk). */
+void orc_symmetrize_field_term(int64_t nk, int64_t nb, const double *symmetrizers, double *field_term) {
+    for (int64_t ik = 0; ik < nk; ++ik) {
+        const double *S = &symmetrizers[9 * ik];
+        for (int64_t ib = 0; ib < nb; ++ib) {
+            double f[3], g[3];
+            for (int d = 0; d < 3; ++d) f[d] = field_term[ik + nk * (ib + nb * d)];
+            for (int i = 0; i < 3; ++i) g[i] = S[i] * f[0] + S[i + 3] * f[1] + S[i + 6] * f[2];
+            for (int d = 0; d < 3; ++d) field_term[ik + nk * (ib + nb * d)] = g[d];
+        }
+    }
+}

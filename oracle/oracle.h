@@ -150,6 +150,8 @@ void orc_transport_coeff(char field, double T, int64_t deg, double chempot, int6
                          const double *vels, double volume, const int64_t mesh[3], const double *response,
                          int64_t nrot, const double *crot, double *hc, double *cc);
 
+void orc_symmetrize_field_term(int64_t nk, int64_t nb, const double *symmetrizers, double *field_term);
+
 #ifdef __cplusplus
 }
 #endif

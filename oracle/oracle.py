@@ -260,6 +260,14 @@ def field_term(field, nequiv, ibz2fbz_map, T, chempot, ens, vels, rates_ibz, ind
     return out
 
 
+def symmetrize_field_term(symmetrizers, field):
+    """bte.f90:241-244: F(ik, ib, :) <- S_ik F(ik, ib, :)."""
+    F = np.array(field, np.float64, order="F")
+    S = _f64(symmetrizers)
+    lib().orc_symmetrize_field_term(ct.c_int64(F.shape[0]), ct.c_int64(F.shape[1]), _p(S), _p(F))
+    return F
+
+
 def transport_coeff(field, T, deg, chempot, ens, vels, volume, mesh, response, crot=None):
     """calculate_transport_coeff for electrons (bz_sums.f90:484-611):
     (trans_coeff_hc, trans_coeff_cc), each (nb, 3, 3); crot (3, 3, nrot)."""

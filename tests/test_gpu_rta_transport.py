@@ -1,0 +1,128 @@
+"""GPU electron RTA transport (field terms + calculate_transport_coeff, through
+the C ABI) against the oracle, and the transport-delta proxy: sigma and
+sigma*S from the whole device chain (g2 -> X+/- -> rates -> transport) against
+the oracle's whole chain, FP64 within 1e-10 and mixed precision within 1e-4."""
+import itertools
+
+import numpy as np
+import pytest
+
+from elphbolt_amd import interactions, synth
+from elphbolt_amd.bte import ElectronRTA
+from elphbolt_amd.wannier import DeviceBuffer, Wannier
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cubic_group():
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product([1, -1], repeat=3):
+            M = np.zeros((3, 3))
+            for i, p in enumerate(perm):
+                M[i, p] = signs[i]
+            out.append(M)
+    return np.asfortranarray(np.stack(out, axis=2))
+
+
+def _oracle_transport(neq, m, T, mu, ens, vels, rates_ibz, il, vol, mesh, sym, crot, deg=2):
+    out = {}
+    for field in ("E", "T"):
+        F = O.field_term(field, neq, m, T, mu, ens, vels, rates_ibz, il)
+        if sym is not None:
+            F = O.symmetrize_field_term(sym, F)
+        hc, cc = O.transport_coeff(field, T, deg, mu, ens, vels, vol, mesh, F, crot=crot)
+        if field == "E":
+            out["alphabyT"], out["sigma"] = hc / T, cc
+        else:
+            out["kappa0"], out["sigmaS"] = hc, cc
+    return out
+
+
+def _rel_tensor(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("with_sym", [False, True])
+def test_transport_parity(with_sym):
+    print(f"transport parity sym={with_sym}", flush=True)
+    mat = synth.tiny(nb=2, natoms=1, ws=3, wv=4, seed=51)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    mesh = np.array(mat.wvmesh)
+    nk = int(np.prod(mesh))
+    il = np.arange(1, nk + 1, dtype=np.int64)
+    irr, neq, m = O.irred_wedge(mesh, qr, nsr, maps=True)
+    rng = np.random.default_rng(9)
+    nb = 3
+    ens = rng.normal(0.0, 0.05, (nk, nb))
+    vels = rng.normal(0.0, 200.0, (nk, nb, 3))
+    rates = rng.uniform(0.5, 5.0, (len(irr), nb))
+    rates[1, 2] = 0.0
+    sym = rng.normal(size=(3, 3, nk)) if with_sym else None
+    crot = _cubic_group() if with_sym else None
+    T, mu, vol = 300.0, 0.01, 0.041
+    ref = _oracle_transport(neq, m, T, mu, ens, vels, rates, il, vol, mesh, sym, crot)
+    w = Wannier(mat)
+    tr = ElectronRTA(w, ens, vels, il, neq, m, vol, mesh, 2, symmetrizers=sym, crotations=crot)
+    rd = DeviceBuffer(w, (len(irr) * nb,)).upload(rates.ravel())  # state (ik-1)*nb + m
+    got = tr.rta(rd, T, mu)
+    for k in ("kappa0", "sigmaS", "sigma", "alphabyT"):
+        err = _rel_tensor(got[k], ref[k])
+        print(f"  {k}: {err:.2e}", flush=True)
+        assert err < 1e-12, k
+
+
+def _chain_material():
+    return synth.tiny(nb=3, natoms=1, ws=3, wv=6, seed=52, bands_in_window=2)
+
+
+@pytest.fixture(scope="module")
+def chain():
+    mat = _chain_material()
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    mesh = np.array(el.wvmesh)
+    irr, neq, m = O.irred_wedge(mesh, qr, nsr, indexlist=el.indexlist, maps=True)
+    assert np.array_equal(irr, el.indexlist_irred)
+    kv = np.array([O.demux_vector(i, mesh, 0) for i in el.indexlist], float) / mesh
+    _, _, vels = o.el_wann(kv)
+    nst = el.nwv_irred * mat.nb
+    T, mu = 300.0, float(mat.enref) + 0.02
+    tet_o = O.Tetrahedra(el.wvmesh, el.ens, indexlist=el.indexlist)
+    c_o, g_o = O.ibzk_g2(o, el, ph, 1, 1, nst)
+    xp, xm, _, _ = O.ibzk_X(o, el, ph, tet_o, c_o, g_o, T, mu)
+    rates = O.el_rta_rates(c_o, xp, xm).reshape(el.nwv_irred, mat.nb)
+    vol = abs(np.linalg.det(np.asarray(mat.lattvecs)))
+    ref = _oracle_transport(neq, m, T, mu, el.ens, vels, rates, el.indexlist, vol, mesh, None, None)
+    return dict(mat=mat, el=el, ph=ph, neq=neq, m=m, vels=vels, T=T, mu=mu, vol=vol, ref=ref, nst=nst)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_transport_delta_proxy(chain, precision):
+    mat, el, ph = chain["mat"], chain["el"], chain["ph"]
+    nst, T, mu = chain["nst"], chain["T"], chain["mu"]
+    print(f"transport proxy {precision}: {nst} IBZ states", flush=True)
+    w = Wannier(mat)
+    plan = interactions.IbzkPlan(w, el, ph, precision=precision)
+    g2d = DeviceBuffer(w, (nst, plan.nprocs))
+    plan.run(1, nst, out=g2d)
+    tet = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
+    x = plan.run_X(tet, T, mu, 1, nst, g2d, maps=False, probabilities=False)
+    tr = ElectronRTA(w, el.ens, chain["vels"], el.indexlist, chain["neq"], chain["m"], chain["vol"], el.wvmesh, 2)
+    got = tr.rta(x["rates_device"], T, mu)
+    ref = chain["ref"]
+    sig, sig_ref = ElectronRTA.scalar(got["sigma"]), ElectronRTA.scalar(ref["sigma"])
+    ss, ss_ref = ElectronRTA.scalar(got["sigmaS"]), ElectronRTA.scalar(ref["sigmaS"])
+    assert sig_ref > 0
+    d_sig, d_ss = abs(sig / sig_ref - 1), abs(ss / ss_ref - 1)
+    d_S = abs((ss / sig) / (ss_ref / sig_ref) - 1)
+    print(f"  sigma {sig:.6e} (oracle {sig_ref:.6e}) rel {d_sig:.2e}; sigmaS rel {d_ss:.2e}; S rel {d_S:.2e}",
+          flush=True)
+    tol = 1e-10 if precision == "fp64" else 1e-4
+    assert d_sig < tol and d_ss < tol and d_S < tol
+    for k in ("kappa0", "sigmaS", "sigma", "alphabyT"):
+        assert _rel_tensor(got[k], ref[k]) < tol, k
+    plan.close()

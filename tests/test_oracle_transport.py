@@ -321,3 +321,14 @@ def test_field_term_covers_the_wedge_images():
                 want = 0.0 if rates[i, b] == 0 else 1.0
                 assert np.allclose(F[ik, b] * rates[i, b] / qe if want else F[ik, b], vels[ik, b] if want else 0.0,
                                    rtol=1e-14)
+
+
+def test_symmetrize_field_term():
+    rng = np.random.default_rng(8)
+    F = rng.normal(size=(10, 2, 3))
+    I = np.repeat(np.eye(3)[:, :, None], 10, axis=2)
+    assert np.array_equal(O.symmetrize_field_term(I, F), F)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    R = np.repeat(Q[:, :, None], 10, axis=2)
+    G = O.symmetrize_field_term(R, F)
+    assert np.allclose(G, np.einsum("ij,kbj->kbi", Q, F), rtol=1e-14, atol=1e-14)
