@@ -240,6 +240,55 @@ def serving_bench(args):
 BATCH_SEED = 0xE1B05000
 
 
+def transport_bench(args, w, sym, el, plan, g2, nst):
+    """The consumers of one step's g2 rows (all IBZ states, device-resident):
+    calculate_eph_interaction_ibzk('X') with the tetrahedron delta, the e-ph
+    RTA rates, then the RTA transport coefficients."""
+    from elphbolt_amd import bz, interactions
+    from elphbolt_amd.bte import ElectronRTA
+    T, mu = 300.0, float(el.enref)
+    t0 = time.perf_counter()
+    tet = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
+    t_tet = time.perf_counter() - t0
+    bufs = plan.run_X(tet, T, mu, 1, nst, g2)          # allocates the outputs; untimed
+    log(f"[bench] X: tetrahedra {t_tet:.2f} s, first pass {plan.timing()['dominant_ms']:.2f} ms")
+    kms, walls = [], []
+    for i in range(args.transport_steps):
+        t = time.perf_counter()
+        plan.run_X(tet, T, mu, 1, nst, g2, buffers=bufs)
+        walls.append(time.perf_counter() - t)
+        kms.append(plan.timing()["dominant_ms"])
+        log(f"[bench] X step {i}: kernel {kms[-1]:.3f} ms, call {walls[-1] * 1e3:.2f} ms")
+    s = np.arange(nst)
+    act = np.abs(el.ens_irred[s // el.nb, s % el.nb] - el.enref) <= el.fsthick
+    procs = int(act.sum()) * plan.nprocs
+    k_s = float(np.mean(kms)) * 1e-3
+    algo = procs * (8 + 16 + 16)    # |g|^2 read, X+ and X- written, the two state maps written
+    mesh = np.array(el.wvmesh)
+    i0 = el.indexlist - 1
+    kv = np.stack([i0 % mesh[0], (i0 // mesh[0]) % mesh[1], i0 // (mesh[0] * mesh[1])], axis=1) / mesh
+    _, _, vels = w.el_wann(kv, velocities=True, evecs=False)
+    irr, neq, m = bz.irred_wedge(mesh, sym, indexlist=el.indexlist, maps=True)
+    assert np.array_equal(irr, el.indexlist_irred)
+    vol = abs(float(np.linalg.det(np.asarray(w.mat.lattvecs))))
+    tr = ElectronRTA(w, el.ens, vels, el.indexlist, neq, m, vol, mesh, 2)
+    tr.rta(bufs["rates_device"], T, mu)
+    t = time.perf_counter()
+    coef = tr.rta(bufs["rates_device"], T, mu)
+    t_rta = time.perf_counter() - t
+    sig, sigS = ElectronRTA.scalar(coef["sigma"]), ElectronRTA.scalar(coef["sigmaS"])
+    log(f"[bench] RTA transport {t_rta * 1e3:.2f} ms: sigma {sig:.4e} 1/Ohm/m, sigmaS {sigS:.4e} A/m/K")
+    ach = algo / k_s / 1e9
+    return {"T_K": T, "chempot_eV": mu, "states": nst, "processes": procs,
+            "X_kernel_ms": round(k_s * 1e3, 3), "X_call_ms": round(float(np.mean(walls)) * 1e3, 3),
+            "processes_per_s": procs / k_s,
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_ibzk_X",
+                         "bytes_per_launch": algo},
+            "rta_transport_ms": round(t_rta * 1e3, 3), "tetrahedra_s": round(t_tet, 3),
+            "sigma": sig, "sigmaS": sigS, "seebeck_V_per_K": sigS / sig if sig else None}
+
+
 def _batched_material(i, budget):
     from elphbolt_amd import synth
     return synth.multi_material(i, budget_bytes=budget, bands_in_window=4.0)
@@ -375,6 +424,8 @@ def main():
                          "device-to-device copy of --calibration-copy-gb GiB, for hardware-counter runs")
     ap.add_argument("--calibration-copy-gb", type=float, default=1.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-transport", action="store_true")
+    ap.add_argument("--transport-steps", type=int, default=3)
     ap.add_argument("--batched-materials", type=int, default=64)
     ap.add_argument("--batched-budget-gb", type=float, default=2.0)
     ap.add_argument("--batched-out-gb", type=float, default=1.5)
@@ -502,6 +553,9 @@ def main():
                       "vertices_per_step": res_mx["vertices"] // args.steps},
            "mixed_vs_fp64": round(res_mx["value"] / res_64["value"], 3), "error_vs_fp64": err,
            "fp64": fp64, "roofline": roofline}
+    if not args.no_transport and nblocks == 1:
+        res["transport"] = transport_bench(args, w, sym, el, plans["fp64"], out, kb * nb)
+        log_mem("after transport")
     if not args.no_batched:
         log_mem("before batched")
         res["batched"] = batched_bench(args, gen)

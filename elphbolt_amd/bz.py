@@ -42,19 +42,23 @@ class Symmetry:
         self.qrotations = qr[:9 * nsr.value].copy()
 
 
-def irred_wedge(mesh, sym: Symmetry, indexlist=None):
-    """`find_irred_wedge` (src/symmetry.f90:303-451): (indexlist_irred, nequiv)."""
+def irred_wedge(mesh, sym: Symmetry, indexlist=None, maps=False):
+    """`find_irred_wedge` (src/symmetry.f90:303-451): (indexlist_irred, nequiv)
+    and, with maps, ibz2fbz_map (nsymm_rot, nk_irred, 2) column-major."""
     lib = _lib.load()
     m = i64(mesh)
     nlist = int(np.prod(m)) if indexlist is None else len(indexlist)
     irr = np.zeros(nlist, np.int64)
     neq = np.zeros(nlist, np.int64)
     il = i64(indexlist) if indexlist is not None else None
+    mp = np.zeros((sym.nsymm_rot, nlist, 2), np.int64, order="F") if maps else None
     n = ct.c_int64()
     st = lib.elph_irred_wedge(_p(m, c_i64p), sym.nsymm_rot, _p(sym.qrotations), nlist, _p(il, c_i64p),
-                              ct.byref(n), _p(irr, c_i64p), _p(neq, c_i64p), None)
+                              ct.byref(n), _p(irr, c_i64p), _p(neq, c_i64p), _p(mp, c_i64p) if maps else None)
     if st != 0:
         raise ElphError(st, "Severe error: Could not find irreducible wedge.")
+    if maps:
+        return irr[:n.value].copy(), neq[:n.value].copy(), np.asfortranarray(mp[:, :n.value, :])
     return irr[:n.value].copy(), neq[:n.value].copy()
 
 

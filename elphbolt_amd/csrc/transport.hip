@@ -31,6 +31,11 @@ struct elph_tetra {
     DBuf<int> tcount, tmap;
     DBuf<d4> tev;
     DBuf<dv2> erange;
+    // tabulated delta function (see k_delta_table)
+    DBuf<double> dbp;
+    DBuf<d4> dco;
+    DBuf<dv2> dmh;
+    DBuf<int> dnpc;
 };
 
 // Static inputs of the electron RTA transport on the device.
@@ -118,20 +123,32 @@ __device__ __noinline__ double tetra_weight(int iv, double e, double e1, double 
     return w;
 }
 
+// Intervals per (list point, band) of the tabulated delta function: the
+// distinct vertex energies of a point's (at most 24) tetrahedra split the
+// energy axis into at most 95 pieces, on each of which every vertex weight is
+// one cubic polynomial of e.
+constexpr int kBpSlots = 128;  // breakpoints per (point, band), padded with +inf
+constexpr int kMaxPieces = 96;
+
 struct TetraView {
     const int *tcount, *tmap;
     const d4 *tev;
     const dv2 *erange;
     int nb;
     double numtetra;
+    // tabulated sum of vertex weights: breakpoints bp[pair][kBpSlots], number
+    // of pieces npc[pair], and per piece the cubic c0 + c1 x + c2 x^2 + c3 x^3 in
+    // x = (e - mid) * ih (co[pair][piece], mh[pair][piece] = (mid, ih))
+    const double *bp;
+    const d4 *co;
+    const dv2 *mh;
+    const int *npc;
 };
 
-// delta_fn_tetra(e, r + 1, n + 1) (src/delta.f90:61-225): sum of the vertex
-// weights over the tetrahedra of list point r, zero below 1e-12, divided by
-// 6 prod(mesh).  Energies outside every tetrahedron of (r, n) return 0 at once.
-__device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int r, int n) {
-    const dv2 rg = t.erange[(size_t)r * t.nb + n];
-    if (e < rg.x || e > rg.y) return 0.0;
+// Sum over the tetrahedra of list point r of the weight of r's vertex at
+// energy e, band n: the loop of src/delta.f90:92-216 before its clamp and
+// normalization.
+__device__ double weight_sum(const TetraView &t, double e, int r, int n) {
     double d = 0.0;
     const int num = t.tcount[r];
     for (int j = 0; j < num; ++j) {
@@ -140,8 +157,119 @@ __device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int 
         if (e < v.x || e > v.w) continue;
         d += tetra_weight((code & 3) + 1, e, v.x, v.y, v.z, v.w);
     }
+    return d;
+}
+
+// delta_fn_tetra(e, r + 1, n + 1) (src/delta.f90:61-225) from the table: the
+// piece holding e (binary search over the padded breakpoints), its cubic,
+// zero below 1e-12, divided by 6 prod(mesh).  Zero outside the energy range
+// of r's tetrahedra.  Points whose tetrahedra all have one energy (no piece)
+// are summed directly.
+__device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int r, int n) {
+    const size_t pr = (size_t)r * t.nb + n;
+    const dv2 rg = t.erange[pr];
+    if (e < rg.x || e > rg.y) return 0.0;
+    const int m = t.npc[pr];
+    double d;
+    if (m > 0) {
+        const double *bp = t.bp + pr * kBpSlots;
+        int lo = 0;
+#pragma unroll
+        for (int step = kBpSlots / 2; step > 0; step >>= 1)
+            if (bp[lo + step] <= e) lo += step;
+        const int j = min(lo, m - 1);
+        const d4 c = t.co[pr * kMaxPieces + j];
+        const dv2 q = t.mh[pr * kMaxPieces + j];
+        const double x = (e - q.x) * q.y;
+        d = ((c.w * x + c.z) * x + c.y) * x + c.x;
+    } else {
+        d = weight_sum(t, e, r, n);
+    }
     if (d < 1.0e-12) d = 0.0;
     return d / t.numtetra;
+}
+
+struct ChebFit {
+    double minv[16];  // inverse Vandermonde of the nodes, row-major
+    double node[4];   // cos((2k+1) pi / 8)
+};
+
+// Builds the table, one thread per (point, band): the distinct vertex
+// energies in ascending order, and on each piece [a, b] the cubic through the
+// exact weight sums at the four Chebyshev nodes of the piece.
+__global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, ChebFit f, double *bp, d4 *co, dv2 *mh,
+                                                     int *npc) {
+    const int pr = blockIdx.x * 256 + threadIdx.x;
+    if (pr >= npairs) return;
+    const int r = pr / t.nb, n = pr - r * t.nb;
+    double b[kMaxPieces];
+    int nbp = 0;
+    const int num = t.tcount[r];
+    for (int j = 0; j < num; ++j) {
+        const d4 v = t.tev[(size_t)(t.tmap[r * 24 + j] >> 2) * t.nb + n];
+        const double ev[4] = {v.x, v.y, v.z, v.w};
+        for (int i = 0; i < 4; ++i) {
+            // insertion into the ascending list of distinct values
+            double x = ev[i];
+            int k = nbp;
+            bool dup = false;
+            while (k > 0 && b[k - 1] >= x) {
+                if (b[k - 1] == x) dup = true;
+                --k;
+            }
+            if (dup) continue;
+            for (int q = nbp; q > k; --q) b[q] = b[q - 1];
+            b[k] = x;
+            ++nbp;
+        }
+    }
+    double *bo = bp + (size_t)pr * kBpSlots;
+    for (int i = 0; i < kBpSlots; ++i) bo[i] = i < nbp ? b[i] : INFINITY;
+    const int m = nbp > 1 ? nbp - 1 : 0;
+    npc[pr] = m;
+    for (int j = 0; j < m; ++j) {
+        const double a = b[j], z = b[j + 1];
+        const double mid = 0.5 * (a + z), half = 0.5 * (z - a);
+        double sv[4];
+        for (int k = 0; k < 4; ++k) sv[k] = weight_sum(t, mid + half * f.node[k], r, n);
+        double c[4];
+        for (int i = 0; i < 4; ++i)
+            c[i] = f.minv[4 * i] * sv[0] + f.minv[4 * i + 1] * sv[1] + f.minv[4 * i + 2] * sv[2] + f.minv[4 * i + 3] * sv[3];
+        co[(size_t)pr * kMaxPieces + j] = d4{c[0], c[1], c[2], c[3]};
+        mh[(size_t)pr * kMaxPieces + j] = dv2{mid, 1.0 / half};
+    }
+}
+
+// Inverse of V[k][i] = x_k^i for the four Chebyshev nodes (Gauss-Jordan with
+// partial pivoting; V is well conditioned).
+ChebFit cheb_fit() {
+    ChebFit f;
+    double a[4][8];
+    for (int k = 0; k < 4; ++k) {
+        f.node[k] = std::cos((2 * k + 1) * (4.0 * std::atan(1.0)) / 8.0);
+        double p = 1.0;
+        for (int i = 0; i < 4; ++i) {
+            a[k][i] = p;
+            a[k][4 + i] = i == k ? 1.0 : 0.0;
+            p *= f.node[k];
+        }
+    }
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        for (int j = 0; j < 8; ++j) std::swap(a[c][j], a[piv][j]);
+        const double d = a[c][c];
+        for (int j = 0; j < 8; ++j) a[c][j] /= d;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                const double g = a[r][c];
+                for (int j = 0; j < 8; ++j) a[r][j] -= g * a[c][j];
+            }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) f.minv[4 * i + k] = a[i][4 + k];
+    return f;
 }
 
 __global__ void k_delta_samples(TetraView t, int n, const double *e, const int64_t *ik, const int64_t *ib,
@@ -320,6 +448,10 @@ TetraView view_of(const elph_tetra *t) {
     v.tmap = t->tmap.p;
     v.tev = t->tev.p;
     v.erange = t->erange.p;
+    v.bp = t->dbp.p;
+    v.co = t->dco.p;
+    v.mh = t->dmh.p;
+    v.npc = t->dnpc.p;
     v.nb = (int)t->nb;
     v.numtetra = (double)(t->mesh[0] * t->mesh[1] * t->mesh[2] * 6);
     return v;
@@ -423,6 +555,14 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
         t->tmap.upload(hmap, s);
         t->tev.upload(hev, s);
         t->erange.upload(hr, s);
+        const int64_t npairs = nk * nb;
+        t->dbp.alloc(npairs * kBpSlots);
+        t->dco.alloc(npairs * kMaxPieces);
+        t->dmh.alloc(npairs * kMaxPieces);
+        t->dnpc.alloc(npairs);
+        hipLaunchKernelGGL(k_delta_table, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, view_of(t),
+                           (int)npairs, cheb_fit(), t->dbp.p, t->dco.p, t->dmh.p, t->dnpc.p);
+        HCHECK(hipGetLastError());
         HCHECK(hipStreamSynchronize(s));
     } catch (const Error &e) {
         delete t;

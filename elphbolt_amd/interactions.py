@@ -89,24 +89,35 @@ class IbzkPlan:
         return counts[:nst], out
 
     def run_X(self, tet: "Tetrahedra", T: float, chempot: float, istate_first: int, istate_last: int, g2,
-              maps: bool = True, probabilities: bool = True):
+              maps: bool = True, probabilities: bool = True, buffers=None):
         """calculate_eph_interaction_ibzk('X') (src/interactions.f90:2278-2354)
         on the device g2 rows that run() wrote for the same states (a
         DeviceBuffer of shape (nstates, nprocs)), plus the e-ph RTA rate of each
         state (calculate_el_rta_rates, 2957-3063).  Returns a dict of
         DeviceBuffers "Xplus", "Xminus" (THz), "istate_el", "istate_ph" (when
         asked for), the host array "rates" (THz, 0 outside the window) and the
-        same rates as the DeviceBuffer "rates_device"."""
+        same rates as the DeviceBuffer "rates_device".  `buffers`: a dict of
+        that form from an earlier call on as many states, reused in place."""
         from .wannier import DeviceBuffer
         nst = istate_last - istate_first + 1
         shape = (nst, self.nprocs)
         out = {}
-        if probabilities:
+        if buffers is not None:
+            out = {k: v for k, v in buffers.items() if k in ("Xplus", "Xminus", "istate_el", "istate_ph")}
+        if probabilities and "Xplus" not in out:
             out["Xplus"], out["Xminus"] = DeviceBuffer(self.wann, shape), DeviceBuffer(self.wann, shape)
-        if maps:
+        if maps and "istate_el" not in out:
             out["istate_el"] = DeviceBuffer(self.wann, shape, np.int64)
             out["istate_ph"] = DeviceBuffer(self.wann, shape, np.int64)
-        rates = DeviceBuffer(self.wann, (nst,))
+        if not probabilities:
+            out.pop("Xplus", None), out.pop("Xminus", None)
+        if not maps:
+            out.pop("istate_el", None), out.pop("istate_ph", None)
+        for v in out.values():
+            if v.shape != shape:
+                raise ValueError("reused buffers have another shape")
+        rates = buffers["rates_device"] if buffers is not None and "rates_device" in buffers else \
+            DeviceBuffer(self.wann, (nst,))
         ptr = lambda k: out[k].ptr if k in out else None
         g2p, _ = _dev_ptr(g2)
         st = self.lib.elph_ibzk_plan_X(self.plan, tet.h, float(T), float(chempot), istate_first, istate_last, g2p,

@@ -1,7 +1,8 @@
 """GPU electron RTA transport (field terms + calculate_transport_coeff, through
 the C ABI) against the oracle, and the transport-delta proxy: sigma and
 sigma*S from the whole device chain (g2 -> X+/- -> rates -> transport) against
-the oracle's whole chain, FP64 within 1e-10 and mixed precision within 1e-4."""
+the oracle's whole chain, FP64 within 1e-8 (the tabulated delta function
+agrees with the direct sum to 1e-9) and mixed precision within 1e-4."""
 import itertools
 
 import numpy as np
@@ -121,7 +122,7 @@ def test_transport_delta_proxy(chain, precision):
     d_S = abs((ss / sig) / (ss_ref / sig_ref) - 1)
     print(f"  sigma {sig:.6e} (oracle {sig_ref:.6e}) rel {d_sig:.2e}; sigmaS rel {d_ss:.2e}; S rel {d_S:.2e}",
           flush=True)
-    tol = 1e-10 if precision == "fp64" else 1e-4
+    tol = 1e-8 if precision == "fp64" else 1e-4
     assert d_sig < tol and d_ss < tol and d_S < tol
     for k in ("kappa0", "sigmaS", "sigma", "alphabyT"):
         assert _rel_tensor(got[k], ref[k]) < tol, k

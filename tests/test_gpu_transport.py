@@ -1,8 +1,12 @@
 """GPU parity of the g2 consumers (through the C ABI) against the oracle:
-tetrahedra tables (exact), delta_fn_tetra on the device (1e-12 relative),
-X+/- of calculate_eph_interaction_ibzk('X') on the device g2 rows with the
-oracle fed the same rows (1e-11 relative above 1e-9 * max, maps exact), the
-RTA rates (1e-11), and X from mixed-precision g2 against X from the oracle's
+tetrahedra tables (exact); delta_fn_tetra on the device, which evaluates the
+weight sum of a (k', band) as one cubic per piece between consecutive vertex
+energies, fitted through the exact sums at four Chebyshev nodes (agreement
+with the direct sum: 1e-12 of the largest value absolute -- the rounding of
+the case expressions themselves on narrow pieces -- and 1e-9 relative above
+1e-6 * max); X+/- of calculate_eph_interaction_ibzk('X') on the device g2 rows
+with the oracle fed the same rows (1e-9 relative above 1e-6 * max, maps
+exact), the RTA rates (1e-9), and X from mixed-precision g2 against X from the oracle's
 own g2 (1e-5 relative above 1e-6 * max, the tolerance of |g|^2 itself;
 branch sums of a (k', n) block where phonons are interpolated off the
 phonon mesh)."""
@@ -64,8 +68,10 @@ def test_delta_on_device(case):
     d_g = case["tet_g"].delta(e, ik, ib)
     d_o = np.array([o.delta(e[i], ik[i], ib[i]) for i in range(n)])
     assert (d_o > 0).sum() > n // 10
-    assert np.array_equal(d_g == 0, d_o == 0)
-    assert _rel(d_g, d_o, 1e-9) < 1e-12
+    big = d_o.max()
+    assert np.max(np.abs(d_g - d_o)) < 1e-12 * big
+    assert np.all(((d_g == 0) == (d_o == 0)) | (np.maximum(d_g, d_o) < 1e-12 * big))
+    assert _rel(d_g, d_o, 1e-6) < 1e-9
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -88,12 +94,12 @@ def test_X_and_rates(case, precision):
     assert np.array_equal(out["istate_el"].numpy()[act], se_o[act])
     assert np.array_equal(out["istate_ph"].numpy()[act], sp_o[act])
     assert (xp_o[act] > 0).any() and (xm_o[act] > 0).any()
-    err = max(_rel(xp, xp_o[act], 1e-9), _rel(xm, xm_o[act], 1e-9))
+    err = max(_rel(xp, xp_o[act], 1e-6), _rel(xm, xm_o[act], 1e-6))
     print(f"  X same g2: max rel {err:.2e}", flush=True)
-    assert err < 1e-11
+    assert err < 1e-9
     r_o = O.el_rta_rates(counts, xp_o, xm_o)
     assert np.all(out["rates"][~act] == 0.0)
-    assert _rel(out["rates"], r_o, 1e-9) < 1e-11
+    assert _rel(out["rates"], r_o, 1e-9) < 1e-9
     # against the oracle's whole chain (its own g2)
     c_o, g_o = O.ibzk_g2(case["o"], el, ph, mesh_ref, 1, last)
     assert np.array_equal(c_o, counts)
@@ -103,7 +109,7 @@ def test_X_and_rates(case, precision):
     # each side's own diagonalisation: within degenerate branches (same
     # energy, so the same occupation and delta) only the sum over the
     # branches of a (k', n) block is fixed, as for |g|^2 itself.
-    tol = 1e-11 if precision == "fp64" else 1e-5
+    tol = 1e-9 if precision == "fp64" else 1e-5
     xp_r, xm_r, g_r = xp_oo[act], xm_oo[act], g_o[act]
     if mesh_ref > 1:
         tol = max(tol, 1e-8)
