@@ -156,10 +156,13 @@ void launch_polar_lp(hipStream_t st, int n, int nbr, int div, const int *ridx, c
 // Scratch of the consumers of an ibzk plan's g2 rows (transition
 // probabilities and rates), owned by the plan.
 struct ConsumerScratch {
-    DBuf<int> vrow;      // [nprocs] k' row of each process of a row
-    DBuf<int4> st_k;     // per state: k (electron mesh ints), in-window flag
-    DBuf<double> st_e;   // per state: electron energy
+    DBuf<int2> vmap;     // [nprocs] (k' row r, band n << 8 | branch s) of each process of a row
+    DBuf<int4> st_k;     // per in-window state: k (electron mesh ints)
+    DBuf<double> st_e;   // per in-window state: electron energy
+    DBuf<int> st_row;    // per in-window state: its row among the call's states
+    DBuf<int> st_idx;    // per state of the call: index among the in-window ones, -1 outside
     DBuf<double> partial;
+    DBuf<double> bose;   // Bose factors of the phonon modes
 };
 
 // The k'-row tables of an ibzk plan, as its consumers read them.  A g2 row is
@@ -178,7 +181,7 @@ struct IbzkRows {
     double enref, fsthick;
     ConsumerScratch *scr;
 };
-// Builds the vrow table on first use (synchronous).
+// Builds the vmap table on first use (synchronous).
 IbzkRows ibzk_rows(elph_ibzk_plan *p);
 
 }  // namespace elph

@@ -3445,16 +3445,20 @@ IbzkRows ibzk_rows(elph_ibzk_plan *p) {
     v.enref = p->enref;
     v.fsthick = p->fsthick;
     v.scr = &p->cons;
-    if (!p->cons.vrow.p && p->nprocs > 0) {
+    if (!p->cons.vmap.p && p->nprocs > 0) {
+        if (p->nbr > 255) throw Error{ELPH_EINVAL, "more than 255 phonon branches"};
         std::vector<int64_t> offs(p->nwv);
-        std::vector<int> nw(p->nwv);
+        std::vector<int> nw(p->nwv), wb(p->nwv * p->nb);
         HCHECK(hipStreamSynchronize(p->ctx->stream));
         HCHECK(hipMemcpy(offs.data(), p->offs.p, p->nwv * sizeof(int64_t), hipMemcpyDeviceToHost));
         HCHECK(hipMemcpy(nw.data(), p->nwin.p, p->nwv * sizeof(int), hipMemcpyDeviceToHost));
-        std::vector<int> vr(p->nprocs);
+        HCHECK(hipMemcpy(wb.data(), p->wband.p, wb.size() * sizeof(int), hipMemcpyDeviceToHost));
+        std::vector<int2> vm(p->nprocs);
         for (int64_t r = 0; r < p->nwv; ++r)
-            for (int64_t j = 0; j < (int64_t)nw[r] * p->nbr; ++j) vr[offs[r] + j] = (int)r;
-        p->cons.vrow.upload(vr, p->ctx->stream);
+            for (int64_t j = 0; j < nw[r]; ++j)
+                for (int64_t sb = 0; sb < p->nbr; ++sb)
+                    vm[offs[r] + j * p->nbr + sb] = make_int2((int)r, (wb[r * p->nb + j] << 8) | (int)sb);
+        p->cons.vmap.upload(vm, p->ctx->stream);
         HCHECK(hipStreamSynchronize(p->ctx->stream));
     }
     return v;

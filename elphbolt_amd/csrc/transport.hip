@@ -7,8 +7,10 @@
 // Layout in HBM:
 //   tetrahedra  tcount[nk], tmap[nk][24] = 4 (it - 1) + (iv - 1) (it the
 //               tetrahedron, iv the wave vector's vertex label in it), tev[ntet][nb]
-//               the four vertex energies sorted (32 bytes), erange[nk][nb] the lowest
-//               and highest vertex energy over the wave vector's tetrahedra;
+//               the four vertex energies sorted (32 bytes);
+//   delta table per (wave vector, band): a 128-byte header (energy range, piece
+//               count, search starts of 64 uniform bins) and up to 96 pieces of 64
+//               bytes (bounds, cubic coefficients of the vertex-weight sum);
 //   processes   one thread per process of a g2 row (state, k', band, branch): reads
 //               its |g|^2 (8 B), writes X+ and X- (16 B) and optionally the final
 //               electron and phonon state (16 B).
@@ -21,6 +23,23 @@
 
 using namespace elph;
 
+constexpr int kMaxPieces = 96;
+constexpr int kBins = 64;  // uniform bins over a (point, band)'s energy range
+
+// Table header of one (point, band), one 128-byte line: the energy range of
+// the point's tetrahedra, the number of pieces m and, per uniform bin of the
+// range, the piece holding the bin's left edge (where the search starts).
+struct alignas(128) DeltaHead {
+    double lo, hi, binscale;  // binscale = kBins / (hi - lo)
+    int m, pad;
+    unsigned char bins[kBins];
+};
+// One piece [lo, hi] of a (point, band): the weight sum there is
+// c0 + c1 x + c2 x^2 + c3 x^3 with x = (e - mid) * ih.
+struct alignas(64) DeltaPiece {
+    double lo, hi, mid, ih, c0, c1, c2, c3;
+};
+
 struct elph_tetra {
     elph_ctx *ctx = nullptr;
     int64_t nk = 0, nb = 0, ntet = 0, mesh[3] = {0, 0, 0};
@@ -30,12 +49,9 @@ struct elph_tetra {
     std::vector<double> tetra_evals;
     DBuf<int> tcount, tmap;
     DBuf<d4> tev;
-    DBuf<dv2> erange;
     // tabulated delta function (see k_delta_table)
-    DBuf<double> dbp;
-    DBuf<d4> dco;
-    DBuf<dv2> dmh;
-    DBuf<int> dnpc;
+    DBuf<DeltaHead> dhead;
+    DBuf<DeltaPiece> dpiece;
 };
 
 // Static inputs of the electron RTA transport on the device.
@@ -127,22 +143,13 @@ __device__ __noinline__ double tetra_weight(int iv, double e, double e1, double 
 // distinct vertex energies of a point's (at most 24) tetrahedra split the
 // energy axis into at most 95 pieces, on each of which every vertex weight is
 // one cubic polynomial of e.
-constexpr int kBpSlots = 128;  // breakpoints per (point, band), padded with +inf
-constexpr int kMaxPieces = 96;
-
 struct TetraView {
     const int *tcount, *tmap;
     const d4 *tev;
-    const dv2 *erange;
     int nb;
     double numtetra;
-    // tabulated sum of vertex weights: breakpoints bp[pair][kBpSlots], number
-    // of pieces npc[pair], and per piece the cubic c0 + c1 x + c2 x^2 + c3 x^3 in
-    // x = (e - mid) * ih (co[pair][piece], mh[pair][piece] = (mid, ih))
-    const double *bp;
-    const d4 *co;
-    const dv2 *mh;
-    const int *npc;
+    const DeltaHead *head;    // [pair]
+    const DeltaPiece *piece;  // [pair][kMaxPieces]
 };
 
 // Sum over the tetrahedra of list point r of the weight of r's vertex at
@@ -160,33 +167,58 @@ __device__ double weight_sum(const TetraView &t, double e, int r, int n) {
     return d;
 }
 
+// Weight sum at e in [lo, hi] of a pair with m > 0 pieces from the table:
+// start at e's bin, step to the piece holding e (correct from any start),
+// evaluate its cubic.
+__device__ __forceinline__ double table_sum(const DeltaHead *h, const DeltaPiece *pc, int m, double lo,
+                                            double scale, double e) {
+    int b = (int)((e - lo) * scale);
+    b = max(0, min(kBins - 1, b));
+    int j = h->bins[b];
+    DeltaPiece p = pc[j];
+    while (j > 0 && e < p.lo) p = pc[--j];
+    while (j < m - 1 && e >= p.hi) p = pc[++j];
+    const double x = (e - p.mid) * p.ih;
+    return ((p.c3 * x + p.c2) * x + p.c1) * x + p.c0;
+}
+
 // delta_fn_tetra(e, r + 1, n + 1) (src/delta.f90:61-225) from the table: the
-// piece holding e (binary search over the padded breakpoints), its cubic,
-// zero below 1e-12, divided by 6 prod(mesh).  Zero outside the energy range
-// of r's tetrahedra.  Points whose tetrahedra all have one energy (no piece)
-// are summed directly.
+// weight sum, zero below 1e-12, divided by 6 prod(mesh); zero outside the
+// energy range of r's tetrahedra.  Points whose tetrahedra all have one
+// energy (no piece) are summed directly.
 __device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int r, int n) {
     const size_t pr = (size_t)r * t.nb + n;
-    const dv2 rg = t.erange[pr];
-    if (e < rg.x || e > rg.y) return 0.0;
-    const int m = t.npc[pr];
-    double d;
+    const DeltaHead *h = t.head + pr;
+    const double lo = h->lo, hi = h->hi;
+    if (e < lo || e > hi) return 0.0;
+    const int m = h->m;
+    const double d = m > 0 ? table_sum(h, t.piece + pr * kMaxPieces, m, lo, h->binscale, e) : weight_sum(t, e, r, n);
+    return d < 1.0e-12 ? 0.0 : d / t.numtetra;
+}
+
+// delta_tetra at two energies of the same (point, band), one header read.
+__device__ __forceinline__ void delta_pair(const TetraView &t, double e0, double e1, int r, int n, double &d0,
+                                           double &d1) {
+    const size_t pr = (size_t)r * t.nb + n;
+    const DeltaHead *h = t.head + pr;
+    const double lo = h->lo, hi = h->hi;
+    const bool in0 = !(e0 < lo || e0 > hi), in1 = !(e1 < lo || e1 > hi);
+    d0 = 0.0;
+    d1 = 0.0;
+    if (!(in0 || in1)) return;
+    const int m = h->m;
+    double s0 = 0.0, s1 = 0.0;
     if (m > 0) {
-        const double *bp = t.bp + pr * kBpSlots;
-        int lo = 0;
-#pragma unroll
-        for (int step = kBpSlots / 2; step > 0; step >>= 1)
-            if (bp[lo + step] <= e) lo += step;
-        const int j = min(lo, m - 1);
-        const d4 c = t.co[pr * kMaxPieces + j];
-        const dv2 q = t.mh[pr * kMaxPieces + j];
-        const double x = (e - q.x) * q.y;
-        d = ((c.w * x + c.z) * x + c.y) * x + c.x;
+        const DeltaPiece *pc = t.piece + pr * kMaxPieces;
+        const double scale = h->binscale;
+        if (in0) s0 = table_sum(h, pc, m, lo, scale, e0);
+        if (in1) s1 = table_sum(h, pc, m, lo, scale, e1);
     } else {
-        d = weight_sum(t, e, r, n);
+        if (in0) s0 = weight_sum(t, e0, r, n);
+        if (in1) s1 = weight_sum(t, e1, r, n);
     }
-    if (d < 1.0e-12) d = 0.0;
-    return d / t.numtetra;
+    if (s0 >= 1.0e-12) d0 = s0 / t.numtetra;
+    if (s1 >= 1.0e-12) d1 = s1 / t.numtetra;
 }
 
 struct ChebFit {
@@ -197,8 +229,8 @@ struct ChebFit {
 // Builds the table, one thread per (point, band): the distinct vertex
 // energies in ascending order, and on each piece [a, b] the cubic through the
 // exact weight sums at the four Chebyshev nodes of the piece.
-__global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, ChebFit f, double *bp, d4 *co, dv2 *mh,
-                                                     int *npc) {
+__global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, ChebFit f, DeltaHead *head,
+                                                     DeltaPiece *piece) {
     const int pr = blockIdx.x * 256 + threadIdx.x;
     if (pr >= npairs) return;
     const int r = pr / t.nb, n = pr - r * t.nb;
@@ -210,7 +242,7 @@ __global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, Ch
         const double ev[4] = {v.x, v.y, v.z, v.w};
         for (int i = 0; i < 4; ++i) {
             // insertion into the ascending list of distinct values
-            double x = ev[i];
+            const double x = ev[i];
             int k = nbp;
             bool dup = false;
             while (k > 0 && b[k - 1] >= x) {
@@ -223,10 +255,22 @@ __global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, Ch
             ++nbp;
         }
     }
-    double *bo = bp + (size_t)pr * kBpSlots;
-    for (int i = 0; i < kBpSlots; ++i) bo[i] = i < nbp ? b[i] : INFINITY;
     const int m = nbp > 1 ? nbp - 1 : 0;
-    npc[pr] = m;
+    DeltaHead h;
+    h.lo = nbp > 0 ? b[0] : 0.0;
+    h.hi = nbp > 0 ? b[nbp - 1] : -1.0;  // no tetrahedra: an empty range
+    h.binscale = m > 0 ? (double)kBins / (h.hi - h.lo) : 0.0;
+    h.m = m;
+    h.pad = 0;
+    for (int k = 0; k < kBins; ++k) {
+        int j = 0;
+        if (m > 0) {
+            const double edge = h.lo + (h.hi - h.lo) * ((double)k / (double)kBins);
+            while (j < m - 1 && b[j + 1] <= edge) ++j;
+        }
+        h.bins[k] = (unsigned char)j;
+    }
+    head[pr] = h;
     for (int j = 0; j < m; ++j) {
         const double a = b[j], z = b[j + 1];
         const double mid = 0.5 * (a + z), half = 0.5 * (z - a);
@@ -235,8 +279,7 @@ __global__ __launch_bounds__(256) void k_delta_table(TetraView t, int npairs, Ch
         double c[4];
         for (int i = 0; i < 4; ++i)
             c[i] = f.minv[4 * i] * sv[0] + f.minv[4 * i + 1] * sv[1] + f.minv[4 * i + 2] * sv[2] + f.minv[4 * i + 3] * sv[3];
-        co[(size_t)pr * kMaxPieces + j] = d4{c[0], c[1], c[2], c[3]};
-        mh[(size_t)pr * kMaxPieces + j] = dv2{mid, 1.0 / half};
+        piece[(size_t)pr * kMaxPieces + j] = DeltaPiece{a, z, mid, 1.0 / half, c[0], c[1], c[2], c[3]};
     }
 }
 
@@ -282,87 +325,111 @@ struct XArgs {
     TetraView t;
     int nb, nbr, mesh_ref, Nel0, Nel1, Nel2, Nph0, Nph1, Nph2;
     int64_t nprocs;
-    const int *vrow, *kp_int, *wband;
+    const int2 *vmap;
+    const int *kp_int;
     const int64_t *offs;
     const double *om, *omf;
-    const int4 *st_k;
-    const double *st_e;
-    double T, chempot, cnst;
+    const int4 *st_k;      // per in-window state: k (electron mesh ints)
+    const double *st_e;    // its energy
+    const int *st_row;     // its row in g2 / X
+    double T, chempot, cnst, ikT;  // ikT = 1 / (kB T)
+    const double *bose;            // Bose factor of every coarse, then fine, mode
+    size_t ncoarse;
     const double *g2;
     double *xp, *xm;
     int64_t *se, *sp;
     double *partial;  // [state][block] sums of X+ + X-
 };
 
-// One thread per process of a g2 row (blockIdx.y = state of the chunk): the
-// loop body of src/interactions.f90:2278-2332 for that process, X scaled by
-// 2 pi / hbar (2351-2354), and the block's sum of X+ + X- for the rates.
+// Bose factor 1 / expm1(e / kB / T) (src/misc.f90:992-998) of n phonon
+// energies (0 for e == 0, as src/interactions.f90:2287-2291 sets it).
+__global__ void k_bose(int n, const double *en, double T, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double e = en[i];
+    out[i] = e != 0.0 ? 1.0 / expm1(e / kBoltz / T) : 0.0;
+}
+
+constexpr int kXPer = 4;  // processes per thread of k_ibzk_X (strided by the block size)
+
+// Processes of in-window states (blockIdx.y = the state's index among them,
+// st_row its row in g2 / X): per process the loop body of
+// src/interactions.f90:2278-2332, X scaled by 2 pi / hbar (2351-2354), and the
+// block's sum of X+ + X- for the rates.  Block: 256 threads x kXPer
+// processes of one row.
 __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
-    const int st = blockIdx.y;
-    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int4 k = a.st_k[st];
+    const int ia = blockIdx.y;
+    const int4 k = a.st_k[ia];
+    const size_t row = (size_t)a.st_row[ia] * a.nprocs;
+    const double en_el = a.st_e[ia];
     double contrib = 0.0;
-    if (k.w && c < a.nprocs) {
-        const int r = a.vrow[c];
-        const int loc = (int)(c - a.offs[r]);
-        const int rank = loc / a.nbr, s = loc - rank * a.nbr;
-        const int n = a.wband[r * a.nb + rank];
-        const int q0 = pos_mod(a.kp_int[3 * r] - k.x, a.Nel0);
-        const int q1 = pos_mod(a.kp_int[3 * r + 1] - k.y, a.Nel1);
-        const int q2 = pos_mod(a.kp_int[3 * r + 2] - k.z, a.Nel2);
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+        const int64_t c = ((int64_t)blockIdx.x * kXPer + u) * 256 + threadIdx.x;
+        if (c >= a.nprocs) break;
+        const int2 vm = a.vmap[c];
+        const int r = vm.x, n = vm.y >> 8, s = vm.y & 255;
+        // q = k' - k on the electron mesh (both components in [0, N))
+        int q0 = a.kp_int[3 * r] - k.x, q1 = a.kp_int[3 * r + 1] - k.y, q2 = a.kp_int[3 * r + 2] - k.z;
+        q0 += q0 < 0 ? a.Nel0 : 0;
+        q1 += q1 < 0 ? a.Nel1 : 0;
+        q2 += q2 < 0 ? a.Nel2 : 0;
         double en_ph;
         int64_t sph;
-        if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
-            const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-            en_ph = a.om[(size_t)iq * a.nbr + s];
+        size_t bidx;  // the mode's Bose factor: [coarse modes][fine modes]
+        if (a.mesh_ref == 1 || (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0)) {
+            const int iq = a.mesh_ref == 1 ? (q2 * a.Nph1 + q1) * a.Nph0 + q0
+                                           : ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+            bidx = (size_t)iq * a.nbr + s;
+            en_ph = a.om[bidx];
             sph = (int64_t)iq * a.nbr + s + 1;
         } else {
             const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
             en_ph = a.omf[(size_t)iq * a.nbr + s];
+            bidx = a.ncoarse + (size_t)iq * a.nbr + s;
             sph = -((int64_t)iq * a.nbr + s + 1);
         }
-        const double en_el = a.st_e[st];
         double xp = 0.0, xm = 0.0;
         if (en_ph >= 0.5e-3) {
-            const double g = a.g2[(size_t)st * a.nprocs + c];
-            const double bose = 1.0 / expm1(en_ph / kBoltz / a.T);
-            const double fp = 1.0 / (exp((en_el + en_ph - a.chempot) / kBoltz / a.T) + 1.0);
-            const double fm = 1.0 / (exp((en_el - en_ph - a.chempot) / kBoltz / a.T) + 1.0);
-            const double dp = delta_tetra(a.t, en_el + en_ph, r, n);
-            const double dm = delta_tetra(a.t, en_el - en_ph, r, n);
+            const double g = a.g2[row + c];
+            const double bose = a.bose[bidx];
+            const double fp = 1.0 / (exp((en_el + en_ph - a.chempot) * a.ikT) + 1.0);
+            const double fm = 1.0 / (exp((en_el - en_ph - a.chempot) * a.ikT) + 1.0);
+            double dp, dm;
+            delta_pair(a.t, en_el + en_ph, en_el - en_ph, r, n, dp, dm);
             xp = a.cnst * (g * (bose + fp) * dp);
             xm = a.cnst * (g * (1.0 + bose - fm) * dm);
         }
-        const size_t o = (size_t)st * a.nprocs + c;
         if (a.xp) {
-            a.xp[o] = xp;
-            a.xm[o] = xm;
+            a.xp[row + c] = xp;
+            a.xm[row + c] = xm;
         }
         if (a.se) {
-            a.se[o] = (int64_t)r * a.nb + n + 1;
-            a.sp[o] = sph;
+            a.se[row + c] = (int64_t)r * a.nb + n + 1;
+            a.sp[row + c] = sph;
         }
-        contrib = xp + xm;
+        contrib += xp + xm;
     }
     // block sum (fixed order: wave shuffles, then the 4 waves in order)
     for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);
     __shared__ double ws[4];
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = contrib;
     __syncthreads();
-    if (threadIdx.x == 0) a.partial[(size_t)st * gridDim.x + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (threadIdx.x == 0) a.partial[(size_t)ia * gridDim.x + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
-// rates[st] = sum over the state's blocks of the partial sums (0 outside the
-// window), one block per state.
-__global__ __launch_bounds__(256) void k_rates(const double *partial, int nblk, const int4 *st_k, double *rates) {
-    const int st = blockIdx.x;
+// rates[t] = sum over the blocks of state t's partial sums, 0 for a state
+// outside the window (st_idx[t] = -1); one block per state.
+__global__ __launch_bounds__(256) void k_rates(const double *partial, int nblk, const int *st_idx, double *rates) {
+    const int t = blockIdx.x, ia = st_idx[t];
     double v = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += 256) v += partial[(size_t)st * nblk + i];
+    if (ia >= 0)
+        for (int i = threadIdx.x; i < nblk; i += 256) v += partial[(size_t)ia * nblk + i];
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     __shared__ double ws[4];
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0) rates[st] = st_k[st].w ? ((ws[0] + ws[1]) + ws[2]) + ws[3] : 0.0;
+    if (threadIdx.x == 0) rates[t] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // Electron RTA transport sums (src/bte.f90:912-1011 field terms, then
@@ -447,11 +514,8 @@ TetraView view_of(const elph_tetra *t) {
     v.tcount = t->tcount.p;
     v.tmap = t->tmap.p;
     v.tev = t->tev.p;
-    v.erange = t->erange.p;
-    v.bp = t->dbp.p;
-    v.co = t->dco.p;
-    v.mh = t->dmh.p;
-    v.npc = t->dnpc.p;
+    v.head = t->dhead.p;
+    v.piece = t->dpiece.p;
     v.nb = (int)t->nb;
     v.numtetra = (double)(t->mesh[0] * t->mesh[1] * t->mesh[2] * 6);
     return v;
@@ -539,29 +603,15 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
                 for (int iv = 0; iv < 4; ++iv) t->tetra_evals[it + ntet * (b + nb * iv)] = e[iv];
                 hev[it * nb + b] = d4{e[0], e[1], e[2], e[3]};
             }
-        std::vector<dv2> hr(nk * nb);
-        for (int64_t p = 0; p < nk; ++p)
-            for (int64_t b = 0; b < nb; ++b) {
-                double lo = INFINITY, hi = -INFINITY;
-                for (int j = 0; j < hcount[p]; ++j) {
-                    const d4 &v = hev[(size_t)(hmap[p * 24 + j] >> 2) * nb + b];
-                    lo = std::min(lo, v.x);
-                    hi = std::max(hi, v.w);
-                }
-                hr[p * nb + b] = dv2{lo, hi};
-            }
         hipStream_t s = ctx->stream;
         t->tcount.upload(hcount, s);
         t->tmap.upload(hmap, s);
         t->tev.upload(hev, s);
-        t->erange.upload(hr, s);
         const int64_t npairs = nk * nb;
-        t->dbp.alloc(npairs * kBpSlots);
-        t->dco.alloc(npairs * kMaxPieces);
-        t->dmh.alloc(npairs * kMaxPieces);
-        t->dnpc.alloc(npairs);
+        t->dhead.alloc(npairs);
+        t->dpiece.alloc(npairs * kMaxPieces);
         hipLaunchKernelGGL(k_delta_table, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, s, view_of(t),
-                           (int)npairs, cheb_fit(), t->dbp.p, t->dco.p, t->dmh.p, t->dnpc.p);
+                           (int)npairs, cheb_fit(), t->dhead.p, t->dpiece.p);
         HCHECK(hipGetLastError());
         HCHECK(hipStreamSynchronize(s));
     } catch (const Error &e) {
@@ -637,25 +687,48 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
         if (first < 1 || last > v.nirr * v.nb || last < first)
             throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
         const int64_t nst = last - first + 1;
-        const int64_t nblk = (v.nprocs + 255) / 256;
+        const int64_t nblk = (v.nprocs + 256 * kXPer - 1) / (256 * kXPer);
         if (nst > 65535) throw Error{ELPH_EINVAL, "at most 65535 states per call"};
-        std::vector<int4> sk(nst);
-        std::vector<double> se(nst);
+        // in-window states only (the reference skips the others, 2188-2189)
+        std::vector<int4> sk;
+        std::vector<double> se;
+        std::vector<int> srow, sidx(nst, -1);
         for (int64_t i = 0; i < nst; ++i) {
             int64_t m, ik;
             demux_state(first + i, v.nb, &m, &ik);
             const double e = v.ens_irred[(ik - 1) + v.nirr * (m - 1)];
+            if (std::fabs(e - v.enref) > v.fsthick) continue;
             const int *kk = v.kint_irr + 3 * (ik - 1);
-            sk[i] = make_int4(kk[0], kk[1], kk[2], std::fabs(e - v.enref) > v.fsthick ? 0 : 1);
-            se[i] = e;
+            sidx[i] = (int)sk.size();
+            sk.push_back(make_int4(kk[0], kk[1], kk[2], 1));
+            se.push_back(e);
+            srow.push_back((int)i);
         }
+        const int64_t nact = (int64_t)sk.size();
         hipStream_t s = c->stream;
         ConsumerScratch &scr = *v.scr;
-        scr.st_k.upload(sk, s);
-        scr.st_e.upload(se, s);
-        scr.partial.alloc(std::max<int64_t>(1, nst * nblk));
+        if (nact > 0) {
+            scr.st_k.upload(sk, s);
+            scr.st_e.upload(se, s);
+            scr.st_row.upload(srow, s);
+        }
+        scr.st_idx.upload(sidx, s);
+        scr.partial.alloc(std::max<int64_t>(1, nact * nblk));
+        // Bose factors of the phonon modes the processes use
+        const int64_t ncoarse = (int64_t)v.Nph[0] * v.Nph[1] * v.Nph[2] * v.nbr;
+        const int64_t nfine = v.mesh_ref > 1 ? (int64_t)v.Nel[0] * v.Nel[1] * v.Nel[2] * v.nbr : 0;
+        scr.bose.alloc(ncoarse + nfine);
+        hipLaunchKernelGGL(k_bose, dim3((unsigned)((ncoarse + 255) / 256)), dim3(256), 0, s, (int)ncoarse, v.om, T,
+                           scr.bose.p);
+        if (nfine > 0)
+            hipLaunchKernelGGL(k_bose, dim3((unsigned)((nfine + 255) / 256)), dim3(256), 0, s, (int)nfine, v.omf, T,
+                               scr.bose.p + ncoarse);
+        HCHECK(hipGetLastError());
         XArgs a;
         a.t = view_of(tet);
+        a.bose = scr.bose.p;
+        a.ncoarse = (size_t)ncoarse;
+        a.ikT = 1.0 / (kBoltz * T);
         a.nb = v.nb;
         a.nbr = v.nbr;
         a.mesh_ref = v.mesh_ref;
@@ -666,14 +739,14 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
         a.Nph1 = v.Nph[1];
         a.Nph2 = v.Nph[2];
         a.nprocs = v.nprocs;
-        a.vrow = scr.vrow.p;
+        a.vmap = scr.vmap.p;
         a.kp_int = v.kp_int;
-        a.wband = v.wband;
         a.offs = v.offs;
         a.om = v.om;
         a.omf = v.omf;
         a.st_k = scr.st_k.p;
         a.st_e = scr.st_e.p;
+        a.st_row = scr.st_row.p;
         a.T = T;
         a.chempot = chempot;
         a.cnst = 2.0 * (4.0 * std::atan(1.0)) / (kHbar / kQe);
@@ -684,13 +757,13 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
         a.sp = istate_ph;
         a.partial = scr.partial.p;
         HCHECK(hipEventRecord(c->ev_a, s));
-        if (nblk > 0) {
-            hipLaunchKernelGGL(k_ibzk_X, dim3((unsigned)nblk, (unsigned)nst), dim3(256), 0, s, a);
+        if (nblk > 0 && nact > 0) {
+            hipLaunchKernelGGL(k_ibzk_X, dim3((unsigned)nblk, (unsigned)nact), dim3(256), 0, s, a);
             HCHECK(hipGetLastError());
         }
         HCHECK(hipEventRecord(c->ev_b, s));
         if (rates) {
-            hipLaunchKernelGGL(k_rates, dim3((unsigned)nst), dim3(256), 0, s, scr.partial.p, (int)nblk, scr.st_k.p,
+            hipLaunchKernelGGL(k_rates, dim3((unsigned)nst), dim3(256), 0, s, scr.partial.p, (int)nblk, scr.st_idx.p,
                                rates);
             HCHECK(hipGetLastError());
         }
