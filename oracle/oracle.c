@@ -1333,12 +1333,21 @@ double orc_fermi(double e, double chempot, double T) { return 1.0 / (exp((e - ch
  * zero for phonon energies below 0.5 meV, and the final electron and
  * phonon state of each process (fine-mesh phonons negative).  g2 rows have
  * stride nprocs = nstates_inwindow nbr, as orc_ibzk_g2 writes them. */
+/* delta_fn_ptr of get_delta_fn_pointer (delta.f90:47-59): tetrahedra
+ * (nvert 4) or triangles (nvert 3). */
+static double orc_delta_simplex(int nvert, double e, int64_t ik, int64_t ib, const int64_t mesh[3], int64_t nk,
+                                int64_t nb, const int64_t *map, const int64_t *count, const double *evals) {
+    if (nvert == 3) return orc_delta_fn_triang(e, ik, ib, mesh, nk, 2 * nk, nb, map, count, evals);
+    return orc_delta_fn_tetra(e, ik, ib, mesh, nk, 6 * nk, nb, map, count, evals);
+}
+
 int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                int64_t istate_first, int64_t istate_last, const int64_t *counts, const double *g2, double T,
-               double chempot, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals,
-               double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph, int nthreads) {
+               double chempot, int nvert, const int64_t *tetramap, const int64_t *tetracount,
+               const double *tetra_evals, double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph,
+               int nthreads) {
     const int64_t nb = w->nb, nbr = w->nbr, nirr = el->nwv_irred, nwv = el->nwv;
-    const int64_t nprocs = el->nstates_inwindow * nbr, ntet = 6 * nwv;
+    const int64_t nprocs = el->nstates_inwindow * nbr;
     const double cnst = TWOPI_ / (HBAR / QE);
     double rec[9], vol;
     orc_reclattvecs(w->lattvecs, rec, &vol);
@@ -1387,13 +1396,13 @@ int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons 
                         const double bosefac = en_ph != 0.0 ? orc_bose(en_ph, T) : 0.0;
                         const double fp = orc_fermi(en_el + en_ph, chempot, T);
                         const double fm = orc_fermi(en_el - en_ph, chempot, T);
-                        double delta = orc_delta_fn_tetra(en_el + en_ph, ikp, n, el->wvmesh, nwv, ntet, nb, tetramap,
-                                                          tetracount, tetra_evals);
+                        double delta = orc_delta_simplex(nvert, en_el + en_ph, ikp, n, el->wvmesh, nwv, nb, tetramap,
+                                                         tetracount, tetra_evals);
                         double x = 0.0;
                         if (en_ph >= 0.5e-3) x = g[count - 1] * (bosefac + fp) * delta;
                         xp[count - 1] = cnst * x;
-                        delta = orc_delta_fn_tetra(en_el - en_ph, ikp, n, el->wvmesh, nwv, ntet, nb, tetramap,
-                                                   tetracount, tetra_evals);
+                        delta = orc_delta_simplex(nvert, en_el - en_ph, ikp, n, el->wvmesh, nwv, nb, tetramap,
+                                                  tetracount, tetra_evals);
                         x = 0.0;
                         if (en_ph >= 0.5e-3) x = g[count - 1] * (1.0 + bosefac - fm) * delta;
                         xm[count - 1] = cnst * x;
@@ -1535,4 +1544,109 @@ void orc_symmetrize_field_term(int64_t nk, int64_t nb, const double *symmetrizer
             for (int d = 0; d < 3; ++d) field_term[ik + nk * (ib + nb * d)] = g[d];
         }
     }
+}
+
+/* delta.f90:648-737 form_triangles: two triangles per subcell of each z
+ * layer.  triang (2 nk, 3), triangcount (nk), triangmap (2, nk, 6), 1-based;
+ * with blocks != 0 a vertex off indexlist is stored as -1. */
+void orc_form_triangles(int64_t nk, const int64_t mesh[3], int blocks, const int64_t *indexlist, int64_t *triang,
+                        int64_t *triangcount, int64_t *triangmap) {
+    static const int64_t labels[2][3] = {{1, 2, 3}, {1, 3, 4}};
+    const int64_t ntri = 2 * nk;
+    memset(triang, 0, sizeof(int64_t) * ntri * 3);
+    memset(triangcount, 0, sizeof(int64_t) * nk);
+    memset(triangmap, 0, sizeof(int64_t) * 2 * nk * 6);
+    int64_t count = 1;
+    for (int64_t ik = 1; ik <= nk; ++ik) {
+        int64_t ijk[3];
+        orc_demux_vector(blocks ? indexlist[ik - 1] : ik, ijk, mesh, 1);
+        const int64_t i = ijk[0], j = ijk[1], k = ijk[2];
+        const int64_t ip1 = i == mesh[0] ? 1 : i + 1;
+        const int64_t jp1 = j == mesh[1] ? 1 : j + 1;
+        const int64_t sc[4][3] = {{i, j, k}, {ip1, j, k}, {i, jp1, k}, {ip1, jp1, k}};
+        for (int tk = 0; tk < 2; ++tk) {
+            for (int tl = 1; tl <= 3; ++tl) {
+                const int64_t *v = sc[labels[tk][tl - 1] - 1];
+                const int64_t aux = orc_mux_vector(v, mesh, 1);
+                int64_t tmp = aux;
+                if (blocks) tmp = orc_binsearch(indexlist, nk, aux);
+                triang[(count - 1) + ntri * (tl - 1)] = tmp;
+                if (tmp > 0) {
+                    const int64_t c = ++triangcount[tmp - 1];
+                    triangmap[0 + 2 * ((tmp - 1) + nk * (c - 1))] = count;
+                    triangmap[1 + 2 * ((tmp - 1) + nk * (c - 1))] = tl;
+                }
+            }
+            count++;
+        }
+    }
+}
+
+/* delta.f90:739-788 fill_triangles: triang_evals (ntri, nb, 3), 0 off the
+ * list, sorted ascending over the vertices. */
+void orc_fill_triangles(int64_t ntri, const int64_t *triang, int64_t nk, int64_t nb, const double *evals,
+                        double *triang_evals) {
+    memset(triang_evals, 0, sizeof(double) * ntri * nb * 3);
+    for (int64_t it = 0; it < ntri; ++it)
+        for (int iv = 0; iv < 3; ++iv) {
+            const int64_t aux = triang[it + ntri * iv];
+            if (aux > 0)
+                for (int64_t ib = 0; ib < nb; ++ib) triang_evals[it + ntri * (ib + nb * iv)] = evals[(aux - 1) + nk * ib];
+        }
+    for (int64_t it = 0; it < ntri; ++it)
+        for (int64_t ib = 0; ib < nb; ++ib) {
+            double *e[3];
+            for (int iv = 0; iv < 3; ++iv) e[iv] = &triang_evals[it + ntri * (ib + nb * iv)];
+            for (int a = 0; a < 3; ++a) { /* misc.f90:716-736 sort_real */
+                double aux = *e[a];
+                for (int b = a + 1; b < 3; ++b)
+                    if (aux > *e[b]) {
+                        const double t = *e[b];
+                        *e[b] = aux;
+                        *e[a] = t;
+                        aux = t;
+                    }
+            }
+        }
+}
+
+/* delta.f90:790-882 delta_fn_triang (triangle method of Kurganskii et al.):
+ * the weight of the wave vector's vertex label iv with sorted energies
+ * e1 <= e2 <= e3, summed over its triangles, zero below 1e-12, divided by
+ * numtriangs = 2 product(mesh). */
+double orc_delta_fn_triang(double e, int64_t ik, int64_t ib, const int64_t mesh[3], int64_t nk, int64_t ntri,
+                           int64_t nb, const int64_t *triangmap, const int64_t *triangcount,
+                           const double *triang_evals) {
+    double d = 0.0;
+    const int64_t numtriangs = mesh[0] * mesh[1] * mesh[2] * 2;
+    const int64_t num = triangcount[ik - 1];
+    for (int64_t itk = 1; itk <= num; ++itk) {
+        const int64_t it = triangmap[0 + 2 * ((ik - 1) + nk * (itk - 1))];
+        const int64_t iv = triangmap[1 + 2 * ((ik - 1) + nk * (itk - 1))];
+        const double e1 = triang_evals[(it - 1) + ntri * ((ib - 1) + nb * 0)];
+        const double e2 = triang_evals[(it - 1) + ntri * ((ib - 1) + nb * 1)];
+        const double e3 = triang_evals[(it - 1) + ntri * ((ib - 1) + nb * 2)];
+        const int c1 = e <= e1, c2 = e1 < e && e <= e2, c3 = e2 < e && e <= e3, c4 = e3 < e;
+        double tmp = 0.0;
+        if (c1 || c4) continue;
+        const double E12 = (e - e2) / (e1 - e2), E21 = (e - e1) / (e2 - e1), E13 = (e - e3) / (e1 - e3);
+        const double E31 = (e - e1) / (e3 - e1), E23 = (e - e3) / (e2 - e3), E32 = (e - e2) / (e3 - e2);
+        switch (iv) {
+        case 1:
+            if (c2) tmp = E21 * (E12 + E13) / (e3 - e1);
+            else if (c3) tmp = E23 * E13 / (e3 - e1);
+            break;
+        case 2:
+            if (c2) tmp = E21 * E21 / (e3 - e1);
+            else if (c3) tmp = E23 * E23 / (e3 - e1);
+            break;
+        case 3:
+            if (c2) tmp = E21 * E31 / (e3 - e1);
+            else if (c3) tmp = E23 * (E31 + E32) / (e3 - e1);
+            break;
+        }
+        d += tmp;
+    }
+    if (d < 1.0e-12) d = 0.0;
+    return d / (double)numtriangs;
 }

@@ -136,10 +136,19 @@ double orc_delta_fn_tetra(double e, int64_t ik, int64_t ib, const int64_t mesh[3
                           int64_t nb, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals);
 double orc_bose(double e, double T);
 double orc_fermi(double e, double chempot, double T);
+void orc_form_triangles(int64_t nk, const int64_t mesh[3], int blocks, const int64_t *indexlist, int64_t *triang,
+                        int64_t *triangcount, int64_t *triangmap);
+void orc_fill_triangles(int64_t ntri, const int64_t *triang, int64_t nk, int64_t nb, const double *evals,
+                        double *triang_evals);
+double orc_delta_fn_triang(double e, int64_t ik, int64_t ib, const int64_t mesh[3], int64_t nk, int64_t ntri,
+                           int64_t nb, const int64_t *triangmap, const int64_t *triangcount,
+                           const double *triang_evals);
+/* nvert 4: tetrahedra tables, 3: triangle tables (the simplex_* arrays). */
 int orc_ibzk_X(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                int64_t istate_first, int64_t istate_last, const int64_t *counts, const double *g2, double T,
-               double chempot, const int64_t *tetramap, const int64_t *tetracount, const double *tetra_evals,
-               double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph, int nthreads);
+               double chempot, int nvert, const int64_t *tetramap, const int64_t *tetracount,
+               const double *tetra_evals, double *Xplus, double *Xminus, int64_t *istate_el, int64_t *istate_ph,
+               int nthreads);
 void orc_el_rta_rates(int64_t nst, const int64_t *counts, int64_t nprocs, const double *Xplus, const double *Xminus,
                       double *rates);
 

@@ -70,6 +70,7 @@ def lib():
         L.orc_qrotations.restype = ct.c_int64
         L.orc_find_irred_wedge.restype = ct.c_int64
         L.orc_delta_fn_tetra.restype = ct.c_double
+        L.orc_delta_fn_triang.restype = ct.c_double
         L.orc_bose.restype = ct.c_double
         L.orc_fermi.restype = ct.c_double
         _lib = L
@@ -433,11 +434,51 @@ class Tetrahedra:
         lib().orc_fill_tetrahedra_3d(ct.c_int64(6 * nk), _p(self.tetra, c_i64p), ct.c_int64(nk), ct.c_int64(nb),
                                      _p(evals), _p(self.evals))
 
+    nvert = 4
+
+    @property
+    def simplex(self):
+        """(map, count, evals) as el%simplex_map/_count/_evals hold them."""
+        return self.tetramap, self.tetracount, self.evals
+
     def delta(self, e, ik, ib):
         """delta_fn_tetra(e, ik, ib) (delta.f90:61-225), ik/ib 1-based."""
         return lib().orc_delta_fn_tetra(ct.c_double(e), ct.c_int64(ik), ct.c_int64(ib), _p(self.mesh, c_i64p),
                                         ct.c_int64(self.nk), ct.c_int64(self.ntet), ct.c_int64(self.nb),
                                         _p(self.tetramap, c_i64p), _p(self.tetracount, c_i64p), _p(self.evals))
+
+
+class Triangles:
+    """form_triangles + fill_triangles (delta.f90:648-788): triang (2 nk, 3),
+    triangcount (nk), triangmap (2, nk, 6), evals (2 nk, nb, 3)."""
+
+    nvert = 3
+
+    def __init__(self, mesh, evals, indexlist=None):
+        mesh = _i64(mesh)
+        evals = np.asfortranarray(np.asarray(evals, np.float64))
+        nk, nb = evals.shape
+        blocks = indexlist is not None
+        il = _i64(indexlist if blocks else np.zeros(1))
+        self.mesh, self.nk, self.nb, self.ntri = mesh, nk, nb, 2 * nk
+        self.triang = np.zeros((2 * nk, 3), np.int64, order="F")
+        self.triangcount = np.zeros(nk, np.int64)
+        self.triangmap = np.zeros((2, nk, 6), np.int64, order="F")
+        lib().orc_form_triangles(ct.c_int64(nk), _p(mesh, c_i64p), ct.c_int(int(blocks)), _p(il, c_i64p),
+                                 _p(self.triang, c_i64p), _p(self.triangcount, c_i64p), _p(self.triangmap, c_i64p))
+        self.evals = np.zeros((2 * nk, nb, 3), order="F")
+        lib().orc_fill_triangles(ct.c_int64(2 * nk), _p(self.triang, c_i64p), ct.c_int64(nk), ct.c_int64(nb),
+                                 _p(evals), _p(self.evals))
+
+    @property
+    def simplex(self):
+        return self.triangmap, self.triangcount, self.evals
+
+    def delta(self, e, ik, ib):
+        """delta_fn_triang(e, ik, ib) (delta.f90:790-882), ik/ib 1-based."""
+        return lib().orc_delta_fn_triang(ct.c_double(e), ct.c_int64(ik), ct.c_int64(ib), _p(self.mesh, c_i64p),
+                                         ct.c_int64(self.nk), ct.c_int64(self.ntri), ct.c_int64(self.nb),
+                                         _p(self.triangmap, c_i64p), _p(self.triangcount, c_i64p), _p(self.evals))
 
 
 def delta_fn_tetra_raw(e, ik, ib, mesh, tetramap, tetracount, tetra_evals):
@@ -458,9 +499,9 @@ def fermi(e, chempot, T):
     return lib().orc_fermi(ct.c_double(e), ct.c_double(chempot), ct.c_double(T))
 
 
-def ibzk_X(w: Wannier, el, ph, tet: Tetrahedra, counts, g2, T, chempot, mesh_ref=1, istate_first=1, nthreads=0):
+def ibzk_X(w: Wannier, el, ph, tet, counts, g2, T, chempot, mesh_ref=1, istate_first=1, nthreads=0):
     """calculate_eph_interaction_ibzk('X') on g2 rows (counts, g2 as ibzk_g2
-    returns them).  Returns (Xplus, Xminus, istate_el, istate_ph), rows of
+    returns them); tet: Tetrahedra or Triangles (delta.f90:47-59).  Returns (Xplus, Xminus, istate_el, istate_ph), rows of
     length nstates_inwindow*nbr (entries past each count are 0)."""
     counts = _i64(counts)
     g2 = np.ascontiguousarray(g2, np.float64)
@@ -469,9 +510,10 @@ def ibzk_X(w: Wannier, el, ph, tet: Tetrahedra, counts, g2, T, chempot, mesh_ref
     se, sp = np.zeros(g2.shape, np.int64), np.zeros(g2.shape, np.int64)
     es, k1 = _electrons_struct(w, el)
     ps, k2 = _phonons_struct(ph)
+    smap, scount, sev = tet.simplex
     st = lib().orc_ibzk_X(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref), ct.c_int64(istate_first),
                           ct.c_int64(istate_first + nst - 1), _p(counts, c_i64p), _p(g2), ct.c_double(T),
-                          ct.c_double(chempot), _p(tet.tetramap, c_i64p), _p(tet.tetracount, c_i64p), _p(tet.evals),
+                          ct.c_double(chempot), ct.c_int(tet.nvert), _p(smap, c_i64p), _p(scount, c_i64p), _p(sev),
                           _p(xp), _p(xm), _p(se, c_i64p), _p(sp, c_i64p), ct.c_int(nthreads))
     if st == 2:
         raise RuntimeError("Large negative phonon energy found! Stopping!")

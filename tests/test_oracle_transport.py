@@ -332,3 +332,103 @@ def test_symmetrize_field_term():
     R = np.repeat(Q[:, :, None], 10, axis=2)
     G = O.symmetrize_field_term(R, F)
     assert np.allclose(G, np.einsum("ij,kbj->kbi", Q, F), rtol=1e-14, atol=1e-14)
+
+
+# ---- triangle method (delta.f90:648-882, used when tetrahedra = .false.) ----
+def test_triangle_geometry():
+    """The reference's two triangles per subcell (labels [1,2,3], [1,3,4] of the
+    corners (i,j), (i+1,j), (i,j+1), (i+1,j+1), delta.f90:675-679) each have
+    area 1/2 but do not tile the square: they overlap on a quarter of it and
+    leave another quarter uncovered.  Reproduced as is (each triangle's
+    weights still integrate to one)."""
+    mesh = [4, 3, 2]
+    nk = int(np.prod(mesh))
+    tri = O.Triangles(mesh, np.zeros((nk, 1)))
+    assert np.all(tri.triangcount == 6) and np.all(tri.triang > 0)
+    pts = np.random.default_rng(0).random((3000, 2))
+    m = np.array(mesh)
+    for ik in (0, 9, nk - 1):
+        base = np.array(O.demux_vector(ik + 1, mesh, 1))
+        inside = np.zeros(len(pts), int)
+        for tk in range(2):
+            off = np.array([(np.array(O.demux_vector(int(tri.triang[2 * ik + tk, iv]), mesh, 1)) - base) % m
+                            for iv in range(3)], float)
+            assert np.all(off[:, 2] == 0)          # within the z layer
+            M = (off[1:, :2] - off[0, :2]).T
+            assert abs(abs(np.linalg.det(M)) - 1.0) < 1e-12   # area 1/2 of the square
+            lam = np.linalg.solve(M, (pts - off[0, :2]).T)
+            inside += np.all(np.vstack([1 - lam.sum(0), lam]) >= -1e-12, axis=0)
+        frac = [float(np.mean(inside == c)) for c in (0, 1, 2)]
+        assert np.allclose(frac, [0.25, 0.5, 0.25], atol=0.03), frac
+    for ik in range(nk):                            # triangmap inverts triang
+        for j in range(tri.triangcount[ik]):
+            assert tri.triang[tri.triangmap[0, ik, j] - 1, tri.triangmap[1, ik, j] - 1] == ik + 1
+
+
+def _triangle_moment(es, i, p):
+    """int over a triangle of eps^p lambda_i / area: 2! a!b!c! / (a+b+c+2)!."""
+    tot = 0.0
+    for idx in np.ndindex(*([3] * p)):
+        expo = [0, 0, 0]
+        expo[i] += 1
+        w = 1.0
+        for j in idx:
+            expo[j] += 1
+            w *= es[j]
+        tot += w * 2.0 * np.prod([math.factorial(a) for a in expo]) / math.factorial(sum(expo) + 2)
+    return tot
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_triangle_weight_moments(seed):
+    es = np.sort(np.random.default_rng(seed).random(3) * 2 - 1)
+    nk = 1
+    tri = O.Triangles([1, 1, 1], np.zeros((1, 1)))
+    xg, wg = np.polynomial.legendre.leggauss(6)
+    for iv in range(1, 4):
+        tri.triangmap[:] = 0
+        tri.triangmap[0, 0, 0], tri.triangmap[1, 0, 0] = 1, iv
+        tri.triangcount[:] = [1]
+        tri.evals = np.asfortranarray(np.array(es).reshape(1, 1, 3))
+        tri.ntri = 1
+        mom = np.zeros(4)
+        for a, b in ((es[0], es[1]), (es[1], es[2])):
+            for x, wq in zip(xg, wg):
+                e = 0.5 * (b - a) * x + 0.5 * (a + b)
+                mom += 0.5 * (b - a) * wq * 2.0 * tri.delta(e, 1, 1) * e ** np.arange(4)
+        ref = [_triangle_moment(es, iv - 1, p) for p in range(4)]
+        assert np.allclose(mom, ref, rtol=1e-10, atol=1e-12), (iv, mom, ref)
+        assert tri.delta(es[0], 1, 1) == 0.0 and tri.delta(es[2] + 1e-9, 1, 1) == 0.0
+
+
+def test_triangle_dos_normalization():
+    mesh = [4, 5, 2]
+    nk = int(np.prod(mesh))
+    ijk = np.array([O.demux_vector(i, mesh, 0) for i in range(1, nk + 1)], float) / np.array(mesh)
+    ens = (np.cos(2 * np.pi * ijk[:, 0]) + 0.6 * np.sin(2 * np.pi * ijk[:, 1]) + 0.2 * ijk[:, 2])[:, None]
+    tri = O.Triangles(mesh, ens)
+    bps = np.unique(ens)
+    xg, wg = np.polynomial.legendre.leggauss(4)
+    per_k = np.zeros(nk)
+    for a, b in zip(bps[:-1], bps[1:]):
+        for x, wq in zip(xg, wg):
+            e = 0.5 * (b - a) * x + 0.5 * (a + b)
+            per_k += 0.5 * (b - a) * wq * np.array([tri.delta(e, ik, 1) for ik in range(1, nk + 1)])
+    assert abs(per_k.sum() - 1.0) < 1e-10
+    assert np.allclose(per_k, 1.0 / nk, rtol=1e-9)
+
+
+def test_ibzk_X_triangles_against_loops():
+    mat = synth.tiny(nb=3, natoms=1, ws=3, wv=4, seed=22, bands_in_window=2)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    counts, g2 = O.ibzk_g2(o, el, ph, 1, 1, min(el.nwv_irred * mat.nb, 12))
+    tri = O.Triangles(el.wvmesh, el.ens, indexlist=el.indexlist)
+    T, mu = 300.0, float(mat.enref)
+    xp, xm, se, sp = O.ibzk_X(o, el, ph, tri, counts, g2, T, mu)
+    rp, rm, re_, rs = _x_by_loops(mat, o, el, ph, tri, counts, g2, T, mu, 1)
+    assert np.array_equal(se, re_) and np.array_equal(sp, rs)
+    assert np.allclose(xp, rp, rtol=1e-13, atol=0) and np.allclose(xm, rm, rtol=1e-13, atol=0)
+    assert (xp > 0).any() and (xm > 0).any()
