@@ -70,6 +70,8 @@ PROTOTYPES = {
     "elph_ibzk_plan_nprocs": (ct.c_int64, [ct.c_void_p]),
     "elph_tetra_create": (ct.c_int, [ct.c_void_p, ct.c_int64, c_i64p, c_i64p, ct.c_int64, c_dblp,
                                      ct.POINTER(ct.c_void_p)]),
+    "elph_triang_create": (ct.c_int, [ct.c_void_p, ct.c_int64, c_i64p, c_i64p, ct.c_int64, c_dblp,
+                                      ct.POINTER(ct.c_void_p)]),
     "elph_tetra_tables": (ct.c_int, [ct.c_void_p, c_i64p, c_i64p, c_i64p, c_dblp]),
     "elph_tetra_delta": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_i64p, c_i64p, c_dblp]),
     "elph_tetra_destroy": (ct.c_int, [ct.c_void_p]),

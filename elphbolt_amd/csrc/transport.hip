@@ -43,6 +43,7 @@ struct alignas(64) DeltaPiece {
 struct elph_tetra {
     elph_ctx *ctx = nullptr;
     int64_t nk = 0, nb = 0, ntet = 0, mesh[3] = {0, 0, 0};
+    int nvert = 4;  // 4: tetrahedra, 3: triangles
     // host copies in the reference's layouts (tetra (ntet,4), tetracount (nk),
     // tetramap (2,nk,24), tetra_evals (ntet,nb,4))
     std::vector<int64_t> tetra, tetracount, tetramap;
@@ -139,6 +140,28 @@ __device__ __noinline__ double tetra_weight(int iv, double e, double e1, double 
     return w;
 }
 
+// Weight of vertex label iv (1..3) of a triangle with sorted energies
+// e1 <= e2 <= e3 at energy e, e1 < e <= e3: the case expressions of
+// src/delta.f90:842-871 (triangle method of Kurganskii et al.).
+__device__ __noinline__ double triang_weight(int iv, double e, double e1, double e2, double e3) {
+#pragma clang fp contract(off)
+    const bool c2 = e1 < e && e <= e2, c3 = e2 < e && e <= e3;
+    const double E12 = (e - e2) / (e1 - e2), E21 = (e - e1) / (e2 - e1), E13 = (e - e3) / (e1 - e3);
+    const double E31 = (e - e1) / (e3 - e1), E23 = (e - e3) / (e2 - e3), E32 = (e - e2) / (e3 - e2);
+    double w = 0.0;
+    if (iv == 1) {
+        if (c2) w = E21 * (E12 + E13) / (e3 - e1);
+        else if (c3) w = E23 * E13 / (e3 - e1);
+    } else if (iv == 2) {
+        if (c2) w = E21 * E21 / (e3 - e1);
+        else if (c3) w = E23 * E23 / (e3 - e1);
+    } else {
+        if (c2) w = E21 * E31 / (e3 - e1);
+        else if (c3) w = E23 * (E31 + E32) / (e3 - e1);
+    }
+    return w;
+}
+
 // Intervals per (list point, band) of the tabulated delta function: the
 // distinct vertex energies of a point's (at most 24) tetrahedra split the
 // energy axis into at most 95 pieces, on each of which every vertex weight is
@@ -147,7 +170,8 @@ struct TetraView {
     const int *tcount, *tmap;
     const d4 *tev;
     int nb;
-    double numtetra;
+    bool tri;         // triangles (delta_fn_triang) instead of tetrahedra
+    double numtetra;  // number of simplices of the mesh
     const DeltaHead *head;    // [pair]
     const DeltaPiece *piece;  // [pair][kMaxPieces]
 };
@@ -161,8 +185,13 @@ __device__ double weight_sum(const TetraView &t, double e, int r, int n) {
     for (int j = 0; j < num; ++j) {
         const int code = t.tmap[r * 24 + j];
         const d4 v = t.tev[(size_t)(code >> 2) * t.nb + n];
-        if (e < v.x || e > v.w) continue;
-        d += tetra_weight((code & 3) + 1, e, v.x, v.y, v.z, v.w);
+        if (t.tri) {
+            if (e <= v.x || e > v.z) continue;
+            d += triang_weight((code & 3) + 1, e, v.x, v.y, v.z);
+        } else {
+            if (e < v.x || e > v.w) continue;
+            d += tetra_weight((code & 3) + 1, e, v.x, v.y, v.z, v.w);
+        }
     }
     return d;
 }
@@ -517,7 +546,8 @@ TetraView view_of(const elph_tetra *t) {
     v.head = t->dhead.p;
     v.piece = t->dpiece.p;
     v.nb = (int)t->nb;
-    v.numtetra = (double)(t->mesh[0] * t->mesh[1] * t->mesh[2] * 6);
+    v.tri = t->nvert == 3;
+    v.numtetra = (double)(t->mesh[0] * t->mesh[1] * t->mesh[2] * (t->nvert == 3 ? 2 : 6));
     return v;
 }
 
@@ -525,8 +555,13 @@ TetraView view_of(const elph_tetra *t) {
 
 extern "C" {
 
-int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
-                      const double *ens, elph_tetra **out) {
+namespace {
+// form_tetrahedra_3d + fill_tetrahedra_3d (src/delta.f90:494-646), or with
+// triangles form_triangles + fill_triangles (648-788): the simplices of each
+// list point's subcell, the (simplex, vertex label) map of each point, the
+// sorted vertex energies, then the delta table.
+int create_simplices(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                     const double *ens, bool triangles, elph_tetra **out) {
     if (!ctx || !mesh || !ens || !out || nk < 1 || nb < 1) return ELPH_EINVAL;
     *out = nullptr;
     auto *t = new elph_tetra();
@@ -546,28 +581,32 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
                     throw Error{ELPH_EINVAL, "index list must be strictly ascending"};
             }
         }
-        if (6 * nk >= (int64_t)1 << 28) throw Error{ELPH_EINVAL, "too many tetrahedra"};
+        // vertex offsets of a subcell (src/delta.f90:565-573; the first four are
+        // the triangles' subcell of 708-713), the six tetrahedra (522-529) and
+        // the two triangles (675-679) as labels into them
+        static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                         {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        static const int tlabel[6][4] = {{1, 2, 3, 6}, {1, 3, 5, 6}, {3, 5, 6, 7}, {3, 6, 7, 8}, {3, 4, 6, 8}, {2, 3, 4, 6}};
+        static const int flabel[2][4] = {{1, 2, 3, 0}, {1, 3, 4, 0}};
+        const int nper = triangles ? 2 : 6, nv = triangles ? 3 : 4, maxper = triangles ? 6 : 24;
+        if (nper * nk >= (int64_t)1 << 28) throw Error{ELPH_EINVAL, "too many simplices"};
         t->ctx = ctx;
         t->nk = nk;
         t->nb = nb;
-        t->ntet = 6 * nk;
+        t->nvert = nv;
+        t->ntet = nper * nk;
         const int64_t ntet = t->ntet;
-        t->tetra.assign(ntet * 4, 0);
+        t->tetra.assign(ntet * nv, 0);
         t->tetracount.assign(nk, 0);
-        t->tetramap.assign(2 * nk * 24, 0);
-        // vertex offsets of a subcell (src/delta.f90:565-573) and the six
-        // tetrahedra as labels into them (522-529)
-        static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
-                                         {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
-        static const int label[6][4] = {{1, 2, 3, 6}, {1, 3, 5, 6}, {3, 5, 6, 7}, {3, 6, 7, 8}, {3, 4, 6, 8}, {2, 3, 4, 6}};
+        t->tetramap.assign(2 * nk * maxper, 0);
         std::vector<int> hcount(nk, 0), hmap(nk * 24, 0);
         for (int64_t p = 0; p < nk; ++p) {
             int64_t v[3];
             demux_vector0(blocks ? indexlist[p] : p + 1, v, mesh);
-            for (int tk = 0; tk < 6; ++tk) {
-                const int64_t it = 6 * p + tk;  // 0-based tetrahedron
-                for (int tl = 0; tl < 4; ++tl) {
-                    const int *cn = corner[label[tk][tl] - 1];
+            for (int tk = 0; tk < nper; ++tk) {
+                const int64_t it = nper * p + tk;  // 0-based simplex
+                for (int tl = 0; tl < nv; ++tl) {
+                    const int *cn = corner[(triangles ? flabel[tk][tl] : tlabel[tk][tl]) - 1];
                     int64_t w[3];
                     for (int d = 0; d < 3; ++d) w[d] = (v[d] + cn[d]) % mesh[d];
                     const int64_t muxed = (w[2] * mesh[1] + w[1]) * mesh[0] + w[0] + 1;
@@ -579,7 +618,7 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
                     t->tetra[it + ntet * tl] = idx;
                     if (idx > 0) {
                         const int64_t c = t->tetracount[idx - 1]++;
-                        if (c >= 24) throw Error{ELPH_EINVAL, "more than 24 tetrahedra at a wave vector"};
+                        if (c >= maxper) throw Error{ELPH_EINVAL, "too many simplices at a wave vector"};
                         t->tetramap[0 + 2 * ((idx - 1) + nk * c)] = it + 1;
                         t->tetramap[1 + 2 * ((idx - 1) + nk * c)] = tl + 1;
                         hmap[(idx - 1) * 24 + c] = (int)(4 * it + tl);
@@ -588,19 +627,21 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
                 }
             }
         }
-        // vertex energies (0 off the list), sorted per tetrahedron and band
-        // (src/delta.f90:607-646)
-        t->tetra_evals.assign(ntet * nb * 4, 0.0);
+        // vertex energies (0 off the list), sorted per simplex and band
+        // (src/delta.f90:607-646, 739-788); a triangle's highest energy is
+        // repeated in the fourth slot
+        t->tetra_evals.assign(ntet * nb * nv, 0.0);
         std::vector<d4> hev(ntet * nb);
         for (int64_t it = 0; it < ntet; ++it)
             for (int64_t b = 0; b < nb; ++b) {
                 double e[4];
-                for (int iv = 0; iv < 4; ++iv) {
+                for (int iv = 0; iv < nv; ++iv) {
                     const int64_t idx = t->tetra[it + ntet * iv];
                     e[iv] = idx > 0 ? ens[(idx - 1) + nk * b] : 0.0;
                 }
-                std::sort(e, e + 4);
-                for (int iv = 0; iv < 4; ++iv) t->tetra_evals[it + ntet * (b + nb * iv)] = e[iv];
+                std::sort(e, e + nv);
+                if (nv == 3) e[3] = e[2];
+                for (int iv = 0; iv < nv; ++iv) t->tetra_evals[it + ntet * (b + nb * iv)] = e[iv];
                 hev[it * nb + b] = d4{e[0], e[1], e[2], e[3]};
             }
         hipStream_t s = ctx->stream;
@@ -620,6 +661,17 @@ int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int6
     }
     *out = t;
     return ELPH_OK;
+}
+}  // namespace
+
+int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                      const double *ens, elph_tetra **out) {
+    return create_simplices(ctx, nk, mesh, indexlist, nb, ens, false, out);
+}
+
+int elph_triang_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                       const double *ens, elph_tetra **out) {
+    return create_simplices(ctx, nk, mesh, indexlist, nb, ens, true, out);
 }
 
 int elph_tetra_tables(const elph_tetra *t, int64_t *tetra, int64_t *tetracount, int64_t *tetramap,

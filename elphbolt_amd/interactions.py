@@ -149,6 +149,9 @@ class Tetrahedra:
     device: over el%indexlist (electron.f90:582-587) when indexlist is given,
     else over the full mesh; ens (nk, nb) eV."""
 
+    _create = "elph_tetra_create"
+    nper, nvert, maxper = 6, 4, 24
+
     def __init__(self, wann: Wannier, mesh, ens, indexlist=None):
         self.wann, self.lib = wann, wann.lib
         ens = np.asfortranarray(np.asarray(ens, np.float64))
@@ -156,18 +159,18 @@ class Tetrahedra:
         self.mesh = np.asarray(mesh, np.int64)
         il = None if indexlist is None else np.ascontiguousarray(indexlist, np.int64)
         h = ct.c_void_p()
-        st = self.lib.elph_tetra_create(wann.ctx, self.nk, _p(self.mesh, c_i64p),
-                                        None if il is None else _p(il, c_i64p), self.nb, _p(ens), ct.byref(h))
-        wann._check(st, "tetrahedra")
+        st = getattr(self.lib, self._create)(wann.ctx, self.nk, _p(self.mesh, c_i64p),
+                                             None if il is None else _p(il, c_i64p), self.nb, _p(ens), ct.byref(h))
+        wann._check(st, "simplices")
         self.h = h
 
     def tables(self):
         """(tetra, tetracount, tetramap, tetra_evals) in the reference's shapes."""
-        nk, nb = self.nk, self.nb
-        tetra = np.zeros((6 * nk, 4), np.int64, order="F")
+        nk, nb, npr, nv = self.nk, self.nb, self.nper, self.nvert
+        tetra = np.zeros((npr * nk, nv), np.int64, order="F")
         count = np.zeros(nk, np.int64)
-        tmap = np.zeros((2, nk, 24), np.int64, order="F")
-        ev = np.zeros((6 * nk, nb, 4), order="F")
+        tmap = np.zeros((2, nk, self.maxper), np.int64, order="F")
+        ev = np.zeros((npr * nk, nb, nv), order="F")
         st = self.lib.elph_tetra_tables(self.h, _p(tetra, c_i64p), _p(count, c_i64p), _p(tmap, c_i64p), _p(ev))
         self.wann._check(st, "tetrahedra tables")
         return tetra, count, tmap, ev
@@ -192,6 +195,14 @@ class Tetrahedra:
             self.close()
         except Exception:
             pass
+
+
+class Triangles(Tetrahedra):
+    """form_triangles + fill_triangles (src/delta.f90:648-788) on the device;
+    its delta function is delta_fn_triang (790-882)."""
+
+    _create = "elph_triang_create"
+    nper, nvert, maxper = 2, 3, 6
 
 
 class Batch:

@@ -169,20 +169,27 @@ int elph_ibzq_plan_destroy(elph_ibzq_plan *plan);
  * context's device. */
 int elph_tetra_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
                       const double *ens, elph_tetra **out);
+/* The same for the triangle method, form_triangles + fill_triangles
+ * (src/delta.f90:648-788, used by the reference when tetrahedra = .false.):
+ * the delta function of the returned object is delta_fn_triang (790-882). */
+int elph_triang_create(elph_ctx *ctx, int64_t nk, const int64_t *mesh, const int64_t *indexlist, int64_t nb,
+                       const double *ens, elph_tetra **out);
 /* Host copies in the reference's layouts, any pointer may be NULL: tetra
  * (6 nk, 4) (-1 = vertex off the list), tetracount (nk), tetramap (2, nk, 24),
- * tetra_evals (6 nk, nb, 4) sorted over the vertices. */
+ * tetra_evals (6 nk, nb, 4) sorted over the vertices; for triangles (2 nk, 3),
+ * (nk), (2, nk, 6), (2 nk, nb, 3). */
 int elph_tetra_tables(const elph_tetra *t, int64_t *tetra, int64_t *tetracount, int64_t *tetramap,
                       double *tetra_evals);
-/* delta_fn_tetra (src/delta.f90:61-225) evaluated on the device for n host
- * samples (energy e[i], 1-based list index ik[i], 1-based band ib[i]). */
+/* delta_fn_tetra (src/delta.f90:61-225) or delta_fn_triang (790-882)
+ * evaluated on the device for n host samples (energy e[i], 1-based list
+ * index ik[i], 1-based band ib[i]). */
 int elph_tetra_delta(elph_tetra *t, int64_t n, const double *e, const int64_t *ik, const int64_t *ib, double *out);
 int elph_tetra_destroy(elph_tetra *t);
 
 /* calculate_eph_interaction_ibzk, key 'X' (src/interactions.f90:2278-2354),
  * on the device rows g2 (stride nprocs) that elph_ibzk_plan_run wrote for the
- * same istate_first..istate_last, the tetrahedra formed on the plan's
- * electron list: X+ and X- (THz, rows of stride nprocs; rows of states
+ * same istate_first..istate_last, the tetrahedra (or triangles) formed on
+ * the plan's electron list: X+ and X- (THz, rows of stride nprocs; rows of states
  * outside the window are left untouched), the final electron state
  * mux_state(nb, n, ik') and phonon state mux_state(nbr, s, iq) (negative for
  * a phonon off the phonon mesh), and the e-ph RTA rate of each state, sum X+

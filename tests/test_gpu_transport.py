@@ -121,3 +121,42 @@ def test_X_and_rates(case, precision):
     assert err2 < tol
     assert _rel(out["rates"], O.el_rta_rates(c_o, xp_oo, xm_oo), 1e-9) < tol
     plan.close()
+
+
+def test_triangle_method(case):
+    """The triangle method (tetrahedra = .false.): tables exact, delta_fn_triang
+    on the device and X+/- from the same g2 rows, tolerances as above."""
+    mat, el, ph, w = case["mat"], case["el"], case["ph"], case["w"]
+    print(f"triangles {case['name']}", flush=True)
+    tri_o = O.Triangles(el.wvmesh, el.ens, indexlist=el.indexlist)
+    tri_g = interactions.Triangles(w, el.wvmesh, el.ens, indexlist=el.indexlist)
+    t, c, m, ev = tri_g.tables()
+    assert np.array_equal(t, tri_o.triang) and np.array_equal(c, tri_o.triangcount)
+    assert np.array_equal(m, tri_o.triangmap) and np.array_equal(ev, tri_o.evals)
+    rng = np.random.default_rng(4)
+    n = 3000
+    ik = rng.integers(1, el.nwv + 1, n)
+    ib = rng.integers(1, el.nb + 1, n)
+    e = el.ens[ik - 1, ib - 1] + rng.normal(0, 0.05, n)
+    d_g = tri_g.delta(e, ik, ib)
+    d_o = np.array([tri_o.delta(e[i], ik[i], ib[i]) for i in range(n)])
+    big = d_o.max()
+    assert (d_o > 0).sum() > n // 10
+    assert np.max(np.abs(d_g - d_o)) < 1e-12 * big
+    assert _rel(d_g, d_o, 1e-6) < 1e-9
+    mesh_ref = case["mesh_ref"]
+    last = min(el.nwv_irred * mat.nb, 16)
+    from elphbolt_amd.wannier import DeviceBuffer
+    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=mesh_ref, precision="fp64")
+    g2d = DeviceBuffer(w, (last, plan.nprocs))
+    counts, _ = plan.run(1, last, out=g2d)
+    mu = float(mat.enref) + 0.02
+    out = plan.run_X(tri_g, T_K, mu, 1, last, g2d, maps=False)
+    act = counts >= 0
+    xp_o, xm_o, _, _ = O.ibzk_X(case["o"], el, ph, tri_o, counts, g2d.numpy(), T_K, mu, mesh_ref=mesh_ref)
+    assert (xp_o[act] > 0).any()
+    err = max(_rel(out["Xplus"].numpy()[act], xp_o[act], 1e-6), _rel(out["Xminus"].numpy()[act], xm_o[act], 1e-6))
+    print(f"  X (triangles) max rel {err:.2e}", flush=True)
+    assert err < 1e-9
+    assert _rel(out["rates"], O.el_rta_rates(counts, xp_o, xm_o), 1e-9) < 1e-9
+    plan.close()
