@@ -91,6 +91,8 @@ PROTOTYPES = {
     "elph_ibzq_plan_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p, ct.c_int32]),
     "elph_ibzq_plan_stride": (ct.c_int64, [ct.c_void_p]),
     "elph_ibzq_plan_destroy": (ct.c_int, [ct.c_void_p]),
+    "elph_write_X_files": (ct.c_int, [ct.c_char_p, ct.c_int64, ct.c_int64, c_i64p, c_dblp, c_dblp, c_i64p, c_i64p,
+                                      ct.c_int64]),
     "elph_write_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
     "elph_symmetry": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p, c_dblp]),
     "elph_irred_wedge": (ct.c_int, [c_i64p, ct.c_int64, c_dblp, ct.c_int64, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),

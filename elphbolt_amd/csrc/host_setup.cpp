@@ -341,6 +341,31 @@ extern "C" int elph_write_g2_files(const char *dir, char kind, int64_t istate_fi
     return ELPH_OK;
 }
 
+extern "C" int elph_write_X_files(const char *dir, int64_t istate_first, int64_t nstates, const int64_t *counts,
+                                  const double *xplus, const double *xminus, const int64_t *istate_el,
+                                  const int64_t *istate_ph, int64_t row_stride) {
+    if (!dir || !counts || !xplus || !xminus || !istate_el || !istate_ph || nstates < 0) return ELPH_EINVAL;
+    for (int64_t t = 0; t < nstates; ++t) {
+        if (counts[t] < 0) continue;
+        const int64_t c = counts[t];
+        if (c > row_stride) return ELPH_EINVAL;
+        for (int pm = 0; pm < 2; ++pm) {
+            char name[64];
+            std::snprintf(name, sizeof(name), "%s.istate%lld", pm == 0 ? "Xplus" : "Xminus",
+                          (long long)(istate_first + t));
+            FILE *f = std::fopen((std::string(dir) + "/" + name).c_str(), "wb");
+            if (!f) return ELPH_EIO;
+            const size_t o = (size_t)t * row_stride;
+            bool ok = std::fwrite(&c, sizeof(int64_t), 1, f) == 1;
+            ok = ok && std::fwrite((pm == 0 ? xplus : xminus) + o, sizeof(double), c, f) == (size_t)c;
+            ok = ok && std::fwrite(istate_el + o, sizeof(int64_t), c, f) == (size_t)c;
+            ok = ok && std::fwrite(istate_ph + o, sizeof(int64_t), c, f) == (size_t)c;
+            if (std::fclose(f) != 0 || !ok) return ELPH_EIO;
+        }
+    }
+    return ELPH_OK;
+}
+
 extern "C" int elph_symmetry(const double *lattvecs, int64_t natoms, const double *basis, const int64_t *atomtypes,
                              double symprec, const int64_t *mesh, int64_t *nsymm, int64_t *nsymm_rot,
                              double *qrotations) {

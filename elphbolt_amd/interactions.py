@@ -337,6 +337,34 @@ def read_g2_file(path):
     return count, vals
 
 
+def write_X_files(xdir, istate_first, counts, xplus, xminus, istate_el, istate_ph):
+    """Xplus.istate<i> / Xminus.istate<i> in the reference's byte layout
+    (src/interactions.f90:2356-2377) from host rows."""
+    lib = _lib.load()
+    os.makedirs(xdir, exist_ok=True)
+    c = i64(counts)
+    xp, xm = (np.ascontiguousarray(a, dtype=np.float64) for a in (xplus, xminus))
+    se, sp = (np.ascontiguousarray(a, dtype=np.int64) for a in (istate_el, istate_ph))
+    st = lib.elph_write_X_files(str(xdir).encode(), istate_first, len(c), _p(c, c_i64p), _p(xp), _p(xm),
+                                _p(se, c_i64p), _p(sp, c_i64p), xp.shape[1])
+    if st != 0:
+        raise ElphError(st, f"writing X files into {xdir} failed")
+
+
+def read_transition_probs_e(path):
+    """read_transition_probs_e (src/interactions.f90:3065-3091): (N, TP,
+    istate1, istate2)."""
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(8), np.int64)[0])
+        tp = np.frombuffer(f.read(8 * n), np.float64)
+        s1 = np.frombuffer(f.read(8 * n), np.int64)
+        s2 = np.frombuffer(f.read(8 * n), np.int64)
+        rest = f.read()
+    if rest:
+        raise ValueError(f"{path}: {len(rest)} trailing bytes")
+    return n, tp, s1, s2
+
+
 def calculate_eph_interaction_ibzk(wann: Wannier, el, ph, key="g", mesh_ref=1, precision="fp64",
                                    istate_first=1, istate_last=None, g2dir=None):
     if key not in ("g",):

@@ -229,6 +229,14 @@ int elph_el_transport_destroy(elph_el_transport *t);
  * int64 count + count doubles, 1877-1889) for states with counts >= 0. */
 int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, const int64_t *counts,
                         const double *g2, int64_t row_stride);
+/* Writes Xplus.istate<i> and Xminus.istate<i> as calculate_eph_interaction_ibzk
+ * key 'X' does (src/interactions.f90:2356-2377) and read_transition_probs_e
+ * reads them (3065-3091): int64 count, count doubles X, count int64
+ * istate_el, count int64 istate_ph; states with counts >= 0; host rows of
+ * stride row_stride (copy the device rows of elph_ibzk_plan_X first). */
+int elph_write_X_files(const char *dir, int64_t istate_first, int64_t nstates, const int64_t *counts,
+                       const double *xplus, const double *xminus, const int64_t *istate_el, const int64_t *istate_ph,
+                       int64_t row_stride);
 
 /* Setup the vertex path consumes (host + device): symmetry operations
  * (src/symmetry.f90:66-212, found by a lattice/basis search in place of

@@ -66,3 +66,27 @@ def test_symmetry_host_path():
     s2 = bz.Symmetry(L, basis, [28.0, 12.0], [6, 6, 6])
     assert s2.nsymm == 24
     assert len(bz.irred_wedge([6, 6, 6], s2)[0]) == 16
+
+
+def test_write_and_read_X_files(tmp_path):
+    """Xplus/Xminus.istate<i>: int64 count, count X, count istate_el, count
+    istate_ph (src/interactions.f90:2356-2377), as read_transition_probs_e
+    reads them (3065-3091); no file for a state outside the window."""
+    import numpy as np
+
+    from elphbolt_amd import interactions
+    counts = np.array([3, -1, 0, 4], np.int64)
+    rng = np.random.default_rng(0)
+    xp, xm = rng.random((4, 5)), rng.random((4, 5))
+    se, sp = rng.integers(1, 99, (4, 5)), -rng.integers(1, 99, (4, 5))
+    interactions.write_X_files(str(tmp_path), 7, counts, xp, xm, se, sp)
+    for t, c in enumerate(counts):
+        for name, x in (("Xplus", xp), ("Xminus", xm)):
+            path = tmp_path / f"{name}.istate{7 + t}"
+            if c < 0:
+                assert not path.exists()
+                continue
+            n, tp, s1, s2 = interactions.read_transition_probs_e(str(path))
+            assert n == c and np.array_equal(tp, x[t, :c])
+            assert np.array_equal(s1, se[t, :c]) and np.array_equal(s2, sp[t, :c])
+            assert path.stat().st_size == 8 + 24 * c
