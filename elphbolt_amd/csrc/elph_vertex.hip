@@ -403,15 +403,15 @@ __device__ __forceinline__ double pair_phase_packed(const double2 *Tsh, const in
 // Phase-contraction core of the mesh drivers: the phase of row r and cell R_K
 // is the exact root of unity T[(coef_r . R_K) mod L] (see gemm_core_f64).
 template <int CB, int RB, int KC>
-__device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int Kp, const int (&coef)[RB][3], int L,
-                                                const double2 *Tsh, const int4 *Rsh, double2 *buf0, double2 *buf1,
-                                                d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+__device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int ldb, int Kp, const int (&coef)[RB][3],
+                                                int L, const double2 *Tsh, const int4 *Rsh, double2 *buf0,
+                                                double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
     auto aop = [&](int rb, int k) {
         const int4 R = Rsh[k];
         const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
         return Tsh[pos_mod(x, L)];
     };
-    gemm_core_f64<CB, RB, KC>(hst, 16 * CB, Kp, aop, buf0, buf1, cr, ci);
+    gemm_core_f64<CB, RB, KC>(hst, ldb, Kp, aop, buf0, buf1, cr, ci);
 }
 
 // PACKED: phase indices by pair_phase_packed (L and the mesh up to 255).
@@ -1312,6 +1312,105 @@ struct PairQArgs {
     int64_t stride;
 };
 
+// ibzq epilogue of one vertex: in-window band ranks (mr at k row `row`, nr at
+// k' row kp), Y row Yr (column np + nb mp):
+//   g = sum_mp U_k(m,mp) sum_np conj(U_k'(n,np)) Y(np + nb mp) (+ polar term),
+//   |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu   (0 when omega == 0).
+__device__ __forceinline__ double ibzq_vertex(const PairQArgs &a, const double2 *Yr, int row, int kp, int mr, int nr,
+                                              double w, int st) {
+    const int nb = a.nb;
+    const int m = a.wband[row * nb + mr], n = a.wband[kp * nb + nr];
+    const double2 *Uk = a.Vk + ((size_t)row * nb + m) * nb;
+    const double2 *Ukp = a.Vk + ((size_t)kp * nb + n) * nb;
+    double2 g = make_double2(0.0, 0.0);
+    for (int mp = 0; mp < nb; ++mp) {
+        double2 z = make_double2(0.0, 0.0);
+        for (int np = 0; np < nb; ++np) {
+            const double2 v = Ukp[np];
+            const double2 y = Yr[np + nb * mp];
+            z.x = fma(v.x, y.x, z.x);
+            z.x = fma(v.y, y.y, z.x);
+            z.y = fma(v.x, y.y, z.y);
+            z.y = fma(-v.y, y.x, z.y);
+        }
+        cfma(g, Uk[mp], z);
+    }
+    if (a.st_lp) {  // src/wannier.f90:998-1006
+        double2 unm = make_double2(0.0, 0.0);
+        for (int i = 0; i < nb; ++i) {
+            const double2 z = cmulc(Ukp[i], Uk[i]);
+            unm.x += z.x;
+            unm.y += z.y;
+        }
+        cfma(g, a.st_lp[st], unm);
+    }
+    return w != 0.0 ? 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit : 0.0;
+}
+
+// General ibzq path (nb^2 beyond the fused kernel's columns): Y of a chunk of
+// states into HBM ([state of the chunk][k row][CP] complex), one 16*CB column
+// tile per blockIdx.z, then k_ibzq_epi.
+template <int CB>
+__global__ __launch_bounds__(256) void k_ibzq_y(PairQArgs a, int st0, double2 *__restrict__ Y) {
+    constexpr int KC = 32;
+    constexpr int RB = 2;
+    constexpr int CPt = 16 * CB;
+    extern __shared__ double2 smem[];
+    double2 *buf0 = smem;
+    double2 *buf1 = smem + KC * CPt;
+    double2 *Tsh = smem + 2 * KC * CPt;
+    int4 *Rsh = reinterpret_cast<int4 *>(Tsh + a.L);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int sl = blockIdx.y, st = st0 + sl, tile = blockIdx.z;
+    const int row_base = blockIdx.x * 128 + wave * 32;
+    for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
+    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rk[4 * i], a.Rk[4 * i + 1], a.Rk[4 * i + 2], 0);
+    int coef[RB][3];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+        const int r = row_base + rb * 16 + li;
+        coef[rb][0] = coef[rb][1] = coef[rb][2] = 0;
+        if (r < a.nwv) {
+            coef[rb][0] = (a.kint[3 * r] * a.Lf0) % a.L;
+            coef[rb][1] = (a.kint[3 * r + 1] * a.Lf1) % a.L;
+            coef[rb][2] = (a.kint[3 * r + 2] * a.Lf2) % a.L;
+        }
+    }
+    d4 cr[RB][CB], ci[RB][CB];
+    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * a.CP + tile * CPt, a.CP, a.Kp, coef, a.L, Tsh, Rsh, buf0,
+                                buf1, cr, ci);
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row_base + rb * 16 + lk + 4 * r;
+                if (row < a.nwv)
+                    Y[((size_t)sl * a.nwv + row) * a.CP + tile * CPt + c * 16 + li] =
+                        make_double2(cr[rb][c][r], ci[rb][c][r]);
+            }
+}
+
+// Epilogue of the general ibzq path: 4 lanes per k row, Y read from HBM.
+__global__ __launch_bounds__(256) void k_ibzq_epi(PairQArgs a, int st0, const double2 *__restrict__ Y) {
+    const int row = blockIdx.x * 64 + (threadIdx.x >> 2), lk = threadIdx.x & 3;
+    const int sl = blockIdx.y, st = st0 + sl;
+    if (row >= a.nwv) return;
+    const int kp = a.rowmap[(size_t)st * a.nwv + row];
+    if (kp < 0) return;
+    const int nm = a.nwin[row], nn = a.nwin[kp];
+    const int64_t base = a.rowoff[(size_t)st * a.nwv + row];
+    const double2 *Yr = Y + ((size_t)sl * a.nwv + row) * a.CP;
+    double *outrow = a.out + a.st_slot[st] * a.stride;
+    const double w = a.st_om[st];
+    for (int t = lk; t < nm * nn; t += 4) {
+        const int mr = t / nn, nr = t % nn;
+        outrow[base + (int64_t)mr * nn + nr] = ibzq_vertex(a, Yr, row, kp, mr, nr, w, st);
+    }
+}
+
 template <int CB>
 __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     constexpr int KC = 32;
@@ -1340,7 +1439,7 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
         }
     }
     d4 cr[RB][CB], ci[RB][CB];
-    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * CP, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
+    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * CP, CP, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
 
     const int YS = CP + 1;
     double2 *Y = smem + wave * 16 * YS;
@@ -1363,34 +1462,7 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
                 const double2 *Yr = Y + li * YS;
                 for (int t = lk; t < nm * nn; t += 4) {
                     const int mr = t / nn, nr = t % nn;
-                    const int m = a.wband[row * nb + mr], n = a.wband[kp * nb + nr];
-                    const double2 *Uk = a.Vk + ((size_t)row * nb + m) * nb;
-                    const double2 *Ukp = a.Vk + ((size_t)kp * nb + n) * nb;
-                    double2 g = make_double2(0.0, 0.0);
-                    for (int mp = 0; mp < nb; ++mp) {
-                        double2 z = make_double2(0.0, 0.0);
-                        for (int np = 0; np < nb; ++np) {
-                            const double2 v = Ukp[np];
-                            const double2 y = Yr[np + nb * mp];
-                            z.x = fma(v.x, y.x, z.x);
-                            z.x = fma(v.y, y.y, z.x);
-                            z.y = fma(v.x, y.y, z.y);
-                            z.y = fma(-v.y, y.x, z.y);
-                        }
-                        cfma(g, Uk[mp], z);
-                    }
-                    if (a.st_lp) {  // src/wannier.f90:998-1006
-                        double2 unm = make_double2(0.0, 0.0);
-                        for (int i = 0; i < nb; ++i) {
-                            const double2 z = cmulc(Ukp[i], Uk[i]);
-                            unm.x += z.x;
-                            unm.y += z.y;
-                        }
-                        cfma(g, a.st_lp[st], unm);
-                    }
-                    double g2 = 0.0;
-                    if (w != 0.0) g2 = 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
-                    outrow[base + (int64_t)mr * nn + nr] = g2;
+                    outrow[base + (int64_t)mr * nn + nr] = ibzq_vertex(a, Yr, row, kp, mr, nr, w, st);
                 }
             }
         }
@@ -3795,6 +3867,8 @@ struct elph_ibzq_plan {
     DBuf<int> st_qb, st_qe, rowmap;
     DBuf<int64_t> rowoff, st_tot, st_slot;
     DBuf<double> g2stage;
+    bool general = false;  // nb^2 beyond the fused kernel: Y tiles in HBM, then the epilogue
+    DBuf<double2> Y;
 };
 
 extern "C" {
@@ -3840,8 +3914,9 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         p->nbr = nbr;
         p->nwv = nwv;
         p->stride = el->nstates_inwindow * nb;
-        if (nb * nb > 96) throw Error{ELPH_EINVAL, "numwannbands^2 > 96 is not supported by the fused ibzq kernel"};
-        p->CP = ((nb * nb + 15) / 16) * 16;
+        // the fused kernel holds up to 96 columns; beyond, column tiles of 32
+        p->general = nb * nb > 96;
+        p->CP = p->general ? ((nb * nb + 31) / 32) * 32 : ((nb * nb + 15) / 16) * 16;
         p->Kp = ((ctx->nwsk + 31) / 32) * 32;
         p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
@@ -4015,7 +4090,21 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
                                         (size_t)64 * (p->CP + 1) * sizeof(double2));
             dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst);
             HCHECK(hipEventRecord(c->ev_a, s));
-            switch (CB) {
+            if (p->general) {
+                // chunks of states whose Y fits 512 MB
+                constexpr int CBg = 2;
+                const int64_t per = p->nwv * p->CP * (int64_t)sizeof(double2);
+                const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per));
+                p->Y.alloc(chunk * p->nwv * p->CP);
+                const size_t ldsg = (size_t)(2 * 32 * 16 * CBg + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+                for (int64_t t0 = 0; t0 < nst; t0 += chunk) {
+                    const int64_t nc = std::min(chunk, nst - t0);
+                    dim3 gy((unsigned)((p->nwv + 127) / 128), (unsigned)nc, (unsigned)(p->CP / (16 * CBg)));
+                    hipLaunchKernelGGL(k_ibzq_y<CBg>, gy, dim3(256), ldsg, s, a, (int)t0, p->Y.p);
+                    dim3 ge((unsigned)((p->nwv + 63) / 64), (unsigned)nc);
+                    hipLaunchKernelGGL(k_ibzq_epi, ge, dim3(256), 0, s, a, (int)t0, p->Y.p);
+                }
+            } else switch (CB) {
                 case 1: hipLaunchKernelGGL(k_pair_q_fp64<1>, grid, dim3(256), lds, s, a); break;
                 case 2: hipLaunchKernelGGL(k_pair_q_fp64<2>, grid, dim3(256), lds, s, a); break;
                 case 3: hipLaunchKernelGGL(k_pair_q_fp64<3>, grid, dim3(256), lds, s, a); break;
