@@ -152,7 +152,8 @@ int elph_ibzk_g2(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph
 /* calculate_eph_interaction_ibzq, key 'g' (src/interactions.f90:1673-1930).
  * Phonon state = (iq-1)*nbr + s over IBZ q.  g2 row stride =
  * nstates_inwindow*nb (the reference sizes its buffer nstates_inwindow*nbr,
- * 1724, which a window with more than nbr band pairs per k would overrun). */
+ * 1724, which a window with more than nbr band pairs per k would overrun).
+ * The ibzq contractions run in FP64 for either precision value. */
 int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int32_t precision,
                           elph_ibzq_plan **plan);
 int elph_ibzq_plan_run(elph_ibzq_plan *plan, int64_t istate_first, int64_t istate_last, int64_t *counts,
