@@ -289,6 +289,40 @@ def transport_bench(args, w, sym, el, plan, g2, nst):
             "sigma": sig, "sigmaS": sigS, "seebeck_V_per_K": sigS / sig if sig else None}
 
 
+def ibzq_bench(args, w, el, ph, mat):
+    """calculate_eph_interaction_ibzq('g') over all IBZ phonon states of the
+    same workload (FP64), output rows in HBM."""
+    from elphbolt_amd import interactions
+    plan = interactions.IbzqPlan(w, el, ph, precision="fp64")
+    nst = ph.nwv_irred * mat.nbr
+    out = w.device_buffer((nst, plan.stride))
+    counts, _ = plan.run(1, nst, out=out)            # warm-up
+    dom, tot, walls = [], [], []
+    for i in range(args.ibzq_steps):
+        w.synchronize()
+        t = time.perf_counter()
+        counts, _ = plan.run(1, nst, out=out)
+        w.synchronize()
+        walls.append(time.perf_counter() - t)
+        tm = plan.timing()
+        dom.append(tm["dominant_ms"])
+        tot.append(tm["all_ms"])
+        log(f"[bench] ibzq step {i}: {walls[-1] * 1e3:.1f} ms wall, pair kernels {tm['dominant_ms']:.1f} ms "
+            f"in {tm['launches']} launches")
+    vertices = int(counts.sum())
+    k2 = -(-pair_rows(mat.rcells_k) // 32) * 32    # cos/sin rows of the Re cell pairs, padded
+    flops = 4.0 * k2 * mat.nb ** 2 * el.nwv * nst   # real phase x complex h, per (state, k row)
+    ach = flops / (np.mean(dom) * 1e-3) / 1e12
+    out.free()
+    plan.close()
+    return {"workload": f"all {nst} IBZ phonon states (q, s), {el.nwv} k rows each, FP64",
+            "vertices_per_step": vertices, "value": vertices / float(np.mean(walls)), "unit": "vertices/s",
+            "ms_per_step": round(float(np.mean(walls)) * 1e3, 2),
+            "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(ach / FP64_PEAK_TFLOPS, 4), "kernel": "k_pair_q_fp64",
+                         "pair_ms_per_step": round(float(np.mean(dom)), 2), "flops_per_step": flops}}
+
+
 def _batched_material(i, budget):
     from elphbolt_amd import synth
     return synth.multi_material(i, budget_bytes=budget, bands_in_window=4.0)
@@ -425,6 +459,8 @@ def main():
     ap.add_argument("--calibration-copy-gb", type=float, default=1.0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-transport", action="store_true")
+    ap.add_argument("--no-ibzq", action="store_true")
+    ap.add_argument("--ibzq-steps", type=int, default=2)
     ap.add_argument("--transport-steps", type=int, default=3)
     ap.add_argument("--batched-materials", type=int, default=64)
     ap.add_argument("--batched-budget-gb", type=float, default=2.0)
@@ -437,7 +473,7 @@ def main():
                                    "described in DESIGN.md, not run)"}))
         sys.exit(2)
     if args.profile:
-        args.no_batched = args.no_serving = args.no_cpu_baseline = True
+        args.no_batched = args.no_serving = args.no_cpu_baseline = args.no_ibzq = True
     gen = None if args.no_batched else start_batched_generation(args)
 
     from elphbolt_amd import bz, interactions, synth
@@ -556,6 +592,9 @@ def main():
     if not args.no_transport and nblocks == 1:
         res["transport"] = transport_bench(args, w, sym, el, plans["fp64"], out, kb * nb)
         log_mem("after transport")
+    if not args.no_ibzq:
+        res["ibzq"] = ibzq_bench(args, w, el, ph, mat)
+        log_mem("after ibzq")
     if not args.no_batched:
         log_mem("before batched")
         res["batched"] = batched_bench(args, gen)

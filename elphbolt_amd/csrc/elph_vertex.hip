@@ -406,10 +406,12 @@ template <int CB, int RB, int KC>
 __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int ldb, int Kp, const int (&coef)[RB][3],
                                                 int L, const double2 *Tsh, const int4 *Rsh, double2 *buf0,
                                                 double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+    // row 2p: cos(2 pi k.R_p), row 2p + 1: sin(2 pi k.R_p) (Rsh[p] = R_p)
     auto aop = [&](int rb, int k) {
-        const int4 R = Rsh[k];
+        const int4 R = Rsh[k >> 1];
         const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
-        return Tsh[pos_mod(x, L)];
+        const double2 t = Tsh[pos_mod(x, L)];
+        return (k & 1) ? t.y : t.x;
     };
     gemm_core_f64<CB, RB, KC>(hst, ldb, Kp, aop, buf0, buf1, cr, ci);
 }
@@ -1226,26 +1228,38 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
 
 // Phonon-branch rotation of gReq for phonon state (q, s):
 //   hq[st][Re][c] = sum_sp u(s,sp) gReq[q](np,mp,sp,Re) / elwsdeg(Re),  c = np + nb mp
-__global__ void k_rotate_hq(int nst, int nb, int nbr, int nwsk, int Kp, int CP, const double2 *__restrict__ gq,
+// hq[st][K][CP] for the ibzq contraction in cell-pair form over Re: with
+// h(Re) = sum_sp u(st, sp) g(Re, q)[:, sp] / elwsdeg(Re) (column c = np + nb mp),
+// row 2p = h(R_p) + h(-R_p), row 2p + 1 = i (h(R_p) - h(-R_p)); pairs[p] =
+// (index of R_p, index of -R_p or -1); rows past 2 npair are zero.
+__global__ void k_rotate_hq(int nst, int nb, int nbr, int npair, int Kp, int CP, const double2 *__restrict__ gq,
                             int64_t gq_stride, const int *__restrict__ st_qb, const double2 *__restrict__ st_u,
-                            const int *__restrict__ degk, double2 *__restrict__ h) {
+                            const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
+                            double2 *__restrict__ h) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t per = (int64_t)Kp * CP;
     if (idx >= per * nst) return;
     const int st = (int)(idx / per);
     const int rem = (int)(idx % per);
-    const int re = rem / CP, c = rem % CP;
+    const int row = rem / CP, c = rem % CP, pr = row >> 1;
     const int nb2 = nb * nb;
-    double2 acc = make_double2(0.0, 0.0);
-    if (re < nwsk && c < nb2) {
-        const double2 *g = gq + (size_t)st_qb[st] * gq_stride + c + (size_t)nb2 * nbr * re;
+    double2 out = make_double2(0.0, 0.0);
+    if (pr < npair && c < nb2) {
         const double2 *u = st_u + (size_t)st * nbr;
-        for (int sp = 0; sp < nbr; ++sp) cfma(acc, u[sp], g[(size_t)sp * nb2]);
-        const double d = (double)degk[re];
-        acc.x /= d;
-        acc.y /= d;
+        const double2 *gb = gq + (size_t)st_qb[st] * gq_stride + c;
+        auto hcell = [&](int re) {
+            double2 acc = make_double2(0.0, 0.0);
+            const double2 *g = gb + (size_t)nb2 * nbr * re;
+            for (int sp = 0; sp < nbr; ++sp) cfma(acc, u[sp], g[(size_t)sp * nb2]);
+            const double d = invdeg[re];
+            return make_double2(acc.x * d, acc.y * d);
+        };
+        const int2 q = pairs[pr];
+        const double2 hi = hcell(q.x);
+        const double2 hj = q.y >= 0 ? hcell(q.y) : make_double2(0.0, 0.0);
+        out = (row & 1) ? make_double2(-(hi.y - hj.y), hi.x - hj.x) : make_double2(hi.x + hj.x, hi.y + hj.y);
     }
-    h[idx] = acc;
+    h[idx] = out;
 }
 
 // Per state and k row: k' = k + q on the electron mesh (vector_allreps_add,
@@ -1299,7 +1313,8 @@ struct PairQArgs {
     const int *kint;           // [nwv][3]
     int L, Lf0, Lf1, Lf2;
     const double2 *T;
-    const int *Rk;             // [Kp][4]
+    const int *Rk;             // [npair][3]: first cell R_p of each Re pair
+    int npair;
     const double2 *Vk;         // [nwv][n][np]
     const int *nwin, *wband;
     const int *rowmap;         // [st][nwv]
@@ -1311,6 +1326,8 @@ struct PairQArgs {
     double *out;
     int64_t stride;
 };
+
+constexpr int kPairQKC = 32;  // K rows per LDS chunk of the fused ibzq kernel
 
 // ibzq epilogue of one vertex: in-window band ranks (mr at k row `row`, nr at
 // k' row kp), Y row Yr (column np + nb mp):
@@ -1365,7 +1382,8 @@ __global__ __launch_bounds__(256) void k_ibzq_y(PairQArgs a, int st0, double2 *_
     const int sl = blockIdx.y, st = st0 + sl, tile = blockIdx.z;
     const int row_base = blockIdx.x * 128 + wave * 32;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rk[4 * i], a.Rk[4 * i + 1], a.Rk[4 * i + 2], 0);
+    for (int i = tid; i < a.Kp / 2; i += 256)
+        Rsh[i] = i < a.npair ? make_int4(a.Rk[3 * i], a.Rk[3 * i + 1], a.Rk[3 * i + 2], 0) : make_int4(0, 0, 0, 0);
     int coef[RB][3];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -1413,7 +1431,7 @@ __global__ __launch_bounds__(256) void k_ibzq_epi(PairQArgs a, int st0, const do
 
 template <int CB>
 __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
-    constexpr int KC = 32;
+    constexpr int KC = kPairQKC;
     constexpr int RB = 2;
     const int CP = 16 * CB;
     extern __shared__ double2 smem[];
@@ -1426,7 +1444,8 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     const int st = blockIdx.y;
     const int row_base = blockIdx.x * 128 + wave * 32;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp; i += 256) Rsh[i] = make_int4(a.Rk[4 * i], a.Rk[4 * i + 1], a.Rk[4 * i + 2], 0);
+    for (int i = tid; i < a.Kp / 2; i += 256)
+        Rsh[i] = i < a.npair ? make_int4(a.Rk[3 * i], a.Rk[3 * i + 1], a.Rk[3 * i + 2], 0) : make_int4(0, 0, 0, 0);
     int coef[RB][3];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -1445,7 +1464,6 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     double2 *Y = smem + wave * 16 * YS;
     double *outrow = a.out + a.st_slot[st] * a.stride;
     const double w = a.st_om[st];
-    const int nb = a.nb;
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
 #pragma unroll
@@ -3860,7 +3878,7 @@ struct elph_ibzq_plan {
     std::vector<int64_t> q_irr;   // IBZ q muxed (phonon mesh)
     std::vector<double> ph_ens;   // (nq, nbr)
     std::vector<double> ph_evecs; // (nq, nbr, nbr) c128
-    DBuf<int> kint, nwin, wband, lut, Rk;
+    DBuf<int> kint, nwin, wband, lut;
     DBuf<double2> Vk, T;
     DBuf<double2> gq, h, st_u, Lq, st_lp;
     DBuf<double> qv, st_om;
@@ -3917,7 +3935,8 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         // the fused kernel holds up to 96 columns; beyond, column tiles of 32
         p->general = nb * nb > 96;
         p->CP = p->general ? ((nb * nb + 31) / 32) * 32 : ((nb * nb + 15) / 16) * 16;
-        p->Kp = ((ctx->nwsk + 31) / 32) * 32;
+        // cos/sin rows of the Re cell pairs, padded to the chunk of 32 rows
+        p->Kp = ((2 * ctx->npairk + 31) / 32) * 32;
         p->L = lcm64(lcm64(p->Nel[0], p->Nel[1]), p->Nel[2]);
         for (int d = 0; d < 3; ++d) p->Lf[d] = p->L / p->Nel[d];
         hipStream_t s = ctx->stream;
@@ -3950,10 +3969,6 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             T[j] = make_double2(std::cos(x), std::sin(x));
         }
         p->T.upload(T, s);
-        std::vector<int> R(4 * p->Kp, 0);
-        for (int64_t i = 0; i < ctx->nwsk; ++i)
-            for (int d = 0; d < 3; ++d) R[4 * i + d] = (int)ctx->rcells_k[i + ctx->nwsk * d];
-        p->Rk.upload(R, s);
         p->nirr_q = ph->nwv_irred;
         p->nq = ph->nwv;
         p->q_irr.assign(ph->indexlist_irred, ph->indexlist_irred + ph->nwv_irred);
@@ -4049,8 +4064,8 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             p->h.alloc(nst * p->Kp * p->CP);
             const int64_t tot = nst * p->Kp * p->CP;
             hipLaunchKernelGGL(k_rotate_hq, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, (int)nst, (int)nb,
-                               (int)nbr, (int)c->nwsk, (int)p->Kp, (int)p->CP, p->gq.p, gblk, p->st_qb.p, p->st_u.p,
-                               c->degk.p, p->h.p);
+                               (int)nbr, (int)c->npairk, (int)p->Kp, (int)p->CP, p->gq.p, gblk, p->st_qb.p, p->st_u.p,
+                               c->gpair.p, c->ginvdeg.p, p->h.p);
             p->rowmap.alloc(nst * p->nwv);
             p->rowoff.alloc(nst * p->nwv);
             p->st_tot.alloc(nst);
@@ -4073,7 +4088,8 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             a.L = (int)p->L;
             a.Lf0 = (int)p->Lf[0]; a.Lf1 = (int)p->Lf[1]; a.Lf2 = (int)p->Lf[2];
             a.T = p->T.p;
-            a.Rk = p->Rk.p;
+            a.Rk = c->gpcell.p;
+            a.npair = (int)c->npairk;
             a.Vk = p->Vk.p;
             a.nwin = p->nwin.p;
             a.wband = p->wband.p;
@@ -4086,7 +4102,8 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
             a.out = dst;
             a.stride = p->stride;
             const int CB = (int)(p->CP / 16);
-            const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4),
+            const size_t lds = std::max((size_t)(2 * kPairQKC * p->CP + p->L) * sizeof(double2) +
+                                            (size_t)(p->Kp / 2) * sizeof(int4),
                                         (size_t)64 * (p->CP + 1) * sizeof(double2));
             dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst);
             HCHECK(hipEventRecord(c->ev_a, s));
