@@ -402,16 +402,26 @@ __device__ __forceinline__ double pair_phase_packed(const double2 *Tsh, const in
 
 // Phase-contraction core of the mesh drivers: the phase of row r and cell R_K
 // is the exact root of unity T[(coef_r . R_K) mod L] (see gemm_core_f64).
-template <int CB, int RB, int KC>
+// PACKED (L and the mesh up to 255): the index by pair_phase_packed from the
+// byte-packed coefficients cpk and Rsh[p].w = R_p mod the mesh.
+template <int CB, int RB, int KC, bool PACKED>
 __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst, int ldb, int Kp, const int (&coef)[RB][3],
-                                                int L, const double2 *Tsh, const int4 *Rsh, double2 *buf0,
-                                                double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+                                                int L, unsigned lmul, const double2 *Tsh, const int4 *Rsh,
+                                                double2 *buf0, double2 *buf1, d4 (&cr)[RB][CB], d4 (&ci)[RB][CB]) {
+    unsigned cpk[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+        cpk[rb] = (unsigned)coef[rb][0] | ((unsigned)coef[rb][1] << 8) | ((unsigned)coef[rb][2] << 16);
     // row 2p: cos(2 pi k.R_p), row 2p + 1: sin(2 pi k.R_p) (Rsh[p] = R_p)
     auto aop = [&](int rb, int k) {
-        const int4 R = Rsh[k >> 1];
-        const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
-        const double2 t = Tsh[pos_mod(x, L)];
-        return (k & 1) ? t.y : t.x;
+        if constexpr (PACKED) {
+            return pair_phase_packed(Tsh, Rsh, cpk[rb], lmul, -L, k);
+        } else {
+            const int4 R = Rsh[k >> 1];
+            const int x = coef[rb][0] * R.x + coef[rb][1] * R.y + coef[rb][2] * R.z;
+            const double2 t = Tsh[pos_mod(x, L)];
+            return (k & 1) ? t.y : t.x;
+        }
     };
     gemm_core_f64<CB, RB, KC>(hst, ldb, Kp, aop, buf0, buf1, cr, ci);
 }
@@ -1315,6 +1325,8 @@ struct PairQArgs {
     const double2 *T;
     const int *Rk;             // [npair][3]: first cell R_p of each Re pair
     int npair;
+    int Nel0, Nel1, Nel2;      // electron mesh
+    unsigned Lmul;             // ceil(2^32 / L): index mod L by multiply-high
     const double2 *Vk;         // [nwv][n][np]
     const int *nwin, *wband;
     const int *rowmap;         // [st][nwv]
@@ -1329,45 +1341,59 @@ struct PairQArgs {
 
 constexpr int kPairQKC = 32;  // K rows per LDS chunk of the fused ibzq kernel
 
-// ibzq epilogue of one vertex: in-window band ranks (mr at k row `row`, nr at
-// k' row kp), Y row Yr (column np + nb mp):
-//   g = sum_mp U_k(m,mp) sum_np conj(U_k'(n,np)) Y(np + nb mp) (+ polar term),
-//   |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu   (0 when omega == 0).
-__device__ __forceinline__ double ibzq_vertex(const PairQArgs &a, const double2 *Yr, int row, int kp, int mr, int nr,
-                                              double w, int st) {
+// ibzq epilogue of one k row (k' row kp), in-window band ranks mr = lane0,
+// lane0 + nlanes, ...: W(np) = sum_mp U_k(m,mp) Y(np + nb mp) once per m, then
+// for every in-window n, g = sum_np conj(U_k'(n,np)) W(np) (+ polar term,
+// src/wannier.f90:998-1006) and |g|^2 = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu
+// (0 when omega == 0): the per-vertex sum sum_mp U_k(m,mp) sum_np
+// conj(U_k'(n,np)) Y re-associated, nb^2 + nn nb products per m instead of
+// nn (nb^2 + nb).  out: the row's vertices (mr * nn + nr).  nb <= NBX.
+template <int NBX>
+__device__ __forceinline__ void ibzq_row(const PairQArgs &a, const double2 *Yr, int row, int kp, int nm, int nn,
+                                         int lane0, int nlanes, double w, int st, double *out) {
     const int nb = a.nb;
-    const int m = a.wband[row * nb + mr], n = a.wband[kp * nb + nr];
-    const double2 *Uk = a.Vk + ((size_t)row * nb + m) * nb;
-    const double2 *Ukp = a.Vk + ((size_t)kp * nb + n) * nb;
-    double2 g = make_double2(0.0, 0.0);
-    for (int mp = 0; mp < nb; ++mp) {
-        double2 z = make_double2(0.0, 0.0);
-        for (int np = 0; np < nb; ++np) {
-            const double2 v = Ukp[np];
-            const double2 y = Yr[np + nb * mp];
-            z.x = fma(v.x, y.x, z.x);
-            z.x = fma(v.y, y.y, z.x);
-            z.y = fma(v.x, y.y, z.y);
-            z.y = fma(-v.y, y.x, z.y);
+    for (int mr = lane0; mr < nm; mr += nlanes) {
+        const int m = a.wband[row * nb + mr];
+        const double2 *Uk = a.Vk + ((size_t)row * nb + m) * nb;
+        double2 W[NBX];
+#pragma unroll
+        for (int np = 0; np < NBX; ++np) W[np] = make_double2(0.0, 0.0);
+        for (int mp = 0; mp < nb; ++mp) {
+            const double2 u = Uk[mp];
+            const double2 *Ym = Yr + nb * mp;
+#pragma unroll
+            for (int np = 0; np < NBX; ++np)
+                if (np < nb) cfma(W[np], u, Ym[np]);
         }
-        cfma(g, Uk[mp], z);
-    }
-    if (a.st_lp) {  // src/wannier.f90:998-1006
-        double2 unm = make_double2(0.0, 0.0);
-        for (int i = 0; i < nb; ++i) {
-            const double2 z = cmulc(Ukp[i], Uk[i]);
-            unm.x += z.x;
-            unm.y += z.y;
+        for (int nr = 0; nr < nn; ++nr) {
+            const int n = a.wband[kp * nb + nr];
+            const double2 *Ukp = a.Vk + ((size_t)kp * nb + n) * nb;
+            double2 g = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int np = 0; np < NBX; ++np)
+                if (np < nb) {
+                    const double2 z = cmulc(Ukp[np], W[np]);
+                    g.x += z.x;
+                    g.y += z.y;
+                }
+            if (a.st_lp) {  // src/wannier.f90:998-1006
+                double2 unm = make_double2(0.0, 0.0);
+                for (int i = 0; i < nb; ++i) {
+                    const double2 z = cmulc(Ukp[i], Uk[i]);
+                    unm.x += z.x;
+                    unm.y += z.y;
+                }
+                cfma(g, a.st_lp[st], unm);
+            }
+            out[(int64_t)mr * nn + nr] = w != 0.0 ? 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit : 0.0;
         }
-        cfma(g, a.st_lp[st], unm);
     }
-    return w != 0.0 ? 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit : 0.0;
 }
 
 // General ibzq path (nb^2 beyond the fused kernel's columns): Y of a chunk of
 // states into HBM ([state of the chunk][k row][CP] complex), one 16*CB column
 // tile per blockIdx.z, then k_ibzq_epi.
-template <int CB>
+template <int CB, bool PACKED>
 __global__ __launch_bounds__(256) void k_ibzq_y(PairQArgs a, int st0, double2 *__restrict__ Y) {
     constexpr int KC = 32;
     constexpr int RB = 2;
@@ -1382,8 +1408,16 @@ __global__ __launch_bounds__(256) void k_ibzq_y(PairQArgs a, int st0, double2 *_
     const int sl = blockIdx.y, st = st0 + sl, tile = blockIdx.z;
     const int row_base = blockIdx.x * 128 + wave * 32;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp / 2; i += 256)
-        Rsh[i] = i < a.npair ? make_int4(a.Rk[3 * i], a.Rk[3 * i + 1], a.Rk[3 * i + 2], 0) : make_int4(0, 0, 0, 0);
+    for (int i = tid; i < a.Kp / 2; i += 256) {
+        int4 R = make_int4(0, 0, 0, 0);
+        if (i < a.npair) {
+            R.x = a.Rk[3 * i];
+            R.y = a.Rk[3 * i + 1];
+            R.z = a.Rk[3 * i + 2];
+            R.w = pos_mod(R.x, a.Nel0) | (pos_mod(R.y, a.Nel1) << 8) | (pos_mod(R.z, a.Nel2) << 16);
+        }
+        Rsh[i] = R;
+    }
     int coef[RB][3];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -1396,8 +1430,8 @@ __global__ __launch_bounds__(256) void k_ibzq_y(PairQArgs a, int st0, double2 *_
         }
     }
     d4 cr[RB][CB], ci[RB][CB];
-    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * a.CP + tile * CPt, a.CP, a.Kp, coef, a.L, Tsh, Rsh, buf0,
-                                buf1, cr, ci);
+    phase_gemm_core<CB, RB, KC, PACKED>(a.h + (size_t)st * a.Kp * a.CP + tile * CPt, a.CP, a.Kp, coef, a.L, a.Lmul,
+                                        Tsh, Rsh, buf0, buf1, cr, ci);
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -1422,14 +1456,10 @@ __global__ __launch_bounds__(256) void k_ibzq_epi(PairQArgs a, int st0, const do
     const int64_t base = a.rowoff[(size_t)st * a.nwv + row];
     const double2 *Yr = Y + ((size_t)sl * a.nwv + row) * a.CP;
     double *outrow = a.out + a.st_slot[st] * a.stride;
-    const double w = a.st_om[st];
-    for (int t = lk; t < nm * nn; t += 4) {
-        const int mr = t / nn, nr = t % nn;
-        outrow[base + (int64_t)mr * nn + nr] = ibzq_vertex(a, Yr, row, kp, mr, nr, w, st);
-    }
+    ibzq_row<32>(a, Yr, row, kp, nm, nn, lk, 4, a.st_om[st], st, outrow + base);
 }
 
-template <int CB>
+template <int CB, bool PACKED>
 __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     constexpr int KC = kPairQKC;
     constexpr int RB = 2;
@@ -1444,8 +1474,16 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
     const int st = blockIdx.y;
     const int row_base = blockIdx.x * 128 + wave * 32;
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
-    for (int i = tid; i < a.Kp / 2; i += 256)
-        Rsh[i] = i < a.npair ? make_int4(a.Rk[3 * i], a.Rk[3 * i + 1], a.Rk[3 * i + 2], 0) : make_int4(0, 0, 0, 0);
+    for (int i = tid; i < a.Kp / 2; i += 256) {
+        int4 R = make_int4(0, 0, 0, 0);
+        if (i < a.npair) {
+            R.x = a.Rk[3 * i];
+            R.y = a.Rk[3 * i + 1];
+            R.z = a.Rk[3 * i + 2];
+            R.w = pos_mod(R.x, a.Nel0) | (pos_mod(R.y, a.Nel1) << 8) | (pos_mod(R.z, a.Nel2) << 16);
+        }
+        Rsh[i] = R;
+    }
     int coef[RB][3];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
@@ -1458,7 +1496,8 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
         }
     }
     d4 cr[RB][CB], ci[RB][CB];
-    phase_gemm_core<CB, RB, KC>(a.h + (size_t)st * a.Kp * CP, CP, a.Kp, coef, a.L, Tsh, Rsh, buf0, buf1, cr, ci);
+    phase_gemm_core<CB, RB, KC, PACKED>(a.h + (size_t)st * a.Kp * CP, CP, a.Kp, coef, a.L, a.Lmul, Tsh, Rsh, buf0, buf1,
+                                        cr, ci);
 
     const int YS = CP + 1;
     double2 *Y = smem + wave * 16 * YS;
@@ -1477,11 +1516,7 @@ __global__ __launch_bounds__(256) void k_pair_q_fp64(PairQArgs a) {
             if (kp >= 0) {
                 const int nm = a.nwin[row], nn = a.nwin[kp];
                 const int64_t base = a.rowoff[(size_t)st * a.nwv + row];
-                const double2 *Yr = Y + li * YS;
-                for (int t = lk; t < nm * nn; t += 4) {
-                    const int mr = t / nn, nr = t % nn;
-                    outrow[base + (int64_t)mr * nn + nr] = ibzq_vertex(a, Yr, row, kp, mr, nr, w, st);
-                }
+                ibzq_row<9>(a, Y + li * YS, row, kp, nm, nn, lk, 4, w, st, outrow + base);
             }
         }
         __syncthreads();
@@ -4107,29 +4142,42 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
                                         (size_t)64 * (p->CP + 1) * sizeof(double2));
             dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst);
             HCHECK(hipEventRecord(c->ev_a, s));
-            if (p->general) {
-                // chunks of states whose Y fits 512 MB
-                constexpr int CBg = 2;
-                const int64_t per = p->nwv * p->CP * (int64_t)sizeof(double2);
-                const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per));
-                p->Y.alloc(chunk * p->nwv * p->CP);
-                const size_t ldsg = (size_t)(2 * 32 * 16 * CBg + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
-                for (int64_t t0 = 0; t0 < nst; t0 += chunk) {
-                    const int64_t nc = std::min(chunk, nst - t0);
-                    dim3 gy((unsigned)((p->nwv + 127) / 128), (unsigned)nc, (unsigned)(p->CP / (16 * CBg)));
-                    hipLaunchKernelGGL(k_ibzq_y<CBg>, gy, dim3(256), ldsg, s, a, (int)t0, p->Y.p);
-                    dim3 ge((unsigned)((p->nwv + 63) / 64), (unsigned)nc);
-                    hipLaunchKernelGGL(k_ibzq_epi, ge, dim3(256), 0, s, a, (int)t0, p->Y.p);
+            a.Nel0 = (int)p->Nel[0];
+            a.Nel1 = (int)p->Nel[1];
+            a.Nel2 = (int)p->Nel[2];
+            a.Lmul = phase_lmul(p->L);
+            const bool packed = p->L <= 255 && p->Nel[0] <= 255 && p->Nel[1] <= 255 && p->Nel[2] <= 255;
+            auto launch = [&](auto pk) {
+                constexpr bool PK = decltype(pk)::value;
+                if (p->general) {
+                    // chunks of states whose Y fits 512 MB
+                    constexpr int CBg = 2;
+                    const int64_t per = p->nwv * p->CP * (int64_t)sizeof(double2);
+                    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per));
+                    p->Y.alloc(chunk * p->nwv * p->CP);
+                    const size_t ldsg =
+                        (size_t)(2 * 32 * 16 * CBg + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+                    for (int64_t t0 = 0; t0 < nst; t0 += chunk) {
+                        const int64_t nc = std::min(chunk, nst - t0);
+                        dim3 gy((unsigned)((p->nwv + 127) / 128), (unsigned)nc, (unsigned)(p->CP / (16 * CBg)));
+                        hipLaunchKernelGGL((k_ibzq_y<CBg, PK>), gy, dim3(256), ldsg, s, a, (int)t0, p->Y.p);
+                        dim3 ge((unsigned)((p->nwv + 63) / 64), (unsigned)nc);
+                        hipLaunchKernelGGL(k_ibzq_epi, ge, dim3(256), 0, s, a, (int)t0, p->Y.p);
+                    }
+                    return;
                 }
-            } else switch (CB) {
-                case 1: hipLaunchKernelGGL(k_pair_q_fp64<1>, grid, dim3(256), lds, s, a); break;
-                case 2: hipLaunchKernelGGL(k_pair_q_fp64<2>, grid, dim3(256), lds, s, a); break;
-                case 3: hipLaunchKernelGGL(k_pair_q_fp64<3>, grid, dim3(256), lds, s, a); break;
-                case 4: hipLaunchKernelGGL(k_pair_q_fp64<4>, grid, dim3(256), lds, s, a); break;
-                case 5: hipLaunchKernelGGL(k_pair_q_fp64<5>, grid, dim3(256), lds, s, a); break;
-                case 6: hipLaunchKernelGGL(k_pair_q_fp64<6>, grid, dim3(256), lds, s, a); break;
-                default: throw Error{ELPH_EINVAL, "unsupported column count"};
-            }
+                switch (CB) {
+                    case 1: hipLaunchKernelGGL((k_pair_q_fp64<1, PK>), grid, dim3(256), lds, s, a); break;
+                    case 2: hipLaunchKernelGGL((k_pair_q_fp64<2, PK>), grid, dim3(256), lds, s, a); break;
+                    case 3: hipLaunchKernelGGL((k_pair_q_fp64<3, PK>), grid, dim3(256), lds, s, a); break;
+                    case 4: hipLaunchKernelGGL((k_pair_q_fp64<4, PK>), grid, dim3(256), lds, s, a); break;
+                    case 5: hipLaunchKernelGGL((k_pair_q_fp64<5, PK>), grid, dim3(256), lds, s, a); break;
+                    case 6: hipLaunchKernelGGL((k_pair_q_fp64<6, PK>), grid, dim3(256), lds, s, a); break;
+                    default: throw Error{ELPH_EINVAL, "unsupported column count"};
+                }
+            };
+            if (packed) launch(std::true_type{});
+            else launch(std::false_type{});
             HCHECK(hipGetLastError());
             HCHECK(hipEventRecord(c->ev_b, s));
             std::vector<int64_t> tot_h(nst);
