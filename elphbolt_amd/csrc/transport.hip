@@ -225,29 +225,14 @@ __device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int 
     return d < 1.0e-12 ? 0.0 : d / t.numtetra;
 }
 
-// delta_tetra at two energies of the same (point, band), one header read.
-__device__ __forceinline__ void delta_pair(const TetraView &t, double e0, double e1, int r, int n, double &d0,
-                                           double &d1) {
-    const size_t pr = (size_t)r * t.nb + n;
+// delta_fn_tetra at e of the pair pr = r nb + n, e known to lie in the
+// pair's range.
+__device__ __forceinline__ double delta_of_pair(const TetraView &t, double e, int pr) {
     const DeltaHead *h = t.head + pr;
-    const double lo = h->lo, hi = h->hi;
-    const bool in0 = !(e0 < lo || e0 > hi), in1 = !(e1 < lo || e1 > hi);
-    d0 = 0.0;
-    d1 = 0.0;
-    if (!(in0 || in1)) return;
     const int m = h->m;
-    double s0 = 0.0, s1 = 0.0;
-    if (m > 0) {
-        const DeltaPiece *pc = t.piece + pr * kMaxPieces;
-        const double scale = h->binscale;
-        if (in0) s0 = table_sum(h, pc, m, lo, scale, e0);
-        if (in1) s1 = table_sum(h, pc, m, lo, scale, e1);
-    } else {
-        if (in0) s0 = weight_sum(t, e0, r, n);
-        if (in1) s1 = weight_sum(t, e1, r, n);
-    }
-    if (s0 >= 1.0e-12) d0 = s0 / t.numtetra;
-    if (s1 >= 1.0e-12) d1 = s1 / t.numtetra;
+    const double d = m > 0 ? table_sum(h, t.piece + (size_t)pr * kMaxPieces, m, h->lo, h->binscale, e)
+                           : weight_sum(t, e, pr / t.nb, pr % t.nb);
+    return d >= 1.0e-12 ? d / t.numtetra : 0.0;
 }
 
 struct ChebFit {
@@ -385,17 +370,31 @@ constexpr int kXPer = 4;  // processes per thread of k_ibzk_X (strided by the bl
 // st_row its row in g2 / X): per process the loop body of
 // src/interactions.f90:2278-2332, X scaled by 2 pi / hbar (2351-2354), and the
 // block's sum of X+ + X- for the rates.  Block: 256 threads x kXPer
-// processes of one row.
+// processes of one row.  The few energies that fall inside a (k', n)
+// tetrahedra range are gathered in LDS and their deltas evaluated by the
+// whole block, so no wave runs the table search for one or two lanes.
 __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
+    constexpr int QMAX = 2 * kXPer * 256;
+    __shared__ double qe[QMAX];  // queued energies, then their delta values
+    __shared__ int qp[QMAX];     // (point, band) pair of each queued energy
+    __shared__ int qn;
     const int ia = blockIdx.y;
     const int4 k = a.st_k[ia];
     const size_t row = (size_t)a.st_row[ia] * a.nprocs;
     const double en_el = a.st_e[ia];
-    double contrib = 0.0;
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+    // 1. per process: phonon mode, occupations, state maps; the energies
+    // e +- omega that fall inside their (k', n) tetrahedra range are queued
+    double gw[kXPer], occp[kXPer], occm[kXPer];
+    int slp[kXPer], slm[kXPer];
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
+        slp[u] = slm[u] = -1;
+        gw[u] = 0.0;
+        occp[u] = occm[u] = 0.0;
         const int64_t c = ((int64_t)blockIdx.x * kXPer + u) * 256 + threadIdx.x;
-        if (c >= a.nprocs) break;
+        if (c >= a.nprocs) continue;
         const int2 vm = a.vmap[c];
         const int r = vm.x, n = vm.y >> 8, s = vm.y & 255;
         // q = k' - k on the electron mesh (both components in [0, N))
@@ -418,24 +417,46 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
             bidx = a.ncoarse + (size_t)iq * a.nbr + s;
             sph = -((int64_t)iq * a.nbr + s + 1);
         }
-        double xp = 0.0, xm = 0.0;
-        if (en_ph >= 0.5e-3) {
-            const double g = a.g2[row + c];
-            const double bose = a.bose[bidx];
-            const double fp = 1.0 / (exp((en_el + en_ph - a.chempot) * a.ikT) + 1.0);
-            const double fm = 1.0 / (exp((en_el - en_ph - a.chempot) * a.ikT) + 1.0);
-            double dp, dm;
-            delta_pair(a.t, en_el + en_ph, en_el - en_ph, r, n, dp, dm);
-            xp = a.cnst * (g * (bose + fp) * dp);
-            xm = a.cnst * (g * (1.0 + bose - fm) * dm);
-        }
-        if (a.xp) {
-            a.xp[row + c] = xp;
-            a.xm[row + c] = xm;
-        }
         if (a.se) {
             a.se[row + c] = (int64_t)r * a.nb + n + 1;
             a.sp[row + c] = sph;
+        }
+        if (en_ph >= 0.5e-3) {
+            gw[u] = a.g2[row + c];
+            const double bose = a.bose[bidx];
+            occp[u] = bose + 1.0 / (exp((en_el + en_ph - a.chempot) * a.ikT) + 1.0);
+            occm[u] = 1.0 + bose - 1.0 / (exp((en_el - en_ph - a.chempot) * a.ikT) + 1.0);
+            const int pr = r * a.t.nb + n;
+            const DeltaHead *h = a.t.head + pr;
+            const double lo = h->lo, hi = h->hi, ep = en_el + en_ph, em = en_el - en_ph;
+            if (!(ep < lo || ep > hi)) {
+                slp[u] = atomicAdd(&qn, 1);
+                qe[slp[u]] = ep;
+                qp[slp[u]] = pr;
+            }
+            if (!(em < lo || em > hi)) {
+                slm[u] = atomicAdd(&qn, 1);
+                qe[slm[u]] = em;
+                qp[slm[u]] = pr;
+            }
+        }
+    }
+    __syncthreads();
+    // 2. the block's queued deltas, all lanes busy
+    const int nq = qn;
+    for (int i = threadIdx.x; i < nq; i += 256) qe[i] = delta_of_pair(a.t, qe[i], qp[i]);
+    __syncthreads();
+    // 3. X+- = 2 pi / hbar g2 occ delta
+    double contrib = 0.0;
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+        const int64_t c = ((int64_t)blockIdx.x * kXPer + u) * 256 + threadIdx.x;
+        if (c >= a.nprocs) continue;
+        const double dp = slp[u] >= 0 ? qe[slp[u]] : 0.0, dm = slm[u] >= 0 ? qe[slm[u]] : 0.0;
+        const double xp = a.cnst * (gw[u] * occp[u] * dp), xm = a.cnst * (gw[u] * occm[u] * dm);
+        if (a.xp) {
+            a.xp[row + c] = xp;
+            a.xm[row + c] = xm;
         }
         contrib += xp + xm;
     }
