@@ -92,6 +92,12 @@ struct elph_ctx {
     DBuf<int> gpcell;
     DBuf<double> ginvdeg;
     int64_t npairk = 0, gk2p = 0;
+    // the same over the Rp cells of the g set (rcells_g) for the cell-pair
+    // form of gReq: rgpair, rgpcell, rginvdeg = 1 / gwsdeg
+    DBuf<int2> rgpair;
+    DBuf<int> rgpcell;
+    DBuf<double> rginvdeg;
+    int64_t npairg = 0;
     // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
     DBuf<int> term_off, term_iuc, term_T, term_div;
     // polar crystals: reciprocal vectors of the (2N+1)^3 box (1/bohr), Born

@@ -1194,8 +1194,9 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
 // rows X = i + nb2 * Re; one wave per 16-row block.
 // ---------------------------------------------------------------------------
 template <int NCB>
-__global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int nwsg,
-                                              const double2 *__restrict__ phiT, int ldphi, int nq,
+__global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
+                                              const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
+                                              const double *__restrict__ phiR, int ldphi, int K2, int nq,
                                               double2 *__restrict__ out, int64_t out_qstride) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
@@ -1209,16 +1210,28 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
     const int64_t row = rb * 16 + li;
-    for (int k0 = 0; k0 < nwsg; k0 += 4) {
+    for (int k0 = 0; k0 < K2; k0 += 4) {
         const int kk = k0 + lk;
+        // folded gwann row kk: G(2p) = g(R_p)/d + g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
         double2 a = make_double2(0.0, 0.0);
-        if (row < nrows && kk < nwsg) a = A[(size_t)kk * sstride + row];
-        const double nai = -a.y;
+        const int p = kk >> 1;
+        if (row < nrows && p < npair) {
+            const int2 pr = pairs[p];
+            const double2 ga = A[(size_t)pr.x * sstride + row];
+            const double da = invdeg[pr.x];
+            double2 x = make_double2(ga.x * da, ga.y * da), y = make_double2(0.0, 0.0);
+            if (pr.y >= 0) {
+                const double2 gb = A[(size_t)pr.y * sstride + row];
+                const double db = invdeg[pr.y];
+                y = make_double2(gb.x * db, gb.y * db);
+            }
+            a = (kk & 1) ? make_double2(y.y - x.y, x.x - y.x) : make_double2(x.x + y.x, x.y + y.y);
+        }
 #pragma unroll
         for (int c = 0; c < NCB; ++c) {
-            double2 b = make_double2(0.0, 0.0);
-            if (kk < nwsg) b = phiT[(size_t)kk * ldphi + c * 16 + li];
-            cmfma(a.x, a.y, nai, b.x, b.y, cr[c], ci[c]);
+            const double b = phiR[(size_t)kk * ldphi + c * 16 + li];
+            cr[c] = mfma_f64(a.x, b, cr[c]);
+            ci[c] = mfma_f64(a.y, b, ci[c]);
         }
     }
 #pragma unroll
@@ -2264,6 +2277,19 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
         std::vector<double> inv(c->nwsk);
         for (int64_t q = 0; q < c->nwsk; ++q) inv[q] = 1.0 / (double)c->elwsdeg[q];
         c->ginvdeg.upload(inv, c->stream);
+    }
+    {
+        // Rp cells of the g set in pairs (R, -R) for the cell-pair form of gReq
+        const std::vector<int2> pr = cell_pairs(c->rcells_g.data(), c->nwsg, c->nwsg);
+        c->npairg = (int64_t)pr.size();
+        std::vector<int> pc(3 * c->npairg);
+        for (int64_t q = 0; q < c->npairg; ++q)
+            for (int d = 0; d < 3; ++d) pc[3 * q + d] = (int)c->rcells_g[pr[q].x + c->nwsg * d];
+        c->rgpair.upload(pr, c->stream);
+        c->rgpcell.upload(pc, c->stream);
+        std::vector<double> inv(c->nwsg);
+        for (int64_t q = 0; q < c->nwsg; ++q) inv[q] = 1.0 / (double)c->gwsdeg[q];
+        c->rginvdeg.upload(inv, c->stream);
     }
     build_phonon_terms(c);
     if (c->polar) build_polar_terms(c);
@@ -3880,22 +3906,27 @@ int elph_ibzk_g2(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph
 namespace {
 
 // Batched gReq into device memory: out[q] (nb,nb,nbr,nwsk) for q vectors on the device.
+// g(Re, q) for nq device q vectors in cell-pair form over Rp: the real
+// cos/sin table of the pairs (KT q per launch, zero-padded rows and columns)
+// against gwann folded on the fly (1/gwsdeg included).
 void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, int64_t out_qstride) {
     const int64_t nb2 = c->nb * c->nb;
     const int64_t KT = 64;
-    DBuf<double2> phiT;
-    phiT.alloc(c->nwsg * KT);
+    const int64_t K2 = ((2 * c->npairg + 3) / 4) * 4;
+    DBuf<double2> phiR;  // K2 x KT doubles, as double2 pairs for the zero fill
+    phiR.alloc((K2 * KT + 1) / 2);
+    double *phi = reinterpret_cast<double *>(phiR.p);
     const int64_t nrb = (nb2 * c->nwsk + 15) / 16;
     for (int64_t q0 = 0; q0 < nq; q0 += KT) {
         const int64_t n = std::min(KT, nq - q0);
-        hipLaunchKernelGGL(k_zero2, dim3((unsigned)((c->nwsg * KT + 255) / 256)), dim3(256), 0, c->stream, phiT.p,
-                           c->nwsg * KT);
-        dim3 grid((unsigned)((c->nwsg + 127) / 128), (unsigned)n);
-        hipLaunchKernelGGL(k_phase_list, grid, dim3(128), 0, c->stream, (int)n, (int)c->nwsg, d_qvecs + q0, nq,
-                           c->rg.p, c->degg.p, phiT.p, (int64_t)1, KT);
+        hipLaunchKernelGGL(k_zero2, dim3((unsigned)(((K2 * KT + 1) / 2 + 255) / 256)), dim3(256), 0, c->stream, phiR.p,
+                           (K2 * KT + 1) / 2);
+        dim3 grid((unsigned)((c->npairg + 127) / 128), (unsigned)n);
+        hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, c->stream, (int)n, (int)c->npairg, d_qvecs + q0, nq,
+                           c->rgpcell.p, phi, (int64_t)1, KT);
         hipLaunchKernelGGL(k_gReq<4>, dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256), 0, c->stream,
-                           c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->nwsg, phiT.p, (int)KT, (int)n,
-                           out + (size_t)q0 * out_qstride, out_qstride);
+                           c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->npairg, c->rgpair.p,
+                           c->rginvdeg.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
         HCHECK(hipGetLastError());
     }
 }
