@@ -1,17 +1,20 @@
 // Consumers of the vertex rows on the device: the linear tetrahedron delta
 // function (src/delta.f90:61-225) on the tetrahedra of src/delta.f90:494-646,
-// the transition probabilities X+/- of calculate_eph_interaction_ibzk key 'X'
-// (src/interactions.f90:2278-2354) and the e-ph RTA scattering rates of
-// calculate_el_rta_rates (src/interactions.f90:2957-3063).
+// or the triangle method (648-882); the transition probabilities X+/- of
+// calculate_eph_interaction_ibzk key 'X' (src/interactions.f90:2278-2354);
+// the e-ph RTA scattering rates of calculate_el_rta_rates
+// (src/interactions.f90:2957-3063); and the electron RTA transport
+// coefficients (src/bte.f90:912-1011, src/bz_sums.f90:484-611).
 //
 // Layout in HBM:
-//   tetrahedra  tcount[nk], tmap[nk][24] = 4 (it - 1) + (iv - 1) (it the
-//               tetrahedron, iv the wave vector's vertex label in it), tev[ntet][nb]
-//               the four vertex energies sorted (32 bytes);
+//   simplices   tcount[nk], tmap[nk][24] = 4 (it - 1) + (iv - 1) (it the
+//               tetrahedron or triangle, iv the wave vector's vertex label in it),
+//               tev[ntet][nb] the vertex energies sorted (32 bytes; a triangle's
+//               highest energy repeated);
 //   delta table per (wave vector, band): a 128-byte header (energy range, piece
 //               count, search starts of 64 uniform bins) and up to 96 pieces of 64
 //               bytes (bounds, cubic coefficients of the vertex-weight sum);
-//   processes   one thread per process of a g2 row (state, k', band, branch): reads
+//   processes   four per thread of k_ibzk_X, consecutive in a g2 row: each reads
 //               its |g|^2 (8 B), writes X+ and X- (16 B) and optionally the final
 //               electron and phonon state (16 B).
 #include <algorithm>
