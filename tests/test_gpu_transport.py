@@ -160,3 +160,26 @@ def test_triangle_method(case):
     assert err < 1e-9
     assert _rel(out["rates"], O.el_rta_rates(counts, xp_o, xm_o), 1e-9) < 1e-9
     plan.close()
+
+
+def test_X_argument_errors(case):
+    """Invalid inputs are refused with an error (the reference stops with
+    exit_with_message); no kernel runs."""
+    from elphbolt_amd._lib import ElphError
+    from elphbolt_amd.wannier import DeviceBuffer
+    mat, el, ph, w = case["mat"], case["el"], case["ph"], case["w"]
+    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=case["mesh_ref"], precision="fp64")
+    g2d = DeviceBuffer(w, (2, plan.nprocs))
+    plan.run(1, 2, out=g2d)
+    with pytest.raises(ElphError):
+        plan.run_X(case["tet_g"], 0.0, 0.0, 1, 2, g2d)                 # T <= 0
+    with pytest.raises(ElphError):
+        plan.run_X(case["tet_g"], T_K, 0.0, 0, 2, g2d)                 # istate below 1
+    with pytest.raises(ElphError):
+        plan.run_X(case["tet_g"], T_K, 0.0, 1, el.nwv_irred * mat.nb + 1, g2d)  # past the last state
+    other = interactions.Tetrahedra(w, el.wvmesh, el.ens[:-1], indexlist=el.indexlist[:-1])
+    with pytest.raises(ElphError):
+        plan.run_X(other, T_K, 0.0, 1, 2, g2d)                          # tetrahedra of another list
+    with pytest.raises(ElphError):
+        interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist[::-1])  # not ascending
+    plan.close()
