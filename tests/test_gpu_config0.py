@@ -1,0 +1,82 @@
+"""The reference's own CPU-sized case (config 0: Si-like, 6^3 WS cells, 12^3
+mesh, window +-0.5 eV) as a parity case: every IBZ state's |g|^2 against the
+oracle (FP64 1e-12, mixed 1e-5, relative above 1e-6 * max; counts exact), and
+the transport-delta proxy over the whole chain -- g2 -> X+/- (tetrahedra) ->
+RTA rates -> sigma, sigma*S and S -- against the oracle's chain (FP64 1e-8,
+mixed 1e-4)."""
+import numpy as np
+import pytest
+
+from elphbolt_amd import interactions, synth
+from elphbolt_amd.bte import ElectronRTA
+from elphbolt_amd.wannier import DeviceBuffer, Wannier
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+T_K = 300.0
+
+
+def _rel(x, ref, floor=1e-6):
+    x, ref = np.asarray(x).ravel(), np.asarray(ref).ravel()
+    sel = np.abs(ref) > floor * np.abs(ref).max()
+    return float(np.max(np.abs(x[sel] - ref[sel]) / np.abs(ref[sel]))) if sel.any() else 0.0
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    mat = synth.config(0)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    nst = el.nwv_irred * mat.nb
+    print(f"config 0: {el.nwv} k' in the window, {el.nwv_irred} IBZ k, {nst} states", flush=True)
+    c_o, g_o = O.ibzk_g2(o, el, ph, 1, 1, nst)
+    mu = float(mat.enref)
+    tet = O.Tetrahedra(el.wvmesh, el.ens, indexlist=el.indexlist)
+    xp, xm, _, _ = O.ibzk_X(o, el, ph, tet, c_o, g_o, T_K, mu)
+    rates = O.el_rta_rates(c_o, xp, xm).reshape(el.nwv_irred, mat.nb)
+    mesh = np.array(el.wvmesh)
+    irr, neq, m = O.irred_wedge(mesh, qr, nsr, indexlist=el.indexlist, maps=True)
+    assert np.array_equal(irr, el.indexlist_irred)
+    kv = np.array([O.demux_vector(i, mesh, 0) for i in el.indexlist], float) / mesh
+    _, _, vels = o.el_wann(kv)
+    vol = abs(np.linalg.det(np.asarray(mat.lattvecs)))
+    ref = {}
+    for field in ("E", "T"):
+        F = O.field_term(field, neq, m, T_K, mu, el.ens, vels, rates, el.indexlist)
+        hc, cc = O.transport_coeff(field, T_K, 2, mu, el.ens, vels, vol, mesh, F)
+        if field == "E":
+            ref["sigma"] = cc
+        else:
+            ref["sigmaS"] = cc
+    return dict(mat=mat, el=el, ph=ph, nst=nst, c_o=c_o, g_o=g_o, mu=mu, neq=neq, m=m, vels=vels, vol=vol, ref=ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config0_g2_and_transport(cfg0, precision):
+    mat, el, ph, nst = cfg0["mat"], cfg0["el"], cfg0["ph"], cfg0["nst"]
+    print(f"config 0 {precision}", flush=True)
+    w = Wannier(mat)
+    plan = interactions.IbzkPlan(w, el, ph, precision=precision)
+    g2d = DeviceBuffer(w, (nst, plan.nprocs))
+    counts, _ = plan.run(1, nst, out=g2d)
+    c_o, g_o = cfg0["c_o"], cfg0["g_o"]
+    assert np.array_equal(counts, c_o)
+    act = counts >= 0
+    err = _rel(g2d.numpy()[act], g_o[act])
+    print(f"  |g|^2 max rel {err:.2e} over {int(act.sum())} in-window states", flush=True)
+    assert err < (1e-12 if precision == "fp64" else 1e-5)
+    tet = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
+    x = plan.run_X(tet, T_K, cfg0["mu"], 1, nst, g2d, maps=False, probabilities=False)
+    tr = ElectronRTA(w, el.ens, cfg0["vels"], el.indexlist, cfg0["neq"], cfg0["m"], cfg0["vol"], el.wvmesh, 2)
+    got = tr.rta(x["rates_device"], T_K, cfg0["mu"])
+    sig, sig_r = ElectronRTA.scalar(got["sigma"]), ElectronRTA.scalar(cfg0["ref"]["sigma"])
+    ss, ss_r = ElectronRTA.scalar(got["sigmaS"]), ElectronRTA.scalar(cfg0["ref"]["sigmaS"])
+    d_sig, d_ss, d_S = abs(sig / sig_r - 1), abs(ss / ss_r - 1), abs((ss / sig) / (ss_r / sig_r) - 1)
+    print(f"  sigma {sig:.6e} (oracle {sig_r:.6e}) rel {d_sig:.2e}; sigmaS rel {d_ss:.2e}; S rel {d_S:.2e}",
+          flush=True)
+    tol = 1e-8 if precision == "fp64" else 1e-4
+    assert sig_r > 0 and d_sig < tol and d_ss < tol and d_S < tol
+    plan.close()
