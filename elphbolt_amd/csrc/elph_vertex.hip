@@ -726,7 +726,6 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
     extern __shared__ v4i smem8[];
     uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
-    unsigned *sink = Rsh + nkc * 32;  // 256 bytes per wave, never read
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int st = blockIdx.y;
@@ -744,30 +743,8 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
         cpk = (unsigned)((q0 * a.Lf0) % a.L) | ((unsigned)((q1 * a.Lf1) % a.L) << 8) |
               ((unsigned)((q2 * a.Lf2) % a.L) << 16);
     }
-    // the epilogue's per-row data (rows of U_k', u_q) pulled into the caches
-    // while the contraction runs: 4-byte global -> LDS copies into a sink
-    auto warm = [&]() {
-        const int nrow = min(16, a.nwv - row_base);
-        if (nrow <= 0) return;
-        const char *v0 = reinterpret_cast<const char *>(a.Vkp + (size_t)row_base * a.nb * a.nb);
-        const int vlines = (nrow * a.nb * a.nb * 16 + 127) / 128;
-        for (int l = lane; l < vlines; l += 64) lds_dma4(v0 + (size_t)l * 128, sink + wave * 64);
-        if (li < nrow) {
-            const bool coarse = q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0;
-            const char *u0;
-            if (coarse) {
-                const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-                u0 = reinterpret_cast<const char *>(a.u + (size_t)iq * a.nbr * a.nbr);
-            } else {
-                const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
-                u0 = reinterpret_cast<const char *>(a.uf + (size_t)iq * a.nbr * a.nbr);
-            }
-            const int ubytes = a.nbr * a.nbr * 16;
-            for (int l = lg; l * 128 < ubytes; l += 4) lds_dma4(u0 + l * 128, sink + wave * 64);
-        }
-    };
     v4i acc[CB][NS][2];
-    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc, warm);
+    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc, [] {});
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
     constexpr int YS = CP + 1;
     double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * YS;
@@ -3172,7 +3149,7 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
     if (p->precision == ELPH_MIXED) {
         const size_t lds8 = std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) +
                                          (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
-                                         (size_t)p->nkc * 32 * sizeof(unsigned) + 1024,
+                                         (size_t)p->nkc * 32 * sizeof(unsigned),
                                      (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;
