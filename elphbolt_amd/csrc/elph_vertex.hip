@@ -624,32 +624,37 @@ __global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const dou
 // fills Tsh and Rsh before the call (the first barrier here publishes them)
 // and owns buf again after it.  Two buffers keep the LDS of a block small
 // enough for three blocks per CU (CB <= 3), which hides the copy latency
-// better than a deeper prefetch with two blocks.  after_setup() runs once
-// after the first barrier (its global -> LDS copies complete within the first
-// chunk).
+// better than a deeper prefetch with two blocks.  The caller issues chunk 0
+// (i8_copy_chunk(bst, bkc, 0, buf)) before it fills the tables, so that copy
+// overlaps the table loads.
 // (NBM: the epilogue's unrolled band bound, whose registers fit three blocks
 // per CU up to 8 bands)
 __host__ __device__ constexpr int pair_blocks_per_cu(int CB, int NBM = 0) { return CB <= 3 && NBM <= 8 ? 3 : 2; }
-template <int CB, typename F>
+// global -> LDS copy of B chunk kn into buffer kn & 1 (the whole block)
+template <int CB>
+__device__ __forceinline__ void i8_copy_chunk(const v4i *__restrict__ bst, size_t bkc, int kn, v4i *buf) {
+    constexpr int BCH = CB * 2 * kNS * 64;
+    constexpr int NR = BCH / 256;  // copy instructions per wave and chunk
+    static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const v4i *src = bst + (size_t)kn * bkc;
+    v4i *dst = buf + (kn & 1) * BCH;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int r = 4 * j + wave;
+        lds_dma16(src + r * 64 + lane, dst + r * 64);
+    }
+}
+template <int CB>
 __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t bkc, int nkc, const uint2 *Tsh,
                                              const unsigned *Rsh, unsigned cpk, unsigned lmul, int negL, v4i *buf,
-                                             v4i (&acc)[CB][kNS][2], F after_setup) {
+                                             v4i (&acc)[CB][kNS][2]) {
     constexpr int NS = kNS;
     static_assert(NS == 4, "digit packing assumes 4 digits per part");
     constexpr int BCH = CB * 2 * NS * 64;
-    constexpr int NR = BCH / 256;  // copy instructions per wave and chunk
-    static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lg = lane >> 4;
+    const int lane = threadIdx.x & 63, lg = lane >> 4;
     auto bufp = [&](int kc) { return buf + (kc & 1) * BCH; };
-    auto copy_b = [&](int kn) {
-        const v4i *src = bst + (size_t)kn * bkc;
-        v4i *dst = bufp(kn);
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            const int r = 4 * j + wave;
-            lds_dma16(src + r * 64 + lane, dst + r * 64);
-        }
-    };
+    auto copy_b = [&](int kn) { i8_copy_chunk<CB>(bst, bkc, kn, buf); };
     // chunk kc + 1 complete in LDS for the whole block.  No LDS stores are
     // pending here, so the barrier needs no fence; the memory clobber keeps the
     // compiler's LDS reads on their side of it.
@@ -658,10 +663,8 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
     for (int c = 0; c < CB; ++c)
 #pragma unroll
         for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
-    copy_b(0);
     lds_dma_wait();
     __syncthreads();
-    after_setup();
     // dword w of the A operands of chunk kk (4 levels)
     auto gather = [&](int kk, int w, v4i (&A)[NS]) {
         const uint2 rp = *reinterpret_cast<const uint2 *>(Rsh + kk * 32 + lg * 8 + 2 * w);
@@ -731,6 +734,8 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
     const int st = blockIdx.y;
     const int row_base = blockIdx.x * 64 + wave * 16;
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
+    const v4i *bst = bsl + (size_t)st * nkc * BCH;
+    i8_copy_chunk<CB>(bst, BCH, 0, smem8);
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
     for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
     unsigned cpk = 0;
@@ -744,7 +749,7 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
               ((unsigned)((q2 * a.Lf2) % a.L) << 16);
     }
     v4i acc[CB][NS][2];
-    i8_pair_core<CB>(bsl + (size_t)st * nkc * BCH, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc, [] {});
+    i8_pair_core<CB>(bst, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
     constexpr int YS = CP + 1;
     double2 *Y = reinterpret_cast<double2 *>(smem8) + wave * 16 * YS;
@@ -840,15 +845,16 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB)) void k_ibzk_y_i8(PairA
     const int li = lane & 15, lg = lane >> 4;
     const int stl = blockIdx.y, st = st0 + stl, cb0 = blockIdx.z * CB;
     const int row_base = blockIdx.x * 64 + wave * 16;
+    const size_t bkc = (size_t)CBT * 2 * NS * 64;
+    const v4i *bst = bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 2 * NS * 64;
+    i8_copy_chunk<CB>(bst, bkc, 0, smem8);
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
     for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
     int c0, c1, c2;
     ibzk_coef(a, st, row_base + li, c0, c1, c2);
     const unsigned cpk = (unsigned)c0 | ((unsigned)c1 << 8) | ((unsigned)c2 << 16);
     v4i acc[CB][NS][2];
-    const size_t bkc = (size_t)CBT * 2 * NS * 64;
-    i8_pair_core<CB>(bsl + (size_t)st * nkc * bkc + (size_t)cb0 * 2 * NS * 64, bkc, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L,
-                     smem8, acc, [] {});
+    i8_pair_core<CB>(bst, bkc, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
     double wl[NS];
     wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
