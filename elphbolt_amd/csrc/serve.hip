@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256, 2) void k_srv_y_i8(const v4i *__restrict__ bsl
 
 struct EpiArgs {
     const double2 *Y;      // [st - st0][r][CPT]
-    int CPT, nqs, st0;
+    int CPT, nqs, st0, ns;  // states st0 .. st0 + ns of the segment
     int nb, nbr;
     int k_unit0, k_req0, q0, nq;
     const double2 *Ukq;    // [(kseg * nqs + r)][n][np]
@@ -419,66 +419,78 @@ struct EpiArgs {
     int G;                 // q rows handled together by one block
 };
 
-// Epilogue of one state (k, m) and G q rows per block:
+// Epilogue of the states (k, m) of one k and G q rows per block:
 //   Z(n,sp) = sum_np conj(U_k+q(n,np)) Y(np + nb sp)
 //   g(n,s)  = sum_sp u_q(s,sp) Z(n,sp) [+ lp(q,s) sum_i conj(U_k+q(n,i)) U_k(m,i)]
 //   |g|^2   = 1/2 |g|^2 / omega * Ryd2eV^3 Ryd2amu (0 when omega == 0).
+// The block loads U_k+q and u_q once and loops over the bands m of its k that
+// lie in the launch's state range [st0, st0 + ns) (states st = k nb + m of the
+// segment; Y holds them from st0 on).
 __global__ __launch_bounds__(256) void k_srv_epi(EpiArgs a) {
     extern __shared__ double2 sm[];
     const int nb = a.nb, nbr = a.nbr, nbnr = nb * nbr;
     const int G = a.G, TPQ = 256 / G;
     const int tid = threadIdx.x, sub = tid / TPQ, tt = tid % TPQ;
-    // grid (states, q-row groups): consecutive blocks are the states (k, m)
-    // of one q-row group, which share U_k+q and u_q while they are in L2
-    const int st = a.st0 + blockIdx.x;  // state within the segment
-    const int kseg = st / nb, m = st % nb;
+    // grid (k of the launch, q-row groups): consecutive blocks are the k of
+    // one q-row group, which share u_q while it is in L2
+    const int kseg = a.st0 / nb + blockIdx.x;
+    const int m_lo = max(0, a.st0 - kseg * nb), m_hi = min(nb, a.st0 + a.ns - kseg * nb);
     const int r = blockIdx.y * G + sub;
     const int per = 3 * nbnr + nb * nb + nbr * nbr;
     double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *Us = Zs + nbnr, *us = Us + nb * nb, *un = us + nbr * nbr;
     const bool live = sub < G && r < a.nqs;
     const int q = a.q0 + r;
-    const double2 *Ukm = a.Uk + ((size_t)(a.k_unit0 + kseg) * nb + m) * nb;
     if (live) {
-        const double2 *Yr = a.Y + ((size_t)blockIdx.x * a.nqs + r) * a.CPT;
         const double2 *Ur = a.Ukq + ((size_t)kseg * a.nqs + r) * nb * nb;
         const double2 *ur = a.u + (size_t)q * nbr * nbr;
-        for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
         for (int i = tt; i < nb * nb; i += TPQ) Us[i] = Ur[i];
         for (int i = tt; i < nbr * nbr; i += TPQ) us[i] = ur[i];
     }
-    __syncthreads();
-    if (live) {
-        for (int i = tt; i < nbnr; i += TPQ) {
-            const int n = i % nb, sp = i / nb;
-            double2 z = make_double2(0.0, 0.0);
-            for (int np = 0; np < nb; ++np) {
-                const double2 t = cmulc(Us[n * nb + np], Ys[np + nb * sp]);
-                z.x += t.x;
-                z.y += t.y;
-            }
-            Zs[n + nb * sp] = z;
+    for (int m = m_lo; m < m_hi; ++m) {
+        const int st = kseg * nb + m;
+        if (live) {
+            const double2 *Yr = a.Y + ((size_t)(st - a.st0) * a.nqs + r) * a.CPT;
+            for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
         }
-        if (a.lp)
-            for (int n = tt; n < nb; n += TPQ) {
+        // Ys (and, before the first band, Us and us) complete; Zs and un of
+        // the previous band no longer read
+        __syncthreads();
+        if (live) {
+            for (int i = tt; i < nbnr; i += TPQ) {
+                const int n = i % nb, sp = i / nb;
                 double2 z = make_double2(0.0, 0.0);
-                for (int i = 0; i < nb; ++i) {
-                    const double2 t = cmulc(Us[n * nb + i], Ukm[i]);
+                for (int np = 0; np < nb; ++np) {
+                    const double2 t = cmulc(Us[n * nb + np], Ys[np + nb * sp]);
                     z.x += t.x;
                     z.y += t.y;
                 }
-                un[n] = z;
+                Zs[n + nb * sp] = z;
             }
-    }
-    __syncthreads();
-    if (live) {
-        float *o = a.out + ((((size_t)(a.k_req0 + kseg) * a.nq + q) * nb + m) * nb) * nbr;
-        for (int i = tt; i < nbnr; i += TPQ) {
-            const int n = i / nbr, s = i % nbr;
-            double2 g = make_double2(0.0, 0.0);
-            for (int sp = 0; sp < nbr; ++sp) cfma(g, us[s * nbr + sp], Zs[n + nb * sp]);
-            if (a.lp) cfma(g, a.lp[(size_t)q * nbr + s], un[n]);
-            const double w = a.om[(size_t)q * nbr + s];
-            o[i] = (w == 0.0) ? 0.0f : (float)(0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit);
+            if (a.lp) {
+                const double2 *Ukm = a.Uk + ((size_t)(a.k_unit0 + kseg) * nb + m) * nb;
+                for (int n = tt; n < nb; n += TPQ) {
+                    double2 z = make_double2(0.0, 0.0);
+                    for (int i = 0; i < nb; ++i) {
+                        const double2 t = cmulc(Us[n * nb + i], Ukm[i]);
+                        z.x += t.x;
+                        z.y += t.y;
+                    }
+                    un[n] = z;
+                }
+            }
+        }
+        // Zs and un complete; Ys no longer read
+        __syncthreads();
+        if (live) {
+            float *o = a.out + ((((size_t)(a.k_req0 + kseg) * a.nq + q) * nb + m) * nb) * nbr;
+            for (int i = tt; i < nbnr; i += TPQ) {
+                const int n = i / nbr, s = i % nbr;
+                double2 g = make_double2(0.0, 0.0);
+                for (int sp = 0; sp < nbr; ++sp) cfma(g, us[s * nbr + sp], Zs[n + nb * sp]);
+                if (a.lp) cfma(g, a.lp[(size_t)q * nbr + s], un[n]);
+                const double w = a.om[(size_t)q * nbr + s];
+                o[i] = (w == 0.0) ? 0.0f : (float)(0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit);
+            }
         }
     }
 }
@@ -897,6 +909,7 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
             a.CPT = (int)m.CPT;
             a.nqs = (int)sg.nqs;
             a.st0 = (int)s0;
+            a.ns = ns;
             a.nb = (int)nb;
             a.nbr = (int)nbr;
             a.k_unit0 = (int)sg.k_unit0;
@@ -911,7 +924,7 @@ void launch_unit(elph_server *s, Slot &sl, int mat, std::vector<Segment> &segs) 
             a.out = r.d_out;
             a.G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (nb * nbr)));
             const size_t lds = (size_t)a.G * (3 * nb * nbr + nb * nb + nbr * nbr) * sizeof(double2);
-            dim3 grid((unsigned)ns, (unsigned)((sg.nqs + a.G - 1) / a.G));
+            dim3 grid((unsigned)((s0 + ns - 1) / nb - s0 / nb + 1), (unsigned)((sg.nqs + a.G - 1) / a.G));
             hipLaunchKernelGGL(k_srv_epi, grid, dim3(256), lds, st, a);
         }
         HCHECK(hipGetLastError());
