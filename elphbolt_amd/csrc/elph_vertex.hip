@@ -329,10 +329,11 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                     if (w != 0.0) g2[s] = 0.5 * (gb.x * gb.x + gb.y * gb.y) / w * kG2Unit;
                 }
             }
+            // written once: nontemporal, so the rows' U_k' and u_q stay in L2
             double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
 #pragma unroll
             for (int s = 0; s < NBRM; ++s)
-                if (s < nbr) o[s] = g2[s];
+                if (s < nbr) __builtin_nontemporal_store(g2[s], o + s);
         }
         return;
     }
