@@ -143,6 +143,34 @@ def test_same_alone_and_in_the_mix(mats, precision):
     assert stats["max_slots_busy"] >= 2   # units of different requests overlapped on the device
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_state_chunks_split_the_bands_of_a_k(precision):
+    """The Y of a work unit is cut into chunks of at most 512 MB of states,
+    which need not hold all bands m of a k: with nb = 8, nbr = 24 (192
+    columns) and 1024 q rows a chunk holds 170 of the unit's 256 states, so
+    the epilogue of k 21 runs in two launches (m 0-1, then m 2-7).  The
+    result equals, bit for bit, the same pairs served as four requests of 256
+    q each, whose units fit one chunk."""
+    mat = synth.tiny(nb=8, natoms=8, ws=3, wv=4, seed=54)
+    rng = np.random.default_rng(13)
+    k, q = rng.random((32, 3)), rng.random((1024, 3))
+    print(f"serving chunked {precision}: nk=32 nq=1024 nb=8 nbr=24", flush=True)
+    w = Wannier(mat)
+    with Server(precision, pool_bytes=1 << 30, nslots=2) as srv:
+        mid = srv.add_material(w)
+        rid = srv.submit(mid, k, q)
+        assert srv.wait(rid)[0] == 0
+        whole = srv.fetch(rid, release=True)
+        parts = []
+        for j in range(4):
+            r = srv.submit(mid, k, q[256 * j:256 * (j + 1)])
+            assert srv.wait(r)[0] == 0
+            parts.append(srv.fetch(r, release=True))
+    assert whole.shape == (32, 1024, 8, 8, 24)
+    assert np.abs(whole).max() > 0
+    assert np.array_equal(whole, np.concatenate(parts, axis=1))
+
+
 def test_admission_by_pool(mats):
     w = Wannier(mats["nb4"])
     mat = mats["nb4"]
