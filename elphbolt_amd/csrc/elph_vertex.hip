@@ -257,9 +257,15 @@ struct PairArgs {
 // unrolled to those bounds (guarded by the runtime counts), the row of U_k'
 // is held in registers and every load of a rank is issued before its
 // arithmetic.  NBM == 0: loops over the runtime counts.
-template <int NBM, int NBRM>
+// STAGE (NBM > 0 only): instead of global stores, the row's |g|^2 go to
+// stage[offs[row] - sbase + ...] after the last read of Y by the wave (the
+// caller then writes the wave's contiguous range out with full-line stores);
+// stage may be the wave's own Y rows.
+template <int NBM, int NBRM, bool STAGE = false>
 __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int ys, int row, int yi, int lk,
-                                                  int st, int k0i, int k1i, int k2i, double *outrow) {
+                                                  int st, int k0i, int k1i, int k2i, double *outrow,
+                                                  double *stage = nullptr, int64_t sbase = 0) {
+    static_assert(!STAGE || NBM > 0, "staging needs the unrolled band bound");
     if (row >= a.nwv) return;
     const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
     const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
@@ -282,7 +288,8 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
     const double2 *Yr = Y + yi * ys;
     if constexpr (NBM > 0) {
         const int nb = a.nb, nbr = a.nbr;
-        for (int rank = lk; rank < nn; rank += 4) {
+        // |g|^2 of in-window rank `rank` of the row, all branches
+        auto rank_g2 = [&](int rank, double (&g2)[NBRM]) {
             const int n = a.wband[row * nb + rank];
             const double2 *V = a.Vkp + ((size_t)row * nb + n) * nb;
             double2 v[NBM];
@@ -315,7 +322,6 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                         unm.y += t.y;
                     }
             // g(s) = sum_sp u_q(s,sp) z(sp) (+ polar); the stores follow all loads
-            double g2[NBRM];
 #pragma unroll
             for (int s = 0; s < NBRM; ++s) {
                 g2[s] = 0.0;
@@ -329,11 +335,33 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                     if (w != 0.0) g2[s] = 0.5 * (gb.x * gb.x + gb.y * gb.y) / w * kG2Unit;
                 }
             }
-            // written once: nontemporal, so the rows' U_k' and u_q stay in L2
-            double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
+        };
+        if constexpr (STAGE) {
+            constexpr int NR = (NBM + 3) / 4;  // ranks per lane
+            double keep[NR][NBRM];
 #pragma unroll
-            for (int s = 0; s < NBRM; ++s)
-                if (s < nbr) __builtin_nontemporal_store(g2[s], o + s);
+            for (int j = 0; j < NR; ++j)
+                if (lk + 4 * j < nn) rank_g2(lk + 4 * j, keep[j]);
+            // the wave's reads of Y are all issued above (LDS serves a wave's
+            // requests in order); keep the compiler from moving stores up
+            asm volatile("" ::: "memory");
+            double *o = stage + (a.offs[row] - sbase);
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (lk + 4 * j < nn)
+#pragma unroll
+                    for (int s = 0; s < NBRM; ++s)
+                        if (s < nbr) o[(lk + 4 * j) * nbr + s] = keep[j][s];
+        } else {
+            for (int rank = lk; rank < nn; rank += 4) {
+                double g2[NBRM];
+                rank_g2(rank, g2);
+                // written once: nontemporal, so the rows' U_k' and u_q stay in L2
+                double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
+#pragma unroll
+                for (int s = 0; s < NBRM; ++s)
+                    if (s < nbr) __builtin_nontemporal_store(g2[s], o + s);
+            }
         }
         return;
     }
@@ -775,7 +803,23 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
         }
     }
     __syncthreads();
-    ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
+    if constexpr (NBM > 0 && NBM <= 8) {
+        // the |g|^2 of the wave's 16 rows are one contiguous range of the
+        // output: staged in the wave's own Y rows (after their last read),
+        // then written out by consecutive lanes, so HBM sees whole lines
+        // rather than 8-byte pieces at the branch stride
+        double *stage = reinterpret_cast<double *>(Y);
+        const int64_t b0 = row_base < a.nwv ? a.offs[row_base] : 0;
+        ibzk_epilogue_row<NBM, NBRM, true>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow, stage, b0);
+        __syncthreads();
+        if (row_base < a.nwv) {
+            const int64_t b1 = row_base + 16 < a.nwv ? a.offs[row_base + 16] : a.nprocs;
+            const int cnt = (int)(b1 - b0);
+            for (int i = lane; i < cnt; i += 64) __builtin_nontemporal_store(stage[i], outrow + b0 + i);
+        }
+    } else {
+        ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
+    }
 }
 
 // ---------------------------------------------------------------------------
