@@ -356,11 +356,13 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
             for (int rank = lk; rank < nn; rank += 4) {
                 double g2[NBRM];
                 rank_g2(rank, g2);
-                // written once: nontemporal, so the rows' U_k' and u_q stay in L2
+                // 8-byte stores at the branch stride: through L2, which
+                // merges them into whole lines (nontemporal stores did not
+                // change the time but wrote 2.4x the bytes)
                 double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
 #pragma unroll
                 for (int s = 0; s < NBRM; ++s)
-                    if (s < nbr) __builtin_nontemporal_store(g2[s], o + s);
+                    if (s < nbr) o[s] = g2[s];
             }
         }
         return;
