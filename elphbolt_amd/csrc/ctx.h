@@ -107,6 +107,7 @@ struct elph_ctx {
     int64_t pol_nG = 0;
     double pol_alpha = 0, pol_fac = 0;
     std::string err;
+    int64_t lds_max = 0;  // dynamic LDS a workgroup may allocate on this device (bytes)
     double t_dominant_ms = 0, t_all_ms = 0;
     int64_t n_dominant = 0;
 };
@@ -115,6 +116,10 @@ struct elph_ctx {
 namespace elph {
 
 int ret_error(elph_ctx *ctx, const Error &e);
+// Throws ELPH_EINVAL when a kernel's dynamic LDS (`bytes`) exceeds what a
+// workgroup may allocate on the context's device; `what` names the kernel and
+// the table that sets the size.
+void require_lds(const elph_ctx *c, size_t bytes, const char *what);
 int64_t lcm64(int64_t a, int64_t b);
 // Pairs (i, j) of opposite cells R_j = -R_i among the first n columns of the
 // 3 x ld column-major cell list, in order of first appearance; j = -1 when -R_i
@@ -126,7 +131,7 @@ std::vector<int2> cell_pairs(const int64_t *cells, int64_t ld, int64_t n);
 // phiT: gk_phi_count complex scratch; bphi (mixed only): gk_bphi_count v4i scratch.
 void launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out, int64_t out_kstride,
                  bool mixed, double2 *phiT, v4i *bphi);
-void ensure_gwann_digits(elph_ctx *c);  // int8 digits of gwann, once per context (synchronous)
+void ensure_gwann_digits(elph_ctx *c);  // int8 digits of gwann, once per context; returns when they are built
 int64_t gk_phi_count(const elph_ctx *c);
 int64_t gk_bphi_count(const elph_ctx *c);
 // h(Rp) = sum_mp U(kb,m,mp) gk[kb](np,mp,sp,Rp) / phwsdeg(Rp), folded over the

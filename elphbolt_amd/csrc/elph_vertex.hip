@@ -2041,6 +2041,14 @@ __global__ void k_mfma_probe(const double *A /*16x4 row-major*/, const double *B
 // ---------------------------------------------------------------------------
 }  // namespace
 
+void elph::require_lds(const elph_ctx *c, size_t bytes, const char *what) {
+    if (c->lds_max > 0 && (int64_t)bytes > c->lds_max)
+        throw Error{ELPH_EINVAL, std::string(what) + " needs " + std::to_string(bytes) +
+                                     " bytes of LDS per workgroup, above the device's " +
+                                     std::to_string(c->lds_max) +
+                                     " (the phase table has lcm(mesh) entries: use a mesh with a smaller lcm)"};
+}
+
 int elph::ret_error(elph_ctx *ctx, const Error &e) {
     if (ctx) ctx->err = e.msg;
     return e.code;
@@ -2256,6 +2264,9 @@ __global__ void k_synth_vertex(int64_t n, int nb2, int nwsk, int nbr, const doub
 
 void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, const double *gwann) {
     HCHECK(hipSetDevice(c->device));
+    int lds = 0;
+    HCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    c->lds_max = lds;
     HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HCHECK(hipEventCreate(&c->ev_a));
     HCHECK(hipEventCreate(&c->ev_b));
@@ -2742,6 +2753,8 @@ void elph::ensure_gwann_digits(elph_ctx *c) {
                        (int)c->nwsk, (int)nslices, (int)c->gnrb, (int)c->gnkc, (int)c->npairk, c->gpair.p, c->ginvdeg.p,
                        c->gwann.p, c->gscale.p, c->gdig.p);
     HCHECK(hipGetLastError());
+    // callers enqueue on other streams (batch, server) right after this
+    HCHECK(hipStreamSynchronize(c->stream));
 }
 
 int64_t elph::gk_phi_count(const elph_ctx *c) { return c->gk2p * 64 / 2; }
@@ -3130,6 +3143,32 @@ void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScrat
 }
 
 void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkScratch &sc, hipStream_t s, int CB);
+
+// Dynamic LDS of the pair-stage kernels for the plan's current path (the
+// sizes fused_launch, launch_ibzk_y_* and general_launch's epilogue request).
+size_t ibzk_pair_lds(const elph_ibzk_plan *p) {
+    const size_t epi_fused = (size_t)64 * (p->CP + 1) * sizeof(double2);
+    if (!p->general) {
+        if (p->precision == ELPH_MIXED) {
+            const int CB = (int)(p->CP / 16);
+            return std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
+                                (size_t)p->nkc * 32 * sizeof(unsigned),
+                            epi_fused);
+        }
+        return std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
+                        epi_fused);
+    }
+    const size_t y = p->precision == ELPH_MIXED
+                         ? (size_t)2 * p->gcb * 2 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
+                               (size_t)p->nkc * 32 * sizeof(unsigned)
+                         : (size_t)(2 * 32 * 16 * p->gcb + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4);
+    size_t e = 0;
+    if (!(p->nb <= 16 && p->nbr <= 12)) {
+        const int64_t G = std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
+        e = (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb + p->nb * p->nb) * sizeof(double2);
+    }
+    return std::max(y, e);
+}
 
 // Pair stage of one group on stream s; with `timed`, HIP events bracket it on
 // s and the time is added to the context's dominant-kernel total (synchronous).
@@ -3530,6 +3569,7 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             p->Rpk.upload(rpk, s);
             HCHECK(hipGetLastError());
         }
+        require_lds(ctx, ibzk_pair_lds(p.get()), "the ibzk pair kernel");
         // IBZ data
         p->kint_irr.resize(3 * p->nirr);
         p->ens_irred.assign(el->ens_irred, el->ens_irred + p->nirr * nb);
@@ -3725,6 +3765,7 @@ elph_batch_class_t autotune(elph_batch *b, elph_batch::Item &it, hipStream_t st,
     TileCand bc = cands.front();
     for (const TileCand &t : cands) {
         apply_cand(p, t);
+        if (p->ctx->lds_max > 0 && (int64_t)ibzk_pair_lds(p) > p->ctx->lds_max) continue;
         ibzk_reserve(p, sc, g.nkb, g.nst);
         ibzk_enqueue(p, g, sc, st, it.out, false);  // warm-up (also builds h for this layout)
         float tmin = 1e30f;
@@ -4022,6 +4063,9 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             p->Nph[d] = ph->wvmesh[d];
         }
         p->mesh_ref = p->Nel[0] / p->Nph[0];
+        for (int d = 1; d < 3; ++d)
+            if (p->Nel[d] != p->Nph[d] * p->mesh_ref)
+                throw Error{ELPH_EINVAL, "electron mesh must equal phonon mesh * mesh_ref on every axis"};
         p->ctx = ctx;
         p->precision = precision;
         p->nb = nb;
@@ -4039,6 +4083,7 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
         const int64_t nmesh = p->Nel[0] * p->Nel[1] * p->Nel[2];
         std::vector<int> ki(3 * nwv), nw(nwv), wb(nwv * nb, 0), lut(nmesh, -1);
         std::vector<double2> V(nwv * nb * nb);
+        int64_t total_win = 0;
         for (int64_t r = 0; r < nwv; ++r) {
             int64_t v[3];
             demux_vector0(el->indexlist[r], v, el->wvmesh);
@@ -4048,11 +4093,23 @@ int elph_ibzq_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             for (int64_t n = 0; n < nb; ++n)
                 if (std::fabs(el->ens[r + nwv * n] - el->enref) <= el->fsthick) wb[r * nb + cnt++] = (int)n;
             nw[r] = cnt;
+            total_win += cnt;
             for (int64_t n = 0; n < nb; ++n)
                 for (int64_t j = 0; j < nb; ++j) {
                     const int64_t o = r + nwv * (n + nb * j);
                     V[(r * nb + n) * nb + j] = make_double2(el->evecs[2 * o], el->evecs[2 * o + 1]);
                 }
+        }
+        // a row holds sum_k nw(k) nw(k+q) <= nb * sum_k nw(k) entries: the row
+        // stride nb * nstates_inwindow bounds it only when the count is right
+        if (total_win != el->nstates_inwindow)
+            throw Error{ELPH_EINVAL, "nstates_inwindow does not match the in-window bands of ens"};
+        {
+            const size_t fused = std::max((size_t)(2 * kPairQKC * p->CP + p->L) * sizeof(double2) +
+                                              (size_t)(p->Kp / 2) * sizeof(int4),
+                                          (size_t)64 * (p->CP + 1) * sizeof(double2));
+            const size_t general = (size_t)(2 * 32 * 16 * 2 + p->L) * sizeof(double2) + (size_t)p->Kp * sizeof(int4);
+            require_lds(ctx, p->general ? general : fused, "the ibzq pair kernel");
         }
         p->kint.upload(ki, s);
         p->nwin.upload(nw, s);

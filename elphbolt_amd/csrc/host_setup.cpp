@@ -324,19 +324,44 @@ int read_epw(const std::string &dir, EpwData &o, std::string &err) {
 
 extern "C" int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates,
                                    const int64_t *counts, const double *g2, int64_t row_stride) {
-    if (kind != 'k' && kind != 'q') return ELPH_EINVAL;
+    if (!dir || !counts || !g2 || nstates < 0 || row_stride < 0 || (kind != 'k' && kind != 'q')) return ELPH_EINVAL;
     for (int64_t t = 0; t < nstates; ++t) {
         if (counts[t] < 0) continue;
+        const int64_t c = counts[t];
+        if (c > row_stride) return ELPH_EINVAL;
         char name[64];
         std::snprintf(name, sizeof(name), "%s.istate%lld", kind == 'k' ? "gk2" : "gq2",
                       (long long)(istate_first + t));
         FILE *f = std::fopen((std::string(dir) + "/" + name).c_str(), "wb");
         if (!f) return ELPH_EIO;
-        int64_t c = counts[t];
-        std::fwrite(&c, sizeof(int64_t), 1, f);
+        // gk2: the whole row buffer (src/interactions.f90:2344-2348); gq2: count values (1884-1888)
         const int64_t nwrite = kind == 'k' ? row_stride : c;
-        std::fwrite(g2 + t * row_stride, sizeof(double), nwrite, f);
+        bool ok = std::fwrite(&c, sizeof(int64_t), 1, f) == 1;
+        ok = ok && std::fwrite(g2 + t * row_stride, sizeof(double), nwrite, f) == (size_t)nwrite;
+        if (std::fclose(f) != 0 || !ok) return ELPH_EIO;
+    }
+    return ELPH_OK;
+}
+
+extern "C" int elph_read_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, int64_t *counts,
+                                  double *g2, int64_t row_stride) {
+    if (!dir || !counts || !g2 || nstates < 0 || row_stride < 0 || (kind != 'k' && kind != 'q')) return ELPH_EINVAL;
+    for (int64_t t = 0; t < nstates; ++t) {
+        char name[64];
+        std::snprintf(name, sizeof(name), "%s.istate%lld", kind == 'k' ? "gk2" : "gq2",
+                      (long long)(istate_first + t));
+        FILE *f = std::fopen((std::string(dir) + "/" + name).c_str(), "rb");
+        if (!f) {
+            // the drivers write no file for a state outside the window
+            counts[t] = -1;
+            continue;
+        }
+        int64_t c = -1;
+        bool ok = std::fread(&c, sizeof(int64_t), 1, f) == 1 && c >= 0 && c <= row_stride;
+        ok = ok && std::fread(g2 + t * row_stride, sizeof(double), c, f) == (size_t)c;
         std::fclose(f);
+        if (!ok) return ELPH_EIO;
+        counts[t] = c;
     }
     return ELPH_OK;
 }

@@ -328,6 +328,20 @@ def write_g2_files(g2dir, kind, istate_first, counts, g2):
         raise ElphError(st, f"writing g2 files into {g2dir} failed")
 
 
+def read_g2_files(g2dir, kind, istate_first, nstates, row_stride):
+    """The resume path of `read_gk2` / `read_gq2` (app/elphbolt.f90:185,212):
+    (counts, rows) of states istate_first..istate_first+nstates-1 read from
+    gk2/gq2.istate<i> files; counts -1 (row zero) where no file exists."""
+    lib = _lib.load()
+    counts = np.zeros(max(nstates, 1), np.int64)
+    g = np.zeros((max(nstates, 1), row_stride))
+    st = lib.elph_read_g2_files(str(g2dir).encode(), kind.encode(), istate_first, nstates, _p(counts, c_i64p), _p(g),
+                                row_stride)
+    if st != 0:
+        raise ElphError(st, f"reading g2 files from {g2dir} failed")
+    return counts[:nstates], g[:nstates]
+
+
 def read_g2_file(path):
     """Reads one gk2/gq2 file back: (count, values) (the reader of
     src/interactions.f90:2202-2212 / 3065-3091 takes the first `count`)."""

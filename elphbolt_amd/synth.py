@@ -333,13 +333,17 @@ _WS_COUNT: dict = {}
 _MASSES = [28.0855, 12.0107, 15.999, 24.305, 69.723, 74.92, 10.81, 30.97, 26.98, 14.007, 72.63, 118.71]
 
 
-def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", window: bool = True,
-                   seed_base: int = MULTI_SEED_BASE, bands_in_window=None) -> Material:
-    """Material i of the multi-material generator: atoms per cell uniform in
-    1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
-    uniform in 4..9 (about 150..900 cells), fine mesh from {12, 16, 20, 24}^3.
-    Draws are repeated (same stream) until 16 nb^2 nbr nws^2 <= budget_bytes.
-    The vertex follows the law of config 0 (device-generated by default)."""
+def ws_count(n: int) -> int:
+    """Number of cells of the WS set of an fcc n^3 grid (independent of a)."""
+    if n not in _WS_COUNT:
+        _WS_COUNT[n] = ws_cells(fcc_lattice(4.36), np.array([n, n, n], np.int64))[0].shape[0]
+    return _WS_COUNT[n]
+
+
+def multi_material_draw(i: int, budget_bytes: float = 4e9, seed_base: int = MULTI_SEED_BASE):
+    """The size draw of material i of the multi-material generator, without
+    building it: dict(nat, nb, n, wv, a, nws, rng) where rng is the stream
+    positioned after the draw (the basis is drawn from it next)."""
     rng = np.random.default_rng(seed_base + i)
     for _ in range(10000):
         nat = int(rng.integers(1, 9))
@@ -347,13 +351,21 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
         n = int(rng.integers(4, 10))
         wv = int(rng.choice([12, 16, 20, 24]))
         a = float(rng.uniform(4.0, 6.0))
-        if n not in _WS_COUNT:  # the WS set of an fcc n^3 grid does not depend on a
-            _WS_COUNT[n] = ws_cells(fcc_lattice(4.36), np.array([n, n, n], np.int64))[0].shape[0]
-        nws = _WS_COUNT[n]
+        nws = ws_count(n)
         if 16 * nb * nb * 3 * nat * nws * nws <= budget_bytes:
-            break
-    else:
-        raise ValueError("no draw within the budget")
+            return dict(nat=nat, nb=nb, n=n, wv=wv, a=a, nws=nws, rng=rng)
+    raise ValueError("no draw within the budget")
+
+
+def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", window: bool = True,
+                   seed_base: int = MULTI_SEED_BASE, bands_in_window=None) -> Material:
+    """Material i of the multi-material generator: atoms per cell uniform in
+    1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
+    uniform in 4..9 (about 150..900 cells), fine mesh from {12, 16, 20, 24}^3.
+    Draws are repeated (same stream) until 16 nb^2 nbr nws^2 <= budget_bytes.
+    The vertex follows the law of config 0 (device-generated by default)."""
+    d = multi_material_draw(i, budget_bytes, seed_base)
+    nat, nb, n, wv, a, rng = d["nat"], d["nb"], d["n"], d["wv"], d["a"], d["rng"]
     masses = [_MASSES[j % len(_MASSES)] for j in range(nat)]
     basis = np.zeros((3, nat))
     for j in range(nat):
@@ -361,6 +373,24 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
     return make_material(f"multi{i}_nb{nb}_na{nat}_ws{n}_k{wv}", a, masses, nb, [n] * 3, [wv] * 3,
                          seed_base + i, basis=basis, vertex=vertex, window=window,
                          bands_in_window=nb / 2.0 if bands_in_window is None else bands_in_window)
+
+
+SERVING_BUDGET = 12e9
+SERVING_MIN_BYTES = 2e9
+
+
+def serving_indices(count: int = 8, budget_bytes: float = SERVING_BUDGET,
+                    min_bytes: float = SERVING_MIN_BYTES) -> list:
+    """The resident set of the serving workload: the first `count` draws of the
+    multi-material generator (vertex cap `budget_bytes`) whose Wannier vertex
+    holds at least `min_bytes` (eight draws: 38.8 GB in total)."""
+    out, i = [], 0
+    while len(out) < count:
+        d = multi_material_draw(i, budget_bytes)
+        if 16 * d["nb"] ** 2 * 3 * d["nat"] * d["nws"] ** 2 >= min_bytes:
+            out.append(i)
+        i += 1
+    return out
 
 
 def tiny(nb=2, natoms=1, ws=3, wv=4, seed=7, bands_in_window=None, mesh_ref=1) -> Material:

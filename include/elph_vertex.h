@@ -230,6 +230,14 @@ int elph_el_transport_destroy(elph_el_transport *t);
  * int64 count + count doubles, 1877-1889) for states with counts >= 0. */
 int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, const int64_t *counts,
                         const double *g2, int64_t row_stride);
+/* The resume side (`read_gk2` / `read_gq2`, app/elphbolt.f90:185,212 and
+ * src/numerics.f90:197-198; the consumers' reads, src/interactions.f90:
+ * 2194-2217 and src/eliashberg.f90:176-189): reads gk2/gq2.istate<i> of
+ * states istate_first.. into host rows of stride row_stride, the first
+ * counts[t] values of each; counts[t] = -1 where no file exists (a state the
+ * driver skipped).  ELPH_EIO on a short or corrupt file. */
+int elph_read_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, int64_t *counts,
+                       double *g2, int64_t row_stride);
 /* Writes Xplus.istate<i> and Xminus.istate<i> as calculate_eph_interaction_ibzk
  * key 'X' does (src/interactions.f90:2356-2377) and read_transition_probs_e
  * reads them (3065-3091): int64 count, count doubles X, count int64

@@ -1024,14 +1024,17 @@ int orc_ibzk_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons
     return orc_ibzk_g2_sample(w, el, ph, mesh_ref, istate_first, istate_last, out_count, out_g2, nthreads, -1);
 }
 
-/* Same driver restricted to the first ikp_limit final wave vectors of each
- * state when ikp_limit >= 0 (a bounded sample for CPU timing). */
-int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+/* Same driver restricted to the final wave vectors ikp_first..ikp_first+nkp-1
+ * (1-based) of the window; out_count[t] counts the vertices of that range and
+ * row t holds them from its first entry on.  The full driver's row has them at
+ * nbr times the number of in-window (k', n) with k' < ikp_first. */
+int orc_ibzk_g2_window(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                        int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
-                       int64_t ikp_limit) {
+                       int64_t ikp_first, int64_t ikp_count) {
     const int64_t nb = w->nb, nbr = w->nbr, nirr = el->nwv_irred;
     const int64_t nwv = el->nwv;
-    const int64_t nkp = (ikp_limit >= 0 && ikp_limit < el->nwv) ? ikp_limit : el->nwv;
+    if (ikp_first < 1) ikp_first = 1;
+    const int64_t ikp_end = (ikp_count >= 0 && ikp_first - 1 + ikp_count < nwv) ? ikp_first - 1 + ikp_count : nwv;
     const int64_t nprocs = el->nstates_inwindow * nbr;
     const int64_t gsz = nb * nb * nbr * w->nwsq;
     double rec[9], vol;
@@ -1059,7 +1062,7 @@ int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_
             if (cur_ik != ik) { orc_gkRp(w, kv.frac, gk); cur_ik = ik; }
             int64_t count = 0;
             double *row = &out_g2[t * nprocs];
-            for (int64_t ikp = 1; ikp <= nkp; ++ikp) {
+            for (int64_t ikp = ikp_first; ikp <= ikp_end; ++ikp) {
                 orc_vec_create(el->indexlist[ikp - 1], el->wvmesh, rec, &kpv);
                 orc_vec_sub(&kpv, &kv, el->wvmesh, rec, &qv);
                 int need_fine = 0;
@@ -1097,6 +1100,35 @@ int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_
         free(gk);
     }
     return status;
+}
+
+/* The first ikp_limit final wave vectors of each state when ikp_limit >= 0 (a
+ * bounded sample for CPU timing); the whole window otherwise. */
+int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+                       int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
+                       int64_t ikp_limit) {
+    return orc_ibzk_g2_window(w, el, ph, mesh_ref, istate_first, istate_last, out_count, out_g2, nthreads, 1,
+                              ikp_limit);
+}
+
+/* orc_g2 over a list of n vertices, OpenMP over the list (each call is the
+ * reference's g2 as restated above).  Vertex i reads kvecs/qvecs row i
+ * ((n,3) column-major), el_evec_k/el_evec_kp (n,nb), ph_evec_q (n,nbr),
+ * ph_en (n) and gmixed block gidx[i] of gsz complex entries. */
+void orc_g2_list(const orc_wannier *w, int64_t n, const double *kvecs, const double *qvecs,
+                 const zdouble *el_evec_k, const zdouble *el_evec_kp, const zdouble *ph_evec_q,
+                 const double *ph_en, const zdouble *gmixed, int64_t gsz, const int64_t *gidx, int wannspace_ph,
+                 double *out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t i = 0; i < n; ++i) {
+        double k[3] = {kvecs[i], kvecs[i + n], kvecs[i + 2 * n]};
+        double q[3] = {qvecs[i], qvecs[i + n], qvecs[i + 2 * n]};
+        out[i] = orc_g2(w, k, q, &el_evec_k[i], n, &el_evec_kp[i], n, &ph_evec_q[i], n, ph_en[i],
+                        &gmixed[gidx[i] * gsz], wannspace_ph);
+    }
 }
 
 /* interactions.f90:1673-1930 calculate_eph_interaction_ibzq, key 'g'.

@@ -122,6 +122,13 @@ int orc_ibzk_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons
 int orc_ibzk_g2_sample(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
                        int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
                        int64_t ikp_limit);
+int orc_ibzk_g2_window(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph, int64_t mesh_ref,
+                       int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2, int nthreads,
+                       int64_t ikp_first, int64_t ikp_count);
+void orc_g2_list(const orc_wannier *w, int64_t n, const double *kvecs, const double *qvecs,
+                 const zdouble *el_evec_k, const zdouble *el_evec_kp, const zdouble *ph_evec_q,
+                 const double *ph_en, const zdouble *gmixed, int64_t gsz, const int64_t *gidx, int wannspace_ph,
+                 double *out, int nthreads);
 /* calculate_eph_interaction_ibzq, key 'g'; out row stride nprocs = nstates_inwindow*nb. */
 int orc_ibzq_g2(const orc_wannier *w, const orc_electrons *el, const orc_phonons *ph,
                 int64_t istate_first, int64_t istate_last, int64_t *out_count, double *out_g2,

@@ -394,6 +394,45 @@ def ibzk_g2(w: Wannier, el, ph, mesh_ref=1, istate_first=1, istate_last=None, nt
     return counts, out
 
 
+def ibzk_g2_window(w: Wannier, el, ph, istate_first, istate_last, ikp_first, ikp_count, mesh_ref=1, nthreads=0):
+    """The ibzk driver over the final wave vectors ikp_first..ikp_first+ikp_count-1
+    (1-based) of the window only.  Returns (counts, rows, offsets): rows[t, :counts[t]]
+    are the entries that the full row t holds from offsets[t] on."""
+    nb, nbr = w.mat.nb, w.mat.nbr
+    nst = istate_last - istate_first + 1
+    counts = np.zeros(nst, np.int64)
+    inwin = np.abs(np.asarray(el.ens) - el.enref) <= el.fsthick          # (nwv, nb)
+    before = int(inwin[:ikp_first - 1].sum()) * nbr
+    out = np.zeros((nst, el.nstates_inwindow * nbr))     # the driver's row stride
+    es, k1 = _electrons_struct(w, el)
+    ps, k2 = _phonons_struct(ph)
+    st = lib().orc_ibzk_g2_window(ct.byref(w.s), ct.byref(es), ct.byref(ps), ct.c_int64(mesh_ref),
+                                  ct.c_int64(istate_first), ct.c_int64(istate_last), _p(counts, c_i64p), _p(out),
+                                  ct.c_int(nthreads), ct.c_int64(ikp_first), ct.c_int64(ikp_count))
+    if st == 2:
+        raise RuntimeError("Large negative phonon energy found! Stopping!")
+    assert st == 0, st
+    return counts, out, np.full(nst, before, np.int64)
+
+
+def g2_list(w: Wannier, kvecs, qvecs, el_evec_k, el_evec_kp, ph_evec_q, ph_en, gmixed, gidx, wannspace,
+            nthreads=0):
+    """Wannier%g2 (orc_g2) for a list of vertices, OpenMP over the list.
+    gmixed: (ng, nb, nb, nbr, nws) blocks; vertex i uses block gidx[i]."""
+    n = len(ph_en)
+    k, q = _f64(np.reshape(kvecs, (n, 3))), _f64(np.reshape(qvecs, (n, 3)))
+    a, b, c = _c128(el_evec_k), _c128(el_evec_kp), _c128(ph_evec_q)
+    en = _f64(ph_en)
+    gm = np.ascontiguousarray(np.stack([np.asfortranarray(g).ravel(order="F") for g in gmixed]))
+    gsz = gm.shape[1]
+    gi = _i64(gidx)
+    out = np.zeros(n)
+    lib().orc_g2_list(ct.byref(w.s), ct.c_int64(n), _p(k), _p(q), _p(a), _p(b), _p(c), _p(en), _p(gm),
+                      ct.c_int64(gsz), _p(gi, c_i64p), ct.c_int(1 if wannspace == "ph" else 0), _p(out),
+                      ct.c_int(nthreads))
+    return out
+
+
 def ibzq_g2(w: Wannier, el, ph, istate_first=1, istate_last=None, nthreads=0):
     nb, nbr = w.mat.nb, w.mat.nbr
     if istate_last is None:

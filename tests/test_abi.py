@@ -51,6 +51,38 @@ def test_write_and_read_g2_files(tmp_path):
     assert c == 2 and np.array_equal(v, g2[2, :2])    # 'q': count + count values
 
 
+def test_g2_files_round_trip_through_the_abi(tmp_path):
+    """elph_read_g2_files (the read_gk2/read_gq2 resume) returns what
+    elph_write_g2_files wrote; missing files are states the driver skipped;
+    counts beyond the row stride, truncated files and a missing directory are
+    errors, not silent truncation."""
+    import numpy as np
+
+    from elphbolt_amd import interactions
+    from elphbolt_amd._lib import ELPH_EINVAL, ELPH_EIO, ElphError
+    rng = np.random.default_rng(3)
+    counts = np.array([5, -1, 0, 7], np.int64)
+    g2 = rng.random((4, 7))
+    for kind in ("k", "q"):
+        interactions.write_g2_files(str(tmp_path), kind, 11, counts, g2)
+        c, g = interactions.read_g2_files(str(tmp_path), kind, 11, 4, 7)
+        assert np.array_equal(c, counts)
+        for t in np.where(counts >= 0)[0]:
+            assert np.array_equal(g[t, :counts[t]], g2[t, :counts[t]])
+    with pytest.raises(ElphError) as ei:
+        interactions.write_g2_files(str(tmp_path), "q", 1, np.array([8], np.int64), g2[:1])
+    assert ei.value.code == ELPH_EINVAL
+    # a truncated file (count says 5 values, 2 present)
+    path = tmp_path / "gk2.istate11"
+    path.write_bytes(path.read_bytes()[:8 + 16])
+    with pytest.raises(ElphError) as ei:
+        interactions.read_g2_files(str(tmp_path), "k", 11, 1, 7)
+    assert ei.value.code == ELPH_EIO
+    st = _lib.load().elph_write_g2_files(str(tmp_path / "no" / "such").encode(), b"k", 1, 1,
+                                         counts.ctypes.data_as(_lib.c_i64p), g2.ctypes.data_as(_lib.c_dblp), 7)
+    assert st == ELPH_EIO
+
+
 def test_symmetry_host_path():
     """elph_symmetry / elph_irred_wedge are host code: same counts as the
     reference's published values."""

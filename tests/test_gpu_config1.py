@@ -1,0 +1,155 @@
+"""The benchmarked workload itself under oracle parity: the 3C-SiC-like
+material (fcc a = 4.36 A, 2 atoms, 8 bands, 6 branches, WS sets of the 8^3
+coarse grid = 617 cells, 24^3 k = q mesh, window keeping ~4 of 8 bands) as
+calculate_eph_interaction_ibzk('g') (src/interactions.f90:2095-2400) runs it.
+
+* setup: the product's calculate_electrons / calculate_phonons at 24^3 give
+  the oracle's index lists, IBZ, window counts and energies;
+* sampled states against the oracle: every in-window band of the first, a
+  middle and the last IBZ k, over three k' windows of the 13824-long row
+  (its start, middle and end), so the phase tables of every q = k' - k region
+  and the tail of the row are covered; FP64 within 1e-12 and mixed within
+  1e-5 relative on entries above 1e-6 * max|g|^2, counts and zeros exact;
+* the whole job (all 3304 IBZ states, 1786 in the window, 13824 k' each):
+  mixed against FP64 within 1e-5 above 1e-6 * the global max, counts, row
+  lengths and exact zeros equal.
+"""
+import numpy as np
+import pytest
+
+from elphbolt_amd import bz, interactions, synth
+from elphbolt_amd.wannier import Wannier
+from oracle import oracle as O
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+FLOOR = 1e-6
+KP_WINDOW = 96
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    _flush("config 1: generating the material")
+    mat = synth.config(1)
+    o = O.Wannier(mat)
+    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    _flush("config 1: oracle setup")
+    el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = O.Phonons(o, qr, nsr, mat.phmesh)
+    _flush(f"config 1: nwv={el.nwv} nirr={el.nwv_irred} nstates_inwindow={el.nstates_inwindow}")
+    w = Wannier(mat)
+    nb = mat.nb
+    act = np.abs(el.ens_irred - el.enref) <= el.fsthick      # (nirr, nb)
+    kin = np.where(act.any(axis=1))[0]
+    picks = sorted({int(kin[0]), int(kin[len(kin) // 2]), int(kin[-1])})
+    # states of the picked IBZ k (all bands; the out-of-window ones must give -1)
+    ranges = [(ik * nb + 1, ik * nb + nb) for ik in picks]
+    windows = [1, el.nwv // 2 - KP_WINDOW // 2, el.nwv - KP_WINDOW + 1]
+    ref = {}
+    for (a, b) in ranges:
+        for kp0 in windows:
+            ref[(a, kp0)] = O.ibzk_g2_window(o, el, ph, a, b, kp0, KP_WINDOW)
+    _flush(f"config 1: oracle rows for IBZ k {picks} x k' windows {windows}")
+    return dict(mat=mat, o=o, el=el, ph=ph, w=w, qr=qr, nsr=nsr, ranges=ranges, windows=windows, ref=ref)
+
+
+def test_config1_setup_matches_oracle(cfg1):
+    """calculate_electrons / calculate_phonons of the product at the
+    benchmark's 24^3 mesh against the oracle's."""
+    mat, el_o, ph_o, w = cfg1["mat"], cfg1["el"], cfg1["ph"], cfg1["w"]
+    sym = bz.Symmetry(mat.lattvecs, mat.basis, mat.masses, mat.wvmesh)
+    assert sym.nsymm == 48
+    el = bz.Electrons(w, sym, mat.wvmesh, mat.enref, mat.fsthick)
+    ph = bz.Phonons(w, sym, mat.phmesh)
+    assert el.nwv_irred == el_o.nwv_irred == 413
+    assert np.array_equal(el.indexlist, el_o.indexlist)
+    assert np.array_equal(el.indexlist_irred, el_o.indexlist_irred)
+    assert el.nstates_inwindow == el_o.nstates_inwindow
+    assert np.max(np.abs(el.ens - el_o.ens)) < 1e-11
+    assert np.max(np.abs(el.ens_irred - el_o.ens_irred)) < 1e-11
+    assert np.array_equal(ph.indexlist_irred, ph_o.indexlist_irred)
+    assert np.max(np.abs(ph.ens - ph_o.ens)) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config1_sampled_states_vs_oracle(cfg1, precision):
+    mat, el, ph, w = cfg1["mat"], cfg1["el"], cfg1["ph"], cfg1["w"]
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    plan = interactions.IbzkPlan(w, el, ph, precision=precision)
+    nprocs = el.nstates_inwindow * mat.nbr
+    assert plan.nprocs == nprocs
+    pairs = []
+    for (a, b) in cfg1["ranges"]:
+        c_g, g_g = plan.run(a, b)
+        for kp0 in cfg1["windows"]:
+            c_o, g_o, off = cfg1["ref"][(a, kp0)]
+            assert np.array_equal(c_g < 0, c_o < 0)
+            assert np.all(c_g[c_g >= 0] == nprocs)
+            for t in np.where(c_o >= 0)[0]:
+                got = g_g[t, off[t]:off[t] + c_o[t]]
+                pairs.append((got, g_o[t, :c_o[t]]))
+    plan.close()
+    assert pairs
+    gmax = max(float(np.abs(r).max()) for _, r in pairs)
+    worst, n = 0.0, 0
+    for got, r in pairs:
+        assert np.array_equal(got == 0.0, r == 0.0)
+        sel = np.abs(r) > FLOOR * gmax
+        n += int(sel.sum())
+        if sel.any():
+            worst = max(worst, float(np.max(np.abs(got[sel] - r[sel]) / np.abs(r[sel]))))
+    _flush(f"config 1 {precision}: {len(pairs)} (state, k' window) blocks, {n} entries, max rel {worst:.2e}")
+    assert worst < tol
+
+
+def test_config1_whole_job_mixed_vs_fp64(cfg1):
+    """All IBZ states of the benchmark step, mixed against FP64 (both on the
+    device), in chunks of 64 IBZ k."""
+    mat, el, ph, w = cfg1["mat"], cfg1["el"], cfg1["ph"], cfg1["w"]
+    nb = mat.nb
+    plans = {p: interactions.IbzkPlan(w, el, ph, precision=p) for p in ("fp64", "mixed")}
+    nprocs = plans["fp64"].nprocs
+    nst = el.nwv_irred * nb
+    chunk = 64 * nb
+    bufs = {p: w.device_buffer((chunk, nprocs)) for p in plans}
+    act_ref = np.abs(el.ens_irred - el.enref) <= el.fsthick
+    # pass 1: the global max of the FP64 rows
+    gmax = 0.0
+    for s0 in range(1, nst + 1, chunk):
+        s1 = min(nst, s0 + chunk - 1)
+        c, _ = plans["fp64"].run(s0, s1, out=bufs["fp64"])
+        act = c >= 0
+        if act.any():
+            gmax = max(gmax, float(bufs["fp64"].numpy()[:s1 - s0 + 1][act].max()))
+    # pass 2: compare
+    worst, nsel, nact, med = 0.0, 0, 0, []
+    for s0 in range(1, nst + 1, chunk):
+        s1 = min(nst, s0 + chunk - 1)
+        c64, _ = plans["fp64"].run(s0, s1, out=bufs["fp64"])
+        cmx, _ = plans["mixed"].run(s0, s1, out=bufs["mixed"])
+        assert np.array_equal(c64, cmx)
+        s = np.arange(s0 - 1, s1)
+        assert np.array_equal(c64 >= 0, act_ref[s // nb, s % nb])
+        act = c64 >= 0
+        assert np.all(c64[act] == nprocs)
+        nact += int(act.sum())
+        a = bufs["fp64"].numpy()[:s1 - s0 + 1][act]
+        b = bufs["mixed"].numpy()[:s1 - s0 + 1][act]
+        assert np.array_equal(a == 0.0, b == 0.0)
+        sel = a > FLOOR * gmax
+        nsel += int(sel.sum())
+        if sel.any():
+            rel = np.abs(b[sel] - a[sel]) / a[sel]
+            worst = max(worst, float(rel.max()))
+            med.append(float(np.median(rel)))
+    for p in plans:
+        bufs[p].free()
+        plans[p].close()
+    _flush(f"config 1 whole job: {nact} in-window states, {nsel} entries above the floor, mixed vs FP64 max rel "
+           f"{worst:.2e} (chunk medians <= {max(med):.2e})")
+    assert nact == int(act_ref.sum()) == 1786
+    assert worst < 1e-5

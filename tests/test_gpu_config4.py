@@ -1,0 +1,116 @@
+"""The serving workload's resident set under oracle parity: the eight
+materials of the serving benchmark (the first eight draws of the
+multi-material generator with a 12 GB vertex cap and a vertex of at least
+2 GB; 38.8 GB of Wannier vertex) resident in one server, with probe requests
+on two of them -- 14 bands x 24 branches on 165 WS cells, and 14 bands x 6
+branches on 423 cells -- whose vertex is also held on the host for the
+oracle.  Every (k, q, m, n, s) of a probe is compared with the oracle's
+per-vertex g2 (src/wannier.f90:903-1011, 'ph' form; U_k, U_k+q from el_wann,
+e_q from ph_wann, gkRp(k)) after summing over degenerate levels (the server
+computes its own eigenvectors): FP64 server within 1e-6 (FP32 output), mixed
+within 1e-5, on sums above 1e-6 * max; exact zeros at Gamma stay zero.  The
+probes run while the other materials' requests are in flight."""
+import numpy as np
+import pytest
+
+from elphbolt_amd import synth
+from elphbolt_amd.serving import Server
+from elphbolt_amd.wannier import Wannier
+from oracle import oracle as O
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
+
+FLOOR = 1e-6
+TOL = {"fp64": 1e-6, "mixed": 1e-5}
+ORACLE_PICKS = (5, 9)
+
+
+def _groups(e, tol=1e-7):
+    lab = np.zeros(len(e), int)
+    for i in range(1, len(e)):
+        lab[i] = lab[i - 1] + (0 if abs(e[i] - e[i - 1]) <= tol * max(1.0, abs(e[i])) else 1)
+    return lab
+
+
+def _reduce(x, lm, ln, ls):
+    out = np.zeros((lm.max() + 1, ln.max() + 1, ls.max() + 1))
+    np.add.at(out, (lm[:, None, None], ln[None, :, None], ls[None, None, :]), x)
+    return out
+
+
+def _oracle_blocks(mat, k, q):
+    """Per (k, q) pair: the oracle's |g|^2 (nb, nb, nbr) and the degenerate-level labels."""
+    o = O.Wannier(mat)
+    nb, nbr = mat.nb, mat.nbr
+    blocks = []
+    for ik in range(k.shape[0]):
+        ek, vk, _ = o.el_wann(k[ik][None], velocities=False)
+        gm = o.gkRp(k[ik])
+        for iq in range(q.shape[0]):
+            ekq, vkq, _ = o.el_wann((k[ik] + q[iq])[None], velocities=False)
+            wq, uq, _ = o.ph_wann(q[iq][None])
+            m, n, s = np.meshgrid(np.arange(nb), np.arange(nb), np.arange(nbr), indexing="ij")
+            m, n, s = m.ravel(), n.ravel(), s.ravel()
+            cnt = m.size
+            g = O.g2_list(o, np.repeat(k[ik][None], cnt, 0), np.repeat(q[iq][None], cnt, 0), vk[0, m], vkq[0, n],
+                          uq[0, s], wq[0, s], [gm], np.zeros(cnt, np.int64), "ph")
+            blocks.append((g.reshape(nb, nb, nbr), (_groups(ek[0]), _groups(ekq[0]), _groups(wq[0], 1e-9))))
+    return blocks
+
+
+@pytest.fixture(scope="module")
+def serving_set():
+    idx = synth.serving_indices()
+    mats = {}
+    for i in idx:
+        vertex = "host" if i in ORACLE_PICKS else "device"
+        mats[i] = synth.multi_material(i, budget_bytes=synth.SERVING_BUDGET, vertex=vertex, window=False)
+    gb = sum(m.gwann_bytes for m in mats.values()) / 1e9
+    print(f"config 4 resident set: {[m.name for m in mats.values()]}, {gb:.1f} GB of vertex", flush=True)
+    assert 35.0 < gb < 45.0
+    probes = {}
+    for j, i in enumerate(ORACLE_PICKS):
+        rng = np.random.default_rng(100 + j)
+        k, q = rng.random((2, 3)), rng.random((2, 3))
+        q[0] = 0.0                      # Gamma: acoustic modes give exact zeros
+        probes[i] = (k, q, _oracle_blocks(mats[i], k, q))
+        print(f"  oracle probe on {mats[i].name}: {k.shape[0]} k x {q.shape[0]} q", flush=True)
+    ws = {i: Wannier(m) for i, m in mats.items()}
+    return dict(mats=mats, ws=ws, probes=probes)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config4_server_vs_oracle(serving_set, precision):
+    mats, ws, probes = serving_set["mats"], serving_set["ws"], serving_set["probes"]
+    rng = np.random.default_rng(7)
+    with Server(precision, pool_bytes=8 << 30, nslots=4) as srv:
+        mids = {i: srv.add_material(w) for i, w in ws.items()}
+        # background load on the other materials, then the probes
+        bg = [srv.submit(mids[i], rng.random((64, 3)), rng.random((48, 3)))
+              for i in mids if i not in ORACLE_PICKS]
+        rid = {i: srv.submit(mids[i], probes[i][0], probes[i][1]) for i in ORACLE_PICKS}
+        res = {}
+        for i, r in rid.items():
+            st, _ = srv.wait(r)
+            assert st == 0
+            res[i] = srv.fetch(r, release=True)
+        for r in bg:
+            assert srv.wait(r)[0] == 0
+            srv.release(r)
+        assert srv.stats()["failed"] == 0
+    for i in ORACLE_PICKS:
+        k, q, blocks = probes[i]
+        mat = mats[i]
+        assert res[i].shape == (k.shape[0], q.shape[0], mat.nb, mat.nb, mat.nbr)
+        gmax = max(b[0].max() for b in blocks)
+        worst = 0.0
+        for p, (ref, (lm, ln, ls)) in enumerate(blocks):
+            ik, iq = divmod(p, q.shape[0])
+            a = _reduce(res[i][ik, iq].astype(np.float64), lm, ln, ls)
+            b = _reduce(ref, lm, ln, ls)
+            assert np.all(a[b == 0.0] == 0.0)
+            sel = b > FLOOR * gmax
+            if sel.any():
+                worst = max(worst, float(np.max(np.abs(a[sel] - b[sel]) / b[sel])))
+        print(f"config 4 {precision} {mat.name}: max rel {worst:.2e}", flush=True)
+        assert worst < TOL[precision]
