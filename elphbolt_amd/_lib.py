@@ -55,6 +55,9 @@ PROTOTYPES = {
                                    c_i64p, c_i64p, c_i64p, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp, c_dblp]),
     "elph_ctx_create_epw": (ct.c_int, [ct.POINTER(ct.c_void_p), ct.c_int, ct.c_char_p, ct.c_int64, c_dblp, c_dblp,
                                        c_dblp, c_i64p, ct.c_int32, c_dblp, c_dblp]),
+    "elph_epw_read_dims": (ct.c_int, [ct.c_char_p, c_i64p]),
+    "elph_epw_read": (ct.c_int, [ct.c_char_p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_dblp, c_dblp,
+                                 c_dblp]),
     "elph_ctx_destroy": (ct.c_int, [ct.c_void_p]),
     "elph_last_error": (ct.c_char_p, [ct.c_void_p]),
     "elph_ctx_dims": (ct.c_int, [ct.c_void_p, c_i64p]),

@@ -1,6 +1,7 @@
 // Host-side pieces of the vertex path: index search, crystal geometry,
 // symmetry operations and the irreducible wedge, the EPW reader, WS wrapping
 // vectors and the g2 file writer.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -259,35 +260,164 @@ int ws_wrapping_vectors(const double lvec[9], const double vl[3], double eps, st
     return (int)(wrappers.size() / 3);
 }
 
-// Fortran list-directed complex "(re,im)".
-static bool parse_complex(const std::string &s, double &re, double &im) {
-    size_t a = s.find('('), b = s.find(','), c = s.find(')');
-    if (a == std::string::npos || b == std::string::npos || c == std::string::npos) return false;
-    re = std::strtod(s.c_str() + a + 1, nullptr);
-    im = std::strtod(s.c_str() + b + 1, nullptr);
-    return true;
-}
+// Fortran list-directed input (the `read(u, *)` statements of
+// src/wannier.f90:313-340, 379-427): every statement starts at a new record
+// and skips what is left of its last record; values are separated by blanks
+// or a comma, may continue onto the next records, and a slash ends the
+// statement; reals take E, D or Q exponents (or a signed exponent with no
+// letter); a complex is "(re, im)" with blanks allowed around both parts and
+// the pair allowed to continue onto the next record.
+class ListDirected {
+  public:
+    explicit ListDirected(std::istream &in) {
+        std::string line;
+        while (std::getline(in, line)) {
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            recs_.push_back(line);
+        }
+    }
+    // starts the next statement: the rest of the current record is skipped
+    void statement() {
+        if (started_) ++rec_;
+        started_ = true;
+        pos_ = 0;
+        ended_ = false;
+        first_ = true;
+    }
+    bool real(double &x) {
+        std::string t;
+        if (!item(t)) return false;
+        return to_real(t, x);
+    }
+    bool integer(int64_t &x) {
+        std::string t;
+        if (!item(t)) return false;
+        char *e = nullptr;
+        x = std::strtoll(t.c_str(), &e, 10);
+        return e && *e == '\0' && !t.empty();
+    }
+    bool complex(double &re, double &im) {
+        if (!skip_separators()) return false;
+        if (cur() != '(') return false;
+        ++pos_;
+        std::string a, b;
+        if (!blanks_across() || !number(a) || !blanks_across() || cur() != ',') return false;
+        ++pos_;
+        if (!blanks_across() || !number(b) || !blanks_across() || cur() != ')') return false;
+        ++pos_;
+        first_ = false;
+        return to_real(a, re) && to_real(b, im);
+    }
+
+  private:
+    std::vector<std::string> recs_;
+    size_t rec_ = 0, pos_ = 0;
+    bool started_ = false, ended_ = false, first_ = true;  // first_: no value read yet in this statement
+
+    char cur() const { return rec_ < recs_.size() && pos_ < recs_[rec_].size() ? recs_[rec_][pos_] : '\0'; }
+    bool eor() const { return rec_ >= recs_.size() || pos_ >= recs_[rec_].size(); }
+    static bool blank(char c) { return c == ' ' || c == '\t'; }
+    bool next_record() {
+        if (rec_ + 1 >= recs_.size()) return false;
+        ++rec_;
+        pos_ = 0;
+        return true;
+    }
+    // blanks within a record and record ends (inside a complex constant)
+    bool blanks_across() {
+        for (;;) {
+            while (!eor() && blank(cur())) ++pos_;
+            if (!eor()) return true;
+            if (!next_record()) return false;
+        }
+    }
+    // moves to the start of the next value: blanks, record ends and at most
+    // one comma after the previous value
+    bool skip_separators() {
+        if (ended_) return false;
+        bool comma = false;
+        for (;;) {
+            while (!eor() && blank(cur())) ++pos_;
+            if (eor()) {  // a record end separates like a blank
+                if (!next_record()) return false;
+                continue;
+            }
+            const char c = cur();
+            if (c == '/') { ended_ = true; return false; }
+            if (c == ',') {
+                if (comma || first_) return false;  // null values are not accepted here
+                comma = true;
+                ++pos_;
+                continue;
+            }
+            return true;
+        }
+    }
+    bool number(std::string &t) {
+        t.clear();
+        while (!eor()) {
+            const char c = cur();
+            if (blank(c) || c == ',' || c == ')' || c == '/' || c == '(') break;
+            t.push_back(c);
+            ++pos_;
+        }
+        return !t.empty();
+    }
+    bool item(std::string &t) {
+        if (!skip_separators()) return false;
+        if (!number(t)) return false;
+        first_ = false;
+        return true;
+    }
+    static bool to_real(const std::string &t, double &x) {
+        std::string u;
+        bool expo = false, digits = false;
+        for (size_t i = 0; i < t.size(); ++i) {
+            char c = t[i];
+            if (c == 'd' || c == 'D' || c == 'q' || c == 'Q' || c == 'e' || c == 'E') {
+                c = 'E';
+                expo = true;
+            } else if ((c == '+' || c == '-') && i > 0 && !expo && digits) {
+                u.push_back('E');  // "1.0+05": exponent without a letter
+                expo = true;
+            } else if (std::isdigit((unsigned char)c)) {
+                digits = true;
+            }
+            u.push_back(c);
+        }
+        char *e = nullptr;
+        x = std::strtod(u.c_str(), &e);
+        return digits && e && *e == '\0';
+    }
+};
 
 int read_epw(const std::string &dir, EpwData &o, std::string &err) {
     std::ifstream f(dir + "/epwdata.fmt");
     if (!f) { err = "cannot open " + dir + "/epwdata.fmt"; return ELPH_EIO; }
+    ListDirected in(f);
     double ef;
-    f >> ef;
-    f >> o.nb >> o.nwsk >> o.nbr >> o.nwsq >> o.nwsg;
-    if (!f || o.nb <= 0 || o.nbr <= 0 || o.nwsk <= 0 || o.nwsq <= 0 || o.nwsg <= 0) {
-        err = "bad header in epwdata.fmt"; return ELPH_EIO;
+    in.statement();
+    if (!in.real(ef)) { err = "bad Fermi energy record in epwdata.fmt"; return ELPH_EIO; }
+    in.statement();
+    if (!in.integer(o.nb) || !in.integer(o.nwsk) || !in.integer(o.nbr) || !in.integer(o.nwsq) ||
+        !in.integer(o.nwsg) || o.nb <= 0 || o.nbr <= 0 || o.nwsk <= 0 || o.nwsq <= 0 || o.nwsg <= 0) {
+        err = "bad header in epwdata.fmt";
+        return ELPH_EIO;
     }
     const int64_t ndummy = (o.nbr / 3 + 1) * 9;  // Born charges + epsilon, read and ignored
-    for (int64_t i = 0; i < ndummy; ++i) { double x; f >> x; }
-    std::string tok;
+    in.statement();
+    for (int64_t i = 0; i < ndummy; ++i) {
+        double x;
+        if (!in.real(x)) { err = "bad Born charge / epsilon records in epwdata.fmt"; return ELPH_EIO; }
+    }
     auto read_block = [&](std::vector<double> &dst, int64_t nws, int64_t n) -> bool {
         dst.assign(2 * nws * n * n, 0.0);
         for (int64_t ib = 0; ib < n; ++ib)
             for (int64_t jb = 0; jb < n; ++jb)
                 for (int64_t iuc = 0; iuc < nws; ++iuc) {
-                    if (!(f >> tok)) return false;
                     double re, im;
-                    if (!parse_complex(tok, re, im)) return false;
+                    in.statement();
+                    if (!in.complex(re, im)) return false;
                     const int64_t off = iuc + nws * (ib + n * jb);
                     dst[2 * off] = re;
                     dst[2 * off + 1] = im;
@@ -306,11 +436,14 @@ int read_epw(const std::string &dir, EpwData &o, std::string &err) {
     auto read_cells = [&](const char *tag, int64_t n, std::vector<int64_t> &cells, std::vector<int64_t> &deg) -> bool {
         std::ifstream fc(dir + "/rcells_" + tag), fd(dir + "/wsdeg_" + tag);
         if (!fc || !fd) return false;
+        ListDirected lc(fc), ld(fd);
         cells.assign(3 * n, 0);
         deg.assign(n, 0);
-        for (int64_t i = 0; i < n; ++i) {  // (nws,3) column-major
-            if (!(fc >> cells[i] >> cells[i + n] >> cells[i + 2 * n])) return false;
-            if (!(fd >> deg[i])) return false;
+        for (int64_t i = 0; i < n; ++i) {  // (nws,3) column-major, one statement per cell
+            lc.statement();
+            ld.statement();
+            if (!lc.integer(cells[i]) || !lc.integer(cells[i + n]) || !lc.integer(cells[i + 2 * n])) return false;
+            if (!ld.integer(deg[i])) return false;
         }
         return true;
     };
@@ -321,6 +454,48 @@ int read_epw(const std::string &dir, EpwData &o, std::string &err) {
 }
 
 }  // namespace elph
+
+extern "C" int elph_epw_read_dims(const char *dir, int64_t dims[5]) {
+    if (!dir || !dims) return ELPH_EINVAL;
+    elph::EpwData e;
+    std::string err;
+    const int st = elph::read_epw(dir, e, err);
+    if (st != ELPH_OK) {
+        std::fprintf(stderr, "elph_epw_read_dims: %s\n", err.c_str());
+        return st;
+    }
+    const int64_t d[5] = {e.nb, e.nwsk, e.nbr, e.nwsq, e.nwsg};
+    std::memcpy(dims, d, sizeof(d));
+    return ELPH_OK;
+}
+
+extern "C" int elph_epw_read(const char *dir, const int64_t dims[5], int64_t *rcells_k, int64_t *rcells_q,
+                             int64_t *rcells_g, int64_t *elwsdeg, int64_t *phwsdeg, int64_t *gwsdeg, double *Hwann,
+                             double *Dphwann, double *gwann) {
+    if (!dir || !dims) return ELPH_EINVAL;
+    elph::EpwData e;
+    std::string err;
+    const int st = elph::read_epw(dir, e, err);
+    if (st != ELPH_OK) {
+        std::fprintf(stderr, "elph_epw_read: %s\n", err.c_str());
+        return st;
+    }
+    if (dims[0] != e.nb || dims[1] != e.nwsk || dims[2] != e.nbr || dims[3] != e.nwsq || dims[4] != e.nwsg)
+        return ELPH_EINVAL;
+    auto put = [](auto *dst, const auto &v) {
+        if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(rcells_k, e.rcells_k);
+    put(rcells_q, e.rcells_q);
+    put(rcells_g, e.rcells_g);
+    put(elwsdeg, e.elwsdeg);
+    put(phwsdeg, e.phwsdeg);
+    put(gwsdeg, e.gwsdeg);
+    put(Hwann, e.Hwann);
+    put(Dphwann, e.Dphwann);
+    put(gwann, e.gwann);
+    return ELPH_OK;
+}
 
 extern "C" int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates,
                                    const int64_t *counts, const double *g2, int64_t row_stride) {

@@ -437,30 +437,57 @@ def _fmt_c(z: complex) -> str:
     return f"({z.real:.17E},{z.imag:.17E})"
 
 
-def write_epw(mat: Material, outdir: str, ef: float = 0.0) -> None:
+def _fmt_c_fortran(z: complex, i: int) -> str:
+    """Other forms a Fortran list-directed read accepts for the same value:
+    blanks around both parts, D exponents, the pair split over two records,
+    values after the first on the record (ignored by the read)."""
+    re = f"{z.real:.17E}".replace("E", "D")
+    im = f"{z.imag:.17E}".replace("E", "d") if i % 2 else f"{z.imag:.17E}"
+    if i % 3 == 0:
+        return f"  ( {re} ,\n   {im} )  9.9 -1"
+    return f" (  {re}  , {im}  )   ,7.0"
+
+
+def write_epw(mat: Material, outdir: str, ef: float = 0.0, style: str = "plain") -> None:
     """Write epwdata.fmt, epmatwp1, rcells_{k,q,g}, wsdeg_{k,q,g}
-    (layout of src/wannier.f90:313-427)."""
+    (layout of src/wannier.f90:313-427).  style "fortran" writes the text
+    files in other forms that the reference's list-directed reads accept
+    (blanks, D exponents, comma separators, records split and continued,
+    trailing values), with the same values."""
     os.makedirs(outdir, exist_ok=True)
     nb, nbr = mat.nb, mat.nbr
+    fort = style == "fortran"
+    ndummy = (nbr // 3 + 1) * 9
     with open(os.path.join(outdir, "epwdata.fmt"), "w") as f:
-        f.write(f"{ef:.17E}\n")
-        f.write(f"{nb} {mat.nwsk} {nbr} {mat.nwsq} {mat.nwsg}\n")
-        ndummy = (nbr // 3 + 1) * 9
-        f.write(" ".join(["0.0"] * ndummy) + "\n")
+        if fort:
+            f.write(f"  {ef:.17E}   123.0  unused\n".replace("E", "D"))
+            f.write(f" {nb},{mat.nwsk} , {nbr}\n   {mat.nwsq}   {mat.nwsg}  99\n")
+            vals = ["0.0D0"] * ndummy
+            for j in range(0, ndummy, 4):
+                f.write(" , ".join(vals[j:j + 4]) + "\n")
+        else:
+            f.write(f"{ef:.17E}\n")
+            f.write(f"{nb} {mat.nwsk} {nbr} {mat.nwsq} {mat.nwsg}\n")
+            f.write(" ".join(["0.0"] * ndummy) + "\n")
+        i = 0
         for ib in range(nb):
             for jb in range(nb):
                 for iuc in range(mat.nwsk):
-                    f.write(_fmt_c(mat.Hwann[iuc, ib, jb]) + "\n")
+                    z = mat.Hwann[iuc, ib, jb]
+                    f.write((_fmt_c_fortran(z, i) if fort else _fmt_c(z)) + "\n")
+                    i += 1
         for ib in range(nbr):
             for jb in range(nbr):
                 for iuc in range(mat.nwsq):
-                    f.write(_fmt_c(mat.Dphwann[iuc, ib, jb]) + "\n")
+                    z = mat.Dphwann[iuc, ib, jb]
+                    f.write((_fmt_c_fortran(z, i) if fort else _fmt_c(z)) + "\n")
+                    i += 1
     np.asfortranarray(mat.gwann).ravel(order="F").tofile(os.path.join(outdir, "epmatwp1"))
     for tag, cells, deg in (("k", mat.rcells_k, mat.elwsdeg), ("q", mat.rcells_q, mat.phwsdeg),
                             ("g", mat.rcells_g, mat.gwsdeg)):
         with open(os.path.join(outdir, f"rcells_{tag}"), "w") as f:
             for c in cells:
-                f.write(f"{c[0]:10d} {c[1]:10d} {c[2]:10d}\n")
+                f.write(f" {c[0]}, {c[1]},{c[2]}  0\n" if fort else f"{c[0]:10d} {c[1]:10d} {c[2]:10d}\n")
         with open(os.path.join(outdir, f"wsdeg_{tag}"), "w") as f:
             for d in deg:
-                f.write(f"{d:10d}\n")
+                f.write(f"  {d} 5\n" if fort else f"{d:10d}\n")

@@ -178,6 +178,31 @@ class Wannier:
         return out
 
 
+def read_epw(dirname):
+    """`read_EPW_Wannier` (src/wannier.f90:289-428) on the host only: a dict of
+    the arrays with the reference's shapes (rcells_* (nws,3), *wsdeg (nws),
+    Hwann (nwsk,nb,nb), Dphwann (nwsq,nbr,nbr), gwann (nb,nb,nwsk,nbr,nwsg),
+    Fortran order), read by the library's list-directed reader."""
+    lib = _lib.load()
+    d = np.zeros(5, np.int64)
+    st = lib.elph_epw_read_dims(str(dirname).encode(), _p(d, c_i64p))
+    if st != 0:
+        raise ElphError(st, f"reading EPW files in {dirname} failed")
+    nb, nwsk, nbr, nwsq, nwsg = map(int, d)
+    out = {"rcells_k": np.zeros((nwsk, 3), np.int64, order="F"), "rcells_q": np.zeros((nwsq, 3), np.int64, order="F"),
+           "rcells_g": np.zeros((nwsg, 3), np.int64, order="F"), "elwsdeg": np.zeros(nwsk, np.int64),
+           "phwsdeg": np.zeros(nwsq, np.int64), "gwsdeg": np.zeros(nwsg, np.int64),
+           "Hwann": np.zeros((nwsk, nb, nb), np.complex128, order="F"),
+           "Dphwann": np.zeros((nwsq, nbr, nbr), np.complex128, order="F"),
+           "gwann": np.zeros((nb, nb, nwsk, nbr, nwsg), np.complex128, order="F")}
+    ints = [_p(out[k], c_i64p) for k in ("rcells_k", "rcells_q", "rcells_g", "elwsdeg", "phwsdeg", "gwsdeg")]
+    cplx = [_p(out[k]) for k in ("Hwann", "Dphwann", "gwann")]
+    st = lib.elph_epw_read(str(dirname).encode(), _p(d, c_i64p), *ints, *cplx)
+    if st != 0:
+        raise ElphError(st, f"reading EPW files in {dirname} failed")
+    return out
+
+
 class DeviceBuffer:
     """Device memory allocated through the library (no torch in the process:
     torch wheels bundle their own HIP runtime)."""

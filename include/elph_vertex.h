@@ -66,6 +66,16 @@ int elph_ctx_create_epw(elph_ctx **ctx, int device, const char *dir, int64_t nat
                         const double *basis, const double *masses, const int64_t *coarse_qmesh, int32_t polar,
                         const double *epsilon, const double *born);
 
+/* The reader of elph_ctx_create_epw on its own, host only (no device): the
+ * sizes (nb, nwsk, nbr, nwsq, nwsg) of the EPW files in `dir`, then the arrays
+ * in the layouts of elph_ctx_create (any output may be NULL).  The text files
+ * are read as the reference's list-directed `read(u, *)` statements read them
+ * (src/wannier.f90:313-340, 379-427). */
+int elph_epw_read_dims(const char *dir, int64_t dims[5]);
+int elph_epw_read(const char *dir, const int64_t dims[5], int64_t *rcells_k, int64_t *rcells_q, int64_t *rcells_g,
+                  int64_t *elwsdeg, int64_t *phwsdeg, int64_t *gwsdeg, double *Hwann, double *Dphwann,
+                  double *gwann);
+
 int elph_ctx_destroy(elph_ctx *ctx);
 const char *elph_last_error(const elph_ctx *ctx);
 /* Sizes read back from a context (nb, nbr, nwsk, nwsq, nwsg, natoms). */
