@@ -84,11 +84,34 @@ def survey_flops(nst, nwv, nstates_inwindow, nb, nbr, nwsg, nm):
     return pairs * (8.0 * nwsg * nb * nb * nbr + rot) + 4.0 * nst * nstates_inwindow * nbr
 
 
-def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
+def host_cpu_info():
+    """The host's CPU as this process sees it: logical CPUs (nproc), the CPUs
+    this process may run on, OMP_NUM_THREADS and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "model": model}
+
+
+def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=None):
     """The oracle (literal C restatement of the reference's per-vertex g2 in the
     ibzk loop order, OpenMP over states) on a bounded sample: the given IBZ
-    states, each over the first `ikp_limit` final k' of the window."""
+    states, each over the first `ikp_limit` final k' of the window.  Threads:
+    the CPUs this process may use, capped by OMP_NUM_THREADS when set."""
     from oracle import oracle as O
+    info = host_cpu_info()
+    if threads is None:
+        threads = info["affinity"]
+        if info["omp_num_threads"]:
+            threads = min(threads, int(info["omp_num_threads"]))
     o = O.Wannier(mat)
     first, last = states
     # two pilot sizes: the difference removes the per-state g(k,Rp) setup cost
@@ -105,8 +128,9 @@ def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=16):
     dt = time.perf_counter() - t0
     nvert = int(c[c >= 0].sum())
     return {"value": nvert / dt, "unit": "vertices/s", "cores": threads, "kind": "port",
+            "host": info,
             "sample": f"ibzk states {first}..{last} ({nact} in window), first {ikp_limit} of {el.nwv} k' each, "
-                      f"{nvert} vertices in {dt:.1f} s"}
+                      f"{nvert} vertices in {dt:.1f} s on {threads} OpenMP threads"}
 
 
 SERVING_SEED = 0xE1B04000
@@ -240,13 +264,19 @@ def serving_bench(args):
 BATCH_SEED = 0xE1B05000
 
 
-def transport_bench(args, w, sym, el, plan, g2, nst):
+def transport_bench(args, w, sym, el, plans, g2, nst):
     """The consumers of one step's g2 rows (all IBZ states, device-resident):
     calculate_eph_interaction_ibzk('X') with the tetrahedron delta, the e-ph
-    RTA rates, then the RTA transport coefficients."""
-    from elphbolt_amd import bz, interactions
+    RTA rates, then the RTA transport coefficients with the field terms
+    multiplied by the symmetrizers and the tensors symmetrized over the
+    Cartesian operations, as the reference's dragless RTA step does
+    (src/bte.f90:236-277).  The chain runs on the FP64 rows (timed) and on the
+    mixed rows of the same states; sigma, sigma*S and S of both are reported
+    with their relative differences."""
+    from elphbolt_amd import interactions
     from elphbolt_amd.bte import ElectronRTA
     T, mu = 300.0, float(el.enref)
+    plan = plans["fp64"]
     t0 = time.perf_counter()
     tet = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
     t_tet = time.perf_counter() - t0
@@ -264,20 +294,27 @@ def transport_bench(args, w, sym, el, plan, g2, nst):
     procs = int(act.sum()) * plan.nprocs
     k_s = float(np.mean(kms)) * 1e-3
     algo = procs * (8 + 16 + 16)    # |g|^2 read, X+ and X- written, the two state maps written
-    mesh = np.array(el.wvmesh)
-    i0 = el.indexlist - 1
-    kv = np.stack([i0 % mesh[0], (i0 // mesh[0]) % mesh[1], i0 // (mesh[0] * mesh[1])], axis=1) / mesh
-    _, _, vels = w.el_wann(kv, velocities=True, evecs=False)
-    irr, neq, m = bz.irred_wedge(mesh, sym, indexlist=el.indexlist, maps=True)
-    assert np.array_equal(irr, el.indexlist_irred)
+    ti = el.transport_inputs(w, sym)
     vol = abs(float(np.linalg.det(np.asarray(w.mat.lattvecs))))
-    tr = ElectronRTA(w, el.ens, vels, el.indexlist, neq, m, vol, mesh, 2)
+    tr = ElectronRTA(w, el.ens, ti["vels"], el.indexlist, ti["nequiv"], ti["ibz2fbz_map"], vol, el.wvmesh, 2,
+                     symmetrizers=ti["symmetrizers"], crotations=ti["crotations"])
     tr.rta(bufs["rates_device"], T, mu)
     t = time.perf_counter()
-    coef = tr.rta(bufs["rates_device"], T, mu)
+    coef = {"fp64": tr.rta(bufs["rates_device"], T, mu)}
     t_rta = time.perf_counter() - t
-    sig, sigS = ElectronRTA.scalar(coef["sigma"]), ElectronRTA.scalar(coef["sigmaS"])
-    log(f"[bench] RTA transport {t_rta * 1e3:.2f} ms: sigma {sig:.4e} 1/Ohm/m, sigmaS {sigS:.4e} A/m/K")
+    # the same chain on the mixed-precision rows of the same states
+    g2_mx = w.device_buffer((nst, plan.nprocs))
+    plans["mixed"].run(1, nst, out=g2_mx)
+    plans["mixed"].run_X(tet, T, mu, 1, nst, g2_mx, buffers=bufs)
+    coef["mixed"] = tr.rta(bufs["rates_device"], T, mu)
+    g2_mx.free()
+    vals = {}
+    for prec, c in coef.items():
+        sig, sigS = ElectronRTA.scalar(c["sigma"]), ElectronRTA.scalar(c["sigmaS"])
+        vals[prec] = {"sigma": sig, "sigmaS": sigS, "seebeck_V_per_K": sigS / sig if sig else None}
+    d = {k: abs(vals["mixed"][k] / vals["fp64"][k] - 1.0) for k in ("sigma", "sigmaS", "seebeck_V_per_K")}
+    log(f"[bench] RTA transport {t_rta * 1e3:.2f} ms: sigma {vals['fp64']['sigma']:.6e} 1/Ohm/m (mixed rel diff "
+        f"{d['sigma']:.2e}), sigmaS rel diff {d['sigmaS']:.2e}, S rel diff {d['seebeck_V_per_K']:.2e}")
     ach = algo / k_s / 1e9
     return {"T_K": T, "chempot_eV": mu, "states": nst, "processes": procs,
             "X_kernel_ms": round(k_s * 1e3, 3), "X_call_ms": round(float(np.mean(walls)) * 1e3, 3),
@@ -286,7 +323,10 @@ def transport_bench(args, w, sym, el, plan, g2, nst):
                          "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_ibzk_X",
                          "bytes_per_launch": algo},
             "rta_transport_ms": round(t_rta * 1e3, 3), "tetrahedra_s": round(t_tet, 3),
-            "sigma": sig, "sigmaS": sigS, "seebeck_V_per_K": sigS / sig if sig else None}
+            "symmetrized": "field terms x symmetrizers, tensors over the Cartesian operations",
+            "fp64": vals["fp64"], "mixed": vals["mixed"], "mixed_vs_fp64_rel_diff": d,
+            "sigma": vals["fp64"]["sigma"], "sigmaS": vals["fp64"]["sigmaS"],
+            "seebeck_V_per_K": vals["fp64"]["seebeck_V_per_K"]}
 
 
 def ibzq_bench(args, w, el, ph, mat):
@@ -590,7 +630,7 @@ def main():
            "mixed_vs_fp64": round(res_mx["value"] / res_64["value"], 3), "error_vs_fp64": err,
            "fp64": fp64, "roofline": roofline}
     if not args.no_transport and nblocks == 1:
-        res["transport"] = transport_bench(args, w, sym, el, plans["fp64"], out, kb * nb)
+        res["transport"] = transport_bench(args, w, sym, el, plans, out, kb * nb)
         log_mem("after transport")
     if not args.no_ibzq:
         res["ibzq"] = ibzq_bench(args, w, el, ph, mat)
@@ -611,7 +651,7 @@ def main():
                 first_active = s
                 break
         res["cpu_baseline"] = cpu_baseline(mat, el, ph, (first_active, min(first_active + 15, el.nwv_irred * nb)),
-                                           target_s=args.cpu_seconds, threads=16)
+                                           target_s=args.cpu_seconds)
     print(json.dumps(res), flush=True)
 
 

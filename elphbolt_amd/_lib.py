@@ -99,6 +99,9 @@ PROTOTYPES = {
     "elph_write_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
     "elph_read_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
     "elph_symmetry": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p, c_dblp]),
+    "elph_symmetry_cartesian": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p,
+                                           c_dblp, c_dblp]),
+    "elph_symmetrizers": (ct.c_int, [c_i64p, ct.c_int64, c_dblp, c_dblp, ct.c_int64, c_i64p, c_dblp]),
     "elph_irred_wedge": (ct.c_int, [c_i64p, ct.c_int64, c_dblp, ct.c_int64, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
     "elph_calculate_electrons": (ct.c_int, [ct.c_void_p, ct.c_int64, c_dblp, c_i64p, ct.c_double, ct.c_double,
                                             ct.POINTER(ct.c_void_p), ct.POINTER(ElphElectrons)]),

@@ -40,6 +40,31 @@ class Symmetry:
         self.nsymm = ns.value
         self.nsymm_rot = nsr.value
         self.qrotations = qr[:9 * nsr.value].copy()
+        # Cartesian operations (src/symmetry.f90:131-137): crotations_orig
+        # (3, 3, nsymm) and crotations (3, 3, nsymm_rot) in the order of qrotations
+        co, cr = np.zeros(9 * 48), np.zeros(9 * 96)
+        ns2, nsr2 = ct.c_int64(), ct.c_int64()
+        st = lib.elph_symmetry_cartesian(_p(L), len(masses), _p(b), _p(t, c_i64p), symprec, _p(m, c_i64p),
+                                         ct.byref(ns2), ct.byref(nsr2), _p(co), _p(cr))
+        if st != 0 or ns2.value != self.nsymm or nsr2.value != self.nsymm_rot:
+            raise ElphError(st or _lib.ELPH_EINVAL, "Cartesian symmetry operations failed")
+        self.crotations_orig = co[:9 * self.nsymm].reshape((3, 3, self.nsymm), order="F").copy(order="F")
+        self.crotations = cr[:9 * self.nsymm_rot].reshape((3, 3, self.nsymm_rot), order="F").copy(order="F")
+
+
+def symmetrizers(mesh, sym: Symmetry, indexlist=None):
+    """`el%symmetrizers` (src/electron.f90:462-476): (3, 3, nk) for the points
+    of indexlist (the whole mesh when None)."""
+    lib = _lib.load()
+    m = i64(mesh)
+    il = i64(indexlist) if indexlist is not None else None
+    nk = int(np.prod(m)) if indexlist is None else len(indexlist)
+    out = np.zeros((3, 3, nk), order="F")
+    co = np.asfortranarray(sym.crotations_orig)
+    st = lib.elph_symmetrizers(_p(m, c_i64p), sym.nsymm, _p(sym.qrotations), _p(co), nk, _p(il, c_i64p), _p(out))
+    if st != 0:
+        raise ElphError(st, "symmetrizers failed")
+    return out
 
 
 def irred_wedge(mesh, sym: Symmetry, indexlist=None, maps=False):
@@ -107,6 +132,30 @@ class Electrons:
         for k, val in kw.items():
             setattr(obj, k, val)
         return obj
+
+    def transport_inputs(self, wann: Wannier, sym: Symmetry):
+        """What the electron RTA step reads besides the rates, built as
+        calculate_electrons leaves them: the IBZ wedge of the window's list
+        (nequiv, ibz2fbz_map), the symmetrizers (src/electron.f90:462-476) and
+        the velocities (nwv, nb, 3) km/s -- each list point takes its IBZ
+        point's el_wann velocities rotated by the mapping operation and
+        multiplied by its symmetrizer (500-522)."""
+        irr, neq, m = irred_wedge(self.wvmesh, sym, indexlist=self.indexlist, maps=True)
+        if not np.array_equal(irr, self.indexlist_irred):
+            raise ElphError(_lib.ELPH_EWEDGE, "IBZ of the list differs from the electron state's")
+        S = symmetrizers(self.wvmesh, sym, self.indexlist)
+        mesh = self.wvmesh
+        i0 = irr - 1
+        kirr = np.stack([i0 % mesh[0], (i0 // mesh[0]) % mesh[1], i0 // (mesh[0] * mesh[1])], axis=1) / mesh
+        _, _, v_irr = wann.el_wann(kirr, velocities=True, evecs=False)
+        vels = np.zeros((self.nwv, self.nb, 3), order="F")
+        for i in range(len(irr)):
+            il = m[:neq[i], i, 1]
+            rot = m[:neq[i], i, 0] - 1
+            aux = np.searchsorted(self.indexlist, il)
+            v = np.einsum("cdl,bd->lbc", sym.crotations[:, :, rot], v_irr[i])        # (l, nb, 3)
+            vels[aux] = np.einsum("cdl,lbd->lbc", S[:, :, aux], v)
+        return dict(nequiv=neq, ibz2fbz_map=m, symmetrizers=S, vels=vels, crotations=sym.crotations)
 
 
 class Phonons:

@@ -55,7 +55,14 @@ void reciprocal_lattice(const double L[9], double rec[9], double *volume);
 // Symmetry (src/symmetry.f90) and irreducible wedge.
 int find_point_group(const double L[9], const double *basis, const int64_t *types, int64_t natoms, double symprec,
                      std::vector<int> &rot);
-int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_t mesh[3], std::vector<double> &qrot);
+// qrot (and, when crot is given, the Cartesian operations in the same order):
+// the operations and their time-reversed negatives that are compatible with
+// the mesh, duplicates removed (src/symmetry.f90:131-205).
+int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_t mesh[3], std::vector<double> &qrot,
+                    std::vector<double> *crot = nullptr);
+std::vector<double> cartesian_rotations(const double L[9], const std::vector<int> &rot);
+void symmetrizers(const int64_t mesh[3], int64_t nsymm, const double *qrot, const double *crot_orig, int64_t nk,
+                  const int64_t *indexlist, double *out);
 int64_t irred_wedge(const int64_t mesh[3], int64_t nsr, const double *qrot, const int64_t *indexlist, int64_t nlist,
                     int64_t *irr, int64_t *nequiv, int64_t *ibz2fbz);
 

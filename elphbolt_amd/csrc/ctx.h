@@ -29,17 +29,28 @@ template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool owned = true;  // false: a view of memory owned elsewhere (borrow)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         n = 0;
+        owned = true;
+    }
+    // Uses count elements at ptr (not freed by this buffer); a later alloc or
+    // upload of at most count elements writes there without allocating.
+    void borrow(T *ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        owned = false;
     }
     void alloc(size_t count) {
         if (count <= n && p) return;
+        if (!owned) throw Error{ELPH_ENOMEM, "a borrowed device buffer is too small"};
         release();
         if (count == 0) count = 1;
         hipError_t e = hipMalloc(&p, count * sizeof(T));
@@ -65,6 +76,11 @@ struct elph_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
+    // event pairs bracketing the dominant kernel launches of a run (created
+    // with the context; read when full and at the end of the run, so the
+    // host does not wait after each launch)
+    std::vector<hipEvent_t> tev_start, tev_stop;
+    int tev_used = 0;
     int64_t nb = 0, nbr = 0, nwsk = 0, nwsq = 0, nwsg = 0, natoms = 0;
     int64_t coarse_qmesh[3] = {0, 0, 0};
     double lattvecs[9], reclattvecs[9], volume = 0;
@@ -194,5 +210,12 @@ struct IbzkRows {
 };
 // Builds the vmap table on first use (synchronous).
 IbzkRows ibzk_rows(elph_ibzk_plan *p);
+
+// Timing of dominant launches: timed_begin records the start event of the
+// next pair on `st`, timed_end its stop event; timing_flush waits for the
+// recorded pairs and adds their times to t_dominant_ms / n_dominant.
+void timed_begin(elph_ctx *c, hipStream_t st);
+void timed_end(elph_ctx *c, hipStream_t st);
+void timing_flush(elph_ctx *c);
 
 }  // namespace elph

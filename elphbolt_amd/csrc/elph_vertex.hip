@@ -23,9 +23,11 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -2049,6 +2051,25 @@ void elph::require_lds(const elph_ctx *c, size_t bytes, const char *what) {
                                      " (the phase table has lcm(mesh) entries: use a mesh with a smaller lcm)"};
 }
 
+void elph::timing_flush(elph_ctx *c) {
+    if (c->tev_used == 0) return;
+    HCHECK(hipEventSynchronize(c->tev_stop[c->tev_used - 1]));
+    for (int i = 0; i < c->tev_used; ++i) {
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->tev_start[i], c->tev_stop[i]));
+        c->t_dominant_ms += ms;
+        c->n_dominant += 1;
+    }
+    c->tev_used = 0;
+}
+
+void elph::timed_begin(elph_ctx *c, hipStream_t st) {
+    if (c->tev_used == (int)c->tev_start.size()) timing_flush(c);
+    HCHECK(hipEventRecord(c->tev_start[c->tev_used], st));
+}
+
+void elph::timed_end(elph_ctx *c, hipStream_t st) { HCHECK(hipEventRecord(c->tev_stop[c->tev_used++], st)); }
+
 int elph::ret_error(elph_ctx *ctx, const Error &e) {
     if (ctx) ctx->err = e.msg;
     return e.code;
@@ -2270,6 +2291,12 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HCHECK(hipEventCreate(&c->ev_a));
     HCHECK(hipEventCreate(&c->ev_b));
+    c->tev_start.assign(64, nullptr);
+    c->tev_stop.assign(64, nullptr);
+    for (int i = 0; i < 64; ++i) {
+        HCHECK(hipEventCreate(&c->tev_start[i]));
+        HCHECK(hipEventCreate(&c->tev_stop[i]));
+    }
     HCHECK(hipEventCreate(&c->ev_c));
     HCHECK(hipEventCreate(&c->ev_d));
     reciprocal_lattice(c->lattvecs, c->reclattvecs, &c->volume);
@@ -2620,6 +2647,10 @@ int elph_ctx_destroy(elph_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->ev_a) (void)hipEventDestroy(ctx->ev_a);
     if (ctx->ev_b) (void)hipEventDestroy(ctx->ev_b);
+    for (hipEvent_t e : ctx->tev_start)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->tev_stop)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev_c) (void)hipEventDestroy(ctx->ev_c);
     if (ctx->ev_d) (void)hipEventDestroy(ctx->ev_d);
     hipStream_t s = ctx->stream;
@@ -3074,6 +3105,7 @@ struct elph_ibzk_plan {
     // (FP64) 64*grb rows per block
     bool general = false;
     int gcb = 3, grb = 2;
+    int64_t y_bytes_cap = 0;  // general path: bytes of Y per chunk of states (0: 512 MiB)
     int64_t CPf = 0;  // column count of the fused layout (16-rounded nb*nbr)
     ConsumerScratch cons;
 };
@@ -3099,7 +3131,8 @@ void launch_ibzk_y_i8(hipStream_t s, const PairArgs &a, const elph_ibzk_plan *p,
 
 int64_t general_chunk(const elph_ibzk_plan *p, int64_t nst) {
     const int64_t per_state = p->nwv * p->CP * (int64_t)sizeof(double2);
-    return std::max<int64_t>(1, std::min<int64_t>(nst, ((int64_t)1 << 29) / per_state));
+    const int64_t cap = p->y_bytes_cap > 0 ? p->y_bytes_cap : ((int64_t)1 << 29);
+    return std::max<int64_t>(1, std::min<int64_t>(nst, cap / per_state));
 }
 
 // General path: Y of a chunk of states into HBM, then the epilogue.
@@ -3170,8 +3203,9 @@ size_t ibzk_pair_lds(const elph_ibzk_plan *p) {
     return std::max(y, e);
 }
 
-// Pair stage of one group on stream s; with `timed`, HIP events bracket it on
-// s and the time is added to the context's dominant-kernel total (synchronous).
+// Pair stage of one group on stream s; with `timed`, a pair of the context's
+// timing events brackets it on s (timing_flush adds the times to the
+// dominant-kernel total; nothing here waits).
 void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
     elph_ctx *c = p->ctx;
     const int64_t nst_batch = gr.nst, nprocs = p->nprocs;
@@ -3209,21 +3243,14 @@ void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStr
     a.out = out;
     a.nprocs = nprocs;
     const int CB = (int)(p->CP / 16);
-    if (timed) HCHECK(hipEventRecord(c->ev_a, s));
+    if (timed) timed_begin(c, s);
     if (p->general) {
         general_launch(p, a, nst_batch, sc, s);
     } else {
         fused_launch(p, a, nst_batch, sc, s, CB);
     }
     HCHECK(hipGetLastError());
-    if (timed) {
-        HCHECK(hipEventRecord(c->ev_b, s));
-        HCHECK(hipEventSynchronize(c->ev_b));
-        float ms = 0;
-        HCHECK(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
-        c->t_dominant_ms += ms;
-        c->n_dominant += 1;
-    }
+    if (timed) timed_end(c, s);
 }
 
 // Epilogue variant of the fused kernels: 1 for nb <= 8 and nbr <= 6, 2 for
@@ -3293,12 +3320,23 @@ void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkS
     }
 }
 
+// Host side of a group of IBZ k and their in-window states (the contents of
+// an IbzkGroup's device tables).
+struct GroupHost {
+    int64_t nkb = 0, nst = 0;
+    std::vector<int64_t> slot;
+    std::vector<double> kv;
+    std::vector<double2> Ukb;
+    std::vector<int> skb, sm, ski;
+    std::vector<int64_t> sslot;
+};
+
 // Active states of first..last (counts -1 for out-of-window states) grouped by
-// up to 64 IBZ k, with their device tables; st_slot is the output row: relative
-// to `first` when on_device (or, compact, the index among active states), else
-// the row within the group.
-void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device, int64_t *counts, hipStream_t s,
-                 std::vector<std::unique_ptr<IbzkGroup>> &groups, bool compact = false) {
+// up to KB IBZ k; st_slot is the output row: relative to `first` when
+// on_device (or, compact, the index among active states), else the row
+// within the group.
+std::vector<GroupHost> ibzk_group_hosts(const elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device,
+                                        int64_t *counts, bool compact, int64_t KB) {
     const int64_t nb = p->nb, nb2 = nb * nb;
     struct St { int64_t ik, m, slot; };
     std::vector<St> act;
@@ -3311,7 +3349,7 @@ void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device,
         counts[t] = p->nprocs;
         act.push_back({ik - 1, m - 1, compact ? (int64_t)act.size() : t});
     }
-    const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
+    std::vector<GroupHost> out;
     size_t i0 = 0;
     while (i0 < act.size()) {
         std::vector<int64_t> ks;
@@ -3323,54 +3361,82 @@ void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device,
             }
             ++i1;
         }
-        auto g = std::make_unique<IbzkGroup>();
+        GroupHost g;
         const int64_t nkb = (int64_t)ks.size(), nst = (int64_t)(i1 - i0);
-        g->nkb = nkb;
-        g->nst = nst;
-        std::vector<double> kv(3 * nkb);
-        std::vector<double2> Ukb(nkb * nb2);
+        g.nkb = nkb;
+        g.nst = nst;
+        g.kv.resize(3 * nkb);
+        g.Ukb.resize(nkb * nb2);
         for (int64_t b = 0; b < nkb; ++b) {
-            for (int d = 0; d < 3; ++d) kv[b + nkb * d] = (double)p->kint_irr[3 * ks[b] + d] / (double)p->Nel[d];
-            std::memcpy(&Ukb[b * nb2], &p->Uk_host[ks[b] * nb2], nb2 * sizeof(double2));
+            for (int d = 0; d < 3; ++d) g.kv[b + nkb * d] = (double)p->kint_irr[3 * ks[b] + d] / (double)p->Nel[d];
+            std::memcpy(&g.Ukb[b * nb2], &p->Uk_host[ks[b] * nb2], nb2 * sizeof(double2));
         }
-        std::vector<int> skb(nst), sm(nst), ski(3 * nst);
-        std::vector<int64_t> sslot(nst);
+        g.skb.resize(nst);
+        g.sm.resize(nst);
+        g.ski.resize(3 * nst);
+        g.sslot.resize(nst);
         int64_t bidx = -1, prev = -1;
         for (int64_t j = 0; j < nst; ++j) {
             const St &a = act[i0 + j];
             if (a.ik != prev) { ++bidx; prev = a.ik; }
-            skb[j] = (int)bidx;
-            sm[j] = (int)a.m;
-            for (int d = 0; d < 3; ++d) ski[3 * j + d] = p->kint_irr[3 * a.ik + d];
-            sslot[j] = on_device ? a.slot : j;
-            g->slot.push_back(a.slot);
+            g.skb[j] = (int)bidx;
+            g.sm[j] = (int)a.m;
+            for (int d = 0; d < 3; ++d) g.ski[3 * j + d] = p->kint_irr[3 * a.ik + d];
+            g.sslot[j] = on_device ? a.slot : j;
+            g.slot.push_back(a.slot);
         }
-        g->kv.upload(kv, s);
-        g->Ukb.upload(Ukb, s);
-        g->st_kb.upload(skb, s);
-        g->st_m.upload(sm, s);
-        g->st_kint.upload(ski, s);
-        g->st_slot.upload(sslot, s);
-        HCHECK(hipStreamSynchronize(s));  // the host vectors go out of scope
-        groups.push_back(std::move(g));
+        out.push_back(std::move(g));
         i0 = i1;
+    }
+    return out;
+}
+
+// Device tables of a group (buffers of `g` that are borrowed and large enough
+// are written in place); returns after the copies.
+void ibzk_group_upload(const GroupHost &h, IbzkGroup &g, hipStream_t s) {
+    g.nkb = h.nkb;
+    g.nst = h.nst;
+    g.slot = h.slot;
+    g.kv.upload(h.kv, s);
+    g.Ukb.upload(h.Ukb, s);
+    g.st_kb.upload(h.skb, s);
+    g.st_m.upload(h.sm, s);
+    g.st_kint.upload(h.ski, s);
+    g.st_slot.upload(h.sslot, s);
+    HCHECK(hipStreamSynchronize(s));  // the host vectors may go out of scope
+}
+
+void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device, int64_t *counts, hipStream_t s,
+                 std::vector<std::unique_ptr<IbzkGroup>> &groups, bool compact = false) {
+    const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
+    for (const GroupHost &h : ibzk_group_hosts(p, first, last, on_device, counts, compact, KB)) {
+        auto g = std::make_unique<IbzkGroup>();
+        ibzk_group_upload(h, *g, s);
+        groups.push_back(std::move(g));
     }
 }
 
-// Scratch for groups of up to nkb IBZ k and nst states.
-void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t nst) {
+// Scratch for groups of up to nkb IBZ k and nst states; get(buffer, count)
+// provides each buffer (allocates, or carves from a fixed region).
+template <typename Get>
+void ibzk_scratch(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t nst, Get &&get) {
     elph_ctx *c = p->ctx;
     const int64_t nb2 = p->nb * p->nb, nslices = p->nbr * c->nwsg;
-    sc.gk.alloc(nkb * nb2 * nslices);
-    sc.h.alloc(nst * p->Kp * p->CP);
-    sc.phiT.alloc(gk_phi_count(c));
+    get(sc.gk, nkb * nb2 * nslices);
+    get(sc.h, nst * p->Kp * p->CP);
+    get(sc.phiT, gk_phi_count(c));
     if (p->precision == ELPH_MIXED) {
-        sc.bphi.alloc(gk_bphi_count(c));
-        sc.bscale.alloc(nst * p->CP);
-        sc.bsl.alloc(nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
+        get(sc.bphi, gk_bphi_count(c));
+        get(sc.bscale, nst * p->CP);
+        get(sc.bsl, nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
     }
-    if (p->general) sc.Y.alloc(general_chunk(p, nst) * p->nwv * p->CP);
+    if (p->general) get(sc.Y, general_chunk(p, nst) * p->nwv * p->CP);
 }
+
+void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t nst) {
+    ibzk_scratch(p, sc, nkb, nst, [](auto &buf, int64_t n) { buf.alloc((size_t)n); });
+}
+
 
 // One group: g(k,Rp) of its k, h = U_k g / phwsdeg, (mixed) digits, pair stage.
 void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
@@ -3570,6 +3636,9 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             HCHECK(hipGetLastError());
         }
         require_lds(ctx, ibzk_pair_lds(p.get()), "the ibzk pair kernel");
+        // the int8 digits of gwann, once per context, so that runs allocate no
+        // context data
+        if (precision == ELPH_MIXED) ensure_gwann_digits(ctx);
         // IBZ data
         p->kint_irr.resize(3 * p->nirr);
         p->ens_irred.assign(el->ens_irred, el->ens_irred + p->nirr * nb);
@@ -3609,6 +3678,7 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
         hipStream_t s = c->stream;
         c->t_dominant_ms = 0;
         c->n_dominant = 0;
+        c->tev_used = 0;
         std::vector<std::unique_ptr<IbzkGroup>> groups;
         ibzk_groups(p, first, last, on_device != 0, counts, s, groups);
         int64_t nkb = 1, nst = 1;
@@ -3634,7 +3704,9 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
         float ms = 0;
         HCHECK(hipEventElapsedTime(&ms, c->ev_c, c->ev_d));
         c->t_all_ms = ms;
+        timing_flush(c);
     } catch (const Error &e) {
+        c->tev_used = 0;
         return ret_error(c, e);
     }
     return ELPH_OK;

@@ -141,7 +141,8 @@ static int64_t star_image(const double *qr, const int64_t ijk[3], const int64_t 
 // src/symmetry.f90:134-211: q-space rotations G R G^-1 (G = L^T L), time
 // reversal appended, mesh-incompatible rotations and duplicates of the
 // real-space matrices dropped.
-int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_t mesh[3], std::vector<double> &qrot) {
+int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_t mesh[3], std::vector<double> &qrot,
+                    std::vector<double> *crot) {
     const int64_t ns = (int64_t)(rot.size() / 9), n2 = 2 * ns;
     double G[9], Gi[9];
     for (int i = 0; i < 3; ++i)
@@ -186,7 +187,66 @@ int64_t q_rotations(const double L[9], const std::vector<int> &rot, const int64_
     qrot.clear();
     for (int64_t s = 0; s < n2; ++s)
         if (valid[s]) qrot.insert(qrot.end(), &qr[9 * s], &qr[9 * s] + 9);
+    if (crot) {
+        // the Cartesian operations in the same order (src/symmetry.f90:131-137,
+        // 154-159, 191-205): L W L^-1 for the first half, its negative for the
+        // time-reversed half
+        const std::vector<double> c = cartesian_rotations(L, rot);
+        crot->clear();
+        for (int64_t s = 0; s < n2; ++s) {
+            if (!valid[s]) continue;
+            for (int i = 0; i < 9; ++i) crot->push_back(s < ns ? c[9 * s + i] : -c[9 * (s - ns) + i]);
+        }
+    }
     return (int64_t)(qrot.size() / 9);
+}
+
+// src/symmetry.f90:131-137: the Cartesian form C = L W L^-1 of each operation
+// W (fractional coordinates, x' = W x + w), column-major 3x3 blocks.
+std::vector<double> cartesian_rotations(const double L[9], const std::vector<int> &rot) {
+    const int64_t ns = (int64_t)(rot.size() / 9);
+    double Li[9];
+    inv3(L, Li);
+    std::vector<double> out(9 * ns);
+    for (int64_t s = 0; s < ns; ++s) {
+        double LW[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0;
+                for (int a = 0; a < 3; ++a) v += L[i + 3 * a] * rot[9 * s + a + 3 * j];
+                LW[i + 3 * j] = v;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double v = 0;
+                for (int a = 0; a < 3; ++a) v += LW[i + 3 * a] * Li[a + 3 * j];
+                out[9 * s + i + 3 * j] = v;
+            }
+    }
+    return out;
+}
+
+// src/electron.f90:462-476 (and phonon.f90:245-264): the symmetrizer of list
+// point i is the mean of the Cartesian operations crot_orig[jj], jj < nsymm,
+// whose reciprocal operation qrot[jj] maps the point onto itself (ShengBTE
+// style; the reference pairs the first nsymm entries of the filtered
+// reciprocal list with the unfiltered Cartesian list, as done here).  Points
+// whose only such operation is one keep it undivided.
+void symmetrizers(const int64_t mesh[3], int64_t nsymm, const double *qrot, const double *crot_orig, int64_t nk,
+                  const int64_t *indexlist, double *out) {
+    for (int64_t i = 0; i < nk; ++i) {
+        const int64_t ii = indexlist ? indexlist[i] : i + 1;
+        int64_t ijk[3];
+        demux_vector0(ii, ijk, mesh);
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int64_t kk = 0;
+        for (int64_t jj = 0; jj < nsymm; ++jj) {
+            if (star_image(&qrot[9 * jj], ijk, mesh) != ii) continue;
+            for (int e = 0; e < 9; ++e) acc[e] += crot_orig[9 * jj + e];
+            ++kk;
+        }
+        for (int e = 0; e < 9; ++e) out[9 * i + e] = kk > 1 ? acc[e] / (double)kk : acc[e];
+    }
 }
 
 // src/symmetry.f90:303-451 find_irred_wedge.  A point starts a new orbit when
@@ -577,6 +637,30 @@ extern "C" int elph_symmetry(const double *lattvecs, int64_t natoms, const doubl
     *nsymm = ns;
     *nsymm_rot = nsr;
     std::memcpy(qrotations, qr.data(), sizeof(double) * qr.size());
+    return ELPH_OK;
+}
+
+extern "C" int elph_symmetry_cartesian(const double *lattvecs, int64_t natoms, const double *basis,
+                                       const int64_t *atomtypes, double symprec, const int64_t *mesh, int64_t *nsymm,
+                                       int64_t *nsymm_rot, double *crotations_orig, double *crotations) {
+    std::vector<int> rot;
+    int ns = elph::find_point_group(lattvecs, basis, atomtypes, natoms, symprec, rot);
+    if (ns <= 0) return ELPH_EINVAL;
+    std::vector<double> qr, cr;
+    int64_t nsr = elph::q_rotations(lattvecs, rot, mesh, qr, &cr);
+    const std::vector<double> co = elph::cartesian_rotations(lattvecs, rot);
+    *nsymm = ns;
+    *nsymm_rot = nsr;
+    if (crotations_orig) std::memcpy(crotations_orig, co.data(), sizeof(double) * co.size());
+    if (crotations) std::memcpy(crotations, cr.data(), sizeof(double) * cr.size());
+    return ELPH_OK;
+}
+
+extern "C" int elph_symmetrizers(const int64_t *mesh, int64_t nsymm, const double *qrotations,
+                                 const double *crotations_orig, int64_t nk, const int64_t *indexlist,
+                                 double *symmetrizers) {
+    if (!mesh || !qrotations || !crotations_orig || !symmetrizers || nk < 0 || nsymm <= 0) return ELPH_EINVAL;
+    elph::symmetrizers(mesh, nsymm, qrotations, crotations_orig, nk, indexlist, symmetrizers);
     return ELPH_OK;
 }
 

@@ -265,6 +265,22 @@ int elph_write_X_files(const char *dir, int64_t istate_first, int64_t nstates, c
 int elph_symmetry(const double *lattvecs, int64_t natoms, const double *basis, const int64_t *atomtypes,
                   double symprec, const int64_t *mesh, int64_t *nsymm, int64_t *nsymm_rot,
                   double *qrotations /* (3,3,96) */);
+/* The Cartesian operations of elph_symmetry (src/symmetry.f90:131-137,
+ * 154-159, 191-205): crotations_orig (3,3,nsymm) = L W L^-1 of each point
+ * operation W (`sym%crotations_orig`), crotations (3,3,nsymm_rot) in the order
+ * of elph_symmetry's qrotations (`sym%crotations`, which the transport tensors
+ * are symmetrized with).  Either output may be NULL. */
+int elph_symmetry_cartesian(const double *lattvecs, int64_t natoms, const double *basis, const int64_t *atomtypes,
+                            double symprec, const int64_t *mesh, int64_t *nsymm, int64_t *nsymm_rot,
+                            double *crotations_orig, double *crotations);
+/* The symmetrizers of wave-vector dependent vectors (src/electron.f90:
+ * 462-476): for each point of indexlist (nk muxed 1-based, NULL = the whole
+ * mesh), the mean of crotations_orig(:,:,jj), jj = 1..nsymm, over the
+ * operations whose qrotations(:,:,jj) map the point onto itself; out
+ * (3,3,nk).  This is `el%symmetrizers`, which the field terms are multiplied
+ * by (src/bte.f90:242-246, 262-266; elph_el_transport_create). */
+int elph_symmetrizers(const int64_t *mesh, int64_t nsymm, const double *qrotations, const double *crotations_orig,
+                      int64_t nk, const int64_t *indexlist, double *symmetrizers);
 int elph_irred_wedge(const int64_t *mesh, int64_t nsymm_rot, const double *qrotations, int64_t nlist,
                      const int64_t *indexlist /* NULL = full mesh */, int64_t *nirr,
                      int64_t *indexlist_irred /* (nlist) */, int64_t *nequiv /* (nlist) */,

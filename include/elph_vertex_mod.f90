@@ -342,6 +342,25 @@ module elph_vertex_mod
        real(c_double), value :: symprec
      end function elph_symmetry
 
+     integer(c_int) function elph_symmetry_cartesian(lattvecs, natoms, basis, atomtypes, symprec, mesh, nsymm, &
+          nsymm_rot, crotations_orig, crotations) bind(C, name="elph_symmetry_cartesian")
+       import
+       real(c_double) :: lattvecs(3, 3), basis(3, *)
+       integer(c_int64_t), value :: natoms
+       integer(c_int64_t) :: atomtypes(*), mesh(3), nsymm, nsymm_rot
+       real(c_double), value :: symprec
+       type(c_ptr), value :: crotations_orig, crotations
+     end function elph_symmetry_cartesian
+
+     integer(c_int) function elph_symmetrizers(mesh, nsymm, qrotations, crotations_orig, nk, indexlist, &
+          symmetrizers) bind(C, name="elph_symmetrizers")
+       import
+       integer(c_int64_t) :: mesh(3)
+       integer(c_int64_t), value :: nsymm, nk
+       real(c_double) :: qrotations(3, 3, *), crotations_orig(3, 3, *), symmetrizers(3, 3, nk)
+       type(c_ptr), value :: indexlist
+     end function elph_symmetrizers
+
      integer(c_int) function elph_irred_wedge(mesh, nsymm_rot, qrotations, nlist, indexlist, nirr, indexlist_irred, &
           nequiv, ibz2fbz) bind(C, name="elph_irred_wedge")
        import

@@ -766,7 +766,8 @@ static void inv3(const double a[9], double r[9]) {
 /* symmetry.f90:134-211: qrotations = G R G^-1 (G = L^T L), doubled by time
  * reversal, then rotations incompatible with the mesh and duplicates of the
  * real-space rotations are dropped.  Returns nsymm_rot. */
-int64_t orc_qrotations(const double L[9], const int32_t *rot, int64_t nsymm, const int64_t mesh[3], double *qrot_out) {
+int64_t orc_qrotations(const double L[9], const int32_t *rot, int64_t nsymm, const int64_t mesh[3], double *qrot_out,
+                       double *crot_out) {
     double G[9], Gi[9];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
@@ -812,6 +813,18 @@ int64_t orc_qrotations(const double L[9], const int32_t *rot, int64_t nsymm, con
     int64_t k = 0;
     for (int64_t s = 0; s < n2; ++s)
         if (valid[s]) { memcpy(&qrot_out[9 * k], &qr[9 * s], 9 * sizeof(double)); k++; }
+    if (crot_out) {
+        /* symmetry.f90:138, 159, 191-205: crotations in the same filtered order */
+        double *c = malloc(sizeof(double) * 9 * nsymm);
+        orc_cartesian_rotations(L, rot, nsymm, c);
+        int64_t kc = 0;
+        for (int64_t s = 0; s < n2; ++s) {
+            if (!valid[s]) continue;
+            for (int e = 0; e < 9; ++e) crot_out[9 * kc + e] = s < nsymm ? c[9 * s + e] : -c[9 * (s - nsymm) + e];
+            kc++;
+        }
+        free(c);
+    }
     free(qr); free(rr); free(emap); free(valid);
     return k;
 }
@@ -1129,6 +1142,78 @@ void orc_g2_list(const orc_wannier *w, int64_t n, const double *kvecs, const dou
         out[i] = orc_g2(w, k, q, &el_evec_k[i], n, &el_evec_kp[i], n, &ph_evec_q[i], n, ph_en[i],
                         &gmixed[gidx[i] * gsz], wannspace_ph);
     }
+}
+
+/* symmetry.f90:131-137: tmp1 = transpose(lattvecs), tmp2 = transpose(lattvecs
+ * . rotations_orig(:,:,i)), dgesv(tmp1, tmp2), crotations_orig(:,:,i) =
+ * transpose(tmp2).  rot (3,3,nsymm) are the operations on fractional
+ * coordinates (rotations_orig), column-major. */
+static void solve3(const double A_in[9], double B[9]) {
+    /* B <- A^-1 B by Gaussian elimination with partial pivoting (dgesv) */
+    double A[9];
+    memcpy(A, A_in, sizeof(A));
+    for (int c = 0; c < 3; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 3; ++r)
+            if (fabs(A[r + 3 * c]) > fabs(A[piv + 3 * c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < 3; ++j) {
+                double t = A[c + 3 * j]; A[c + 3 * j] = A[piv + 3 * j]; A[piv + 3 * j] = t;
+                t = B[c + 3 * j]; B[c + 3 * j] = B[piv + 3 * j]; B[piv + 3 * j] = t;
+            }
+        for (int r = c + 1; r < 3; ++r) {
+            double f = A[r + 3 * c] / A[c + 3 * c];
+            for (int j = 0; j < 3; ++j) { A[r + 3 * j] -= f * A[c + 3 * j]; B[r + 3 * j] -= f * B[c + 3 * j]; }
+        }
+    }
+    for (int c = 2; c >= 0; --c)
+        for (int j = 0; j < 3; ++j) {
+            double v = B[c + 3 * j];
+            for (int k = c + 1; k < 3; ++k) v -= A[c + 3 * k] * B[k + 3 * j];
+            B[c + 3 * j] = v / A[c + 3 * c];
+        }
+}
+
+void orc_cartesian_rotations(const double L[9], const int32_t *rot, int64_t nsymm, double *crot_out) {
+    for (int64_t s = 0; s < nsymm; ++s) {
+        double tmp1[9], LW[9], tmp2[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                tmp1[i + 3 * j] = L[j + 3 * i];
+                double v = 0;
+                for (int a = 0; a < 3; ++a) v += L[i + 3 * a] * rot[9 * s + a + 3 * j];
+                LW[i + 3 * j] = v;
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) tmp2[i + 3 * j] = LW[j + 3 * i];
+        solve3(tmp1, tmp2);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) crot_out[9 * s + i + 3 * j] = tmp2[j + 3 * i];
+    }
+}
+
+/* electron.f90:462-476: symmetrizers(:,:,i) = sum of crotations_orig(:,:,jj)
+ * over jj = 1..nsymm with equiv_map(jj, i) == indexlist(i), divided by the
+ * count when it exceeds 1; equiv_map from find_equiv_map over the list with
+ * the (filtered) qrotations. */
+void orc_symmetrizers(const int64_t mesh[3], int64_t nsymm_rot, const double *qrot, int64_t nsymm,
+                      const double *crot_orig, int64_t nk, const int64_t *indexlist, double *out) {
+    int64_t *emap = malloc(sizeof(int64_t) * nsymm_rot * nk);
+    orc_find_equiv_map(nsymm_rot, emap, mesh, qrot, indexlist, nk);
+    for (int64_t i = 0; i < nk; ++i) {
+        int64_t ii = indexlist ? indexlist[i] : i + 1;
+        double *S = &out[9 * i];
+        for (int e = 0; e < 9; ++e) S[e] = 0.0;
+        int64_t kk = 0;
+        for (int64_t jj = 0; jj < nsymm; ++jj)
+            if (emap[jj + nsymm_rot * i] == ii) {
+                for (int e = 0; e < 9; ++e) S[e] += crot_orig[9 * jj + e];
+                kk++;
+            }
+        if (kk > 1)
+            for (int e = 0; e < 9; ++e) S[e] /= (double)kk;
+    }
+    free(emap);
 }
 
 /* interactions.f90:1673-1930 calculate_eph_interaction_ibzq, key 'g'.

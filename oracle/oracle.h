@@ -76,13 +76,19 @@ void orc_gReq(const orc_wannier *w, const double qvec[3], zdouble *gmixed);
 /* ---- symmetry.f90 ---- */
 int orc_find_rotations(const double lattvecs[9], const double *basis, const int64_t *types, int64_t natoms,
                        double symprec, int32_t *rot_out /* (3,3,48) */);
+/* qrot_out (3,3,2*nsymm); crot_out (the Cartesian operations in the same
+ * filtered order) may be NULL */
 int64_t orc_qrotations(const double lattvecs[9], const int32_t *rot, int64_t nsymm, const int64_t mesh[3],
-                       double *qrot_out /* (3,3,2*nsymm) */);
+                       double *qrot_out, double *crot_out);
 void orc_find_equiv_map(int64_t nsymm_rot, int64_t *equiv_map, const int64_t mesh[3], const double *qrot,
                         const int64_t *indexlist, int64_t nlist);
 int64_t orc_find_irred_wedge(const int64_t mesh[3], int64_t nsymm_rot, const double *qrot,
                              const int64_t *indexlist, int64_t nlist, int64_t *indexlist_irred,
                              int64_t *nequiv, int64_t *ibz2fbz /* (nsymm_rot, nlist, 2) */);
+
+void orc_cartesian_rotations(const double L[9], const int32_t *rot, int64_t nsymm, double *crot_out);
+void orc_symmetrizers(const int64_t mesh[3], int64_t nsymm_rot, const double *qrot, int64_t nsymm,
+                      const double *crot_orig, int64_t nk, const int64_t *indexlist, double *out);
 
 /* ---- electron.f90 calculate_electrons (vertex-relevant subset) ---- */
 typedef struct {

@@ -228,8 +228,35 @@ def qrotations(mat, mesh):
     q = np.zeros(9 * 2 * rot.shape[0])
     r = np.ascontiguousarray(rot.ravel(), np.int32)
     mesh = _i64(mesh)
-    n = lib().orc_qrotations(_p(L), _p(r, c_i32p), ct.c_int64(rot.shape[0]), _p(mesh, c_i64p), _p(q))
+    n = lib().orc_qrotations(_p(L), _p(r, c_i32p), ct.c_int64(rot.shape[0]), _p(mesh, c_i64p), _p(q), None)
     return q[:9 * n].copy(), int(n), int(rot.shape[0])
+
+
+def cartesian_rotations(mat, mesh):
+    """(crotations_orig (3,3,nsymm), crotations (3,3,nsymm_rot) in the order of
+    qrotations(mat, mesh)) -- src/symmetry.f90:131-137, 154-159, 191-205."""
+    rot = find_rotations(mat)
+    L = _f64(mat.lattvecs)
+    r = np.ascontiguousarray(rot.ravel(), np.int32)
+    ns = rot.shape[0]
+    co = np.zeros(9 * ns)
+    lib().orc_cartesian_rotations(_p(L), _p(r, c_i32p), ct.c_int64(ns), _p(co))
+    q, c = np.zeros(9 * 2 * ns), np.zeros(9 * 2 * ns)
+    n = lib().orc_qrotations(_p(L), _p(r, c_i32p), ct.c_int64(ns), _p(_i64(mesh), c_i64p), _p(q), _p(c))
+    return (co.reshape((3, 3, ns), order="F").copy(order="F"),
+            c[:9 * n].reshape((3, 3, n), order="F").copy(order="F"))
+
+
+def symmetrizers(mesh, qrot, nsr, nsymm, crot_orig, indexlist=None):
+    """el%symmetrizers (src/electron.f90:462-476), (3, 3, nk)."""
+    mesh = _i64(mesh)
+    il = _i64(indexlist) if indexlist is not None else None
+    nk = int(np.prod(mesh)) if indexlist is None else len(indexlist)
+    out = np.zeros((3, 3, nk), order="F")
+    co = _f64(crot_orig)
+    lib().orc_symmetrizers(_p(mesh, c_i64p), ct.c_int64(nsr), _p(qrot), ct.c_int64(nsymm), _p(co), ct.c_int64(nk),
+                           _p(il, c_i64p) if il is not None else None, _p(out))
+    return out
 
 
 def irred_wedge(mesh, qrot, nsr, indexlist=None, maps=False):

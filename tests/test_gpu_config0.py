@@ -2,12 +2,14 @@
 mesh, window +-0.5 eV) as a parity case: every IBZ state's |g|^2 against the
 oracle (FP64 1e-12, mixed 1e-5, relative above 1e-6 * max; counts exact), and
 the transport-delta proxy over the whole chain -- g2 -> X+/- (tetrahedra) ->
-RTA rates -> sigma, sigma*S and S -- against the oracle's chain (FP64 1e-8,
-mixed 1e-4)."""
+RTA rates -> field terms multiplied by the symmetrizers -> sigma, sigma*S and
+S symmetrized over the Cartesian operations -- against the oracle's chain
+(FP64 1e-8, mixed 1e-4); each side builds its own symmetrizers and
+operations (src/electron.f90:462-476, src/symmetry.f90:131-137)."""
 import numpy as np
 import pytest
 
-from elphbolt_amd import interactions, synth
+from elphbolt_amd import bz, interactions, synth
 from elphbolt_amd.bte import ElectronRTA
 from elphbolt_amd.wannier import DeviceBuffer, Wannier
 from oracle import oracle as O
@@ -27,7 +29,7 @@ def _rel(x, ref, floor=1e-6):
 def cfg0():
     mat = synth.config(0)
     o = O.Wannier(mat)
-    qr, nsr, _ = O.qrotations(mat, mat.wvmesh)
+    qr, nsr, nsymm = O.qrotations(mat, mat.wvmesh)
     el = O.Electrons(o, qr, nsr, mat.wvmesh, mat.enref, mat.fsthick)
     ph = O.Phonons(o, qr, nsr, mat.phmesh)
     nst = el.nwv_irred * mat.nb
@@ -40,13 +42,25 @@ def cfg0():
     mesh = np.array(el.wvmesh)
     irr, neq, m = O.irred_wedge(mesh, qr, nsr, indexlist=el.indexlist, maps=True)
     assert np.array_equal(irr, el.indexlist_irred)
-    kv = np.array([O.demux_vector(i, mesh, 0) for i in el.indexlist], float) / mesh
-    _, _, vels = o.el_wann(kv)
+    # velocities as calculate_electrons leaves them (src/electron.f90:500-522):
+    # the IBZ point's, rotated by the mapping operation, then symmetrized
+    co, cr = O.cartesian_rotations(mat, mesh)
+    S = O.symmetrizers(mesh, qr, nsr, nsymm, co, indexlist=el.indexlist)
+    kirr = np.array([O.demux_vector(i, mesh, 0) for i in el.indexlist_irred], float) / mesh
+    _, _, v_irr = o.el_wann(kirr)
+    vels = np.zeros((el.nwv, mat.nb, 3), order="F")
+    for i in range(len(irr)):
+        for l in range(neq[i]):
+            aux = O.binsearch(el.indexlist, int(m[l, i, 1])) - 1
+            v = (cr[:, :, m[l, i, 0] - 1] @ v_irr[i].T).T
+            vels[aux] = (S[:, :, aux] @ v.T).T
     vol = abs(np.linalg.det(np.asarray(mat.lattvecs)))
     ref = {}
     for field in ("E", "T"):
-        F = O.field_term(field, neq, m, T_K, mu, el.ens, vels, rates, el.indexlist)
-        hc, cc = O.transport_coeff(field, T_K, 2, mu, el.ens, vels, vol, mesh, F)
+        # field term symmetrized (src/bte.f90:242-246, 262-266), tensors
+        # symmetrized over the Cartesian operations (src/bz_sums.f90:590-609)
+        F = O.symmetrize_field_term(S, O.field_term(field, neq, m, T_K, mu, el.ens, vels, rates, el.indexlist))
+        hc, cc = O.transport_coeff(field, T_K, 2, mu, el.ens, vels, vol, mesh, F, crot=cr)
         if field == "E":
             ref["sigma"] = cc
         else:
@@ -70,7 +84,11 @@ def test_config0_g2_and_transport(cfg0, precision):
     assert err < (1e-12 if precision == "fp64" else 1e-5)
     tet = interactions.Tetrahedra(w, el.wvmesh, el.ens, indexlist=el.indexlist)
     x = plan.run_X(tet, T_K, cfg0["mu"], 1, nst, g2d, maps=False, probabilities=False)
-    tr = ElectronRTA(w, el.ens, cfg0["vels"], el.indexlist, cfg0["neq"], cfg0["m"], cfg0["vol"], el.wvmesh, 2)
+    # the product builds its own symmetrizers and Cartesian operations
+    sym = bz.Symmetry(mat.lattvecs, mat.basis, mat.masses, mat.wvmesh)
+    S = bz.symmetrizers(mat.wvmesh, sym, el.indexlist)
+    tr = ElectronRTA(w, el.ens, cfg0["vels"], el.indexlist, cfg0["neq"], cfg0["m"], cfg0["vol"], el.wvmesh, 2,
+                     symmetrizers=S, crotations=sym.crotations)
     got = tr.rta(x["rates_device"], T_K, cfg0["mu"])
     sig, sig_r = ElectronRTA.scalar(got["sigma"]), ElectronRTA.scalar(cfg0["ref"]["sigma"])
     ss, ss_r = ElectronRTA.scalar(got["sigmaS"]), ElectronRTA.scalar(cfg0["ref"]["sigmaS"])

@@ -1,0 +1,119 @@
+"""Why the mixed path uses signed int8 digits with exact accumulation rather
+than split-BF16 products with FP32 accumulation: both are emulated here on
+the benchmark's own data (the 3C-SiC-like material, 617 WS cells, 24^3 mesh)
+for two IBZ states and 4096 final k' each, through the same cell-pair form of
+the Rp contraction and an FP64 band/mode rotation, and compared with the FP64
+contraction on |g|^2 entries above 1e-6 * max|g|^2 (the bound: 1e-5):
+
+* FP32 inputs and FP32 accumulation (the best any split-BF16 scheme with
+  FP32 accumulation can reach): misses the bound;
+* split-BF16, 3 products (A0 B0 + A0 B1 + A1 B0), FP32 accumulation: misses it;
+* 4 signed int8 digits per operand (x ~ sum_s d_s / (127 * 254^s)), the 10
+  digit products with i + j <= 3 summed exactly in integers, FP64
+  recombination with weights 1 / (127^2 254^l) -- the kernel's scheme: meets it.
+"""
+import numpy as np
+import pytest
+
+from elphbolt_amd import synth
+from oracle import oracle as O
+
+BOUND = 1e-5
+N = 24
+
+
+def _bf16(x):
+    """Round-to-nearest-even float32 -> bfloat16 (returned as float32)."""
+    b = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def _digits(x, ns=4):
+    """split_digits of the kernels: d_s = rint(r * 127 * 254^s), r -= d_s / (...)."""
+    r, sc, out = x.copy(), 127.0, []
+    for _ in range(ns):
+        q = np.rint(r * sc)
+        out.append(q.astype(np.int64))
+        r = r - q / sc
+        sc *= 254.0
+    return out
+
+
+def _cell_pairs(cells):
+    lut = {tuple(c): i for i, c in enumerate(cells)}
+    pairs, used = [], set()
+    for i, c in enumerate(map(tuple, cells)):
+        if i in used:
+            continue
+        used.add(i)
+        j = lut.get(tuple(-x for x in c), -1)
+        if j in used:
+            j = -1
+        if j >= 0:
+            used.add(j)
+        pairs.append((i, j))
+    return np.array(pairs)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    mat = synth.config(1)
+    return mat, O.Wannier(mat)
+
+
+def test_int8_digits_meet_the_bound_fp32_accumulation_does_not(cfg1):
+    mat, o = cfg1
+    rng = np.random.default_rng(0)
+    cells = mat.rcells_q[:mat.nwsg]
+    P = _cell_pairs(cells)
+    worst = {"fp32": 0.0, "bf16x3": 0.0, "int8x4": 0.0}
+    for kint in ([1, 2, 3], [5, 0, 7]):
+        k = np.array(kint) / N
+        gk = o.gkRp(k)
+        _, U, _ = o.el_wann(k[None])
+        # h(np + nb sp, Rp) = sum_mp U_k(m, mp) g(k, Rp)(np, mp, sp) / phwsdeg(Rp), band m = 4
+        h = np.einsum("p,apsr->asr", U[0, 4], gk[:, :, :, :mat.nwsg]) / mat.phwsdeg[None, None, :mat.nwsg]
+        h = h.transpose(1, 0, 2).reshape(mat.nb * mat.nbr, -1)
+        hi = h[:, P[:, 0]]
+        hj = np.where(P[:, 1][None, :] >= 0, h[:, np.maximum(P[:, 1], 0)], 0)
+        B = np.empty((2 * len(P), h.shape[0]), complex)
+        B[0::2], B[1::2] = (hi + hj).T, (1j * (hi - hj)).T
+        Br = np.concatenate([B.real, B.imag], axis=1)
+        rows = rng.integers(0, N, (4096, 3))
+        q = (rows - np.array(kint)) % N / N
+        th = 2 * np.pi * (q @ cells[P[:, 0]].T)
+        A = np.empty((len(q), 2 * len(P)))
+        A[:, 0::2], A[:, 1::2] = np.cos(th), np.sin(th)
+        Y = {"ref": A @ Br}
+        A32, B32 = A.astype(np.float32), Br.astype(np.float32)
+        Y["fp32"] = (A32 @ B32).astype(np.float64)
+        A0, B0 = _bf16(A32), _bf16(B32)
+        A1, B1 = _bf16(A32 - A0), _bf16(B32 - B0)
+        Y["bf16x3"] = (A0 @ B0 + A0 @ B1 + A1 @ B0).astype(np.float64)
+        sc = np.abs(Br).max(axis=0)
+        sc[sc == 0] = 1.0
+        dA, dB = _digits(A), _digits(Br / sc)
+        yi = np.zeros_like(Y["ref"])
+        for lev in range(4):
+            acc = sum(dA[i] @ dB[lev - i] for i in range(lev + 1))   # exact int64
+            yi += acc / (127.0 * 127.0 * 254.0 ** lev)
+        Y["int8x4"] = yi * sc
+        _, Ukp, _ = o.el_wann(rows / N)
+        _, u, _ = o.ph_wann(q)
+        ms = np.repeat(mat.masses, 3)
+
+        def g2(y):
+            yc = (y[:, :mat.nb * mat.nbr] + 1j * y[:, mat.nb * mat.nbr:]).reshape(-1, mat.nbr, mat.nb)
+            z = np.einsum("rnp,rsp->rns", Ukp.conj(), yc)
+            return np.abs(np.einsum("rtp,rnp->rnt", u / np.sqrt(ms)[None, None, :], z)) ** 2
+
+        ref = g2(Y["ref"])
+        sel = ref > 1e-6 * ref.max()
+        for name in worst:
+            got = g2(Y[name])
+            worst[name] = max(worst[name], float(np.max(np.abs(got[sel] - ref[sel]) / ref[sel])))
+    print("max rel error on |g|^2 above 1e-6 max:", {k: f"{v:.2e}" for k, v in worst.items()})
+    assert worst["fp32"] > BOUND
+    assert worst["bf16x3"] > BOUND
+    assert worst["int8x4"] < BOUND
