@@ -97,6 +97,12 @@ PROTOTYPES = {
     "elph_write_X_files": (ct.c_int, [ct.c_char_p, ct.c_int64, ct.c_int64, c_i64p, c_dblp, c_dblp, c_i64p, c_i64p,
                                       ct.c_int64]),
     "elph_write_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
+    "elph_g2_writer_create": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64,
+                                         ct.POINTER(ct.c_void_p)]),
+    "elph_g2_writer_submit": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p]),
+    "elph_g2_writer_wait": (ct.c_int, [ct.c_void_p]),
+    "elph_g2_writer_error": (ct.c_char_p, [ct.c_void_p]),
+    "elph_g2_writer_destroy": (ct.c_int, [ct.c_void_p]),
     "elph_read_g2_files": (ct.c_int, [ct.c_char_p, ct.c_char, ct.c_int64, ct.c_int64, c_i64p, c_dblp, ct.c_int64]),
     "elph_symmetry": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p, c_dblp]),
     "elph_symmetry_cartesian": (ct.c_int, [c_dblp, ct.c_int64, c_dblp, c_i64p, ct.c_double, c_i64p, c_i64p, c_i64p,

@@ -124,6 +124,7 @@ struct elph_ctx {
     double pol_alpha = 0, pol_fac = 0;
     std::string err;
     int64_t lds_max = 0;  // dynamic LDS a workgroup may allocate on this device (bytes)
+    int64_t ncu = 0;      // compute units of the device (persistent launches)
     double t_dominant_ms = 0, t_all_ms = 0;
     int64_t n_dominant = 0;
 };

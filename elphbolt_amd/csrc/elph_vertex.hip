@@ -38,6 +38,7 @@ using namespace elph;
 
 namespace {
 
+
 // ---------------------------------------------------------------------------
 // Phases
 // ---------------------------------------------------------------------------
@@ -639,6 +640,93 @@ __global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const dou
     }
 }
 
+// Mixed-precision B operand straight from g(k,Rp), for the fused and general
+// ibzk paths: per (state, 16-column block) the folded h of k_rotate_h (same
+// expressions, so the same values), its column scale as k_colscale takes it
+// and its int8 digits in k_slice_b_ri's layout ([st][kc][cb][Re, Im][digit]
+// [lane]), without h going through memory.  Pass 1 takes the column maxima
+// over all rows, pass 2 recomputes h and writes the digits; thread t always
+// holds column cb 16 + (t & 15).
+__global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npair, int nkc, int CB, int CP,
+                                                       const double2 *__restrict__ gk, int64_t gk_kstride,
+                                                       const int *__restrict__ st_kb, const int *__restrict__ st_m,
+                                                       const double2 *__restrict__ Uk, const int *__restrict__ degq,
+                                                       const int2 *__restrict__ pairs, double *__restrict__ scale,
+                                                       v4i *__restrict__ bsl) {
+    __shared__ double red[256];
+    const int st = blockIdx.x, cb = blockIdx.y, t = threadIdx.x;
+    const int col = cb * 16 + (t & 15);
+    const bool colok = col < nb * nbr;
+    const int np = colok ? col % nb : 0, sp = colok ? col / nb : 0;
+    const int kb = st_kb[st], m = st_m[st];
+    const double2 *u = Uk + ((size_t)kb * nb + m) * nb;
+    auto rot = [&](int rp) {
+        const double2 *g = gk + (size_t)kb * gk_kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int mp = 0; mp < nb; ++mp) cfma(acc, u[mp], g[(size_t)mp * nb]);
+        const double d = (double)degq[rp];
+        return make_double2(acc.x / d, acc.y / d);
+    };
+    // rows 2p (f0) and 2p + 1 (f1) of the folded h at this column
+    auto pair_h = [&](int p, double2 &f0, double2 &f1) {
+        f0 = f1 = make_double2(0.0, 0.0);
+        if (p < npair && colok) {
+            const int2 pr = pairs[p];
+            const double2 hi = rot(pr.x);
+            const double2 hj = pr.y >= 0 ? rot(pr.y) : make_double2(0.0, 0.0);
+            f0 = make_double2(hi.x + hj.x, hi.y + hj.y);
+            f1 = make_double2(hj.y - hi.y, hi.x - hj.x);
+        }
+    };
+    const int ncombo = nkc * 64;  // (K chunk, lane) pairs of the block
+    double mx = 0.0;
+    for (int j = t; j < ncombo; j += 256) {
+        const int k0 = (j >> 6) * 64 + ((j & 63) >> 4) * 16;
+        for (int b = 0; b < 16; b += 2) {
+            double2 f0, f1;
+            pair_h((k0 + b) >> 1, f0, f1);
+            mx = fmax(mx, fmax(fmax(fabs(f0.x), fabs(f0.y)), fmax(fabs(f1.x), fabs(f1.y))));
+        }
+    }
+    red[t] = mx;
+    __syncthreads();
+    if (t < 16) {
+        double m2 = 0.0;
+        for (int r = 0; r < 16; ++r) m2 = fmax(m2, red[t + 16 * r]);
+        red[t] = m2 > 0.0 ? m2 : 1.0;
+    }
+    __syncthreads();
+    const double sc = red[t & 15];
+    if (t < 16) scale[(size_t)st * CP + col] = sc;
+    const double inv = 1.0 / sc;
+    for (int j = t; j < ncombo; j += 256) {
+        const int kc = j >> 6, lane = j & 63;
+        const int k0 = kc * 64 + (lane >> 4) * 16;
+        unsigned ure[kNS][4] = {}, uim[kNS][4] = {};
+        for (int b = 0; b < 16; b += 2) {
+            double2 f[2];
+            pair_h((k0 + b) >> 1, f[0], f[1]);
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                signed char dr[kNS], di[kNS];
+                split_digits<kNS>(f[e].x * inv, dr);
+                split_digits<kNS>(f[e].y * inv, di);
+#pragma unroll
+                for (int sd = 0; sd < kNS; ++sd) {
+                    put_byte(ure[sd], b + e, dr[sd]);
+                    put_byte(uim[sd], b + e, di[sd]);
+                }
+            }
+        }
+        v4i *o = bsl + ((((size_t)st * nkc + kc) * CB + cb) * 2) * kNS * 64 + lane;
+#pragma unroll
+        for (int sd = 0; sd < kNS; ++sd) {
+            o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
+            o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
+        }
+    }
+}
+
 // Mixed-precision contraction over the cell-pair rows (K2 = 2p: cos, 2p+1:
 // sin; k_rotate_h) for one block of 4 waves:
 //   acc[c][l][0|1] (the wave's 16 rows x column block c) = sum over K2 and
@@ -1109,12 +1197,16 @@ __global__ void k_phase_digits(int nkc, int ncb, const double *__restrict__ phi,
     for (int sd = 0; sd < kNS; ++sd) o[sd * 64] = as_v4i(u[sd]);
 }
 
-// One workgroup per (sp,Rp) slice; wave w takes row blocks w, w+4, ...; all
-// NCB column blocks (16 k each).  out[k][slice][i] FP64 complex.  The phase
-// digits (the same for every slice) are copied global -> LDS one chunk ahead
-// into two buffers shared by the 4 waves; the wave's A digits (its rows of
-// the folded gwann, streamed from HBM) are loaded one chunk ahead into the
-// second of two register sets.
+// Persistent workgroups: work item = (sp,Rp) slice x round of 4 row blocks
+// (wave w takes row block 4 round + w); workgroup b takes items b, b + grid,
+// ... and walks them as one stream of K chunks (item j, chunk kc), so the
+// copies of the next chunk run across item boundaries and an item's
+// recombination and stores overlap the next item's first loads.  All NCB
+// column blocks (16 k each).  out[k][slice][i] FP64 complex.  The phase
+// digits (the same for every slice) go global -> LDS one chunk ahead into two
+// buffers shared by the 4 waves; the wave's A digits (its rows of the folded
+// gwann, streamed from HBM) are loaded one chunk ahead into the second of two
+// register sets.
 template <int NCB>
 __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig, const double *__restrict__ gscale,
                                                     int nb2, int nrb, int nkc, int nslices,
@@ -1124,108 +1216,135 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
     constexpr int BCH = NCB * NS * 64;  // v4i per chunk of phase digits
     static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
     extern __shared__ v4i bsm[];
-    const int slice = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
-    auto copy_b = [&](int kc) {
-        const v4i *src = bphi + (size_t)kc * BCH;
-        v4i *dst = bsm + (kc & 1) * BCH;
+    const int nround = (nrb + 3) / 4;
+    const int64_t nitems = (int64_t)nslices * nround;
+    if ((int64_t)blockIdx.x >= nitems) return;  // uniform over the workgroup
+    const int64_t nmine = (nitems - blockIdx.x + gridDim.x - 1) / gridDim.x;
+    const int64_t nsteps = nmine * nkc;
+    // step st: item j = st / nkc, chunk kc = st % nkc
+    auto item = [&](int64_t st, int &slice, int &rb) {
+        const int64_t it = blockIdx.x + (st / nkc) * (int64_t)gridDim.x;
+        slice = (int)(it / nround);
+        rb = (int)(it % nround) * 4 + wave;
+    };
+    auto copy_b = [&](int64_t st) {
+        const v4i *src = bphi + (size_t)(st % nkc) * BCH;
+        v4i *dst = bsm + (st & 1) * BCH;
 #pragma unroll
         for (int q = 0; q < BCH / 256; ++q) {
             const int r = 4 * q + wave;
             lds_dma16(src + r * 64 + lane, dst + r * 64);
         }
     };
-    for (int rb0 = 0; rb0 < nrb; rb0 += 4) {
-        const int rb = rb0 + wave;
-        const bool live = rb < nrb;  // every wave joins the copies and barriers
-        v4i acc[NCB][NS][2];
+    auto load_a = [&](int64_t st, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
+        int slice, rb;
+        item(st, slice, rb);
+        if (rb >= nrb) return;
+        const int kc = (int)(st % nkc);
+        const v4i *ga = gdig + (((size_t)slice * nrb + rb) * nkc) * 2 * NS * 64 + lane;
+#pragma unroll
+        for (int sd = 0; sd < NS; ++sd) {
+            Ar[sd] = ga[((size_t)kc * 2 * NS + 0 * NS + sd) * 64];
+            Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
+        }
+    };
+    v4i acc[NCB][NS][2];
+    auto zero_acc = [&]() {
 #pragma unroll
         for (int c = 0; c < NCB; ++c)
 #pragma unroll
             for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
-        const v4i *ga = gdig + (((size_t)slice * nrb + rb) * nkc) * 2 * NS * 64 + lane;
-        auto load_a = [&](int kc, v4i (&Ar)[NS], v4i (&Ai)[NS]) {
+    };
+    auto chunk = [&](int64_t st, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
+        const v4i *bb = bsm + (st & 1) * BCH + lane;
 #pragma unroll
-            for (int sd = 0; sd < NS; ++sd) {
-                Ar[sd] = ga[((size_t)kc * 2 * NS + 0 * NS + sd) * 64];
-                Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
-            }
-        };
-        auto chunk = [&](int kc, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
-            const v4i *bb = bsm + (kc & 1) * BCH + lane;
+        for (int j = 0; j < NS; ++j) {
+            v4i B[NCB];
 #pragma unroll
-            for (int j = 0; j < NS; ++j) {
-                v4i B[NCB];
+            for (int c = 0; c < NCB; ++c) B[c] = bb[(c * NS + j) * 64];
 #pragma unroll
-                for (int c = 0; c < NCB; ++c) B[c] = bb[(c * NS + j) * 64];
+            for (int i = 0; i < NS - j; ++i)
 #pragma unroll
-                for (int i = 0; i < NS - j; ++i)
-#pragma unroll
-                    for (int c = 0; c < NCB; ++c) {
-                        acc[c][i + j][0] = mfma_i8(Ar[i], B[c], acc[c][i + j][0]);
-                        acc[c][i + j][1] = mfma_i8(Ai[i], B[c], acc[c][i + j][1]);
-                    }
-            }
-        };
-        // chunk kc + 1 in LDS for the whole block (no LDS stores pending: the
-        // barrier needs no fence; the clobber keeps LDS reads on their side)
-        auto next_ready = [&]() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); };
-        v4i Ar0[NS], Ai0[NS], Ar1[NS], Ai1[NS];
-        copy_b(0);
-        if (live) load_a(0, Ar0, Ai0);
-        lds_dma_wait();
-        __syncthreads();
-        for (int kc = 0;; kc += 2) {
-            if (kc + 1 < nkc) {
-                copy_b(kc + 1);
-                if (live) load_a(kc + 1, Ar1, Ai1);
-            }
-            if (live) chunk(kc, Ar0, Ai0);
-            if (kc + 1 >= nkc) break;
-            next_ready();
-            if (kc + 2 < nkc) {
-                copy_b(kc + 2);
-                if (live) load_a(kc + 2, Ar0, Ai0);
-            }
-            if (live) chunk(kc + 1, Ar1, Ai1);
-            if (kc + 2 >= nkc) break;
-            next_ready();
+                for (int c = 0; c < NCB; ++c) {
+                    acc[c][i + j][0] = mfma_i8(Ar[i], B[c], acc[c][i + j][0]);
+                    acc[c][i + j][1] = mfma_i8(Ai[i], B[c], acc[c][i + j][1]);
+                }
         }
-        lds_dma_wait();
-        __syncthreads();  // buffers free for the next row-block round
-        if (!live) continue;
-        double wl[NS];
-        wl[0] = 1.0 / (127.0 * 127.0);
+    };
+    // the step's chunk, and after an item's last chunk its recombination,
+    // stores and a fresh accumulator
+    auto step = [&](int64_t st, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
+        int slice, rb;
+        item(st, slice, rb);
+        const bool live = rb < nrb;
+        if (live) chunk(st, Ar, Ai);
+        if (st % nkc != nkc - 1) return;
+        if (live) {
+            double wl[NS];
+            wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
-        for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
+            for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
 #pragma unroll
-        for (int c = 0; c < NCB; ++c) {
-            const int col = c * 16 + li;
+            for (int c = 0; c < NCB; ++c) {
+                const int col = c * 16 + li;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = rb * 16 + 4 * lg + r;
-                if (col < nk && i < nb2) {
-                    double re = 0.0, im = 0.0;
+                for (int r = 0; r < 4; ++r) {
+                    const int i = rb * 16 + 4 * lg + r;
+                    if (col < nk && i < nb2) {
+                        double re = 0.0, im = 0.0;
 #pragma unroll
-                    for (int l = NS - 1; l >= 0; --l) {
-                        re += wl[l] * (double)acc[c][l][0][r];
-                        im += wl[l] * (double)acc[c][l][1][r];
+                        for (int l = NS - 1; l >= 0; --l) {
+                            re += wl[l] * (double)acc[c][l][0][r];
+                            im += wl[l] * (double)acc[c][l][1][r];
+                        }
+                        const double sc = gscale[(size_t)slice * nrb * 16 + i];
+                        out[(size_t)col * out_kstride + (size_t)slice * nb2 + i] = make_double2(re * sc, im * sc);
                     }
-                    const double sc = gscale[(size_t)slice * nrb * 16 + i];
-                    out[(size_t)col * out_kstride + (size_t)slice * nb2 + i] = make_double2(re * sc, im * sc);
                 }
             }
         }
+        zero_acc();
+    };
+    // step st + 1 in LDS for the whole workgroup (no LDS stores pending: the
+    // barrier needs no fence; the clobber keeps LDS reads on their side)
+    auto next_ready = [&]() { asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory"); };
+    zero_acc();
+    v4i Ar0[NS], Ai0[NS], Ar1[NS], Ai1[NS];
+    copy_b(0);
+    load_a(0, Ar0, Ai0);
+    lds_dma_wait();
+    __syncthreads();
+    for (int64_t st = 0;; st += 2) {
+        if (st + 1 < nsteps) {
+            copy_b(st + 1);
+            load_a(st + 1, Ar1, Ai1);
+        }
+        step(st, Ar0, Ai0);
+        if (st + 1 >= nsteps) break;
+        next_ready();
+        if (st + 2 < nsteps) {
+            copy_b(st + 2);
+            load_a(st + 2, Ar0, Ai0);
+        }
+        step(st + 1, Ar1, Ai1);
+        if (st + 2 >= nsteps) break;
+        next_ready();
     }
+    lds_dma_wait();
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
 // Rp contraction of the IBZ-q side (gReq) as an FP64 MFMA GEMM per branch s:
 //   out[q][(i, s, Re)] = sum_Rp gwann[(i,Re)][s][Rp] * phiT[Rp][q]
-// rows X = i + nb2 * Re; one wave per 16-row block.
+// rows X = i + nb2 * Re; one wave per 16-row block and all 16 NCB q columns,
+// so one pass over gwann serves 16 NCB q.  The K loop issues the gwann loads
+// of KU steps of 4 rows before their matrix instructions (K2 is a multiple
+// of 4 KU; rows past 2 npair read nothing and have zero phases).
 // ---------------------------------------------------------------------------
-template <int NCB>
+template <int NCB, int KU>
 __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
                                               const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
                                               const double *__restrict__ phiR, int ldphi, int K2, int nq,
@@ -1242,28 +1361,36 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
     const int64_t row = rb * 16 + li;
-    for (int k0 = 0; k0 < K2; k0 += 4) {
-        const int kk = k0 + lk;
-        // folded gwann row kk: G(2p) = g(R_p)/d + g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
-        double2 a = make_double2(0.0, 0.0);
-        const int p = kk >> 1;
-        if (row < nrows && p < npair) {
-            const int2 pr = pairs[p];
-            const double2 ga = A[(size_t)pr.x * sstride + row];
-            const double da = invdeg[pr.x];
-            double2 x = make_double2(ga.x * da, ga.y * da), y = make_double2(0.0, 0.0);
-            if (pr.y >= 0) {
-                const double2 gb = A[(size_t)pr.y * sstride + row];
-                const double db = invdeg[pr.y];
-                y = make_double2(gb.x * db, gb.y * db);
+    for (int k0 = 0; k0 < K2; k0 += 4 * KU) {
+        // folded gwann rows: G(2p) = g(R_p)/d + g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
+        double2 x[KU], y[KU];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+            const int p = (k0 + 4 * u + lk) >> 1;
+            x[u] = y[u] = make_double2(0.0, 0.0);
+            if (row < nrows && p < npair) {
+                const int2 pr = pairs[p];
+                const double2 ga = A[(size_t)pr.x * sstride + row];
+                const double da = invdeg[pr.x];
+                x[u] = make_double2(ga.x * da, ga.y * da);
+                if (pr.y >= 0) {
+                    const double2 gb = A[(size_t)pr.y * sstride + row];
+                    const double db = invdeg[pr.y];
+                    y[u] = make_double2(gb.x * db, gb.y * db);
+                }
             }
-            a = (kk & 1) ? make_double2(y.y - x.y, x.x - y.x) : make_double2(x.x + y.x, x.y + y.y);
         }
 #pragma unroll
-        for (int c = 0; c < NCB; ++c) {
-            const double b = phiR[(size_t)kk * ldphi + c * 16 + li];
-            cr[c] = mfma_f64(a.x, b, cr[c]);
-            ci[c] = mfma_f64(a.y, b, ci[c]);
+        for (int u = 0; u < KU; ++u) {
+            const int kk = k0 + 4 * u + lk;
+            const double2 a = (kk & 1) ? make_double2(y[u].y - x[u].y, x[u].x - y[u].x)
+                                       : make_double2(x[u].x + y[u].x, x[u].y + y[u].y);
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) {
+                const double b = phiR[(size_t)kk * ldphi + c * 16 + li];
+                cr[c] = mfma_f64(a.x, b, cr[c]);
+                ci[c] = mfma_f64(a.y, b, ci[c]);
+            }
         }
     }
 #pragma unroll
@@ -2288,6 +2415,9 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     int lds = 0;
     HCHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     c->lds_max = lds;
+    int ncu = 0;
+    HCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->ncu = ncu;
     HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HCHECK(hipEventCreate(&c->ev_a));
     HCHECK(hipEventCreate(&c->ev_b));
@@ -2814,7 +2944,9 @@ void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_
             const int nt = (int)(c->gnkc * (KT / 16) * 64);
             hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->gnkc,
                                (int)(KT / 16), phi, (int)KT, bphi);
-            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)nslices), dim3(256), 2 * 4 * kNS * 64 * sizeof(v4i), st,
+            const int64_t nitems = nslices * ((c->gnrb + 3) / 4);
+            const int64_t grid = std::min<int64_t>(nitems, 2 * std::max<int64_t>(c->ncu, 1));
+            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)grid), dim3(256), 2 * 4 * kNS * 64 * sizeof(v4i), st,
                                c->gdig.p, c->gscale.p,
                                (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
                                out + (size_t)k0 * out_kstride, out_kstride);
@@ -3423,7 +3555,7 @@ void ibzk_scratch(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t
     elph_ctx *c = p->ctx;
     const int64_t nb2 = p->nb * p->nb, nslices = p->nbr * c->nwsg;
     get(sc.gk, nkb * nb2 * nslices);
-    get(sc.h, nst * p->Kp * p->CP);
+    if (p->precision != ELPH_MIXED) get(sc.h, nst * p->Kp * p->CP);  // mixed: digits straight from gk
     get(sc.phiT, gk_phi_count(c));
     if (p->precision == ELPH_MIXED) {
         get(sc.bphi, gk_bphi_count(c));
@@ -3443,12 +3575,16 @@ void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStr
     elph_ctx *c = p->ctx;
     const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb, nslices = nbr * c->nwsg;
     launch_gkRp(c, s, g.nkb, g.kv.p, sc.gk.p, nb2 * nslices, p->precision == ELPH_MIXED, sc.phiT.p, sc.bphi.p);
-    launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)p->npair, (int)p->Kp, (int)p->CP, sc.gk.p, nb2 * nslices,
-                    g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.h.p);
     if (p->precision == ELPH_MIXED) {
-        launch_colscale(s, (int)g.nst, (int)p->Kp, (int)p->CP, sc.h.p, sc.bscale.p);
-        launch_slice_b_ri(s, (int)g.nst, (int)p->nkc, (int)(p->CP / 16), (int)p->Kp, (int)p->CP, sc.h.p,
-                          sc.bscale.p, sc.bsl.p);
+        if (g.nst > 0)
+            hipLaunchKernelGGL(k_rotate_digits, dim3((unsigned)g.nst, (unsigned)(p->CP / 16)), dim3(256), 0, s,
+                               (int)nb, (int)nbr, (int)p->npair, (int)p->nkc, (int)(p->CP / 16), (int)p->CP, sc.gk.p,
+                               nb2 * nslices, g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.bscale.p,
+                               sc.bsl.p);
+        HCHECK(hipGetLastError());
+    } else {
+        launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)p->npair, (int)p->Kp, (int)p->CP, sc.gk.p,
+                        nb2 * nslices, g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.h.p);
     }
     pair_launch(p, g, sc, s, out, timed);
 }
@@ -4052,10 +4188,11 @@ namespace {
 // g(Re, q) for nq device q vectors in cell-pair form over Rp: the real
 // cos/sin table of the pairs (KT q per launch, zero-padded rows and columns)
 // against gwann folded on the fly (1/gwsdeg included).
+constexpr int kGReqNCB = 8, kGReqKU = 2;  // 128 q per pass over gwann; loads two K steps ahead
 void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, int64_t out_qstride) {
     const int64_t nb2 = c->nb * c->nb;
-    const int64_t KT = 64;
-    const int64_t K2 = ((2 * c->npairg + 3) / 4) * 4;
+    const int64_t KT = 16 * kGReqNCB;
+    const int64_t K2 = ((2 * c->npairg + 4 * kGReqKU - 1) / (4 * kGReqKU)) * (4 * kGReqKU);
     DBuf<double2> phiR;  // K2 x KT doubles, as double2 pairs for the zero fill
     phiR.alloc((K2 * KT + 1) / 2);
     double *phi = reinterpret_cast<double *>(phiR.p);
@@ -4067,7 +4204,8 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
         dim3 grid((unsigned)((c->npairg + 127) / 128), (unsigned)n);
         hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, c->stream, (int)n, (int)c->npairg, d_qvecs + q0, nq,
                            c->rgpcell.p, phi, (int64_t)1, KT);
-        hipLaunchKernelGGL(k_gReq<4>, dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL((k_gReq<kGReqNCB, kGReqKU>), dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256), 0,
+                           c->stream,
                            c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->npairg, c->rgpair.p,
                            c->rginvdeg.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
         HCHECK(hipGetLastError());
@@ -4226,7 +4364,7 @@ int elph_ibzq_plan_run(elph_ibzq_plan *p, int64_t first, int64_t last, int64_t *
         c->t_dominant_ms = 0;
         c->n_dominant = 0;
         HCHECK(hipEventRecord(c->ev_c, s));
-        const int64_t QB = 32;
+        const int64_t QB = 16 * kGReqNCB;  // IBZ q per gReq pass (one pass over gwann)
         const int64_t gblk = nb2 * nbr * c->nwsk;
         int64_t istate = first;
         while (istate <= last) {

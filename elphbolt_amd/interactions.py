@@ -328,6 +328,49 @@ def write_g2_files(g2dir, kind, istate_first, counts, g2):
         raise ElphError(st, f"writing g2 files into {g2dir} failed")
 
 
+class G2Writer:
+    """gk2/gq2.istate* files written on a host thread while the device
+    computes the next states (elph_g2_writer_*): submit() takes the device
+    rows a run just wrote and returns at once; the next run may write into
+    the same device buffer (it waits on the device for the copy)."""
+
+    def __init__(self, wann: Wannier, g2dir, kind, row_stride, max_states):
+        self.wann, self.lib = wann, wann.lib
+        os.makedirs(g2dir, exist_ok=True)
+        h = ct.c_void_p()
+        st = self.lib.elph_g2_writer_create(wann.ctx, str(g2dir).encode(), kind.encode(), int(row_stride),
+                                            int(max_states), ct.byref(h))
+        wann._check(st, "g2 writer")
+        self.h = h
+
+    def submit(self, istate_first, counts, g2_device):
+        c = i64(counts)
+        ptr, dev = _dev_ptr(g2_device)
+        if not dev:
+            raise ValueError("the rows must be device memory")
+        st = self.lib.elph_g2_writer_submit(self.h, int(istate_first), len(c), _p(c, c_i64p), ptr)
+        if st != 0:
+            raise ElphError(st, f"g2 writer: {self.lib.elph_g2_writer_error(self.h).decode()}")
+
+    def wait(self):
+        st = self.lib.elph_g2_writer_wait(self.h)
+        if st != 0:
+            raise ElphError(st, f"g2 writer: {self.lib.elph_g2_writer_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            st = self.lib.elph_g2_writer_destroy(self.h)
+            self.h = None
+            if st != 0:
+                raise ElphError(st, "g2 writer: a batch was not written")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def read_g2_files(g2dir, kind, istate_first, nstates, row_stride):
     """The resume path of `read_gk2` / `read_gq2` (app/elphbolt.f90:185,212):
     (counts, rows) of states istate_first..istate_first+nstates-1 read from

@@ -343,15 +343,18 @@ def ws_count(n: int) -> int:
 def multi_material_draw(i: int, budget_bytes: float = 4e9, seed_base: int = MULTI_SEED_BASE):
     """The size draw of material i of the multi-material generator, without
     building it: dict(nat, nb, n, wv, a, nws, rng) where rng is the stream
-    positioned after the draw (the basis is drawn from it next)."""
+    positioned after the draw (the basis is drawn from it next).  The WS grid
+    n^3 (n uniform in 4..9), the fine mesh and the lattice constant are drawn
+    once; atoms and bands are drawn again until the vertex fits the budget, so
+    the grid sizes stay uniform."""
     rng = np.random.default_rng(seed_base + i)
+    n = int(rng.integers(4, 10))
+    wv = int(rng.choice([12, 16, 20, 24]))
+    a = float(rng.uniform(4.0, 6.0))
+    nws = ws_count(n)
     for _ in range(10000):
         nat = int(rng.integers(1, 9))
         nb = int(rng.integers(4, 27))
-        n = int(rng.integers(4, 10))
-        wv = int(rng.choice([12, 16, 20, 24]))
-        a = float(rng.uniform(4.0, 6.0))
-        nws = ws_count(n)
         if 16 * nb * nb * 3 * nat * nws * nws <= budget_bytes:
             return dict(nat=nat, nb=nb, n=n, wv=wv, a=a, nws=nws, rng=rng)
     raise ValueError("no draw within the budget")
@@ -361,8 +364,9 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
                    seed_base: int = MULTI_SEED_BASE, bands_in_window=None) -> Material:
     """Material i of the multi-material generator: atoms per cell uniform in
     1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
-    uniform in 4..9 (about 150..900 cells), fine mesh from {12, 16, 20, 24}^3.
-    Draws are repeated (same stream) until 16 nb^2 nbr nws^2 <= budget_bytes.
+    uniform in 4..9 (93..857 cells for fcc), fine mesh from {12, 16, 20, 24}^3.
+    Atoms and bands are drawn again (same stream) until 16 nb^2 nbr nws^2 <=
+    budget_bytes.
     The vertex follows the law of config 0 (device-generated by default)."""
     d = multi_material_draw(i, budget_bytes, seed_base)
     nat, nb, n, wv, a, rng = d["nat"], d["nb"], d["n"], d["wv"], d["a"], d["rng"]
@@ -375,7 +379,7 @@ def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", wi
                          bands_in_window=nb / 2.0 if bands_in_window is None else bands_in_window)
 
 
-SERVING_BUDGET = 12e9
+SERVING_BUDGET = 8e9
 SERVING_MIN_BYTES = 2e9
 
 
@@ -383,7 +387,7 @@ def serving_indices(count: int = 8, budget_bytes: float = SERVING_BUDGET,
                     min_bytes: float = SERVING_MIN_BYTES) -> list:
     """The resident set of the serving workload: the first `count` draws of the
     multi-material generator (vertex cap `budget_bytes`) whose Wannier vertex
-    holds at least `min_bytes` (eight draws: 38.8 GB in total)."""
+    holds at least `min_bytes` (eight draws: 40.3 GB in total)."""
     out, i = [], 0
     while len(out) < count:
         d = multi_material_draw(i, budget_bytes)

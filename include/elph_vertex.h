@@ -248,6 +248,25 @@ int elph_write_g2_files(const char *dir, char kind, int64_t istate_first, int64_
  * driver skipped).  ELPH_EIO on a short or corrupt file. */
 int elph_read_g2_files(const char *dir, char kind, int64_t istate_first, int64_t nstates, int64_t *counts,
                        double *g2, int64_t row_stride);
+/* The same files written while the device computes the next states (the
+ * reference writes each state's file as soon as it has it,
+ * src/interactions.f90:2337-2349, 1877-1889).  submit() takes device rows
+ * (stride row_stride) of up to max_states states, queues their copy to pinned
+ * host memory behind the work already on the context's stream, makes later
+ * work on that stream wait for the copy -- so the caller may run the next
+ * states into the same device buffer at once -- and returns; a host thread
+ * writes the files.  Two staging buffers alternate (submit waits while both
+ * hold batches not yet on disk).  wait() returns when every submitted state
+ * is on disk, with the first error (ELPH_EIO when a file could not be
+ * written).  No counterpart in the reference as a separate object. */
+typedef struct elph_g2_writer elph_g2_writer;
+int elph_g2_writer_create(elph_ctx *ctx, const char *dir, char kind, int64_t row_stride, int64_t max_states,
+                          elph_g2_writer **writer);
+int elph_g2_writer_submit(elph_g2_writer *writer, int64_t istate_first, int64_t nstates, const int64_t *counts,
+                          const double *g2_device);
+int elph_g2_writer_wait(elph_g2_writer *writer);
+const char *elph_g2_writer_error(const elph_g2_writer *writer);
+int elph_g2_writer_destroy(elph_g2_writer *writer);
 /* Writes Xplus.istate<i> and Xminus.istate<i> as calculate_eph_interaction_ibzk
  * key 'X' does (src/interactions.f90:2356-2377) and read_transition_probs_e
  * reads them (3065-3091): int64 count, count doubles X, count int64

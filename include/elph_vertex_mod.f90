@@ -313,6 +313,39 @@ module elph_vertex_mod
        real(c_double) :: g2(*)
      end function elph_write_g2_files
 
+     integer(c_int) function elph_g2_writer_create(ctx, dir, kind, stride, max_states, writer) &
+          bind(C, name="elph_g2_writer_create")
+       import
+       type(c_ptr), value :: ctx
+       character(kind=c_char) :: dir(*)
+       character(kind=c_char), value :: kind
+       integer(c_int64_t), value :: stride, max_states
+       type(c_ptr) :: writer
+     end function elph_g2_writer_create
+
+     integer(c_int) function elph_g2_writer_submit(writer, first, nstates, counts, g2_device) &
+          bind(C, name="elph_g2_writer_submit")
+       import
+       type(c_ptr), value :: writer, g2_device
+       integer(c_int64_t), value :: first, nstates
+       integer(c_int64_t) :: counts(*)
+     end function elph_g2_writer_submit
+
+     integer(c_int) function elph_g2_writer_wait(writer) bind(C, name="elph_g2_writer_wait")
+       import
+       type(c_ptr), value :: writer
+     end function elph_g2_writer_wait
+
+     type(c_ptr) function elph_g2_writer_error(writer) bind(C, name="elph_g2_writer_error")
+       import
+       type(c_ptr), value :: writer
+     end function elph_g2_writer_error
+
+     integer(c_int) function elph_g2_writer_destroy(writer) bind(C, name="elph_g2_writer_destroy")
+       import
+       type(c_ptr), value :: writer
+     end function elph_g2_writer_destroy
+
      integer(c_int) function elph_read_g2_files(dir, kind, first, nstates, counts, g2, stride) &
           bind(C, name="elph_read_g2_files")
        import
