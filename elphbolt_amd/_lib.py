@@ -71,6 +71,7 @@ PROTOTYPES = {
                                          ct.c_int32, ct.POINTER(ct.c_void_p)]),
     "elph_ibzk_plan_run": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p, ct.c_int32]),
     "elph_ibzk_plan_nprocs": (ct.c_int64, [ct.c_void_p]),
+    "elph_ibzk_plan_run_noalloc": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int64, c_i64p, ct.c_void_p]),
     "elph_tetra_create": (ct.c_int, [ct.c_void_p, ct.c_int64, c_i64p, c_i64p, ct.c_int64, c_dblp,
                                      ct.POINTER(ct.c_void_p)]),
     "elph_triang_create": (ct.c_int, [ct.c_void_p, ct.c_int64, c_i64p, c_i64p, ct.c_int64, c_dblp,

@@ -123,6 +123,9 @@ struct elph_ctx {
     int64_t pol_nG = 0;
     double pol_alpha = 0, pol_fac = 0;
     std::string err;
+    // device address of the library's file-scope scratch array on this device
+    // (the allocation-free entry points carve their buffers from it)
+    unsigned char *noalloc_scratch = nullptr;
     int64_t lds_max = 0;  // dynamic LDS a workgroup may allocate on this device (bytes)
     int64_t ncu = 0;      // compute units of the device (persistent launches)
     double t_dominant_ms = 0, t_all_ms = 0;

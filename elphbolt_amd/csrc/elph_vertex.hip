@@ -36,8 +36,46 @@
 
 using namespace elph;
 
+// Scratch of the allocation-free entry point (elph_ibzk_plan_run_noalloc):
+// one array per device, held from the library's first use on that device
+// until the process ends; every call on a device shares it.  Host code never
+// takes the name as an address: it uses the pointer hipGetSymbolAddress
+// returns with the device current (elph_ctx::noalloc_scratch).
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
+// Device address of `scratch` on `device` (current), looked up once per device.
+unsigned char *noalloc_scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, unsigned char *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void *ptr = nullptr;
+    HCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error{ELPH_EDEVICE, "hipGetSymbolAddress returned no address for the scratch array"};
+    addr[device] = static_cast<unsigned char *>(ptr);
+    return addr[device];
+}
+
+// Bump allocator over a fixed device region: hands out 256-byte aligned
+// views, throws ELPH_ENOMEM past the end.
+struct Carve {
+    unsigned char *base;
+    size_t cap, used = 0;
+    template <typename T>
+    void take(DBuf<T> &b, size_t count) {
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off + bytes > cap)
+            throw Error{ELPH_ENOMEM, "the request needs more than the " + std::to_string(cap >> 20) +
+                                         " MiB scratch of the allocation-free entry point"};
+        b.borrow(reinterpret_cast<T *>(base + off), std::max<size_t>(count, 1));
+        used = off + bytes;
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Phases
@@ -2418,6 +2456,7 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
     int ncu = 0;
     HCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
     c->ncu = ncu;
+    c->noalloc_scratch = noalloc_scratch_address(c->device);
     HCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HCHECK(hipEventCreate(&c->ev_a));
     HCHECK(hipEventCreate(&c->ev_b));
@@ -3569,6 +3608,27 @@ void ibzk_reserve(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t
     ibzk_scratch(p, sc, nkb, nst, [](auto &buf, int64_t n) { buf.alloc((size_t)n); });
 }
 
+// Bytes of scratch and group tables for groups of up to nkb IBZ k and nst
+// states (each buffer rounded up to 256 bytes, as Carve places them).
+size_t ibzk_noalloc_bytes(const elph_ibzk_plan *p, int64_t nkb, int64_t nst, bool with_y) {
+    size_t total = 0;
+    auto add = [&total](auto &buf, int64_t n) {
+        using T = std::remove_pointer_t<decltype(buf.p)>;
+        total += ((std::max<int64_t>(n, 1) * sizeof(T)) + 255) & ~(size_t)255;
+    };
+    IbzkScratch probe;
+    ibzk_scratch(p, probe, nkb, nst, [&](auto &buf, int64_t n) {
+        if (with_y || (void *)&buf != (void *)&probe.Y) add(buf, n);
+    });
+    IbzkGroup g;
+    add(g.kv, 3 * nkb);
+    add(g.Ukb, nkb * p->nb * p->nb);
+    add(g.st_kb, nst);
+    add(g.st_m, nst);
+    add(g.st_kint, 3 * nst);
+    add(g.st_slot, nst);
+    return total;
+}
 
 // One group: g(k,Rp) of its k, h = U_k g / phwsdeg, (mixed) digits, pair stage.
 void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
@@ -3845,6 +3905,74 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
         c->tev_used = 0;
         return ret_error(c, e);
     }
+    return ELPH_OK;
+}
+
+int elph_ibzk_plan_run_noalloc(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *counts, double *g2_device) {
+    if (!p || !g2_device || !counts) return ELPH_EINVAL;
+    elph_ctx *c = p->ctx;
+    const int64_t y_cap = p->y_bytes_cap;
+    try {
+        HCHECK(hipSetDevice(c->device));
+        if (first < 1 || last > p->nirr * p->nb || last < first - 1)
+            throw Error{ELPH_EINVAL, "istate range outside 1..nwv_irred*numwannbands"};
+        if (!c->noalloc_scratch) throw Error{ELPH_EDEVICE, "no scratch array address for this device"};
+        if (p->precision == ELPH_MIXED && !c->gdig.p) throw Error{ELPH_EINVAL, "gwann digits missing"};
+        const size_t cap = kNoAllocScratchBytes;
+        // the largest group of IBZ k whose tables and scratch fit the array
+        // (a general-path plan also bounds its chunk of Y by what is left)
+        const int64_t nb = p->nb;
+        auto need = [&](int64_t nkb) {
+            if (p->general) {
+                const size_t fixed = ibzk_noalloc_bytes(p, nkb, nkb * nb, false);
+                const int64_t per = p->nwv * p->CP * (int64_t)sizeof(double2);
+                if (fixed + per + 256 > cap) return cap + 1;
+                p->y_bytes_cap = ((int64_t)(cap - fixed) - 256) / per * per;
+            }
+            return ibzk_noalloc_bytes(p, nkb, nkb * nb, true);
+        };
+        int64_t KB = 64;
+        while (KB > 1 && need(KB) > cap) KB = (KB + 1) / 2;
+        if (need(KB) > cap)
+            throw Error{ELPH_ENOMEM, "one IBZ k of this material needs more than the " + std::to_string(cap >> 20) +
+                                         " MiB scratch of the allocation-free entry point"};
+        hipStream_t s = c->stream;
+        c->t_dominant_ms = 0;
+        c->n_dominant = 0;
+        c->tev_used = 0;
+        const std::vector<GroupHost> hosts = ibzk_group_hosts(p, first, last, true, counts, false, KB);
+        int64_t nkb = 1, nst = 1;
+        for (const GroupHost &h : hosts) {
+            nkb = std::max(nkb, h.nkb);
+            nst = std::max(nst, h.nst);
+        }
+        Carve cv{c->noalloc_scratch, cap};
+        IbzkScratch sc;
+        ibzk_scratch(p, sc, nkb, nst, [&cv](auto &buf, int64_t n) { cv.take(buf, (size_t)n); });
+        IbzkGroup g;
+        cv.take(g.kv, 3 * nkb);
+        cv.take(g.Ukb, nkb * nb * nb);
+        cv.take(g.st_kb, nst);
+        cv.take(g.st_m, nst);
+        cv.take(g.st_kint, 3 * nst);
+        cv.take(g.st_slot, nst);
+        HCHECK(hipEventRecord(c->ev_c, s));
+        for (const GroupHost &h : hosts) {
+            ibzk_group_upload(h, g, s);  // stream-ordered after the previous group's kernels
+            ibzk_enqueue(p, g, sc, s, g2_device, true);
+        }
+        HCHECK(hipEventRecord(c->ev_d, s));
+        HCHECK(hipEventSynchronize(c->ev_d));
+        float ms = 0;
+        HCHECK(hipEventElapsedTime(&ms, c->ev_c, c->ev_d));
+        c->t_all_ms = ms;
+        timing_flush(c);
+    } catch (const Error &e) {
+        p->y_bytes_cap = y_cap;
+        c->tev_used = 0;
+        return ret_error(c, e);
+    }
+    p->y_bytes_cap = y_cap;
     return ELPH_OK;
 }
 

@@ -154,6 +154,18 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
 int elph_ibzk_plan_run(elph_ibzk_plan *plan, int64_t istate_first, int64_t istate_last, int64_t *counts,
                        double *g2, int32_t g2_on_device);
 int64_t elph_ibzk_plan_nprocs(const elph_ibzk_plan *plan);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): elph_ibzk_plan_run with the g2 rows in device memory (row i =
+ * state istate_first+i, stride nprocs), allocating nothing.  Its group tables
+ * and the scratch of its kernels come from one 128 MiB array per GPU that
+ * the library holds from its first use on that GPU until the process ends;
+ * the IBZ k of a call are processed in groups sized to fit it (ELPH_ENOMEM
+ * when a single IBZ k needs more).  Every call on a GPU shares that array:
+ * calls that overlap on one GPU (other threads, other contexts) race.  Create
+ * the plan (which allocates) beforehand. */
+int elph_ibzk_plan_run_noalloc(elph_ibzk_plan *plan, int64_t istate_first, int64_t istate_last, int64_t *counts,
+                               double *g2_device);
 int elph_ibzk_plan_destroy(elph_ibzk_plan *plan);
 /* One-shot form (plan + run + destroy). */
 int elph_ibzk_g2(elph_ctx *ctx, const elph_electrons *el, const elph_phonons *ph, int64_t mesh_ref,

@@ -181,6 +181,14 @@ module elph_vertex_mod
        integer(c_int32_t), value :: on_device
      end function elph_ibzk_plan_run
 
+     integer(c_int) function elph_ibzk_plan_run_noalloc(plan, first, last, counts, g2_device) &
+          bind(C, name="elph_ibzk_plan_run_noalloc")
+       import
+       type(c_ptr), value :: plan, g2_device
+       integer(c_int64_t), value :: first, last
+       integer(c_int64_t) :: counts(*)
+     end function elph_ibzk_plan_run_noalloc
+
      integer(c_int64_t) function elph_ibzk_plan_nprocs(plan) bind(C, name="elph_ibzk_plan_nprocs")
        import
        type(c_ptr), value :: plan
