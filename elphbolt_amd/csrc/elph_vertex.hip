@@ -298,39 +298,44 @@ struct PairArgs {
 // unrolled to those bounds (guarded by the runtime counts), the row of U_k'
 // is held in registers and every load of a rank is issued before its
 // arithmetic.  NBM == 0: loops over the runtime counts.
-// STAGE (NBM > 0 only): instead of global stores, the row's |g|^2 go to
-// stage[offs[row] - sbase + ...] after the last read of Y by the wave (the
-// caller then writes the wave's contiguous range out with full-line stores);
-// stage may be the wave's own Y rows.
-template <int NBM, int NBRM, bool STAGE = false>
+// The phonon row of q = k' - k (mesh_ref > 1: the fine-mesh table for q off
+// the phonon mesh).
+struct PhRow {
+    const double2 *uq = nullptr, *lq = nullptr;
+    const double *omq = nullptr;
+};
+__device__ __forceinline__ PhRow phonon_row(const PairArgs &a, int q0, int q1, int q2) {
+    PhRow r;
+    if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
+        const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+        r.uq = a.u + (size_t)iq * a.nbr * a.nbr;
+        r.omq = a.om + (size_t)iq * a.nbr;
+        if (a.lp) r.lq = a.lp + (size_t)iq * a.nbr;
+    } else {
+        const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+        r.uq = a.uf + (size_t)iq * a.nbr * a.nbr;
+        r.omq = a.omf + (size_t)iq * a.nbr;
+        if (a.lpf) r.lq = a.lpf + (size_t)iq * a.nbr;
+    }
+    return r;
+}
+
+template <int NBM, int NBRM>
 __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const double2 *Y, int ys, int row, int yi, int lk,
-                                                  int st, int k0i, int k1i, int k2i, double *outrow,
-                                                  double *stage = nullptr, int64_t sbase = 0) {
-    static_assert(!STAGE || NBM > 0, "staging needs the unrolled band bound");
+                                                  int st, int k0i, int k1i, int k2i, double *outrow) {
     if (row >= a.nwv) return;
     const int q0 = pos_mod(a.kp_int[3 * row] - k0i, a.Nel0);
     const int q1 = pos_mod(a.kp_int[3 * row + 1] - k1i, a.Nel1);
     const int q2 = pos_mod(a.kp_int[3 * row + 2] - k2i, a.Nel2);
-    const double2 *uq, *lq = nullptr;
-    const double *omq;
-    if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
-        const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-        uq = a.u + (size_t)iq * a.nbr * a.nbr;
-        omq = a.om + (size_t)iq * a.nbr;
-        if (a.lp) lq = a.lp + (size_t)iq * a.nbr;
-    } else {
-        const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
-        uq = a.uf + (size_t)iq * a.nbr * a.nbr;
-        omq = a.omf + (size_t)iq * a.nbr;
-        if (a.lpf) lq = a.lpf + (size_t)iq * a.nbr;
-    }
+    const PhRow ph = phonon_row(a, q0, q1, q2);
+    const double2 *uq = ph.uq, *lq = ph.lq;
+    const double *omq = ph.omq;
     const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb : nullptr;
     const int nn = a.nwin[row];
     const double2 *Yr = Y + yi * ys;
     if constexpr (NBM > 0) {
         const int nb = a.nb, nbr = a.nbr;
-        // |g|^2 of in-window rank `rank` of the row, all branches
-        auto rank_g2 = [&](int rank, double (&g2)[NBRM]) {
+        for (int rank = lk; rank < nn; rank += 4) {
             const int n = a.wband[row * nb + rank];
             const double2 *V = a.Vkp + ((size_t)row * nb + n) * nb;
             double2 v[NBM];
@@ -362,10 +367,13 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                         unm.x += t.x;
                         unm.y += t.y;
                     }
-            // g(s) = sum_sp u_q(s,sp) z(sp) (+ polar); the stores follow all loads
+            // g(s) = sum_sp u_q(s,sp) z(sp) (+ polar); 8-byte stores at the
+            // branch stride go through L2, which merges them into whole lines
+            // (nontemporal stores did not change the time but wrote 2.4x the
+            // bytes)
+            double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
 #pragma unroll
-            for (int s = 0; s < NBRM; ++s) {
-                g2[s] = 0.0;
+            for (int s = 0; s < NBRM; ++s)
                 if (s < nbr) {
                     double2 gb = make_double2(0.0, 0.0);
 #pragma unroll
@@ -373,38 +381,8 @@ __device__ __forceinline__ void ibzk_epilogue_row(const PairArgs &a, const doubl
                         if (sp < nbr) cfma(gb, uq[s * nbr + sp], z[sp]);
                     if (lq) cfma(gb, lq[s], unm);
                     const double w = omq[s];
-                    if (w != 0.0) g2[s] = 0.5 * (gb.x * gb.x + gb.y * gb.y) / w * kG2Unit;
+                    o[s] = w != 0.0 ? 0.5 * (gb.x * gb.x + gb.y * gb.y) / w * kG2Unit : 0.0;
                 }
-            }
-        };
-        if constexpr (STAGE) {
-            constexpr int NR = (NBM + 3) / 4;  // ranks per lane
-            double keep[NR][NBRM];
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (lk + 4 * j < nn) rank_g2(lk + 4 * j, keep[j]);
-            // the wave's reads of Y are all issued above (LDS serves a wave's
-            // requests in order); keep the compiler from moving stores up
-            asm volatile("" ::: "memory");
-            double *o = stage + (a.offs[row] - sbase);
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (lk + 4 * j < nn)
-#pragma unroll
-                    for (int s = 0; s < NBRM; ++s)
-                        if (s < nbr) o[(lk + 4 * j) * nbr + s] = keep[j][s];
-        } else {
-            for (int rank = lk; rank < nn; rank += 4) {
-                double g2[NBRM];
-                rank_g2(rank, g2);
-                // 8-byte stores at the branch stride: through L2, which
-                // merges them into whole lines (nontemporal stores did not
-                // change the time but wrote 2.4x the bytes)
-                double *o = outrow + a.offs[row] + (int64_t)rank * nbr;
-#pragma unroll
-                for (int s = 0; s < NBRM; ++s)
-                    if (s < nbr) o[s] = g2[s];
-            }
         }
         return;
     }
@@ -679,88 +657,169 @@ __global__ void k_slice_b_ri(int nst, int nkc, int CB, int Kp, int CP, const dou
 }
 
 // Mixed-precision B operand straight from g(k,Rp), for the fused and general
-// ibzk paths: per (state, 16-column block) the folded h of k_rotate_h (same
-// expressions, so the same values), its column scale as k_colscale takes it
-// and its int8 digits in k_slice_b_ri's layout ([st][kc][cb][Re, Im][digit]
-// [lane]), without h going through memory.  Pass 1 takes the column maxima
-// over all rows, pass 2 recomputes h and writes the digits; thread t always
-// holds column cb 16 + (t & 15).
-__global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npair, int nkc, int CB, int CP,
+// ibzk paths: the folded h of k_rotate_h (same expressions, so the same
+// values), its column scale as k_colscale takes it and its int8 digits in
+// k_slice_b_ri's layout ([st][kc][cb][Re, Im][digit][lane]), without h going
+// through memory.  Workgroup (k, 16-column block, z) holds all in-window
+// states of IBZ k `kb` (st_kb is nondecreasing: they are consecutive).  A run
+// is the 16 K2 rows (8 cell pairs) of one lane's digit dwords, run = 4 kc + lg;
+// thread (run, w, column) takes dword w of it -- cell pairs 2w and 2w + 1 --
+// for up to kRotMC states at a time, so each g(k,Rp) value is loaded once for
+// all of them (NBX > 0: the band loop unrolled to NBX, loads issued together).
+// Pass MAX writes the column maxima of the workgroup's runs to
+// hmax[st][z][col]; pass DIGITS reduces them over z into the scale,
+// recomputes h and writes the digits.
+constexpr int kRotRuns = 4;  // runs per workgroup (x 4 dwords x 16 columns = 256 threads)
+constexpr int kRotMC = 4;    // states per register set
+
+__device__ __forceinline__ int lower_bound_int(const int *a, int n, int v) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool DIGITS, int NBX>
+__global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npair, int nkc, int CB, int CP, int nst,
                                                        const double2 *__restrict__ gk, int64_t gk_kstride,
                                                        const int *__restrict__ st_kb, const int *__restrict__ st_m,
                                                        const double2 *__restrict__ Uk, const int *__restrict__ degq,
-                                                       const int2 *__restrict__ pairs, double *__restrict__ scale,
-                                                       v4i *__restrict__ bsl) {
-    __shared__ double red[256];
-    const int st = blockIdx.x, cb = blockIdx.y, t = threadIdx.x;
-    const int col = cb * 16 + (t & 15);
+                                                       const int2 *__restrict__ pairs, double *__restrict__ hmax,
+                                                       double *__restrict__ scale, v4i *__restrict__ bsl) {
+    constexpr int NBL = NBX > 0 ? NBX : 1;  // register copies of g (runtime loop when NBX == 0)
+    extern __shared__ double2 ush[];        // u rows of the k's states, [m][mp]
+    __shared__ double red[4 * kRotRuns][16];
+    const int kb = blockIdx.x, cb = blockIdx.y, z = blockIdx.z, nz = gridDim.z, t = threadIdx.x;
+    const int li = t & 15, w = (t >> 4) & 3, run = z * kRotRuns + (t >> 6);
+    const int s0 = lower_bound_int(st_kb, nst, kb), s1 = lower_bound_int(st_kb, nst, kb + 1);
+    const int nm = s1 - s0;
+    for (int i = t; i < nm * nb; i += blockDim.x) {
+        const int m = i / nb, mp = i % nb;
+        ush[i] = Uk[((size_t)kb * nb + st_m[s0 + m]) * nb + mp];
+    }
+    __syncthreads();
+    const int col = cb * 16 + li;
     const bool colok = col < nb * nbr;
     const int np = colok ? col % nb : 0, sp = colok ? col / nb : 0;
-    const int kb = st_kb[st], m = st_m[st];
-    const double2 *u = Uk + ((size_t)kb * nb + m) * nb;
-    auto rot = [&](int rp) {
-        const double2 *g = gk + (size_t)kb * gk_kstride + ((size_t)(sp + nbr * rp)) * nb * nb + np;
-        double2 acc = make_double2(0.0, 0.0);
-        for (int mp = 0; mp < nb; ++mp) cfma(acc, u[mp], g[(size_t)mp * nb]);
-        const double d = (double)degq[rp];
-        return make_double2(acc.x / d, acc.y / d);
-    };
-    // rows 2p (f0) and 2p + 1 (f1) of the folded h at this column
-    auto pair_h = [&](int p, double2 &f0, double2 &f1) {
-        f0 = f1 = make_double2(0.0, 0.0);
-        if (p < npair && colok) {
-            const int2 pr = pairs[p];
-            const double2 hi = rot(pr.x);
-            const double2 hj = pr.y >= 0 ? rot(pr.y) : make_double2(0.0, 0.0);
-            f0 = make_double2(hi.x + hj.x, hi.y + hj.y);
-            f1 = make_double2(hj.y - hi.y, hi.x - hj.x);
-        }
-    };
-    const int ncombo = nkc * 64;  // (K chunk, lane) pairs of the block
-    double mx = 0.0;
-    for (int j = t; j < ncombo; j += 256) {
-        const int k0 = (j >> 6) * 64 + ((j & 63) >> 4) * 16;
-        for (int b = 0; b < 16; b += 2) {
-            double2 f0, f1;
-            pair_h((k0 + b) >> 1, f0, f1);
-            mx = fmax(mx, fmax(fmax(fabs(f0.x), fabs(f0.y)), fmax(fabs(f1.x), fabs(f1.y))));
-        }
-    }
-    red[t] = mx;
-    __syncthreads();
-    if (t < 16) {
-        double m2 = 0.0;
-        for (int r = 0; r < 16; ++r) m2 = fmax(m2, red[t + 16 * r]);
-        red[t] = m2 > 0.0 ? m2 : 1.0;
-    }
-    __syncthreads();
-    const double sc = red[t & 15];
-    if (t < 16) scale[(size_t)st * CP + col] = sc;
-    const double inv = 1.0 / sc;
-    for (int j = t; j < ncombo; j += 256) {
-        const int kc = j >> 6, lane = j & 63;
-        const int k0 = kc * 64 + (lane >> 4) * 16;
-        unsigned ure[kNS][4] = {}, uim[kNS][4] = {};
-        for (int b = 0; b < 16; b += 2) {
-            double2 f[2];
-            pair_h((k0 + b) >> 1, f[0], f[1]);
+    const bool live = colok && run < nkc * 4;
+    const int kc = run >> 2, lg = run & 3;
+    const double2 *gbase = gk + (size_t)kb * gk_kstride + (size_t)sp * nb * nb + np;
+    const int64_t rstride = (int64_t)nbr * nb * nb;  // next Rp
+    for (int m0 = 0; m0 < nm; m0 += kRotMC) {
+        const int mc = min(kRotMC, nm - m0);
+        double mx[kRotMC], inv[kRotMC];
+        unsigned ure[kRotMC][kNS], uim[kRotMC][kNS];
 #pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                signed char dr[kNS], di[kNS];
-                split_digits<kNS>(f[e].x * inv, dr);
-                split_digits<kNS>(f[e].y * inv, di);
+        for (int m = 0; m < kRotMC; ++m) {
+            mx[m] = 0.0;
+            inv[m] = 1.0;
 #pragma unroll
-                for (int sd = 0; sd < kNS; ++sd) {
-                    put_byte(ure[sd], b + e, dr[sd]);
-                    put_byte(uim[sd], b + e, di[sd]);
+            for (int sd = 0; sd < kNS; ++sd) ure[m][sd] = uim[m][sd] = 0u;
+        }
+        if constexpr (DIGITS) {
+#pragma unroll
+            for (int m = 0; m < kRotMC; ++m)
+                if (m < mc) {
+                    double sc = 0.0;
+                    for (int zz = 0; zz < nz; ++zz) sc = fmax(sc, hmax[((size_t)(s0 + m0 + m) * nz + zz) * CP + col]);
+                    sc = sc > 0.0 ? sc : 1.0;
+                    inv[m] = 1.0 / sc;
+                    if (z == 0 && t < 16) scale[(size_t)(s0 + m0 + m) * CP + col] = sc;
+                }
+        }
+        if (live) {
+#pragma unroll 1
+            for (int j = 0; j < 2; ++j) {
+                const int pidx = kc * 32 + lg * 8 + 2 * w + j;
+                if (pidx < npair) {
+                    const int2 pr = pairs[pidx];
+                    const double2 *gi = gbase + pr.x * rstride;
+                    const double2 *gj = gbase + (pr.y >= 0 ? pr.y : 0) * rstride;
+                    double2 ga[NBL], gb[NBL];
+                    if constexpr (NBX > 0) {
+#pragma unroll
+                        for (int mp = 0; mp < NBX; ++mp) {
+                            ga[mp] = mp < nb ? gi[(size_t)mp * nb] : make_double2(0.0, 0.0);
+                            gb[mp] = mp < nb && pr.y >= 0 ? gj[(size_t)mp * nb] : make_double2(0.0, 0.0);
+                        }
+                    }
+                    const double di = (double)degq[pr.x], dj = pr.y >= 0 ? (double)degq[pr.y] : 1.0;
+#pragma unroll
+                    for (int m = 0; m < kRotMC; ++m)
+                        if (m < mc) {
+                            const double2 *u = ush + (m0 + m) * nb;
+                            double2 hi = make_double2(0.0, 0.0), hj = make_double2(0.0, 0.0);
+                            if constexpr (NBX > 0) {
+#pragma unroll
+                                for (int mp = 0; mp < NBX; ++mp)
+                                    if (mp < nb) {
+                                        cfma(hi, u[mp], ga[mp]);
+                                        cfma(hj, u[mp], gb[mp]);
+                                    }
+                            } else {
+                                for (int mp = 0; mp < nb; ++mp) {
+                                    cfma(hi, u[mp], gi[(size_t)mp * nb]);
+                                    cfma(hj, u[mp], pr.y >= 0 ? gj[(size_t)mp * nb] : make_double2(0.0, 0.0));
+                                }
+                            }
+                            const double2 a = make_double2(hi.x / di, hi.y / di);
+                            const double2 b = pr.y >= 0 ? make_double2(hj.x / dj, hj.y / dj) : make_double2(0.0, 0.0);
+                            const double2 f0 = make_double2(a.x + b.x, a.y + b.y);
+                            const double2 f1 = make_double2(b.y - a.y, a.x - b.x);
+                            if constexpr (!DIGITS) {
+                                mx[m] = fmax(mx[m], fmax(fmax(fabs(f0.x), fabs(f0.y)), fmax(fabs(f1.x), fabs(f1.y))));
+                            } else {
+                                signed char dr[kNS], dim[kNS];
+                                split_digits<kNS>(f0.x * inv[m], dr);
+                                split_digits<kNS>(f0.y * inv[m], dim);
+#pragma unroll
+                                for (int sd = 0; sd < kNS; ++sd) {
+                                    ure[m][sd] |= (unsigned)(unsigned char)dr[sd] << (16 * j);
+                                    uim[m][sd] |= (unsigned)(unsigned char)dim[sd] << (16 * j);
+                                }
+                                split_digits<kNS>(f1.x * inv[m], dr);
+                                split_digits<kNS>(f1.y * inv[m], dim);
+#pragma unroll
+                                for (int sd = 0; sd < kNS; ++sd) {
+                                    ure[m][sd] |= (unsigned)(unsigned char)dr[sd] << (16 * j + 8);
+                                    uim[m][sd] |= (unsigned)(unsigned char)dim[sd] << (16 * j + 8);
+                                }
+                            }
+                        }
                 }
             }
         }
-        v4i *o = bsl + ((((size_t)st * nkc + kc) * CB + cb) * 2) * kNS * 64 + lane;
+        if constexpr (!DIGITS) {
+            // column maxima over the workgroup's runs and dwords
 #pragma unroll
-        for (int sd = 0; sd < kNS; ++sd) {
-            o[(0 * kNS + sd) * 64] = as_v4i(ure[sd]);
-            o[(1 * kNS + sd) * 64] = as_v4i(uim[sd]);
+            for (int m = 0; m < kRotMC; ++m) {
+                if (m >= mc) break;
+                red[t >> 4][li] = mx[m];
+                __syncthreads();
+                if (t < 16) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 4 * kRotRuns; ++r) v = fmax(v, red[r][li]);
+                    hmax[((size_t)(s0 + m0 + m) * nz + z) * CP + col] = v;
+                }
+                __syncthreads();
+            }
+        } else if (run < nkc * 4) {
+            const int lane = lg * 16 + li;
+#pragma unroll
+            for (int m = 0; m < kRotMC; ++m)
+                if (m < mc) {
+                    unsigned *o = reinterpret_cast<unsigned *>(
+                                      bsl + ((((size_t)(s0 + m0 + m) * nkc + kc) * CB + cb) * 2) * kNS * 64 + lane) + w;
+#pragma unroll
+                    for (int sd = 0; sd < kNS; ++sd) {
+                        o[(0 * kNS + sd) * 64 * 4] = ure[m][sd];
+                        o[(1 * kNS + sd) * 64 * 4] = uim[m][sd];
+                    }
+                }
         }
     }
 }
@@ -770,8 +829,10 @@ __global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npai
 //   acc[c][l][0|1] (the wave's 16 rows x column block c) = sum over K2 and
 //   digit pairs i + j = l of A_i(row, K2) B_j(K2, col) for Re|Im of B,
 // exact in int32.  A digits are gathered from the digit table of the roots of
-// unity: Tsh[j] holds the kNS digits of cos (bytes 0-3) and sin (bytes 4-7)
-// of 2 pi j / L, Rsh[p] the cell of pair p reduced modulo the electron mesh
+// unity: Tsh[j] holds the kNS digits of cos and sin of 2 pi j / L as byte
+// pairs (cos_l, sin_l) for levels l = 0..3 (bytes 2l, 2l + 1), so a level's
+// dword of two pairs is one byte permute of their table entries; Rsh[p] the
+// cell of pair p reduced modulo the electron mesh
 // (bytes 0-2), so the phase index of row q is (c . Rsh[p]) mod L with
 // c_d = q_d L / N_d (cpk, byte-packed; phase_index).  Lane (li, lg) holds
 // K2 = 64 kk + 16 lg + 4 w + b of chunk kk: pair 32 kk + 8 lg + 2 w + b / 2.
@@ -804,6 +865,14 @@ __device__ __forceinline__ void i8_copy_chunk(const v4i *__restrict__ bst, size_
         lds_dma16(src + r * 64 + lane, dst + r * 64);
     }
 }
+// Dword w of the four level operands from the table entries of pairs 2w
+// (t0) and 2w + 1 (t1): level l = bytes (cos_l, sin_l) of each.
+__device__ __forceinline__ void level_dwords(uint2 t0, uint2 t1, v4i (&A)[kNS], int w) {
+    A[0][w] = (int)__builtin_amdgcn_perm(t1.x, t0.x, 0x05040100u);
+    A[1][w] = (int)__builtin_amdgcn_perm(t1.x, t0.x, 0x07060302u);
+    A[2][w] = (int)__builtin_amdgcn_perm(t1.y, t0.y, 0x05040100u);
+    A[3][w] = (int)__builtin_amdgcn_perm(t1.y, t0.y, 0x07060302u);
+}
 template <int CB>
 __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t bkc, int nkc, const uint2 *Tsh,
                                              const unsigned *Rsh, unsigned cpk, unsigned lmul, int negL, v4i *buf,
@@ -829,13 +898,17 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
         const uint2 rp = *reinterpret_cast<const uint2 *>(Rsh + kk * 32 + lg * 8 + 2 * w);
         const uint2 t0 = Tsh[phase_index(cpk, rp.x, lmul, negL)];
         const uint2 t1 = Tsh[phase_index(cpk, rp.y, lmul, negL)];
-        unsigned o0, o1, o2, o3;
-        transpose4x4_bytes(t0.x, t0.y, t1.x, t1.y, o0, o1, o2, o3);
-        A[0][w] = (int)o0; A[1][w] = (int)o1; A[2][w] = (int)o2; A[3][w] = (int)o3;
+        level_dwords(t0, t1, A, w);
     };
-    // all products of chunk B with A; dword j of chunk kn's operands is
-    // gathered after the products of B digit j
+    // all products of chunk B with A, with chunk kn's A operands gathered in
+    // three stages between them, so the two dependent LDS reads of the
+    // gather (cells, then table entries) are in flight while the matrix
+    // cores work: the cells of the 8 pairs before digit 0, the table entries
+    // after its products, the byte transposes after those of digits 1 and 2
     auto chunk = [&](const v4i *B, const v4i (&A)[NS], int kn, v4i (&nA)[NS]) {
+        const uint4 ra = *reinterpret_cast<const uint4 *>(Rsh + kn * 32 + lg * 8);
+        const uint4 rb = *reinterpret_cast<const uint4 *>(Rsh + kn * 32 + lg * 8 + 4);
+        uint2 t[8];
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
             v4i Bre[CB], Bim[CB];
@@ -851,7 +924,14 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
                     acc[c][i + j][0] = mfma_i8(A[i], Bre[c], acc[c][i + j][0]);
                     acc[c][i + j][1] = mfma_i8(A[i], Bim[c], acc[c][i + j][1]);
                 }
-            gather(kn, j, nA);
+            if (j == 0) {
+                const unsigned rr[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) t[e] = Tsh[phase_index(cpk, rr[e], lmul, negL)];
+            } else if (j < 3) {
+#pragma unroll
+                for (int w = 2 * (j - 1); w < 2 * j; ++w) level_dwords(t[2 * w], t[2 * w + 1], nA, w);
+            }
         }
     };
     v4i A0[NS], A1[NS];
@@ -874,6 +954,121 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
     __syncthreads();
 }
 
+// Epilogue of the fused mixed-precision kernel for up to 8 bands: lane
+// (row li, lk) takes in-window ranks lk and lk + 4 of its k' row together, so
+// the row's Y values and u_q(s, sp) are read once for both.  Everything that
+// does not depend on Y -- the phonon row of q, the in-window count and bands,
+// the output offset -- is set up before the contraction (EpiRow), so those
+// loads hide behind the matrix work.  The |g|^2 go to stage[off - sbase + ...] after the last
+// read of Y by the wave (stage may be the wave's own Y rows).  Same
+// expressions and order as ibzk_epilogue_row, so the same values.
+struct EpiRow {
+    PhRow ph;
+    int nn = 0, na = 0, nb = 0;  // in-window count; bands of ranks lk and lk + 4
+    int64_t off = 0;
+    const double2 *Ukm = nullptr;
+};
+__device__ __forceinline__ void epi_row_setup(const PairArgs &a, int row, int q0, int q1, int q2, int lk, int st,
+                                              EpiRow &e) {
+    if (row >= a.nwv) return;
+    e.ph = phonon_row(a, q0, q1, q2);
+    e.nn = a.nwin[row];
+    // wband holds nb entries per row (0 past the in-window count)
+    e.na = lk < a.nb ? a.wband[row * a.nb + lk] : 0;
+    e.nb = lk + 4 < a.nb ? a.wband[row * a.nb + lk + 4] : 0;
+    e.off = a.offs[row];
+    if (e.ph.lq) e.Ukm = a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb;
+}
+template <int NBM>
+__device__ __forceinline__ void epi_load_v(const PairArgs &a, int row, const EpiRow &e, int lk, double2 (&va)[NBM],
+                                           double2 (&vb)[NBM]) {
+    const int nb = a.nb;
+    const bool ha = lk < e.nn, hb = lk + 4 < e.nn;
+    const double2 *Va = a.Vkp + ((size_t)row * nb + e.na) * nb, *Vb = a.Vkp + ((size_t)row * nb + e.nb) * nb;
+#pragma unroll
+    for (int np = 0; np < NBM; ++np) {
+        va[np] = np < nb && ha ? Va[np] : make_double2(0.0, 0.0);
+        vb[np] = np < nb && hb ? Vb[np] : make_double2(0.0, 0.0);
+    }
+}
+template <int NBM, int NBRM>
+__device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, const double2 *Yr, const EpiRow &e, int lk,
+                                                double *stage, int64_t sbase) {
+    if (lk >= e.nn) return;
+    const int nb = a.nb, nbr = a.nbr;
+    const bool hb = lk + 4 < e.nn;
+    double2 va[NBM], vb[NBM];
+    epi_load_v<NBM>(a, row, e, lk, va, vb);
+    double2 za[NBRM], zb[NBRM];
+#pragma unroll
+    for (int sp = 0; sp < NBRM; ++sp) {
+        double2 ta = make_double2(0.0, 0.0), tb = make_double2(0.0, 0.0);
+        if (sp < nbr)
+#pragma unroll
+            for (int np = 0; np < NBM; ++np)
+                if (np < nb) {
+                    const double2 y = Yr[np + nb * sp];
+                    ta.x = fma(va[np].x, y.x, ta.x);
+                    ta.x = fma(va[np].y, y.y, ta.x);
+                    ta.y = fma(va[np].x, y.y, ta.y);
+                    ta.y = fma(-va[np].y, y.x, ta.y);
+                    tb.x = fma(vb[np].x, y.x, tb.x);
+                    tb.x = fma(vb[np].y, y.y, tb.x);
+                    tb.y = fma(vb[np].x, y.y, tb.y);
+                    tb.y = fma(-vb[np].y, y.x, tb.y);
+                }
+        za[sp] = ta;
+        zb[sp] = tb;
+    }
+    double2 ua = make_double2(0.0, 0.0), ub = make_double2(0.0, 0.0);
+    if (e.ph.lq)
+#pragma unroll
+        for (int i = 0; i < NBM; ++i)
+            if (i < nb) {
+                const double2 t = cmulc(va[i], e.Ukm[i]);
+                ua.x += t.x;
+                ua.y += t.y;
+                const double2 t2 = cmulc(vb[i], e.Ukm[i]);
+                ub.x += t2.x;
+                ub.y += t2.y;
+            }
+    double ga[NBRM], gb[NBRM];
+#pragma unroll
+    for (int s = 0; s < NBRM; ++s) {
+        ga[s] = gb[s] = 0.0;
+        if (s < nbr) {
+            double2 gA = make_double2(0.0, 0.0), gB = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int sp = 0; sp < NBRM; ++sp)
+                if (sp < nbr) {
+                    const double2 u = e.ph.uq[s * nbr + sp];
+                    cfma(gA, u, za[sp]);
+                    cfma(gB, u, zb[sp]);
+                }
+            if (e.ph.lq) {
+                cfma(gA, e.ph.lq[s], ua);
+                cfma(gB, e.ph.lq[s], ub);
+            }
+            const double w = e.ph.omq[s];
+            if (w != 0.0) {
+                ga[s] = 0.5 * (gA.x * gA.x + gA.y * gA.y) / w * kG2Unit;
+                gb[s] = 0.5 * (gB.x * gB.x + gB.y * gB.y) / w * kG2Unit;
+            }
+        }
+    }
+    // the wave's reads of Y are all issued above (LDS serves a wave's
+    // requests in order); keep the compiler from moving stores up
+    asm volatile("" ::: "memory");
+    double *o = stage + (e.off - sbase) + (int64_t)lk * nbr;
+#pragma unroll
+    for (int s = 0; s < NBRM; ++s)
+        if (s < nbr) o[s] = ga[s];
+    if (hb)
+#pragma unroll
+        for (int s = 0; s < NBRM; ++s)
+            if (s < nbr) o[4 * nbr + s] = gb[s];
+}
+
 // Fused mixed-precision pair kernel: the cell-pair contraction of i8_pair_core,
 // FP64 recombination of the digit levels with the column scale of B, and the
 // FP64 rotation / |g|^2 epilogue.  One wave owns 16 k' rows and all CP
@@ -885,6 +1080,7 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
     constexpr int NS = kNS;
     constexpr int CP = 16 * CB;
     constexpr int BCH = CB * 2 * NS * 64;  // v4i per K2 chunk of one state
+    constexpr bool STAGED = NBM > 0 && NBM <= 8;
     extern __shared__ v4i smem8[];
     uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
@@ -906,6 +1102,15 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
         q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
         cpk = (unsigned)((q0 * a.Lf0) % a.L) | ((unsigned)((q1 * a.Lf1) % a.L) << 8) |
               ((unsigned)((q2 * a.Lf2) % a.L) << 16);
+    }
+    EpiRow er;
+    int64_t b0 = 0, b1 = 0;  // the wave's range of the output row
+    if constexpr (STAGED) {
+        epi_row_setup(a, r, q0, q1, q2, lg, st, er);
+        if (row_base < a.nwv) {
+            b0 = a.offs[row_base];
+            b1 = row_base + 16 < a.nwv ? a.offs[row_base + 16] : a.nprocs;
+        }
     }
     v4i acc[CB][NS][2];
     i8_pair_core<CB>(bst, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
@@ -933,20 +1138,16 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
         }
     }
     __syncthreads();
-    if constexpr (NBM > 0 && NBM <= 8) {
+    if constexpr (STAGED) {
         // the |g|^2 of the wave's 16 rows are one contiguous range of the
         // output: staged in the wave's own Y rows (after their last read),
         // then written out by consecutive lanes, so HBM sees whole lines
         // rather than 8-byte pieces at the branch stride
         double *stage = reinterpret_cast<double *>(Y);
-        const int64_t b0 = row_base < a.nwv ? a.offs[row_base] : 0;
-        ibzk_epilogue_row<NBM, NBRM, true>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow, stage, b0);
+        if (r < a.nwv) epi_pair_staged<NBM, NBRM>(a, r, Y + li * YS, er, lg, stage, b0);
         __syncthreads();
-        if (row_base < a.nwv) {
-            const int64_t b1 = row_base + 16 < a.nwv ? a.offs[row_base + 16] : a.nprocs;
-            const int cnt = (int)(b1 - b0);
-            for (int i = lane; i < cnt; i += 64) __builtin_nontemporal_store(stage[i], outrow + b0 + i);
-        }
+        const int cnt = (int)(b1 - b0);
+        for (int i = lane; i < cnt; i += 64) __builtin_nontemporal_store(stage[i], outrow + b0 + i);
     } else {
         ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
     }
@@ -3240,7 +3441,7 @@ struct IbzkGroup {
 struct IbzkScratch {
     DBuf<double2> gk, h, phiT, Y;
     DBuf<v4i> bphi, bsl;
-    DBuf<double> bscale;
+    DBuf<double> bscale, hmax;
 };
 
 }  // namespace
@@ -3282,6 +3483,9 @@ struct elph_ibzk_plan {
 };
 
 namespace {
+
+// workgroups along the K runs of k_rotate_digits (4 runs of 8 pairs per chunk)
+int64_t rot_digits_nz(const elph_ibzk_plan *p) { return (p->nkc * 4 + kRotRuns - 1) / kRotRuns; }
 
 template <int CB, int RB>
 void launch_ibzk_y_fp64(hipStream_t s, const PairArgs &a, int st0, int ns, double2 *Y) {
@@ -3599,6 +3803,7 @@ void ibzk_scratch(const elph_ibzk_plan *p, IbzkScratch &sc, int64_t nkb, int64_t
     if (p->precision == ELPH_MIXED) {
         get(sc.bphi, gk_bphi_count(c));
         get(sc.bscale, nst * p->CP);
+        get(sc.hmax, nst * rot_digits_nz(p) * p->CP);
         get(sc.bsl, nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
     }
     if (p->general) get(sc.Y, general_chunk(p, nst) * p->nwv * p->CP);
@@ -3636,11 +3841,22 @@ void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStr
     const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb, nslices = nbr * c->nwsg;
     launch_gkRp(c, s, g.nkb, g.kv.p, sc.gk.p, nb2 * nslices, p->precision == ELPH_MIXED, sc.phiT.p, sc.bphi.p);
     if (p->precision == ELPH_MIXED) {
-        if (g.nst > 0)
-            hipLaunchKernelGGL(k_rotate_digits, dim3((unsigned)g.nst, (unsigned)(p->CP / 16)), dim3(256), 0, s,
-                               (int)nb, (int)nbr, (int)p->npair, (int)p->nkc, (int)(p->CP / 16), (int)p->CP, sc.gk.p,
-                               nb2 * nslices, g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.bscale.p,
-                               sc.bsl.p);
+        if (g.nst > 0) {
+            const dim3 grid((unsigned)g.nkb, (unsigned)(p->CP / 16), (unsigned)rot_digits_nz(p));
+            const size_t lds = (size_t)nb * nb * sizeof(double2);
+            auto go = [&](auto kmax, auto kdig) {
+                hipLaunchKernelGGL(kmax, grid, dim3(256), lds, s, (int)nb, (int)nbr, (int)p->npair, (int)p->nkc,
+                                   (int)(p->CP / 16), (int)p->CP, (int)g.nst, sc.gk.p, nb2 * nslices, g.st_kb.p,
+                                   g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.hmax.p, sc.bscale.p, sc.bsl.p);
+                HCHECK(hipGetLastError());
+                hipLaunchKernelGGL(kdig, grid, dim3(256), lds, s, (int)nb, (int)nbr, (int)p->npair, (int)p->nkc,
+                                   (int)(p->CP / 16), (int)p->CP, (int)g.nst, sc.gk.p, nb2 * nslices, g.st_kb.p,
+                                   g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.hmax.p, sc.bscale.p, sc.bsl.p);
+            };
+            if (nb <= 8) go(k_rotate_digits<false, 8>, k_rotate_digits<true, 8>);
+            else if (nb <= 16) go(k_rotate_digits<false, 16>, k_rotate_digits<true, 16>);
+            else go(k_rotate_digits<false, 0>, k_rotate_digits<true, 0>);
+        }
         HCHECK(hipGetLastError());
     } else {
         launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)p->npair, (int)p->Kp, (int)p->CP, sc.gk.p,
@@ -3810,12 +4026,13 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
                 signed char dr[kNS], di[kNS];
                 split_digits<kNS>(T[j].x, dr);
                 split_digits<kNS>(T[j].y, di);
-                unsigned lo = 0, hi = 0;
-                for (int sd = 0; sd < kNS; ++sd) {
-                    lo |= (unsigned)(unsigned char)dr[sd] << (8 * sd);
-                    hi |= (unsigned)(unsigned char)di[sd] << (8 * sd);
-                }
-                tdig[j] = make_uint2(lo, hi);
+                // digit pairs (cos, sin) of levels 0, 1 | 2, 3: the 16-bit
+                // halves the A operands take (i8_pair_core)
+                unsigned w[2] = {0u, 0u};
+                for (int sd = 0; sd < kNS; ++sd)
+                    w[sd >> 1] |= ((unsigned)(unsigned char)dr[sd] | (unsigned)(unsigned char)di[sd] << 8)
+                                  << (16 * (sd & 1));
+                tdig[j] = make_uint2(w[0], w[1]);
             }
             p->Tdig.upload(tdig, s);
             // cells reduced modulo the electron mesh, packed one byte per component
@@ -3832,6 +4049,9 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
             HCHECK(hipGetLastError());
         }
         require_lds(ctx, ibzk_pair_lds(p.get()), "the ibzk pair kernel");
+        if (precision == ELPH_MIXED)
+            require_lds(ctx, (size_t)nb * nb * sizeof(double2) + 4 * kRotRuns * 16 * sizeof(double),
+                        "the mixed-precision B-operand kernel");
         // the int8 digits of gwann, once per context, so that runs allocate no
         // context data
         if (precision == ELPH_MIXED) ensure_gwann_digits(ctx);

@@ -78,20 +78,6 @@ __device__ __forceinline__ v4i as_v4i(const unsigned (&w)[4]) {
     return v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
 }
 
-// 4x4 byte transpose: o_e byte b = a_b byte e (v_perm_b32 selects from the
-// 8 bytes {S0:S1}, S1 = bytes 0-3).
-__device__ __forceinline__ void transpose4x4_bytes(unsigned a0, unsigned a1, unsigned a2, unsigned a3, unsigned &o0,
-                                                   unsigned &o1, unsigned &o2, unsigned &o3) {
-    const unsigned t0 = __builtin_amdgcn_perm(a1, a0, 0x05010400u);
-    const unsigned t1 = __builtin_amdgcn_perm(a1, a0, 0x07030602u);
-    const unsigned t2 = __builtin_amdgcn_perm(a3, a2, 0x05010400u);
-    const unsigned t3 = __builtin_amdgcn_perm(a3, a2, 0x07030602u);
-    o0 = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
-    o1 = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
-    o2 = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
-    o3 = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
-}
-
 // Phase-table index (c . R) mod L of byte-packed coefficients c (c_d < L <= 255)
 // and cell R (R_d < 256): one 4-way byte dot product, x <= 3 * 255 * 255 < 2^18,
 // then the quotient floor(x / L) as the high word of x * ceil(2^32 / L), which
