@@ -3465,9 +3465,20 @@ struct elph_ibzk_plan {
     DBuf<int64_t> offs;
     DBuf<double2> Vkp, u, uf, T, lp, lpf;
     DBuf<double> om, omf;
-    // per-run scratch of plan_run
-    IbzkScratch scr;
+    // per-run scratch of plan_run; FP64 with device output and several groups:
+    // the preparation of group i + 1 (g(k,Rp), h) runs on prep_stream into
+    // the other scratch set while the pair kernel of group i runs
+    IbzkScratch scr, scr2;
     DBuf<double> g2stage;
+    hipStream_t prep_stream = nullptr;
+    hipEvent_t ev_prep[2] = {nullptr, nullptr}, ev_pair[2] = {nullptr, nullptr};
+    ~elph_ibzk_plan() {
+        for (int i = 0; i < 2; ++i) {
+            if (ev_prep[i]) (void)hipEventDestroy(ev_prep[i]);
+            if (ev_pair[i]) (void)hipEventDestroy(ev_pair[i]);
+        }
+        if (prep_stream) (void)hipStreamDestroy(prep_stream);
+    }
     // mixed precision: digit table of the phases (all q of the electron mesh)
     int64_t nkc = 0;
     DBuf<uint2> Tdig;
@@ -3836,7 +3847,7 @@ size_t ibzk_noalloc_bytes(const elph_ibzk_plan *p, int64_t nkb, int64_t nst, boo
 }
 
 // One group: g(k,Rp) of its k, h = U_k g / phwsdeg, (mixed) digits, pair stage.
-void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
+void ibzk_prep(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s) {
     elph_ctx *c = p->ctx;
     const int64_t nb = p->nb, nbr = p->nbr, nb2 = nb * nb, nslices = nbr * c->nwsg;
     launch_gkRp(c, s, g.nkb, g.kv.p, sc.gk.p, nb2 * nslices, p->precision == ELPH_MIXED, sc.phiT.p, sc.bphi.p);
@@ -3862,6 +3873,10 @@ void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStr
         launch_rotate_h(s, (int)g.nst, (int)nb, (int)nbr, (int)p->npair, (int)p->Kp, (int)p->CP, sc.gk.p,
                         nb2 * nslices, g.st_kb.p, g.st_m.p, g.Ukb.p, c->degq.p, p->pairs.p, sc.h.p);
     }
+}
+
+void ibzk_enqueue(elph_ibzk_plan *p, const IbzkGroup &g, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
+    ibzk_prep(p, g, sc, s);
     pair_launch(p, g, sc, s, out, timed);
 }
 
@@ -4080,6 +4095,10 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
 int64_t elph_ibzk_plan_nprocs(const elph_ibzk_plan *p) { return p ? p->nprocs : -1; }
 
 int elph_ibzk_plan_destroy(elph_ibzk_plan *p) {
+    if (p) {
+        (void)hipSetDevice(p->ctx->device);
+        if (p->prep_stream) (void)hipStreamSynchronize(p->prep_stream);
+    }
     delete p;
     return ELPH_OK;
 }
@@ -4106,13 +4125,41 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
         ibzk_reserve(p, p->scr, nkb, nst);
         if (!on_device) p->g2stage.alloc(nst * p->nprocs);
         HCHECK(hipEventRecord(c->ev_c, s));
-        for (auto &g : groups) {
-            ibzk_enqueue(p, *g, p->scr, s, on_device ? g2 : p->g2stage.p, true);
-            if (!on_device) {
-                for (int64_t j = 0; j < g->nst; ++j)
-                    HCHECK(hipMemcpyAsync(g2 + g->slot[j] * p->nprocs, p->g2stage.p + j * p->nprocs,
-                                          p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
-                HCHECK(hipStreamSynchronize(s));
+        if (on_device && groups.size() > 1 && p->precision == ELPH_FP64) {
+            // group i's preparation on prep_stream into scratch set i % 2,
+            // its pair kernel on the context's stream after it; the
+            // preparation of group i + 2 waits for the pair kernel of group i.
+            // (FP64 only: the FP64 g(k,Rp) contraction fills the matrix
+            // cores' idle time; the mixed pair kernel slows by more than its
+            // preparation takes when they share the device.)
+            ibzk_reserve(p, p->scr2, nkb, nst);
+            if (!p->prep_stream) {
+                HCHECK(hipStreamCreateWithFlags(&p->prep_stream, hipStreamNonBlocking));
+                for (int i = 0; i < 2; ++i) {
+                    HCHECK(hipEventCreateWithFlags(&p->ev_prep[i], hipEventDisableTiming));
+                    HCHECK(hipEventCreateWithFlags(&p->ev_pair[i], hipEventDisableTiming));
+                }
+            }
+            hipStream_t ps = p->prep_stream;
+            HCHECK(hipStreamWaitEvent(ps, c->ev_c, 0));
+            for (size_t i = 0; i < groups.size(); ++i) {
+                IbzkScratch &sc = (i & 1) ? p->scr2 : p->scr;
+                if (i >= 2) HCHECK(hipStreamWaitEvent(ps, p->ev_pair[i & 1], 0));
+                ibzk_prep(p, *groups[i], sc, ps);
+                HCHECK(hipEventRecord(p->ev_prep[i & 1], ps));
+                HCHECK(hipStreamWaitEvent(s, p->ev_prep[i & 1], 0));
+                pair_launch(p, *groups[i], sc, s, g2, true);
+                HCHECK(hipEventRecord(p->ev_pair[i & 1], s));
+            }
+        } else {
+            for (auto &g : groups) {
+                ibzk_enqueue(p, *g, p->scr, s, on_device ? g2 : p->g2stage.p, true);
+                if (!on_device) {
+                    for (int64_t j = 0; j < g->nst; ++j)
+                        HCHECK(hipMemcpyAsync(g2 + g->slot[j] * p->nprocs, p->g2stage.p + j * p->nprocs,
+                                              p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
+                    HCHECK(hipStreamSynchronize(s));
+                }
             }
         }
         HCHECK(hipEventRecord(c->ev_d, s));
