@@ -384,7 +384,7 @@ def ibzq_bench(args, w, el, ph, mat):
 
 def _batched_material(i, budget):
     from elphbolt_amd import synth
-    return synth.multi_material(i, budget_bytes=budget, bands_in_window=4.0)
+    return synth.multi_material(i, budget_bytes=budget, bands_in_window=4.0, pair_floor=True)
 
 
 def start_batched_generation(args):
@@ -519,8 +519,9 @@ def main():
     ap.add_argument("--serving-pool-gb", type=float, default=128.0)
     ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--profile", action="store_true",
-                    help="config-1 pair kernels only (no serving, batched or CPU sections), after a "
-                         "device-to-device copy of --calibration-copy-gb GiB, for hardware-counter runs")
+                    help="config-1 kernels only (no serving, batched or CPU sections; ibzq unless "
+                         "--no-ibzq), after a device-to-device copy of --calibration-copy-gb GiB, for "
+                         "hardware-counter runs")
     ap.add_argument("--calibration-copy-gb", type=float, default=1.0)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-transport", action="store_true")
@@ -539,7 +540,7 @@ def main():
                                    "described in DESIGN.md, not run)"}))
         sys.exit(2)
     if args.profile:
-        args.no_batched = args.no_serving = args.no_cpu_baseline = args.no_ibzq = True
+        args.no_batched = args.no_serving = args.no_cpu_baseline = True
     gen = None if args.no_batched else start_batched_generation(args)
 
     from elphbolt_amd import bz, interactions, synth

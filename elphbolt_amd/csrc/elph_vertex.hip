@@ -476,9 +476,12 @@ __device__ __forceinline__ void phase_gemm_core(const double2 *__restrict__ hst,
     gemm_core_f64<CB, RB, KC>(hst, ldb, Kp, aop, buf0, buf1, cr, ci);
 }
 
-// PACKED: phase indices by pair_phase_packed (L and the mesh up to 255).
+// One tile (state st, k' rows 128 rblk .. 128 rblk + 127) of the FP64 fused pair
+// kernel; PACKED: phase indices by pair_phase_packed (L and the mesh up to
+// 255).  The caller's workgroup is 256 threads with the dynamic LDS of
+// fused_launch; the LDS is free again when the call returns.
 template <int CB, int NBM, int NBRM, bool PACKED>
-__global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
+__device__ __forceinline__ void pair_fp64_tile(const PairArgs &a, int st, int rblk) {
     constexpr int KC = 32;
     constexpr int RB = 2;
     const int CP = 16 * CB;
@@ -490,8 +493,7 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
-    const int st = blockIdx.y;
-    const int row_base = blockIdx.x * 128 + wave * 32;
+    const int row_base = rblk * 128 + wave * 32;
     const double2 *hst = a.h + (size_t)st * a.Kp * CP;
 
     for (int i = tid; i < a.L; i += 256) Tsh[i] = a.T[i];
@@ -545,6 +547,10 @@ __global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
         ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + rb * 16 + li, li, lk, st, k0i, k1i, k2i, outrow);
         __syncthreads();
     }
+}
+template <int CB, int NBM, int NBRM, bool PACKED>
+__global__ __launch_bounds__(256) void k_pair_fp64(PairArgs a) {
+    pair_fp64_tile<CB, NBM, NBRM, PACKED>(a, blockIdx.y, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -961,7 +967,8 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
 // the output offset -- is set up before the contraction (EpiRow), so those
 // loads hide behind the matrix work.  The |g|^2 go to stage[off - sbase + ...] after the last
 // read of Y by the wave (stage may be the wave's own Y rows).  Same
-// expressions and order as ibzk_epilogue_row, so the same values.
+// expressions and order as ibzk_epilogue_row up to the last step, where
+// 0.5 / omega * unit is formed once for both ranks.
 struct EpiRow {
     PhRow ph;
     int nn = 0, na = 0, nb = 0;  // in-window count; bands of ranks lk and lk + 4
@@ -1049,10 +1056,13 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
                 cfma(gA, e.ph.lq[s], ua);
                 cfma(gB, e.ph.lq[s], ub);
             }
+            // one division for both ranks (the mixed path's tolerance covers
+            // the reordered rounding of 0.5 |g|^2 / omega * unit)
             const double w = e.ph.omq[s];
             if (w != 0.0) {
-                ga[s] = 0.5 * (gA.x * gA.x + gA.y * gA.y) / w * kG2Unit;
-                gb[s] = 0.5 * (gB.x * gB.x + gB.y * gB.y) / w * kG2Unit;
+                const double f = 0.5 * kG2Unit / w;
+                ga[s] = (gA.x * gA.x + gA.y * gA.y) * f;
+                gb[s] = (gB.x * gB.x + gB.y * gB.y) * f;
             }
         }
     }
@@ -1069,14 +1079,16 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
             if (s < nbr) o[4 * nbr + s] = gb[s];
 }
 
-// Fused mixed-precision pair kernel: the cell-pair contraction of i8_pair_core,
-// FP64 recombination of the digit levels with the column scale of B, and the
-// FP64 rotation / |g|^2 epilogue.  One wave owns 16 k' rows and all CP
-// columns of one state.
+// Fused mixed-precision pair tile (state st, k' rows 64 rblk .. 64 rblk +
+// 63): the cell-pair contraction of i8_pair_core, FP64 recombination of the
+// digit levels with the column scale of B, and the FP64 rotation / |g|^2
+// epilogue.  One wave owns 16 k' rows and all CP columns of the state.  The
+// caller's workgroup is 256 threads with the dynamic LDS of fused_launch; the
+// LDS is free again when the call returns.
 template <int CB, int NBM, int NBRM>
-__global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
-                                                    const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
-                                                    const double *__restrict__ bscale, int nkc) {
+__device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__restrict__ Tdig,
+                                             const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
+                                             const double *__restrict__ bscale, int nkc, int st, int rblk) {
     constexpr int NS = kNS;
     constexpr int CP = 16 * CB;
     constexpr int BCH = CB * 2 * NS * 64;  // v4i per K2 chunk of one state
@@ -1086,8 +1098,7 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const int st = blockIdx.y;
-    const int row_base = blockIdx.x * 64 + wave * 16;
+    const int row_base = rblk * 64 + wave * 16;
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
     const v4i *bst = bsl + (size_t)st * nkc * BCH;
     i8_copy_chunk<CB>(bst, BCH, 0, smem8);
@@ -1150,6 +1161,60 @@ __global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(Pa
         for (int i = lane; i < cnt; i += 64) __builtin_nontemporal_store(stage[i], outrow + b0 + i);
     } else {
         ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
+    }
+}
+template <int CB, int NBM, int NBRM>
+__global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8(PairArgs a, const uint2 *__restrict__ Tdig,
+                                                    const unsigned *__restrict__ Rpk, const v4i *__restrict__ bsl,
+                                                    const double *__restrict__ bscale, int nkc) {
+    pair_i8_tile<CB, NBM, NBRM>(a, Tdig, Rpk, bsl, bscale, nkc, blockIdx.y, blockIdx.x);
+}
+
+// One group's share of a multi-material pair launch: its arguments (plan,
+// group tables, operands, output), the mixed-precision tables, its row blocks
+// and the first tile of its range in the launch's tile numbering (tiles of a
+// group: state-major, then row block).
+struct PairJob {
+    PairArgs a;
+    const uint2 *Tdig;
+    const unsigned *Rpk;
+    const v4i *bsl;
+    const double *bscale;
+    int nkc, nrb;
+    int64_t tile0;
+};
+// the job whose tile range holds tile t (jobs ascending in tile0, tile0[0] = 0)
+__device__ __forceinline__ int job_of(const PairJob *__restrict__ jobs, int njobs, int64_t t) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// Multi-material forms of the fused pair kernels: persistent workgroups walk
+// the tiles t = blockIdx.x, blockIdx.x + gridDim.x, ... of all jobs of one
+// launch (the groups of every plan whose pair stage has this kernel shape);
+// the barrier frees the LDS for the next tile.
+template <int CB, int NBM, int NBRM>
+__global__ __launch_bounds__(256, pair_blocks_per_cu(CB, NBM)) void k_pair_i8_jobs(const PairJob *__restrict__ jobs,
+                                                                                    int njobs, int64_t ntiles) {
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PairJob &J = jobs[job_of(jobs, njobs, t)];
+        const int64_t local = t - J.tile0;
+        pair_i8_tile<CB, NBM, NBRM>(J.a, J.Tdig, J.Rpk, J.bsl, J.bscale, J.nkc, (int)(local / J.nrb),
+                                    (int)(local % J.nrb));
+        __syncthreads();
+    }
+}
+template <int CB, int NBM, int NBRM, bool PACKED>
+__global__ __launch_bounds__(256) void k_pair_fp64_jobs(const PairJob *__restrict__ jobs, int njobs, int64_t ntiles) {
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const PairJob &J = jobs[job_of(jobs, njobs, t)];
+        const int64_t local = t - J.tile0;
+        pair_fp64_tile<CB, NBM, NBRM, PACKED>(J.a, (int)(local / J.nrb), (int)(local % J.nrb));
+        __syncthreads();
     }
 }
 
@@ -1268,84 +1333,68 @@ __global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const dou
                                  a.st_kint[3 * st + 1], a.st_kint[3 * st + 2], outrow);
 }
 
-// General-path epilogue for larger nb, nbr: one state and G k' rows per block
-// (256 / G threads per row), with the row's Y, u_q and in-window rows of
-// U_k' staged in LDS:
+// General-path epilogue, one wave per k' row (four rows of one state per
+// workgroup, no workgroup barriers): the lanes take the (rank, sp) entries of
 //   Z(rank, sp) = sum_np conj(U_k'(n_rank, np)) Y(np + nb sp)
+// reading the row's Y and U_k' rows straight from memory, put Z (and the
+// polar overlaps) in the wave's LDS area, then take the (rank, s) entries of
 //   g(rank, s)  = sum_sp u_q(s, sp) Z(rank, sp) [+ lp(q,s) sum_i conj(U_k'(n,i)) U_k(m,i)]
-// written like ibzk_epilogue_row.
-__global__ __launch_bounds__(256) void k_ibzk_epi_lds(PairArgs a, int st0, const double2 *__restrict__ Y, int G) {
-    extern __shared__ double2 sm[];
-    const int nb = a.nb, nbr = a.nbr, nbnr = nb * nbr;
-    const int TPQ = 256 / G;
-    const int tid = threadIdx.x, sub = tid / TPQ, tt = tid % TPQ;
+// and write |g|^2 with consecutive lanes on consecutive outputs.  Products
+// in the order of ibzk_epilogue_row.  LDS: 4 (nb nbr + nb) double2.
+__global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, const double2 *__restrict__ Y) {
+    extern __shared__ double2 zsm[];
+    const int nb = a.nb, nbr = a.nbr;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int stl = blockIdx.y, st = st0 + stl;
-    const int row = blockIdx.x * G + sub;
-    const int per = 2 * nbnr + nbr * nbr + nb + nb * nb;
-    double2 *Ys = sm + sub * per, *Zs = Ys + nbnr, *us = Zs + nbnr, *un = us + nbr * nbr, *Vs = un + nb;
-    const bool live = sub < G && row < a.nwv;
-    const double2 *uq = nullptr, *lq = nullptr;
-    const double *omq = nullptr;
-    int nn = 0;
-    if (live) {
-        const int q0 = pos_mod(a.kp_int[3 * row] - a.st_kint[3 * st], a.Nel0);
-        const int q1 = pos_mod(a.kp_int[3 * row + 1] - a.st_kint[3 * st + 1], a.Nel1);
-        const int q2 = pos_mod(a.kp_int[3 * row + 2] - a.st_kint[3 * st + 2], a.Nel2);
-        if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0) {
-            const int iq = ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-            uq = a.u + (size_t)iq * nbr * nbr;
-            omq = a.om + (size_t)iq * nbr;
-            if (a.lp) lq = a.lp + (size_t)iq * nbr;
-        } else {
-            const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
-            uq = a.uf + (size_t)iq * nbr * nbr;
-            omq = a.omf + (size_t)iq * nbr;
-            if (a.lpf) lq = a.lpf + (size_t)iq * nbr;
+    const int row = blockIdx.x * 4 + wave;
+    if (row >= a.nwv) return;  // no workgroup barrier follows
+    double2 *Zs = zsm + wave * (nb * nbr + nb), *un = Zs + nb * nbr;
+    const int q0 = pos_mod(a.kp_int[3 * row] - a.st_kint[3 * st], a.Nel0);
+    const int q1 = pos_mod(a.kp_int[3 * row + 1] - a.st_kint[3 * st + 1], a.Nel1);
+    const int q2 = pos_mod(a.kp_int[3 * row + 2] - a.st_kint[3 * st + 2], a.Nel2);
+    const PhRow ph = phonon_row(a, q0, q1, q2);
+    const int nn = a.nwin[row];
+    const double2 *Yr = Y + ((size_t)stl * a.nwv + row) * a.CP;
+    const double2 *Vr = a.Vkp + (size_t)row * nb * nb;
+    for (int i = lane; i < nn * nbr; i += 64) {
+        const int rank = i % nn, sp = i / nn;
+        const double2 *V = Vr + (size_t)a.wband[row * nb + rank] * nb;
+        double2 z = make_double2(0.0, 0.0);
+        for (int np = 0; np < nb; ++np) {
+            const double2 v = V[np];
+            const double2 y = Yr[np + nb * sp];
+            z.x = fma(v.x, y.x, z.x);  // conj(v) * y
+            z.x = fma(v.y, y.y, z.x);
+            z.y = fma(v.x, y.y, z.y);
+            z.y = fma(-v.y, y.x, z.y);
         }
-        nn = a.nwin[row];
-        const double2 *Yr = Y + ((size_t)stl * a.nwv + row) * a.CP;
-        for (int i = tt; i < nbnr; i += TPQ) Ys[i] = Yr[i];
-        for (int i = tt; i < nbr * nbr; i += TPQ) us[i] = uq[i];
-        const double2 *Vr = a.Vkp + (size_t)row * nb * nb;
-        for (int i = tt; i < nn * nb; i += TPQ) Vs[i] = Vr[(size_t)a.wband[row * nb + i / nb] * nb + i % nb];
+        Zs[rank + nb * sp] = z;
     }
-    __syncthreads();
-    if (live) {
-        const double2 *Ukm = lq ? a.Ukb + ((size_t)a.st_kb[st] * nb + a.st_m[st]) * nb : nullptr;
-        for (int i = tt; i < nn * nbr; i += TPQ) {
-            const int rank = i % nn, sp = i / nn;
-            const double2 *V = Vs + rank * nb;
+    if (ph.lq) {
+        const double2 *Ukm = a.Ukb + ((size_t)a.st_kb[st] * nb + a.st_m[st]) * nb;
+        for (int rank = lane; rank < nn; rank += 64) {
+            const double2 *V = Vr + (size_t)a.wband[row * nb + rank] * nb;
             double2 z = make_double2(0.0, 0.0);
-            for (int np = 0; np < nb; ++np) {
-                const double2 t = cmulc(V[np], Ys[np + nb * sp]);
+            for (int i = 0; i < nb; ++i) {
+                const double2 t = cmulc(V[i], Ukm[i]);
                 z.x += t.x;
                 z.y += t.y;
             }
-            Zs[rank + nb * sp] = z;
+            un[rank] = z;
         }
-        if (lq)
-            for (int rank = tt; rank < nn; rank += TPQ) {
-                const double2 *V = Vs + rank * nb;
-                double2 z = make_double2(0.0, 0.0);
-                for (int i = 0; i < nb; ++i) {
-                    const double2 t = cmulc(V[i], Ukm[i]);
-                    z.x += t.x;
-                    z.y += t.y;
-                }
-                un[rank] = z;
-            }
     }
-    __syncthreads();
-    if (live) {
-        double *o = a.out + a.st_slot[st] * a.nprocs + a.offs[row];
-        for (int i = tt; i < nn * nbr; i += TPQ) {
-            const int rank = i / nbr, s = i % nbr;
-            double2 g = make_double2(0.0, 0.0);
-            for (int sp = 0; sp < nbr; ++sp) cfma(g, us[s * nbr + sp], Zs[rank + nb * sp]);
-            if (lq) cfma(g, lq[s], un[rank]);
-            const double w = omq[s];
-            o[(int64_t)rank * nbr + s] = (w == 0.0) ? 0.0 : 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit;
-        }
+    // the wave's LDS writes complete before its reads (a wave's LDS requests
+    // are served in order; the wait keeps it explicit)
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    double *o = a.out + a.st_slot[st] * a.nprocs + a.offs[row];
+    for (int i = lane; i < nn * nbr; i += 64) {
+        const int rank = i / nbr, s = i % nbr;
+        double2 g = make_double2(0.0, 0.0);
+        for (int sp = 0; sp < nbr; ++sp) cfma(g, ph.uq[s * nbr + sp], Zs[rank + nb * sp]);
+        if (ph.lq) cfma(g, ph.lq[s], un[rank]);
+        const double w = ph.omq[s];
+        o[i] = w != 0.0 ? 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit : 0.0;
     }
 }
 
@@ -1579,9 +1628,10 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
 // Rp contraction of the IBZ-q side (gReq) as an FP64 MFMA GEMM per branch s:
 //   out[q][(i, s, Re)] = sum_Rp gwann[(i,Re)][s][Rp] * phiT[Rp][q]
 // rows X = i + nb2 * Re; one wave per 16-row block and all 16 NCB q columns,
-// so one pass over gwann serves 16 NCB q.  The K loop issues the gwann loads
-// of KU steps of 4 rows before their matrix instructions (K2 is a multiple
-// of 4 KU; rows past 2 npair read nothing and have zero phases).
+// so one pass over gwann serves 16 NCB q.  The K loop loads the gwann rows of
+// the next KU steps of 4 rows while the matrix instructions of the current KU
+// steps run (K2 is a multiple of 4 KU; rows past 2 npair read nothing and
+// have zero phases).
 // ---------------------------------------------------------------------------
 template <int NCB, int KU>
 __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
@@ -1600,9 +1650,10 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
     const int64_t row = rb * 16 + li;
-    for (int k0 = 0; k0 < K2; k0 += 4 * KU) {
-        // folded gwann rows: G(2p) = g(R_p)/d + g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
-        double2 x[KU], y[KU];
+    // folded gwann rows of K steps k0 .. k0 + 4 KU - 1: G(2p) = g(R_p)/d +
+    // g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d') from x = g(R_p)/d and
+    // y = g(-R_p)/d'
+    auto load = [&](int k0, double2 (&x)[KU], double2 (&y)[KU]) {
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
             const int p = (k0 + 4 * u + lk) >> 1;
@@ -1619,6 +1670,8 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
                 }
             }
         }
+    };
+    auto steps = [&](int k0, const double2 (&x)[KU], const double2 (&y)[KU]) {
 #pragma unroll
         for (int u = 0; u < KU; ++u) {
             const int kk = k0 + 4 * u + lk;
@@ -1631,6 +1684,18 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
                 ci[c] = mfma_f64(a.y, b, ci[c]);
             }
         }
+    };
+    // the gwann rows of the next 4 KU K steps are loaded while the matrix
+    // instructions of the current ones run
+    double2 x0[KU], y0[KU], x1[KU], y1[KU];
+    load(0, x0, y0);
+    for (int k0 = 0;; k0 += 8 * KU) {
+        if (k0 + 4 * KU < K2) load(k0 + 4 * KU, x1, y1);
+        steps(k0, x0, y0);
+        if (k0 + 4 * KU >= K2) break;
+        if (k0 + 8 * KU < K2) load(k0 + 8 * KU, x0, y0);
+        steps(k0 + 4 * KU, x1, y1);
+        if (k0 + 8 * KU >= K2) break;
     }
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
@@ -3465,20 +3530,9 @@ struct elph_ibzk_plan {
     DBuf<int64_t> offs;
     DBuf<double2> Vkp, u, uf, T, lp, lpf;
     DBuf<double> om, omf;
-    // per-run scratch of plan_run; FP64 with device output and several groups:
-    // the preparation of group i + 1 (g(k,Rp), h) runs on prep_stream into
-    // the other scratch set while the pair kernel of group i runs
-    IbzkScratch scr, scr2;
+    // per-run scratch of plan_run
+    IbzkScratch scr;
     DBuf<double> g2stage;
-    hipStream_t prep_stream = nullptr;
-    hipEvent_t ev_prep[2] = {nullptr, nullptr}, ev_pair[2] = {nullptr, nullptr};
-    ~elph_ibzk_plan() {
-        for (int i = 0; i < 2; ++i) {
-            if (ev_prep[i]) (void)hipEventDestroy(ev_prep[i]);
-            if (ev_pair[i]) (void)hipEventDestroy(ev_pair[i]);
-        }
-        if (prep_stream) (void)hipStreamDestroy(prep_stream);
-    }
     // mixed precision: digit table of the phases (all q of the electron mesh)
     int64_t nkc = 0;
     DBuf<uint2> Tdig;
@@ -3548,14 +3602,10 @@ void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScrat
         dim3 eg((unsigned)((p->nwv + 63) / 64), (unsigned)ns);
         if (p->nb <= 8 && p->nbr <= 6) {
             hipLaunchKernelGGL((k_ibzk_epi<8, 6>), eg, dim3(256), 0, s, a, (int)s0, sc.Y.p);
-        } else if (p->nb <= 16 && p->nbr <= 12) {
-            hipLaunchKernelGGL((k_ibzk_epi<16, 12>), eg, dim3(256), 0, s, a, (int)s0, sc.Y.p);
         } else {
-            const int G = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
-            const size_t elds =
-                (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb + p->nb * p->nb) * sizeof(double2);
-            dim3 lg((unsigned)((p->nwv + G - 1) / G), (unsigned)ns);
-            hipLaunchKernelGGL(k_ibzk_epi_lds, lg, dim3(256), elds, s, a, (int)s0, sc.Y.p, G);
+            const size_t elds = (size_t)4 * (p->nb * p->nbr + p->nb) * sizeof(double2);
+            const dim3 wg((unsigned)((p->nwv + 3) / 4), (unsigned)ns);
+            hipLaunchKernelGGL(k_ibzk_epi_wave, wg, dim3(256), elds, s, a, (int)s0, sc.Y.p);
         }
         HCHECK(hipGetLastError());
     }
@@ -3582,19 +3632,13 @@ size_t ibzk_pair_lds(const elph_ibzk_plan *p) {
                                (size_t)p->nkc * 32 * sizeof(unsigned)
                          : (size_t)(2 * 32 * 16 * p->gcb + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4);
     size_t e = 0;
-    if (!(p->nb <= 16 && p->nbr <= 12)) {
-        const int64_t G = std::max<int64_t>(1, std::min<int64_t>(8, 256 / (p->nb * p->nbr)));
-        e = (size_t)G * (2 * p->nb * p->nbr + p->nbr * p->nbr + p->nb + p->nb * p->nb) * sizeof(double2);
-    }
+    if (!(p->nb <= 8 && p->nbr <= 6)) e = (size_t)4 * (p->nb * p->nbr + p->nb) * sizeof(double2);
     return std::max(y, e);
 }
 
-// Pair stage of one group on stream s; with `timed`, a pair of the context's
-// timing events brackets it on s (timing_flush adds the times to the
-// dominant-kernel total; nothing here waits).
-void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
-    elph_ctx *c = p->ctx;
-    const int64_t nst_batch = gr.nst, nprocs = p->nprocs;
+// Arguments of the pair stage of group gr (operands in sc, output rows at out).
+PairArgs pair_args(const elph_ibzk_plan *p, const IbzkGroup &gr, const IbzkScratch &sc, double *out) {
+    const elph_ctx *c = p->ctx;
     PairArgs a{};
     a.h = sc.h.p;
     a.Kp = (int)p->Kp;
@@ -3627,7 +3671,17 @@ void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStr
     a.nb = (int)p->nb;
     a.nbr = (int)p->nbr;
     a.out = out;
-    a.nprocs = nprocs;
+    a.nprocs = p->nprocs;
+    return a;
+}
+
+// Pair stage of one group on stream s; with `timed`, a pair of the context's
+// timing events brackets it on s (timing_flush adds the times to the
+// dominant-kernel total; nothing here waits).
+void pair_launch(elph_ibzk_plan *p, const IbzkGroup &gr, IbzkScratch &sc, hipStream_t s, double *out, bool timed) {
+    elph_ctx *c = p->ctx;
+    const int64_t nst_batch = gr.nst;
+    const PairArgs a = pair_args(p, gr, sc, out);
     const int CB = (int)(p->CP / 16);
     if (timed) timed_begin(c, s);
     if (p->general) {
@@ -3647,63 +3701,124 @@ int epilogue_class(int64_t nb, int64_t nbr) {
     return 0;
 }
 
+// Kernel shape of the fused pair stage of a plan: precision, column blocks,
+// epilogue class (FP64: the unrolled nb <= 16 form only where it keeps two
+// waves per SIMD, CB <= 2) and byte-packed phase indices (FP64).
+struct FusedShape {
+    int32_t precision = 0;
+    int CB = 0, e = 0;
+    bool packed = false;
+    bool operator<(const FusedShape &o) const {
+        return std::tie(precision, CB, e, packed) < std::tie(o.precision, o.CB, o.e, o.packed);
+    }
+};
+FusedShape fused_shape(const elph_ibzk_plan *p) {
+    FusedShape f;
+    f.precision = p->precision;
+    f.CB = (int)(p->CP / 16);
+    f.e = epilogue_class(p->nb, p->nbr);
+    if (p->precision != ELPH_MIXED && f.e == 2 && f.CB > 2) f.e = 0;
+    f.packed = p->precision != ELPH_MIXED && p->packed;
+    return f;
+}
+// Dynamic LDS of the fused pair kernels of plan p.
+size_t fused_lds(const elph_ibzk_plan *p) {
+    if (p->precision == ELPH_MIXED)
+        return std::max((size_t)2 * (p->CP / 16) * 2 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
+                            (size_t)p->nkc * 32 * sizeof(unsigned),
+                        (size_t)64 * (p->CP + 1) * sizeof(double2));
+    return std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
+                    (size_t)64 * (p->CP + 1) * sizeof(double2));
+}
+// Rows of k' per tile of the fused kernels.
+int64_t fused_rows(int32_t precision) { return precision == ELPH_MIXED ? 64 : 128; }
+
+// Calls go(kernel) with the fused pair kernel of shape f: the per-group form
+// (JOBS false) or the multi-material form (JOBS true); mixed precision, then FP64.
+template <bool JOBS, typename Go>
+void fused_dispatch_i8(const FusedShape &f, Go &&go) {
+    {
+        auto k8 = [&](auto cb, auto nbm, auto nbrm) {
+            constexpr int C = decltype(cb)::value, M = decltype(nbm)::value, R = decltype(nbrm)::value;
+            if constexpr (JOBS) go(k_pair_i8_jobs<C, M, R>);
+            else go(k_pair_i8<C, M, R>);
+        };
+        using std::integral_constant;
+        // (nb <= 8 and nbr <= 6 implies CB <= 3)
+        switch (f.CB * 4 + f.e) {
+            case 4 + 1: k8(integral_constant<int, 1>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 8 + 1: k8(integral_constant<int, 2>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 12 + 1: k8(integral_constant<int, 3>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 4 + 2: k8(integral_constant<int, 1>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 8 + 2: k8(integral_constant<int, 2>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 12 + 2: k8(integral_constant<int, 3>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 16 + 2: k8(integral_constant<int, 4>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 4: k8(integral_constant<int, 1>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 8: k8(integral_constant<int, 2>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 12: k8(integral_constant<int, 3>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 16: k8(integral_constant<int, 4>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            default: throw Error{ELPH_EINVAL, "mixed precision fused kernel holds numwannbands*numbranches <= 64"};
+        }
+    }
+}
+template <bool JOBS, typename Go>
+void fused_dispatch_fp64(const FusedShape &f, Go &&go) {
+    auto launch = [&](auto packed) {
+        constexpr bool P = decltype(packed)::value;
+        auto k64 = [&](auto cb, auto nbm, auto nbrm) {
+            constexpr int C = decltype(cb)::value, M = decltype(nbm)::value, R = decltype(nbrm)::value;
+            if constexpr (JOBS) go(k_pair_fp64_jobs<C, M, R, P>);
+            else go(k_pair_fp64<C, M, R, P>);
+        };
+        using std::integral_constant;
+        switch (f.CB * 4 + f.e) {
+            case 4 + 1: k64(integral_constant<int, 1>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 8 + 1: k64(integral_constant<int, 2>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 12 + 1: k64(integral_constant<int, 3>{}, integral_constant<int, 8>{}, integral_constant<int, 6>{}); break;
+            case 4 + 2: k64(integral_constant<int, 1>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 8 + 2: k64(integral_constant<int, 2>{}, integral_constant<int, 16>{}, integral_constant<int, 12>{}); break;
+            case 4: k64(integral_constant<int, 1>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 8: k64(integral_constant<int, 2>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 12: k64(integral_constant<int, 3>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 16: k64(integral_constant<int, 4>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 20: k64(integral_constant<int, 5>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            case 24: k64(integral_constant<int, 6>{}, integral_constant<int, 0>{}, integral_constant<int, 0>{}); break;
+            default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
+        }
+    };
+    if (f.packed) launch(std::true_type{});
+    else launch(std::false_type{});
+}
+
 void fused_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst_batch, IbzkScratch &sc, hipStream_t s, int CB) {
-    const size_t lds = std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
-                                (size_t)64 * (p->CP + 1) * sizeof(double2));
-    dim3 grid((unsigned)((p->nwv + 127) / 128), (unsigned)nst_batch);
+    (void)CB;
+    const FusedShape f = fused_shape(p);
+    const dim3 grid((unsigned)((p->nwv + fused_rows(p->precision) - 1) / fused_rows(p->precision)),
+                    (unsigned)nst_batch);
+    const size_t lds = fused_lds(p);
     if (p->precision == ELPH_MIXED) {
-        const size_t lds8 = std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) +
-                                         (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
-                                         (size_t)p->nkc * 32 * sizeof(unsigned),
-                                     (size_t)64 * (p->CP + 1) * sizeof(double2));
         const uint2 *at = p->Tdig.p;
         const unsigned *rpk = p->Rpk.p;
         const v4i *bs = sc.bsl.p;
-        grid.x = (unsigned)((p->nwv + 63) / 64);
         const double *bsc = sc.bscale.p;
         const int nkc = (int)p->nkc;
-        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds8, s, a, at, rpk, bs, bsc, nkc); };
-        // (nb <= 8 and nbr <= 6 implies CB <= 3)
-        switch (CB * 4 + epilogue_class(p->nb, p->nbr)) {
-            case 4 + 1: go(k_pair_i8<1, 8, 6>); break;
-            case 8 + 1: go(k_pair_i8<2, 8, 6>); break;
-            case 12 + 1: go(k_pair_i8<3, 8, 6>); break;
-            case 4 + 2: go(k_pair_i8<1, 16, 12>); break;
-            case 8 + 2: go(k_pair_i8<2, 16, 12>); break;
-            case 12 + 2: go(k_pair_i8<3, 16, 12>); break;
-            case 16 + 2: go(k_pair_i8<4, 16, 12>); break;
-            case 4: go(k_pair_i8<1, 0, 0>); break;
-            case 8: go(k_pair_i8<2, 0, 0>); break;
-            case 12: go(k_pair_i8<3, 0, 0>); break;
-            case 16: go(k_pair_i8<4, 0, 0>); break;
-            default: throw Error{ELPH_EINVAL, "mixed precision fused kernel holds numwannbands*numbranches <= 64"};
-        }
+        fused_dispatch_i8<false>(f, [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a, at, rpk, bs, bsc, nkc);
+        });
     } else {
-        auto go = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a); };
-        // the unrolled epilogue of nb <= 16, nbr <= 12 only where it keeps two
-        // waves per SIMD (CB <= 2)
-        int e = epilogue_class(p->nb, p->nbr);
-        if (e == 2 && CB > 2) e = 0;
-        auto launch = [&](auto packed) {
-            constexpr bool P = decltype(packed)::value;
-            switch (CB * 4 + e) {
-                case 4 + 1: go(k_pair_fp64<1, 8, 6, P>); break;
-                case 8 + 1: go(k_pair_fp64<2, 8, 6, P>); break;
-                case 12 + 1: go(k_pair_fp64<3, 8, 6, P>); break;
-                case 4 + 2: go(k_pair_fp64<1, 16, 12, P>); break;
-                case 8 + 2: go(k_pair_fp64<2, 16, 12, P>); break;
-                case 4: go(k_pair_fp64<1, 0, 0, P>); break;
-                case 8: go(k_pair_fp64<2, 0, 0, P>); break;
-                case 12: go(k_pair_fp64<3, 0, 0, P>); break;
-                case 16: go(k_pair_fp64<4, 0, 0, P>); break;
-                case 20: go(k_pair_fp64<5, 0, 0, P>); break;
-                case 24: go(k_pair_fp64<6, 0, 0, P>); break;
-                default: throw Error{ELPH_EINVAL, "FP64 fused kernel holds numwannbands*numbranches <= 96"};
-            }
-        };
-        if (p->packed) launch(std::true_type{});
-        else launch(std::false_type{});
+        fused_dispatch_fp64<false>(f, [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a); });
     }
+}
+
+// One launch of the multi-material fused kernel of shape f over the jobs at
+// d_jobs (njobs, ntiles tiles in all), with the largest LDS of their plans.
+void fused_jobs_launch(const FusedShape &f, const PairJob *d_jobs, int njobs, int64_t ntiles, size_t lds, int ncu,
+                       hipStream_t s) {
+    const int per_cu = f.precision == ELPH_MIXED ? pair_blocks_per_cu(f.CB, f.e == 1 ? 8 : (f.e == 2 ? 16 : 0)) : 2;
+    const unsigned nblk = (unsigned)std::min<int64_t>(ntiles, (int64_t)std::max(1, ncu) * per_cu);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(nblk), dim3(256), lds, s, d_jobs, njobs, ntiles); };
+    if (f.precision == ELPH_MIXED) fused_dispatch_i8<true>(f, go);
+    else fused_dispatch_fp64<true>(f, go);
 }
 
 // Host side of a group of IBZ k and their in-window states (the contents of
@@ -4095,10 +4210,6 @@ int elph_ibzk_plan_create(elph_ctx *ctx, const elph_electrons *el, const elph_ph
 int64_t elph_ibzk_plan_nprocs(const elph_ibzk_plan *p) { return p ? p->nprocs : -1; }
 
 int elph_ibzk_plan_destroy(elph_ibzk_plan *p) {
-    if (p) {
-        (void)hipSetDevice(p->ctx->device);
-        if (p->prep_stream) (void)hipStreamSynchronize(p->prep_stream);
-    }
     delete p;
     return ELPH_OK;
 }
@@ -4125,41 +4236,13 @@ int elph_ibzk_plan_run(elph_ibzk_plan *p, int64_t first, int64_t last, int64_t *
         ibzk_reserve(p, p->scr, nkb, nst);
         if (!on_device) p->g2stage.alloc(nst * p->nprocs);
         HCHECK(hipEventRecord(c->ev_c, s));
-        if (on_device && groups.size() > 1 && p->precision == ELPH_FP64) {
-            // group i's preparation on prep_stream into scratch set i % 2,
-            // its pair kernel on the context's stream after it; the
-            // preparation of group i + 2 waits for the pair kernel of group i.
-            // (FP64 only: the FP64 g(k,Rp) contraction fills the matrix
-            // cores' idle time; the mixed pair kernel slows by more than its
-            // preparation takes when they share the device.)
-            ibzk_reserve(p, p->scr2, nkb, nst);
-            if (!p->prep_stream) {
-                HCHECK(hipStreamCreateWithFlags(&p->prep_stream, hipStreamNonBlocking));
-                for (int i = 0; i < 2; ++i) {
-                    HCHECK(hipEventCreateWithFlags(&p->ev_prep[i], hipEventDisableTiming));
-                    HCHECK(hipEventCreateWithFlags(&p->ev_pair[i], hipEventDisableTiming));
-                }
-            }
-            hipStream_t ps = p->prep_stream;
-            HCHECK(hipStreamWaitEvent(ps, c->ev_c, 0));
-            for (size_t i = 0; i < groups.size(); ++i) {
-                IbzkScratch &sc = (i & 1) ? p->scr2 : p->scr;
-                if (i >= 2) HCHECK(hipStreamWaitEvent(ps, p->ev_pair[i & 1], 0));
-                ibzk_prep(p, *groups[i], sc, ps);
-                HCHECK(hipEventRecord(p->ev_prep[i & 1], ps));
-                HCHECK(hipStreamWaitEvent(s, p->ev_prep[i & 1], 0));
-                pair_launch(p, *groups[i], sc, s, g2, true);
-                HCHECK(hipEventRecord(p->ev_pair[i & 1], s));
-            }
-        } else {
-            for (auto &g : groups) {
-                ibzk_enqueue(p, *g, p->scr, s, on_device ? g2 : p->g2stage.p, true);
-                if (!on_device) {
-                    for (int64_t j = 0; j < g->nst; ++j)
-                        HCHECK(hipMemcpyAsync(g2 + g->slot[j] * p->nprocs, p->g2stage.p + j * p->nprocs,
-                                              p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
-                    HCHECK(hipStreamSynchronize(s));
-                }
+        for (auto &g : groups) {
+            ibzk_enqueue(p, *g, p->scr, s, on_device ? g2 : p->g2stage.p, true);
+            if (!on_device) {
+                for (int64_t j = 0; j < g->nst; ++j)
+                    HCHECK(hipMemcpyAsync(g2 + g->slot[j] * p->nprocs, p->g2stage.p + j * p->nprocs,
+                                          p->nprocs * sizeof(double), hipMemcpyDeviceToHost, s));
+                HCHECK(hipStreamSynchronize(s));
             }
         }
         HCHECK(hipEventRecord(c->ev_d, s));
@@ -4291,9 +4374,13 @@ IbzkRows ibzk_rows(elph_ibzk_plan *p) {
 
 // ===========================================================================
 // Batched multi-material driver: the ibzk work of many plans (materials of
-// different band, mode and WS counts) captured once into a hipGraph whose
-// branches run on several streams; the pair-stage tile shape of each size
-// class is picked by timing the candidates on the class's first group.
+// different band, mode and WS counts) captured once into a hipGraph.  The
+// preparations of all groups (g(k,Rp) and the B operand) run on several
+// streams; the pair stage of every group on the fused path then runs in one
+// multi-material launch per kernel shape (k_pair_i8_jobs / k_pair_fp64_jobs
+// over a work list of all those groups), and groups on the column-tiled path
+// run their pair stage on their stream.  The pair-stage tile shape of each
+// size class is picked by timing the candidates on the class's first group.
 // ===========================================================================
 struct elph_batch {
     struct Item {
@@ -4303,6 +4390,23 @@ struct elph_batch {
         std::vector<std::unique_ptr<IbzkGroup>> groups;
         int stream = 0, cls = -1;
         double work = 0;
+        // fused path: each group's pair operands (B digits and column scales,
+        // or the folded h) in buffers of its own, read by the shared launch
+        struct Ops {
+            DBuf<v4i> bsl;
+            DBuf<double> bscale;
+            DBuf<double2> h;
+        };
+        std::vector<std::unique_ptr<Ops>> ops;
+    };
+    // one multi-material pair launch: the jobs of one fused kernel shape
+    struct JobList {
+        FusedShape shape;
+        std::vector<PairJob> host;
+        DBuf<PairJob> dev;
+        int64_t ntiles = 0;
+        size_t lds = 0;
+        int ncu = 0;
     };
     struct Class {
         int32_t precision = 0;
@@ -4312,11 +4416,13 @@ struct elph_batch {
     };
     int device = 0;
     std::vector<hipStream_t> streams;
-    std::vector<hipEvent_t> joins;
+    std::vector<hipEvent_t> joins, prepped;  // per stream: all work / fused preparations done
+    hipStream_t lstream = nullptr;           // the multi-material pair launches
     hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
     std::vector<Item> items;
     std::vector<Class> classes;
     std::vector<std::unique_ptr<IbzkScratch>> scr;  // per stream
+    std::vector<std::unique_ptr<JobList>> lists;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::string err;
@@ -4417,9 +4523,12 @@ int elph_batch_create(elph_batch **out, int device, int32_t nstreams) {
             hipEvent_t e;
             HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             b->joins.push_back(e);
+            HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            b->prepped.push_back(e);
             b->scr.push_back(std::make_unique<IbzkScratch>());
         }
         HCHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HCHECK(hipStreamCreateWithFlags(&b->lstream, hipStreamNonBlocking));
         HCHECK(hipEventCreate(&b->ev0));
         HCHECK(hipEventCreate(&b->ev1));
     } catch (const Error &e) {
@@ -4498,19 +4607,111 @@ int elph_batch_prepare(elph_batch *b, int32_t autotune_tiles) {
         }
         for (auto &it : b->items)
             for (auto &g : it.groups) ibzk_reserve(it.p, *b->scr[it.stream], g->nkb, g->nst);
+        // fused path: per-group operand buffers and the work lists of the
+        // multi-material launches (one per kernel shape)
+        std::map<FusedShape, size_t> shape2list;
+        b->lists.clear();
+        for (auto &it : b->items) {
+            it.ops.clear();
+            if (it.p->general) continue;
+            elph_ibzk_plan *p = it.p;
+            const FusedShape f = fused_shape(p);
+            auto fl = shape2list.find(f);
+            if (fl == shape2list.end()) {
+                auto jl = std::make_unique<elph_batch::JobList>();
+                jl->shape = f;
+                jl->ncu = (int)p->ctx->ncu;
+                fl = shape2list.emplace(f, b->lists.size()).first;
+                b->lists.push_back(std::move(jl));
+            }
+            elph_batch::JobList &jl = *b->lists[fl->second];
+            jl.lds = std::max(jl.lds, fused_lds(p));
+            const int nrb = (int)((p->nwv + fused_rows(p->precision) - 1) / fused_rows(p->precision));
+            for (auto &g : it.groups) {
+                auto o = std::make_unique<elph_batch::Item::Ops>();
+                IbzkScratch view;
+                if (p->precision == ELPH_MIXED) {
+                    o->bsl.alloc((size_t)g->nst * p->nkc * (p->CP / 16) * 2 * kNS * 64);
+                    o->bscale.alloc((size_t)g->nst * p->CP);
+                    view.bsl.borrow(o->bsl.p, o->bsl.n);
+                    view.bscale.borrow(o->bscale.p, o->bscale.n);
+                } else {
+                    o->h.alloc((size_t)g->nst * p->Kp * p->CP);
+                    view.h.borrow(o->h.p, o->h.n);
+                }
+                PairJob j{};
+                j.a = pair_args(p, *g, view, it.out);
+                j.Tdig = p->Tdig.p;
+                j.Rpk = p->Rpk.p;
+                j.bsl = view.bsl.p;
+                j.bscale = view.bscale.p;
+                j.nkc = (int)p->nkc;
+                j.nrb = nrb;
+                j.tile0 = jl.ntiles;
+                jl.ntiles += g->nst * nrb;
+                if (g->nst > 0) jl.host.push_back(j);
+                it.ops.push_back(std::move(o));
+            }
+        }
+        for (auto &jl : b->lists) jl->dev.upload(jl->host, b->streams[0]);
         HCHECK(hipDeviceSynchronize());
-        // capture: fork from stream 0, every stream runs its items, join on stream 0
+        // capture: fork from stream 0; every stream first prepares its fused
+        // items' groups (event `prepped`), then runs its column-tiled items;
+        // the multi-material pair launches run on lstream once every stream's
+        // fused preparations are done, alongside the column-tiled work; all
+        // join on stream 0
         hipStream_t s0 = b->streams[0];
         HCHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
         HCHECK(hipEventRecord(b->fork, s0));
         for (size_t i = 1; i < b->streams.size(); ++i) HCHECK(hipStreamWaitEvent(b->streams[i], b->fork, 0));
-        for (auto &it : b->items)
-            for (auto &g : it.groups)
-                ibzk_enqueue(it.p, *g, *b->scr[it.stream], b->streams[it.stream], it.out, false);
+        HCHECK(hipStreamWaitEvent(b->lstream, b->fork, 0));
+        std::vector<const elph_batch::Item *> order2;
+        for (const auto &it : b->items)
+            if (!it.p->general) order2.push_back(&it);
+        for (const auto &it : b->items)
+            if (it.p->general) order2.push_back(&it);
+        bool general_phase = false;
+        for (const elph_batch::Item *itp : order2) {
+            const elph_batch::Item &it = *itp;
+            if (it.p->general && !general_phase) {
+                general_phase = true;
+                for (size_t i = 0; i < b->streams.size(); ++i) HCHECK(hipEventRecord(b->prepped[i], b->streams[i]));
+            }
+            hipStream_t st = b->streams[it.stream];
+            IbzkScratch &ss = *b->scr[it.stream];
+            for (size_t gi = 0; gi < it.groups.size(); ++gi) {
+                const IbzkGroup &g = *it.groups[gi];
+                if (it.p->general) {
+                    ibzk_enqueue(it.p, g, ss, st, it.out, false);
+                    continue;
+                }
+                // the stream's preparation buffers, the group's operand buffers
+                IbzkScratch view;
+                view.gk.borrow(ss.gk.p, ss.gk.n);
+                view.phiT.borrow(ss.phiT.p, ss.phiT.n);
+                view.bphi.borrow(ss.bphi.p, ss.bphi.n);
+                view.hmax.borrow(ss.hmax.p, ss.hmax.n);
+                elph_batch::Item::Ops &o = *it.ops[gi];
+                view.bsl.borrow(o.bsl.p, o.bsl.n);
+                view.bscale.borrow(o.bscale.p, o.bscale.n);
+                view.h.borrow(o.h.p, o.h.n);
+                ibzk_prep(it.p, g, view, st);
+            }
+        }
+        if (!general_phase)
+            for (size_t i = 0; i < b->streams.size(); ++i) HCHECK(hipEventRecord(b->prepped[i], b->streams[i]));
+        for (size_t i = 0; i < b->streams.size(); ++i) HCHECK(hipStreamWaitEvent(b->lstream, b->prepped[i], 0));
+        for (auto &jl : b->lists)
+            if (!jl->host.empty())
+                fused_jobs_launch(jl->shape, jl->dev.p, (int)jl->host.size(), jl->ntiles, jl->lds, jl->ncu,
+                                  b->lstream);
+        HCHECK(hipGetLastError());
         for (size_t i = 1; i < b->streams.size(); ++i) {
             HCHECK(hipEventRecord(b->joins[i], b->streams[i]));
             HCHECK(hipStreamWaitEvent(s0, b->joins[i], 0));
         }
+        HCHECK(hipEventRecord(b->joins[0], b->lstream));
+        HCHECK(hipStreamWaitEvent(s0, b->joins[0], 0));
         HCHECK(hipStreamEndCapture(s0, &b->graph));
         HCHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     } catch (const Error &e) {
@@ -4553,11 +4754,14 @@ int elph_batch_destroy(elph_batch *b) {
     for (auto st : b->streams) (void)hipStreamSynchronize(st);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->lstream) (void)hipStreamSynchronize(b->lstream);
     for (auto e : b->joins) (void)hipEventDestroy(e);
+    for (auto e : b->prepped) (void)hipEventDestroy(e);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (auto st : b->streams) (void)hipStreamDestroy(st);
+    if (b->lstream) (void)hipStreamDestroy(b->lstream);
     delete b;
     return ELPH_OK;
 }
@@ -4583,7 +4787,7 @@ namespace {
 // g(Re, q) for nq device q vectors in cell-pair form over Rp: the real
 // cos/sin table of the pairs (KT q per launch, zero-padded rows and columns)
 // against gwann folded on the fly (1/gwsdeg included).
-constexpr int kGReqNCB = 8, kGReqKU = 2;  // 128 q per pass over gwann; loads two K steps ahead
+constexpr int kGReqNCB = 4, kGReqKU = 4;  // 64 q per pass over gwann (two waves per SIMD); loads four K steps ahead
 void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, int64_t out_qstride) {
     const int64_t nb2 = c->nb * c->nb;
     const int64_t KT = 16 * kGReqNCB;

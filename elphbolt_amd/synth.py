@@ -340,13 +340,17 @@ def ws_count(n: int) -> int:
     return _WS_COUNT[n]
 
 
-def multi_material_draw(i: int, budget_bytes: float = 4e9, seed_base: int = MULTI_SEED_BASE):
+def multi_material_draw(i: int, budget_bytes: float = 4e9, seed_base: int = MULTI_SEED_BASE,
+                        pair_floor: bool = False):
     """The size draw of material i of the multi-material generator, without
     building it: dict(nat, nb, n, wv, a, nws, rng) where rng is the stream
     positioned after the draw (the basis is drawn from it next).  The WS grid
     n^3 (n uniform in 4..9), the fine mesh and the lattice constant are drawn
     once; atoms and bands are drawn again until the vertex fits the budget, so
-    the grid sizes stay uniform."""
+    the grid sizes stay uniform.  pair_floor: also drawn again for one atom on
+    a 12^3 mesh -- a single atom keeps the full cubic group, whose 72 IBZ k
+    give at most 72 * 1728 = 124416 k-q pairs, below the batched workload's
+    floor of 0.5e6 (every other draw reaches it)."""
     rng = np.random.default_rng(seed_base + i)
     n = int(rng.integers(4, 10))
     wv = int(rng.choice([12, 16, 20, 24]))
@@ -355,20 +359,22 @@ def multi_material_draw(i: int, budget_bytes: float = 4e9, seed_base: int = MULT
     for _ in range(10000):
         nat = int(rng.integers(1, 9))
         nb = int(rng.integers(4, 27))
+        if pair_floor and nat == 1 and wv == 12:
+            continue
         if 16 * nb * nb * 3 * nat * nws * nws <= budget_bytes:
             return dict(nat=nat, nb=nb, n=n, wv=wv, a=a, nws=nws, rng=rng)
     raise ValueError("no draw within the budget")
 
 
 def multi_material(i: int, budget_bytes: float = 4e9, vertex: str = "device", window: bool = True,
-                   seed_base: int = MULTI_SEED_BASE, bands_in_window=None) -> Material:
+                   seed_base: int = MULTI_SEED_BASE, bands_in_window=None, pair_floor: bool = False) -> Material:
     """Material i of the multi-material generator: atoms per cell uniform in
     1..8 (3..24 branches), Wannier bands uniform in 4..26, WS grid n^3 with n
     uniform in 4..9 (93..857 cells for fcc), fine mesh from {12, 16, 20, 24}^3.
     Atoms and bands are drawn again (same stream) until 16 nb^2 nbr nws^2 <=
     budget_bytes.
     The vertex follows the law of config 0 (device-generated by default)."""
-    d = multi_material_draw(i, budget_bytes, seed_base)
+    d = multi_material_draw(i, budget_bytes, seed_base, pair_floor)
     nat, nb, n, wv, a, rng = d["nat"], d["nb"], d["n"], d["wv"], d["a"], d["rng"]
     masses = [_MASSES[j % len(_MASSES)] for j in range(nat)]
     basis = np.zeros((3, nat))

@@ -33,7 +33,7 @@ KP_WINDOW = 8
 def batch_mats():
     out = []
     for i in PICKS:
-        mat = synth.multi_material(i, budget_bytes=BUDGET, vertex="host", bands_in_window=4.0)
+        mat = synth.multi_material(i, budget_bytes=BUDGET, vertex="host", bands_in_window=4.0, pair_floor=True)
         w = Wannier(mat)
         sym = bz.Symmetry(mat.lattvecs, mat.basis, mat.masses, mat.wvmesh)
         el = bz.Electrons(w, sym, mat.wvmesh, mat.enref, mat.fsthick)
