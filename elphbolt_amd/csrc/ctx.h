@@ -109,10 +109,11 @@ struct elph_ctx {
     DBuf<double> ginvdeg;
     int64_t npairk = 0, gk2p = 0;
     // the same over the Rp cells of the g set (rcells_g) for the cell-pair
-    // form of gReq: rgpair, rgpcell, rginvdeg = 1 / gwsdeg
+    // form of gReq: rgpair, rgpcell, rgpwt = 1 / gwsdeg of R_p and of -R_p
+    // (0 when missing) per pair
     DBuf<int2> rgpair;
     DBuf<int> rgpcell;
-    DBuf<double> rginvdeg;
+    DBuf<double2> rgpwt;
     int64_t npairg = 0;
     // D(q) interpolation terms (wrapped cells of src/wannier.f90:631-650)
     DBuf<int> term_off, term_iuc, term_T, term_div;

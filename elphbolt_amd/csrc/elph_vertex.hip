@@ -1628,74 +1628,81 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
 // Rp contraction of the IBZ-q side (gReq) as an FP64 MFMA GEMM per branch s:
 //   out[q][(i, s, Re)] = sum_Rp gwann[(i,Re)][s][Rp] * phiT[Rp][q]
 // rows X = i + nb2 * Re; one wave per 16-row block and all 16 NCB q columns,
-// so one pass over gwann serves 16 NCB q.  The K loop loads the gwann rows of
-// the next KU steps of 4 rows while the matrix instructions of the current KU
-// steps run (K2 is a multiple of 4 KU; rows past 2 npair read nothing and
-// have zero phases).
+// so one pass over gwann serves 16 NCB q.  Each wave copies the gwann rows of
+// its next 16 K rows global -> LDS asynchronously (its own two buffers, no
+// workgroup barrier) while the matrix instructions of the current 16 run;
+// K2 is a multiple of 16, and rows past 2 npair have zero weight and phases.
 // ---------------------------------------------------------------------------
-template <int NCB, int KU>
+template <int NCB>
 __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
-                                              const int2 *__restrict__ pairs, const double *__restrict__ invdeg,
+                                              const int2 *__restrict__ pairs, const double2 *__restrict__ pwt,
                                               const double *__restrict__ phiR, int ldphi, int K2, int nq,
                                               double2 *__restrict__ out, int64_t out_qstride) {
+    constexpr int KI = 16;  // K rows (8 cell pairs) per block of the K loop
+    extern __shared__ double2 gsm[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int s = blockIdx.y;
     const int64_t nrows = (int64_t)nb2 * nwsk;
     const int64_t rb = (int64_t)blockIdx.x * 4 + wave;
-    if (rb * 16 >= nrows) return;
+    if (rb * 16 >= nrows) return;  // the wave's LDS and copies are its own: no workgroup barrier
     const int64_t sstride = nrows * nbr;  // stride of Rp in gwann
     const double2 *A = gw + (size_t)s * nrows;
+    double2 *buf = gsm + wave * 2 * (KI / 2) * 2 * 16;  // [2][pair j][cell R_p | -R_p][row]
     d4 cr[NCB], ci[NCB];
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
     const int64_t row = rb * 16 + li;
-    // folded gwann rows of K steps k0 .. k0 + 4 KU - 1: G(2p) = g(R_p)/d +
-    // g(-R_p)/d', G(2p+1) = i (g(R_p)/d - g(-R_p)/d') from x = g(R_p)/d and
-    // y = g(-R_p)/d'
-    auto load = [&](int k0, double2 (&x)[KU], double2 (&y)[KU]) {
+    // global -> LDS copy of the gwann rows of K rows k0 .. k0 + KI - 1 (cells
+    // R_p and -R_p of pairs k0/2 .. k0/2 + 7, the wave's 16 rows) into buffer
+    // b; a missing -R_p copies R_p (its weight is 0 below), rows past the end
+    // copy the tile's first row, pairs past npair copy cell 0 (zero phases)
+    auto copy = [&](int k0, int b) {
+        double2 *dst = buf + b * (KI / 2) * 2 * 16;
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
-            const int p = (k0 + 4 * u + lk) >> 1;
-            x[u] = y[u] = make_double2(0.0, 0.0);
-            if (row < nrows && p < npair) {
+        for (int t = 0; t < (KI / 2) * 2 * 16 / 64; ++t) {
+            const int e = t * 64 + lane;
+            const int r = e & 15, c = (e >> 4) & 1, jp = e >> 5;
+            const int p = (k0 >> 1) + jp;
+            int cell = 0;
+            if (p < npair) {
                 const int2 pr = pairs[p];
-                const double2 ga = A[(size_t)pr.x * sstride + row];
-                const double da = invdeg[pr.x];
-                x[u] = make_double2(ga.x * da, ga.y * da);
-                if (pr.y >= 0) {
-                    const double2 gb = A[(size_t)pr.y * sstride + row];
-                    const double db = invdeg[pr.y];
-                    y[u] = make_double2(gb.x * db, gb.y * db);
-                }
+                cell = (c && pr.y >= 0) ? pr.y : pr.x;
             }
+            const int64_t rr = rb * 16 + r < nrows ? rb * 16 + r : rb * 16;
+            lds_dma16(A + (size_t)cell * sstride + rr, dst + t * 64);
         }
     };
-    auto steps = [&](int k0, const double2 (&x)[KU], const double2 (&y)[KU]) {
+    // the 4 K steps of buffer b: G(2p) = g(R_p)/d + g(-R_p)/d',
+    // G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
+    auto steps = [&](int k0, int b) {
+        const double2 *src = buf + b * (KI / 2) * 2 * 16;
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
+        for (int u = 0; u < KI / 4; ++u) {
             const int kk = k0 + 4 * u + lk;
-            const double2 a = (kk & 1) ? make_double2(y[u].y - x[u].y, x[u].x - y[u].x)
-                                       : make_double2(x[u].x + y[u].x, x[u].y + y[u].y);
+            const int p = kk >> 1, jp = (4 * u + lk) >> 1;
+            const double2 w = p < npair ? pwt[p] : make_double2(0.0, 0.0);
+            const double dx = w.x, dy = w.y;
+            const double2 gx = src[(jp * 2 + 0) * 16 + li], gy = src[(jp * 2 + 1) * 16 + li];
+            const double2 x = make_double2(gx.x * dx, gx.y * dx), y = make_double2(gy.x * dy, gy.y * dy);
+            const double2 a = (kk & 1) ? make_double2(y.y - x.y, x.x - y.x) : make_double2(x.x + y.x, x.y + y.y);
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
-                const double b = phiR[(size_t)kk * ldphi + c * 16 + li];
-                cr[c] = mfma_f64(a.x, b, cr[c]);
-                ci[c] = mfma_f64(a.y, b, ci[c]);
+                const double bv = phiR[(size_t)kk * ldphi + c * 16 + li];
+                cr[c] = mfma_f64(a.x, bv, cr[c]);
+                ci[c] = mfma_f64(a.y, bv, ci[c]);
             }
         }
     };
-    // the gwann rows of the next 4 KU K steps are loaded while the matrix
-    // instructions of the current ones run
-    double2 x0[KU], y0[KU], x1[KU], y1[KU];
-    load(0, x0, y0);
-    for (int k0 = 0;; k0 += 8 * KU) {
-        if (k0 + 4 * KU < K2) load(k0 + 4 * KU, x1, y1);
-        steps(k0, x0, y0);
-        if (k0 + 4 * KU >= K2) break;
-        if (k0 + 8 * KU < K2) load(k0 + 8 * KU, x0, y0);
-        steps(k0 + 4 * KU, x1, y1);
-        if (k0 + 8 * KU >= K2) break;
+    // the copy of the next KI rows runs while the current ones multiply
+    copy(0, 0);
+    for (int k0 = 0; k0 < K2; k0 += KI) {
+        const int b = (k0 / KI) & 1;
+        if (k0 + KI < K2) copy(k0 + KI, b ^ 1);
+        // this block's copy is complete (and, with it, the next one's issued)
+        if (k0 + KI < K2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((KI / 2) * 2 * 16 / 64) : "memory");
+        else lds_dma_wait();
+        steps(k0, b);
     }
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
@@ -2790,9 +2797,12 @@ void ctx_init_device(elph_ctx *c, const double *Hwann, const double *Dphwann, co
             for (int d = 0; d < 3; ++d) pc[3 * q + d] = (int)c->rcells_g[pr[q].x + c->nwsg * d];
         c->rgpair.upload(pr, c->stream);
         c->rgpcell.upload(pc, c->stream);
-        std::vector<double> inv(c->nwsg);
-        for (int64_t q = 0; q < c->nwsg; ++q) inv[q] = 1.0 / (double)c->gwsdeg[q];
-        c->rginvdeg.upload(inv, c->stream);
+        // weights (1 / gwsdeg of R_p, of -R_p or 0 when it is missing) per pair
+        std::vector<double2> wt(c->npairg);
+        for (int64_t q = 0; q < c->npairg; ++q)
+            wt[q] = make_double2(1.0 / (double)c->gwsdeg[pr[q].x],
+                                 pr[q].y >= 0 ? 1.0 / (double)c->gwsdeg[pr[q].y] : 0.0);
+        c->rgpwt.upload(wt, c->stream);
     }
     build_phonon_terms(c);
     if (c->polar) build_polar_terms(c);
@@ -4787,11 +4797,11 @@ namespace {
 // g(Re, q) for nq device q vectors in cell-pair form over Rp: the real
 // cos/sin table of the pairs (KT q per launch, zero-padded rows and columns)
 // against gwann folded on the fly (1/gwsdeg included).
-constexpr int kGReqNCB = 4, kGReqKU = 4;  // 64 q per pass over gwann (two waves per SIMD); loads four K steps ahead
+constexpr int kGReqNCB = 8;  // 128 q per pass over gwann
 void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, int64_t out_qstride) {
     const int64_t nb2 = c->nb * c->nb;
     const int64_t KT = 16 * kGReqNCB;
-    const int64_t K2 = ((2 * c->npairg + 4 * kGReqKU - 1) / (4 * kGReqKU)) * (4 * kGReqKU);
+    const int64_t K2 = ((2 * c->npairg + 15) / 16) * 16;  // whole blocks of the K loop
     DBuf<double2> phiR;  // K2 x KT doubles, as double2 pairs for the zero fill
     phiR.alloc((K2 * KT + 1) / 2);
     double *phi = reinterpret_cast<double *>(phiR.p);
@@ -4803,10 +4813,11 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
         dim3 grid((unsigned)((c->npairg + 127) / 128), (unsigned)n);
         hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, c->stream, (int)n, (int)c->npairg, d_qvecs + q0, nq,
                            c->rgpcell.p, phi, (int64_t)1, KT);
-        hipLaunchKernelGGL((k_gReq<kGReqNCB, kGReqKU>), dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256), 0,
+        hipLaunchKernelGGL((k_gReq<kGReqNCB>), dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256),
+                           (size_t)4 * 2 * 8 * 2 * 16 * sizeof(double2),
                            c->stream,
                            c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->npairg, c->rgpair.p,
-                           c->rginvdeg.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
+                           c->rgpwt.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
         HCHECK(hipGetLastError());
     }
 }
