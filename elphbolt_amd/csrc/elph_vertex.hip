@@ -1485,28 +1485,34 @@ __global__ void k_phase_digits(int nkc, int ncb, const double *__restrict__ phi,
     for (int sd = 0; sd < kNS; ++sd) o[sd * 64] = as_v4i(u[sd]);
 }
 
-// Persistent workgroups: work item = (sp,Rp) slice x round of 4 row blocks
-// (wave w takes row block 4 round + w); workgroup b takes items b, b + grid,
-// ... and walks them as one stream of K chunks (item j, chunk kc), so the
-// copies of the next chunk run across item boundaries and an item's
-// recombination and stores overlap the next item's first loads.  All NCB
-// column blocks (16 k each).  out[k][slice][i] FP64 complex.  The phase
-// digits (the same for every slice) go global -> LDS one chunk ahead into two
-// buffers shared by the 4 waves; the wave's A digits (its rows of the folded
-// gwann, streamed from HBM) are loaded one chunk ahead into the second of two
-// register sets.
-template <int NCB>
-__global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig, const double *__restrict__ gscale,
+// Persistent workgroups: work item = (sp,Rp) slice x round of 4 / CS row
+// blocks; wave w takes row block (4 / CS) round + w / CS and column group
+// w % CS (NCBW column blocks of 16 k each, CS NCBW blocks per launch), so the
+// CS waves of one row block share its A loads through the caches and one pass
+// over the digits of gwann (the launch's HBM traffic) serves 16 CS NCBW k.
+// Workgroup b takes items b, b + grid, ... and walks them as one stream of K
+// chunks (item j, chunk kc), so the copies of the next chunk run across item
+// boundaries and an item's recombination and stores overlap the next item's
+// first loads.  out[k][slice][i] FP64 complex.  The phase digits (the same
+// for every slice) go global -> LDS one chunk ahead into two buffers shared by
+// the 4 waves; the wave's A digits (its rows of the folded gwann, streamed
+// from HBM) are loaded one chunk ahead into the second of two register sets.
+template <int NCBW, int CS>
+__global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *__restrict__ gdig, const double *__restrict__ gscale,
                                                     int nb2, int nrb, int nkc, int nslices,
                                                     const v4i *__restrict__ bphi, int nk, double2 *__restrict__ out,
                                                     int64_t out_kstride) {
     constexpr int NS = kNS;
+    constexpr int NCB = NCBW * CS;      // column blocks of the launch
+    constexpr int RBI = 4 / CS;         // row blocks of an item
+    static_assert(RBI * CS == 4, "the 4 waves split into row blocks x column groups");
     constexpr int BCH = NCB * NS * 64;  // v4i per chunk of phase digits
     static_assert(BCH % 256 == 0, "a chunk is a whole number of 1 KiB rows per wave");
     extern __shared__ v4i bsm[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lg = lane >> 4;
-    const int nround = (nrb + 3) / 4;
+    const int cg = wave % CS;           // the wave's column group
+    const int nround = (nrb + RBI - 1) / RBI;
     const int64_t nitems = (int64_t)nslices * nround;
     if ((int64_t)blockIdx.x >= nitems) return;  // uniform over the workgroup
     const int64_t nmine = (nitems - blockIdx.x + gridDim.x - 1) / gridDim.x;
@@ -1515,7 +1521,7 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
     auto item = [&](int64_t st, int &slice, int &rb) {
         const int64_t it = blockIdx.x + (st / nkc) * (int64_t)gridDim.x;
         slice = (int)(it / nround);
-        rb = (int)(it % nround) * 4 + wave;
+        rb = (int)(it % nround) * RBI + wave / CS;
     };
     auto copy_b = [&](int64_t st) {
         const v4i *src = bphi + (size_t)(st % nkc) * BCH;
@@ -1538,24 +1544,24 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
             Ai[sd] = ga[((size_t)kc * 2 * NS + 1 * NS + sd) * 64];
         }
     };
-    v4i acc[NCB][NS][2];
+    v4i acc[NCBW][NS][2];
     auto zero_acc = [&]() {
 #pragma unroll
-        for (int c = 0; c < NCB; ++c)
+        for (int c = 0; c < NCBW; ++c)
 #pragma unroll
             for (int l = 0; l < NS; ++l) { acc[c][l][0] = v4i{0, 0, 0, 0}; acc[c][l][1] = v4i{0, 0, 0, 0}; }
     };
     auto chunk = [&](int64_t st, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
-        const v4i *bb = bsm + (st & 1) * BCH + lane;
+        const v4i *bb = bsm + (st & 1) * BCH + cg * NCBW * NS * 64 + lane;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            v4i B[NCB];
+            v4i B[NCBW];
 #pragma unroll
-            for (int c = 0; c < NCB; ++c) B[c] = bb[(c * NS + j) * 64];
+            for (int c = 0; c < NCBW; ++c) B[c] = bb[(c * NS + j) * 64];
 #pragma unroll
             for (int i = 0; i < NS - j; ++i)
 #pragma unroll
-                for (int c = 0; c < NCB; ++c) {
+                for (int c = 0; c < NCBW; ++c) {
                     acc[c][i + j][0] = mfma_i8(Ar[i], B[c], acc[c][i + j][0]);
                     acc[c][i + j][1] = mfma_i8(Ai[i], B[c], acc[c][i + j][1]);
                 }
@@ -1575,8 +1581,8 @@ __global__ __launch_bounds__(256, 2) void k_gkRp_i8(const v4i *__restrict__ gdig
 #pragma unroll
             for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
 #pragma unroll
-            for (int c = 0; c < NCB; ++c) {
-                const int col = c * 16 + li;
+            for (int c = 0; c < NCBW; ++c) {
+                const int col = (cg * NCBW + c) * 16 + li;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = rb * 16 + 4 * lg + r;
@@ -3233,8 +3239,12 @@ void elph::ensure_gwann_digits(elph_ctx *c) {
     HCHECK(hipStreamSynchronize(c->stream));
 }
 
-int64_t elph::gk_phi_count(const elph_ctx *c) { return c->gk2p * 64 / 2; }
-int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * 4 * kNS * 64; }
+// k per launch of the Re contraction: FP64 k_gkRp<4> (4 column blocks, one
+// wave per 16 rows) and the int8 k_gkRp_i8<4, 2> (two column groups of 4
+// blocks per pass over the digits of gwann)
+constexpr int64_t kGkKT = 64, kGkKTMixed = 128;
+int64_t elph::gk_phi_count(const elph_ctx *c) { return c->gk2p * kGkKTMixed / 2; }
+int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * (kGkKTMixed / 16) * kNS * 64; }
 
 // Batched gkRp into device memory: out[k] (nb*nb, nbr*nwsg) for kvecs (nk,3)
 // column-major on the device; FP64 or int8-digit (mixed) contraction, both in
@@ -3243,14 +3253,14 @@ int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * 4 * kNS
 void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
                         int64_t out_kstride, bool mixed, double2 *phiT, v4i *bphi) {
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
-    const int64_t KT = 64;
+    const int64_t KT = mixed ? kGkKTMixed : kGkKT;
     double *phi = reinterpret_cast<double *>(phiT);
     if (!mixed && c->npairk * (int64_t)sizeof(double4) > 160 * 1024)
         throw Error{ELPH_EINVAL, "FP64 Re contraction holds up to 5120 cell pairs (10240 electron WS cells)"};
     for (int64_t k0 = 0; k0 < nk; k0 += KT) {
         const int64_t n = std::min(KT, nk - k0);
-        // real phases in [K2][k] layout (64 columns; unused rows and columns zero)
-        const int64_t nz = gk_phi_count(c);
+        // real phases in [K2][k] layout (KT columns; unused rows and columns zero)
+        const int64_t nz = c->gk2p * KT / 2;
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, phiT, nz);
         dim3 grid((unsigned)((c->npairk + 127) / 128), (unsigned)n);
         hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, st, (int)n, (int)c->npairk, d_kvecs + k0, nk,
@@ -3259,10 +3269,11 @@ void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_
             const int nt = (int)(c->gnkc * (KT / 16) * 64);
             hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->gnkc,
                                (int)(KT / 16), phi, (int)KT, bphi);
-            const int64_t nitems = nslices * ((c->gnrb + 3) / 4);
+            constexpr int NCBW = 4, CS = (int)(kGkKTMixed / 64);
+            const int64_t nitems = nslices * ((c->gnrb + 4 / CS - 1) / (4 / CS));
             const int64_t grid = std::min<int64_t>(nitems, 2 * std::max<int64_t>(c->ncu, 1));
-            hipLaunchKernelGGL(k_gkRp_i8<4>, dim3((unsigned)grid), dim3(256), 2 * 4 * kNS * 64 * sizeof(v4i), st,
-                               c->gdig.p, c->gscale.p,
+            hipLaunchKernelGGL((k_gkRp_i8<NCBW, CS>), dim3((unsigned)grid), dim3(256),
+                               2 * NCBW * CS * kNS * 64 * sizeof(v4i), st, c->gdig.p, c->gscale.p,
                                (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
                                out + (size_t)k0 * out_kstride, out_kstride);
         } else {
@@ -3919,7 +3930,8 @@ void ibzk_group_upload(const GroupHost &h, IbzkGroup &g, hipStream_t s) {
 
 void ibzk_groups(elph_ibzk_plan *p, int64_t first, int64_t last, bool on_device, int64_t *counts, hipStream_t s,
                  std::vector<std::unique_ptr<IbzkGroup>> &groups, bool compact = false) {
-    const int64_t KB = 64;  // IBZ k per gkRp batch (the k_gkRp<4> column tile)
+    // IBZ k per group: the k of one launch of the Re contraction
+    const int64_t KB = p->precision == ELPH_MIXED ? kGkKTMixed : kGkKT;
     for (const GroupHost &h : ibzk_group_hosts(p, first, last, on_device, counts, compact, KB)) {
         auto g = std::make_unique<IbzkGroup>();
         ibzk_group_upload(h, *g, s);
