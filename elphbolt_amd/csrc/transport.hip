@@ -346,6 +346,8 @@ struct XArgs {
     const int *kp_int;
     const int64_t *offs;
     const double *om, *omf;
+    int nblk;       // blocks of kXPer x 256 processes per row
+    int states_x;   // 1: blockIdx.x = state, blockIdx.y = block of the row; 0: the reverse
     const int4 *st_k;      // per in-window state: k (electron mesh ints)
     const double *st_e;    // its energy
     const int *st_row;     // its row in g2 / X
@@ -369,8 +371,12 @@ __global__ void k_bose(int n, const double *en, double T, double *out) {
 
 constexpr int kXPer = 2;  // processes per thread of k_ibzk_X (strided by the block size)
 
-// Processes of in-window states (blockIdx.y = the state's index among them,
-// st_row its row in g2 / X): per process the loop body of
+// Processes of in-window states (the state's index among them and the block
+// of its row from the grid, st_row its row in g2 / X).  With the states along
+// x, consecutive workgroups are the same processes of consecutive states, so
+// the tables of those k' rows (the delta-table heads, phonon energies and Bose
+// factors, the process map) are read from L2 by all states instead of being
+// streamed in full once per state.  Per process the loop body of
 // src/interactions.f90:2278-2332, X scaled by 2 pi / hbar (2351-2354), and the
 // block's sum of X+ + X- for the rates.  Block: 256 threads x kXPer
 // processes of one row.  The few energies that fall inside a (k', n)
@@ -378,13 +384,18 @@ constexpr int kXPer = 2;  // processes per thread of k_ibzk_X (strided by the bl
 // whole block, so no wave runs the table search for one or two lanes.
 // |g|^2 is read and X+-, and the state maps written, with nontemporal hints
 // (each is touched once; -12% against cached accesses, with kXPer 2 rather
-// than 4: -3%).
+// than 4: -3%).  The kernel is bound by the latency of its dependent table
+// reads (process map -> k' row -> phonon mode; delta-table head -> bin ->
+// piece), so each step's loads of all of a thread's processes, and of its
+// queued deltas, are issued together (addresses of processes past the row
+// end clamped into it, their results unused).
 __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     constexpr int QMAX = 2 * kXPer * 256;
     __shared__ double qe[QMAX];  // queued energies, then their delta values
     __shared__ int qp[QMAX];     // (point, band) pair of each queued energy
     __shared__ int qn;
-    const int ia = blockIdx.y;
+    const int ia = a.states_x ? blockIdx.x : blockIdx.y;
+    const int64_t blk = a.states_x ? blockIdx.y : blockIdx.x;
     const int4 k = a.st_k[ia];
     const size_t row = (size_t)a.st_row[ia] * a.nprocs;
     const double en_el = a.st_e[ia];
@@ -392,71 +403,157 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     __syncthreads();
     // 1. per process: phonon mode, occupations, state maps; the energies
     // e +- omega that fall inside their (k', n) tetrahedra range are queued
-    double gw[kXPer], occp[kXPer], occm[kXPer];
+    int64_t cc[kXPer];
+    bool live[kXPer];
+    int2 vm[kXPer];
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+        const int64_t c = (blk * kXPer + u) * 256 + threadIdx.x;
+        live[u] = c < a.nprocs;
+        cc[u] = live[u] ? c : a.nprocs - 1;
+        vm[u] = a.vmap[cc[u]];
+    }
+    int kr[kXPer][3];
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+        const int r = vm[u].x;
+        kr[u][0] = a.kp_int[3 * r];
+        kr[u][1] = a.kp_int[3 * r + 1];
+        kr[u][2] = a.kp_int[3 * r + 2];
+    }
+    double en_ph[kXPer], bose[kXPer], gw[kXPer], lo[kXPer], hi[kXPer];
+    int64_t sph[kXPer];
+#pragma unroll
+    for (int u = 0; u < kXPer; ++u) {
+        const int r = vm[u].x, n = vm[u].y >> 8, s = vm[u].y & 255;
+        // q = k' - k on the electron mesh (both components in [0, N))
+        int q0 = kr[u][0] - k.x, q1 = kr[u][1] - k.y, q2 = kr[u][2] - k.z;
+        q0 += q0 < 0 ? a.Nel0 : 0;
+        q1 += q1 < 0 ? a.Nel1 : 0;
+        q2 += q2 < 0 ? a.Nel2 : 0;
+        // the mode's energy and Bose factor: [coarse modes][fine modes]
+        const bool coarse =
+            a.mesh_ref == 1 || (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0);
+        const int iqc = a.mesh_ref == 1 ? (q2 * a.Nph1 + q1) * a.Nph0 + q0
+                                        : ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+        const int iqf = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
+        const size_t mi = coarse ? (size_t)iqc * a.nbr + s : (size_t)iqf * a.nbr + s;
+        en_ph[u] = (coarse ? a.om : a.omf)[mi];
+        bose[u] = a.bose[coarse ? mi : a.ncoarse + mi];
+        sph[u] = coarse ? (int64_t)iqc * a.nbr + s + 1 : -((int64_t)iqf * a.nbr + s + 1);
+        gw[u] = __builtin_nontemporal_load(a.g2 + row + cc[u]);
+        const DeltaHead *h = a.t.head + (r * a.t.nb + n);
+        lo[u] = h->lo;
+        hi[u] = h->hi;
+    }
+    double occp[kXPer], occm[kXPer];
     int slp[kXPer], slm[kXPer];
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
         slp[u] = slm[u] = -1;
-        gw[u] = 0.0;
         occp[u] = occm[u] = 0.0;
-        const int64_t c = ((int64_t)blockIdx.x * kXPer + u) * 256 + threadIdx.x;
-        if (c >= a.nprocs) continue;
-        const int2 vm = a.vmap[c];
-        const int r = vm.x, n = vm.y >> 8, s = vm.y & 255;
-        // q = k' - k on the electron mesh (both components in [0, N))
-        int q0 = a.kp_int[3 * r] - k.x, q1 = a.kp_int[3 * r + 1] - k.y, q2 = a.kp_int[3 * r + 2] - k.z;
-        q0 += q0 < 0 ? a.Nel0 : 0;
-        q1 += q1 < 0 ? a.Nel1 : 0;
-        q2 += q2 < 0 ? a.Nel2 : 0;
-        double en_ph;
-        int64_t sph;
-        size_t bidx;  // the mode's Bose factor: [coarse modes][fine modes]
-        if (a.mesh_ref == 1 || (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0)) {
-            const int iq = a.mesh_ref == 1 ? (q2 * a.Nph1 + q1) * a.Nph0 + q0
-                                           : ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
-            bidx = (size_t)iq * a.nbr + s;
-            en_ph = a.om[bidx];
-            sph = (int64_t)iq * a.nbr + s + 1;
-        } else {
-            const int iq = (q2 * a.Nel1 + q1) * a.Nel0 + q0;
-            en_ph = a.omf[(size_t)iq * a.nbr + s];
-            bidx = a.ncoarse + (size_t)iq * a.nbr + s;
-            sph = -((int64_t)iq * a.nbr + s + 1);
-        }
+        if (!live[u]) continue;
+        const int r = vm[u].x, n = vm[u].y >> 8;
         if (a.se) {
-            __builtin_nontemporal_store((int64_t)r * a.nb + n + 1, a.se + row + c);
-            __builtin_nontemporal_store(sph, a.sp + row + c);
+            __builtin_nontemporal_store((int64_t)r * a.nb + n + 1, a.se + row + cc[u]);
+            __builtin_nontemporal_store(sph[u], a.sp + row + cc[u]);
         }
-        if (en_ph >= 0.5e-3) {
-            gw[u] = __builtin_nontemporal_load(a.g2 + row + c);
-            const double bose = a.bose[bidx];
-            occp[u] = bose + 1.0 / (exp((en_el + en_ph - a.chempot) * a.ikT) + 1.0);
-            occm[u] = 1.0 + bose - 1.0 / (exp((en_el - en_ph - a.chempot) * a.ikT) + 1.0);
+        if (en_ph[u] >= 0.5e-3) {
+            occp[u] = bose[u] + 1.0 / (exp((en_el + en_ph[u] - a.chempot) * a.ikT) + 1.0);
+            occm[u] = 1.0 + bose[u] - 1.0 / (exp((en_el - en_ph[u] - a.chempot) * a.ikT) + 1.0);
             const int pr = r * a.t.nb + n;
-            const DeltaHead *h = a.t.head + pr;
-            const double lo = h->lo, hi = h->hi, ep = en_el + en_ph, em = en_el - en_ph;
-            if (!(ep < lo || ep > hi)) {
+            const double ep = en_el + en_ph[u], em = en_el - en_ph[u];
+            if (!(ep < lo[u] || ep > hi[u])) {
                 slp[u] = atomicAdd(&qn, 1);
                 qe[slp[u]] = ep;
                 qp[slp[u]] = pr;
             }
-            if (!(em < lo || em > hi)) {
+            if (!(em < lo[u] || em > hi[u])) {
                 slm[u] = atomicAdd(&qn, 1);
                 qe[slm[u]] = em;
                 qp[slm[u]] = pr;
             }
+        } else {
+            gw[u] = 0.0;
         }
     }
     __syncthreads();
-    // 2. the block's queued deltas, all lanes busy
+    // 2. the block's queued deltas, all lanes busy, two per thread and round
+    // with each step's loads issued for both: the heads, the starting piece
+    // of each energy's bin, the piece (then the rare steps to the piece
+    // holding e; delta_of_pair in that order)
     const int nq = qn;
-    for (int i = threadIdx.x; i < nq; i += 256) qe[i] = delta_of_pair(a.t, qe[i], qp[i]);
+    for (int base = 0; base < nq; base += 512) {
+        constexpr int NV = 2;
+        int pr[NV], m[NV], j[NV];
+        double e[NV], hlo[NV], bs[NV];
+        bool act[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int i = base + threadIdx.x + 256 * v;
+            act[v] = i < nq;
+            const int ii = act[v] ? i : 0;
+            e[v] = qe[ii];
+            pr[v] = qp[ii];
+            const DeltaHead *h = a.t.head + pr[v];
+            m[v] = h->m;
+            hlo[v] = h->lo;
+            bs[v] = h->binscale;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            int b = (int)((e[v] - hlo[v]) * bs[v]);
+            b = max(0, min(kBins - 1, b));
+            // bins of a pair without pieces are not used (and kept in range)
+            j[v] = min((int)a.t.head[pr[v]].bins[b], kMaxPieces - 1);
+        }
+        double plo[NV], phi[NV], pmid[NV], pih[NV], c0[NV], c1[NV], c2[NV], c3[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const DeltaPiece q = a.t.piece[(size_t)pr[v] * kMaxPieces + j[v]];
+            plo[v] = q.lo;
+            phi[v] = q.hi;
+            pmid[v] = q.mid;
+            pih[v] = q.ih;
+            c0[v] = q.c0;
+            c1[v] = q.c1;
+            c2[v] = q.c2;
+            c3[v] = q.c3;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (!act[v]) continue;
+            double d;
+            if (m[v] > 0) {
+                const DeltaPiece *pc = a.t.piece + (size_t)pr[v] * kMaxPieces;
+                int jj = j[v];
+                auto take = [&](int t) {
+                    const DeltaPiece q = pc[t];
+                    plo[v] = q.lo;
+                    phi[v] = q.hi;
+                    pmid[v] = q.mid;
+                    pih[v] = q.ih;
+                    c0[v] = q.c0;
+                    c1[v] = q.c1;
+                    c2[v] = q.c2;
+                    c3[v] = q.c3;
+                };
+                while (jj > 0 && e[v] < plo[v]) take(--jj);
+                while (jj < m[v] - 1 && e[v] >= phi[v]) take(++jj);
+                const double x = (e[v] - pmid[v]) * pih[v];
+                d = ((c3[v] * x + c2[v]) * x + c1[v]) * x + c0[v];
+            } else {
+                d = weight_sum(a.t, e[v], pr[v] / a.t.nb, pr[v] % a.t.nb);
+            }
+            qe[base + threadIdx.x + 256 * v] = d >= 1.0e-12 ? d / a.t.numtetra : 0.0;
+        }
+    }
     __syncthreads();
     // 3. X+- = 2 pi / hbar g2 occ delta
     double contrib = 0.0;
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
-        const int64_t c = ((int64_t)blockIdx.x * kXPer + u) * 256 + threadIdx.x;
+        const int64_t c = (blk * kXPer + u) * 256 + threadIdx.x;
         if (c >= a.nprocs) continue;
         const double dp = slp[u] >= 0 ? qe[slp[u]] : 0.0, dm = slm[u] >= 0 ? qe[slm[u]] : 0.0;
         const double xp = a.cnst * (gw[u] * occp[u] * dp), xm = a.cnst * (gw[u] * occm[u] * dm);
@@ -471,7 +568,7 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     __shared__ double ws[4];
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = contrib;
     __syncthreads();
-    if (threadIdx.x == 0) a.partial[(size_t)ia * gridDim.x + blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (threadIdx.x == 0) a.partial[(size_t)ia * a.nblk + blk] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 // rates[t] = sum over the blocks of state t's partial sums, 0 for a state
@@ -836,8 +933,11 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
         a.sp = istate_ph;
         a.partial = scr.partial.p;
         HCHECK(hipEventRecord(c->ev_a, s));
+        a.nblk = (int)nblk;
+        a.states_x = nblk <= 65535 ? 1 : 0;  // grid y holds at most 65535 blocks
         if (nblk > 0 && nact > 0) {
-            hipLaunchKernelGGL(k_ibzk_X, dim3((unsigned)nblk, (unsigned)nact), dim3(256), 0, s, a);
+            const dim3 grid = a.states_x ? dim3((unsigned)nact, (unsigned)nblk) : dim3((unsigned)nblk, (unsigned)nact);
+            hipLaunchKernelGGL(k_ibzk_X, grid, dim3(256), 0, s, a);
             HCHECK(hipGetLastError());
         }
         HCHECK(hipEventRecord(c->ev_b, s));
