@@ -201,41 +201,37 @@ __device__ double weight_sum(const TetraView &t, double e, int r, int n) {
 
 // Weight sum at e in [lo, hi] of a pair with m > 0 pieces from the table:
 // start at e's bin, step to the piece holding e (correct from any start),
-// evaluate its cubic.
-__device__ __forceinline__ double table_sum(const DeltaHead *h, const DeltaPiece *pc, int m, double lo,
-                                            double scale, double e) {
+// evaluate its cubic.  False when e is a piece bound (a vertex energy): the
+// sum can jump there (a tetrahedron with two equal vertex energies takes its
+// degenerate-case value at exactly that energy), so the caller sums the
+// tetrahedra directly.
+__device__ __forceinline__ bool table_sum(const DeltaHead *h, const DeltaPiece *pc, int m, double lo, double scale,
+                                          double e, double &d) {
     int b = (int)((e - lo) * scale);
     b = max(0, min(kBins - 1, b));
     int j = h->bins[b];
     DeltaPiece p = pc[j];
     while (j > 0 && e < p.lo) p = pc[--j];
     while (j < m - 1 && e >= p.hi) p = pc[++j];
+    if (e == p.lo || e == p.hi) return false;
     const double x = (e - p.mid) * p.ih;
-    return ((p.c3 * x + p.c2) * x + p.c1) * x + p.c0;
+    d = ((p.c3 * x + p.c2) * x + p.c1) * x + p.c0;
+    return true;
 }
 
 // delta_fn_tetra(e, r + 1, n + 1) (src/delta.f90:61-225) from the table: the
 // weight sum, zero below 1e-12, divided by 6 prod(mesh); zero outside the
 // energy range of r's tetrahedra.  Points whose tetrahedra all have one
-// energy (no piece) are summed directly.
+// energy (no piece) and energies at a piece bound are summed directly.
 __device__ __forceinline__ double delta_tetra(const TetraView &t, double e, int r, int n) {
     const size_t pr = (size_t)r * t.nb + n;
     const DeltaHead *h = t.head + pr;
     const double lo = h->lo, hi = h->hi;
     if (e < lo || e > hi) return 0.0;
     const int m = h->m;
-    const double d = m > 0 ? table_sum(h, t.piece + pr * kMaxPieces, m, lo, h->binscale, e) : weight_sum(t, e, r, n);
+    double d;
+    if (!(m > 0 && table_sum(h, t.piece + pr * kMaxPieces, m, lo, h->binscale, e, d))) d = weight_sum(t, e, r, n);
     return d < 1.0e-12 ? 0.0 : d / t.numtetra;
-}
-
-// delta_fn_tetra at e of the pair pr = r nb + n, e known to lie in the
-// pair's range.
-__device__ __forceinline__ double delta_of_pair(const TetraView &t, double e, int pr) {
-    const DeltaHead *h = t.head + pr;
-    const int m = h->m;
-    const double d = m > 0 ? table_sum(h, t.piece + (size_t)pr * kMaxPieces, m, h->lo, h->binscale, e)
-                           : weight_sum(t, e, pr / t.nb, pr % t.nb);
-    return d >= 1.0e-12 ? d / t.numtetra : 0.0;
 }
 
 struct ChebFit {
@@ -369,7 +365,7 @@ __global__ void k_bose(int n, const double *en, double T, double *out) {
     out[i] = e != 0.0 ? 1.0 / expm1(e / kBoltz / T) : 0.0;
 }
 
-constexpr int kXPer = 2;  // processes per thread of k_ibzk_X (strided by the block size)
+constexpr int kXPer = 2;  // consecutive processes per thread of k_ibzk_X
 
 // Processes of in-window states (the state's index among them and the block
 // of its row from the grid, st_row its row in g2 / X).  With the states along
@@ -379,7 +375,7 @@ constexpr int kXPer = 2;  // processes per thread of k_ibzk_X (strided by the bl
 // streamed in full once per state.  Per process the loop body of
 // src/interactions.f90:2278-2332, X scaled by 2 pi / hbar (2351-2354), and the
 // block's sum of X+ + X- for the rates.  Block: 256 threads x kXPer
-// processes of one row.  The few energies that fall inside a (k', n)
+// consecutive processes of one row.  The few energies that fall inside a (k', n)
 // tetrahedra range are gathered in LDS and their deltas evaluated by the
 // whole block, so no wave runs the table search for one or two lanes.
 // |g|^2 is read and X+-, and the state maps written, with nontemporal hints
@@ -403,15 +399,27 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     __syncthreads();
     // 1. per process: phonon mode, occupations, state maps; the energies
     // e +- omega that fall inside their (k', n) tetrahedra range are queued
+    // the thread's kXPer consecutive processes; with an even row length every
+    // row starts 16-byte aligned and a pair is one 16-byte access per array
+    const int64_t cb = (blk * 256 + threadIdx.x) * kXPer;
+    const bool pair = kXPer == 2 && cb + 1 < a.nprocs;
+    const bool vrow = pair && (a.nprocs & 1) == 0;
     int64_t cc[kXPer];
     bool live[kXPer];
     int2 vm[kXPer];
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
-        const int64_t c = (blk * kXPer + u) * 256 + threadIdx.x;
+        const int64_t c = cb + u;
         live[u] = c < a.nprocs;
         cc[u] = live[u] ? c : a.nprocs - 1;
-        vm[u] = a.vmap[cc[u]];
+    }
+    if (pair) {
+        const v4i v = *reinterpret_cast<const v4i *>(a.vmap + cb);
+        vm[0] = make_int2(v.x, v.y);
+        vm[kXPer - 1] = make_int2(v.z, v.w);
+    } else {
+#pragma unroll
+        for (int u = 0; u < kXPer; ++u) vm[u] = a.vmap[cc[u]];
     }
     int kr[kXPer][3];
 #pragma unroll
@@ -441,10 +449,32 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
         en_ph[u] = (coarse ? a.om : a.omf)[mi];
         bose[u] = a.bose[coarse ? mi : a.ncoarse + mi];
         sph[u] = coarse ? (int64_t)iqc * a.nbr + s + 1 : -((int64_t)iqf * a.nbr + s + 1);
-        gw[u] = __builtin_nontemporal_load(a.g2 + row + cc[u]);
+        if (!vrow) gw[u] = __builtin_nontemporal_load(a.g2 + row + cc[u]);
         const DeltaHead *h = a.t.head + (r * a.t.nb + n);
         lo[u] = h->lo;
         hi[u] = h->hi;
+    }
+    if (vrow) {
+        const dv2 g = __builtin_nontemporal_load(reinterpret_cast<const dv2 *>(a.g2 + row + cb));
+        gw[0] = g.x;
+        gw[kXPer - 1] = g.y;
+    }
+    if (a.se) {
+        int64_t sel[kXPer];
+#pragma unroll
+        for (int u = 0; u < kXPer; ++u) sel[u] = (int64_t)vm[u].x * a.nb + (vm[u].y >> 8) + 1;
+        if (vrow) {
+            typedef long long l2 __attribute__((ext_vector_type(2)));
+            __builtin_nontemporal_store(l2{sel[0], sel[kXPer - 1]}, reinterpret_cast<l2 *>(a.se + row + cb));
+            __builtin_nontemporal_store(l2{sph[0], sph[kXPer - 1]}, reinterpret_cast<l2 *>(a.sp + row + cb));
+        } else {
+#pragma unroll
+            for (int u = 0; u < kXPer; ++u)
+                if (live[u]) {
+                    __builtin_nontemporal_store(sel[u], a.se + row + cc[u]);
+                    __builtin_nontemporal_store(sph[u], a.sp + row + cc[u]);
+                }
+        }
     }
     double occp[kXPer], occm[kXPer];
     int slp[kXPer], slm[kXPer];
@@ -454,10 +484,6 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
         occp[u] = occm[u] = 0.0;
         if (!live[u]) continue;
         const int r = vm[u].x, n = vm[u].y >> 8;
-        if (a.se) {
-            __builtin_nontemporal_store((int64_t)r * a.nb + n + 1, a.se + row + cc[u]);
-            __builtin_nontemporal_store(sph[u], a.sp + row + cc[u]);
-        }
         if (en_ph[u] >= 0.5e-3) {
             occp[u] = bose[u] + 1.0 / (exp((en_el + en_ph[u] - a.chempot) * a.ikT) + 1.0);
             occm[u] = 1.0 + bose[u] - 1.0 / (exp((en_el - en_ph[u] - a.chempot) * a.ikT) + 1.0);
@@ -478,10 +504,11 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
         }
     }
     __syncthreads();
-    // 2. the block's queued deltas, all lanes busy, two per thread and round
-    // with each step's loads issued for both: the heads, the starting piece
-    // of each energy's bin, the piece (then the rare steps to the piece
-    // holding e; delta_of_pair in that order)
+    // 2. the block's queued deltas (each e known to lie in its pair's range),
+    // all lanes busy, two per thread and round with each step's loads issued
+    // for both: the heads, the starting piece of each energy's bin, the piece
+    // (then the rare steps to the piece holding e; delta_tetra's lookup in
+    // that order)
     const int nq = qn;
     for (int base = 0; base < nq; base += 512) {
         constexpr int NV = 2;
@@ -541,7 +568,8 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
                 while (jj > 0 && e[v] < plo[v]) take(--jj);
                 while (jj < m[v] - 1 && e[v] >= phi[v]) take(++jj);
                 const double x = (e[v] - pmid[v]) * pih[v];
-                d = ((c3[v] * x + c2[v]) * x + c1[v]) * x + c0[v];
+                d = e[v] == plo[v] || e[v] == phi[v] ? weight_sum(a.t, e[v], pr[v] / a.t.nb, pr[v] % a.t.nb)
+                                                    : ((c3[v] * x + c2[v]) * x + c1[v]) * x + c0[v];
             } else {
                 d = weight_sum(a.t, e[v], pr[v] / a.t.nb, pr[v] % a.t.nb);
             }
@@ -551,17 +579,26 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     __syncthreads();
     // 3. X+- = 2 pi / hbar g2 occ delta
     double contrib = 0.0;
+    double xpv[kXPer], xmv[kXPer];
 #pragma unroll
     for (int u = 0; u < kXPer; ++u) {
-        const int64_t c = (blk * kXPer + u) * 256 + threadIdx.x;
-        if (c >= a.nprocs) continue;
         const double dp = slp[u] >= 0 ? qe[slp[u]] : 0.0, dm = slm[u] >= 0 ? qe[slm[u]] : 0.0;
-        const double xp = a.cnst * (gw[u] * occp[u] * dp), xm = a.cnst * (gw[u] * occm[u] * dm);
-        if (a.xp) {
-            __builtin_nontemporal_store(xp, a.xp + row + c);
-            __builtin_nontemporal_store(xm, a.xm + row + c);
+        xpv[u] = a.cnst * (gw[u] * occp[u] * dp);
+        xmv[u] = a.cnst * (gw[u] * occm[u] * dm);
+        if (live[u]) contrib += xpv[u] + xmv[u];
+    }
+    if (a.xp) {
+        if (vrow) {
+            __builtin_nontemporal_store(dv2{xpv[0], xpv[kXPer - 1]}, reinterpret_cast<dv2 *>(a.xp + row + cb));
+            __builtin_nontemporal_store(dv2{xmv[0], xmv[kXPer - 1]}, reinterpret_cast<dv2 *>(a.xm + row + cb));
+        } else {
+#pragma unroll
+            for (int u = 0; u < kXPer; ++u)
+                if (live[u]) {
+                    __builtin_nontemporal_store(xpv[u], a.xp + row + cc[u]);
+                    __builtin_nontemporal_store(xmv[u], a.xm + row + cc[u]);
+                }
         }
-        contrib += xp + xm;
     }
     // block sum (fixed order: wave shuffles, then the 4 waves in order)
     for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off, 64);

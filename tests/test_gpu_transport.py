@@ -28,9 +28,12 @@ def _rel(x, ref, floor):
     return float(np.max(np.abs(x[sel] - ref[sel]) / np.abs(ref[sel]))) if sel.any() else 0.0
 
 
+# rows of 1302 and 972 processes (even: the X pass reads and writes pairs of
+# processes with 16-byte accesses) and of 795 (odd: 8-byte accesses)
 CASES = [
     ("nb3_k6", dict(nb=3, natoms=1, ws=3, wv=6, seed=31, bands_in_window=2), 1),
     ("nb4_na2_k4_ref2", dict(nb=4, natoms=2, ws=3, wv=4, seed=32, bands_in_window=2, mesh_ref=2), 2),
+    ("nb3_k5_odd", dict(nb=3, natoms=1, ws=3, wv=5, seed=34, bands_in_window=2), 1),
 ]
 
 
