@@ -1336,11 +1336,17 @@ __global__ __launch_bounds__(256) void k_ibzk_epi(PairArgs a, int st0, const dou
 // General-path epilogue, one wave per k' row (four rows of one state per
 // workgroup, no workgroup barriers): the lanes take the (rank, sp) entries of
 //   Z(rank, sp) = sum_np conj(U_k'(n_rank, np)) Y(np + nb sp)
-// reading the row's Y and U_k' rows straight from memory, put Z (and the
-// polar overlaps) in the wave's LDS area, then take the (rank, s) entries of
+// put Z (and the polar overlaps) in the wave's LDS area, then take the
+// (rank, s) entries of
 //   g(rank, s)  = sum_sp u_q(s, sp) Z(rank, sp) [+ lp(q,s) sum_i conj(U_k'(n,i)) U_k(m,i)]
 // and write |g|^2 with consecutive lanes on consecutive outputs.  Products
-// in the order of ibzk_epilogue_row.  LDS: 4 (nb nbr + nb) double2.
+// in the order of ibzk_epilogue_row.  STAGE: the row's Y and the U_k' rows of
+// its in-window bands are first copied into the wave's LDS area with
+// consecutive lanes on consecutive elements, so the Z products read LDS
+// instead of issuing two scattered global loads per multiply-add; without
+// STAGE (shapes whose staging would not leave two workgroups per CU) they are
+// read straight from memory.  LDS: epi_wave_lds.
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, const double2 *__restrict__ Y) {
     extern __shared__ double2 zsm[];
     const int nb = a.nb, nbr = a.nbr;
@@ -1348,7 +1354,9 @@ __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, cons
     const int stl = blockIdx.y, st = st0 + stl;
     const int row = blockIdx.x * 4 + wave;
     if (row >= a.nwv) return;  // no workgroup barrier follows
-    double2 *Zs = zsm + wave * (nb * nbr + nb), *un = Zs + nb * nbr;
+    const int ncol = nb * nbr;
+    double2 *Zs = zsm + wave * (STAGE ? 2 * ncol + nb * nb + nb : ncol + nb), *un = Zs + ncol;
+    double2 *Ys = un + nb, *Vs = Ys + ncol;  // STAGE only: the row's Y, its U_k' rows [rank][np]
     const int q0 = pos_mod(a.kp_int[3 * row] - a.st_kint[3 * st], a.Nel0);
     const int q1 = pos_mod(a.kp_int[3 * row + 1] - a.st_kint[3 * st + 1], a.Nel1);
     const int q2 = pos_mod(a.kp_int[3 * row + 2] - a.st_kint[3 * st + 2], a.Nel2);
@@ -1356,13 +1364,29 @@ __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, cons
     const int nn = a.nwin[row];
     const double2 *Yr = Y + ((size_t)stl * a.nwv + row) * a.CP;
     const double2 *Vr = a.Vkp + (size_t)row * nb * nb;
+    if constexpr (STAGE) {
+        for (int i = lane; i < ncol; i += 64) Ys[i] = Yr[i];
+        for (int i = lane; i < nn * nb; i += 64) {
+            const int rank = i / nb, np = i - rank * nb;
+            Vs[i] = Vr[(size_t)a.wband[row * nb + rank] * nb + np];
+        }
+        // the wave's LDS writes complete before its reads (a wave's LDS
+        // requests are served in order; the wait keeps it explicit)
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+    }
+    auto vrow = [&](int rank) {
+        if constexpr (STAGE) return (const double2 *)(Vs + rank * nb);
+        else return Vr + (size_t)a.wband[row * nb + rank] * nb;
+    };
+    const double2 *Ysrc = STAGE ? (const double2 *)Ys : Yr;
     for (int i = lane; i < nn * nbr; i += 64) {
         const int rank = i % nn, sp = i / nn;
-        const double2 *V = Vr + (size_t)a.wband[row * nb + rank] * nb;
+        const double2 *V = vrow(rank);
         double2 z = make_double2(0.0, 0.0);
         for (int np = 0; np < nb; ++np) {
             const double2 v = V[np];
-            const double2 y = Yr[np + nb * sp];
+            const double2 y = Ysrc[np + nb * sp];
             z.x = fma(v.x, y.x, z.x);  // conj(v) * y
             z.x = fma(v.y, y.y, z.x);
             z.y = fma(v.x, y.y, z.y);
@@ -1373,7 +1397,7 @@ __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, cons
     if (ph.lq) {
         const double2 *Ukm = a.Ukb + ((size_t)a.st_kb[st] * nb + a.st_m[st]) * nb;
         for (int rank = lane; rank < nn; rank += 64) {
-            const double2 *V = Vr + (size_t)a.wband[row * nb + rank] * nb;
+            const double2 *V = vrow(rank);
             double2 z = make_double2(0.0, 0.0);
             for (int i = 0; i < nb; ++i) {
                 const double2 t = cmulc(V[i], Ukm[i]);
@@ -1383,8 +1407,6 @@ __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, cons
             un[rank] = z;
         }
     }
-    // the wave's LDS writes complete before its reads (a wave's LDS requests
-    // are served in order; the wait keeps it explicit)
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
     double *o = a.out + a.st_slot[st] * a.nprocs + a.offs[row];
@@ -1396,6 +1418,12 @@ __global__ __launch_bounds__(256) void k_ibzk_epi_wave(PairArgs a, int st0, cons
         const double w = ph.omq[s];
         o[i] = w != 0.0 ? 0.5 * (g.x * g.x + g.y * g.y) / w * kG2Unit : 0.0;
     }
+}
+// Dynamic LDS of k_ibzk_epi_wave (4 waves) and whether it stages Y and U_k'
+// (when that leaves room for two workgroups per CU).
+bool epi_wave_stage(int64_t nb, int64_t nbr) { return (size_t)4 * (2 * nb * nbr + nb * nb + nb) * 16 <= 80 * 1024; }
+size_t epi_wave_lds(int64_t nb, int64_t nbr) {
+    return (size_t)4 * (epi_wave_stage(nb, nbr) ? 2 * nb * nbr + nb * nb + nb : nb * nbr + nb) * sizeof(double2);
 }
 
 // ---------------------------------------------------------------------------
@@ -3624,9 +3652,12 @@ void general_launch(elph_ibzk_plan *p, const PairArgs &a, int64_t nst, IbzkScrat
         if (p->nb <= 8 && p->nbr <= 6) {
             hipLaunchKernelGGL((k_ibzk_epi<8, 6>), eg, dim3(256), 0, s, a, (int)s0, sc.Y.p);
         } else {
-            const size_t elds = (size_t)4 * (p->nb * p->nbr + p->nb) * sizeof(double2);
+            const size_t elds = epi_wave_lds(p->nb, p->nbr);
             const dim3 wg((unsigned)((p->nwv + 3) / 4), (unsigned)ns);
-            hipLaunchKernelGGL(k_ibzk_epi_wave, wg, dim3(256), elds, s, a, (int)s0, sc.Y.p);
+            if (epi_wave_stage(p->nb, p->nbr))
+                hipLaunchKernelGGL(k_ibzk_epi_wave<true>, wg, dim3(256), elds, s, a, (int)s0, sc.Y.p);
+            else
+                hipLaunchKernelGGL(k_ibzk_epi_wave<false>, wg, dim3(256), elds, s, a, (int)s0, sc.Y.p);
         }
         HCHECK(hipGetLastError());
     }
@@ -3653,7 +3684,7 @@ size_t ibzk_pair_lds(const elph_ibzk_plan *p) {
                                (size_t)p->nkc * 32 * sizeof(unsigned)
                          : (size_t)(2 * 32 * 16 * p->gcb + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4);
     size_t e = 0;
-    if (!(p->nb <= 8 && p->nbr <= 6)) e = (size_t)4 * (p->nb * p->nbr + p->nb) * sizeof(double2);
+    if (!(p->nb <= 8 && p->nbr <= 6)) e = epi_wave_lds(p->nb, p->nbr);
     return std::max(y, e);
 }
 
