@@ -961,12 +961,13 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
 }
 
 // Epilogue of the fused mixed-precision kernel for up to 8 bands: lane
-// (row li, lk) takes in-window ranks lk and lk + 4 of its k' row together, so
-// the row's Y values and u_q(s, sp) are read once for both.  Everything that
-// does not depend on Y -- the phonon row of q, the in-window count and bands,
-// the output offset -- is set up before the contraction (EpiRow), so those
-// loads hide behind the matrix work.  The |g|^2 go to stage[off - sbase + ...] after the last
-// read of Y by the wave (stage may be the wave's own Y rows).  Same
+// (li, lk) takes in-window ranks lk and lk + 4 of its epilogue row together,
+// so the row's Y values (LDS) and u_q(s, sp) are read once for both.  The
+// rows of a tile are handed to the lanes in the order of their in-window
+// count (pair_i8_tile), so the waves whose rows have at most four leave the
+// second rank out.  Everything that does not depend on Y -- the phonon row of
+// q, the in-window count and bands, the output offset -- is set up before the
+// contraction (EpiRow), so those loads hide behind the matrix work.  Same
 // expressions and order as ibzk_epilogue_row up to the last step, where
 // 0.5 / omega * unit is formed once for both ranks.
 struct EpiRow {
@@ -998,12 +999,13 @@ __device__ __forceinline__ void epi_load_v(const PairArgs &a, int row, const Epi
         vb[np] = np < nb && hb ? Vb[np] : make_double2(0.0, 0.0);
     }
 }
-template <int NBM, int NBRM>
-__device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, const double2 *Yr, const EpiRow &e, int lk,
-                                                double *stage, int64_t sbase) {
-    if (lk >= e.nn) return;
+// |g|^2 of in-window ranks lk (ga) and, with TWO, lk + 4 (gb) of the lane's
+// row over the branches; the caller knows lk < e.nn.  Without TWO (no lane of
+// the wave has a second rank) the second rank's work is left out.
+template <int NBM, int NBRM, bool TWO>
+__device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int row, const double2 *Yr, const EpiRow &e, int lk,
+                                              double (&ga)[NBRM], double (&gb)[NBRM]) {
     const int nb = a.nb, nbr = a.nbr;
-    const bool hb = lk + 4 < e.nn;
     double2 va[NBM], vb[NBM];
     epi_load_v<NBM>(a, row, e, lk, va, vb);
     double2 za[NBRM], zb[NBRM];
@@ -1019,10 +1021,12 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
                     ta.x = fma(va[np].y, y.y, ta.x);
                     ta.y = fma(va[np].x, y.y, ta.y);
                     ta.y = fma(-va[np].y, y.x, ta.y);
-                    tb.x = fma(vb[np].x, y.x, tb.x);
-                    tb.x = fma(vb[np].y, y.y, tb.x);
-                    tb.y = fma(vb[np].x, y.y, tb.y);
-                    tb.y = fma(-vb[np].y, y.x, tb.y);
+                    if constexpr (TWO) {
+                        tb.x = fma(vb[np].x, y.x, tb.x);
+                        tb.x = fma(vb[np].y, y.y, tb.x);
+                        tb.y = fma(vb[np].x, y.y, tb.y);
+                        tb.y = fma(-vb[np].y, y.x, tb.y);
+                    }
                 }
         za[sp] = ta;
         zb[sp] = tb;
@@ -1035,11 +1039,12 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
                 const double2 t = cmulc(va[i], e.Ukm[i]);
                 ua.x += t.x;
                 ua.y += t.y;
-                const double2 t2 = cmulc(vb[i], e.Ukm[i]);
-                ub.x += t2.x;
-                ub.y += t2.y;
+                if constexpr (TWO) {
+                    const double2 t2 = cmulc(vb[i], e.Ukm[i]);
+                    ub.x += t2.x;
+                    ub.y += t2.y;
+                }
             }
-    double ga[NBRM], gb[NBRM];
 #pragma unroll
     for (int s = 0; s < NBRM; ++s) {
         ga[s] = gb[s] = 0.0;
@@ -1050,11 +1055,11 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
                 if (sp < nbr) {
                     const double2 u = e.ph.uq[s * nbr + sp];
                     cfma(gA, u, za[sp]);
-                    cfma(gB, u, zb[sp]);
+                    if constexpr (TWO) cfma(gB, u, zb[sp]);
                 }
             if (e.ph.lq) {
                 cfma(gA, e.ph.lq[s], ua);
-                cfma(gB, e.ph.lq[s], ub);
+                if constexpr (TWO) cfma(gB, e.ph.lq[s], ub);
             }
             // one division for both ranks (the mixed path's tolerance covers
             // the reordered rounding of 0.5 |g|^2 / omega * unit)
@@ -1062,21 +1067,10 @@ __device__ __forceinline__ void epi_pair_staged(const PairArgs &a, int row, cons
             if (w != 0.0) {
                 const double f = 0.5 * kG2Unit / w;
                 ga[s] = (gA.x * gA.x + gA.y * gA.y) * f;
-                gb[s] = (gB.x * gB.x + gB.y * gB.y) * f;
+                if constexpr (TWO) gb[s] = (gB.x * gB.x + gB.y * gB.y) * f;
             }
         }
     }
-    // the wave's reads of Y are all issued above (LDS serves a wave's
-    // requests in order); keep the compiler from moving stores up
-    asm volatile("" ::: "memory");
-    double *o = stage + (e.off - sbase) + (int64_t)lk * nbr;
-#pragma unroll
-    for (int s = 0; s < NBRM; ++s)
-        if (s < nbr) o[s] = ga[s];
-    if (hb)
-#pragma unroll
-        for (int s = 0; s < NBRM; ++s)
-            if (s < nbr) o[4 * nbr + s] = gb[s];
 }
 
 // Fused mixed-precision pair tile (state st, k' rows 64 rblk .. 64 rblk +
@@ -1115,13 +1109,44 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
               ((unsigned)((q2 * a.Lf2) % a.L) << 16);
     }
     EpiRow er;
-    int64_t b0 = 0, b1 = 0;  // the wave's range of the output row
+    const int R0 = rblk * 64;  // first k' row of the tile (R0 < nwv)
+    int re = 0;                // the lane's epilogue row
+    int64_t B0 = 0, B1 = 0;    // the tile's range of the output row
     if constexpr (STAGED) {
-        epi_row_setup(a, r, q0, q1, q2, lg, st, er);
-        if (row_base < a.nwv) {
-            b0 = a.offs[row_base];
-            b1 = row_base + 16 < a.nwv ? a.offs[row_base + 16] : a.nprocs;
+        // epilogue rows in the order in-window count descending, then row:
+        // the rows with a second rank (count > 4) fill the first waves' slots,
+        // the others' waves leave the second rank out; the waves' shares of
+        // the order rotate with the tile, spreading the second ranks over the
+        // SIMDs.  Slot p goes to lane li of wave (p / 16 - rblk - st) mod 4.
+        const int nnl = R0 + lane < a.nwv ? a.nwin[R0 + lane] : 0;
+        const int p = 16 * ((wave + rblk + st) & 3) + li;
+        int before = 0, rank = 0;
+        unsigned long long mv = 0;
+#pragma unroll
+        for (int v = NBM; v >= 0; --v) {
+            const unsigned long long m = __ballot(nnl == v);
+            const int c = __popcll(m);
+            if (p >= before && p < before + c) {
+                mv = m;
+                rank = p - before;
+            }
+            before += c;
         }
+        // the rank-th set bit of mv: the largest lo with popc(mv below lo) <= rank
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__popcll(mv & ((1ull << (lo + step)) - 1)) <= rank) lo += step;
+        re = R0 + lo;
+        int p0 = 0, p1 = 0, p2 = 0;
+        if (re < a.nwv) {
+            p0 = pos_mod(a.kp_int[3 * re] - k0i, a.Nel0);
+            p1 = pos_mod(a.kp_int[3 * re + 1] - k1i, a.Nel1);
+            p2 = pos_mod(a.kp_int[3 * re + 2] - k2i, a.Nel2);
+        }
+        epi_row_setup(a, re, p0, p1, p2, lg, st, er);
+        B0 = a.offs[R0];
+        B1 = R0 + 64 < a.nwv ? a.offs[R0 + 64] : a.nprocs;
     }
     v4i acc[CB][NS][2];
     i8_pair_core<CB>(bst, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
@@ -1150,15 +1175,35 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
     }
     __syncthreads();
     if constexpr (STAGED) {
-        // the |g|^2 of the wave's 16 rows are one contiguous range of the
-        // output: staged in the wave's own Y rows (after their last read),
-        // then written out by consecutive lanes, so HBM sees whole lines
-        // rather than 8-byte pieces at the branch stride
-        double *stage = reinterpret_cast<double *>(Y);
-        if (r < a.nwv) epi_pair_staged<NBM, NBRM>(a, r, Y + li * YS, er, lg, stage, b0);
+        // tile row i's Y at Yt + i YS (the waves' row blocks are consecutive)
+        const double2 *Yt = reinterpret_cast<const double2 *>(smem8);
+        double ga[NBRM], gb[NBRM];
+        const bool has = re < a.nwv && lg < er.nn;
+        if (__any(lg + 4 < er.nn)) {
+            if (has) epi_pair_rows<NBM, NBRM, true>(a, re, Yt + (re - R0) * YS, er, lg, ga, gb);
+        } else {
+            if (has) epi_pair_rows<NBM, NBRM, false>(a, re, Yt + (re - R0) * YS, er, lg, ga, gb);
+        }
+        // the |g|^2 of the tile's 64 rows are one contiguous range of the
+        // output: staged over the tile's Y rows once every wave has read
+        // them, then written out by consecutive lanes, so HBM sees whole
+        // lines rather than 8-byte pieces at the branch stride
         __syncthreads();
-        const int cnt = (int)(b1 - b0);
-        for (int i = lane; i < cnt; i += 64) __builtin_nontemporal_store(stage[i], outrow + b0 + i);
+        double *stage = reinterpret_cast<double *>(smem8);
+        if (has) {
+            const int nbr = a.nbr;
+            double *o = stage + (er.off - B0) + (int64_t)lg * nbr;
+#pragma unroll
+            for (int s = 0; s < NBRM; ++s)
+                if (s < nbr) o[s] = ga[s];
+            if (lg + 4 < er.nn)
+#pragma unroll
+                for (int s = 0; s < NBRM; ++s)
+                    if (s < nbr) o[4 * nbr + s] = gb[s];
+        }
+        __syncthreads();
+        const int cnt = (int)(B1 - B0);
+        for (int i = tid; i < cnt; i += 256) __builtin_nontemporal_store(stage[i], outrow + B0 + i);
     } else {
         ibzk_epilogue_row<NBM, NBRM>(a, Y, YS, row_base + li, li, lg, st, k0i, k1i, k2i, outrow);
     }
