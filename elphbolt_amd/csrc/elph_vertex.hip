@@ -1118,7 +1118,7 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
         // the others' waves leave the second rank out; the waves' shares of
         // the order rotate with the tile, spreading the second ranks over the
         // SIMDs.  Slot p goes to lane li of wave (p / 16 - rblk - st) mod 4.
-        const int nnl = R0 + lane < a.nwv ? a.nwin[R0 + lane] : 0;
+        const int nnl = R0 + lane < a.nwv ? min(a.nwin[R0 + lane], NBM) : 0;  // (at most nb <= NBM)
         const int p = 16 * ((wave + rblk + st) & 3) + li;
         int before = 0, rank = 0;
         unsigned long long mv = 0;

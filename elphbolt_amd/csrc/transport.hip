@@ -344,6 +344,7 @@ struct XArgs {
     const double *om, *omf;
     int nblk;       // blocks of kXPer x 256 processes per row
     int states_x;   // 1: blockIdx.x = state, blockIdx.y = block of the row; 0: the reverse
+    int pairs16;    // 1: g2 and the outputs start 16-byte aligned (pairs of processes in 16-byte accesses)
     const int4 *st_k;      // per in-window state: k (electron mesh ints)
     const double *st_e;    // its energy
     const int *st_row;     // its row in g2 / X
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(256) void k_ibzk_X(XArgs a) {
     // row starts 16-byte aligned and a pair is one 16-byte access per array
     const int64_t cb = (blk * 256 + threadIdx.x) * kXPer;
     const bool pair = kXPer == 2 && cb + 1 < a.nprocs;
-    const bool vrow = pair && (a.nprocs & 1) == 0;
+    const bool vrow = pair && a.pairs16 && (a.nprocs & 1) == 0;
     int64_t cc[kXPer];
     bool live[kXPer];
     int2 vm[kXPer];
@@ -972,6 +973,10 @@ int elph_ibzk_plan_X(elph_ibzk_plan *plan, const elph_tetra *tet, double T, doub
         HCHECK(hipEventRecord(c->ev_a, s));
         a.nblk = (int)nblk;
         a.states_x = nblk <= 65535 ? 1 : 0;  // grid y holds at most 65535 blocks
+        {
+            auto a16 = [](const void *q) { return q == nullptr || ((uintptr_t)q & 15) == 0; };
+            a.pairs16 = a16(a.g2) && a16(a.xp) && a16(a.xm) && a16(a.se) && a16(a.sp) ? 1 : 0;
+        }
         if (nblk > 0 && nact > 0) {
             const dim3 grid = a.states_x ? dim3((unsigned)nact, (unsigned)nblk) : dim3((unsigned)nblk, (unsigned)nact);
             hipLaunchKernelGGL(k_ibzk_X, grid, dim3(256), 0, s, a);

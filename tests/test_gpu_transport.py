@@ -126,6 +126,36 @@ def test_X_and_rates(case, precision):
     plan.close()
 
 
+def test_X_unaligned_rows(case):
+    """g2 rows starting 8 bytes past a 16-byte boundary: the X pass takes its
+    8-byte accesses instead of the 16-byte pairs and gives the same bits as
+    on the aligned rows (X+-, maps and rates)."""
+    import ctypes as ct
+    mat, el, ph, w = case["mat"], case["el"], case["ph"], case["w"]
+    last = min(el.nwv_irred * mat.nb, 24)
+    print(f"X unaligned {case['name']} states 1..{last}", flush=True)
+    from elphbolt_amd.wannier import DeviceBuffer
+    plan = interactions.IbzkPlan(w, el, ph, mesh_ref=case["mesh_ref"], precision="fp64")
+    g2d = DeviceBuffer(w, (last, plan.nprocs))
+    counts, _ = plan.run(1, last, out=g2d)
+    act = counts >= 0  # the rows of states outside the window are not written
+    assert act.any()
+    mu = float(mat.enref) + 0.02
+    ref = plan.run_X(case["tet_g"], T_K, mu, 1, last, g2d)
+    big = DeviceBuffer(w, (last * plan.nprocs + 2,))
+
+    class Rows:
+        ptr = ct.c_void_p(int(big.ptr.value) + 8)
+        shape = g2d.shape
+
+    assert w.lib.elph_device_copy(w.ctx, Rows.ptr, g2d.ptr, g2d.nbytes) == 0
+    out = plan.run_X(case["tet_g"], T_K, mu, 1, last, Rows)
+    for key in ("Xplus", "Xminus", "istate_el", "istate_ph"):
+        assert np.array_equal(out[key].numpy()[act], ref[key].numpy()[act]), key
+    assert np.array_equal(out["rates"], ref["rates"])
+    plan.close()
+
+
 def test_triangle_method(case):
     """The triangle method (tetrahedra = .false.): tables exact, delta_fn_triang
     on the device and X+/- from the same g2 rows, tolerances as above."""
