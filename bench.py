@@ -116,12 +116,13 @@ def cpu_baseline(mat, el, ph, states, target_s=15.0, threads=None):
     first, last = states
     # two pilot sizes: the difference removes the per-state g(k,Rp) setup cost
     t = []
-    for lim in (2, 10):
+    pilots = (10, 80)
+    for lim in pilots:
         t0 = time.perf_counter()
         c, _ = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=lim)
         t.append(time.perf_counter() - t0)
     nact = int((c >= 0).sum())
-    per_row = max((t[1] - t[0]) / 8.0, 1e-6)
+    per_row = max((t[1] - t[0]) / (pilots[1] - pilots[0]), 1e-6)
     ikp_limit = int(max(10, min(el.nwv, (target_s - t[0]) / per_row)))
     t0 = time.perf_counter()
     c, g = O.ibzk_g2(o, el, ph, 1, first, last, nthreads=threads, ikp_limit=ikp_limit)
