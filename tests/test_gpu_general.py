@@ -1,7 +1,9 @@
 """The general (column-tiled) ibzk path, used when numwannbands * numbranches
 exceeds what the fused pair kernels hold (96 columns in FP64, 64 in mixed):
 Y of a chunk of states through HBM, then the rotation / |g|^2 epilogue (the
-row epilogue up to 16 bands and 12 branches, the LDS-staged one beyond).
+row epilogue up to 8 bands and 6 branches, beyond that one wave per row,
+which stages the row's Y and U_k' rows in LDS unless they would leave fewer
+than two workgroups per CU: nb26_na4 reads them from memory).
 Against the oracle: counts bit-exact, |g|^2 within 1e-12 (FP64) / 1e-5
 (mixed) relative above 1e-6 * max, exact zeros equal."""
 import numpy as np
@@ -38,11 +40,12 @@ CASES = [
     ("nb6_na4", dict(nb=6, natoms=4, ws=3, wv=4, seed=63), None, "mixed"),
     ("nb10_na4_polar", dict(nb=10, natoms=4, ws=3, wv=4, seed=64), "random", "fp64"),
     ("nb10_na4_polar", dict(nb=10, natoms=4, ws=3, wv=4, seed=64), "random", "mixed"),
-    # beyond nb <= 16, nbr <= 12: the LDS-staged epilogue
+    # beyond nb <= 8, nbr <= 6: one wave per row
     ("nb7_na5", dict(nb=7, natoms=5, ws=3, wv=4, seed=66), None, "fp64"),
     ("nb7_na5", dict(nb=7, natoms=5, ws=3, wv=4, seed=66), None, "mixed"),
     ("nb18_na2", dict(nb=18, natoms=2, ws=3, wv=4, seed=67), None, "fp64"),
     ("nb18_na2_polar", dict(nb=18, natoms=2, ws=3, wv=4, seed=68), "random", "mixed"),
+    ("nb26_na4", dict(nb=26, natoms=4, ws=3, wv=4, seed=69), None, "mixed"),
 ]
 
 
