@@ -971,21 +971,41 @@ __device__ __forceinline__ void i8_pair_core(const v4i *__restrict__ bst, size_t
 // expressions and order as ibzk_epilogue_row up to the last step, where
 // 0.5 / omega * unit is formed once for both ranks.
 struct EpiRow {
-    PhRow ph;
+    int ph = 0;                  // phonon_code of q
     int nn = 0, na = 0, nb = 0;  // in-window count; bands of ranks lk and lk + 4
     int64_t off = 0;
-    const double2 *Ukm = nullptr;
 };
-__device__ __forceinline__ void epi_row_setup(const PairArgs &a, int row, int q0, int q1, int q2, int lk, int st,
-                                              EpiRow &e) {
+// The phonon row of q as one integer (its pointers are formed again in the
+// epilogue, so the row's state held across the contraction stays small):
+// iq on the phonon mesh, or -(iq + 1) for a q of the electron mesh off it
+// (the choice of phonon_row).
+__device__ __forceinline__ int phonon_code(const PairArgs &a, int q0, int q1, int q2) {
+    if (q0 % a.mesh_ref == 0 && q1 % a.mesh_ref == 0 && q2 % a.mesh_ref == 0)
+        return ((q2 / a.mesh_ref) * a.Nph1 + q1 / a.mesh_ref) * a.Nph0 + q0 / a.mesh_ref;
+    return -((q2 * a.Nel1 + q1) * a.Nel0 + q0) - 1;
+}
+__device__ __forceinline__ PhRow phonon_ptrs(const PairArgs &a, int code) {
+    PhRow r;
+    if (code >= 0) {
+        r.uq = a.u + (size_t)code * a.nbr * a.nbr;
+        r.omq = a.om + (size_t)code * a.nbr;
+        if (a.lp) r.lq = a.lp + (size_t)code * a.nbr;
+    } else {
+        const int iq = -code - 1;
+        r.uq = a.uf + (size_t)iq * a.nbr * a.nbr;
+        r.omq = a.omf + (size_t)iq * a.nbr;
+        if (a.lpf) r.lq = a.lpf + (size_t)iq * a.nbr;
+    }
+    return r;
+}
+__device__ __forceinline__ void epi_row_setup(const PairArgs &a, int row, int q0, int q1, int q2, int lk, EpiRow &e) {
     if (row >= a.nwv) return;
-    e.ph = phonon_row(a, q0, q1, q2);
+    e.ph = phonon_code(a, q0, q1, q2);
     e.nn = a.nwin[row];
     // wband holds nb entries per row (0 past the in-window count)
     e.na = lk < a.nb ? a.wband[row * a.nb + lk] : 0;
     e.nb = lk + 4 < a.nb ? a.wband[row * a.nb + lk + 4] : 0;
     e.off = a.offs[row];
-    if (e.ph.lq) e.Ukm = a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb;
 }
 template <int NBM>
 __device__ __forceinline__ void epi_load_v(const PairArgs &a, int row, const EpiRow &e, int lk, double2 (&va)[NBM],
@@ -1003,9 +1023,11 @@ __device__ __forceinline__ void epi_load_v(const PairArgs &a, int row, const Epi
 // row over the branches; the caller knows lk < e.nn.  Without TWO (no lane of
 // the wave has a second rank) the second rank's work is left out.
 template <int NBM, int NBRM, bool TWO>
-__device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int row, const double2 *Yr, const EpiRow &e, int lk,
-                                              double (&ga)[NBRM], double (&gb)[NBRM]) {
+__device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int st, int row, const double2 *Yr, const EpiRow &e,
+                                              int lk, double (&ga)[NBRM], double (&gb)[NBRM]) {
     const int nb = a.nb, nbr = a.nbr;
+    const PhRow ph = phonon_ptrs(a, e.ph);
+    const double2 *Ukm = ph.lq ? a.Ukb + ((size_t)a.st_kb[st] * a.nb + a.st_m[st]) * a.nb : nullptr;
     double2 va[NBM], vb[NBM];
     epi_load_v<NBM>(a, row, e, lk, va, vb);
     double2 za[NBRM], zb[NBRM];
@@ -1032,15 +1054,15 @@ __device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int row, const 
         zb[sp] = tb;
     }
     double2 ua = make_double2(0.0, 0.0), ub = make_double2(0.0, 0.0);
-    if (e.ph.lq)
+    if (ph.lq)
 #pragma unroll
         for (int i = 0; i < NBM; ++i)
             if (i < nb) {
-                const double2 t = cmulc(va[i], e.Ukm[i]);
+                const double2 t = cmulc(va[i], Ukm[i]);
                 ua.x += t.x;
                 ua.y += t.y;
                 if constexpr (TWO) {
-                    const double2 t2 = cmulc(vb[i], e.Ukm[i]);
+                    const double2 t2 = cmulc(vb[i], Ukm[i]);
                     ub.x += t2.x;
                     ub.y += t2.y;
                 }
@@ -1053,17 +1075,17 @@ __device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int row, const 
 #pragma unroll
             for (int sp = 0; sp < NBRM; ++sp)
                 if (sp < nbr) {
-                    const double2 u = e.ph.uq[s * nbr + sp];
+                    const double2 u = ph.uq[s * nbr + sp];
                     cfma(gA, u, za[sp]);
                     if constexpr (TWO) cfma(gB, u, zb[sp]);
                 }
-            if (e.ph.lq) {
-                cfma(gA, e.ph.lq[s], ua);
-                if constexpr (TWO) cfma(gB, e.ph.lq[s], ub);
+            if (ph.lq) {
+                cfma(gA, ph.lq[s], ua);
+                if constexpr (TWO) cfma(gB, ph.lq[s], ub);
             }
             // one division for both ranks (the mixed path's tolerance covers
             // the reordered rounding of 0.5 |g|^2 / omega * unit)
-            const double w = e.ph.omq[s];
+            const double w = ph.omq[s];
             if (w != 0.0) {
                 const double f = 0.5 * kG2Unit / w;
                 ga[s] = (gA.x * gA.x + gA.y * gA.y) * f;
@@ -1144,7 +1166,7 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
             p1 = pos_mod(a.kp_int[3 * re + 1] - k1i, a.Nel1);
             p2 = pos_mod(a.kp_int[3 * re + 2] - k2i, a.Nel2);
         }
-        epi_row_setup(a, re, p0, p1, p2, lg, st, er);
+        epi_row_setup(a, re, p0, p1, p2, lg, er);
         B0 = a.offs[R0];
         B1 = R0 + 64 < a.nwv ? a.offs[R0 + 64] : a.nprocs;
     }
@@ -1180,9 +1202,9 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
         double ga[NBRM], gb[NBRM];
         const bool has = re < a.nwv && lg < er.nn;
         if (__any(lg + 4 < er.nn)) {
-            if (has) epi_pair_rows<NBM, NBRM, true>(a, re, Yt + (re - R0) * YS, er, lg, ga, gb);
+            if (has) epi_pair_rows<NBM, NBRM, true>(a, st, re, Yt + (re - R0) * YS, er, lg, ga, gb);
         } else {
-            if (has) epi_pair_rows<NBM, NBRM, false>(a, re, Yt + (re - R0) * YS, er, lg, ga, gb);
+            if (has) epi_pair_rows<NBM, NBRM, false>(a, st, re, Yt + (re - R0) * YS, er, lg, ga, gb);
         }
         // the |g|^2 of the tile's 64 rows are one contiguous range of the
         // output: staged over the tile's Y rows once every wave has read
