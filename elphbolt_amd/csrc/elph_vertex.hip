@@ -3336,7 +3336,7 @@ void elph::ensure_gwann_digits(elph_ctx *c) {
 
 // k per launch of the Re contraction: FP64 k_gkRp<4> (4 column blocks, one
 // wave per 16 rows) and the int8 k_gkRp_i8<4, 2> (two column groups of 4
-// blocks per pass over the digits of gwann)
+// blocks per pass over the digits of gwann; k_gkRp_i8<4, 1> for up to 64 k)
 constexpr int64_t kGkKT = 64, kGkKTMixed = 128;
 int64_t elph::gk_phi_count(const elph_ctx *c) { return c->gk2p * kGkKTMixed / 2; }
 int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * (kGkKTMixed / 16) * kNS * 64; }
@@ -3348,7 +3348,9 @@ int64_t elph::gk_bphi_count(const elph_ctx *c) { return (c->gk2p / 64) * (kGkKTM
 void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_kvecs, double2 *out,
                         int64_t out_kstride, bool mixed, double2 *phiT, v4i *bphi) {
     const int64_t nb2 = c->nb * c->nb, nslices = c->nbr * c->nwsg;
-    const int64_t KT = mixed ? kGkKTMixed : kGkKT;
+    // the int8 form takes 128 k per pass when there are more than 64 (fewer:
+    // the 64-k form, whose pass over the gwann digits costs less)
+    const int64_t KT = mixed && nk > kGkKT ? kGkKTMixed : kGkKT;
     double *phi = reinterpret_cast<double *>(phiT);
     if (!mixed && c->npairk * (int64_t)sizeof(double4) > 160 * 1024)
         throw Error{ELPH_EINVAL, "FP64 Re contraction holds up to 5120 cell pairs (10240 electron WS cells)"};
@@ -3364,13 +3366,16 @@ void elph::launch_gkRp(elph_ctx *c, hipStream_t st, int64_t nk, const double *d_
             const int nt = (int)(c->gnkc * (KT / 16) * 64);
             hipLaunchKernelGGL(k_phase_digits, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, (int)c->gnkc,
                                (int)(KT / 16), phi, (int)KT, bphi);
-            constexpr int NCBW = 4, CS = (int)(kGkKTMixed / 64);
-            const int64_t nitems = nslices * ((c->gnrb + 4 / CS - 1) / (4 / CS));
-            const int64_t grid = std::min<int64_t>(nitems, 2 * std::max<int64_t>(c->ncu, 1));
-            hipLaunchKernelGGL((k_gkRp_i8<NCBW, CS>), dim3((unsigned)grid), dim3(256),
-                               2 * NCBW * CS * kNS * 64 * sizeof(v4i), st, c->gdig.p, c->gscale.p,
-                               (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi, (int)n,
-                               out + (size_t)k0 * out_kstride, out_kstride);
+            constexpr int NCBW = 4;
+            auto go = [&](auto kern, int CS) {
+                const int64_t nitems = nslices * ((c->gnrb + 4 / CS - 1) / (4 / CS));
+                const int64_t grid = std::min<int64_t>(nitems, 2 * std::max<int64_t>(c->ncu, 1));
+                hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 2 * NCBW * CS * kNS * 64 * sizeof(v4i), st,
+                                   c->gdig.p, c->gscale.p, (int)nb2, (int)c->gnrb, (int)c->gnkc, (int)nslices, bphi,
+                                   (int)n, out + (size_t)k0 * out_kstride, out_kstride);
+            };
+            if (KT == kGkKTMixed) go(k_gkRp_i8<NCBW, (int)(kGkKTMixed / 64)>, (int)(kGkKTMixed / 64));
+            else go(k_gkRp_i8<NCBW, 1>, 1);
         } else {
             hipLaunchKernelGGL(k_gkRp<4>, dim3((unsigned)nslices), dim3(256), c->npairk * sizeof(double4), st,
                                c->gwann.p, (int)nb2,
@@ -3607,7 +3612,8 @@ int elph_selftest_mfma(int device, double *max_err) {
 // ===========================================================================
 namespace {
 
-// Device tables of one group of up to 64 IBZ k of a run and of their in-window
+// Device tables of one group of IBZ k of a run (up to the k of one launch of the
+// Re contraction: 64 in FP64, 128 in mixed runs) and of their in-window
 // states (built once; kernels then run from them without host work).
 struct IbzkGroup {
     int64_t nkb = 0, nst = 0;
