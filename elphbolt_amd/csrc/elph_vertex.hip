@@ -698,6 +698,7 @@ __global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npai
     constexpr int NBL = NBX > 0 ? NBX : 1;  // register copies of g (runtime loop when NBX == 0)
     extern __shared__ double2 ush[];        // u rows of the k's states, [m][mp]
     __shared__ double red[4 * kRotRuns][16];
+    static_assert(kRotMC == 4 && kRotRuns >= kRotMC, "the scale reduction maps 4 waves x 4 states onto red");
     const int kb = blockIdx.x, cb = blockIdx.y, z = blockIdx.z, nz = gridDim.z, t = threadIdx.x;
     const int li = t & 15, w = (t >> 4) & 3, run = z * kRotRuns + (t >> 6);
     const int s0 = lower_bound_int(st_kb, nst, kb), s1 = lower_bound_int(st_kb, nst, kb + 1);
@@ -726,15 +727,25 @@ __global__ __launch_bounds__(256) void k_rotate_digits(int nb, int nbr, int npai
             for (int sd = 0; sd < kNS; ++sd) ure[m][sd] = uim[m][sd] = 0u;
         }
         if constexpr (DIGITS) {
+            // column scale of each state = max over the nz partial maxima:
+            // wave `part` reduces partials part, part + 4, ... of state pm,
+            // then every thread combines the four waves' results
+            const int pm = (t >> 4) & 3, part = t >> 6;
+            double v = 0.0;
+            if (pm < mc)
+                for (int zz = part; zz < nz; zz += 4) v = fmax(v, hmax[((size_t)(s0 + m0 + pm) * nz + zz) * CP + col]);
+            red[part * kRotMC + pm][li] = v;
+            __syncthreads();
 #pragma unroll
             for (int m = 0; m < kRotMC; ++m)
                 if (m < mc) {
-                    double sc = 0.0;
-                    for (int zz = 0; zz < nz; ++zz) sc = fmax(sc, hmax[((size_t)(s0 + m0 + m) * nz + zz) * CP + col]);
+                    double sc = fmax(fmax(red[m][li], red[kRotMC + m][li]),
+                                     fmax(red[2 * kRotMC + m][li], red[3 * kRotMC + m][li]));
                     sc = sc > 0.0 ? sc : 1.0;
                     inv[m] = 1.0 / sc;
                     if (z == 0 && t < 16) scale[(size_t)(s0 + m0 + m) * CP + col] = sc;
                 }
+            __syncthreads();  // red is written again for the next states
         }
         if (live) {
 #pragma unroll 1
@@ -1753,7 +1764,6 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
     d4 cr[NCB], ci[NCB];
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
-    const int64_t row = rb * 16 + li;
     // global -> LDS copy of the gwann rows of K rows k0 .. k0 + KI - 1 (cells
     // R_p and -R_p of pairs k0/2 .. k0/2 + 7, the wave's 16 rows) into buffer
     // b; a missing -R_p copies R_p (its weight is 0 below), rows past the end

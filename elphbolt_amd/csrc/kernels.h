@@ -58,14 +58,16 @@ constexpr int kNS = 4;
 
 // Signed-digit split of x in [-1, 1] into NS int8 digits d_s in [-127, 127]:
 // x ~= sum_s d_s / (127 * 254^s), error below 1 / (2 * 127 * 254^(NS-1)).
+// The remainder is carried scaled to the current digit (y_0 = 127 x, y_{s+1}
+// = 254 (y_s - d_s), |y_s - d_s| <= 1/2 exactly, so |d_s| <= 127), which
+// needs no division.
 template <int NS>
 __host__ __device__ inline void split_digits(double x, signed char (&d)[NS]) {
-    double r = x, scale = 127.0;
+    double y = x * 127.0;
     for (int s = 0; s < NS; ++s) {
-        const double q = rint(r * scale);
+        const double q = rint(y);
         d[s] = (signed char)q;
-        r -= q / scale;
-        scale *= 254.0;
+        y = (y - q) * 254.0;
     }
 }
 

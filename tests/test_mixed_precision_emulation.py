@@ -30,14 +30,28 @@ def _bf16(x):
 
 
 def _digits(x, ns=4):
-    """split_digits of the kernels: d_s = rint(r * 127 * 254^s), r -= d_s / (...)."""
-    r, sc, out = x.copy(), 127.0, []
+    """split_digits of the kernels: y = 127 x; d_s = rint(y), y = 254 (y - d_s)."""
+    y, out = x * 127.0, []
     for _ in range(ns):
-        q = np.rint(r * sc)
+        q = np.rint(y)
         out.append(q.astype(np.int64))
-        r = r - q / sc
-        sc *= 254.0
+        y = (y - q) * 254.0
     return out
+
+
+def test_digit_split_range_and_error():
+    """Digits stay in [-127, 127] and reconstruct x to within half the last
+    digit's weight, 1 / (2 * 127 * 254^3), over [-1, 1] including its ends,
+    the half-way points of every level and random values."""
+    rng = np.random.default_rng(3)
+    w = 1.0 / (127.0 * 254.0 ** np.arange(4))
+    ties = np.concatenate([(np.arange(-127, 127) + 0.5) * w[l] for l in range(4)])
+    x = np.concatenate([[-1.0, 1.0, 0.0, 1.0 - 2.0 ** -53, -1.0 + 2.0 ** -53], ties[np.abs(ties) <= 1],
+                        rng.uniform(-1, 1, 200000)])
+    d = np.array(_digits(x))
+    assert np.abs(d).max() <= 127
+    err = np.abs(x - (d * w[:, None]).sum(0))
+    assert err.max() <= 0.5 * w[3] * (1 + 1e-6)
 
 
 def _cell_pairs(cells):
