@@ -1740,27 +1740,42 @@ __global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *_
 // Rp contraction of the IBZ-q side (gReq) as an FP64 MFMA GEMM per branch s:
 //   out[q][(i, s, Re)] = sum_Rp gwann[(i,Re)][s][Rp] * phiT[Rp][q]
 // rows X = i + nb2 * Re; one wave per 16-row block and all 16 NCB q columns,
-// so one pass over gwann serves 16 NCB q.  Each wave copies the gwann rows of
-// its next 16 K rows global -> LDS asynchronously (its own two buffers, no
-// workgroup barrier) while the matrix instructions of the current 16 run;
-// K2 is a multiple of 16, and rows past 2 npair have zero weight and phases.
+// so one pass over gwann serves 16 NCB q.  Per block of 16 K rows each wave
+// copies the gwann rows of its row block into its own LDS buffer and a
+// quarter of the block's phase rows into the workgroup's shared one (LDS DMA,
+// one block ahead, two buffers each), so the 4 waves read the phase rows from
+// LDS rather than each from L2.  Waves past the last row block copy phase
+// rows and meet the barriers but do no products.  K2 is a multiple of 16,
+// rows past 2 npair have zero weight and phases; phiR rows are 16 NCB doubles
+// (ldphi = 16 NCB).
 // ---------------------------------------------------------------------------
+constexpr int kGReqKI = 16;  // K rows (8 cell pairs) per block of the K loop
+// phase-row stride in LDS (doubles): consecutive rows start 128 bytes apart
+// modulo 256, so the four K rows a wave reads at once fall on both halves of
+// the banks
+__host__ __device__ constexpr int greq_ldb(int NCB) { return 16 * NCB + 16; }
+__host__ __device__ constexpr size_t greq_lds(int NCB) {
+    return (size_t)4 * 2 * (kGReqKI / 2) * 2 * 16 * 16 + (size_t)2 * kGReqKI * greq_ldb(NCB) * 8;
+}
 template <int NCB>
 __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
                                               const int2 *__restrict__ pairs, const double2 *__restrict__ pwt,
                                               const double *__restrict__ phiR, int ldphi, int K2, int nq,
                                               double2 *__restrict__ out, int64_t out_qstride) {
-    constexpr int KI = 16;  // K rows (8 cell pairs) per block of the K loop
+    constexpr int KI = kGReqKI;
+    constexpr int LDB = greq_ldb(NCB);
+    static_assert(16 * NCB == 2 * 64, "a phase row is one 16-byte copy per lane");
     extern __shared__ double2 gsm[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int s = blockIdx.y;
     const int64_t nrows = (int64_t)nb2 * nwsk;
     const int64_t rb = (int64_t)blockIdx.x * 4 + wave;
-    if (rb * 16 >= nrows) return;  // the wave's LDS and copies are its own: no workgroup barrier
+    const bool live = rb * 16 < nrows;
     const int64_t sstride = nrows * nbr;  // stride of Rp in gwann
     const double2 *A = gw + (size_t)s * nrows;
     double2 *buf = gsm + wave * 2 * (KI / 2) * 2 * 16;  // [2][pair j][cell R_p | -R_p][row]
+    double *bsh = reinterpret_cast<double *>(gsm + 4 * 2 * (KI / 2) * 2 * 16);  // [2][K row][LDB]
     d4 cr[NCB], ci[NCB];
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
@@ -1784,10 +1799,20 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
             lds_dma16(A + (size_t)cell * sstride + rr, dst + t * 64);
         }
     };
+    // the wave's rows 4 j + wave of the phase block k0 .. k0 + KI - 1
+    auto copy_b = [&](int k0, int b) {
+        double *dst = bsh + b * KI * LDB;
+#pragma unroll
+        for (int j = 0; j < KI / 4; ++j) {
+            const int r = 4 * j + wave;
+            lds_dma16(phiR + (size_t)(k0 + r) * ldphi + 2 * lane, dst + r * LDB);
+        }
+    };
     // the 4 K steps of buffer b: G(2p) = g(R_p)/d + g(-R_p)/d',
     // G(2p+1) = i (g(R_p)/d - g(-R_p)/d')
     auto steps = [&](int k0, int b) {
         const double2 *src = buf + b * (KI / 2) * 2 * 16;
+        const double *bs = bsh + b * KI * LDB;
 #pragma unroll
         for (int u = 0; u < KI / 4; ++u) {
             const int kk = k0 + 4 * u + lk;
@@ -1799,21 +1824,26 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
             const double2 a = (kk & 1) ? make_double2(y.y - x.y, x.x - y.x) : make_double2(x.x + y.x, x.y + y.y);
 #pragma unroll
             for (int c = 0; c < NCB; ++c) {
-                const double bv = phiR[(size_t)kk * ldphi + c * 16 + li];
+                const double bv = bs[(4 * u + lk) * LDB + c * 16 + li];
                 cr[c] = mfma_f64(a.x, bv, cr[c]);
                 ci[c] = mfma_f64(a.y, bv, ci[c]);
             }
         }
     };
-    // the copy of the next KI rows runs while the current ones multiply
-    copy(0, 0);
+    if (live) copy(0, 0);
+    copy_b(0, 0);
     for (int k0 = 0; k0 < K2; k0 += KI) {
         const int b = (k0 / KI) & 1;
-        if (k0 + KI < K2) copy(k0 + KI, b ^ 1);
-        // this block's copy is complete (and, with it, the next one's issued)
-        if (k0 + KI < K2) asm volatile("s_waitcnt vmcnt(%0)" ::"n"((KI / 2) * 2 * 16 / 64) : "memory");
-        else lds_dma_wait();
-        steps(k0, b);
+        // block k0's copies are complete in every wave, and every wave is
+        // done with buffer b ^ 1 (block k0 - KI), which the next copies fill;
+        // no LDS stores are pending, so the barrier needs no fence, and the
+        // clobber keeps the LDS reads on their side of it
+        asm volatile("s_waitcnt vmcnt(0)\n\ts_barrier" ::: "memory");
+        if (k0 + KI < K2) {
+            if (live) copy(k0 + KI, b ^ 1);
+            copy_b(k0 + KI, b ^ 1);
+        }
+        if (live) steps(k0, b);
     }
 #pragma unroll
     for (int c = 0; c < NCB; ++c) {
@@ -4932,6 +4962,7 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
     phiR.alloc((K2 * KT + 1) / 2);
     double *phi = reinterpret_cast<double *>(phiR.p);
     const int64_t nrb = (nb2 * c->nwsk + 15) / 16;
+    require_lds(c, greq_lds(kGReqNCB), "k_gReq (vertex and phase blocks)");
     for (int64_t q0 = 0; q0 < nq; q0 += KT) {
         const int64_t n = std::min(KT, nq - q0);
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)(((K2 * KT + 1) / 2 + 255) / 256)), dim3(256), 0, c->stream, phiR.p,
@@ -4940,8 +4971,7 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
         hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, c->stream, (int)n, (int)c->npairg, d_qvecs + q0, nq,
                            c->rgpcell.p, phi, (int64_t)1, KT);
         hipLaunchKernelGGL((k_gReq<kGReqNCB>), dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256),
-                           (size_t)4 * 2 * 8 * 2 * 16 * sizeof(double2),
-                           c->stream,
+                           greq_lds(kGReqNCB), c->stream,
                            c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->npairg, c->rgpair.p,
                            c->rgpwt.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
         HCHECK(hipGetLastError());
