@@ -1744,18 +1744,20 @@ __global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *_
 // copies the gwann rows of its row block into its own LDS buffer and a
 // quarter of the block's phase rows into the workgroup's shared one (LDS DMA,
 // one block ahead, two buffers each), so the 4 waves read the phase rows from
-// LDS rather than each from L2.  Waves past the last row block copy phase
-// rows and meet the barriers but do no products.  K2 is a multiple of 16,
-// rows past 2 npair have zero weight and phases; phiR rows are 16 NCB doubles
-// (ldphi = 16 NCB).
+// LDS rather than each from L2; the pair table and the per-pair weights sit
+// in LDS for the whole launch, so no global load waits behind the copies in
+// flight (the vector-memory counter completes in order).  Waves past
+// the last row block copy phase rows and meet the barriers but do no
+// products.  K2 is a multiple of 16, rows past 2 npair have zero weight and
+// phases; phiR rows are 16 NCB doubles (ldphi = 16 NCB).
 // ---------------------------------------------------------------------------
 constexpr int kGReqKI = 16;  // K rows (8 cell pairs) per block of the K loop
 // phase-row stride in LDS (doubles): consecutive rows start 128 bytes apart
 // modulo 256, so the four K rows a wave reads at once fall on both halves of
 // the banks
 __host__ __device__ constexpr int greq_ldb(int NCB) { return 16 * NCB + 16; }
-__host__ __device__ constexpr size_t greq_lds(int NCB) {
-    return (size_t)4 * 2 * (kGReqKI / 2) * 2 * 16 * 16 + (size_t)2 * kGReqKI * greq_ldb(NCB) * 8;
+__host__ __device__ constexpr size_t greq_lds(int NCB, int K2) {
+    return (size_t)4 * 2 * (kGReqKI / 2) * 2 * 16 * 16 + (size_t)2 * kGReqKI * greq_ldb(NCB) * 8 + (size_t)K2 / 2 * (16 + 8);
 }
 template <int NCB>
 __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, int nb2, int nwsk, int nbr, int npair,
@@ -1776,6 +1778,12 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
     const double2 *A = gw + (size_t)s * nrows;
     double2 *buf = gsm + wave * 2 * (KI / 2) * 2 * 16;  // [2][pair j][cell R_p | -R_p][row]
     double *bsh = reinterpret_cast<double *>(gsm + 4 * 2 * (KI / 2) * 2 * 16);  // [2][K row][LDB]
+    double2 *wsh = reinterpret_cast<double2 *>(bsh + 2 * KI * LDB);             // [K2 / 2] pair weights
+    int2 *psh = reinterpret_cast<int2 *>(wsh + K2 / 2);                          // [K2 / 2] pair cells
+    for (int i = threadIdx.x; i < K2 / 2; i += 256) {
+        wsh[i] = i < npair ? pwt[i] : make_double2(0.0, 0.0);
+        psh[i] = i < npair ? pairs[i] : make_int2(0, -1);  // pairs past npair copy cell 0
+    }
     d4 cr[NCB], ci[NCB];
 #pragma unroll
     for (int c = 0; c < NCB; ++c) { cr[c] = d4{0, 0, 0, 0}; ci[c] = d4{0, 0, 0, 0}; }
@@ -1789,12 +1797,8 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
         for (int t = 0; t < (KI / 2) * 2 * 16 / 64; ++t) {
             const int e = t * 64 + lane;
             const int r = e & 15, c = (e >> 4) & 1, jp = e >> 5;
-            const int p = (k0 >> 1) + jp;
-            int cell = 0;
-            if (p < npair) {
-                const int2 pr = pairs[p];
-                cell = (c && pr.y >= 0) ? pr.y : pr.x;
-            }
+            const int2 pr = psh[(k0 >> 1) + jp];
+            const int cell = (c && pr.y >= 0) ? pr.y : pr.x;
             const int64_t rr = rb * 16 + r < nrows ? rb * 16 + r : rb * 16;
             lds_dma16(A + (size_t)cell * sstride + rr, dst + t * 64);
         }
@@ -1817,7 +1821,7 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
         for (int u = 0; u < KI / 4; ++u) {
             const int kk = k0 + 4 * u + lk;
             const int p = kk >> 1, jp = (4 * u + lk) >> 1;
-            const double2 w = p < npair ? pwt[p] : make_double2(0.0, 0.0);
+            const double2 w = wsh[p];
             const double dx = w.x, dy = w.y;
             const double2 gx = src[(jp * 2 + 0) * 16 + li], gy = src[(jp * 2 + 1) * 16 + li];
             const double2 x = make_double2(gx.x * dx, gx.y * dx), y = make_double2(gy.x * dy, gy.y * dy);
@@ -1830,6 +1834,7 @@ __global__ __launch_bounds__(256) void k_gReq(const double2 *__restrict__ gw, in
             }
         }
     };
+    __syncthreads();  // the weights
     if (live) copy(0, 0);
     copy_b(0, 0);
     for (int k0 = 0; k0 < K2; k0 += KI) {
@@ -4962,7 +4967,7 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
     phiR.alloc((K2 * KT + 1) / 2);
     double *phi = reinterpret_cast<double *>(phiR.p);
     const int64_t nrb = (nb2 * c->nwsk + 15) / 16;
-    require_lds(c, greq_lds(kGReqNCB), "k_gReq (vertex and phase blocks)");
+    require_lds(c, greq_lds(kGReqNCB, (int)K2), "k_gReq (vertex and phase blocks, pair weights)");
     for (int64_t q0 = 0; q0 < nq; q0 += KT) {
         const int64_t n = std::min(KT, nq - q0);
         hipLaunchKernelGGL(k_zero2, dim3((unsigned)(((K2 * KT + 1) / 2 + 255) / 256)), dim3(256), 0, c->stream, phiR.p,
@@ -4971,7 +4976,7 @@ void gReq_device(elph_ctx *c, int64_t nq, const double *d_qvecs, double2 *out, i
         hipLaunchKernelGGL(k_phase_pairs, grid, dim3(128), 0, c->stream, (int)n, (int)c->npairg, d_qvecs + q0, nq,
                            c->rgpcell.p, phi, (int64_t)1, KT);
         hipLaunchKernelGGL((k_gReq<kGReqNCB>), dim3((unsigned)((nrb + 3) / 4), (unsigned)c->nbr), dim3(256),
-                           greq_lds(kGReqNCB), c->stream,
+                           greq_lds(kGReqNCB, (int)K2), c->stream,
                            c->gwann.p, (int)nb2, (int)c->nwsk, (int)c->nbr, (int)c->npairg, c->rgpair.p,
                            c->rgpwt.p, phi, (int)KT, (int)K2, (int)n, out + (size_t)q0 * out_qstride, out_qstride);
         HCHECK(hipGetLastError());
