@@ -1673,9 +1673,22 @@ __global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *_
                 }
         }
     };
+    // row scales of the lane's 4 rows when st is an item's last chunk, loaded
+    // before the next step's copies are issued (a use waits for every
+    // vector-memory access issued before it)
+    auto load_sc = [&](int64_t st, double (&sc)[4]) {
+        int slice, rb;
+        item(st, slice, rb);
+        if (st % nkc != nkc - 1 || rb >= nrb) return;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = rb * 16 + 4 * lg + r;
+            sc[r] = i < nb2 ? gscale[(size_t)slice * nrb * 16 + i] : 0.0;
+        }
+    };
     // the step's chunk, and after an item's last chunk its recombination,
     // stores and a fresh accumulator
-    auto step = [&](int64_t st, const v4i (&Ar)[NS], const v4i (&Ai)[NS]) {
+    auto step = [&](int64_t st, const v4i (&Ar)[NS], const v4i (&Ai)[NS], const double (&scv)[4]) {
         int slice, rb;
         item(st, slice, rb);
         const bool live = rb < nrb;
@@ -1699,7 +1712,7 @@ __global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *_
                             re += wl[l] * (double)acc[c][l][0][r];
                             im += wl[l] * (double)acc[c][l][1][r];
                         }
-                        const double sc = gscale[(size_t)slice * nrb * 16 + i];
+                        const double sc = scv[r];
                         out[(size_t)col * out_kstride + (size_t)slice * nb2 + i] = make_double2(re * sc, im * sc);
                     }
                 }
@@ -1717,18 +1730,22 @@ __global__ __launch_bounds__(256, NCBW <= 4 ? 2 : 1) void k_gkRp_i8(const v4i *_
     lds_dma_wait();
     __syncthreads();
     for (int64_t st = 0;; st += 2) {
+        double sc0[4] = {0.0, 0.0, 0.0, 0.0};
+        load_sc(st, sc0);
         if (st + 1 < nsteps) {
             copy_b(st + 1);
             load_a(st + 1, Ar1, Ai1);
         }
-        step(st, Ar0, Ai0);
+        step(st, Ar0, Ai0, sc0);
         if (st + 1 >= nsteps) break;
         next_ready();
+        double sc1[4] = {0.0, 0.0, 0.0, 0.0};
+        load_sc(st + 1, sc1);
         if (st + 2 < nsteps) {
             copy_b(st + 2);
             load_a(st + 2, Ar0, Ai0);
         }
-        step(st + 1, Ar1, Ai1);
+        step(st + 1, Ar1, Ai1, sc1);
         if (st + 2 >= nsteps) break;
         next_ready();
     }
