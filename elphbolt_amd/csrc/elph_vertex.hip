@@ -1106,6 +1106,14 @@ __device__ __forceinline__ void epi_pair_rows(const PairArgs &a, int st, int row
     }
 }
 
+// Byte offset of the fused int8 tile's column scales in its LDS: past the two
+// B chunk buffers and the digit and cell tables, and past the epilogue's Y
+// tile (64 rows of 16 CB + 1 complex), which reuses the buffers and tables.
+__host__ __device__ constexpr size_t pair_i8_scale_off(int CB, int L, int nkc) {
+    const size_t tables = (size_t)2 * CB * 2 * kNS * 64 * 16 + (size_t)((L + 1) & ~1) * 8 + (size_t)nkc * 32 * 4;
+    const size_t ytile = (size_t)64 * (16 * CB + 1) * 16;
+    return ((tables > ytile ? tables : ytile) + 15) & ~(size_t)15;
+}
 // Fused mixed-precision pair tile (state st, k' rows 64 rblk .. 64 rblk +
 // 63): the cell-pair contraction of i8_pair_core, FP64 recombination of the
 // digit levels with the column scale of B, and the FP64 rotation / |g|^2
@@ -1123,6 +1131,8 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
     extern __shared__ v4i smem8[];
     uint2 *Tsh = reinterpret_cast<uint2 *>(smem8 + 2 * BCH);
     unsigned *Rsh = reinterpret_cast<unsigned *>(Tsh + ((a.L + 1) & ~1));
+    // the state's column scales, past the tables and the epilogue's Y tile
+    double *Ssh = reinterpret_cast<double *>(reinterpret_cast<char *>(smem8) + pair_i8_scale_off(CB, a.L, nkc));
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int row_base = rblk * 64 + wave * 16;
@@ -1131,6 +1141,7 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
     i8_copy_chunk<CB>(bst, BCH, 0, smem8);
     for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
     for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
+    for (int i = tid; i < CP; i += 256) Ssh[i] = bscale[(size_t)st * CP + i];
     unsigned cpk = 0;
     int q0 = 0, q1 = 0, q2 = 0;
     const int r = row_base + li;
@@ -1191,10 +1202,9 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
     wl[0] = 1.0 / (127.0 * 127.0);
 #pragma unroll
     for (int l = 1; l < NS; ++l) wl[l] = wl[l - 1] / 254.0;
-    const double *sc = bscale + (size_t)st * CP;
 #pragma unroll
     for (int c = 0; c < CB; ++c) {
-        const double s = sc[c * 16 + li];
+        const double s = Ssh[c * 16 + li];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             double re = 0.0, im = 0.0;
@@ -3807,9 +3817,7 @@ size_t ibzk_pair_lds(const elph_ibzk_plan *p) {
     if (!p->general) {
         if (p->precision == ELPH_MIXED) {
             const int CB = (int)(p->CP / 16);
-            return std::max((size_t)2 * CB * 2 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
-                                (size_t)p->nkc * 32 * sizeof(unsigned),
-                            epi_fused);
+            return pair_i8_scale_off(CB, (int)p->L, (int)p->nkc) + (size_t)p->CP * sizeof(double);
         }
         return std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
                         epi_fused);
@@ -3911,9 +3919,7 @@ FusedShape fused_shape(const elph_ibzk_plan *p) {
 // Dynamic LDS of the fused pair kernels of plan p.
 size_t fused_lds(const elph_ibzk_plan *p) {
     if (p->precision == ELPH_MIXED)
-        return std::max((size_t)2 * (p->CP / 16) * 2 * kNS * 64 * sizeof(v4i) + (size_t)((p->L + 1) & ~1) * sizeof(uint2) +
-                            (size_t)p->nkc * 32 * sizeof(unsigned),
-                        (size_t)64 * (p->CP + 1) * sizeof(double2));
+        return pair_i8_scale_off((int)(p->CP / 16), (int)p->L, (int)p->nkc) + (size_t)p->CP * sizeof(double);
     return std::max((size_t)(2 * 32 * p->CP + p->L) * sizeof(double2) + (size_t)(p->Kp / 2) * sizeof(int4),
                     (size_t)64 * (p->CP + 1) * sizeof(double2));
 }
