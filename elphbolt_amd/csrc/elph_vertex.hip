@@ -1136,33 +1136,52 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lg = lane >> 4;
     const int row_base = rblk * 64 + wave * 16;
+    // The prologue's global loads complete in issue order, so the row data
+    // whose uses come first (this lane's k' row and, staged, the tile's
+    // in-window counts and output range) are issued before the chunk-0 copy
+    // and the table loads, and the tables are loaded into registers and
+    // written to LDS only after the rows' dependent loads are issued.
     const int k0i = a.st_kint[3 * st], k1i = a.st_kint[3 * st + 1], k2i = a.st_kint[3 * st + 2];
     const v4i *bst = bsl + (size_t)st * nkc * BCH;
-    i8_copy_chunk<CB>(bst, BCH, 0, smem8);
-    for (int i = tid; i < a.L; i += 256) Tsh[i] = Tdig[i];
-    for (int i = tid; i < nkc * 32; i += 256) Rsh[i] = Rpk[i];
-    for (int i = tid; i < CP; i += 256) Ssh[i] = bscale[(size_t)st * CP + i];
-    unsigned cpk = 0;
-    int q0 = 0, q1 = 0, q2 = 0;
     const int r = row_base + li;
+    int kr0 = 0, kr1 = 0, kr2 = 0;
     if (r < a.nwv) {
-        q0 = pos_mod(a.kp_int[3 * r] - k0i, a.Nel0);
-        q1 = pos_mod(a.kp_int[3 * r + 1] - k1i, a.Nel1);
-        q2 = pos_mod(a.kp_int[3 * r + 2] - k2i, a.Nel2);
+        kr0 = a.kp_int[3 * r];
+        kr1 = a.kp_int[3 * r + 1];
+        kr2 = a.kp_int[3 * r + 2];
+    }
+    const int R0 = rblk * 64;  // first k' row of the tile (R0 < nwv)
+    int nraw = 0;              // in-window count of tile row `lane`
+    int64_t B0 = 0, B1 = 0;    // the tile's range of the output row
+    if constexpr (STAGED) {
+        nraw = R0 + lane < a.nwv ? a.nwin[R0 + lane] : 0;
+        B0 = a.offs[R0];
+        B1 = R0 + 64 < a.nwv ? a.offs[R0 + 64] : a.nprocs;
+    }
+    i8_copy_chunk<CB>(bst, BCH, 0, smem8);
+    // L <= 255 and CP <= 256 (one entry per thread); the cell table in two
+    // registers per thread, any rest written by a loop below
+    const int nrp = nkc * 32;
+    const uint2 tv = tid < a.L ? Tdig[tid] : make_uint2(0u, 0u);
+    const unsigned rv0 = tid < nrp ? Rpk[tid] : 0u, rv1 = tid + 256 < nrp ? Rpk[tid + 256] : 0u;
+    const double sv = tid < CP ? bscale[(size_t)st * CP + tid] : 0.0;
+    unsigned cpk = 0;
+    if (r < a.nwv) {
+        const int q0 = pos_mod(kr0 - k0i, a.Nel0);
+        const int q1 = pos_mod(kr1 - k1i, a.Nel1);
+        const int q2 = pos_mod(kr2 - k2i, a.Nel2);
         cpk = (unsigned)((q0 * a.Lf0) % a.L) | ((unsigned)((q1 * a.Lf1) % a.L) << 8) |
               ((unsigned)((q2 * a.Lf2) % a.L) << 16);
     }
     EpiRow er;
-    const int R0 = rblk * 64;  // first k' row of the tile (R0 < nwv)
-    int re = 0;                // the lane's epilogue row
-    int64_t B0 = 0, B1 = 0;    // the tile's range of the output row
+    int re = 0;  // the lane's epilogue row
     if constexpr (STAGED) {
         // epilogue rows in the order in-window count descending, then row:
         // the rows with a second rank (count > 4) fill the first waves' slots,
         // the others' waves leave the second rank out; the waves' shares of
         // the order rotate with the tile, spreading the second ranks over the
         // SIMDs.  Slot p goes to lane li of wave (p / 16 - rblk - st) mod 4.
-        const int nnl = R0 + lane < a.nwv ? min(a.nwin[R0 + lane], NBM) : 0;  // (at most nb <= NBM)
+        const int nnl = min(nraw, NBM);  // (at most nb <= NBM)
         const int p = 16 * ((wave + rblk + st) & 3) + li;
         int before = 0, rank = 0;
         unsigned long long mv = 0;
@@ -1189,9 +1208,12 @@ __device__ __forceinline__ void pair_i8_tile(const PairArgs &a, const uint2 *__r
             p2 = pos_mod(a.kp_int[3 * re + 2] - k2i, a.Nel2);
         }
         epi_row_setup(a, re, p0, p1, p2, lg, er);
-        B0 = a.offs[R0];
-        B1 = R0 + 64 < a.nwv ? a.offs[R0 + 64] : a.nprocs;
     }
+    if (tid < a.L) Tsh[tid] = tv;
+    if (tid < nrp) Rsh[tid] = rv0;
+    if (tid + 256 < nrp) Rsh[tid + 256] = rv1;
+    for (int i = tid + 512; i < nrp; i += 256) Rsh[i] = Rpk[i];
+    if (tid < CP) Ssh[tid] = sv;
     v4i acc[CB][NS][2];
     i8_pair_core<CB>(bst, BCH, nkc, Tsh, Rsh, cpk, a.Lmul, -a.L, smem8, acc);
     // ---- recombine levels in FP64 and run the FP64 epilogue ----
